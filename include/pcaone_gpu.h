@@ -1,0 +1,130 @@
+/*
+ * pcaone_gpu.h -- C ABI of the MI355X (gfx950) implementation of PCAone's
+ * randomized-SVD hot path.
+ *
+ * Plain C: opaque handle, plain pointers and sizes, int status (0 = OK, < 0 =
+ * error; the message is kept per handle, see pcaone_gpu_last_error).  All host
+ * pointers belong to the caller; the handle owns its device buffers and one HIP
+ * stream.  A handle is not thread-safe.  Calls that return host data return
+ * after that data is written.  Matrices crossing the boundary are FP64
+ * column-major (Eigen's default, the layout of the reference's Mat2D).
+ *
+ * Reference interfaces each entry point replaces are cited as path:line of
+ * PCAone v0.6.0.
+ */
+#ifndef PCAONE_GPU_H
+#define PCAONE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCAONE_OK 0
+#define PCAONE_EINVAL (-1)  /* bad argument or size                          */
+#define PCAONE_EHIP (-2)    /* HIP runtime error                             */
+#define PCAONE_ESTATE (-3)  /* call out of order (e.g. no data loaded)       */
+#define PCAONE_ENUM (-4)    /* numerical breakdown (e.g. Cholesky pivot <= 0) */
+
+/* arithmetic arms of the genotype contractions */
+#define PCAONE_PREC_FP64 0       /* decode -> FP64, FP64 FMA                         */
+#define PCAONE_PREC_FP32 1       /* decode -> FP32, FP32 FMA (plain drop-in)         */
+#define PCAONE_PREC_SPLIT_BF16 2 /* genotypes exact in BF16, operand = hi+lo BF16,
+                                    FP32 MFMA accumulation                          */
+
+/* RSVD variants: --svd 1 and --svd 2 of the reference (src/Cmd.cpp:129-138) */
+#define PCAONE_RSVD_NORMAL 1 /* NormalRsvdOpData (src/Halko.hpp:44-62)  */
+#define PCAONE_RSVD_FANCY 2  /* FancyRsvdOpData  (src/Halko.hpp:64-85)  */
+
+typedef struct pcaone_gpu pcaone_gpu;
+
+/* ---- lifecycle -------------------------------------------------------- */
+int pcaone_gpu_create(int device, pcaone_gpu** out);
+void pcaone_gpu_destroy(pcaone_gpu* h);
+const char* pcaone_gpu_last_error(const pcaone_gpu* h);
+/* Number of HIP devices visible (0 on a host without GPU); never fails. */
+int pcaone_gpu_device_count(void);
+
+/* ---- data residency ----------------------------------------------------
+ * Replaces FileBed::read_all (src/FilePlink.cpp:29-131): `payload` is the .bed
+ * body after the 3-byte magic (src/FilePlink.hpp:22-26), nsnps rows of
+ * (nsamples+3)>>2 bytes.  The bytes stay packed in HBM; allele frequencies
+ * are computed on the device from exact code counts. */
+int pcaone_gpu_load_bed(pcaone_gpu* h, const uint8_t* payload, uint32_t nsamples, uint32_t nsnps);
+/* F (nsnps) as estimated in src/FilePlink.cpp:36-62 (original SNP order) and,
+ * optionally, counts (nsnps x 4, row-major: codes 00, 01, 10, 11). */
+int pcaone_gpu_allele_freq(pcaone_gpu* h, double* F, int64_t* counts);
+
+/* ---- RSVD operator (RsvdOpData, src/Halko.hpp:7-42) -------------------- */
+typedef struct {
+  int variant;     /* PCAONE_RSVD_NORMAL | PCAONE_RSVD_FANCY                        */
+  int k;           /* -k                                                           */
+  int oversamples; /* --oversamples, after Param's max(os, k) (src/Cmd.cpp:190)     */
+  int bands;       /* -w/--batches: power of two >= 4 (src/Cmd.cpp:213-214)        */
+  int precision;   /* PCAONE_PREC_*                                                */
+  int seed;        /* --seed (src/Cmd.hpp:86)                                      */
+  int rand;        /* --rand: 1 Gaussian, 0 uniform(-1, 1) (src/Cmd.hpp:50)        */
+  int perm;        /* winSVD column shuffle, Param::perm (src/Cmd.cpp:216)         */
+  int standardize; /* divide by sqrt(ploidy F (1-F)) (src/Data.cpp:290-299)        */
+  int ploidy;      /* 2, or 1 with --haploid                                       */
+} pcaone_rsvd_params;
+
+/* Builds the HBM layout for the operator (SNP shuffle, SNP-major and windowed
+ * sample-major packed copies, missing lists) and draws Omega with libstdc++'s
+ * default_random_engine seeded by p->seed (RsvdOpData::initOmg,
+ * src/Halko.cpp:14-22).  `omega` (nsamples x l, column-major) replaces that
+ * draw when non-NULL; `perm` (nsnps) replaces the std::shuffle permutation
+ * (src/RSVD.hpp:63-79) when non-NULL and p->perm is set. */
+int pcaone_gpu_rsvd_init(pcaone_gpu* h, const pcaone_rsvd_params* p, const double* omega, const int32_t* perm);
+/* rows() = nsnps, cols() = nsamples, size() = k + oversamples
+ * (src/Halko.hpp:21-25). */
+int pcaone_gpu_rsvd_dims(const pcaone_gpu* h, int64_t* rows, int64_t* cols, int64_t* size);
+/* The SNP permutation in use: row j of G / V belongs to SNP perm[j]
+ * (identity unless the winSVD shuffle is on). */
+int pcaone_gpu_rsvd_perm(const pcaone_gpu* h, int32_t* perm);
+/* computeGandH(G, H, pi) (src/Halko.hpp:27; Normal src/Halko.cpp:97-126,
+ * Fancy src/Halko.cpp:153-226).  G and H stay in HBM; G (rows() x size(),
+ * rows in operator order) and H (cols() x size()) are copied out when
+ * non-NULL. */
+int pcaone_gpu_rsvd_gandh(pcaone_gpu* h, int pi, double* G, double* H);
+/* computeUSV(p, tol) (src/Halko.cpp:45-95): U (nsamples x k), S (k),
+ * V (nsnps x k, ORIGINAL SNP order, i.e. data->perm * V as written by
+ * src/Halko.cpp:339-340).  epochs: computeGandH calls made; diffs (optional,
+ * maxp+1 entries): 1 - mev per epoch (diffs[0] = 0). */
+int pcaone_gpu_rsvd_usv(pcaone_gpu* h, int maxp, double tol, double* U, double* S, double* V, int* epochs,
+                        double* diffs);
+
+/* Host-only view of the winSVD window schedule of one computeGandH call
+ * (src/Halko.cpp:159-223): `bandsize` is the operator state carried between
+ * calls (reset at pi == 0).  For each window b: acc[b] in {1, 2} (H1 or H2),
+ * update[b] (Omega refreshed after b), zero[b] (accumulator cleared by that
+ * refresh, 0 if none); *reset_both = both accumulators cleared first. */
+int pcaone_fancy_plan(int bands, int pi, uint64_t* bandsize, int* acc, int* update, int* zero, int* reset_both);
+
+/* ---- device timing ------------------------------------------------------
+ * When on, every kernel the handle launches is bracketed by HIP events on the
+ * handle's stream.  Stats are grouped by kernel family name. */
+int pcaone_gpu_set_profiling(pcaone_gpu* h, int on);
+int pcaone_gpu_reset_stats(pcaone_gpu* h);
+/* Returns the number of families when idx < 0; else fills entry idx. */
+int pcaone_gpu_kernel_stats(pcaone_gpu* h, int idx, char* name, size_t name_len, int64_t* count,
+                            double* total_ms);
+int pcaone_gpu_synchronize(pcaone_gpu* h);
+
+/* ---- synthetic data (host only, no GPU needed) ---------------------------
+ * Balding-Nichols genotypes straight into a .bed payload (nsnps rows of
+ * (nsamples+3)>>2 bytes): ancestral p ~ U(pmin, pmax), population
+ * frequencies ~ Beta(p(1-fst)/fst, (1-p)(1-fst)/fst), genotypes ~
+ * Binomial(2, p_k), each genotype missing with probability miss_rate,
+ * monomorphic SNPs redrawn; sample j is in population floor(j*npops/N).
+ * p_anc (optional, nsnps) receives the ancestral frequencies. */
+int pcaone_sim_bed(uint64_t seed, uint32_t nsamples, uint32_t nsnps, uint32_t npops, double fst, double pmin,
+                   double pmax, double miss_rate, uint8_t* payload, double* p_anc, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCAONE_GPU_H */

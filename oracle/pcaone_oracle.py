@@ -1,0 +1,420 @@
+"""CPU oracle for the PCAone randomized-SVD hot path (FP64, numpy).
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product imports this module: only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` use it, and only as the checker.  It is a restatement, written from
+the reference's behaviour, of:
+
+* PLINK ``.bed`` decode and allele frequency  (src/FilePlink.cpp:29-131,
+  src/Common.hpp:57-58)
+* standardisation                             (src/Data.cpp:290-299, :7)
+* the Omega draw through libstdc++ ``std::default_random_engine`` (minstd_rand0)
+  and ``std::normal_distribution`` / ``std::uniform_real_distribution``
+  (src/Halko.cpp:14-22, src/RSVD.hpp:25-54), filled column-major
+* the SNP shuffle ``std::shuffle`` with a default-seeded engine
+  (src/RSVD.hpp:63-79)
+* ``flipOmg``                                  (src/RSVD.hpp:81-90)
+* ``NormalRsvdOpData::computeGandH`` in-core   (src/Halko.cpp:97-126)
+* ``FancyRsvdOpData::computeGandH`` in-core    (src/Halko.cpp:153-226)
+* ``RsvdOpData::computeUSV``                   (src/Halko.cpp:45-95)
+* ``mev``                                      (src/Utils.cpp:192-198)
+* output scaling ``eigvals = S^2/M*ploidy``    (src/Halko.cpp:338-343,
+  src/Data.cpp:177-194)
+
+The third-party pieces restated here come from libstdc++ (GCC 11.4, the
+toolchain the reference builds with): the Park-Miller minimal-standard LCG
+(a=16807, m=2^31-1), ``generate_canonical<double,53>`` (two draws),
+Marsaglia's polar method with one cached value, and the "downscaling"
+``uniform_int_distribution`` together with the two-indices-per-draw
+``std::shuffle``.  Householder QR signs follow LAPACK ``dlarfg`` (numpy), which is
+the same convention as Eigen ``HouseholderQR`` (R_ii = -sign(alpha)*norm).
+
+Pinning: see tests/test_oracle.py -- brute force over all 256 byte values for
+decode, closed-form allele frequencies, libstdc++ outputs from a g++-built
+probe for the RNG/shuffle, numpy ``eigh`` of X X^T for the converged RSVD, and
+the reference binary's own outputs on seeded inputs (tests/golden/).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# src/Common.hpp:57-58 -- code 00 -> 1, 01 -> missing, 10 -> 0.5, 11 -> 0
+BED2GENO = np.array([1.0, -9.0, 0.5, 0.0])
+BED_MISSING_CODE = 1
+VAR_TOL = 1e-9  # src/Data.hpp:7
+
+
+# ----------------------------------------------------------------------------
+# PLINK decode (src/FilePlink.cpp:36-62, 70-115)
+# ----------------------------------------------------------------------------
+def bed_bytes_per_snp(nsamples: int) -> int:
+    """(N+3)>>2, src/FilePlink.hpp:18."""
+    return (nsamples + 3) >> 2
+
+
+def decode_codes(payload: np.ndarray, nsamples: int, nsnps: int) -> np.ndarray:
+    """2-bit codes, shape (M, N), LSB-first within each byte; padding dropped."""
+    b = bed_bytes_per_snp(nsamples)
+    raw = np.asarray(payload, dtype=np.uint8).reshape(nsnps, b)
+    codes = np.empty((nsnps, b * 4), dtype=np.uint8)
+    for s in range(4):
+        codes[:, s::4] = (raw >> (2 * s)) & 3
+    return codes[:, :nsamples]
+
+
+def allele_freq(codes: np.ndarray):
+    """F_i = sum_nonmiss BED2GENO / n_nonmiss  (src/FilePlink.cpp:36-62).
+
+    The partial sums of {1, 0.5, 0} are exact in double, so the reference's
+    sequential sum equals ((n00 + 0.5*n10) / n) evaluated once.  Returns
+    (F, counts) with counts[:, c] = number of samples with code c.
+    """
+    counts = np.stack([(codes == c).sum(axis=1) for c in range(4)], axis=1).astype(np.int64)
+    n = counts[:, 0] + counts[:, 2] + counts[:, 3]
+    s = counts[:, 0].astype(np.float64) + 0.5 * counts[:, 2].astype(np.float64)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        F = np.where(n > 0, s / np.maximum(n, 1), 0.0)
+    return F, counts
+
+
+def centered_matrix(codes: np.ndarray, F: np.ndarray) -> np.ndarray:
+    """X (N x M): BED2GENO[c] - F_i, missing -> 0 (src/FilePlink.cpp:106-114)."""
+    g = BED2GENO[codes]  # (M, N)
+    X = g - F[:, None]
+    X[codes == BED_MISSING_CODE] = 0.0
+    return np.ascontiguousarray(X.T)
+
+
+def standardize(X: np.ndarray, F: np.ndarray, ploidy: int = 2) -> np.ndarray:
+    """Divide column i by sqrt(ploidy*F_i*(1-F_i)) when above VAR_TOL
+    (src/Data.cpp:290-299).  Evaluation order (ploidy*F)*(1-F) as in C++."""
+    sd = np.sqrt((float(ploidy) * F) * (1.0 - F))
+    out = X.copy()
+    ok = sd > VAR_TOL
+    out[:, ok] = out[:, ok] / sd[ok]
+    return out
+
+
+# ----------------------------------------------------------------------------
+# libstdc++ random facilities
+# ----------------------------------------------------------------------------
+class MinstdRand0:
+    """std::minstd_rand0 == std::default_random_engine in libstdc++."""
+
+    A, M = 16807, 2147483647
+    MIN, MAX = 1, 2147483646
+
+    def __init__(self, seed: int = 1):
+        self.seed(seed)
+
+    def seed(self, s: int):
+        s = int(s) % self.M
+        self.x = 1 if s == 0 else s
+
+    def __call__(self) -> int:
+        self.x = (self.A * self.x) % self.M
+        return self.x
+
+
+def generate_canonical(g: MinstdRand0) -> float:
+    """generate_canonical<double, 53>: k = 2 draws for a range of 2^31-2."""
+    r = float(g.MAX - g.MIN + 1)
+    s = float(g() - g.MIN)
+    s += float(g() - g.MIN) * r
+    ret = s / (r * r)
+    if ret >= 1.0:
+        ret = math.nextafter(1.0, 0.0)
+    return ret
+
+
+class NormalDistribution:
+    """std::normal_distribution<double>: Marsaglia polar, caches the 2nd value."""
+
+    def __init__(self, mean=0.0, stddev=1.0):
+        self.mean, self.stddev = mean, stddev
+        self.saved = None
+
+    def __call__(self, g: MinstdRand0) -> float:
+        if self.saved is not None:
+            ret, self.saved = self.saved, None
+        else:
+            while True:
+                x = 2.0 * generate_canonical(g) - 1.0
+                y = 2.0 * generate_canonical(g) - 1.0
+                r2 = x * x + y * y
+                if not (r2 > 1.0 or r2 == 0.0):
+                    break
+            mult = math.sqrt(-2.0 * math.log(r2) / r2)
+            self.saved = x * mult
+            ret = y * mult
+        return ret * self.stddev + self.mean
+
+
+def omega(nrows: int, ncols: int, seed: int = 101, rand: int = 1) -> np.ndarray:
+    """RsvdOpData::initOmg (src/Halko.cpp:14-22): column-major NullaryExpr fill."""
+    g = MinstdRand0(seed)
+    vals = np.empty(nrows * ncols)
+    if rand:
+        d = NormalDistribution(0.0, 1.0)
+        for i in range(nrows * ncols):
+            vals[i] = d(g)
+    else:
+        for i in range(nrows * ncols):
+            vals[i] = generate_canonical(g) * 2.0 - 1.0
+    return vals.reshape(ncols, nrows).T.copy()
+
+
+def _uniform_int(g: MinstdRand0, a: int, b: int) -> int:
+    """uniform_int_distribution<unsigned long>{a,b}(g): the downscaling branch
+    (the engine range 2^31-3 is neither 2^32-1 nor 2^64-1)."""
+    urngrange = g.MAX - g.MIN
+    urange = b - a
+    if urngrange > urange:
+        uerange = urange + 1
+        scaling = urngrange // uerange
+        past = uerange * scaling
+        while True:
+            ret = g() - g.MIN
+            if ret < past:
+                break
+        ret //= scaling
+    elif urngrange == urange:
+        ret = g() - g.MIN
+    else:  # pragma: no cover - not reachable for the sizes used here
+        raise NotImplementedError("upscaling branch")
+    return ret + a
+
+
+def shuffle_indices(n: int, seed: int = 1) -> np.ndarray:
+    """std::shuffle of 0..n-1 with a default_random_engine{} (src/RSVD.hpp:69-71)."""
+    idx = list(range(n))
+    if n == 0:
+        return np.array(idx, dtype=np.int64)
+    g = MinstdRand0(seed)
+    urngrange = g.MAX - g.MIN
+    urange = n
+    if urngrange // urange >= urange:
+        i = 1
+        if urange % 2 == 0:
+            j = _uniform_int(g, 0, 1)
+            idx[i], idx[j] = idx[j], idx[i]
+            i += 1
+        while i != n:
+            swap_range = i + 1
+            x = _uniform_int(g, 0, swap_range * (swap_range + 1) - 1)
+            p1, p2 = x // (swap_range + 1), x % (swap_range + 1)
+            idx[i], idx[p1] = idx[p1], idx[i]
+            i += 1
+            idx[i], idx[p2] = idx[p2], idx[i]
+            i += 1
+        return np.array(idx, dtype=np.int64)
+    for i in range(1, n):
+        j = _uniform_int(g, 0, i)
+        idx[i], idx[j] = idx[j], idx[i]
+    return np.array(idx, dtype=np.int64)
+
+
+# ----------------------------------------------------------------------------
+# RSVD (src/Halko.cpp, src/RSVD.hpp)
+# ----------------------------------------------------------------------------
+def flip_omg(omg2: np.ndarray, omg: np.ndarray):
+    """flipOmg (src/RSVD.hpp:81-90); returns (new Omg2, new Omg)."""
+    omg = omg.copy()
+    for c in range(omg.shape[1]):
+        if np.abs(omg2[:, c] - omg[:, c]).sum() > 2.0 * np.abs(omg2[:, c] + omg[:, c]).sum():
+            omg[:, c] *= -1.0
+    return omg.copy(), omg
+
+
+def householder_q(H: np.ndarray) -> np.ndarray:
+    """Thin Q of a Householder QR (LAPACK dgeqrf/dorgqr; same R-sign rule as Eigen)."""
+    q, _ = np.linalg.qr(H, mode="reduced")
+    return q
+
+
+def mev(X: np.ndarray, Y: np.ndarray) -> float:
+    """src/Utils.cpp:192-198."""
+    return float(np.linalg.norm(X.T @ Y, axis=0).sum() / X.shape[1])
+
+
+def fancy_windows(nsnps: int, bands: int):
+    """Window bounds [start, stop) for in-core winSVD: blocksize = ceil(M/bands)
+    (src/Halko.cpp:178, 190-191)."""
+    blocksize = int(math.ceil(nsnps / bands))
+    out = []
+    for b in range(bands):
+        start = b * blocksize
+        stop = nsnps if (b + 1) * blocksize >= nsnps else (b + 1) * blocksize
+        out.append((start, stop))
+    return out
+
+
+class FancyState:
+    """The bandsize / H1 / H2 state that FancyRsvdOpData keeps between calls."""
+
+    def __init__(self, bands: int):
+        self.bands = bands
+        self.bandsize = 1
+
+
+def fancy_epoch_plan(state: FancyState, pi: int):
+    """Advance ``state`` by one computeGandH call and return its plan:
+    (reset_both, [(b, acc, update, zero), ...]) following src/Halko.cpp:159-223."""
+    bands = state.bands
+    reset = 2 ** pi >= bands
+    if pi == 0:
+        state.bandsize = 1
+    state.bandsize = min(state.bandsize * 2, bands)
+    bandsize = state.bandsize
+    steps = []
+    i = 1
+    for b in range(bands):
+        acc = 1 if i <= bandsize // 2 else 2
+        adjacent = pi > 0 and (b + 1) == 2 ** (pi - 1) and 2 ** pi < bands
+        update, zero = False, 0
+        if not ((b + 1) < bandsize and not adjacent):
+            if (i == bandsize) or (i == bandsize // 2) or adjacent:
+                update = True
+                if i == bandsize:
+                    zero = 1
+                    i = 0
+                else:
+                    zero = 2
+        steps.append((b, acc, update, zero))
+        i += 1
+    return reset, steps
+
+
+@dataclass
+class RsvdOracle:
+    """FP64 restatement of RsvdOpData + Normal/Fancy computeGandH (in-core)."""
+
+    X: np.ndarray                 # N x M, already centred / standardised
+    k: int
+    os: int
+    fancy: bool = False
+    bands: int = 64
+    seed: int = 101
+    rand: int = 1
+    perm: bool = True             # winSVD shuffles SNP columns at pi == 0
+    use_mev: bool = True
+    omega_override: np.ndarray | None = None
+    # state
+    omg: np.ndarray = field(init=False, default=None)
+    omg2: np.ndarray = field(init=False, default=None)
+    permutation: np.ndarray | None = field(init=False, default=None)
+    trace: list = field(init=False, default_factory=list)
+
+    def __post_init__(self):
+        self.N, self.M = self.X.shape
+        self.l = self.k + self.os
+        self.H1 = np.zeros((self.N, self.l))
+        self.H2 = np.zeros((self.N, self.l))
+        self.state = FancyState(self.bands)
+        self.init_omg()
+
+    def init_omg(self):
+        if self.omega_override is not None:
+            self.omg = self.omega_override.copy()
+        else:
+            self.omg = omega(self.N, self.l, self.seed, self.rand)
+        self.omg2 = self.omg.copy()
+
+    def _refresh(self, H):
+        q = householder_q(H)
+        self.omg2, self.omg = flip_omg(self.omg2, q)
+
+    def compute_gandh(self, pi: int):
+        if not self.fancy:
+            if pi == 0:
+                self.init_omg()
+            else:
+                self._refresh(self.H)
+            G = self.X.T @ self.omg
+            self.H = self.X @ G
+            return G, self.H
+        if pi == 0:
+            self.init_omg()
+        reset, steps = fancy_epoch_plan(self.state, pi)
+        if reset:
+            self.H1[:] = 0.0
+            self.H2[:] = 0.0
+        if pi == 0 and self.perm and self.permutation is None:
+            self.permutation = shuffle_indices(self.M)
+            self.X = np.ascontiguousarray(self.X[:, self.permutation])
+        G = np.zeros((self.M, self.l))
+        H = getattr(self, "H", np.zeros((self.N, self.l)))
+        for (b, (s, e)), (_, acc, update, zero) in zip(enumerate(fancy_windows(self.M, self.bands)), steps):
+            Xb = self.X[:, s:e]
+            G[s:e] = Xb.T @ self.omg
+            if acc == 1:
+                self.H1 += Xb @ G[s:e]
+            else:
+                self.H2 += Xb @ G[s:e]
+            if update:
+                H = self.H1 + self.H2
+                self._refresh(H)
+                if zero == 1:
+                    self.H1[:] = 0.0
+                else:
+                    self.H2[:] = 0.0
+                self.trace.append((pi, b))
+        self.H = H
+        return G, H
+
+    def compute_usv(self, p: int, tol: float):
+        """RsvdOpData::computeUSV (src/Halko.cpp:45-95).  Returns epochs run."""
+        k, l = self.k, self.l
+        upre = None
+        pi = 0
+        epochs = 0
+        self.diffs = []
+        while pi <= p:
+            G, H = self.compute_gandh(pi)
+            epochs += 1
+            q1, r1 = np.linalg.qr(G, mode="reduced")
+            q2, r2 = np.linalg.qr(q1, mode="reduced")
+            R = r2 @ r1
+            B = np.linalg.solve(R.T, H.T)          # R^T B = H^T
+            usvd, s, vt = np.linalg.svd(B, full_matrices=False)
+            ucur = vt.T[:, :k]
+            if pi > 0:
+                diff = 1.0 - mev(ucur, upre) if self.use_mev else None
+                self.diffs.append(diff)
+                if diff < tol or pi == p:
+                    if self.fancy and 2 ** pi < self.bands:
+                        p = int(math.log2(self.bands))
+                    else:
+                        self.U = ucur
+                        self.V = q2 @ usvd[:, :k]
+                        self.S = s[:k]
+                        break
+                else:
+                    upre = ucur
+            else:
+                upre = ucur
+            pi += 1
+        self.U = ucur
+        return epochs
+
+    def loadings_unpermuted(self) -> np.ndarray:
+        """data->perm * V (src/Halko.cpp:339-340): row j of V belongs to SNP perm[j]."""
+        if self.permutation is None:
+            return self.V
+        out = np.empty_like(self.V)
+        out[self.permutation] = self.V
+        return out
+
+    def eigvals(self, ploidy: int = 2) -> np.ndarray:
+        return self.S ** 2 / self.M * ploidy
+
+
+def principal_angles_max(A: np.ndarray, B: np.ndarray) -> float:
+    """Largest principal angle (radians) between span(A) and span(B)."""
+    qa, _ = np.linalg.qr(A)
+    qb, _ = np.linalg.qr(B)
+    s = np.linalg.svd(qa.T @ qb, compute_uv=False)
+    return float(np.arccos(np.clip(s.min(), -1.0, 1.0)))

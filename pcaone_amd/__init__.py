@@ -1,0 +1,231 @@
+"""pcaone_amd -- MI355X (gfx950) implementation of PCAone's randomized-SVD hot path.
+
+The product is the C ABI in ``include/pcaone_gpu.h`` implemented by
+``pcaone_amd/lib/libpcaone_amd.so`` (hand-written HIP kernels + C++ host
+engine).  This module is a thin ctypes view of that ABI for tests and the
+benchmark; it never computes anything itself and raises if the library is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "lib" / "libpcaone_amd.so"
+
+PREC_FP64, PREC_FP32, PREC_SPLIT_BF16 = 0, 1, 2
+RSVD_NORMAL, RSVD_FANCY = 1, 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x2": PREC_SPLIT_BF16}
+
+_lib = None
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class RsvdParams(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("variant", "k", "oversamples", "bands", "precision", "seed", "rand",
+                                       "perm", "standardize", "ploidy")]
+
+
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+_U8 = C.POINTER(C.c_uint8)
+_U64 = C.POINTER(C.c_uint64)
+_INT = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes); every symbol include/pcaone_gpu.h declares
+SIGNATURES = {
+    "pcaone_gpu_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "pcaone_gpu_destroy": (None, [C.c_void_p]),
+    "pcaone_gpu_last_error": (C.c_char_p, [C.c_void_p]),
+    "pcaone_gpu_device_count": (C.c_int, []),
+    "pcaone_gpu_load_bed": (C.c_int, [C.c_void_p, _U8, C.c_uint32, C.c_uint32]),
+    "pcaone_gpu_allele_freq": (C.c_int, [C.c_void_p, _D, _I64]),
+    "pcaone_gpu_rsvd_init": (C.c_int, [C.c_void_p, C.POINTER(RsvdParams), _D, _I32]),
+    "pcaone_gpu_rsvd_dims": (C.c_int, [C.c_void_p, _I64, _I64, _I64]),
+    "pcaone_gpu_rsvd_perm": (C.c_int, [C.c_void_p, _I32]),
+    "pcaone_gpu_rsvd_gandh": (C.c_int, [C.c_void_p, C.c_int, _D, _D]),
+    "pcaone_gpu_rsvd_usv": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, _D, _D, _INT, _D]),
+    "pcaone_fancy_plan": (C.c_int, [C.c_int, C.c_int, _U64, _INT, _INT, _INT, _INT]),
+    "pcaone_gpu_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "pcaone_gpu_reset_stats": (C.c_int, [C.c_void_p]),
+    "pcaone_gpu_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_size_t, _I64, _D]),
+    "pcaone_gpu_synchronize": (C.c_int, [C.c_void_p]),
+    "pcaone_sim_bed": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, _U8, _D, C.c_int]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load the in-tree shared library (fails loudly when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pcaone_amd.build`")
+        _lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+def _p(a: np.ndarray | None, ctype):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ctype))
+
+
+def device_count() -> int:
+    return lib().pcaone_gpu_device_count()
+
+
+def sim_bed(seed: int, nsamples: int, nsnps: int, npops: int = 3, fst: float = 0.1, pmin: float = 0.05,
+            pmax: float = 0.5, miss: float = 0.01, nthreads: int = 0, want_p: bool = False):
+    """Balding-Nichols .bed payload (host only)."""
+    nb = (nsamples + 3) >> 2
+    out = np.empty(nsnps * nb, dtype=np.uint8)
+    panc = np.empty(nsnps) if want_p else None
+    rc = lib().pcaone_sim_bed(seed, nsamples, nsnps, npops, fst, pmin, pmax, miss, _p(out, C.c_uint8),
+                              _p(panc, C.c_double), nthreads)
+    if rc != 0:
+        raise GpuError(rc, "pcaone_sim_bed failed")
+    return (out, panc) if want_p else out
+
+
+def fancy_plan(bands: int, pi: int, bandsize: int):
+    """Host schedule of one winSVD computeGandH call -> (acc, update, zero, reset, new_bandsize)."""
+    bs = C.c_uint64(bandsize)
+    acc = np.zeros(bands, dtype=np.intc)
+    upd = np.zeros(bands, dtype=np.intc)
+    zer = np.zeros(bands, dtype=np.intc)
+    rst = C.c_int(0)
+    rc = lib().pcaone_fancy_plan(bands, pi, C.byref(bs), _p(acc, C.c_int), _p(upd, C.c_int), _p(zer, C.c_int),
+                                 C.byref(rst))
+    if rc != 0:
+        raise GpuError(rc, "pcaone_fancy_plan failed")
+    return acc, upd.astype(bool), zer, bool(rst.value), bs.value
+
+
+class Gpu:
+    """One handle: device buffers + one HIP stream (not thread-safe)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.pcaone_gpu_create(device, C.byref(h))
+        if rc != 0:
+            raise GpuError(rc, f"pcaone_gpu_create({device}) failed")
+        self.h = h
+        self.N = self.M = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.pcaone_gpu_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc < 0:
+            raise GpuError(rc, self._L.pcaone_gpu_last_error(self.h).decode())
+        return rc
+
+    # ---- data ---------------------------------------------------------------
+    def load_bed(self, payload: np.ndarray, nsamples: int, nsnps: int):
+        payload = np.ascontiguousarray(payload, dtype=np.uint8)
+        if payload.size != nsnps * ((nsamples + 3) >> 2):
+            raise ValueError("payload size does not match nsnps * ceil(nsamples/4)")
+        self._check(self._L.pcaone_gpu_load_bed(self.h, _p(payload, C.c_uint8), nsamples, nsnps))
+        self.N, self.M = nsamples, nsnps
+
+    def allele_freq(self):
+        F = np.empty(self.M)
+        cnt = np.empty((self.M, 4), dtype=np.int64)
+        self._check(self._L.pcaone_gpu_allele_freq(self.h, _p(F, C.c_double), _p(cnt, C.c_int64)))
+        return F, cnt
+
+    # ---- RSVD operator --------------------------------------------------------
+    def rsvd_init(self, k: int, oversamples: int, variant: int = RSVD_FANCY, bands: int = 64,
+                  precision: int = PREC_SPLIT_BF16, seed: int = 101, rand: int = 1, perm: bool = True,
+                  standardize: bool = True, ploidy: int = 2, omega: np.ndarray | None = None,
+                  permutation: np.ndarray | None = None):
+        p = RsvdParams(variant, k, oversamples, bands, precision, seed, rand, int(perm), int(standardize), ploidy)
+        om = None if omega is None else np.asfortranarray(omega, dtype=np.float64)
+        pm = None if permutation is None else np.ascontiguousarray(permutation, dtype=np.int32)
+        self._check(self._L.pcaone_gpu_rsvd_init(self.h, C.byref(p), _p(om, C.c_double), _p(pm, C.c_int32)))
+        self.k, self.l = k, k + oversamples
+
+    def perm(self) -> np.ndarray:
+        out = np.empty(self.M, dtype=np.int32)
+        self._check(self._L.pcaone_gpu_rsvd_perm(self.h, _p(out, C.c_int32)))
+        return out
+
+    def gandh(self, pi: int, want_g: bool = True, want_h: bool = True):
+        G = np.empty((self.M, self.l), order="F") if want_g else None
+        H = np.empty((self.N, self.l), order="F") if want_h else None
+        self._check(self._L.pcaone_gpu_rsvd_gandh(self.h, pi, _p(G, C.c_double), _p(H, C.c_double)))
+        return G, H
+
+    def usv(self, maxp: int, tol: float):
+        U = np.empty((self.N, self.k), order="F")
+        S = np.empty(self.k)
+        V = np.empty((self.M, self.k), order="F")
+        ep = C.c_int(0)
+        diffs = np.zeros(maxp + 1)
+        self._check(self._L.pcaone_gpu_rsvd_usv(self.h, maxp, tol, _p(U, C.c_double), _p(S, C.c_double),
+                                                 _p(V, C.c_double), C.byref(ep), _p(diffs, C.c_double)))
+        return U, S, V, ep.value, diffs
+
+    # ---- timing ---------------------------------------------------------------
+    def profiling(self, on: bool):
+        self._check(self._L.pcaone_gpu_set_profiling(self.h, int(on)))
+
+    def reset_stats(self):
+        self._check(self._L.pcaone_gpu_reset_stats(self.h))
+
+    def kernel_stats(self) -> dict:
+        n = self._check(self._L.pcaone_gpu_kernel_stats(self.h, -1, None, 0, None, None))
+        out = {}
+        buf = C.create_string_buffer(128)
+        for i in range(n):
+            cnt = C.c_int64()
+            ms = C.c_double()
+            self._check(self._L.pcaone_gpu_kernel_stats(self.h, i, buf, 128, C.byref(cnt), C.byref(ms)))
+            out[buf.value.decode()] = (cnt.value, ms.value)
+        return out
+
+    def synchronize(self):
+        self._check(self._L.pcaone_gpu_synchronize(self.h))
+
+
+def write_plink(prefix: str | os.PathLike, payload: np.ndarray, nsamples: int, nsnps: int,
+                chrom: np.ndarray | None = None, pos: np.ndarray | None = None):
+    """Write .bed/.bim/.fam for a payload (used to drive the reference CLI)."""
+    prefix = str(prefix)
+    with open(prefix + ".bed", "wb") as f:
+        f.write(bytes([0x6C, 0x1B, 0x01]))
+        f.write(np.ascontiguousarray(payload, dtype=np.uint8).tobytes())
+    with open(prefix + ".fam", "w") as f:
+        for j in range(nsamples):
+            f.write(f"f{j} i{j} 0 0 0 -9\n")
+    if chrom is None:
+        chrom = np.ones(nsnps, dtype=np.int64)
+    if pos is None:
+        pos = np.arange(1, nsnps + 1, dtype=np.int64) * 100
+    with open(prefix + ".bim", "w") as f:
+        for i in range(nsnps):
+            f.write(f"{chrom[i]}\trs{i}\t0\t{pos[i]}\tA\tG\n")

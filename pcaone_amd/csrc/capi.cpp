@@ -1,0 +1,175 @@
+// extern "C" boundary: every entry point catches, records the message on the
+// handle and returns a negative status (the reference's cao.error throws
+// std::runtime_error, src/Logger.hpp:96-105; the C++ wrapper on the other side
+// of this ABI turns the status back into an exception).
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/pcaone_gpu.h"
+#include "engine.hpp"
+
+struct pcaone_gpu {
+  pcaone::Engine* eng = nullptr;
+  std::string err;
+};
+
+namespace {
+
+template <class F>
+int guarded(pcaone_gpu* h, F&& f) {
+  if (!h || !h->eng) return PCAONE_EINVAL;
+  try {
+    f(*h->eng);
+    h->err.clear();
+    return PCAONE_OK;
+  } catch (const pcaone::Error& e) {
+    h->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    h->err = "host allocation failed";
+    return PCAONE_EINVAL;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return PCAONE_EINVAL;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcaone_gpu_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int pcaone_gpu_create(int device, pcaone_gpu** out) {
+  if (!out) return PCAONE_EINVAL;
+  *out = nullptr;
+  auto* h = new (std::nothrow) pcaone_gpu();
+  if (!h) return PCAONE_EINVAL;
+  try {
+    h->eng = new pcaone::Engine(device);
+  } catch (const pcaone::Error& e) {
+    const int c = e.code;
+    delete h;
+    return c;
+  } catch (...) {
+    delete h;
+    return PCAONE_EHIP;
+  }
+  *out = h;
+  return PCAONE_OK;
+}
+
+void pcaone_gpu_destroy(pcaone_gpu* h) {
+  if (!h) return;
+  delete h->eng;
+  delete h;
+}
+
+const char* pcaone_gpu_last_error(const pcaone_gpu* h) { return h ? h->err.c_str() : "null handle"; }
+
+int pcaone_gpu_load_bed(pcaone_gpu* h, const uint8_t* payload, uint32_t nsamples, uint32_t nsnps) {
+  return guarded(h, [&](pcaone::Engine& e) { e.load_bed(payload, nsamples, nsnps); });
+}
+
+int pcaone_gpu_allele_freq(pcaone_gpu* h, double* F, int64_t* counts) {
+  return guarded(h, [&](pcaone::Engine& e) { e.allele_freq(F, counts); });
+}
+
+int pcaone_gpu_rsvd_init(pcaone_gpu* h, const pcaone_rsvd_params* p, const double* omega, const int32_t* perm) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    PCAONE_REQUIRE(p != nullptr, PCAONE_EINVAL, "rsvd_init: null params");
+    pcaone::RsvdParams q;
+    q.variant = p->variant;
+    q.k = p->k;
+    q.os = p->oversamples;
+    q.bands = p->bands;
+    q.precision = p->precision;
+    q.seed = p->seed;
+    q.rand = p->rand;
+    q.perm = p->perm;
+    q.standardize = p->standardize;
+    q.ploidy = p->ploidy;
+    e.rsvd_init(q, omega, perm);
+  });
+}
+
+int pcaone_gpu_rsvd_dims(const pcaone_gpu* h, int64_t* rows, int64_t* cols, int64_t* size) {
+  return guarded(const_cast<pcaone_gpu*>(h), [&](pcaone::Engine& e) { e.dims(rows, cols, size); });
+}
+
+int pcaone_gpu_rsvd_perm(const pcaone_gpu* h, int32_t* perm) {
+  return guarded(const_cast<pcaone_gpu*>(h), [&](pcaone::Engine& e) {
+    PCAONE_REQUIRE(perm != nullptr, PCAONE_EINVAL, "rsvd_perm: null output");
+    e.rsvd_perm(perm);
+  });
+}
+
+int pcaone_gpu_rsvd_gandh(pcaone_gpu* h, int pi, double* G, double* H) {
+  return guarded(h, [&](pcaone::Engine& e) { e.rsvd_gandh(pi, G, H); });
+}
+
+int pcaone_gpu_rsvd_usv(pcaone_gpu* h, int maxp, double tol, double* U, double* S, double* V, int* epochs,
+                        double* diffs) {
+  return guarded(h, [&](pcaone::Engine& e) { e.rsvd_usv(maxp, tol, U, S, V, epochs, diffs); });
+}
+
+int pcaone_fancy_plan(int bands, int pi, uint64_t* bandsize, int* acc, int* update, int* zero, int* reset_both) {
+  if (bands < 4 || (bands & (bands - 1)) || pi < 0 || !bandsize || !acc || !update || !zero) return PCAONE_EINVAL;
+  const pcaone::FancyPlan pl = pcaone::fancy_plan(bands, pi, *bandsize);
+  for (int b = 0; b < bands; ++b) {
+    acc[b] = pl.steps[b].acc;
+    update[b] = pl.steps[b].update ? 1 : 0;
+    zero[b] = pl.steps[b].zero;
+  }
+  if (reset_both) *reset_both = pl.reset_both ? 1 : 0;
+  return PCAONE_OK;
+}
+
+int pcaone_gpu_set_profiling(pcaone_gpu* h, int on) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    e.sync();
+    e.timer.resolve();
+    e.timer.on = on != 0;
+  });
+}
+
+int pcaone_gpu_reset_stats(pcaone_gpu* h) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    e.sync();
+    e.timer.clear();
+  });
+}
+
+int pcaone_gpu_kernel_stats(pcaone_gpu* h, int idx, char* name, size_t name_len, int64_t* count,
+                            double* total_ms) {
+  int ret = 0;
+  const int st = guarded(h, [&](pcaone::Engine& e) {
+    e.sync();
+    e.timer.resolve();
+    if (idx < 0) {
+      ret = (int)e.timer.agg.size();
+      return;
+    }
+    PCAONE_REQUIRE((size_t)idx < e.timer.agg.size(), PCAONE_EINVAL, "kernel_stats: index out of range");
+    auto it = e.timer.agg.begin();
+    std::advance(it, idx);
+    if (name && name_len) {
+      std::strncpy(name, it->first.c_str(), name_len - 1);
+      name[name_len - 1] = '\0';
+    }
+    if (count) *count = it->second.first;
+    if (total_ms) *total_ms = it->second.second;
+  });
+  return st < 0 ? st : ret;
+}
+
+int pcaone_gpu_synchronize(pcaone_gpu* h) {
+  return guarded(h, [&](pcaone::Engine& e) { e.sync(); });
+}
+
+}  // extern "C"

@@ -1,0 +1,620 @@
+// Host side of the RSVD hot path: HBM layout, the Normal / Fancy computeGandH
+// schedules and computeUSV, all driving the gfx950 kernels on one stream.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+
+namespace pcaone {
+
+// ---------------------------------------------------------------------------
+// kernel timing
+// ---------------------------------------------------------------------------
+void KernelTimer::resolve() {
+  for (auto& r : pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      auto& e = agg[r.name];
+      e.first += 1;
+      e.second += ms;
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  pending.clear();
+}
+
+void KernelTimer::clear() {
+  resolve();
+  agg.clear();
+}
+
+KernelTimer::~KernelTimer() {
+  for (auto& r : pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+}
+
+template <class F>
+void Engine::timed(const char* name, F&& launch) {
+  if (!timer.on) {
+    launch();
+    PCAONE_HIP(hipGetLastError());
+    return;
+  }
+  KernelTimer::Rec r{name, nullptr, nullptr};
+  PCAONE_HIP(hipEventCreate(&r.a));
+  PCAONE_HIP(hipEventCreate(&r.b));
+  PCAONE_HIP(hipEventRecord(r.a, stream_));
+  launch();
+  PCAONE_HIP(hipGetLastError());
+  PCAONE_HIP(hipEventRecord(r.b, stream_));
+  timer.pending.push_back(r);
+  if (timer.pending.size() > 4096) timer.resolve();
+}
+
+// ---------------------------------------------------------------------------
+// winSVD window schedule, src/Halko.cpp:159-223
+// ---------------------------------------------------------------------------
+FancyPlan fancy_plan(int bands, int pi, uint64_t& bandsize) {
+  FancyPlan pl;
+  pl.reset_both = std::ldexp(1.0, pi) >= (double)bands;
+  if (pi == 0) bandsize = 1;
+  bandsize = std::min<uint64_t>(bandsize * 2, (uint64_t)bands);
+  uint64_t i = 1;
+  for (int b = 0; b < bands; ++b, ++i) {
+    WindowStep st{(i <= bandsize / 2) ? 1 : 2, false, 0};
+    const bool adjacent = pi > 0 && (double)(b + 1) == std::ldexp(1.0, pi - 1) && std::ldexp(1.0, pi) < bands;
+    if (!((uint64_t)(b + 1) < bandsize && !adjacent)) {
+      if (i == bandsize || i == bandsize / 2 || adjacent) {
+        st.update = true;
+        if (i == bandsize) {
+          st.zero = 1;
+          i = 0;
+        } else {
+          st.zero = 2;
+        }
+      }
+    }
+    pl.steps.push_back(st);
+  }
+  return pl;
+}
+
+// ---------------------------------------------------------------------------
+// lifecycle
+// ---------------------------------------------------------------------------
+Engine::Engine(int device) : device_(device) {
+  int n = 0;
+  PCAONE_HIP(hipGetDeviceCount(&n));
+  PCAONE_REQUIRE(device >= 0 && device < n, -1, "no such HIP device");
+  PCAONE_HIP(hipSetDevice(device));
+  PCAONE_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+Engine::~Engine() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  timer.resolve();
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::sync() { PCAONE_HIP(hipStreamSynchronize(stream_)); }
+
+// ---------------------------------------------------------------------------
+// data residency, src/FilePlink.cpp:29-131
+// ---------------------------------------------------------------------------
+void Engine::load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps) {
+  PCAONE_REQUIRE(payload && nsamples > 0 && nsnps > 0, -1, "load_bed: empty input");
+  PCAONE_HIP(hipSetDevice(device_));
+  have_op_ = false;
+  N_ = nsamples;
+  M_ = nsnps;
+  const uint64_t nb = (N_ + 3) >> 2;
+  raw_.reserve(nb * M_);
+  PCAONE_HIP(hipMemcpyAsync(raw_.p, payload, nb * M_, hipMemcpyHostToDevice, stream_));
+  // identity SNP-major layout to count codes
+  const uint32_t rb = (uint32_t)round_up(nb, 64);
+  DevBuf<uint8_t> tmp;
+  tmp.reserve((uint64_t)rb * M_);
+  DevBuf<int32_t> cnt;
+  cnt.reserve((uint64_t)M_ * 4);
+  timed("bed_layout", [&] { launch_bed_to_snpmajor(raw_.p, N_, M_, nullptr, tmp.p, rb, stream_); });
+  timed("count_codes", [&] { launch_count_codes(tmp.p, M_, rb, N_, cnt.p, stream_); });
+  std::vector<int32_t> h((size_t)M_ * 4);
+  PCAONE_HIP(hipMemcpyAsync(h.data(), cnt.p, h.size() * 4, hipMemcpyDeviceToHost, stream_));
+  sync();
+  F_orig_.assign(M_, 0.0);
+  counts_orig_.assign((size_t)M_ * 4, 0);
+  for (uint32_t i = 0; i < M_; ++i) {
+    for (int c = 0; c < 4; ++c) counts_orig_[4 * (size_t)i + c] = h[4 * (size_t)i + c];
+    const int64_t n00 = h[4 * (size_t)i], n10 = h[4 * (size_t)i + 2], n11 = h[4 * (size_t)i + 3];
+    const int64_t n = n00 + n10 + n11;
+    // sum of BED2GENO over non-missing (exact in double) / count
+    F_orig_[i] = n ? ((double)n00 + 0.5 * (double)n10) / (double)n : 0.0;
+  }
+  have_bed_ = true;
+}
+
+void Engine::allele_freq(double* F, int64_t* counts) const {
+  PCAONE_REQUIRE(have_bed_, -3, "allele_freq: no data loaded");
+  if (F) std::memcpy(F, F_orig_.data(), F_orig_.size() * sizeof(double));
+  if (counts) std::memcpy(counts, counts_orig_.data(), counts_orig_.size() * sizeof(int64_t));
+}
+
+void Engine::dims(int64_t* rows, int64_t* cols, int64_t* size) const {
+  PCAONE_REQUIRE(have_op_, -3, "rsvd_dims: operator not initialised");
+  if (rows) *rows = M_;
+  if (cols) *cols = N_;
+  if (size) *size = l_;
+}
+
+void Engine::rsvd_perm(int32_t* perm) const {
+  PCAONE_REQUIRE(have_op_, -3, "rsvd_perm: operator not initialised");
+  std::memcpy(perm, perm_.data(), perm_.size() * sizeof(int32_t));
+}
+
+// ---------------------------------------------------------------------------
+// operator set-up
+// ---------------------------------------------------------------------------
+static uint32_t split_for(uint64_t blocks, uint32_t kchunks, uint32_t& cps) {
+  uint64_t want = ceil_div(1024, std::max<uint64_t>(blocks, 1));
+  want = std::min<uint64_t>(std::max<uint64_t>(want, 1), std::min<uint32_t>(kchunks, 16));
+  cps = (uint32_t)ceil_div(kchunks, want);
+  return (uint32_t)ceil_div(kchunks, cps);
+}
+
+void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* perm) {
+  PCAONE_REQUIRE(have_bed_, -3, "rsvd_init: no data loaded");
+  PCAONE_HIP(hipSetDevice(device_));
+  PCAONE_REQUIRE(p.variant == 1 || p.variant == 2, -1, "rsvd_init: variant must be 1 (normal) or 2 (fancy)");
+  PCAONE_REQUIRE(p.precision >= 0 && p.precision <= 2, -1, "rsvd_init: bad precision");
+  PCAONE_REQUIRE(p.k >= 1 && p.os >= 0, -1, "rsvd_init: bad k / oversamples");
+  PCAONE_REQUIRE(p.bands >= 4 && (p.bands & (p.bands - 1)) == 0, -1,
+                 "the -w/--batches must be a power of 2 and the minimun is 4.");
+  PCAONE_REQUIRE(p.ploidy == 1 || p.ploidy == 2, -1, "rsvd_init: ploidy must be 1 or 2");
+  const uint32_t l = (uint32_t)(p.k + p.os);
+  PCAONE_REQUIRE(l <= 64, -1, "rsvd_init: k + oversamples must be <= 64");
+  PCAONE_REQUIRE(l <= N_ && l <= M_, -1, "rsvd_init: k + oversamples exceeds the matrix dimensions");
+  p_ = p;
+  l_ = l;
+  // windows: winSVD uses blocksize = ceil(M / bands) (src/Halko.cpp:178); the
+  // plain RSVD uses the same cut only as a tiling of the H contraction.
+  bs_ = (uint32_t)ceil_div(M_, (uint64_t)p.bands);
+  nwin_ = (uint32_t)ceil_div(M_, bs_);
+  if (p.variant == 2)
+    PCAONE_REQUIRE(nwin_ == (uint32_t)p.bands, -1,
+                   "winSVD needs every window non-empty: nsnps too small for the number of windows");
+  PCAONE_REQUIRE(N_ >= 2 * l_, -1, "the GPU operator needs nsamples >= 2 (k + oversamples)");
+  // permutation (src/RSVD.hpp:63-79): only the winSVD path shuffles
+  perm_.resize(M_);
+  std::iota(perm_.begin(), perm_.end(), 0);
+  if (p.variant == 2 && p.perm) {
+    if (perm) {
+      std::vector<char> seen(M_, 0);
+      for (uint32_t i = 0; i < M_; ++i) {
+        PCAONE_REQUIRE(perm[i] >= 0 && (uint32_t)perm[i] < M_ && !seen[perm[i]], -1, "rsvd_init: invalid perm");
+        seen[perm[i]] = 1;
+        perm_[i] = perm[i];
+      }
+    } else {
+      auto rng = std::default_random_engine{};
+      std::shuffle(perm_.data(), perm_.data() + perm_.size(), rng);
+    }
+  }
+  omega_given_.clear();
+  if (omega) omega_given_.assign(omega, omega + (size_t)N_ * l_);
+  build_layout();
+
+  // operands and partials
+  const int prec = p.precision;
+  nt16_ = (int)ceil_div(2 * l_, 16);
+  bcols_ = prec == PREC_SPLIT_BF16 ? (uint32_t)nt16_ * 16 : l_;
+  opsz_ = prec == PREC_SPLIT_BF16 ? 2 : prec == PREC_FP32 ? 4 : 8;
+  const size_t partsz = prec == PREC_FP64 ? 8 : 4;
+  const uint64_t ldp = prec == PREC_SPLIT_BF16 ? (uint64_t)nt16_ * 16 : l_;
+  omega_.reserve((uint64_t)N_ * l_);
+  omg2_.reserve((uint64_t)N_ * l_);
+  colsum_.reserve(l_);
+  bg_.reserve((uint64_t)bcols_ * npad_ * opsz_);
+  bh_.reserve((uint64_t)nwin_ * bcols_ * kw_ * opsz_);
+  PCAONE_HIP(hipMemsetAsync(bh_.p, 0, bh_.bytes(), stream_));
+  G_.reserve((uint64_t)M_ * l_);
+  g2_.reserve((uint64_t)M_ * l_);
+  S2_.reserve((uint64_t)nwin_ * l_);
+  for (auto* h : {&H1_, &H2_, &H_}) {
+    h->reserve((uint64_t)N_ * l_);
+    PCAONE_HIP(hipMemsetAsync(h->p, 0, h->bytes(), stream_));
+  }
+  uint64_t gmax = 0, hmax = 0;
+  const uint32_t rows_per_block = prec == PREC_SPLIT_BF16 ? 256 : 512;
+  for (uint32_t nw = 1; nw <= nwin_; ++nw) {
+    const uint64_t rows = std::min<uint64_t>((uint64_t)nw * bs_, M_);
+    uint32_t cps;
+    const uint32_t sg = split_for(ceil_div(rows, rows_per_block), row_bytes_ / 64, cps);
+    gmax = std::max<uint64_t>(gmax, sg * rows);
+    const uint32_t sh = split_for(ceil_div(N_, rows_per_block) * nw, win_bytes_ / 64, cps);
+    hmax = std::max<uint64_t>(hmax, (uint64_t)sh * N_ * nw);
+  }
+  gpart_.reserve(gmax * ldp * partsz);
+  hpart_.reserve(hmax * ldp * partsz);
+  // linear algebra scratch
+  part_.reserve((uint64_t)256 * l_ * l_);
+  fpart_.reserve((uint64_t)256 * 3 * l_);
+  for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &W_}) b->reserve((uint64_t)l_ * l_);
+  sign_.reserve(l_);
+  evals_.reserve(l_);
+  sinv_.reserve(l_);
+  Q_.reserve((uint64_t)N_ * l_);
+  Bt_.reserve((uint64_t)N_ * l_);
+  Ucur_.reserve((uint64_t)N_ * p.k);
+  Upre_.reserve((uint64_t)N_ * p.k);
+  V_.reserve((uint64_t)M_ * p.k);
+  scal_.reserve(4);
+  status_.reserve(2);
+  PCAONE_HIP(hipMemsetAsync(status_.p, 0, 2 * sizeof(int), stream_));
+  bandsize_ = 1;
+  have_op_ = true;
+  init_omega();
+  sync();
+}
+
+void Engine::build_layout() {
+  const uint64_t nb = (N_ + 3) >> 2;
+  row_bytes_ = (uint32_t)round_up(nb, 64);
+  npad_ = row_bytes_ * 4;
+  win_bytes_ = (uint32_t)round_up(ceil_div(bs_, 4), 64);
+  kw_ = (uint64_t)win_bytes_ * 4;
+  d_perm_.reserve(M_);
+  PCAONE_HIP(hipMemcpyAsync(d_perm_.p, perm_.data(), (size_t)M_ * 4, hipMemcpyHostToDevice, stream_));
+  bed_snp_.reserve((uint64_t)row_bytes_ * M_);
+  timed("bed_layout", [&] { launch_bed_to_snpmajor(raw_.p, N_, M_, d_perm_.p, bed_snp_.p, row_bytes_, stream_); });
+  bed_smp_.reserve((uint64_t)nwin_ * npad_ * win_bytes_);
+  PCAONE_HIP(hipMemsetAsync(bed_smp_.p, 0x55, bed_smp_.bytes(), stream_));
+  timed("bed_transpose", [&] {
+    launch_transpose_windows(bed_snp_.p, row_bytes_, M_, bs_, nwin_, bed_smp_.p, npad_, win_bytes_, stream_);
+  });
+  // F and the scale in laid-out order (src/Data.cpp:290-299, VAR_TOL = 1e-9)
+  std::vector<double> F(M_), d(M_);
+  for (uint32_t r = 0; r < M_; ++r) {
+    const double f = F_orig_[perm_[r]];
+    F[r] = f;
+    const double sd = std::sqrt((double)p_.ploidy * f * (1 - f));
+    d[r] = (p_.standardize && sd > 1e-9) ? 1.0 / sd : 1.0;
+  }
+  F_.reserve(M_);
+  d_.reserve(M_);
+  PCAONE_HIP(hipMemcpyAsync(F_.p, F.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(d_.p, d.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  // missing lists
+  auto build = [&](const PackedRows& pr, DevBuf<int64_t>& ptr, DevBuf<int32_t>& idx, const char* nm) {
+    DevBuf<int32_t> cnt;
+    cnt.reserve(pr.nrows);
+    timed(nm, [&] { launch_count_missing(pr, cnt.p, stream_); });
+    std::vector<int32_t> hc(pr.nrows);
+    PCAONE_HIP(hipMemcpyAsync(hc.data(), cnt.p, (size_t)pr.nrows * 4, hipMemcpyDeviceToHost, stream_));
+    sync();
+    std::vector<int64_t> hp((size_t)pr.nrows + 1, 0);
+    for (uint32_t r = 0; r < pr.nrows; ++r) hp[r + 1] = hp[r] + hc[r];
+    ptr.reserve(hp.size());
+    PCAONE_HIP(hipMemcpyAsync(ptr.p, hp.data(), hp.size() * 8, hipMemcpyHostToDevice, stream_));
+    idx.reserve(std::max<int64_t>(hp.back(), 1));
+    timed(nm, [&] { launch_fill_missing(pr, ptr.p, idx.p, stream_); });
+    sync();  // hp must outlive the copy
+  };
+  PackedRows ps{bed_snp_.p, row_bytes_, M_, M_, M_, N_, N_, 1, 0};
+  build(ps, miss_snp_ptr_, miss_snp_idx_, "missing_lists");
+  PackedRows pw{bed_smp_.p, win_bytes_, nwin_ * npad_, npad_, N_, bs_, M_ - (nwin_ - 1) * bs_, nwin_, bs_};
+  build(pw, miss_win_ptr_, miss_win_idx_, "missing_lists");
+}
+
+// RsvdOpData::initOmg, src/Halko.cpp:14-22: default_random_engine seeded with
+// --seed, an N x l NullaryExpr filled in column-major order.
+void Engine::init_omega() {
+  std::vector<double> cm((size_t)N_ * l_);
+  if (!omega_given_.empty()) {
+    cm = omega_given_;
+  } else {
+    auto rng = std::default_random_engine{};
+    rng.seed(p_.seed);
+    if (p_.rand) {
+      std::normal_distribution<double> dist{0, 1};
+      for (auto& v : cm) v = dist(rng);
+    } else {
+      std::uniform_real_distribution<double> dist{-1, 1};
+      for (auto& v : cm) v = dist(rng);
+    }
+  }
+  std::vector<double> rm((size_t)N_ * l_), cs(l_, 0.0);
+  for (uint32_t c = 0; c < l_; ++c)
+    for (uint32_t r = 0; r < N_; ++r) {
+      const double v = cm[(size_t)c * N_ + r];
+      rm[(size_t)r * l_ + c] = v;
+      cs[c] += v;
+    }
+  PCAONE_HIP(hipMemcpyAsync(omega_.p, rm.data(), rm.size() * 8, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(omg2_.p, rm.data(), rm.size() * 8, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(colsum_.p, cs.data(), cs.size() * 8, hipMemcpyHostToDevice, stream_));
+  timed("omega_operand",
+        [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, npad_, stream_); });
+  sync();  // host vectors go out of scope
+}
+
+// Omega <- Householder Q of H (H = H1 + H2 for winSVD), then flipOmg
+// (src/Halko.cpp:118-121, 213-216).  CholeskyQR2 in FP64 with the Householder
+// R-diagonal signs reproduced, so Q equals Eigen's householderQ() columns.
+void Engine::omega_refresh(bool from_two, double* zero_acc) {
+  const uint32_t l = l_;
+  const int nb = gram_blocks(N_ - l);
+  timed("qr_gram", [&] {
+    if (from_two)
+      launch_gram_partial(H1_.p, H2_.p, H_.p, N_, l, l, part_.p, nb, stream_);
+    else
+      launch_gram_partial(H_.p, nullptr, nullptr, N_, l, l, part_.p, nb, stream_);
+  });
+  timed("qr_chol", [&] {
+    launch_chol(part_.p, nb, H_.p, l, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, sign_.p, status_.p,
+                stream_);
+  });
+  const int nb2 = gram_blocks(N_);
+  timed("qr_apply", [&] { launch_apply_rinv(H_.p, R1i_.p, nullptr, Q_.p, N_, l, part_.p, nb2, stream_); });
+  timed("qr_chol", [&] {
+    launch_chol(part_.p, nb2, nullptr, 0, l, R2_.p, R2i_.p, nullptr, nullptr, nullptr, nullptr, nullptr,
+                status_.p + 1, stream_);
+  });
+  timed("qr_apply", [&] { launch_apply_rinv(Q_.p, R2i_.p, sign_.p, omega_.p, N_, l, nullptr, nb2, stream_); });
+  timed("flip", [&] { launch_flip_stats(omega_.p, omg2_.p, N_, l, fpart_.p, nb2, stream_); });
+  timed("flip", [&] { launch_flip_apply(omega_.p, omg2_.p, N_, l, fpart_.p, nb2, colsum_.p, zero_acc, stream_); });
+  timed("omega_operand",
+        [&] { launch_omega_operand(omega_.p, N_, l, p_.precision, bg_.p, bcols_, npad_, stream_); });
+}
+
+// G rows of windows [w0, w1) and their H contributions, routed per window.
+void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route) {
+  const int prec = p_.precision;
+  const uint32_t r0 = w0 * bs_, r1 = std::min<uint32_t>(w1 * bs_, M_);
+  const uint32_t rows = r1 - r0;
+  const uint64_t ldp = prec == PREC_SPLIT_BF16 ? (uint64_t)nt16_ * 16 : l_;
+  const uint32_t rpb = prec == PREC_SPLIT_BF16 ? 256 : 512;
+  // ---- G = X_seg^T Omega --------------------------------------------------
+  PackedGemm g{};
+  g.A = bed_snp_.p + (uint64_t)r0 * row_bytes_;
+  g.lda = row_bytes_;
+  g.a_zstride = 0;
+  g.nrows = rows;
+  g.kchunks = row_bytes_ / 64;
+  g.nsplit = split_for(ceil_div(rows, rpb), g.kchunks, g.chunks_per_split);
+  g.B = bg_.p;
+  g.ldb = npad_;
+  g.b_zstride = 0;
+  g.out = gpart_.p;
+  g.ldo = ldp;
+  g.o_zstride = 0;
+  g.nz = 1;
+  timed("gemm_G", [&] {
+    if (prec == PREC_SPLIT_BF16)
+      launch_packed_gemm_bf16(g, nt16_, stream_);
+    else if (prec == PREC_FP32)
+      launch_packed_gemm_f32(g, (int)l_, stream_);
+    else
+      launch_packed_gemm_f64(g, (int)l_, stream_);
+  });
+  FinalizeG fg{};
+  fg.part = gpart_.p;
+  fg.part_is_f64 = prec == PREC_FP64;
+  fg.nsplit = g.nsplit;
+  fg.ldp = ldp;
+  fg.split_cols = prec == PREC_SPLIT_BF16;
+  fg.r0 = r0;
+  fg.r1 = r1;
+  fg.l = l_;
+  fg.F = F_.p;
+  fg.d = d_.p;
+  fg.colsum = colsum_.p;
+  fg.miss_ptr = miss_snp_ptr_.p;
+  fg.miss_idx = miss_snp_idx_.p;
+  fg.omega = omega_.p;
+  fg.G = G_.p;
+  fg.g2 = g2_.p;
+  fg.bprec = prec;
+  fg.bh = bh_.p;
+  fg.bcols = bcols_;
+  fg.kw = kw_;
+  fg.blocksize = bs_;
+  timed("finalize_G", [&] { launch_finalize_G(fg, stream_); });
+  timed("window_colsum", [&] { launch_window_colsum(g2_.p, l_, M_, bs_, w0, w1, S2_.p, stream_); });
+  // ---- H += X_w G_w per window -------------------------------------------
+  const uint32_t nz = w1 - w0;
+  PackedGemm h{};
+  h.A = bed_smp_.p + (uint64_t)w0 * npad_ * win_bytes_;
+  h.lda = win_bytes_;
+  h.a_zstride = (uint64_t)npad_ * win_bytes_;
+  h.nrows = N_;
+  h.kchunks = win_bytes_ / 64;
+  h.nsplit = split_for(ceil_div(N_, rpb) * nz, h.kchunks, h.chunks_per_split);
+  h.B = bh_.p + (uint64_t)w0 * bcols_ * kw_ * opsz_;
+  h.ldb = kw_;
+  h.b_zstride = (uint64_t)bcols_ * kw_;
+  h.out = hpart_.p;
+  h.ldo = ldp;
+  h.o_zstride = (uint64_t)h.nsplit * N_ * ldp;
+  h.nz = nz;
+  timed("gemm_H", [&] {
+    if (prec == PREC_SPLIT_BF16)
+      launch_packed_gemm_bf16(h, nt16_, stream_);
+    else if (prec == PREC_FP32)
+      launch_packed_gemm_f32(h, (int)l_, stream_);
+    else
+      launch_packed_gemm_f64(h, (int)l_, stream_);
+  });
+  FinalizeH fh{};
+  fh.part = hpart_.p;
+  fh.part_is_f64 = prec == PREC_FP64;
+  fh.nsplit = h.nsplit;
+  fh.ldp = ldp;
+  fh.part_rows = N_;
+  fh.miss_rows = npad_;
+  fh.split_cols = prec == PREC_SPLIT_BF16;
+  fh.nsamples = N_;
+  fh.l = l_;
+  fh.wpart0 = w0;
+  fh.Ha = (p_.variant == 2) ? H1_.p : H_.p;
+  fh.Hb = (p_.variant == 2) ? H2_.p : H_.p;
+  fh.S2 = S2_.p;
+  fh.miss_ptr = miss_win_ptr_.p;
+  fh.miss_idx = miss_win_idx_.p;
+  fh.g2 = g2_.p;
+  for (uint32_t a = w0; a < w1; a += 64) {  // route travels as a 64-bit mask
+    fh.w0 = a;
+    fh.w1 = std::min<uint32_t>(a + 64, w1);
+    fh.route_b = 0;
+    for (uint32_t w = fh.w0; w < fh.w1; ++w)
+      if (route[w] == 2) fh.route_b |= 1ull << (w - a);
+    timed("finalize_H", [&] { launch_finalize_H(fh, stream_); });
+  }
+}
+
+void Engine::rsvd_gandh(int pi, double* Gout, double* Hout) {
+  PCAONE_REQUIRE(have_op_, -3, "computeGandH: operator not initialised");
+  PCAONE_REQUIRE(pi >= 0, -1, "computeGandH: pi must be >= 0");
+  PCAONE_HIP(hipSetDevice(device_));
+  if (p_.variant == 1) {
+    // NormalRsvdOpData in-core, src/Halko.cpp:105-126
+    if (pi == 0)
+      init_omega();
+    else
+      omega_refresh(false, nullptr);
+    PCAONE_HIP(hipMemsetAsync(H_.p, 0, H_.bytes(), stream_));
+    run_segment(0, nwin_, std::vector<int>(nwin_, 1));
+  } else {
+    // FancyRsvdOpData in-core, src/Halko.cpp:153-226
+    if (pi == 0) init_omega();
+    const FancyPlan pl = fancy_plan(p_.bands, pi, bandsize_);
+    if (pl.reset_both) {
+      PCAONE_HIP(hipMemsetAsync(H1_.p, 0, H1_.bytes(), stream_));
+      PCAONE_HIP(hipMemsetAsync(H2_.p, 0, H2_.bytes(), stream_));
+    }
+    std::vector<int> route(nwin_, 1);
+    for (uint32_t b = 0; b < nwin_; ++b) route[b] = pl.steps[b].acc;
+    uint32_t start = 0;
+    for (uint32_t b = 0; b < nwin_; ++b) {
+      const WindowStep& st = pl.steps[b];
+      if (!st.update && b + 1 < nwin_) continue;
+      run_segment(start, b + 1, route);
+      start = b + 1;
+      if (st.update) omega_refresh(true, st.zero == 1 ? H1_.p : H2_.p);
+    }
+  }
+  if (Gout || Hout) {
+    sync();
+    std::vector<double> tmp;
+    if (Gout) {
+      tmp.resize((size_t)M_ * l_);
+      PCAONE_HIP(hipMemcpy(tmp.data(), G_.p, tmp.size() * 8, hipMemcpyDeviceToHost));
+      for (uint32_t r = 0; r < M_; ++r)
+        for (uint32_t c = 0; c < l_; ++c) Gout[(size_t)c * M_ + r] = tmp[(size_t)r * l_ + c];
+    }
+    if (Hout) {
+      tmp.resize((size_t)N_ * l_);
+      PCAONE_HIP(hipMemcpy(tmp.data(), H_.p, tmp.size() * 8, hipMemcpyDeviceToHost));
+      for (uint32_t r = 0; r < N_; ++r)
+        for (uint32_t c = 0; c < l_; ++c) Hout[(size_t)c * N_ + r] = tmp[(size_t)r * l_ + c];
+    }
+  }
+}
+
+void Engine::check_status() {
+  int st[2] = {0, 0};
+  PCAONE_HIP(hipMemcpyAsync(st, status_.p, sizeof(st), hipMemcpyDeviceToHost, stream_));
+  sync();
+  if (st[0] || st[1]) {
+    PCAONE_HIP(hipMemsetAsync(status_.p, 0, 2 * sizeof(int), stream_));
+    throw Error(-4, "CholeskyQR breakdown (pivot " + std::to_string(st[0] ? st[0] : st[1]) +
+                        "): the sketch is numerically rank deficient");
+  }
+}
+
+// One pass of computeUSV's linear algebra (src/Halko.cpp:53-68): Q of G by
+// CholeskyQR2, B^T = H R^-1, B B^T = U_B S^2 U_B^T, Ucur = B^T U_B S^-1.
+void Engine::usv_step() {
+  const uint32_t l = l_;
+  const int nbG = gram_blocks(M_), nbN = gram_blocks(N_);
+  timed("usv_gram", [&] { launch_gram_partial(G_.p, nullptr, nullptr, M_, 0, l, part_.p, nbG, stream_); });
+  timed("usv_chol", [&] {
+    launch_chol(part_.p, nbG, nullptr, 0, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, nullptr, status_.p,
+                stream_);
+  });
+  timed("usv_apply", [&] { launch_apply_rinv(G_.p, R1i_.p, nullptr, G_.p, M_, l, part_.p, nbG, stream_); });
+  timed("usv_chol", [&] {
+    launch_chol(part_.p, nbG, nullptr, 0, l, R2_.p, R2i_.p, R1_.p, R1i_.p, Rt_.p, Rti_.p, nullptr, status_.p + 1,
+                stream_);
+  });
+  timed("usv_apply", [&] { launch_apply_rinv(G_.p, R2i_.p, nullptr, G_.p, M_, l, nullptr, nbG, stream_); });
+  timed("usv_apply", [&] { launch_apply_rinv(H_.p, Rti_.p, nullptr, Bt_.p, N_, l, part_.p, nbN, stream_); });
+  timed("usv_eig", [&] { launch_sym_eig(part_.p, nbN, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, stream_); });
+  timed("usv_rows", [&] {
+    launch_rows_times_small(Bt_.p, N_, l, W_.p, l, (uint32_t)p_.k, sinv_.p, Ucur_.p, stream_);
+  });
+}
+
+void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int* epochs, double* diffs) {
+  PCAONE_REQUIRE(have_op_, -3, "computeUSV: operator not initialised");
+  PCAONE_REQUIRE(maxp >= 0, -1, "computeUSV: p must be >= 0");
+  PCAONE_HIP(hipSetDevice(device_));
+  const uint32_t k = (uint32_t)p_.k;
+  int p = maxp, ep = 0;
+  if (diffs)
+    for (int i = 0; i <= maxp; ++i) diffs[i] = 0.0;
+  bool done = false;
+  for (int pi = 0; pi <= p; ++pi) {
+    rsvd_gandh(pi, nullptr, nullptr);
+    ++ep;
+    usv_step();
+    check_status();
+    if (pi > 0) {
+      const int nbN = gram_blocks(N_);
+      timed("mev", [&] { launch_xty_partial(Ucur_.p, Upre_.p, N_, k, part_.p, nbN, stream_); });
+      timed("mev", [&] { launch_mev_finish(part_.p, nbN, k, scal_.p, stream_); });
+      double m = 0.0;
+      PCAONE_HIP(hipMemcpyAsync(&m, scal_.p, 8, hipMemcpyDeviceToHost, stream_));
+      sync();
+      const double diff = 1.0 - m;
+      if (diffs && pi <= maxp) diffs[pi] = diff;
+      if (diff < tol || pi == p) {
+        if (p_.variant == 2 && std::ldexp(1.0, pi) < (double)p_.bands) {
+          p = (int)std::log2((double)p_.bands);  // winSVD runs log2(bands)+1 epochs at least
+        } else {
+          done = true;
+          break;
+        }
+      } else {
+        std::swap(Ucur_.p, Upre_.p);
+      }
+    } else {
+      std::swap(Ucur_.p, Upre_.p);
+    }
+  }
+  if (!done) std::swap(Ucur_.p, Upre_.p);  // the last Ucur sits in Upre after the swap
+  // V = Q U_B[:, :k] (rows in laid-out order), S = sqrt(eig(B B^T))[:k]
+  timed("usv_rows", [&] { launch_rows_times_small(G_.p, M_, l_, W_.p, l_, k, nullptr, V_.p, stream_); });
+  std::vector<double> ev(l_), u((size_t)N_ * k), v((size_t)M_ * k);
+  PCAONE_HIP(hipMemcpyAsync(ev.data(), evals_.p, l_ * 8, hipMemcpyDeviceToHost, stream_));
+  PCAONE_HIP(hipMemcpyAsync(u.data(), Ucur_.p, u.size() * 8, hipMemcpyDeviceToHost, stream_));
+  PCAONE_HIP(hipMemcpyAsync(v.data(), V_.p, v.size() * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+  if (S)
+    for (uint32_t c = 0; c < k; ++c) S[c] = std::sqrt(std::max(ev[c], 0.0));
+  if (U)
+    for (uint32_t r = 0; r < N_; ++r)
+      for (uint32_t c = 0; c < k; ++c) U[(size_t)c * N_ + r] = u[(size_t)r * k + c];
+  if (V)
+    for (uint32_t r = 0; r < M_; ++r)
+      for (uint32_t c = 0; c < k; ++c) V[(size_t)c * M_ + perm_[r]] = v[(size_t)r * k + c];
+  if (epochs) *epochs = ep;
+}
+
+}  // namespace pcaone

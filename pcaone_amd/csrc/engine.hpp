@@ -1,0 +1,118 @@
+// Host engine behind the C ABI: owns the HIP stream, the HBM layout of the
+// packed genotypes and the RSVD operator state (the members of RsvdOpData /
+// NormalRsvdOpData / FancyRsvdOpData, src/Halko.hpp:7-85).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pcaone {
+
+struct KernelTimer {
+  bool on = false;
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::map<std::string, std::pair<int64_t, double>> agg;  // name -> (count, total ms)
+  void resolve();
+  void clear();
+  ~KernelTimer();
+};
+
+// One step of the winSVD window loop (src/Halko.cpp:189-223).
+struct WindowStep {
+  int acc;     // 1: H1 += window, 2: H2 += window
+  bool update; // Omega refreshed from H1 + H2 after this window
+  int zero;    // accumulator cleared after the refresh (1 or 2), 0 if none
+};
+
+// Plan of one FancyRsvdOpData::computeGandH call; `bandsize` is the state the
+// operator carries between calls (reset to 1 at pi == 0).
+struct FancyPlan {
+  bool reset_both;
+  std::vector<WindowStep> steps;
+};
+FancyPlan fancy_plan(int bands, int pi, uint64_t& bandsize);
+
+struct RsvdParams {
+  int variant = 2, k = 10, os = 10, bands = 64, precision = 2, seed = 101, rand = 1, perm = 1, standardize = 1,
+      ploidy = 2;
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+
+  void load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps);
+  void allele_freq(double* F, int64_t* counts) const;
+
+  void rsvd_init(const RsvdParams& p, const double* omega, const int32_t* perm);
+  void rsvd_gandh(int pi, double* G, double* H);
+  void rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int* epochs, double* diffs);
+  void rsvd_perm(int32_t* perm) const;
+  void dims(int64_t* rows, int64_t* cols, int64_t* size) const;
+
+  KernelTimer timer;
+  void sync();
+  std::string last_error;
+
+ private:
+  template <class F>
+  void timed(const char* name, F&& launch);
+
+  void build_layout();
+  void init_omega();
+  void omega_refresh(bool from_two, double* zero_acc);
+  void run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route);
+  void usv_step();
+  void check_status();
+
+  int device_;
+  hipStream_t stream_ = nullptr;
+
+  // ---- data --------------------------------------------------------------
+  uint32_t N_ = 0, M_ = 0;
+  bool have_bed_ = false;
+  DevBuf<uint8_t> raw_;            // payload as loaded (M x ceil(N/4))
+  std::vector<double> F_orig_;     // allele frequency, original order
+  std::vector<int64_t> counts_orig_;
+
+  // ---- layout (depends on the permutation and the windowing) -------------
+  bool have_op_ = false;
+  RsvdParams p_;
+  uint32_t l_ = 0, nwin_ = 0, bs_ = 0;
+  uint32_t row_bytes_ = 0, npad_ = 0, win_bytes_ = 0;
+  uint64_t kw_ = 0;
+  std::vector<int32_t> perm_;      // laid-out row r holds SNP perm_[r]
+  DevBuf<int32_t> d_perm_;
+  DevBuf<uint8_t> bed_snp_, bed_smp_;
+  DevBuf<double> F_, d_;
+  DevBuf<int64_t> miss_snp_ptr_, miss_win_ptr_;
+  DevBuf<int32_t> miss_snp_idx_, miss_win_idx_;
+
+  // ---- operator state ----------------------------------------------------
+  int nt16_ = 0;        // 16-column tiles of the split operand
+  uint32_t bcols_ = 0;  // columns of the contraction operands
+  size_t opsz_ = 8;     // bytes per operand element
+  std::vector<double> omega_given_;
+  DevBuf<double> omega_, omg2_, colsum_;
+  DevBuf<unsigned char> bg_, bh_;   // operands of the G and H contractions
+  DevBuf<double> G_, g2_, S2_, H1_, H2_, H_;
+  DevBuf<unsigned char> gpart_, hpart_;
+  uint64_t bandsize_ = 1;
+  // linear algebra scratch
+  DevBuf<double> part_, fpart_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, sign_, Q_;
+  DevBuf<double> Bt_, evals_, W_, sinv_, Ucur_, Upre_, V_, scal_;
+  DevBuf<int> status_;
+};
+
+}  // namespace pcaone

@@ -1,0 +1,166 @@
+// Host-side launch wrappers for the gfx950 kernels.  Every wrapper enqueues on
+// the given stream and returns immediately; argument shapes are checked by the
+// callers in rsvd.cpp before anything is launched.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pcaone {
+
+// ---- layout of the packed genotype matrix in HBM --------------------------
+// bed_snp : [M][row_bytes]     SNP-major, row_bytes = round_up(ceil(N/4), 64);
+//           code 01 (missing) in every padding position, so padding decodes
+//           to genotype 0 and is never counted as data.
+// bed_smp : [nwin][Npad][win_bytes] sample-major copy cut into SNP windows,
+//           win_bytes = round_up(ceil(max_window/4), 64), Npad = 4*row_bytes.
+
+void launch_bed_to_snpmajor(const uint8_t* raw, uint32_t nsamples, uint32_t nsnps, const int32_t* perm,
+                            uint8_t* bed_snp, uint32_t row_bytes, hipStream_t s);
+
+void launch_count_codes(const uint8_t* bed_snp, uint32_t nrows, uint32_t row_bytes, uint32_t nvalid,
+                        int32_t* counts4, hipStream_t s);
+
+void launch_transpose_windows(const uint8_t* bed_snp, uint32_t row_bytes, uint32_t nsnps, uint32_t blocksize,
+                              uint32_t nwin, uint8_t* bed_smp, uint32_t npad, uint32_t win_bytes,
+                              hipStream_t s);
+
+// Missing-code lists.  Row r of a packed matrix holds `nvalid(r)` codes; the
+// kernels either count the 01 codes per row or write their positions
+// (+ index_base(r)) into idx[ptr[r] ..].
+struct PackedRows {
+  const uint8_t* data;
+  uint32_t row_bytes;
+  uint32_t nrows;
+  // rows are grouped: group g = r / rows_per_group covers codes
+  // [0, valid_last) when g is the last group, else [0, valid); rows whose
+  // in-group index >= live_rows contribute nothing.
+  uint32_t rows_per_group;
+  uint32_t live_rows;
+  uint32_t valid;
+  uint32_t valid_last;
+  uint32_t ngroups;
+  uint32_t base_step;  // index_base(r) = group * base_step
+};
+void launch_count_missing(const PackedRows& pr, int32_t* counts, hipStream_t s);
+void launch_fill_missing(const PackedRows& pr, const int64_t* ptr, int32_t* idx, hipStream_t s);
+
+// ---- contractions over packed rows ----------------------------------------
+// out[z][split][row][col] = sum_{k in split} y(row,k) * B[z][col][k],
+// y = A1 dosage of the 2-bit code (00->2, 10->1, 11->0, 01->0).
+struct PackedGemm {
+  const uint8_t* A;
+  uint64_t lda;        // bytes per packed row (multiple of 64)
+  uint64_t a_zstride;  // bytes between z-slices of A
+  uint32_t nrows;      // valid rows per z-slice
+  uint32_t kchunks;    // number of 256-code chunks in a row
+  uint32_t chunks_per_split;
+  uint32_t nsplit;
+  const void* B;       // [z][ncols_b][ldb]
+  uint64_t ldb;        // elements
+  uint64_t b_zstride;  // elements
+  void* out;           // [z][nsplit][nrows][ldo]
+  uint64_t ldo;
+  uint64_t o_zstride;  // elements between z-slices
+  uint32_t nz;
+};
+// split-BF16 MFMA: B is bf16 with nt16*16 columns ([hi | lo | 0-pad]); out fp32
+void launch_packed_gemm_bf16(const PackedGemm& g, int nt16, hipStream_t s);
+// FP32 / FP64 vector-FMA: B has `ncols` columns of float / double; out same type
+void launch_packed_gemm_f32(const PackedGemm& g, int ncols, hipStream_t s);
+void launch_packed_gemm_f64(const PackedGemm& g, int ncols, hipStream_t s);
+
+// ---- epilogues -------------------------------------------------------------
+// G rows [r0, r1): reduce split partials, add the missing-mask correction and
+// the centring/scaling, then emit the H-contraction operand for each row.
+struct FinalizeG {
+  const void* part;     // [nsplit][nrows][ldp]  (float or double)
+  int part_is_f64;
+  uint32_t nsplit;
+  uint64_t ldp;
+  int split_cols;       // 1: value = part[c] + part[l + c]
+  uint32_t r0, r1;      // global (laid-out) SNP rows
+  uint32_t l;
+  const double* F;      // [M] allele frequency, laid-out order
+  const double* d;      // [M] 1/sd or 1
+  const double* colsum; // [l]  sum_j Omega[j][c]
+  const int64_t* miss_ptr;  // [M+1]
+  const int32_t* miss_idx;  // sample indices
+  const double* omega;  // [N][l]
+  double* G;            // [M][l]
+  double* g2;           // [M][l]  2 f d G
+  // H operand, per window: [nwin][bcols][kw]
+  int bprec;            // 0 fp64, 1 fp32, 2 bf16 split
+  void* bh;
+  uint32_t bcols;       // columns in the operand (l, or nt16*16 for split)
+  uint64_t kw;          // codes per window row (win_bytes*4)
+  uint32_t blocksize;   // window length (last may be shorter)
+};
+void launch_finalize_G(const FinalizeG& f, hipStream_t s);
+
+// S2[w][c] = sum over window w's rows of g2[r][c] for windows [w0, w1)
+void launch_window_colsum(const double* g2, uint32_t l, uint32_t nsnps, uint32_t blocksize, uint32_t w0,
+                          uint32_t w1, double* S2, hipStream_t s);
+
+struct FinalizeH {
+  const void* part;     // [nwseg][nsplit][part_rows][ldp]
+  int part_is_f64;
+  uint32_t nsplit;
+  uint64_t ldp;
+  uint32_t part_rows;   // rows per split slice of the partials
+  uint32_t miss_rows;   // row stride of the per-window missing pointers (Npad)
+  int split_cols;
+  uint32_t nsamples, l;
+  uint32_t w0, w1;      // windows of this launch (w1 - w0 <= 64)
+  uint32_t wpart0;      // window whose partials sit at index 0 of `part`
+  uint64_t route_b;     // bit (w - w0) set: window w adds into Hb, else Ha
+  double* Ha;
+  double* Hb;
+  const double* S2;     // [nwin][l]
+  const int64_t* miss_ptr;  // [nwin*miss_rows+1]
+  const int32_t* miss_idx;  // SNP rows
+  const double* g2;     // [M][l]
+};
+void launch_finalize_H(const FinalizeH& f, hipStream_t s);
+
+// Omega (FP64 [N][l]) -> G-contraction operand [bcols][kpad] in the arm's type.
+void launch_omega_operand(const double* omega, uint32_t n, uint32_t l, int bprec, void* bg, uint32_t bcols,
+                          uint64_t kpad, hipStream_t s);
+
+void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t s);
+
+// ---- FP64 tall-skinny linear algebra --------------------------------------
+// partial Gram: part[b] = sum_{rows of block b} A[r]^T A[r], rows [row0, nrows)
+// of A (+ A2 when non-null, in which case Aout = A + A2 is stored).
+int gram_blocks(uint64_t nrows);
+void launch_gram_partial(const double* A, const double* A2, double* Aout, uint64_t nrows, uint32_t row0,
+                         uint32_t l, double* part, int nblocks, hipStream_t s);
+// Single workgroup: Gram = sum(part) (+ top^T top for rows [0,row0) of A),
+// R = chol(Gram) upper, Rinv = R^-1, optional Rprev: Rtot = R * Rprev and
+// Rtot_inv = Rprev_inv * Rinv.  sign (optional, l): Householder R-diagonal
+// signs of A reproduced from A's top rows and the bottom Gram.
+void launch_chol(const double* part, int nparts, const double* A, uint32_t row0, uint32_t l, double* R,
+                 double* Rinv, const double* Rprev, const double* Rprev_inv, double* Rtot, double* Rtot_inv,
+                 double* sign, int* status, hipStream_t s);
+// Q = A * Rinv (rows), optional per-column sign; optional partial Gram of Q.
+void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, double* Q, uint64_t nrows,
+                       uint32_t l, double* gram_part, int nblocks, hipStream_t s);
+// flipOmg against Omega2 + colsum: stats per block, then apply.
+void launch_flip_stats(const double* Q, const double* omg2, uint64_t nrows, uint32_t l, double* part,
+                       int nblocks, hipStream_t s);
+void launch_flip_apply(double* Q, double* omg2, uint64_t nrows, uint32_t l, const double* part, int nblocks,
+                       double* colsum, double* zero_acc, hipStream_t s);
+// small dense: C[n][kc] = A[n][l] * W[l][ldw] (first kc columns) * diag(scale)
+void launch_rows_times_small(const double* A, uint64_t n, uint32_t l, const double* W, uint32_t ldw,
+                             uint32_t kc, const double* scale, double* C, hipStream_t s);
+// symmetric eigen-decomposition of sum(part) (l x l) by cyclic Jacobi in one
+// workgroup; eigenvalues descending into evals, eigenvectors (columns) into W.
+void launch_sym_eig(const double* part, int nparts, uint32_t l, double* evals, double* W, double* sinv,
+                    uint32_t kc, hipStream_t s);
+// mev partials: M = X^T Y (k x k) per block
+void launch_xty_partial(const double* X, const double* Y, uint64_t n, uint32_t k, double* part, int nblocks,
+                        hipStream_t s);
+void launch_mev_finish(const double* part, int nparts, uint32_t k, double* out, hipStream_t s);
+
+}  // namespace pcaone

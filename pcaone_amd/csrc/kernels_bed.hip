@@ -1,0 +1,559 @@
+// Packed PLINK genotype kernels for gfx950: HBM layout, code counts, missing
+// lists, the two contraction families (split-BF16 MFMA and FP32/FP64 vector
+// FMA) and their epilogues.
+//
+// The algebra (reference layout X is N x M, column i = SNP i):
+//   X[j,i] = d_i * (BED2GENO[c_ij] - F_i), missing -> 0     (src/FilePlink.cpp:106-114,
+//                                                            src/Data.cpp:290-299)
+// With y = A1 dosage in {0,1,2} (BED2GENO = y/2; y = 0 for missing),
+//   G_i = (X^T Omega)_i = 1/2 d_i ( sum_j y_ij Omega_j - 2 F_i ( colsum(Omega) - sum_{j miss} Omega_j ) )
+//   H_j = (X G)_j       = 1/2 ( sum_i y_ij g'_i - ( S'' - sum_{i miss} g''_i ) ),
+//   g' = d G, g'' = 2 F g', S'' = sum_i g''_i.
+// The dense parts sum_j y_ij Omega_j and sum_i y_ij g'_i run on the packed codes;
+// the missing entries (sparse) are corrected in the epilogues from CSR lists.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pcaone {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// layout
+// ---------------------------------------------------------------------------
+__global__ void k_bed_to_snpmajor(const uint8_t* __restrict__ raw, uint32_t nsamples, uint32_t nsnps,
+                                  const int32_t* __restrict__ perm, uint8_t* __restrict__ out,
+                                  uint32_t row_bytes) {
+  const uint32_t nb = (nsamples + 3) >> 2;
+  const uint32_t dw_per_row = row_bytes >> 2;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)nsnps * dw_per_row) return;
+  const uint32_t r = (uint32_t)(gid / dw_per_row), q = (uint32_t)(gid % dw_per_row);
+  const uint32_t src = perm ? (uint32_t)perm[r] : r;
+  const uint8_t* in = raw + (uint64_t)src * nb;
+  uint32_t word = 0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const uint32_t bi = q * 4 + t;
+    uint32_t b = 0x55u;  // padding: missing codes
+    if (bi < nb) {
+      b = in[bi];
+      const uint32_t first = bi * 4;
+      if (first + 4 > nsamples) {  // last byte: codes past N become missing
+        const uint32_t keep = nsamples - first;
+        const uint32_t mask = (1u << (2 * keep)) - 1u;
+        b = (b & mask) | (0x55u & ~mask);
+      }
+    }
+    word |= b << (8 * t);
+  }
+  reinterpret_cast<uint32_t*>(out)[gid] = word;
+}
+
+void launch_bed_to_snpmajor(const uint8_t* raw, uint32_t nsamples, uint32_t nsnps, const int32_t* perm,
+                            uint8_t* bed_snp, uint32_t row_bytes, hipStream_t s) {
+  const uint64_t total = (uint64_t)nsnps * (row_bytes / 4);
+  const uint32_t blocks = (uint32_t)ceil_div(total, 256);
+  hipLaunchKernelGGL(k_bed_to_snpmajor, dim3(blocks), dim3(256), 0, s, raw, nsamples, nsnps, perm, bed_snp,
+                     row_bytes);
+}
+
+// one wave per row: counts of the four codes among the first nvalid codes
+__global__ void k_count_codes(const uint8_t* __restrict__ bed, uint32_t nrows, uint32_t row_bytes,
+                              uint32_t nvalid, int32_t* __restrict__ counts) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t row = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= nrows) return;
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(bed + row * row_bytes);
+  const uint32_t ndw = row_bytes >> 2;
+  int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+  for (uint32_t q = lane; q < ndw; q += 64) {
+    const uint32_t x = p[q];
+    const uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+    c0 += __popc(~lo & ~hi & 0x55555555u);
+    c1 += __popc(lo & ~hi);
+    c2 += __popc(~lo & hi);
+    c3 += __popc(lo & hi);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    c0 += __shfl_down(c0, off);
+    c1 += __shfl_down(c1, off);
+    c2 += __shfl_down(c2, off);
+    c3 += __shfl_down(c3, off);
+  }
+  if (lane == 0) {
+    counts[4 * row + 0] = c0;
+    counts[4 * row + 1] = c1 - (int)(row_bytes * 4 - nvalid);  // padding codes are 01
+    counts[4 * row + 2] = c2;
+    counts[4 * row + 3] = c3;
+  }
+}
+
+void launch_count_codes(const uint8_t* bed_snp, uint32_t nrows, uint32_t row_bytes, uint32_t nvalid,
+                        int32_t* counts4, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)ceil_div((uint64_t)nrows * 64, 256);
+  hipLaunchKernelGGL(k_count_codes, dim3(blocks), dim3(256), 0, s, bed_snp, nrows, row_bytes, nvalid, counts4);
+}
+
+// Tile transpose: 64 SNP rows x 256 samples of bed_snp -> 256 sample rows x 16
+// bytes of one window of bed_smp.  Rows past the window stay missing (0x55,
+// pre-filled by the caller).
+__global__ __launch_bounds__(256) void k_transpose_windows(const uint8_t* __restrict__ bed_snp,
+                                                           uint32_t row_bytes, uint32_t nsnps,
+                                                           uint32_t blocksize, uint8_t* __restrict__ bed_smp,
+                                                           uint32_t npad, uint32_t win_bytes) {
+  __shared__ uint32_t tile[64][17];  // 64 rows x 64 bytes (+1 dword pad)
+  const uint32_t w = blockIdx.z, tk = blockIdx.x, ts = blockIdx.y;
+  const uint32_t ws = w * blocksize;
+  const uint32_t we = min(ws + blocksize, nsnps);
+  const uint32_t s0 = ws + tk * 64;
+  if (s0 >= we) return;
+  const uint32_t t = threadIdx.x;
+  {
+    const uint32_t row = t >> 2, seg = t & 3;
+    uint4 v = make_uint4(0x55555555u, 0x55555555u, 0x55555555u, 0x55555555u);
+    if (s0 + row < we)
+      v = *reinterpret_cast<const uint4*>(bed_snp + (uint64_t)(s0 + row) * row_bytes + ts * 64 + 16 * seg);
+    tile[row][4 * seg + 0] = v.x;
+    tile[row][4 * seg + 1] = v.y;
+    tile[row][4 * seg + 2] = v.z;
+    tile[row][4 * seg + 3] = v.w;
+  }
+  __syncthreads();
+  uint32_t o[4] = {0u, 0u, 0u, 0u};
+  const uint32_t dw = t >> 4, sh = 2 * (t & 15);  // sample t lives in dword t/16 at bit 2*(t%16)
+#pragma unroll
+  for (int sp = 0; sp < 64; ++sp) {
+    const uint32_t code = (tile[sp][dw] >> sh) & 3u;
+    o[sp >> 4] |= code << (2 * (sp & 15));
+  }
+  const uint64_t j = (uint64_t)ts * 256 + t;
+  uint8_t* dst = bed_smp + ((uint64_t)w * npad + j) * win_bytes + tk * 16;
+  *reinterpret_cast<uint4*>(dst) = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+void launch_transpose_windows(const uint8_t* bed_snp, uint32_t row_bytes, uint32_t nsnps, uint32_t blocksize,
+                              uint32_t nwin, uint8_t* bed_smp, uint32_t npad, uint32_t win_bytes,
+                              hipStream_t s) {
+  dim3 grid((uint32_t)ceil_div(blocksize, 64), npad / 256, nwin);
+  hipLaunchKernelGGL(k_transpose_windows, grid, dim3(256), 0, s, bed_snp, row_bytes, nsnps, blocksize, bed_smp,
+                     npad, win_bytes);
+}
+
+// ---------------------------------------------------------------------------
+// missing lists (code 01)
+// ---------------------------------------------------------------------------
+__device__ inline bool pr_row(const PackedRows& pr, uint64_t r, uint32_t& nvalid, uint32_t& base) {
+  const uint32_t g = (uint32_t)(r / pr.rows_per_group), jr = (uint32_t)(r % pr.rows_per_group);
+  if (jr >= pr.live_rows) return false;
+  nvalid = (g + 1 == pr.ngroups) ? pr.valid_last : pr.valid;
+  base = g * pr.base_step;
+  return true;
+}
+
+__device__ inline uint32_t missing_mask(uint32_t x, uint32_t q, uint32_t nvalid) {
+  const uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+  uint32_t m = lo & ~hi;
+  const uint32_t first = q * 16;
+  if (first >= nvalid) return 0u;
+  if (first + 16 > nvalid) m &= (1u << (2 * (nvalid - first))) - 1u;
+  return m;
+}
+
+__global__ void k_count_missing(PackedRows pr, int32_t* __restrict__ counts) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (r >= pr.nrows) return;
+  uint32_t nvalid, base;
+  int c = 0;
+  if (pr_row(pr, r, nvalid, base)) {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(pr.data + r * pr.row_bytes);
+    const uint32_t ndw = (nvalid + 15) / 16;
+    for (uint32_t q = lane; q < ndw; q += 64) c += __popc(missing_mask(p[q], q, nvalid));
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+  if (lane == 0) counts[r] = c;
+}
+
+__global__ void k_fill_missing(PackedRows pr, const int64_t* __restrict__ ptr, int32_t* __restrict__ idx) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (r >= pr.nrows) return;
+  uint32_t nvalid, base;
+  if (!pr_row(pr, r, nvalid, base)) return;
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(pr.data + r * pr.row_bytes);
+  const uint32_t ndw = (nvalid + 15) / 16;
+  int64_t pos = ptr[r];
+  for (uint32_t q0 = 0; q0 < ndw; q0 += 64) {
+    const uint32_t q = q0 + lane;
+    uint32_t m = (q < ndw) ? missing_mask(p[q], q, nvalid) : 0u;
+    const int cnt = __popc(m);
+    int incl = cnt;  // inclusive prefix over lanes
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off);
+      if ((int)lane >= off) incl += v;
+    }
+    int64_t o = pos + (incl - cnt);
+    while (m) {
+      const uint32_t b = __ffs(m) - 1;  // even bit position
+      idx[o++] = (int32_t)(base + q * 16 + (b >> 1));
+      m &= m - 1;
+    }
+    pos += __shfl(incl, 63);
+  }
+}
+
+void launch_count_missing(const PackedRows& pr, int32_t* counts, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)ceil_div((uint64_t)pr.nrows * 64, 256);
+  hipLaunchKernelGGL(k_count_missing, dim3(blocks), dim3(256), 0, s, pr, counts);
+}
+
+void launch_fill_missing(const PackedRows& pr, const int64_t* ptr, int32_t* idx, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)ceil_div((uint64_t)pr.nrows * 64, 256);
+  hipLaunchKernelGGL(k_fill_missing, dim3(blocks), dim3(256), 0, s, pr, ptr, idx);
+}
+
+// ---------------------------------------------------------------------------
+// split-BF16 MFMA contraction
+// ---------------------------------------------------------------------------
+// Spread four 2-bit codes (one byte) into the low 2 bits of four bytes.
+__device__ inline uint32_t spread4(uint32_t b) {
+  uint32_t t = (b | (b << 12)) & 0x000F000Fu;
+  return (t | (t << 6)) & 0x03030303u;
+}
+
+// Eight consecutive codes (16 bits) -> eight BF16 dosages {00:2, 10:1, 11:0, 01:0}.
+// v_perm picks each BF16 byte from two 4-entry byte tables indexed by the code.
+__device__ inline bf16x8 decode8(uint32_t h) {
+  constexpr uint32_t LUT_HI = 0x003F0040u;  // high bytes of bf16 {2.0, 0, 1.0, 0}
+  constexpr uint32_t LUT_LO = 0x00800000u;  // low bytes
+  const uint32_t e0 = spread4(h & 0xFFu), e1 = spread4((h >> 8) & 0xFFu);
+  const uint32_t f0 = e0 + 0x04040404u, f1 = e1 + 0x04040404u;
+  uint32_t o[4];
+  o[0] = __builtin_amdgcn_perm(LUT_LO, LUT_HI, __builtin_amdgcn_perm(f0, e0, 0x01050004u));
+  o[1] = __builtin_amdgcn_perm(LUT_LO, LUT_HI, __builtin_amdgcn_perm(f0, e0, 0x03070206u));
+  o[2] = __builtin_amdgcn_perm(LUT_LO, LUT_HI, __builtin_amdgcn_perm(f1, e1, 0x01050004u));
+  o[3] = __builtin_amdgcn_perm(LUT_LO, LUT_HI, __builtin_amdgcn_perm(f1, e1, 0x03070206u));
+  bf16x8 r;
+  __builtin_memcpy(&r, o, sizeof(r));
+  return r;
+}
+
+// Block = 4 waves, 256 rows (64 per wave = 4 fragments of 16).  K advances in
+// chunks of 256 codes (64 bytes per row); lane (r, g) holds bytes [16g, 16g+16)
+// of its row's chunk, i.e. codes [64g, 64g+64), and K-step s (0..7) feeds the
+// MFMA with codes 64g+8s..64g+8s+7.  The B operand is staged per chunk in LDS,
+// rows padded to 264 bf16 so the 16 column-lanes of a ds_read_b128 hit
+// distinct banks.
+constexpr int kBStride = 264;
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_packed_gemm_bf16(PackedGemm g) {
+  __shared__ __attribute__((aligned(16))) __bf16 bl[NT * 16 * kBStride];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t z = blockIdx.z, split = blockIdx.y;
+  const uint8_t* A = g.A + (uint64_t)z * g.a_zstride;
+  const __bf16* B = reinterpret_cast<const __bf16*>(g.B) + (uint64_t)z * g.b_zstride;
+  float* out = reinterpret_cast<float*>(g.out) + (uint64_t)z * g.o_zstride;
+  const uint32_t row0 = blockIdx.x * 256 + wave * 64;
+  const uint32_t c0 = split * g.chunks_per_split;
+  const uint32_t c1 = min(c0 + g.chunks_per_split, g.kchunks);
+
+  const uint8_t* arow[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    uint32_t r = row0 + 16 * f + (lane & 15);
+    r = min(r, g.nrows - 1);
+    arow[f] = A + (uint64_t)r * g.lda + 16 * (lane >> 4);
+  }
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[f][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int bn = lane & 15, bg = lane >> 4;
+  for (uint32_t c = c0; c < c1; ++c) {
+    uint4 ar[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) ar[f] = *reinterpret_cast<const uint4*>(arow[f] + (uint64_t)c * 64);
+    __syncthreads();
+    for (int it = threadIdx.x; it < NT * 16 * 32; it += 256) {
+      const int n = it >> 5, seg = it & 31;
+      const uint4 v = *reinterpret_cast<const uint4*>(B + (uint64_t)n * g.ldb + (uint64_t)c * 256 + seg * 8);
+      *reinterpret_cast<uint4*>(bl + n * kBStride + seg * 8) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      bf16x8 b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        b[t] = *reinterpret_cast<const bf16x8*>(bl + (t * 16 + bn) * kBStride + 64 * bg + 8 * s);
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const uint32_t w = (s < 2) ? ar[f].x : (s < 4) ? ar[f].y : (s < 6) ? ar[f].z : ar[f].w;
+        const bf16x8 a = decode8((s & 1) ? (w >> 16) : (w & 0xFFFFu));
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[f][t], 0, 0, 0);
+      }
+    }
+  }
+  float* o = out + (uint64_t)split * g.nrows * g.ldo;
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t r = row0 + 16 * f + 4 * (lane >> 4) + i;
+      if (r < g.nrows) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) o[(uint64_t)r * g.ldo + t * 16 + (lane & 15)] = acc[f][t][i];
+      }
+    }
+}
+
+void launch_packed_gemm_bf16(const PackedGemm& g, int nt16, hipStream_t s) {
+  dim3 grid((uint32_t)ceil_div(g.nrows, 256), g.nsplit, g.nz);
+  switch (nt16) {
+#define PCAONE_NT(n) \
+  case n: hipLaunchKernelGGL(k_packed_gemm_bf16<n>, grid, dim3(256), 0, s, g); break;
+    PCAONE_NT(1) PCAONE_NT(2) PCAONE_NT(3) PCAONE_NT(4) PCAONE_NT(5) PCAONE_NT(6) PCAONE_NT(7) PCAONE_NT(8)
+#undef PCAONE_NT
+    default: throw Error(-1, "split-BF16 contraction supports at most 128 operand columns (l <= 64)");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 / FP64 vector-FMA contraction (plain drop-in and FP64 arm)
+// ---------------------------------------------------------------------------
+// Block = 64 * ncg threads (wave w owns columns [8w, 8w+8)); each thread owns 8
+// rows (tr + 64 i) x 8 columns.  K advances 64 codes at a time; B is staged in
+// LDS transposed to [64][8*ncg].
+template <class T>
+__global__ __launch_bounds__(1024) void k_packed_gemm_valu(PackedGemm g, int ncols) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* bl = reinterpret_cast<T*>(smem_raw);
+  const int ncg = blockDim.x >> 6;
+  const int lc = ncg * 8;
+  const int tr = threadIdx.x & 63, tc = threadIdx.x >> 6;
+  const uint32_t z = blockIdx.z, split = blockIdx.y;
+  const uint8_t* A = g.A + (uint64_t)z * g.a_zstride;
+  const T* B = reinterpret_cast<const T*>(g.B) + (uint64_t)z * g.b_zstride;
+  T* out = reinterpret_cast<T*>(g.out) + (uint64_t)z * g.o_zstride;
+  const uint32_t rbase = blockIdx.x * 512 + tr;
+  const uint32_t c0 = split * g.chunks_per_split;
+  const uint32_t c1 = min(c0 + g.chunks_per_split, g.kchunks);
+  const uint8_t* arow[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) arow[i] = A + (uint64_t)min(rbase + 64 * i, g.nrows - 1) * g.lda;
+  T acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = T(0);
+
+  for (uint32_t c = c0; c < c1; ++c) {
+    for (int sub = 0; sub < 4; ++sub) {
+      const uint64_t kb = (uint64_t)c * 256 + sub * 64;
+      uint32_t a[8][4];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint4 v = *reinterpret_cast<const uint4*>(arow[i] + (uint64_t)c * 64 + sub * 16);
+        const uint32_t xs[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // 2-bit codes -> 2-bit dosages
+          const uint32_t lo = xs[q] & 0x55555555u, hi = (xs[q] >> 1) & 0x55555555u;
+          const uint32_t y1 = ~(lo | hi) & 0x55555555u;  // code 00 -> 2
+          const uint32_t y0 = hi & ~lo;                  // code 10 -> 1
+          a[i][q] = y0 | (y1 << 1);
+        }
+      }
+      __syncthreads();
+      for (int it = threadIdx.x; it < 64 * lc; it += blockDim.x) {
+        const int kk = it & 63, col = it >> 6;
+        bl[kk * lc + col] = (col < ncols) ? B[(uint64_t)col * g.ldb + kb + kk] : T(0);
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < 64; ++kk) {
+        T bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bv[j] = bl[kk * lc + tc * 8 + j];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const T y = (T)((a[i][kk >> 4] >> (2 * (kk & 15))) & 3u);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) acc[i][j] = fma(y, bv[j], acc[i][j]);
+        }
+      }
+    }
+  }
+  T* o = out + (uint64_t)split * g.nrows * g.ldo;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t r = rbase + 64 * i;
+    if (r < g.nrows) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int col = tc * 8 + j;
+        if (col < ncols) o[(uint64_t)r * g.ldo + col] = acc[i][j];
+      }
+    }
+  }
+}
+
+template <class T>
+static void launch_valu(const PackedGemm& g, int ncols, hipStream_t s) {
+  const int ncg = (ncols + 7) / 8;
+  PCAONE_REQUIRE(ncg >= 1 && ncg <= 16, -1, "vector contraction supports 1..128 columns");
+  dim3 grid((uint32_t)ceil_div(g.nrows, 512), g.nsplit, g.nz);
+  const size_t shm = (size_t)64 * ncg * 8 * sizeof(T);
+  hipLaunchKernelGGL(k_packed_gemm_valu<T>, grid, dim3(64 * ncg), shm, s, g, ncols);
+}
+
+void launch_packed_gemm_f32(const PackedGemm& g, int ncols, hipStream_t s) { launch_valu<float>(g, ncols, s); }
+void launch_packed_gemm_f64(const PackedGemm& g, int ncols, hipStream_t s) { launch_valu<double>(g, ncols, s); }
+
+// ---------------------------------------------------------------------------
+// epilogues
+// ---------------------------------------------------------------------------
+__device__ inline double load_part(const void* p, int is64, uint64_t i) {
+  return is64 ? reinterpret_cast<const double*>(p)[i] : (double)reinterpret_cast<const float*>(p)[i];
+}
+
+__global__ void k_finalize_G(FinalizeG f) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nrow = f.r1 - f.r0;
+  if (gid >= nrow * f.l) return;
+  const uint32_t rr = (uint32_t)(gid / f.l), c = (uint32_t)(gid % f.l);
+  const uint32_t r = f.r0 + rr;
+  double raw = 0.0;
+  for (uint32_t s = 0; s < f.nsplit; ++s) {
+    const uint64_t base = ((uint64_t)s * nrow + rr) * f.ldp;
+    raw += load_part(f.part, f.part_is_f64, base + c);
+    if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
+  }
+  double corr = 0.0;
+  for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p) corr += f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
+  const double Fi = f.F[r], di = f.d[r];
+  const double Gv = 0.5 * di * (raw - 2.0 * Fi * (f.colsum[c] - corr));
+  f.G[(uint64_t)r * f.l + c] = Gv;
+  const double gp = di * Gv;
+  f.g2[(uint64_t)r * f.l + c] = 2.0 * Fi * gp;
+  const uint32_t w = r / f.blocksize, k = r - w * f.blocksize;
+  const uint64_t o = ((uint64_t)w * f.bcols + c) * f.kw + k;
+  if (f.bprec == PREC_FP64) {
+    reinterpret_cast<double*>(f.bh)[o] = gp;
+  } else if (f.bprec == PREC_FP32) {
+    reinterpret_cast<float*>(f.bh)[o] = (float)gp;
+  } else {
+    const __bf16 hi = (__bf16)(float)gp;
+    const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
+    __bf16* bh = reinterpret_cast<__bf16*>(f.bh);
+    bh[o] = hi;
+    bh[((uint64_t)w * f.bcols + f.l + c) * f.kw + k] = lo;
+  }
+}
+
+void launch_finalize_G(const FinalizeG& f, hipStream_t s) {
+  const uint64_t total = (uint64_t)(f.r1 - f.r0) * f.l;
+  if (!total) return;
+  hipLaunchKernelGGL(k_finalize_G, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, f);
+}
+
+__global__ __launch_bounds__(256) void k_window_colsum(const double* __restrict__ g2, uint32_t l,
+                                                       uint32_t nsnps, uint32_t blocksize, uint32_t w0,
+                                                       double* __restrict__ S2) {
+  __shared__ double sh[256];
+  const uint32_t w = w0 + blockIdx.x;
+  const uint32_t rs = w * blocksize, re = min(rs + blocksize, nsnps);
+  const uint32_t t = threadIdx.x, R = 256 / l;
+  const uint32_t c = t % l, rr = t / l;
+  double acc = 0.0;
+  if (rr < R)
+    for (uint32_t r = rs + rr; r < re; r += R) acc += g2[(uint64_t)r * l + c];
+  sh[t] = (rr < R) ? acc : 0.0;
+  __syncthreads();
+  if (t < l) {
+    double tot = 0.0;
+    for (uint32_t q = 0; q < R; ++q) tot += sh[q * l + t];
+    S2[(uint64_t)w * l + t] = tot;
+  }
+}
+
+void launch_window_colsum(const double* g2, uint32_t l, uint32_t nsnps, uint32_t blocksize, uint32_t w0,
+                          uint32_t w1, double* S2, hipStream_t s) {
+  PCAONE_REQUIRE(l >= 1 && l <= 256, -1, "l out of range");
+  if (w1 <= w0) return;
+  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0), dim3(256), 0, s, g2, l, nsnps, blocksize, w0, S2);
+}
+
+__global__ void k_finalize_H(FinalizeH f) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)f.nsamples * f.l) return;
+  const uint32_t j = (uint32_t)(gid / f.l), c = (uint32_t)(gid % f.l);
+  for (uint32_t w = f.w0; w < f.w1; ++w) {
+    const uint32_t wz = w - f.wpart0;
+    double raw = 0.0;
+    for (uint32_t s = 0; s < f.nsplit; ++s) {
+      const uint64_t base = (((uint64_t)wz * f.nsplit + s) * f.part_rows + j) * f.ldp;
+      raw += load_part(f.part, f.part_is_f64, base + c);
+      if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
+    }
+    const uint64_t li = (uint64_t)w * f.miss_rows + j;
+    double corr = 0.0;
+    for (int64_t p = f.miss_ptr[li]; p < f.miss_ptr[li + 1]; ++p) corr += f.g2[(uint64_t)f.miss_idx[p] * f.l + c];
+    const double h = 0.5 * (raw - (f.S2[(uint64_t)w * f.l + c] - corr));
+    double* dst = ((f.route_b >> (w - f.w0)) & 1ull) ? f.Hb : f.Ha;
+    dst[(uint64_t)j * f.l + c] += h;
+  }
+}
+
+void launch_finalize_H(const FinalizeH& f, hipStream_t s) {
+  const uint64_t total = (uint64_t)f.nsamples * f.l;
+  hipLaunchKernelGGL(k_finalize_H, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, f);
+}
+
+__global__ void k_omega_operand(const double* __restrict__ omega, uint32_t n, uint32_t l, int bprec, void* bg,
+                                uint32_t bcols, uint64_t kpad) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)bcols * kpad) return;
+  const uint32_t col = (uint32_t)(gid / kpad);
+  const uint64_t k = gid % kpad;
+  if (bprec == PREC_SPLIT_BF16) {
+    __bf16 v = (__bf16)0.0f;
+    if (k < n && col < 2 * l) {
+      const double x = omega[k * l + (col < l ? col : col - l)];
+      const __bf16 hi = (__bf16)(float)x;
+      v = (col < l) ? hi : (__bf16)(float)(x - (double)(float)hi);
+    }
+    reinterpret_cast<__bf16*>(bg)[gid] = v;
+  } else {
+    const double x = (k < n && col < l) ? omega[k * l + col] : 0.0;
+    if (bprec == PREC_FP64)
+      reinterpret_cast<double*>(bg)[gid] = x;
+    else
+      reinterpret_cast<float*>(bg)[gid] = (float)x;
+  }
+}
+
+void launch_omega_operand(const double* omega, uint32_t n, uint32_t l, int bprec, void* bg, uint32_t bcols,
+                          uint64_t kpad, hipStream_t s) {
+  const uint64_t total = (uint64_t)bcols * kpad;
+  hipLaunchKernelGGL(k_omega_operand, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, omega, n, l, bprec,
+                     bg, bcols, kpad);
+}
+
+__global__ void k_fill_f64(double* p, uint64_t n, double v) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid < n) p[gid] = v;
+}
+
+void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_f64, dim3((uint32_t)ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+}  // namespace pcaone

@@ -1,0 +1,151 @@
+// Seeded synthetic genotype generator: Balding-Nichols populations written
+// straight into the PLINK .bed payload (SNP-major, 4 samples per byte,
+// LSB first; code 00 = two A1 copies, 10 = one, 11 = none, 01 = missing).
+//
+// Each SNP draws from its own counter-seeded stream, so the output depends only
+// on (seed, sizes, parameters) and not on the thread count.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/pcaone_gpu.h"
+
+namespace {
+
+inline uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+struct Xoshiro256 {
+  uint64_t s[4];
+  explicit Xoshiro256(uint64_t seed) {
+    for (auto& v : s) v = splitmix64(seed);
+  }
+  static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  inline uint64_t next() {
+    const uint64_t r = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  inline double uniform() { return (next() >> 11) * 0x1.0p-53; }  // [0,1)
+  inline double normal() {  // Box-Muller, one value per call
+    double u1 = uniform(), u2 = uniform();
+    if (u1 < 1e-300) u1 = 1e-300;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+  // Marsaglia-Tsang gamma(shape, 1)
+  double gamma(double a) {
+    if (a < 1.0) {
+      double u = uniform();
+      if (u < 1e-300) u = 1e-300;
+      return gamma(a + 1.0) * std::pow(u, 1.0 / a);
+    }
+    const double d = a - 1.0 / 3.0, c = 1.0 / std::sqrt(9.0 * d);
+    for (;;) {
+      double x = normal(), v = 1.0 + c * x;
+      if (v <= 0) continue;
+      v = v * v * v;
+      double u = uniform();
+      if (u < 1.0 - 0.0331 * x * x * x * x) return d * v;
+      if (std::log(u) < 0.5 * x * x + d * (1.0 - v + std::log(v))) return d * v;
+    }
+  }
+  double beta(double a, double b) {
+    double x = gamma(a), y = gamma(b);
+    return x / (x + y);
+  }
+};
+
+uint64_t snp_seed(uint64_t seed, uint64_t snp, uint64_t attempt) {
+  uint64_t x = seed ^ (snp * 0xD1B54A32D192ED03ull) ^ (attempt * 0x8CB92BA72F3D8DD7ull);
+  return splitmix64(x);
+}
+
+// One SNP row.  Returns true when polymorphic among non-missing genotypes.
+bool gen_row(uint64_t seed, uint32_t snp, uint32_t attempt, uint32_t n, uint32_t npops, double fst, double pmin,
+             double pmax, double miss, uint8_t* row, uint32_t nbytes, double* p_out) {
+  Xoshiro256 rng(snp_seed(seed, snp, attempt));
+  const double p = pmin + (pmax - pmin) * rng.uniform();
+  if (p_out) *p_out = p;
+  std::vector<uint32_t> t2(npops), t1(npops);
+  for (uint32_t k = 0; k < npops; ++k) {
+    double pk = p;
+    if (fst > 0) {
+      const double a = p * (1.0 - fst) / fst, b = (1.0 - p) * (1.0 - fst) / fst;
+      pk = rng.beta(a, b);
+    }
+    const double f2 = pk * pk, f1 = f2 + 2.0 * pk * (1.0 - pk);
+    t2[k] = (uint32_t)std::min(4294967295.0, std::floor(f2 * 4294967296.0));
+    t1[k] = (uint32_t)std::min(4294967295.0, std::floor(f1 * 4294967296.0));
+  }
+  const uint32_t tm = (uint32_t)std::min(4294967295.0, std::floor(miss * 4294967296.0));
+  std::memset(row, 0, nbytes);
+  uint64_t c[4] = {0, 0, 0, 0};
+  for (uint32_t j = 0; j < n; ++j) {
+    const uint32_t k = (uint32_t)(((uint64_t)j * npops) / n);
+    const uint64_t r = rng.next();
+    const uint32_t ug = (uint32_t)(r >> 32), um = (uint32_t)r;
+    uint32_t code;
+    if (miss > 0 && um < tm)
+      code = 1;
+    else if (ug < t2[k])
+      code = 0;  // two A1 copies
+    else if (ug < t1[k])
+      code = 2;  // het
+    else
+      code = 3;  // no A1 copy
+    c[code]++;
+    row[j >> 2] |= (uint8_t)(code << (2 * (j & 3)));
+  }
+  const uint64_t nonmiss = c[0] + c[2] + c[3];
+  if (nonmiss == 0) return false;
+  // F = (n00 + n10/2) / n in {0, 1} means monomorphic
+  if (c[0] + c[2] == 0) return false;
+  if (c[2] + c[3] == 0) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" int pcaone_sim_bed(uint64_t seed, uint32_t nsamples, uint32_t nsnps, uint32_t npops, double fst,
+                              double pmin, double pmax, double miss_rate, uint8_t* payload, double* p_anc,
+                              int nthreads) {
+  if (!payload || nsamples == 0 || nsnps == 0 || npops == 0 || npops > nsamples) return -1;
+  if (!(fst >= 0 && fst < 1) || !(pmin > 0 && pmax < 1 && pmin <= pmax) || !(miss_rate >= 0 && miss_rate < 1))
+    return -1;
+  const uint32_t nbytes = (nsamples + 3) >> 2;
+  unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  unsigned nt = nthreads > 0 ? (unsigned)nthreads : std::min(hw, 16u);
+  nt = std::max(1u, std::min<unsigned>(nt, nsnps));
+  std::vector<std::thread> pool;
+  std::vector<int> fail(nt, 0);
+  for (unsigned t = 0; t < nt; ++t) {
+    pool.emplace_back([&, t]() {
+      const uint64_t lo = (uint64_t)nsnps * t / nt, hi = (uint64_t)nsnps * (t + 1) / nt;
+      for (uint64_t i = lo; i < hi; ++i) {
+        uint8_t* row = payload + i * nbytes;
+        bool ok = false;
+        for (uint32_t a = 0; a < 1000 && !ok; ++a)
+          ok = gen_row(seed, (uint32_t)i, a, nsamples, npops, fst, pmin, pmax, miss_rate, row, nbytes,
+                       p_anc ? p_anc + i : nullptr);
+        if (!ok) fail[t] = 1;
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (int f : fail)
+    if (f) return -3;
+  return 0;
+}

@@ -1,0 +1,52 @@
+"""Shared helpers for the tests: golden-case argument parsing and oracle runs."""
+from __future__ import annotations
+
+import shlex
+
+import numpy as np
+
+import pcaone_oracle as O
+
+
+def parse_args(args) -> dict:
+    """The subset of PCAone's CLI the golden cases use, with Param's defaults
+    (src/Cmd.hpp:28-50) and oversamples = max(os, k) (src/Cmd.cpp:190)."""
+    a = shlex.split(str(args))
+    d = {"svd": 2, "k": 10, "os": 10, "p": 20, "tol": 1e-4, "seed": 101, "rand": 1, "w": 64, "S": False,
+         "scale": 1}
+    i = 0
+    flags = {"--svd": ("svd", int), "-k": ("k", int), "--oversamples": ("os", int), "-p": ("p", int),
+             "--tol-rsvd": ("tol", float), "--seed": ("seed", int), "--rand": ("rand", int), "-w": ("w", int),
+             "--scale": ("scale", int)}
+    while i < len(a):
+        if a[i] == "-S":
+            d["S"] = True
+            i += 1
+        else:
+            key, typ = flags[a[i]]
+            d[key] = typ(a[i + 1])
+            i += 2
+    d["os"] = max(d["os"], d["k"])
+    return d
+
+
+def oracle_matrix(payload, n, m, standardize=True):
+    codes = O.decode_codes(payload, n, m)
+    F, _ = O.allele_freq(codes)
+    X = O.centered_matrix(codes, F)
+    return O.standardize(X, F) if standardize else X
+
+
+def oracle_run(z: dict):
+    d = parse_args(z["args"])
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    X = oracle_matrix(z["payload"], n, m, bool(d["scale"]))
+    orc = O.RsvdOracle(X, d["k"], d["os"], fancy=d["svd"] == 2, bands=d["w"], seed=d["seed"], rand=d["rand"],
+                       perm=not d["S"])
+    ep = orc.compute_usv(d["p"], d["tol"])
+    return orc, ep, d
+
+
+def eig_relerr(a, b) -> float:
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.abs(b)))

@@ -1,0 +1,80 @@
+"""Writes the golden fixtures in tests/golden/ from the reference PCAone CLI.
+
+Run in the build container only (it needs oracle/_ref/PCAone, built from the
+reference sources by `make -f oracle/ref.mk`):
+
+    python tests/golden/make_golden.py
+
+Inputs are seeded Balding-Nichols PLINK sets from pcaone_sim_bed; each case
+stores the .bed payload and the reference's printed outputs (.eigvals,
+.sigvals, .eigvecs, .loadings; 6 significant digits, src/Data.cpp:177-194)
+in one .npz.  The reference is run with -n 1 so its output does not depend on
+OpenMP scheduling.
+"""
+from __future__ import annotations
+
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+import pcaone_amd as pa  # noqa: E402
+
+REF = ROOT / "oracle" / "_ref" / "PCAone"
+OUT = Path(__file__).resolve().parent
+
+DATASETS = {
+    # name: (seed, N, M, npops, fst, miss)
+    "d1": (11, 200, 2000, 3, 0.1, 0.01),
+    "d2": (12, 203, 1601, 4, 0.05, 0.03),   # N % 4 == 3, uneven last window
+}
+
+CASES = {
+    # name: (dataset, CLI args)
+    "svd1_p7": ("d1", "--svd 1 -k 5 --oversamples 5 -p 7 --tol-rsvd 0 --seed 101"),
+    "svd1_p1": ("d1", "--svd 1 -k 5 --oversamples 5 -p 1 --tol-rsvd 0 --seed 101"),
+    "svd1_p1_unif": ("d1", "--svd 1 -k 5 --oversamples 5 -p 1 --tol-rsvd 0 --seed 7 --rand 0"),
+    "svd1_p3_nostd": ("d1", "--svd 1 -k 4 --oversamples 6 -p 3 --tol-rsvd 0 --scale 0"),
+    "svd2_w8": ("d1", "--svd 2 -k 5 --oversamples 5 -w 8"),
+    "svd2_w8_S": ("d1", "--svd 2 -k 5 --oversamples 5 -w 8 -S"),
+    "svd2_w4_p2": ("d2", "--svd 2 -k 3 --oversamples 7 -w 4 -p 2 --tol-rsvd 0 --seed 3"),
+    "svd1_d2": ("d2", "--svd 1 -k 6 --oversamples 6 -p 5 --tol-rsvd 0"),
+}
+
+
+def run_case(tmp: Path, prefix: str, args: str, out: str):
+    cmd = [str(REF), "-b", prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v", "0"]
+    subprocess.run(cmd, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    res = {}
+    res["eigvals"] = np.loadtxt(tmp / (out + ".eigvals"), ndmin=1)
+    with open(tmp / (out + ".sigvals")) as f:
+        f.readline()
+        res["sigvals"] = np.loadtxt(f, ndmin=1)
+    res["eigvecs"] = np.loadtxt(tmp / (out + ".eigvecs"), ndmin=2)
+    res["loadings"] = np.loadtxt(tmp / (out + ".loadings"), ndmin=2)
+    return res
+
+
+def main():
+    if not REF.exists():
+        raise SystemExit(f"{REF} missing: make -f oracle/ref.mk")
+    with tempfile.TemporaryDirectory() as td:
+        tmp = Path(td)
+        payloads = {}
+        for name, (seed, n, m, k, fst, miss) in DATASETS.items():
+            pay = pa.sim_bed(seed, n, m, k, fst, 0.05, 0.5, miss)
+            pa.write_plink(tmp / name, pay, n, m)
+            payloads[name] = (pay, n, m)
+        for case, (ds, args) in CASES.items():
+            pay, n, m = payloads[ds]
+            res = run_case(tmp, ds, args, case)
+            np.savez_compressed(OUT / f"{case}.npz", payload=pay, nsamples=n, nsnps=m, args=args, **res)
+            print(case, res["eigvals"])
+
+
+if __name__ == "__main__":
+    main()
