@@ -96,6 +96,18 @@ int pcaone_gpu_rsvd_gandh(pcaone_gpu* h, int pi, double* G, double* H);
 int pcaone_gpu_rsvd_usv(pcaone_gpu* h, int maxp, double tol, double* U, double* S, double* V, int* epochs,
                         double* diffs);
 
+/* RsvdOpData::setFlags(update, standardize) (src/Halko.hpp:32-35): with
+ * update set, the next computeGandH at pi == 0 re-imputes every missing
+ * genotype from the operator's current U, S, V (fit_with_pi, EMU branch,
+ * src/Data.cpp:272-287); standardize switches the 1/sd scaling. */
+int pcaone_gpu_rsvd_set_flags(pcaone_gpu* h, int update, int standardize);
+/* run_pca_with_halko's EM branch for --emu (src/Halko.cpp:294-336): a plain
+ * solve, then re-imputed solves until 1 - mev(V, Vpre) < tolem (at most
+ * maxiter), then the final standardised solve whose U, S, V are returned.
+ * diffs (optional, maxiter entries) receives 1 - mev per EM iteration. */
+int pcaone_gpu_rsvd_em(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
+                       double* V, int* iters, double* diffs);
+
 /* Host-only view of the winSVD window schedule of one computeGandH call
  * (src/Halko.cpp:159-223): `bandsize` is the operator state carried between
  * calls (reset at pi == 0).  For each window b: acc[b] in {1, 2} (H1 or H2),

@@ -412,6 +412,42 @@ class RsvdOracle:
         return self.S ** 2 / self.M * ploidy
 
 
+def emu_impute(X0: np.ndarray, miss: np.ndarray, F: np.ndarray, U, S, V) -> np.ndarray:
+    """EMU update of the missing entries (src/FilePlink.cpp:253-261,
+    src/Data.cpp:272-287): clamp(sum_k U_jk S_k V_ik, -F_i, 1 - F_i)."""
+    pred = (U * S) @ V.T
+    lo = np.broadcast_to(-F[None, :], pred.shape)
+    hi = np.broadcast_to(1.0 - F[None, :], pred.shape)
+    X = X0.copy()
+    X[miss] = np.minimum(np.maximum(pred[miss], lo[miss]), hi[miss])
+    return X
+
+
+def emu_halko(codes: np.ndarray, k: int, os: int, maxp: int, tol: float, maxiter: int = 100,
+              tolem: float = 1e-5, seed: int = 101, rand: int = 1, ploidy: int = 2):
+    """run_pca_with_halko's EM branch for --emu with the plain RSVD
+    (src/Halko.cpp:294-336, data read through FileBed::read_block_update,
+    src/FilePlink.cpp:226-294).  Returns (oracle op after the final
+    standardised solve, EM iterations, per-iteration diffs)."""
+    F, _ = allele_freq(codes)
+    X0 = centered_matrix(codes, F)
+    miss = (codes == BED_MISSING_CODE).T
+    op = RsvdOracle(X0.copy(), k, os, fancy=False, seed=seed, rand=rand)
+    op.compute_usv(maxp, tol)
+    diffs = []
+    it = 0
+    for it in range(1, maxiter + 1):
+        vpre = op.V.copy()
+        op.X = emu_impute(X0, miss, F, op.U, op.S, op.V)
+        op.compute_usv(maxp, tol)
+        diffs.append(1.0 - mev(op.V, vpre))
+        if diffs[-1] < tolem:
+            break
+    op.X = standardize(emu_impute(X0, miss, F, op.U, op.S, op.V), F, ploidy)
+    op.compute_usv(maxp, tol)
+    return op, it, diffs
+
+
 def principal_angles_max(A: np.ndarray, B: np.ndarray) -> float:
     """Largest principal angle (radians) between span(A) and span(B)."""
     qa, _ = np.linalg.qr(A)

@@ -55,6 +55,9 @@ SIGNATURES = {
     "pcaone_gpu_rsvd_perm": (C.c_int, [C.c_void_p, _I32]),
     "pcaone_gpu_rsvd_gandh": (C.c_int, [C.c_void_p, C.c_int, _D, _D]),
     "pcaone_gpu_rsvd_usv": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, _D, _D, _INT, _D]),
+    "pcaone_gpu_rsvd_set_flags": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "pcaone_gpu_rsvd_em": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, _D, _D, _D, _INT,
+                                     _D]),
     "pcaone_fancy_plan": (C.c_int, [C.c_int, C.c_int, _U64, _INT, _INT, _INT, _INT]),
     "pcaone_gpu_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "pcaone_gpu_reset_stats": (C.c_int, [C.c_void_p]),
@@ -189,6 +192,21 @@ class Gpu:
         self._check(self._L.pcaone_gpu_rsvd_usv(self.h, maxp, tol, _p(U, C.c_double), _p(S, C.c_double),
                                                  _p(V, C.c_double), C.byref(ep), _p(diffs, C.c_double)))
         return U, S, V, ep.value, diffs
+
+    def set_flags(self, update: bool, standardize: bool):
+        self._check(self._L.pcaone_gpu_rsvd_set_flags(self.h, int(update), int(standardize)))
+
+    def em(self, maxp: int, tol: float, maxiter: int = 100, tolem: float = 1e-5):
+        """EMU EM-PCA (--emu); returns U, S, V, iterations, per-iteration diffs."""
+        U = np.empty((self.N, self.k), order="F")
+        S = np.empty(self.k)
+        V = np.empty((self.M, self.k), order="F")
+        it = C.c_int(0)
+        diffs = np.zeros(max(maxiter, 1))
+        self._check(self._L.pcaone_gpu_rsvd_em(self.h, maxp, tol, maxiter, tolem, _p(U, C.c_double),
+                                                _p(S, C.c_double), _p(V, C.c_double), C.byref(it),
+                                                _p(diffs, C.c_double)))
+        return U, S, V, it.value, diffs[:it.value]
 
     # ---- timing ---------------------------------------------------------------
     def profiling(self, on: bool):

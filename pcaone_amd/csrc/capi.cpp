@@ -118,6 +118,15 @@ int pcaone_gpu_rsvd_usv(pcaone_gpu* h, int maxp, double tol, double* U, double* 
   return guarded(h, [&](pcaone::Engine& e) { e.rsvd_usv(maxp, tol, U, S, V, epochs, diffs); });
 }
 
+int pcaone_gpu_rsvd_set_flags(pcaone_gpu* h, int update, int standardize) {
+  return guarded(h, [&](pcaone::Engine& e) { e.set_flags(update != 0, standardize != 0); });
+}
+
+int pcaone_gpu_rsvd_em(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
+                       double* V, int* iters, double* diffs) {
+  return guarded(h, [&](pcaone::Engine& e) { e.rsvd_em(maxp, tol, maxiter, tolem, U, S, V, iters, diffs); });
+}
+
 int pcaone_fancy_plan(int bands, int pi, uint64_t* bandsize, int* acc, int* update, int* zero, int* reset_both) {
   if (bands < 4 || (bands & (bands - 1)) || pi < 0 || !bandsize || !acc || !update || !zero) return PCAONE_EINVAL;
   const pcaone::FancyPlan pl = pcaone::fancy_plan(bands, pi, *bandsize);

@@ -255,10 +255,15 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   Upre_.reserve((uint64_t)N_ * p.k);
   V_.reserve((uint64_t)M_ * p.k);
   scal_.reserve(4);
+  miss_val_.reserve(std::max<int64_t>(nnz_snp_, 1));
+  Sdev_.reserve(p.k);
+  Vpre_.reserve((uint64_t)M_ * p.k);
+  update_ = have_usv_ = have_vals_ = false;
   status_.reserve(2);
   PCAONE_HIP(hipMemsetAsync(status_.p, 0, 2 * sizeof(int), stream_));
   bandsize_ = 1;
   have_op_ = true;
+  draw_omega();
   init_omega();
   sync();
 }
@@ -278,20 +283,13 @@ void Engine::build_layout() {
   timed("bed_transpose", [&] {
     launch_transpose_windows(bed_snp_.p, row_bytes_, M_, bs_, nwin_, bed_smp_.p, npad_, win_bytes_, stream_);
   });
-  // F and the scale in laid-out order (src/Data.cpp:290-299, VAR_TOL = 1e-9)
-  std::vector<double> F(M_), d(M_);
-  for (uint32_t r = 0; r < M_; ++r) {
-    const double f = F_orig_[perm_[r]];
-    F[r] = f;
-    const double sd = std::sqrt((double)p_.ploidy * f * (1 - f));
-    d[r] = (p_.standardize && sd > 1e-9) ? 1.0 / sd : 1.0;
-  }
+  F_lay_.resize(M_);
+  for (uint32_t r = 0; r < M_; ++r) F_lay_[r] = F_orig_[perm_[r]];
   F_.reserve(M_);
-  d_.reserve(M_);
-  PCAONE_HIP(hipMemcpyAsync(F_.p, F.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
-  PCAONE_HIP(hipMemcpyAsync(d_.p, d.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(F_.p, F_lay_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  upload_scale();
   // missing lists
-  auto build = [&](const PackedRows& pr, DevBuf<int64_t>& ptr, DevBuf<int32_t>& idx, const char* nm) {
+  auto build = [&](const PackedRows& pr, DevBuf<int64_t>& ptr, DevBuf<int32_t>& idx, const char* nm) -> int64_t {
     DevBuf<int32_t> cnt;
     cnt.reserve(pr.nrows);
     timed(nm, [&] { launch_count_missing(pr, cnt.p, stream_); });
@@ -305,16 +303,53 @@ void Engine::build_layout() {
     idx.reserve(std::max<int64_t>(hp.back(), 1));
     timed(nm, [&] { launch_fill_missing(pr, ptr.p, idx.p, stream_); });
     sync();  // hp must outlive the copy
+    return hp.back();
   };
   PackedRows ps{bed_snp_.p, row_bytes_, M_, M_, M_, N_, N_, 1, 0};
-  build(ps, miss_snp_ptr_, miss_snp_idx_, "missing_lists");
+  nnz_snp_ = build(ps, miss_snp_ptr_, miss_snp_idx_, "missing_lists");
   PackedRows pw{bed_smp_.p, win_bytes_, nwin_ * npad_, npad_, N_, bs_, M_ - (nwin_ - 1) * bs_, nwin_, bs_};
-  build(pw, miss_win_ptr_, miss_win_idx_, "missing_lists");
+  const int64_t nnz_win = build(pw, miss_win_ptr_, miss_win_idx_, "missing_lists");
+  PCAONE_REQUIRE(nnz_win == nnz_snp_, -2, "missing lists disagree between the two layouts");
+  miss_win_pos_.reserve(std::max<int64_t>(nnz_win, 1));
+  timed("missing_lists", [&] {
+    launch_map_missing(miss_win_ptr_.p, miss_win_idx_.p, nwin_, npad_, N_, miss_snp_ptr_.p, miss_snp_idx_.p,
+                       miss_win_pos_.p, stream_);
+  });
+}
+
+// 1/sd of each laid-out SNP, or 1 when not standardising or sd <= VAR_TOL
+// (src/Data.cpp:290-299, VAR_TOL = 1e-9 at src/Data.hpp:7)
+void Engine::upload_scale() {
+  std::vector<double> d(M_);
+  for (uint32_t r = 0; r < M_; ++r) {
+    const double f = F_lay_[r];
+    const double sd = std::sqrt((double)p_.ploidy * f * (1 - f));
+    d[r] = (p_.standardize && sd > 1e-9) ? 1.0 / sd : 1.0;
+  }
+  d_.reserve(M_);
+  PCAONE_HIP(hipMemcpyAsync(d_.p, d.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  sync();
+}
+
+// RsvdOpData::setFlags (src/Halko.hpp:32-35): `update` re-imputes the missing
+// genotypes from the operator's U, S, V at the next pi == 0 (EMU), and
+// `standardize` switches the 1/sd scaling.
+void Engine::set_flags(bool update, bool standardize) {
+  PCAONE_REQUIRE(have_op_, -3, "setFlags: operator not initialised");
+  PCAONE_REQUIRE(!update || have_usv_, -3, "setFlags: update needs a previous computeUSV");
+  update_ = update;
+  if (!update) have_vals_ = false;
+  if ((int)standardize != p_.standardize) {
+    p_.standardize = standardize;
+    upload_scale();
+  }
 }
 
 // RsvdOpData::initOmg, src/Halko.cpp:14-22: default_random_engine seeded with
-// --seed, an N x l NullaryExpr filled in column-major order.
-void Engine::init_omega() {
+// --seed, an N x l NullaryExpr filled in column-major order.  The draw only
+// depends on (seed, rand, N, l), so it is made once per operator and every
+// pi == 0 restores it from a pristine device copy.
+void Engine::draw_omega() {
   std::vector<double> cm((size_t)N_ * l_);
   if (!omega_given_.empty()) {
     cm = omega_given_;
@@ -336,12 +371,20 @@ void Engine::init_omega() {
       rm[(size_t)r * l_ + c] = v;
       cs[c] += v;
     }
-  PCAONE_HIP(hipMemcpyAsync(omega_.p, rm.data(), rm.size() * 8, hipMemcpyHostToDevice, stream_));
-  PCAONE_HIP(hipMemcpyAsync(omg2_.p, rm.data(), rm.size() * 8, hipMemcpyHostToDevice, stream_));
-  PCAONE_HIP(hipMemcpyAsync(colsum_.p, cs.data(), cs.size() * 8, hipMemcpyHostToDevice, stream_));
+  omega0_.reserve((uint64_t)N_ * l_);
+  colsum0_.reserve(l_);
+  PCAONE_HIP(hipMemcpyAsync(omega0_.p, rm.data(), rm.size() * 8, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(colsum0_.p, cs.data(), cs.size() * 8, hipMemcpyHostToDevice, stream_));
+  sync();  // host vectors go out of scope
+}
+
+void Engine::init_omega() {
+  const size_t nb = (size_t)N_ * l_ * 8;
+  PCAONE_HIP(hipMemcpyAsync(omega_.p, omega0_.p, nb, hipMemcpyDeviceToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(omg2_.p, omega0_.p, nb, hipMemcpyDeviceToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(colsum_.p, colsum0_.p, (size_t)l_ * 8, hipMemcpyDeviceToDevice, stream_));
   timed("omega_operand",
         [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, npad_, stream_); });
-  sync();  // host vectors go out of scope
 }
 
 // Omega <- Householder Q of H (H = H1 + H2 for winSVD), then flipOmg
@@ -425,6 +468,7 @@ void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route
   fg.bcols = bcols_;
   fg.kw = kw_;
   fg.blocksize = bs_;
+  fg.mval = have_vals_ ? miss_val_.p : nullptr;
   timed("finalize_G", [&] { launch_finalize_G(fg, stream_); });
   timed("window_colsum", [&] { launch_window_colsum(g2_.p, l_, M_, bs_, w0, w1, S2_.p, stream_); });
   // ---- H += X_w G_w per window -------------------------------------------
@@ -468,6 +512,10 @@ void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route
   fh.miss_ptr = miss_win_ptr_.p;
   fh.miss_idx = miss_win_idx_.p;
   fh.g2 = g2_.p;
+  fh.mval = have_vals_ ? miss_val_.p : nullptr;
+  fh.mpos = miss_win_pos_.p;
+  fh.G = G_.p;
+  fh.d = d_.p;
   for (uint32_t a = w0; a < w1; a += 64) {  // route travels as a 64-bit mask
     fh.w0 = a;
     fh.w1 = std::min<uint32_t>(a + 64, w1);
@@ -482,21 +530,30 @@ void Engine::rsvd_gandh(int pi, double* Gout, double* Hout) {
   PCAONE_REQUIRE(have_op_, -3, "computeGandH: operator not initialised");
   PCAONE_REQUIRE(pi >= 0, -1, "computeGandH: pi must be >= 0");
   PCAONE_HIP(hipSetDevice(device_));
+  if (pi == 0 && update_) {
+    // fit_with_pi, EMU branch (src/Data.cpp:272-287; src/FilePlink.cpp:253-261)
+    PCAONE_HIP(hipMemcpyAsync(Sdev_.p, last_S_.data(), last_S_.size() * 8, hipMemcpyHostToDevice, stream_));
+    timed("emu_update", [&] {
+      launch_emu_update(miss_snp_ptr_.p, miss_snp_idx_.p, M_, Ucur_.p, Sdev_.p, V_.p, (uint32_t)p_.k, F_.p,
+                        miss_val_.p, stream_);
+    });
+    have_vals_ = true;
+  }
   if (p_.variant == 1) {
     // NormalRsvdOpData in-core, src/Halko.cpp:105-126
     if (pi == 0)
       init_omega();
     else
       omega_refresh(false, nullptr);
-    PCAONE_HIP(hipMemsetAsync(H_.p, 0, H_.bytes(), stream_));
+    timed("fill", [&] { launch_fill_f64(H_.p, (uint64_t)N_ * l_, 0.0, stream_); });
     run_segment(0, nwin_, std::vector<int>(nwin_, 1));
   } else {
     // FancyRsvdOpData in-core, src/Halko.cpp:153-226
     if (pi == 0) init_omega();
     const FancyPlan pl = fancy_plan(p_.bands, pi, bandsize_);
     if (pl.reset_both) {
-      PCAONE_HIP(hipMemsetAsync(H1_.p, 0, H1_.bytes(), stream_));
-      PCAONE_HIP(hipMemsetAsync(H2_.p, 0, H2_.bytes(), stream_));
+      timed("fill", [&] { launch_fill_f64(H1_.p, (uint64_t)N_ * l_, 0.0, stream_); });
+      timed("fill", [&] { launch_fill_f64(H2_.p, (uint64_t)N_ * l_, 0.0, stream_); });
     }
     std::vector<int> route(nwin_, 1);
     for (uint32_t b = 0; b < nwin_; ++b) route[b] = pl.steps[b].acc;
@@ -601,13 +658,21 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
   if (!done) std::swap(Ucur_.p, Upre_.p);  // the last Ucur sits in Upre after the swap
   // V = Q U_B[:, :k] (rows in laid-out order), S = sqrt(eig(B B^T))[:k]
   timed("usv_rows", [&] { launch_rows_times_small(G_.p, M_, l_, W_.p, l_, k, nullptr, V_.p, stream_); });
-  std::vector<double> ev(l_), u((size_t)N_ * k), v((size_t)M_ * k);
+  std::vector<double> ev(l_), u, v;
   PCAONE_HIP(hipMemcpyAsync(ev.data(), evals_.p, l_ * 8, hipMemcpyDeviceToHost, stream_));
-  PCAONE_HIP(hipMemcpyAsync(u.data(), Ucur_.p, u.size() * 8, hipMemcpyDeviceToHost, stream_));
-  PCAONE_HIP(hipMemcpyAsync(v.data(), V_.p, v.size() * 8, hipMemcpyDeviceToHost, stream_));
+  if (U) {
+    u.resize((size_t)N_ * k);
+    PCAONE_HIP(hipMemcpyAsync(u.data(), Ucur_.p, u.size() * 8, hipMemcpyDeviceToHost, stream_));
+  }
+  if (V) {
+    v.resize((size_t)M_ * k);
+    PCAONE_HIP(hipMemcpyAsync(v.data(), V_.p, v.size() * 8, hipMemcpyDeviceToHost, stream_));
+  }
   sync();
-  if (S)
-    for (uint32_t c = 0; c < k; ++c) S[c] = std::sqrt(std::max(ev[c], 0.0));
+  last_S_.assign(k, 0.0);
+  for (uint32_t c = 0; c < k; ++c) last_S_[c] = std::sqrt(std::max(ev[c], 0.0));
+  have_usv_ = true;
+  if (S) std::memcpy(S, last_S_.data(), k * sizeof(double));
   if (U)
     for (uint32_t r = 0; r < N_; ++r)
       for (uint32_t c = 0; c < k; ++c) U[(size_t)c * N_ + r] = u[(size_t)r * k + c];
@@ -615,6 +680,37 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
     for (uint32_t r = 0; r < M_; ++r)
       for (uint32_t c = 0; c < k; ++c) V[(size_t)c * M_ + perm_[r]] = v[(size_t)r * k + c];
   if (epochs) *epochs = ep;
+}
+
+// The EM branch of run_pca_with_halko for --emu (src/Halko.cpp:294-336):
+// one plain solve, then solves with the missing genotypes re-imputed from the
+// previous U, S, V until 1 - mev(V, Vpre) < tolem, then a final standardised
+// solve.  `diffs` (optional, maxiter entries) receives 1 - mev per iteration.
+void Engine::rsvd_em(int maxp, double tol, int maxiter, double tolem, double* U, double* S, double* V, int* iters,
+                     double* diffs) {
+  PCAONE_REQUIRE(have_op_, -3, "EM: operator not initialised");
+  PCAONE_REQUIRE(maxiter >= 0, -1, "EM: maxiter must be >= 0");
+  const uint32_t k = (uint32_t)p_.k;
+  set_flags(false, false);
+  rsvd_usv(maxp, tol, nullptr, nullptr, nullptr, nullptr, nullptr);
+  int it = 0;
+  for (int i = 0; i < maxiter; ++i) {
+    PCAONE_HIP(hipMemcpyAsync(Vpre_.p, V_.p, (size_t)M_ * k * 8, hipMemcpyDeviceToDevice, stream_));
+    set_flags(true, false);
+    rsvd_usv(maxp, tol, nullptr, nullptr, nullptr, nullptr, nullptr);
+    const int nb = gram_blocks(M_);
+    timed("mev", [&] { launch_xty_partial(V_.p, Vpre_.p, M_, k, part_.p, nb, stream_); });
+    timed("mev", [&] { launch_mev_finish(part_.p, nb, k, scal_.p, stream_); });
+    double m = 0.0;
+    PCAONE_HIP(hipMemcpyAsync(&m, scal_.p, 8, hipMemcpyDeviceToHost, stream_));
+    sync();
+    it = i + 1;
+    if (diffs) diffs[i] = 1.0 - m;
+    if (1.0 - m < tolem) break;
+  }
+  set_flags(true, true);  // EMU always ends with a standardised solve (src/Halko.cpp:332-336)
+  rsvd_usv(maxp, tol, U, S, V, nullptr, nullptr);
+  if (iters) *iters = it;
 }
 
 }  // namespace pcaone

@@ -59,6 +59,9 @@ class Engine {
   void rsvd_gandh(int pi, double* G, double* H);
   void rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int* epochs, double* diffs);
   void rsvd_perm(int32_t* perm) const;
+  void set_flags(bool update, bool standardize);
+  void rsvd_em(int maxp, double tol, int maxiter, double tolem, double* U, double* S, double* V, int* iters,
+               double* diffs);
   void dims(int64_t* rows, int64_t* cols, int64_t* size) const;
 
   KernelTimer timer;
@@ -70,11 +73,13 @@ class Engine {
   void timed(const char* name, F&& launch);
 
   void build_layout();
+  void draw_omega();
   void init_omega();
   void omega_refresh(bool from_two, double* zero_acc);
   void run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route);
   void usv_step();
   void check_status();
+  void upload_scale();
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -93,22 +98,29 @@ class Engine {
   uint32_t row_bytes_ = 0, npad_ = 0, win_bytes_ = 0;
   uint64_t kw_ = 0;
   std::vector<int32_t> perm_;      // laid-out row r holds SNP perm_[r]
+  std::vector<double> F_lay_;      // allele frequency in laid-out order
   DevBuf<int32_t> d_perm_;
   DevBuf<uint8_t> bed_snp_, bed_smp_;
   DevBuf<double> F_, d_;
   DevBuf<int64_t> miss_snp_ptr_, miss_win_ptr_;
   DevBuf<int32_t> miss_snp_idx_, miss_win_idx_;
+  DevBuf<int64_t> miss_win_pos_;   // window entry -> SNP-CSR position
+  int64_t nnz_snp_ = 0;
 
   // ---- operator state ----------------------------------------------------
   int nt16_ = 0;        // 16-column tiles of the split operand
   uint32_t bcols_ = 0;  // columns of the contraction operands
   size_t opsz_ = 8;     // bytes per operand element
   std::vector<double> omega_given_;
-  DevBuf<double> omega_, omg2_, colsum_;
+  DevBuf<double> omega_, omg2_, colsum_, omega0_, colsum0_;
   DevBuf<unsigned char> bg_, bh_;   // operands of the G and H contractions
   DevBuf<double> G_, g2_, S2_, H1_, H2_, H_;
   DevBuf<unsigned char> gpart_, hpart_;
   uint64_t bandsize_ = 1;
+  // RsvdOpData::update / U, S, V of the last computeUSV (src/Halko.hpp:11-14)
+  bool update_ = false, have_usv_ = false, have_vals_ = false;
+  std::vector<double> last_S_;
+  DevBuf<double> miss_val_, Sdev_, Vpre_;
   // linear algebra scratch
   DevBuf<double> part_, fpart_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, sign_, Q_;
   DevBuf<double> Bt_, evals_, W_, sinv_, Ucur_, Upre_, V_, scal_;

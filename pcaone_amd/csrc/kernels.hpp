@@ -96,8 +96,18 @@ struct FinalizeG {
   uint32_t bcols;       // columns in the operand (l, or nt16*16 for split)
   uint64_t kw;          // codes per window row (win_bytes*4)
   uint32_t blocksize;   // window length (last may be shorter)
+  const double* mval;   // EMU values of the missing entries (CSR order) or null
 };
 void launch_finalize_G(const FinalizeG& f, hipStream_t s);
+
+// EMU imputation of every missing entry (SNP-CSR order):
+// val = clamp(sum_c U[j][c] S[c] V[i][c], -F_i, 1 - F_i)   (src/FilePlink.cpp:253-261)
+void launch_emu_update(const int64_t* miss_ptr, const int32_t* miss_idx, uint32_t nsnps, const double* U,
+                       const double* S, const double* V, uint32_t k, const double* F, double* val, hipStream_t s);
+// pos[p] = position in the SNP-CSR list of the p-th entry of the window lists
+void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin, uint32_t npad,
+                        uint32_t nsamples, const int64_t* sptr, const int32_t* sidx, int64_t* pos,
+                        hipStream_t s);
 
 // S2[w][c] = sum over window w's rows of g2[r][c] for windows [w0, w1)
 void launch_window_colsum(const double* g2, uint32_t l, uint32_t nsnps, uint32_t blocksize, uint32_t w0,
@@ -121,6 +131,11 @@ struct FinalizeH {
   const int64_t* miss_ptr;  // [nwin*miss_rows+1]
   const int32_t* miss_idx;  // SNP rows
   const double* g2;     // [M][l]
+  // EMU: H_j += sum over missing (i, j) of val * d_i G_i
+  const double* mval;   // SNP-CSR values or null
+  const int64_t* mpos;  // window entry -> SNP-CSR position
+  const double* G;      // [M][l]
+  const double* d;      // [M]
 };
 void launch_finalize_H(const FinalizeH& f, hipStream_t s);
 

@@ -435,10 +435,18 @@ __global__ void k_finalize_G(FinalizeG f) {
     raw += load_part(f.part, f.part_is_f64, base + c);
     if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
   }
-  double corr = 0.0;
-  for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p) corr += f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
+  double corr = 0.0, corr_e = 0.0;
+  if (f.mval) {
+    for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p) {
+      const double om = f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
+      corr += om;
+      corr_e += f.mval[p] * om;
+    }
+  } else {
+    for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p) corr += f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
+  }
   const double Fi = f.F[r], di = f.d[r];
-  const double Gv = 0.5 * di * (raw - 2.0 * Fi * (f.colsum[c] - corr));
+  const double Gv = di * (0.5 * (raw - 2.0 * Fi * (f.colsum[c] - corr)) + corr_e);
   f.G[(uint64_t)r * f.l + c] = Gv;
   const double gp = di * Gv;
   f.g2[(uint64_t)r * f.l + c] = 2.0 * Fi * gp;
@@ -503,9 +511,17 @@ __global__ void k_finalize_H(FinalizeH f) {
       if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
     }
     const uint64_t li = (uint64_t)w * f.miss_rows + j;
-    double corr = 0.0;
-    for (int64_t p = f.miss_ptr[li]; p < f.miss_ptr[li + 1]; ++p) corr += f.g2[(uint64_t)f.miss_idx[p] * f.l + c];
-    const double h = 0.5 * (raw - (f.S2[(uint64_t)w * f.l + c] - corr));
+    double corr = 0.0, corr_e = 0.0;
+    if (f.mval) {
+      for (int64_t p = f.miss_ptr[li]; p < f.miss_ptr[li + 1]; ++p) {
+        const uint64_t i = (uint64_t)f.miss_idx[p];
+        corr += f.g2[i * f.l + c];
+        corr_e += f.mval[f.mpos[p]] * f.d[i] * f.G[i * f.l + c];
+      }
+    } else {
+      for (int64_t p = f.miss_ptr[li]; p < f.miss_ptr[li + 1]; ++p) corr += f.g2[(uint64_t)f.miss_idx[p] * f.l + c];
+    }
+    const double h = 0.5 * (raw - (f.S2[(uint64_t)w * f.l + c] - corr)) + corr_e;
     double* dst = ((f.route_b >> (w - f.w0)) & 1ull) ? f.Hb : f.Ha;
     dst[(uint64_t)j * f.l + c] += h;
   }
@@ -514,6 +530,61 @@ __global__ void k_finalize_H(FinalizeH f) {
 void launch_finalize_H(const FinalizeH& f, hipStream_t s) {
   const uint64_t total = (uint64_t)f.nsamples * f.l;
   hipLaunchKernelGGL(k_finalize_H, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, f);
+}
+
+// one thread per SNP row: EMU value of each of its missing samples
+__global__ void k_emu_update(const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx, uint32_t nsnps,
+                             const double* __restrict__ U, const double* __restrict__ S,
+                             const double* __restrict__ V, uint32_t k, const double* __restrict__ F,
+                             double* __restrict__ val) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsnps) return;
+  double vs[64];
+  for (uint32_t c = 0; c < k; ++c) vs[c] = S[c] * V[i * k + c];
+  const double lo = -F[i], hi = 1.0 - F[i];
+  for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+    const uint64_t j = (uint64_t)idx[p];
+    double e = 0.0;
+    for (uint32_t c = 0; c < k; ++c) e += U[j * k + c] * vs[c];
+    val[p] = fmin(fmax(e, lo), hi);
+  }
+}
+
+void launch_emu_update(const int64_t* miss_ptr, const int32_t* miss_idx, uint32_t nsnps, const double* U,
+                       const double* S, const double* V, uint32_t k, const double* F, double* val, hipStream_t s) {
+  PCAONE_REQUIRE(k >= 1 && k <= 64, -1, "EMU update supports k <= 64");
+  hipLaunchKernelGGL(k_emu_update, dim3((uint32_t)ceil_div(nsnps, 256)), dim3(256), 0, s, miss_ptr, miss_idx,
+                     nsnps, U, S, V, k, F, val);
+}
+
+// one thread per (window, sample) row of the window lists
+__global__ void k_map_missing(const int64_t* __restrict__ wptr, const int32_t* __restrict__ widx, uint32_t nwin,
+                              uint32_t npad, uint32_t nsamples, const int64_t* __restrict__ sptr,
+                              const int32_t* __restrict__ sidx, int64_t* __restrict__ pos) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (uint64_t)nwin * npad) return;
+  const int32_t j = (int32_t)(r % npad);
+  for (int64_t p = wptr[r]; p < wptr[r + 1]; ++p) {
+    const uint64_t i = (uint64_t)widx[p];
+    int64_t lo = sptr[i], hi = sptr[i + 1] - 1;  // sample lists are ascending
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (sidx[mid] < j)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    pos[p] = lo;
+  }
+  (void)nsamples;
+}
+
+void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin, uint32_t npad,
+                        uint32_t nsamples, const int64_t* sptr, const int32_t* sidx, int64_t* pos,
+                        hipStream_t s) {
+  const uint64_t rows = (uint64_t)nwin * npad;
+  hipLaunchKernelGGL(k_map_missing, dim3((uint32_t)ceil_div(rows, 256)), dim3(256), 0, s, wptr, widx, nwin, npad,
+                     nsamples, sptr, sidx, pos);
 }
 
 __global__ void k_omega_operand(const double* __restrict__ omega, uint32_t n, uint32_t l, int bprec, void* bg,

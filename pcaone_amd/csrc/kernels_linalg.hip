@@ -99,8 +99,8 @@ void launch_gram_partial(const double* A, const double* A2, double* Aout, uint64
   PCAONE_REQUIRE(nrows > row0, -1, "Gram over an empty row range");
   const int E = entries_per_thread((int)(l * l));
   switch (E) {
-#define PCAONE_E(n) \
-  case n: hipLaunchKernelGGL(k_gram_partial<n>, dim3(nblocks), dim3(256), 0, s, A, A2, Aout, nrows, row0, l, part); break;
+#define PCAONE_E(EPT) \
+  case EPT: hipLaunchKernelGGL(k_gram_partial<EPT>, dim3(nblocks), dim3(256), 0, s, A, A2, Aout, nrows, row0, l, part); break;
     PCAONE_E(1) PCAONE_E(2) PCAONE_E(4) PCAONE_E(8) PCAONE_E(16)
 #undef PCAONE_E
     default: throw Error(-1, "gram: unsupported l");
@@ -320,9 +320,9 @@ void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, 
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
   const int E = entries_per_thread((int)(l * l));
   switch (E) {
-#define PCAONE_E(n)                                                                                        \
-  case n:                                                                                                  \
-    hipLaunchKernelGGL(k_apply_rinv<n>, dim3(nblocks), dim3(256), 0, s, A, Rinv, sign, Q, nrows, l, gram_part); \
+#define PCAONE_E(EPT)                                                                                        \
+  case EPT:                                                                                                \
+    hipLaunchKernelGGL(k_apply_rinv<EPT>, dim3(nblocks), dim3(256), 0, s, A, Rinv, sign, Q, nrows, l, gram_part); \
     break;
     PCAONE_E(1) PCAONE_E(2) PCAONE_E(4) PCAONE_E(8) PCAONE_E(16)
 #undef PCAONE_E
@@ -592,8 +592,8 @@ void launch_xty_partial(const double* X, const double* Y, uint64_t n, uint32_t k
   PCAONE_REQUIRE(k >= 1 && k <= kMaxL, -1, "k must be in [1, 64]");
   const int E = entries_per_thread((int)(k * k));
   switch (E) {
-#define PCAONE_E(n) \
-  case n: hipLaunchKernelGGL(k_xty_partial<n>, dim3(nblocks), dim3(256), 0, s, X, Y, n, k, part); break;
+#define PCAONE_E(EPT) \
+  case EPT: hipLaunchKernelGGL(k_xty_partial<EPT>, dim3(nblocks), dim3(256), 0, s, X, Y, n, k, part); break;
     PCAONE_E(1) PCAONE_E(2) PCAONE_E(4) PCAONE_E(8) PCAONE_E(16)
 #undef PCAONE_E
     default: throw Error(-1, "xty: unsupported k");

@@ -13,14 +13,14 @@ def parse_args(args) -> dict:
     (src/Cmd.hpp:28-50) and oversamples = max(os, k) (src/Cmd.cpp:190)."""
     a = shlex.split(str(args))
     d = {"svd": 2, "k": 10, "os": 10, "p": 20, "tol": 1e-4, "seed": 101, "rand": 1, "w": 64, "S": False,
-         "scale": 1}
+         "scale": 1, "m": 0.0, "emu": False}
     i = 0
     flags = {"--svd": ("svd", int), "-k": ("k", int), "--oversamples": ("os", int), "-p": ("p", int),
              "--tol-rsvd": ("tol", float), "--seed": ("seed", int), "--rand": ("rand", int), "-w": ("w", int),
-             "--scale": ("scale", int)}
+             "--scale": ("scale", int), "-m": ("m", float)}
     while i < len(a):
-        if a[i] == "-S":
-            d["S"] = True
+        if a[i] in ("-S", "--emu"):
+            d["S" if a[i] == "-S" else "emu"] = True
             i += 1
         else:
             key, typ = flags[a[i]]
@@ -40,6 +40,10 @@ def oracle_matrix(payload, n, m, standardize=True):
 def oracle_run(z: dict):
     d = parse_args(z["args"])
     n, m = int(z["nsamples"]), int(z["nsnps"])
+    if d["emu"]:
+        codes = O.decode_codes(z["payload"], n, m)
+        orc, _, _ = O.emu_halko(codes, d["k"], d["os"], d["p"], d["tol"], seed=d["seed"], rand=d["rand"])
+        return orc, None, d
     X = oracle_matrix(z["payload"], n, m, bool(d["scale"]))
     orc = O.RsvdOracle(X, d["k"], d["os"], fancy=d["svd"] == 2, bands=d["w"], seed=d["seed"], rand=d["rand"],
                        perm=not d["S"])

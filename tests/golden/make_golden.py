@@ -31,6 +31,7 @@ DATASETS = {
     # name: (seed, N, M, npops, fst, miss)
     "d1": (11, 200, 2000, 3, 0.1, 0.01),
     "d2": (12, 203, 1601, 4, 0.05, 0.03),   # N % 4 == 3, uneven last window
+    "d3": (13, 200, 2000, 3, 0.1, 0.05),    # 5 % missing for EMU
 }
 
 CASES = {
@@ -43,6 +44,9 @@ CASES = {
     "svd2_w8_S": ("d1", "--svd 2 -k 5 --oversamples 5 -w 8 -S"),
     "svd2_w4_p2": ("d2", "--svd 2 -k 3 --oversamples 7 -w 4 -p 2 --tol-rsvd 0 --seed 3"),
     "svd1_d2": ("d2", "--svd 1 -k 6 --oversamples 6 -p 5 --tol-rsvd 0"),
+    # EMU through the out-of-core reader (-m): the consistent EM path of the
+    # reference (the in-core one indexes a permutation that --svd 1 never sets)
+    "emu_svd1": ("d3", "--svd 1 -m 0.0005 --emu -k 3 --oversamples 7 -p 5 --tol-rsvd 0"),
 }
 
 
