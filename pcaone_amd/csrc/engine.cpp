@@ -212,15 +212,33 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
 
   // operands and partials
   const int prec = p.precision;
-  nt16_ = (int)ceil_div(2 * l_, 16);
-  bcols_ = prec == PREC_SPLIT_BF16 ? (uint32_t)nt16_ * 16 : l_;
+  const bool dense = src_ == SRC_DENSE;
   opsz_ = prec == PREC_SPLIT_BF16 ? 2 : prec == PREC_FP32 ? 4 : 8;
-  const size_t partsz = prec == PREC_FP64 ? 8 : 4;
-  const uint64_t ldp = prec == PREC_SPLIT_BF16 ? (uint64_t)nt16_ * 16 : l_;
+  part_f64_ = prec == PREC_FP64;
+  const size_t partsz = part_f64_ ? 8 : 4;
+  if (!dense) {
+    // packed: split operand = [hi | lo | 0-pad] in nt16 tiles, partials keep both halves
+    nt16_ = (int)ceil_div(2 * l_, 16);
+    bcols_ = prec == PREC_SPLIT_BF16 ? (uint32_t)nt16_ * 16 : l_;
+    lo_off_ = l_;
+    ldp_ = prec == PREC_SPLIT_BF16 ? (uint64_t)nt16_ * 16 : l_;
+    split_cols_ = prec == PREC_SPLIT_BF16;
+    hf_ = 0.5;
+  } else {
+    // dense: FP32 arm in 16-column tiles; split arm = hi tiles | lo tiles, the
+    // three products land in one accumulator per tile
+    ntiles_ = (int)ceil_div(l_, 16);
+    bcols_ = prec == PREC_FP64 ? l_ : prec == PREC_FP32 ? (uint32_t)ntiles_ * 16 : 2u * ntiles_ * 16;
+    lo_off_ = (uint32_t)ntiles_ * 16;
+    ldp_ = prec == PREC_FP64 ? l_ : (uint64_t)ntiles_ * 16;
+    split_cols_ = false;
+    hf_ = 1.0;
+  }
+  const uint64_t ldp = ldp_;
   omega_.reserve((uint64_t)N_ * l_);
   omg2_.reserve((uint64_t)N_ * l_);
   colsum_.reserve(l_);
-  bg_.reserve((uint64_t)bcols_ * npad_ * opsz_);
+  bg_.reserve((uint64_t)bcols_ * kg_ * opsz_);
   bh_.reserve((uint64_t)nwin_ * bcols_ * kw_ * opsz_);
   PCAONE_HIP(hipMemsetAsync(bh_.p, 0, bh_.bytes(), stream_));
   G_.reserve((uint64_t)M_ * l_);
@@ -231,13 +249,14 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
     PCAONE_HIP(hipMemsetAsync(h->p, 0, h->bytes(), stream_));
   }
   uint64_t gmax = 0, hmax = 0;
-  const uint32_t rows_per_block = prec == PREC_SPLIT_BF16 ? 256 : 512;
+  const uint32_t rows_per_block = rows_per_block_();
+  const uint32_t kcg = (uint32_t)(kg_ / (dense ? 64 : 256)), kch = (uint32_t)(kw_ / (dense ? 64 : 256));
   for (uint32_t nw = 1; nw <= nwin_; ++nw) {
     const uint64_t rows = std::min<uint64_t>((uint64_t)nw * bs_, M_);
     uint32_t cps;
-    const uint32_t sg = split_for(ceil_div(rows, rows_per_block), row_bytes_ / 64, cps);
+    const uint32_t sg = split_for(ceil_div(rows, rows_per_block), kcg, cps);
     gmax = std::max<uint64_t>(gmax, sg * rows);
-    const uint32_t sh = split_for(ceil_div(N_, rows_per_block) * nw, win_bytes_ / 64, cps);
+    const uint32_t sh = split_for(ceil_div(N_, rows_per_block) * nw, kch, cps);
     hmax = std::max<uint64_t>(hmax, (uint64_t)sh * N_ * nw);
   }
   gpart_.reserve(gmax * ldp * partsz);

@@ -53,7 +53,9 @@ class Engine {
   ~Engine();
 
   void load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps);
+  void load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfeatures);
   void allele_freq(double* F, int64_t* counts) const;
+  void feature_means(double* mu) const;
 
   void rsvd_init(const RsvdParams& p, const double* omega, const int32_t* perm);
   void rsvd_gandh(int pi, double* G, double* H);
@@ -80,14 +82,22 @@ class Engine {
   void usv_step();
   void check_status();
   void upload_scale();
+  uint32_t rows_per_block_() const {
+    return (src_ == SRC_DENSE || p_.precision == PREC_SPLIT_BF16) ? 256u : 512u;
+  }
 
   int device_;
   hipStream_t stream_ = nullptr;
 
   // ---- data --------------------------------------------------------------
+  enum Source { SRC_NONE, SRC_BED, SRC_DENSE };
+  Source src_ = SRC_NONE;
   uint32_t N_ = 0, M_ = 0;
   bool have_bed_ = false;
   DevBuf<uint8_t> raw_;            // payload as loaded (M x ceil(N/4))
+  DevBuf<float> rawf_;             // dense features as loaded (M x N)
+  DevBuf<float> dns_, dnt_;        // dense rows (laid-out, padded) and windowed transpose
+  std::vector<double> mu_orig_;    // dense: per-feature mean (FileBin centring)
   std::vector<double> F_orig_;     // allele frequency, original order
   std::vector<int64_t> counts_orig_;
 
@@ -111,6 +121,13 @@ class Engine {
   int nt16_ = 0;        // 16-column tiles of the split operand
   uint32_t bcols_ = 0;  // columns of the contraction operands
   size_t opsz_ = 8;     // bytes per operand element
+  uint32_t lo_off_ = 0; // split operands: row of the lo part of column c is lo_off_ + c
+  uint64_t ldp_ = 0;    // columns of the partial products
+  bool part_f64_ = false, split_cols_ = false;
+  int ntiles_ = 0;      // dense MFMA: 16-column output tiles
+  uint64_t kg_ = 0;     // K extent of the G contraction (padded samples)
+  uint32_t miss_rows_ = 0;
+  double hf_ = 0.5;
   std::vector<double> omega_given_;
   DevBuf<double> omega_, omg2_, colsum_, omega0_, colsum0_;
   DevBuf<unsigned char> bg_, bh_;   // operands of the G and H contractions

@@ -93,10 +93,12 @@ struct FinalizeG {
   // H operand, per window: [nwin][bcols][kw]
   int bprec;            // 0 fp64, 1 fp32, 2 bf16 split
   void* bh;
-  uint32_t bcols;       // columns in the operand (l, or nt16*16 for split)
+  uint32_t bcols;       // rows of the operand (l, or the padded tile count * 16)
+  uint32_t lo_off;      // split operand: row of the lo part of column c is lo_off + c
   uint64_t kw;          // codes per window row (win_bytes*4)
   uint32_t blocksize;   // window length (last may be shorter)
   const double* mval;   // EMU values of the missing entries (CSR order) or null
+  double hf;            // 0.5 for A1 dosages (BED2GENO = y/2), 1 for dense values
 };
 void launch_finalize_G(const FinalizeG& f, hipStream_t s);
 
@@ -136,12 +138,39 @@ struct FinalizeH {
   const int64_t* mpos;  // window entry -> SNP-CSR position
   const double* G;      // [M][l]
   const double* d;      // [M]
+  double hf;            // 0.5 for A1 dosages, 1 for dense values
 };
 void launch_finalize_H(const FinalizeH& f, hipStream_t s);
 
-// Omega (FP64 [N][l]) -> G-contraction operand [bcols][kpad] in the arm's type.
+// Omega (FP64 [N][l]) -> G-contraction operand [bcols][kpad] in the arm's type
+// (split: hi at row c, lo at row lo_off + c).
 void launch_omega_operand(const double* omega, uint32_t n, uint32_t l, int bprec, void* bg, uint32_t bcols,
-                          uint64_t kpad, hipStream_t s);
+                          uint32_t lo_off, uint64_t kpad, hipStream_t s);
+
+// ---- dense FP32 feature-major matrices (src/FileBinary.cpp:21-46) ----------
+struct DenseGemm {
+  const float* A;      // [z][nrows][lda], K contiguous, zero padded to kchunks*64
+  uint64_t lda;
+  uint64_t a_zstride;
+  uint32_t nrows;
+  uint32_t kchunks;    // 64-float chunks
+  uint32_t chunks_per_split;
+  uint32_t nsplit;
+  const void* B;       // [z][cols][ldb]: fp64 (arm 0), fp32 (1), bf16 hi tiles | lo tiles (2)
+  uint64_t ldb;
+  uint64_t b_zstride;
+  void* out;           // [z][nsplit][nrows][ldo], fp64 for arm 0 else fp32
+  uint64_t ldo;
+  uint64_t o_zstride;
+  uint32_t nz;
+};
+void launch_dense_rows(const float* raw, uint32_t nsamples, uint32_t nfeat, const int32_t* perm, float* out,
+                       uint32_t ld, hipStream_t s);
+void launch_dense_mean(const float* raw, uint32_t nsamples, uint32_t nfeat, double* mu, hipStream_t s);
+void launch_dense_transpose(const float* rows, uint32_t ldr, uint32_t nfeat, uint32_t bs, uint32_t nwin,
+                            uint32_t nsamples, float* dt, uint32_t ldt, hipStream_t s);
+// ntiles: 16-column tiles of the output (FP32 arm) or of each half (split arm)
+void launch_dense_gemm(const DenseGemm& g, int arm, int ncols, int ntiles, hipStream_t s);
 
 void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t s);
 

@@ -446,7 +446,7 @@ __global__ void k_finalize_G(FinalizeG f) {
     for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p) corr += f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
   }
   const double Fi = f.F[r], di = f.d[r];
-  const double Gv = di * (0.5 * (raw - 2.0 * Fi * (f.colsum[c] - corr)) + corr_e);
+  const double Gv = di * (f.hf * (raw - 2.0 * Fi * (f.colsum[c] - corr)) + corr_e);
   f.G[(uint64_t)r * f.l + c] = Gv;
   const double gp = di * Gv;
   f.g2[(uint64_t)r * f.l + c] = 2.0 * Fi * gp;
@@ -461,7 +461,7 @@ __global__ void k_finalize_G(FinalizeG f) {
     const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
     __bf16* bh = reinterpret_cast<__bf16*>(f.bh);
     bh[o] = hi;
-    bh[((uint64_t)w * f.bcols + f.l + c) * f.kw + k] = lo;
+    bh[((uint64_t)w * f.bcols + f.lo_off + c) * f.kw + k] = lo;
   }
 }
 
@@ -521,7 +521,7 @@ __global__ void k_finalize_H(FinalizeH f) {
     } else {
       for (int64_t p = f.miss_ptr[li]; p < f.miss_ptr[li + 1]; ++p) corr += f.g2[(uint64_t)f.miss_idx[p] * f.l + c];
     }
-    const double h = 0.5 * (raw - (f.S2[(uint64_t)w * f.l + c] - corr)) + corr_e;
+    const double h = f.hf * (raw - (f.S2[(uint64_t)w * f.l + c] - corr)) + corr_e;
     double* dst = ((f.route_b >> (w - f.w0)) & 1ull) ? f.Hb : f.Ha;
     dst[(uint64_t)j * f.l + c] += h;
   }
@@ -588,17 +588,18 @@ void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin,
 }
 
 __global__ void k_omega_operand(const double* __restrict__ omega, uint32_t n, uint32_t l, int bprec, void* bg,
-                                uint32_t bcols, uint64_t kpad) {
+                                uint32_t bcols, uint32_t lo_off, uint64_t kpad) {
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (uint64_t)bcols * kpad) return;
   const uint32_t col = (uint32_t)(gid / kpad);
   const uint64_t k = gid % kpad;
   if (bprec == PREC_SPLIT_BF16) {
     __bf16 v = (__bf16)0.0f;
-    if (k < n && col < 2 * l) {
-      const double x = omega[k * l + (col < l ? col : col - l)];
+    const bool is_hi = col < l, is_lo = col >= lo_off && col < lo_off + l;
+    if (k < n && (is_hi || is_lo)) {
+      const double x = omega[k * l + (is_hi ? col : col - lo_off)];
       const __bf16 hi = (__bf16)(float)x;
-      v = (col < l) ? hi : (__bf16)(float)(x - (double)(float)hi);
+      v = is_hi ? hi : (__bf16)(float)(x - (double)(float)hi);
     }
     reinterpret_cast<__bf16*>(bg)[gid] = v;
   } else {
@@ -611,10 +612,10 @@ __global__ void k_omega_operand(const double* __restrict__ omega, uint32_t n, ui
 }
 
 void launch_omega_operand(const double* omega, uint32_t n, uint32_t l, int bprec, void* bg, uint32_t bcols,
-                          uint64_t kpad, hipStream_t s) {
+                          uint32_t lo_off, uint64_t kpad, hipStream_t s) {
   const uint64_t total = (uint64_t)bcols * kpad;
   hipLaunchKernelGGL(k_omega_operand, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, omega, n, l, bprec,
-                     bg, bcols, kpad);
+                     bg, bcols, lo_off, kpad);
 }
 
 __global__ void k_fill_f64(double* p, uint64_t n, double v) {
