@@ -53,6 +53,14 @@ int pcaone_gpu_device_count(void);
  * (nsamples+3)>>2 bytes.  The bytes stay packed in HBM; allele frequencies
  * are computed on the device from exact code counts. */
 int pcaone_gpu_load_bed(pcaone_gpu* h, const uint8_t* payload, uint32_t nsamples, uint32_t nsnps);
+/* Replaces FileBin::read_all (src/FileBinary.cpp:21-30): `featmajor` holds
+ * nfeatures rows of nsamples float32 (the .bin / .residuals body after the
+ * u32 M, u32 N header, src/FileBinary.hpp:15-16).  The values stay FP32 in
+ * HBM; each feature is centred by its FP64 mean inside the contractions.  The
+ * RSVD operator on this input never standardises (src/Halko.cpp:290-291). */
+int pcaone_gpu_load_dense_f32(pcaone_gpu* h, const float* featmajor, uint32_t nsamples, uint32_t nfeatures);
+/* Per-feature means (nfeatures, original order) of the loaded dense matrix. */
+int pcaone_gpu_feature_means(pcaone_gpu* h, double* mu);
 /* F (nsnps) as estimated in src/FilePlink.cpp:36-62 (original SNP order) and,
  * optionally, counts (nsnps x 4, row-major: codes 00, 01, 10, 11). */
 int pcaone_gpu_allele_freq(pcaone_gpu* h, double* F, int64_t* counts);
@@ -108,6 +116,25 @@ int pcaone_gpu_rsvd_set_flags(pcaone_gpu* h, int update, int standardize);
 int pcaone_gpu_rsvd_em(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
                        double* V, int* iters, double* diffs);
 
+/* ---- IRAM: the Spectra op and its driver ----------------------------------
+ * Prepares the single-column Gram mat-vec on the loaded matrix (identity
+ * SNP order; standardize / ploidy as for the RSVD operator).  Re-initialises
+ * the RSVD operator of the handle. */
+int pcaone_gpu_gram_init(pcaone_gpu* h, int precision, int standardize, int ploidy);
+/* perform_op(x_in, y_out) (src/Arnoldi.hpp:18; Spectra op concept,
+ * external/Spectra/contrib/PartialSVDSolver.h:16-33): side 0 computes
+ * y = X X^T x (x, y of length nsamples; ArnoldiOpData and SVDWideMatOp),
+ * side 1 computes y = X^T X x (length nfeatures; SVDTallMatOp).  X is the
+ * standardised genotype matrix or the centred dense matrix. */
+int pcaone_gpu_gram_matvec(pcaone_gpu* h, int side, const double* x, double* y);
+/* run_pca_with_arnoldi in-core (src/Arnoldi.cpp:57-68): top-k singular
+ * triplets by a restarted FP64 Lanczos on X^T X (nsamples > nfeatures) or
+ * X X^T, ncv basis vectors, at most maxit restarts, Ritz residual tolerance
+ * tol (--ncv, --imaxiter, --itol).  U (nsamples x k), S (k), V (nfeatures x k)
+ * column-major; nconv = converged pairs, nops = mat-vec count. */
+int pcaone_gpu_iram(pcaone_gpu* h, int k, int ncv, int maxit, double tol, double* U, double* S, double* V,
+                    int* nconv, int* nops);
+
 /* Host-only view of the winSVD window schedule of one computeGandH call
  * (src/Halko.cpp:159-223): `bandsize` is the operator state carried between
  * calls (reset at pi == 0).  For each window b: acc[b] in {1, 2} (H1 or H2),
@@ -134,6 +161,13 @@ int pcaone_gpu_synchronize(pcaone_gpu* h);
  * p_anc (optional, nsnps) receives the ancestral frequencies. */
 int pcaone_sim_bed(uint64_t seed, uint32_t nsamples, uint32_t nsnps, uint32_t npops, double fst, double pmin,
                    double pmax, double miss_rate, uint8_t* payload, double* p_anc, int nthreads);
+
+/* scRNA-like dense matrix (host only): ngenes rows of ncells float32 in
+ * `out`.  Gene means base * LogNormal(0, sdlog), cell size factors
+ * LogNormal(0, 0.3), counts ~ NegBinomial(mean, dispersion phi) drawn as
+ * Gamma-Poisson, value = log1p(1e6 * count / library size). */
+int pcaone_sim_counts(uint64_t seed, uint32_t ncells, uint32_t ngenes, double base, double sdlog, double phi,
+                      float* out, int nthreads);
 
 #ifdef __cplusplus
 }

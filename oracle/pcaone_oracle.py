@@ -88,6 +88,14 @@ def centered_matrix(codes: np.ndarray, F: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(X.T)
 
 
+def dense_centered(featmajor: np.ndarray) -> np.ndarray:
+    """FileBin::read_all (src/FileBinary.cpp:21-30): X (N x M) holds the
+    float32 values cast to double, each feature minus its mean over samples."""
+    X = np.asarray(featmajor, dtype=np.float32).astype(np.float64).T.copy()
+    X -= X.mean(axis=0)
+    return X
+
+
 def standardize(X: np.ndarray, F: np.ndarray, ploidy: int = 2) -> np.ndarray:
     """Divide column i by sqrt(ploidy*F_i*(1-F_i)) when above VAR_TOL
     (src/Data.cpp:290-299).  Evaluation order (ploidy*F)*(1-F) as in C++."""
@@ -410,6 +418,23 @@ class RsvdOracle:
 
     def eigvals(self, ploidy: int = 2) -> np.ndarray:
         return self.S ** 2 / self.M * ploidy
+
+
+@dataclass
+class ExactSvd:
+    """Top-k singular triplets of X (numpy LAPACK): what IRAM converges to
+    (src/Arnoldi.cpp:57-68, PartialSVDSolver)."""
+
+    U: np.ndarray
+    S: np.ndarray
+    V: np.ndarray
+    M: int
+
+    def eigvals(self, ploidy: int = 2) -> np.ndarray:
+        return self.S ** 2 / self.M * ploidy
+
+    def loadings_unpermuted(self) -> np.ndarray:
+        return self.V
 
 
 def emu_impute(X0: np.ndarray, miss: np.ndarray, F: np.ndarray, U, S, V) -> np.ndarray:

@@ -50,6 +50,8 @@ SIGNATURES = {
     "pcaone_gpu_device_count": (C.c_int, []),
     "pcaone_gpu_load_bed": (C.c_int, [C.c_void_p, _U8, C.c_uint32, C.c_uint32]),
     "pcaone_gpu_allele_freq": (C.c_int, [C.c_void_p, _D, _I64]),
+    "pcaone_gpu_load_dense_f32": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]),
+    "pcaone_gpu_feature_means": (C.c_int, [C.c_void_p, _D]),
     "pcaone_gpu_rsvd_init": (C.c_int, [C.c_void_p, C.POINTER(RsvdParams), _D, _I32]),
     "pcaone_gpu_rsvd_dims": (C.c_int, [C.c_void_p, _I64, _I64, _I64]),
     "pcaone_gpu_rsvd_perm": (C.c_int, [C.c_void_p, _I32]),
@@ -58,6 +60,9 @@ SIGNATURES = {
     "pcaone_gpu_rsvd_set_flags": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "pcaone_gpu_rsvd_em": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, _D, _D, _D, _INT,
                                      _D]),
+    "pcaone_gpu_gram_init": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "pcaone_gpu_gram_matvec": (C.c_int, [C.c_void_p, C.c_int, _D, _D]),
+    "pcaone_gpu_iram": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, _D, _D, _D, _INT, _INT]),
     "pcaone_fancy_plan": (C.c_int, [C.c_int, C.c_int, _U64, _INT, _INT, _INT, _INT]),
     "pcaone_gpu_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "pcaone_gpu_reset_stats": (C.c_int, [C.c_void_p]),
@@ -65,6 +70,8 @@ SIGNATURES = {
     "pcaone_gpu_synchronize": (C.c_int, [C.c_void_p]),
     "pcaone_sim_bed": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                  C.c_double, C.c_double, _U8, _D, C.c_int]),
+    "pcaone_sim_counts": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(C.c_float), C.c_int]),
 }
 
 
@@ -101,6 +108,24 @@ def sim_bed(seed: int, nsamples: int, nsnps: int, npops: int = 3, fst: float = 0
     if rc != 0:
         raise GpuError(rc, "pcaone_sim_bed failed")
     return (out, panc) if want_p else out
+
+
+def sim_counts(seed: int, ncells: int, ngenes: int, base: float = 0.05, sdlog: float = 1.5, phi: float = 0.2,
+               nthreads: int = 0) -> np.ndarray:
+    """scRNA-like log1p(CPM) matrix, (ngenes, ncells) float32 (host only)."""
+    out = np.empty((ngenes, ncells), dtype=np.float32)
+    rc = lib().pcaone_sim_counts(seed, ncells, ngenes, base, sdlog, phi, _p(out, C.c_float), nthreads)
+    if rc != 0:
+        raise GpuError(rc, "pcaone_sim_counts failed")
+    return out
+
+
+def write_bin(path: str | os.PathLike, featmajor: np.ndarray):
+    """FileBin layout: u32 nfeatures, u32 nsamples, then the float32 rows."""
+    a = np.ascontiguousarray(featmajor, dtype=np.float32)
+    with open(path, "wb") as f:
+        f.write(np.array(a.shape, dtype=np.uint32).tobytes())
+        f.write(a.tobytes())
 
 
 def fancy_plan(bands: int, pi: int, bandsize: int):
@@ -155,6 +180,18 @@ class Gpu:
         self._check(self._L.pcaone_gpu_load_bed(self.h, _p(payload, C.c_uint8), nsamples, nsnps))
         self.N, self.M = nsamples, nsnps
 
+    def load_dense(self, featmajor: np.ndarray):
+        """featmajor: (nfeatures, nsamples) float32, the .bin body."""
+        a = np.ascontiguousarray(featmajor, dtype=np.float32)
+        m, n = a.shape
+        self._check(self._L.pcaone_gpu_load_dense_f32(self.h, _p(a, C.c_float), n, m))
+        self.N, self.M = n, m
+
+    def feature_means(self):
+        mu = np.empty(self.M)
+        self._check(self._L.pcaone_gpu_feature_means(self.h, _p(mu, C.c_double)))
+        return mu
+
     def allele_freq(self):
         F = np.empty(self.M)
         cnt = np.empty((self.M, 4), dtype=np.int64)
@@ -207,6 +244,26 @@ class Gpu:
                                                 _p(S, C.c_double), _p(V, C.c_double), C.byref(it),
                                                 _p(diffs, C.c_double)))
         return U, S, V, it.value, diffs[:it.value]
+
+    # ---- IRAM -------------------------------------------------------------------
+    def gram_init(self, precision: int = PREC_FP64, standardize: bool = True, ploidy: int = 2):
+        self._check(self._L.pcaone_gpu_gram_init(self.h, precision, int(standardize), ploidy))
+
+    def gram_matvec(self, side: int, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.N if side == 0 else self.M)
+        self._check(self._L.pcaone_gpu_gram_matvec(self.h, side, _p(x, C.c_double), _p(y, C.c_double)))
+        return y
+
+    def iram(self, k: int, ncv: int | None = None, maxit: int = 1000, tol: float = 1e-6):
+        ncv = ncv if ncv is not None else max(20, 2 * k + 1)  # src/Cmd.cpp:189
+        U = np.empty((self.N, k), order="F")
+        S = np.empty(k)
+        V = np.empty((self.M, k), order="F")
+        nconv, nops = C.c_int(0), C.c_int(0)
+        self._check(self._L.pcaone_gpu_iram(self.h, k, ncv, maxit, tol, _p(U, C.c_double), _p(S, C.c_double),
+                                             _p(V, C.c_double), C.byref(nconv), C.byref(nops)))
+        return U, S, V, nconv.value, nops.value
 
     # ---- timing ---------------------------------------------------------------
     def profiling(self, on: bool):

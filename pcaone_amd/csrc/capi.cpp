@@ -76,6 +76,17 @@ int pcaone_gpu_load_bed(pcaone_gpu* h, const uint8_t* payload, uint32_t nsamples
   return guarded(h, [&](pcaone::Engine& e) { e.load_bed(payload, nsamples, nsnps); });
 }
 
+int pcaone_gpu_load_dense_f32(pcaone_gpu* h, const float* featmajor, uint32_t nsamples, uint32_t nfeatures) {
+  return guarded(h, [&](pcaone::Engine& e) { e.load_dense(featmajor, nsamples, nfeatures); });
+}
+
+int pcaone_gpu_feature_means(pcaone_gpu* h, double* mu) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    PCAONE_REQUIRE(mu != nullptr, PCAONE_EINVAL, "feature_means: null output");
+    e.feature_means(mu);
+  });
+}
+
 int pcaone_gpu_allele_freq(pcaone_gpu* h, double* F, int64_t* counts) {
   return guarded(h, [&](pcaone::Engine& e) { e.allele_freq(F, counts); });
 }
@@ -125,6 +136,19 @@ int pcaone_gpu_rsvd_set_flags(pcaone_gpu* h, int update, int standardize) {
 int pcaone_gpu_rsvd_em(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
                        double* V, int* iters, double* diffs) {
   return guarded(h, [&](pcaone::Engine& e) { e.rsvd_em(maxp, tol, maxiter, tolem, U, S, V, iters, diffs); });
+}
+
+int pcaone_gpu_gram_init(pcaone_gpu* h, int precision, int standardize, int ploidy) {
+  return guarded(h, [&](pcaone::Engine& e) { e.gram_init(precision, standardize != 0, ploidy); });
+}
+
+int pcaone_gpu_gram_matvec(pcaone_gpu* h, int side, const double* x, double* y) {
+  return guarded(h, [&](pcaone::Engine& e) { e.gram_matvec(side, x, y); });
+}
+
+int pcaone_gpu_iram(pcaone_gpu* h, int k, int ncv, int maxit, double tol, double* U, double* S, double* V,
+                    int* nconv, int* nops) {
+  return guarded(h, [&](pcaone::Engine& e) { e.iram(k, ncv, maxit, tol, U, S, V, nconv, nops); });
 }
 
 int pcaone_fancy_plan(int bands, int pi, uint64_t* bandsize, int* acc, int* update, int* zero, int* reset_both) {

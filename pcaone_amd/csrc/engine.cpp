@@ -138,10 +138,41 @@ void Engine::load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps)
     F_orig_[i] = n ? ((double)n00 + 0.5 * (double)n10) / (double)n : 0.0;
   }
   have_bed_ = true;
+  src_ = SRC_BED;
+}
+
+// Replaces FileBin::read_all (src/FileBinary.cpp:21-30): nfeatures rows of
+// nsamples float32 stay in HBM; each feature's mean (FP64) is subtracted in
+// the contraction epilogues.
+void Engine::load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfeatures) {
+  PCAONE_REQUIRE(featmajor && nsamples > 0 && nfeatures > 0, -1, "load_dense: empty input");
+  PCAONE_HIP(hipSetDevice(device_));
+  have_op_ = false;
+  N_ = nsamples;
+  M_ = nfeatures;
+  rawf_.reserve((uint64_t)N_ * M_);
+  PCAONE_HIP(hipMemcpyAsync(rawf_.p, featmajor, (uint64_t)N_ * M_ * 4, hipMemcpyHostToDevice, stream_));
+  DevBuf<double> mu;
+  mu.reserve(M_);
+  timed("dense_mean", [&] { launch_dense_mean(rawf_.p, N_, M_, mu.p, stream_); });
+  mu_orig_.assign(M_, 0.0);
+  PCAONE_HIP(hipMemcpyAsync(mu_orig_.data(), mu.p, (size_t)M_ * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+  // the epilogues subtract 2 F_i colsum, so F carries mu / 2
+  F_orig_.resize(M_);
+  for (uint32_t i = 0; i < M_; ++i) F_orig_[i] = 0.5 * mu_orig_[i];
+  counts_orig_.clear();
+  have_bed_ = true;
+  src_ = SRC_DENSE;
+}
+
+void Engine::feature_means(double* mu) const {
+  PCAONE_REQUIRE(have_bed_ && src_ == SRC_DENSE, -3, "feature_means: no dense matrix loaded");
+  std::memcpy(mu, mu_orig_.data(), mu_orig_.size() * sizeof(double));
 }
 
 void Engine::allele_freq(double* F, int64_t* counts) const {
-  PCAONE_REQUIRE(have_bed_, -3, "allele_freq: no data loaded");
+  PCAONE_REQUIRE(have_bed_ && src_ == SRC_BED, -3, "allele_freq: no genotypes loaded");
   if (F) std::memcpy(F, F_orig_.data(), F_orig_.size() * sizeof(double));
   if (counts) std::memcpy(counts, counts_orig_.data(), counts_orig_.size() * sizeof(int64_t));
 }
@@ -161,9 +192,10 @@ void Engine::rsvd_perm(int32_t* perm) const {
 // ---------------------------------------------------------------------------
 // operator set-up
 // ---------------------------------------------------------------------------
+// split-K factor: enough workgroups to cover the 256 CUs twice, few partials
 static uint32_t split_for(uint64_t blocks, uint32_t kchunks, uint32_t& cps) {
-  uint64_t want = ceil_div(1024, std::max<uint64_t>(blocks, 1));
-  want = std::min<uint64_t>(std::max<uint64_t>(want, 1), std::min<uint32_t>(kchunks, 16));
+  uint64_t want = ceil_div(512, std::max<uint64_t>(blocks, 1));
+  want = std::min<uint64_t>(std::max<uint64_t>(want, 1), std::min<uint32_t>(kchunks, 8));
   cps = (uint32_t)ceil_div(kchunks, want);
   return (uint32_t)ceil_div(kchunks, cps);
 }
@@ -171,6 +203,7 @@ static uint32_t split_for(uint64_t blocks, uint32_t kchunks, uint32_t& cps) {
 void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* perm) {
   PCAONE_REQUIRE(have_bed_, -3, "rsvd_init: no data loaded");
   PCAONE_HIP(hipSetDevice(device_));
+  have_mv_ = false;
   PCAONE_REQUIRE(p.variant == 1 || p.variant == 2, -1, "rsvd_init: variant must be 1 (normal) or 2 (fancy)");
   PCAONE_REQUIRE(p.precision >= 0 && p.precision <= 2, -1, "rsvd_init: bad precision");
   PCAONE_REQUIRE(p.k >= 1 && p.os >= 0, -1, "rsvd_init: bad k / oversamples");
@@ -181,6 +214,7 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   PCAONE_REQUIRE(l <= 64, -1, "rsvd_init: k + oversamples must be <= 64");
   PCAONE_REQUIRE(l <= N_ && l <= M_, -1, "rsvd_init: k + oversamples exceeds the matrix dimensions");
   p_ = p;
+  if (src_ == SRC_DENSE) p_.standardize = 0;  // dense inputs are only centred (setFlags(false, false))
   l_ = l;
   // windows: winSVD uses blocksize = ceil(M / bands) (src/Halko.cpp:178); the
   // plain RSVD uses the same cut only as a tiling of the H contraction.
@@ -244,26 +278,18 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   G_.reserve((uint64_t)M_ * l_);
   g2_.reserve((uint64_t)M_ * l_);
   S2_.reserve((uint64_t)nwin_ * l_);
+  s2p_.reserve((uint64_t)nwin_ * finalize_chunks(bs_) * l_);
   for (auto* h : {&H1_, &H2_, &H_}) {
     h->reserve((uint64_t)N_ * l_);
     PCAONE_HIP(hipMemsetAsync(h->p, 0, h->bytes(), stream_));
   }
   uint64_t gmax = 0, hmax = 0;
-  const uint32_t rows_per_block = rows_per_block_();
-  const uint32_t kcg = (uint32_t)(kg_ / (dense ? 64 : 256)), kch = (uint32_t)(kw_ / (dense ? 64 : 256));
-  for (uint32_t nw = 1; nw <= nwin_; ++nw) {
-    const uint64_t rows = std::min<uint64_t>((uint64_t)nw * bs_, M_);
-    uint32_t cps;
-    const uint32_t sg = split_for(ceil_div(rows, rows_per_block), kcg, cps);
-    gmax = std::max<uint64_t>(gmax, sg * rows);
-    const uint32_t sh = split_for(ceil_div(N_, rows_per_block) * nw, kch, cps);
-    hmax = std::max<uint64_t>(hmax, (uint64_t)sh * N_ * nw);
-  }
+  partial_rows(prec, gmax, hmax);
   gpart_.reserve(gmax * ldp * partsz);
   hpart_.reserve(hmax * ldp * partsz);
   // linear algebra scratch
-  part_.reserve((uint64_t)256 * l_ * l_);
-  fpart_.reserve((uint64_t)256 * 3 * l_);
+  part_.reserve((uint64_t)1024 * l_ * l_);
+  fpart_.reserve((uint64_t)1024 * 3 * l_);
   for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &W_}) b->reserve((uint64_t)l_ * l_);
   sign_.reserve(l_);
   evals_.reserve(l_);
@@ -288,11 +314,44 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
 }
 
 void Engine::build_layout() {
+  if (src_ == SRC_DENSE) {
+    // rows (laid-out order, zero padded to 64 samples) for the G contraction and
+    // a windowed transpose [window][sample][feature] for the H contraction
+    kg_ = round_up(N_, 64);
+    kw_ = round_up(bs_, 64);
+    miss_rows_ = N_;
+    d_perm_.reserve(M_);
+    PCAONE_HIP(hipMemcpyAsync(d_perm_.p, perm_.data(), (size_t)M_ * 4, hipMemcpyHostToDevice, stream_));
+    dns_.reserve((uint64_t)M_ * kg_);
+    timed("dense_layout", [&] { launch_dense_rows(rawf_.p, N_, M_, d_perm_.p, dns_.p, (uint32_t)kg_, stream_); });
+    dnt_.reserve((uint64_t)nwin_ * N_ * kw_);
+    timed("dense_layout", [&] {
+      launch_dense_transpose(dns_.p, (uint32_t)kg_, M_, bs_, nwin_, N_, dnt_.p, (uint32_t)kw_, stream_);
+    });
+    F_lay_.resize(M_);
+    for (uint32_t r = 0; r < M_; ++r) F_lay_[r] = F_orig_[perm_[r]];
+    F_.reserve(M_);
+    PCAONE_HIP(hipMemcpyAsync(F_.p, F_lay_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+    upload_scale();
+    // no missing entries
+    nnz_snp_ = 0;
+    miss_snp_ptr_.reserve((uint64_t)M_ + 1);
+    miss_win_ptr_.reserve((uint64_t)nwin_ * N_ + 1);
+    PCAONE_HIP(hipMemsetAsync(miss_snp_ptr_.p, 0, miss_snp_ptr_.bytes(), stream_));
+    PCAONE_HIP(hipMemsetAsync(miss_win_ptr_.p, 0, miss_win_ptr_.bytes(), stream_));
+    miss_snp_idx_.reserve(1);
+    miss_win_idx_.reserve(1);
+    miss_win_pos_.reserve(1);
+    sync();
+    return;
+  }
   const uint64_t nb = (N_ + 3) >> 2;
   row_bytes_ = (uint32_t)round_up(nb, 64);
   npad_ = row_bytes_ * 4;
   win_bytes_ = (uint32_t)round_up(ceil_div(bs_, 4), 64);
   kw_ = (uint64_t)win_bytes_ * 4;
+  kg_ = npad_;
+  miss_rows_ = npad_;
   d_perm_.reserve(M_);
   PCAONE_HIP(hipMemcpyAsync(d_perm_.p, perm_.data(), (size_t)M_ * 4, hipMemcpyHostToDevice, stream_));
   bed_snp_.reserve((uint64_t)row_bytes_ * M_);
@@ -364,6 +423,167 @@ void Engine::set_flags(bool update, bool standardize) {
   }
 }
 
+// Largest partial-product row counts over every possible segment (split-K
+// slices x rows) of the G and H contractions.
+void Engine::partial_rows(int prec, uint64_t& gmax, uint64_t& hmax) const {
+  const bool dense = src_ == SRC_DENSE;
+  const uint32_t rpb = rows_per_block_(prec);
+  const uint32_t kcg = (uint32_t)(kg_ / (dense ? 64 : 256)), kch = (uint32_t)(kw_ / (dense ? 64 : 256));
+  gmax = hmax = 0;
+  for (uint32_t nw = 1; nw <= nwin_; ++nw) {
+    const uint64_t rows = std::min<uint64_t>((uint64_t)nw * bs_, M_);
+    uint32_t cps;
+    const uint32_t sg = split_for(ceil_div(rows, rpb), kcg, cps);
+    gmax = std::max<uint64_t>(gmax, sg * rows);
+    const uint32_t sh = split_for(ceil_div(N_, rpb) * nw, kch, cps);
+    hmax = std::max<uint64_t>(hmax, (uint64_t)sh * N_ * nw);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gram mat-vec: the Spectra op of IRAM (ArnoldiOpData::perform_op,
+// src/Arnoldi.cpp:18-46; SVDTallMatOp / SVDWideMatOp::perform_op,
+// external/Spectra/contrib/PartialSVDSolver.h:65-71, 104-110), on the same
+// contractions as the RSVD operator with one operand column.
+// ---------------------------------------------------------------------------
+void Engine::gram_init(int precision, bool standardize, int ploidy) {
+  PCAONE_REQUIRE(have_bed_, -3, "gram_init: no data loaded");
+  PCAONE_REQUIRE(precision >= 0 && precision <= 2, -1, "gram_init: bad precision");
+  RsvdParams p;
+  p.variant = 1;  // identity SNP order, windows only tile the H contraction
+  p.k = 1;
+  p.os = 1;
+  p.bands = 64;
+  while (p.bands > 4 && (uint64_t)p.bands > M_) p.bands /= 2;
+  p.precision = precision;
+  p.standardize = standardize;
+  p.ploidy = ploidy;
+  rsvd_init(p, nullptr, nullptr);
+  const bool dense = src_ == SRC_DENSE;
+  SegBufs& b = mv_;
+  b = SegBufs{};
+  b.prec = precision;
+  b.l = 1;
+  b.opsz = precision == PREC_SPLIT_BF16 ? 2 : precision == PREC_FP32 ? 4 : 8;
+  b.part_f64 = precision == PREC_FP64;
+  if (!dense) {
+    b.nt16 = 1;
+    b.bcols = precision == PREC_SPLIT_BF16 ? 16 : 1;
+    b.lo_off = 1;
+    b.ldp = precision == PREC_SPLIT_BF16 ? 16 : 1;
+    b.split_cols = precision == PREC_SPLIT_BF16;
+    b.hf = 0.5;
+  } else {
+    b.ntiles = 1;
+    b.bcols = precision == PREC_FP64 ? 1 : precision == PREC_FP32 ? 16 : 32;
+    b.lo_off = 16;
+    b.ldp = precision == PREC_FP64 ? 1 : 16;
+    b.hf = 1.0;
+  }
+  uint64_t gmax, hmax;
+  partial_rows(precision, gmax, hmax);
+  const size_t partsz = b.part_f64 ? 8 : 4;
+  mv_omega_.reserve(N_);
+  mv_colsum_.reserve(1);
+  mv_G_.reserve(M_);
+  mv_g2_.reserve(M_);
+  mv_S2_.reserve(nwin_);
+  mv_s2p_.reserve((uint64_t)nwin_ * finalize_chunks(bs_));
+  mv_H_.reserve(N_);
+  mv_bg_.reserve((uint64_t)b.bcols * kg_ * b.opsz);
+  mv_bh_.reserve((uint64_t)nwin_ * b.bcols * kw_ * b.opsz);
+  PCAONE_HIP(hipMemsetAsync(mv_bh_.p, 0, mv_bh_.bytes(), stream_));
+  mv_gpart_.reserve(gmax * b.ldp * partsz);
+  mv_hpart_.reserve(hmax * b.ldp * partsz);
+  b.omega = mv_omega_.p;
+  b.colsum = mv_colsum_.p;
+  b.G = mv_G_.p;
+  b.g2 = mv_g2_.p;
+  b.S2 = mv_S2_.p;
+  b.s2p = mv_s2p_.p;
+  b.bg = mv_bg_.p;
+  b.bh = mv_bh_.p;
+  b.gpart = mv_gpart_.p;
+  b.hpart = mv_hpart_.p;
+  b.Ha = b.Hb = mv_H_.p;
+  b.mval = nullptr;
+  have_mv_ = true;
+  nops_ = 0;
+  sync();
+}
+
+void Engine::set_omega_vec(const double* dev_v, const double* host_v) {
+  std::vector<double> h;
+  if (!host_v) {
+    h.resize(N_);
+    PCAONE_HIP(hipMemcpyAsync(h.data(), dev_v, (size_t)N_ * 8, hipMemcpyDeviceToHost, stream_));
+    sync();
+    host_v = h.data();
+  } else {
+    PCAONE_HIP(hipMemcpyAsync(mv_omega_.p, host_v, (size_t)N_ * 8, hipMemcpyHostToDevice, stream_));
+  }
+  if (dev_v) PCAONE_HIP(hipMemcpyAsync(mv_omega_.p, dev_v, (size_t)N_ * 8, hipMemcpyDeviceToDevice, stream_));
+  double cs = 0.0;
+  for (uint32_t j = 0; j < N_; ++j) cs += host_v[j];
+  PCAONE_HIP(hipMemcpyAsync(mv_colsum_.p, &cs, 8, hipMemcpyHostToDevice, stream_));
+  timed("omega_operand", [&] {
+    launch_omega_operand(mv_omega_.p, N_, 1, mv_.prec, mv_bg_.p, mv_.bcols, mv_.lo_off, kg_, stream_);
+  });
+  sync();  // cs lives on this stack frame
+}
+
+// v = X x (x: M, original feature order; v: N)
+void Engine::apply_X(const double* x, double* v) {
+  const std::vector<int> route(nwin_, 1);
+  std::vector<double> xl(M_);
+  for (uint32_t r = 0; r < M_; ++r) xl[r] = x[perm_[r]];
+  PCAONE_HIP(hipMemcpyAsync(mv_G_.p, xl.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  FinalizeG fd = finalize_desc(0, nwin_, mv_);
+  timed("finalize_G", [&] { launch_operand_from_G(fd, stream_); });
+  timed("window_colsum", [&] { launch_window_colsum(mv_s2p_.p, 1, bs_, 0, nwin_, mv_S2_.p, stream_); });
+  timed("fill", [&] { launch_fill_f64(mv_H_.p, N_, 0.0, stream_); });
+  seg_H(0, nwin_, route, mv_);
+  if (v) {
+    PCAONE_HIP(hipMemcpyAsync(v, mv_H_.p, (size_t)N_ * 8, hipMemcpyDeviceToHost, stream_));
+  }
+  sync();
+}
+
+// y = X^T v (v: N; y: M, original feature order); v == nullptr uses mv_H_
+void Engine::apply_Xt(const double* v, double* y) {
+  if (v)
+    set_omega_vec(nullptr, v);
+  else
+    set_omega_vec(mv_H_.p, nullptr);
+  seg_G(0, nwin_, mv_);
+  std::vector<double> gl(M_);
+  PCAONE_HIP(hipMemcpyAsync(gl.data(), mv_G_.p, (size_t)M_ * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+  for (uint32_t r = 0; r < M_; ++r) y[perm_[r]] = gl[r];
+}
+
+void Engine::gram_matvec(int side, const double* x, double* y) {
+  PCAONE_REQUIRE(have_mv_, -3, "gram_matvec: call gram_init first");
+  PCAONE_REQUIRE(side == 0 || side == 1, -1, "gram_matvec: side must be 0 (X X^T x) or 1 (X^T X x)");
+  PCAONE_REQUIRE(x && y, -1, "gram_matvec: null vector");
+  PCAONE_HIP(hipSetDevice(device_));
+  if (side == 0) {
+    // y = X (X^T x): x plays Omega (N x 1)
+    const std::vector<int> route(nwin_, 1);
+    set_omega_vec(nullptr, x);
+    timed("fill", [&] { launch_fill_f64(mv_H_.p, N_, 0.0, stream_); });
+    seg_G(0, nwin_, mv_);
+    seg_H(0, nwin_, route, mv_);
+    PCAONE_HIP(hipMemcpyAsync(y, mv_H_.p, (size_t)N_ * 8, hipMemcpyDeviceToHost, stream_));
+    sync();
+  } else {
+    // y = X^T (X x)
+    apply_X(x, nullptr);
+    apply_Xt(nullptr, y);
+  }
+  ++nops_;
+}
+
 // RsvdOpData::initOmg, src/Halko.cpp:14-22: default_random_engine seeded with
 // --seed, an N x l NullaryExpr filled in column-major order.  The draw only
 // depends on (seed, rand, N, l), so it is made once per operator and every
@@ -403,7 +623,7 @@ void Engine::init_omega() {
   PCAONE_HIP(hipMemcpyAsync(omg2_.p, omega0_.p, nb, hipMemcpyDeviceToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(colsum_.p, colsum0_.p, (size_t)l_ * 8, hipMemcpyDeviceToDevice, stream_));
   timed("omega_operand",
-        [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, npad_, stream_); });
+        [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, lo_off_, kg_, stream_); });
 }
 
 // Omega <- Householder Q of H (H = H1 + H2 for winSVD), then flipOmg
@@ -432,109 +652,193 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
   timed("flip", [&] { launch_flip_stats(omega_.p, omg2_.p, N_, l, fpart_.p, nb2, stream_); });
   timed("flip", [&] { launch_flip_apply(omega_.p, omg2_.p, N_, l, fpart_.p, nb2, colsum_.p, zero_acc, stream_); });
   timed("omega_operand",
-        [&] { launch_omega_operand(omega_.p, N_, l, p_.precision, bg_.p, bcols_, npad_, stream_); });
+        [&] { launch_omega_operand(omega_.p, N_, l, p_.precision, bg_.p, bcols_, lo_off_, kg_, stream_); });
 }
 
 // G rows of windows [w0, w1) and their H contributions, routed per window.
-void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route) {
-  const int prec = p_.precision;
+SegBufs Engine::op_bufs() {
+  SegBufs b;
+  b.prec = p_.precision;
+  b.l = l_;
+  b.nt16 = nt16_;
+  b.ntiles = ntiles_;
+  b.bcols = bcols_;
+  b.lo_off = lo_off_;
+  b.ldp = ldp_;
+  b.opsz = opsz_;
+  b.part_f64 = part_f64_;
+  b.split_cols = split_cols_;
+  b.hf = hf_;
+  b.omega = omega_.p;
+  b.colsum = colsum_.p;
+  b.G = G_.p;
+  b.g2 = g2_.p;
+  b.S2 = S2_.p;
+  b.s2p = s2p_.p;
+  b.bg = bg_.p;
+  b.bh = bh_.p;
+  b.gpart = gpart_.p;
+  b.hpart = hpart_.p;
+  b.Ha = (p_.variant == 2) ? H1_.p : H_.p;
+  b.Hb = (p_.variant == 2) ? H2_.p : H_.p;
+  b.mval = have_vals_ ? miss_val_.p : nullptr;
+  return b;
+}
+
+// G rows of windows [w0, w1): G = X_seg^T Omega, its epilogue (centring,
+// missing-entry corrections, H operand) and the per-window sums S''.
+void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
+  const int prec = b.prec;
+  const bool dense = src_ == SRC_DENSE;
   const uint32_t r0 = w0 * bs_, r1 = std::min<uint32_t>(w1 * bs_, M_);
   const uint32_t rows = r1 - r0;
-  const uint64_t ldp = prec == PREC_SPLIT_BF16 ? (uint64_t)nt16_ * 16 : l_;
-  const uint32_t rpb = prec == PREC_SPLIT_BF16 ? 256 : 512;
-  // ---- G = X_seg^T Omega --------------------------------------------------
-  PackedGemm g{};
-  g.A = bed_snp_.p + (uint64_t)r0 * row_bytes_;
-  g.lda = row_bytes_;
-  g.a_zstride = 0;
-  g.nrows = rows;
-  g.kchunks = row_bytes_ / 64;
-  g.nsplit = split_for(ceil_div(rows, rpb), g.kchunks, g.chunks_per_split);
-  g.B = bg_.p;
-  g.ldb = npad_;
-  g.b_zstride = 0;
-  g.out = gpart_.p;
-  g.ldo = ldp;
-  g.o_zstride = 0;
-  g.nz = 1;
-  timed("gemm_G", [&] {
-    if (prec == PREC_SPLIT_BF16)
-      launch_packed_gemm_bf16(g, nt16_, stream_);
-    else if (prec == PREC_FP32)
-      launch_packed_gemm_f32(g, (int)l_, stream_);
-    else
-      launch_packed_gemm_f64(g, (int)l_, stream_);
-  });
+  const uint32_t rpb = rows_per_block_(prec);
+  uint32_t g_nsplit = 0;
+  if (!dense) {
+    PackedGemm g{};
+    g.A = bed_snp_.p + (uint64_t)r0 * row_bytes_;
+    g.lda = row_bytes_;
+    g.nrows = rows;
+    g.kchunks = row_bytes_ / 64;
+    g.nsplit = split_for(ceil_div(rows, rpb), g.kchunks, g.chunks_per_split);
+    g.B = b.bg;
+    g.ldb = kg_;
+    g.out = b.gpart;
+    g.ldo = b.ldp;
+    g.nz = 1;
+    g_nsplit = g.nsplit;
+    timed("gemm_G", [&] {
+      if (prec == PREC_SPLIT_BF16)
+        launch_packed_gemm_bf16(g, b.nt16, stream_);
+      else if (prec == PREC_FP32)
+        launch_packed_gemm_f32(g, (int)b.l, stream_);
+      else
+        launch_packed_gemm_f64(g, (int)b.l, stream_);
+    });
+  } else {
+    DenseGemm g{};
+    g.A = dns_.p + (uint64_t)r0 * kg_;
+    g.lda = kg_;
+    g.nrows = rows;
+    g.kchunks = (uint32_t)(kg_ / 64);
+    g.nsplit = split_for(ceil_div(rows, rpb), g.kchunks, g.chunks_per_split);
+    g.B = b.bg;
+    g.ldb = kg_;
+    g.out = b.gpart;
+    g.ldo = b.ldp;
+    g.nz = 1;
+    g_nsplit = g.nsplit;
+    timed("gemm_G", [&] { launch_dense_gemm(g, prec, (int)b.l, b.ntiles, stream_); });
+  }
+  FinalizeG fg = finalize_desc(w0, w1, b);
+  fg.part = b.gpart;
+  fg.nsplit = g_nsplit;
+  timed("finalize_G", [&] { launch_finalize_G(fg, stream_); });
+  timed("window_colsum", [&] { launch_window_colsum(b.s2p, b.l, bs_, w0, w1, b.S2, stream_); });
+}
+
+FinalizeG Engine::finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) const {
   FinalizeG fg{};
-  fg.part = gpart_.p;
-  fg.part_is_f64 = prec == PREC_FP64;
-  fg.nsplit = g.nsplit;
-  fg.ldp = ldp;
-  fg.split_cols = prec == PREC_SPLIT_BF16;
-  fg.r0 = r0;
-  fg.r1 = r1;
-  fg.l = l_;
+  fg.part_is_f64 = b.part_f64;
+  fg.ldp = b.ldp;
+  fg.split_cols = b.split_cols;
+  fg.r0 = w0 * bs_;
+  fg.r1 = std::min<uint32_t>(w1 * bs_, M_);
+  fg.l = b.l;
   fg.F = F_.p;
   fg.d = d_.p;
-  fg.colsum = colsum_.p;
+  fg.colsum = b.colsum;
   fg.miss_ptr = miss_snp_ptr_.p;
   fg.miss_idx = miss_snp_idx_.p;
-  fg.omega = omega_.p;
-  fg.G = G_.p;
-  fg.g2 = g2_.p;
-  fg.bprec = prec;
-  fg.bh = bh_.p;
-  fg.bcols = bcols_;
+  fg.omega = b.omega;
+  fg.G = b.G;
+  fg.g2 = b.g2;
+  fg.bprec = b.prec;
+  fg.bh = b.bh;
+  fg.bcols = b.bcols;
+  fg.lo_off = b.lo_off;
   fg.kw = kw_;
   fg.blocksize = bs_;
-  fg.mval = have_vals_ ? miss_val_.p : nullptr;
-  timed("finalize_G", [&] { launch_finalize_G(fg, stream_); });
-  timed("window_colsum", [&] { launch_window_colsum(g2_.p, l_, M_, bs_, w0, w1, S2_.p, stream_); });
-  // ---- H += X_w G_w per window -------------------------------------------
+  fg.mval = b.mval;
+  fg.hf = b.hf;
+  fg.w0 = w0;
+  fg.w1 = w1;
+  fg.nsnps = M_;
+  fg.s2p = b.s2p;
+  return fg;
+}
+
+// H contributions of windows [w0, w1) (X_w G_w), routed per window to Ha / Hb.
+void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b) {
+  const int prec = b.prec;
+  const bool dense = src_ == SRC_DENSE;
+  const uint32_t rpb = rows_per_block_(prec);
   const uint32_t nz = w1 - w0;
-  PackedGemm h{};
-  h.A = bed_smp_.p + (uint64_t)w0 * npad_ * win_bytes_;
-  h.lda = win_bytes_;
-  h.a_zstride = (uint64_t)npad_ * win_bytes_;
-  h.nrows = N_;
-  h.kchunks = win_bytes_ / 64;
-  h.nsplit = split_for(ceil_div(N_, rpb) * nz, h.kchunks, h.chunks_per_split);
-  h.B = bh_.p + (uint64_t)w0 * bcols_ * kw_ * opsz_;
-  h.ldb = kw_;
-  h.b_zstride = (uint64_t)bcols_ * kw_;
-  h.out = hpart_.p;
-  h.ldo = ldp;
-  h.o_zstride = (uint64_t)h.nsplit * N_ * ldp;
-  h.nz = nz;
-  timed("gemm_H", [&] {
-    if (prec == PREC_SPLIT_BF16)
-      launch_packed_gemm_bf16(h, nt16_, stream_);
-    else if (prec == PREC_FP32)
-      launch_packed_gemm_f32(h, (int)l_, stream_);
-    else
-      launch_packed_gemm_f64(h, (int)l_, stream_);
-  });
+  uint32_t h_nsplit = 0;
+  if (!dense) {
+    PackedGemm h{};
+    h.A = bed_smp_.p + (uint64_t)w0 * npad_ * win_bytes_;
+    h.lda = win_bytes_;
+    h.a_zstride = (uint64_t)npad_ * win_bytes_;
+    h.nrows = N_;
+    h.kchunks = win_bytes_ / 64;
+    h.nsplit = split_for(ceil_div(N_, rpb) * nz, h.kchunks, h.chunks_per_split);
+    h.B = b.bh + (uint64_t)w0 * b.bcols * kw_ * b.opsz;
+    h.ldb = kw_;
+    h.b_zstride = (uint64_t)b.bcols * kw_;
+    h.out = b.hpart;
+    h.ldo = b.ldp;
+    h.o_zstride = (uint64_t)h.nsplit * N_ * b.ldp;
+    h.nz = nz;
+    h_nsplit = h.nsplit;
+    timed("gemm_H", [&] {
+      if (prec == PREC_SPLIT_BF16)
+        launch_packed_gemm_bf16(h, b.nt16, stream_);
+      else if (prec == PREC_FP32)
+        launch_packed_gemm_f32(h, (int)b.l, stream_);
+      else
+        launch_packed_gemm_f64(h, (int)b.l, stream_);
+    });
+  } else {
+    DenseGemm h{};
+    h.A = dnt_.p + (uint64_t)w0 * N_ * kw_;
+    h.lda = kw_;
+    h.a_zstride = (uint64_t)N_ * kw_;
+    h.nrows = N_;
+    h.kchunks = (uint32_t)(kw_ / 64);
+    h.nsplit = split_for(ceil_div(N_, rpb) * nz, h.kchunks, h.chunks_per_split);
+    h.B = b.bh + (uint64_t)w0 * b.bcols * kw_ * b.opsz;
+    h.ldb = kw_;
+    h.b_zstride = (uint64_t)b.bcols * kw_;
+    h.out = b.hpart;
+    h.ldo = b.ldp;
+    h.o_zstride = (uint64_t)h.nsplit * N_ * b.ldp;
+    h.nz = nz;
+    h_nsplit = h.nsplit;
+    timed("gemm_H", [&] { launch_dense_gemm(h, prec, (int)b.l, b.ntiles, stream_); });
+  }
   FinalizeH fh{};
-  fh.part = hpart_.p;
-  fh.part_is_f64 = prec == PREC_FP64;
-  fh.nsplit = h.nsplit;
-  fh.ldp = ldp;
+  fh.part = b.hpart;
+  fh.part_is_f64 = b.part_f64;
+  fh.nsplit = h_nsplit;
+  fh.ldp = b.ldp;
   fh.part_rows = N_;
-  fh.miss_rows = npad_;
-  fh.split_cols = prec == PREC_SPLIT_BF16;
+  fh.miss_rows = miss_rows_;
+  fh.split_cols = b.split_cols;
   fh.nsamples = N_;
-  fh.l = l_;
+  fh.l = b.l;
   fh.wpart0 = w0;
-  fh.Ha = (p_.variant == 2) ? H1_.p : H_.p;
-  fh.Hb = (p_.variant == 2) ? H2_.p : H_.p;
-  fh.S2 = S2_.p;
+  fh.Ha = b.Ha;
+  fh.Hb = b.Hb;
+  fh.S2 = b.S2;
   fh.miss_ptr = miss_win_ptr_.p;
   fh.miss_idx = miss_win_idx_.p;
-  fh.g2 = g2_.p;
-  fh.mval = have_vals_ ? miss_val_.p : nullptr;
+  fh.g2 = b.g2;
+  fh.mval = b.mval;
   fh.mpos = miss_win_pos_.p;
-  fh.G = G_.p;
+  fh.G = b.G;
   fh.d = d_.p;
+  fh.hf = b.hf;
   for (uint32_t a = w0; a < w1; a += 64) {  // route travels as a 64-bit mask
     fh.w0 = a;
     fh.w1 = std::min<uint32_t>(a + 64, w1);
@@ -543,6 +847,12 @@ void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route
       if (route[w] == 2) fh.route_b |= 1ull << (w - a);
     timed("finalize_H", [&] { launch_finalize_H(fh, stream_); });
   }
+}
+
+void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route) {
+  const SegBufs b = op_bufs();
+  seg_G(w0, w1, b);
+  seg_H(w0, w1, route, b);
 }
 
 void Engine::rsvd_gandh(int pi, double* Gout, double* Hout) {

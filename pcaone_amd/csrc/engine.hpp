@@ -47,6 +47,25 @@ struct RsvdParams {
       ploidy = 2;
 };
 
+// Buffers and operand format of one set of contractions (the RSVD operator's
+// l columns, or the single column of the Gram mat-vec).
+struct SegBufs {
+  int prec = 0;
+  uint32_t l = 0;
+  int nt16 = 0, ntiles = 0;
+  uint32_t bcols = 0, lo_off = 0;
+  uint64_t ldp = 0;
+  size_t opsz = 8;
+  bool part_f64 = false, split_cols = false;
+  double hf = 0.5;
+  const double* omega = nullptr;
+  const double* colsum = nullptr;
+  double *G = nullptr, *g2 = nullptr, *S2 = nullptr, *s2p = nullptr;
+  unsigned char *bg = nullptr, *bh = nullptr, *gpart = nullptr, *hpart = nullptr;
+  double *Ha = nullptr, *Hb = nullptr;
+  const double* mval = nullptr;
+};
+
 class Engine {
  public:
   explicit Engine(int device);
@@ -65,6 +84,11 @@ class Engine {
   void rsvd_em(int maxp, double tol, int maxiter, double tolem, double* U, double* S, double* V, int* iters,
                double* diffs);
   void dims(int64_t* rows, int64_t* cols, int64_t* size) const;
+  void gram_init(int precision, bool standardize, int ploidy);
+  void gram_matvec(int side, const double* x, double* y);
+  void eigs(int side, int k, int ncv, int maxit, double tol, double* evals, double* evecs, int* nconv,
+            int* nops);
+  void iram(int k, int ncv, int maxit, double tol, double* U, double* S, double* V, int* nconv, int* nops);
 
   KernelTimer timer;
   void sync();
@@ -82,9 +106,17 @@ class Engine {
   void usv_step();
   void check_status();
   void upload_scale();
-  uint32_t rows_per_block_() const {
-    return (src_ == SRC_DENSE || p_.precision == PREC_SPLIT_BF16) ? 256u : 512u;
+  uint32_t rows_per_block_(int prec) const {
+    return (src_ == SRC_DENSE || prec == PREC_SPLIT_BF16) ? 256u : 512u;
   }
+  SegBufs op_bufs();
+  FinalizeG finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) const;
+  void seg_G(uint32_t w0, uint32_t w1, const SegBufs& b);
+  void seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b);
+  void partial_rows(int prec, uint64_t& gmax, uint64_t& hmax) const;
+  void set_omega_vec(const double* dev_v, const double* host_v);
+  void apply_X(const double* x, double* v);
+  void apply_Xt(const double* v, double* y);
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -131,7 +163,7 @@ class Engine {
   std::vector<double> omega_given_;
   DevBuf<double> omega_, omg2_, colsum_, omega0_, colsum0_;
   DevBuf<unsigned char> bg_, bh_;   // operands of the G and H contractions
-  DevBuf<double> G_, g2_, S2_, H1_, H2_, H_;
+  DevBuf<double> G_, g2_, S2_, s2p_, H1_, H2_, H_;
   DevBuf<unsigned char> gpart_, hpart_;
   uint64_t bandsize_ = 1;
   // RsvdOpData::update / U, S, V of the last computeUSV (src/Halko.hpp:11-14)
@@ -142,6 +174,12 @@ class Engine {
   DevBuf<double> part_, fpart_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, sign_, Q_;
   DevBuf<double> Bt_, evals_, W_, sinv_, Ucur_, Upre_, V_, scal_;
   DevBuf<int> status_;
+  // Gram mat-vec state (one operand column)
+  bool have_mv_ = false;
+  int64_t nops_ = 0;
+  SegBufs mv_;
+  DevBuf<double> mv_omega_, mv_colsum_, mv_G_, mv_g2_, mv_S2_, mv_s2p_, mv_H_;
+  DevBuf<unsigned char> mv_bg_, mv_bh_, mv_gpart_, mv_hpart_;
 };
 
 }  // namespace pcaone

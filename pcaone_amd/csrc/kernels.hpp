@@ -99,8 +99,15 @@ struct FinalizeG {
   uint32_t blocksize;   // window length (last may be shorter)
   const double* mval;   // EMU values of the missing entries (CSR order) or null
   double hf;            // 0.5 for A1 dosages (BED2GENO = y/2), 1 for dense values
+  uint32_t w0, w1;      // windows of the segment ([r0, r1) are their rows)
+  uint32_t nsnps;
+  double* s2p;          // [nwin][finalize_chunks(blocksize)][l] partial sums of g2
 };
+// grid = (chunks of 32 rows, windows of the segment)
 void launch_finalize_G(const FinalizeG& f, hipStream_t s);
+uint32_t finalize_chunks(uint32_t blocksize);
+// the H operand, g2 and its chunk sums from a given f.G (no contraction before)
+void launch_operand_from_G(const FinalizeG& f, hipStream_t s);
 
 // EMU imputation of every missing entry (SNP-CSR order):
 // val = clamp(sum_c U[j][c] S[c] V[i][c], -F_i, 1 - F_i)   (src/FilePlink.cpp:253-261)
@@ -111,9 +118,10 @@ void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin,
                         uint32_t nsamples, const int64_t* sptr, const int32_t* sidx, int64_t* pos,
                         hipStream_t s);
 
-// S2[w][c] = sum over window w's rows of g2[r][c] for windows [w0, w1)
-void launch_window_colsum(const double* g2, uint32_t l, uint32_t nsnps, uint32_t blocksize, uint32_t w0,
-                          uint32_t w1, double* S2, hipStream_t s);
+// S2[w][c] = sum over window w's rows of g2[r][c] for windows [w0, w1), from
+// the chunk partials finalize_G wrote
+void launch_window_colsum(const double* s2p, uint32_t l, uint32_t blocksize, uint32_t w0, uint32_t w1, double* S2,
+                          hipStream_t s);
 
 struct FinalizeH {
   const void* part;     // [nwseg][nsplit][part_rows][ldp]

@@ -423,79 +423,142 @@ __device__ inline double load_part(const void* p, int is64, uint64_t i) {
   return is64 ? reinterpret_cast<const double*>(p)[i] : (double)reinterpret_cast<const float*>(p)[i];
 }
 
-__global__ void k_finalize_G(FinalizeG f) {
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Grid (row chunk of 32, window); thread (c = t % 64, rs = t / 64) handles rows
+// rs, rs + 4, ... of the chunk for column c.  Besides G, g'' and the H operand
+// it writes the chunk's partial column sums of g'' (s2p), which
+// k_window_colsum reduces into S''.
+constexpr uint32_t kFinRows = 32;
+
+__global__ __launch_bounds__(256) void k_finalize_G(FinalizeG f) {
+  __shared__ double sh[4][64];
+  const uint32_t t = threadIdx.x, c = t & 63, rs = t >> 6;
+  const uint32_t w = f.w0 + blockIdx.y, chunk = blockIdx.x;
+  const uint32_t wstart = w * f.blocksize;
+  const uint32_t wend = min(wstart + f.blocksize, f.nsnps);
+  const uint32_t cb = wstart + chunk * kFinRows;
   const uint64_t nrow = f.r1 - f.r0;
-  if (gid >= nrow * f.l) return;
-  const uint32_t rr = (uint32_t)(gid / f.l), c = (uint32_t)(gid % f.l);
-  const uint32_t r = f.r0 + rr;
-  double raw = 0.0;
-  for (uint32_t s = 0; s < f.nsplit; ++s) {
-    const uint64_t base = ((uint64_t)s * nrow + rr) * f.ldp;
-    raw += load_part(f.part, f.part_is_f64, base + c);
-    if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
-  }
-  double corr = 0.0, corr_e = 0.0;
-  if (f.mval) {
-    for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p) {
-      const double om = f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
-      corr += om;
-      corr_e += f.mval[p] * om;
+  double acc2 = 0.0;
+  if (c < f.l) {
+    for (uint32_t q = rs; q < kFinRows; q += 4) {
+      const uint32_t r = cb + q;
+      if (r >= wend) break;
+      const uint32_t rr = r - f.r0;
+      double raw = 0.0;
+      for (uint32_t s = 0; s < f.nsplit; ++s) {
+        const uint64_t base = ((uint64_t)s * nrow + rr) * f.ldp;
+        raw += load_part(f.part, f.part_is_f64, base + c);
+        if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
+      }
+      double corr = 0.0, corr_e = 0.0;
+      if (f.mval) {
+        for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p) {
+          const double om = f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
+          corr += om;
+          corr_e += f.mval[p] * om;
+        }
+      } else {
+        for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p)
+          corr += f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
+      }
+      const double Fi = f.F[r], di = f.d[r];
+      const double Gv = di * (f.hf * (raw - 2.0 * Fi * (f.colsum[c] - corr)) + corr_e);
+      f.G[(uint64_t)r * f.l + c] = Gv;
+      const double gp = di * Gv;
+      const double g2 = 2.0 * Fi * gp;
+      f.g2[(uint64_t)r * f.l + c] = g2;
+      acc2 += g2;
+      const uint32_t k = r - wstart;
+      const uint64_t o = ((uint64_t)w * f.bcols + c) * f.kw + k;
+      if (f.bprec == PREC_FP64) {
+        reinterpret_cast<double*>(f.bh)[o] = gp;
+      } else if (f.bprec == PREC_FP32) {
+        reinterpret_cast<float*>(f.bh)[o] = (float)gp;
+      } else {
+        const __bf16 hi = (__bf16)(float)gp;
+        const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
+        __bf16* bh = reinterpret_cast<__bf16*>(f.bh);
+        bh[o] = hi;
+        bh[((uint64_t)w * f.bcols + f.lo_off + c) * f.kw + k] = lo;
+      }
     }
-  } else {
-    for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p) corr += f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
   }
-  const double Fi = f.F[r], di = f.d[r];
-  const double Gv = di * (f.hf * (raw - 2.0 * Fi * (f.colsum[c] - corr)) + corr_e);
-  f.G[(uint64_t)r * f.l + c] = Gv;
-  const double gp = di * Gv;
-  f.g2[(uint64_t)r * f.l + c] = 2.0 * Fi * gp;
-  const uint32_t w = r / f.blocksize, k = r - w * f.blocksize;
-  const uint64_t o = ((uint64_t)w * f.bcols + c) * f.kw + k;
-  if (f.bprec == PREC_FP64) {
-    reinterpret_cast<double*>(f.bh)[o] = gp;
-  } else if (f.bprec == PREC_FP32) {
-    reinterpret_cast<float*>(f.bh)[o] = (float)gp;
-  } else {
-    const __bf16 hi = (__bf16)(float)gp;
-    const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
-    __bf16* bh = reinterpret_cast<__bf16*>(f.bh);
-    bh[o] = hi;
-    bh[((uint64_t)w * f.bcols + f.lo_off + c) * f.kw + k] = lo;
-  }
+  sh[rs][c] = acc2;
+  __syncthreads();
+  if (t < f.l)
+    f.s2p[((uint64_t)w * gridDim.x + chunk) * f.l + t] = (sh[0][t] + sh[1][t]) + (sh[2][t] + sh[3][t]);
 }
 
 void launch_finalize_G(const FinalizeG& f, hipStream_t s) {
-  const uint64_t total = (uint64_t)(f.r1 - f.r0) * f.l;
-  if (!total) return;
-  hipLaunchKernelGGL(k_finalize_G, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, f);
+  PCAONE_REQUIRE(f.l >= 1 && f.l <= 64, -1, "finalize: l must be in [1, 64]");
+  if (f.w1 <= f.w0) return;
+  dim3 grid((uint32_t)ceil_div(f.blocksize, kFinRows), f.w1 - f.w0);
+  hipLaunchKernelGGL(k_finalize_G, grid, dim3(256), 0, s, f);
 }
 
-__global__ __launch_bounds__(256) void k_window_colsum(const double* __restrict__ g2, uint32_t l,
-                                                       uint32_t nsnps, uint32_t blocksize, uint32_t w0,
-                                                       double* __restrict__ S2) {
-  __shared__ double sh[256];
-  const uint32_t w = w0 + blockIdx.x;
-  const uint32_t rs = w * blocksize, re = min(rs + blocksize, nsnps);
-  const uint32_t t = threadIdx.x, R = 256 / l;
-  const uint32_t c = t % l, rr = t / l;
-  double acc = 0.0;
-  if (rr < R)
-    for (uint32_t r = rs + rr; r < re; r += R) acc += g2[(uint64_t)r * l + c];
-  sh[t] = (rr < R) ? acc : 0.0;
-  __syncthreads();
-  if (t < l) {
-    double tot = 0.0;
-    for (uint32_t q = 0; q < R; ++q) tot += sh[q * l + t];
-    S2[(uint64_t)w * l + t] = tot;
+uint32_t finalize_chunks(uint32_t blocksize) { return (uint32_t)ceil_div(blocksize, kFinRows); }
+
+// H-contraction operand from a given G (rows [r0, r1) of windows [w0, w1)): the
+// output half of finalize_G for an externally supplied G, so X G can run
+// without a preceding X^T Omega (Gram mat-vec X^T X x starts here).
+__global__ __launch_bounds__(256) void k_operand_from_G(FinalizeG f) {
+  __shared__ double sh[4][64];
+  const uint32_t t = threadIdx.x, c = t & 63, rs = t >> 6;
+  const uint32_t w = f.w0 + blockIdx.y, chunk = blockIdx.x;
+  const uint32_t wstart = w * f.blocksize;
+  const uint32_t wend = min(wstart + f.blocksize, f.nsnps);
+  const uint32_t cb = wstart + chunk * kFinRows;
+  double acc2 = 0.0;
+  if (c < f.l) {
+    for (uint32_t q = rs; q < kFinRows; q += 4) {
+      const uint32_t r = cb + q;
+      if (r >= wend) break;
+      const double gp = f.d[r] * f.G[(uint64_t)r * f.l + c];
+      const double g2 = 2.0 * f.F[r] * gp;
+      f.g2[(uint64_t)r * f.l + c] = g2;
+      acc2 += g2;
+      const uint32_t k = r - wstart;
+      const uint64_t o = ((uint64_t)w * f.bcols + c) * f.kw + k;
+      if (f.bprec == PREC_FP64) {
+        reinterpret_cast<double*>(f.bh)[o] = gp;
+      } else if (f.bprec == PREC_FP32) {
+        reinterpret_cast<float*>(f.bh)[o] = (float)gp;
+      } else {
+        const __bf16 hi = (__bf16)(float)gp;
+        const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
+        __bf16* bh = reinterpret_cast<__bf16*>(f.bh);
+        bh[o] = hi;
+        bh[((uint64_t)w * f.bcols + f.lo_off + c) * f.kw + k] = lo;
+      }
+    }
   }
+  sh[rs][c] = acc2;
+  __syncthreads();
+  if (t < f.l)
+    f.s2p[((uint64_t)w * gridDim.x + chunk) * f.l + t] = (sh[0][t] + sh[1][t]) + (sh[2][t] + sh[3][t]);
 }
 
-void launch_window_colsum(const double* g2, uint32_t l, uint32_t nsnps, uint32_t blocksize, uint32_t w0,
-                          uint32_t w1, double* S2, hipStream_t s) {
-  PCAONE_REQUIRE(l >= 1 && l <= 256, -1, "l out of range");
+void launch_operand_from_G(const FinalizeG& f, hipStream_t s) {
+  PCAONE_REQUIRE(f.l >= 1 && f.l <= 64, -1, "operand: l must be in [1, 64]");
+  if (f.w1 <= f.w0) return;
+  dim3 grid((uint32_t)ceil_div(f.blocksize, kFinRows), f.w1 - f.w0);
+  hipLaunchKernelGGL(k_operand_from_G, grid, dim3(256), 0, s, f);
+}
+
+// S2[w][c] = sum of the chunk partials of window w (fixed order)
+__global__ void k_window_colsum(const double* __restrict__ s2p, uint32_t l, uint32_t nchunk, uint32_t w0,
+                                double* __restrict__ S2) {
+  const uint32_t w = w0 + blockIdx.x, c = threadIdx.x;
+  if (c >= l) return;
+  double tot = 0.0;
+  for (uint32_t q = 0; q < nchunk; ++q) tot += s2p[((uint64_t)w * nchunk + q) * l + c];
+  S2[(uint64_t)w * l + c] = tot;
+}
+
+void launch_window_colsum(const double* s2p, uint32_t l, uint32_t blocksize, uint32_t w0, uint32_t w1, double* S2,
+                          hipStream_t s) {
+  PCAONE_REQUIRE(l >= 1 && l <= 64, -1, "l out of range");
   if (w1 <= w0) return;
-  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0), dim3(256), 0, s, g2, l, nsnps, blocksize, w0, S2);
+  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0), dim3(64), 0, s, s2p, l, finalize_chunks(blocksize), w0, S2);
 }
 
 __global__ void k_finalize_H(FinalizeH f) {

@@ -18,8 +18,24 @@ constexpr int kTile = 32;  // rows staged per LDS tile
 constexpr int kMaxL = 64;  // largest l the single-workgroup kernels hold in LDS
 
 int gram_blocks(uint64_t nrows) {
-  const uint64_t b = ceil_div(nrows, 512);
-  return (int)std::min<uint64_t>(std::max<uint64_t>(b, 1), 256);
+  const uint64_t b = ceil_div(nrows, 64);
+  return (int)std::min<uint64_t>(std::max<uint64_t>(b, 1), 1024);
+}
+
+// part[e] = sum_p part[p * n + e], in place, fixed order (deterministic)
+__global__ void k_reduce_parts(double* part, int nparts, uint32_t n) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = 0.0;
+  for (int p = 0; p < nparts; ++p) s += part[(uint64_t)p * n + e];
+  part[e] = s;
+}
+
+static int reduce_parts(const double* part, int nparts, uint32_t n, hipStream_t s) {
+  if (nparts <= 1) return nparts;
+  hipLaunchKernelGGL(k_reduce_parts, dim3((uint32_t)ceil_div(n, 256)), dim3(256), 0, s, const_cast<double*>(part),
+                     nparts, n);
+  return 1;
 }
 
 // acc[e] += sum_r X[r][i(e)] * Y[r][j(e)] for the entries e = t + 256 q owned
@@ -141,52 +157,56 @@ __device__ void chol_upper_lds(double* a, int l, int* bad) {
 
 // Householder QR of S (rows x l, LDS, row-major) keeping only the sign of each
 // R diagonal: beta_k = -sign(alpha_k) ||x_k|| as in LAPACK dlarfg and Eigen's
-// makeHouseholder (zero tail: beta_k = alpha_k).
+// makeHouseholder (zero tail: beta_k = alpha_k).  Wave 0 forms the tail norm,
+// then thread j applies the reflection to column j (blockDim >= l).
 __device__ void householder_signs_lds(double* S, int rows, int l, double* red, double* sign) {
   const int t = threadIdx.x;
   for (int k = 0; k < l; ++k) {
-    // ||x_k[k+1:]||^2
-    double p = 0.0;
-    for (int r = k + 1 + t; r < rows; r += 256) p += S[r * l + k] * S[r * l + k];
-    red[t] = p;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if (t < off) red[t] += red[t + off];
-      __syncthreads();
-    }
-    const double tail = red[0];
-    const double alpha = S[k * l + k];
-    __syncthreads();
-    double beta, tau;
-    if (tail <= 2.2250738585072014e-308) {
-      beta = alpha;
-      tau = 0.0;
-    } else {
-      beta = sqrt(alpha * alpha + tail);
-      if (alpha >= 0.0) beta = -beta;
-      tau = (beta - alpha) / beta;
-    }
-    if (t == 0) sign[k] = (beta < 0.0) ? -1.0 : 1.0;
-    if (tau != 0.0) {
-      const double inv = 1.0 / (alpha - beta);
-      // u = [1; x_tail * inv]; for columns j > k: w = u^T S[:, j]; S[:, j] -= tau u w
-      for (int j = k + 1; j < l; ++j) {
-        double q = 0.0;
-        for (int r = k + 1 + t; r < rows; r += 256) q += S[r * l + k] * inv * S[r * l + j];
-        red[t] = q;
-        __syncthreads();
-        for (int off = 128; off > 0; off >>= 1) {
-          if (t < off) red[t] += red[t + off];
-          __syncthreads();
+    if (t < 64) {
+      double p = 0.0;
+      for (int r = k + 1 + t; r < rows; r += 64) p += S[r * l + k] * S[r * l + k];
+      for (int off = 32; off > 0; off >>= 1) p += __shfl_down(p, off);
+      if (t == 0) {
+        const double alpha = S[k * l + k];
+        double beta, tau = 0.0, inv = 0.0;
+        if (p <= 2.2250738585072014e-308) {
+          beta = alpha;
+        } else {
+          beta = sqrt(alpha * alpha + p);
+          if (alpha >= 0.0) beta = -beta;
+          tau = (beta - alpha) / beta;
+          inv = 1.0 / (alpha - beta);
         }
-        const double w = S[k * l + j] + red[0];
-        __syncthreads();
-        for (int r = k + t; r < rows; r += 256) {
-          const double u = (r == k) ? 1.0 : S[r * l + k] * inv;
-          S[r * l + j] -= tau * u * w;
-        }
-        __syncthreads();
+        sign[k] = (beta < 0.0) ? -1.0 : 1.0;
+        red[0] = tau;
+        red[1] = inv;
       }
+    }
+    __syncthreads();
+    const double tau = red[0], inv = red[1];
+    if (tau != 0.0 && t > k && t < l) {
+      // u = [1; x_tail * inv]: w = u^T S[:, j]; S[:, j] -= tau u w
+      const int j = t;
+      double w = S[k * l + j];
+      for (int r = k + 1; r < rows; ++r) w += S[r * l + k] * inv * S[r * l + j];
+      S[k * l + j] -= tau * w;
+      for (int r = k + 1; r < rows; ++r) S[r * l + j] -= tau * (S[r * l + k] * inv) * w;
+    }
+    __syncthreads();
+  }
+}
+
+// X = R^-1 (upper) in LDS, rows solved bottom-up, one column per thread.
+__device__ void upper_inverse_lds(const double* a, double* X, int l) {
+  const int t = threadIdx.x;
+  for (int e = t; e < l * l; e += 256) X[e] = (e / l == e % l) ? 1.0 / a[e] : 0.0;
+  __syncthreads();
+  for (int i = l - 2; i >= 0; --i) {
+    if (t > i && t < l) {
+      const int j = t;
+      double s = 0.0;
+      for (int p = i + 1; p <= j; ++p) s += a[i * l + p] * X[p * l + j];
+      X[i * l + j] = -s / a[i * l + i];
     }
     __syncthreads();
   }
@@ -200,7 +220,7 @@ __global__ __launch_bounds__(256) void k_chol(const double* __restrict__ part, i
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   double* a = reinterpret_cast<double*>(smem_raw);  // l*l
   double* S = a + l * l;                             // 2l*l
-  __shared__ double red[256];
+  __shared__ double red[4];
   __shared__ int bad;
   const int t = threadIdx.x;
   const int L = (int)l;
@@ -239,29 +259,19 @@ __global__ __launch_bounds__(256) void k_chol(const double* __restrict__ part, i
     __syncthreads();
   }
   chol_upper_lds(a, L, &bad);
-  for (int e = t; e < L * L; e += 256) R[e] = a[e];
-  // R^-1, one column per thread (upper triangular)
-  if (t < L) {
-    const int j = t;
-    double x[kMaxL];
-    for (int i = 0; i < L; ++i) x[i] = 0.0;
-    x[j] = 1.0 / a[j * L + j];
-    for (int i = j - 1; i >= 0; --i) {
-      double s = 0.0;
-      for (int p = i + 1; p <= j; ++p) s += a[i * L + p] * x[p];
-      x[i] = -s / a[i * L + i];
-    }
-    for (int i = 0; i < L; ++i) Rinv[i * L + j] = x[i];
+  double* X = S;  // R^-1
+  upper_inverse_lds(a, X, L);
+  for (int e = t; e < L * L; e += 256) {
+    R[e] = a[e];
+    Rinv[e] = X[e];
   }
-  __syncthreads();
   if (Rprev) {
-    __threadfence_block();
     for (int e = t; e < L * L; e += 256) {
       const int i = e / L, j = e % L;
       double v = 0.0, w = 0.0;
       for (int p = 0; p < L; ++p) {
         v += a[i * L + p] * Rprev[p * L + j];
-        w += Rprev_inv[i * L + p] * Rinv[p * L + j];
+        w += Rprev_inv[i * L + p] * X[p * L + j];
       }
       Rtot[e] = v;
       Rtot_inv[e] = w;
@@ -274,6 +284,7 @@ void launch_chol(const double* part, int nparts, const double* A, uint32_t row0,
                  double* Rinv, const double* Rprev, const double* Rprev_inv, double* Rtot, double* Rtot_inv,
                  double* sign, int* status, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
+  nparts = reduce_parts(part, nparts, l * l, s);
   const size_t shm = (size_t)3 * l * l * sizeof(double);
   hipLaunchKernelGGL(k_chol, dim3(1), dim3(256), shm, s, part, nparts, A, row0, l, R, Rinv, Rprev, Rprev_inv,
                      Rtot, Rtot_inv, sign, status);
@@ -450,7 +461,7 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
     v[e] = (e / L == e % L) ? 1.0 : 0.0;
   }
   __syncthreads();
-  for (int sweep = 0; sweep < 60; ++sweep) {
+  for (int sweep = 0; sweep < 40; ++sweep) {
     // convergence check
     double off = 0.0, dg = 0.0;
     for (int e = t; e < L * L; e += 256) {
@@ -473,7 +484,8 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
     }
     const double dgs = red[0];
     __syncthreads();
-    if (offs <= 1e-32 * dgs || offs == 0.0) break;
+    // off-diagonal Frobenius norm below 1e-13 of the diagonal's: FP64 rounding level
+    if (offs <= 1e-26 * dgs || offs == 0.0) break;
     for (int round = 0; round < n - 1; ++round) {
       if (t == 0) {
         pos[0] = 0;
@@ -557,6 +569,7 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
 void launch_sym_eig(const double* part, int nparts, uint32_t l, double* evals, double* W, double* sinv,
                     uint32_t kc, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
+  nparts = reduce_parts(part, nparts, l * l, s);
   hipLaunchKernelGGL(k_sym_eig, dim3(1), dim3(256), 0, s, part, nparts, l, evals, W, sinv, kc);
 }
 
@@ -625,6 +638,7 @@ __global__ __launch_bounds__(256) void k_mev_finish(const double* __restrict__ p
 }
 
 void launch_mev_finish(const double* part, int nparts, uint32_t k, double* out, hipStream_t s) {
+  nparts = reduce_parts(part, nparts, k * k, s);
   hipLaunchKernelGGL(k_mev_finish, dim3(1), dim3(256), 0, s, part, nparts, k, out);
 }
 

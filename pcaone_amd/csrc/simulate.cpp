@@ -119,6 +119,83 @@ bool gen_row(uint64_t seed, uint32_t snp, uint32_t attempt, uint32_t n, uint32_t
 
 }  // namespace
 
+namespace {
+
+// Poisson(lambda) by inversion for small lambda, normal approximation with
+// continuity correction above 64 (never reached by the count model below at
+// the sizes used, kept for completeness).
+uint32_t poisson(Xoshiro256& rng, double lam) {
+  if (lam <= 0) return 0;
+  if (lam > 64) {
+    const double x = std::floor(lam + std::sqrt(lam) * rng.normal() + 0.5);
+    return x < 0 ? 0u : (uint32_t)x;
+  }
+  const double L = std::exp(-lam);
+  double p = 1.0;
+  uint32_t k = 0;
+  do {
+    ++k;
+    p *= rng.uniform();
+  } while (p > L);
+  return k - 1;
+}
+
+}  // namespace
+
+// scRNA-like dense matrix, feature-major (ngenes rows of ncells float32):
+// gene means m_g = base * LogNormal(0, sdlog), cell size factors
+// s_c = LogNormal(0, 0.3), counts ~ NegBinomial(mean m_g s_c, dispersion phi)
+// (Gamma-Poisson), value = log1p(1e6 * count / library size of the cell).
+extern "C" int pcaone_sim_counts(uint64_t seed, uint32_t ncells, uint32_t ngenes, double base, double sdlog,
+                                 double phi, float* out, int nthreads) {
+  if (!out || ncells == 0 || ngenes == 0 || !(base > 0) || !(phi > 0)) return -1;
+  unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  unsigned nt = nthreads > 0 ? (unsigned)nthreads : std::min(hw, 16u);
+  nt = std::max(1u, std::min<unsigned>(nt, ngenes));
+  std::vector<double> sc(ncells);
+  {
+    Xoshiro256 rng(snp_seed(seed, 0xFFFFFFFFull, 7));
+    for (auto& v : sc) v = std::exp(0.3 * rng.normal());
+  }
+  std::vector<std::vector<double>> lib(nt, std::vector<double>(ncells, 0.0));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; ++t) {
+    pool.emplace_back([&, t]() {
+      const uint64_t lo = (uint64_t)ngenes * t / nt, hi = (uint64_t)ngenes * (t + 1) / nt;
+      for (uint64_t g = lo; g < hi; ++g) {
+        Xoshiro256 rng(snp_seed(seed, g, 0));
+        const double mg = base * std::exp(sdlog * rng.normal());
+        const double shape = 1.0 / phi;
+        float* row = out + g * ncells;
+        for (uint32_t c = 0; c < ncells; ++c) {
+          const double mean = mg * sc[c];
+          const double lam = rng.gamma(shape) * mean / shape;
+          const uint32_t k = poisson(rng, lam);
+          row[c] = (float)k;
+          lib[t][c] += k;
+        }
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  std::vector<double> L(ncells, 0.0);
+  for (unsigned t = 0; t < nt; ++t)
+    for (uint32_t c = 0; c < ncells; ++c) L[c] += lib[t][c];
+  pool.clear();
+  for (unsigned t = 0; t < nt; ++t) {
+    pool.emplace_back([&, t]() {
+      const uint64_t lo = (uint64_t)ngenes * t / nt, hi = (uint64_t)ngenes * (t + 1) / nt;
+      for (uint64_t g = lo; g < hi; ++g) {
+        float* row = out + g * ncells;
+        for (uint32_t c = 0; c < ncells; ++c)
+          row[c] = (float)std::log1p(L[c] > 0 ? 1e6 * (double)row[c] / L[c] : 0.0);
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  return 0;
+}
+
 extern "C" int pcaone_sim_bed(uint64_t seed, uint32_t nsamples, uint32_t nsnps, uint32_t npops, double fst,
                               double pmin, double pmax, double miss_rate, uint8_t* payload, double* p_anc,
                               int nthreads) {

@@ -40,6 +40,14 @@ def oracle_matrix(payload, n, m, standardize=True):
 def oracle_run(z: dict):
     d = parse_args(z["args"])
     n, m = int(z["nsamples"]), int(z["nsnps"])
+    if "dense" in z:
+        X = O.dense_centered(z["dense"])
+        if d["svd"] == 0:  # IRAM: the converged top-k singular triplets
+            u, s, vt = np.linalg.svd(X, full_matrices=False)
+            return O.ExactSvd(u[:, :d["k"]], s[:d["k"]], vt[:d["k"]].T, m), None, d
+        orc = O.RsvdOracle(X, d["k"], d["os"], fancy=d["svd"] == 2, bands=d["w"], seed=d["seed"], rand=d["rand"],
+                           perm=not d["S"])
+        return orc, orc.compute_usv(d["p"], d["tol"]), d
     if d["emu"]:
         codes = O.decode_codes(z["payload"], n, m)
         orc, _, _ = O.emu_halko(codes, d["k"], d["os"], d["p"], d["tol"], seed=d["seed"], rand=d["rand"])

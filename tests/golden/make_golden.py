@@ -49,9 +49,17 @@ CASES = {
     "emu_svd1": ("d3", "--svd 1 -m 0.0005 --emu -k 3 --oversamples 7 -p 5 --tol-rsvd 0"),
 }
 
+# dense float32 .bin inputs (scRNA-like log1p CPM): name -> (seed, ncells, ngenes, base)
+DENSE = {"b1": (21, 300, 410, 0.035)}
+DENSE_CASES = {
+    "bin_svd1_p5": ("b1", "--svd 1 -k 5 --oversamples 5 -p 5 --tol-rsvd 0"),
+    "bin_svd2_w8": ("b1", "--svd 2 -k 5 --oversamples 5 -w 8"),
+    "bin_svd0": ("b1", "--svd 0 -k 5"),
+}
 
-def run_case(tmp: Path, prefix: str, args: str, out: str):
-    cmd = [str(REF), "-b", prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v", "0"]
+
+def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
+    cmd = [str(REF), flag, prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v", "0"]
     subprocess.run(cmd, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     res = {}
     res["eigvals"] = np.loadtxt(tmp / (out + ".eigvals"), ndmin=1)
@@ -78,6 +86,15 @@ def main():
             res = run_case(tmp, ds, args, case)
             np.savez_compressed(OUT / f"{case}.npz", payload=pay, nsamples=n, nsnps=m, args=args, **res)
             print(case, res["eigvals"])
+        for name, (seed, n, m, base) in DENSE.items():
+            D = pa.sim_counts(seed, n, m, base=base)
+            pa.write_bin(tmp / (name + ".bin"), D)
+            for case, (ds, args) in DENSE_CASES.items():
+                if ds != name:
+                    continue
+                res = run_case(tmp, name + ".bin", args, case, flag="-B")
+                np.savez_compressed(OUT / f"{case}.npz", dense=D, nsamples=n, nsnps=m, args=args, **res)
+                print(case, res["eigvals"])
 
 
 if __name__ == "__main__":
