@@ -162,6 +162,12 @@ int pcaone_gpu_synchronize(pcaone_gpu* h);
 int pcaone_sim_bed(uint64_t seed, uint32_t nsamples, uint32_t nsnps, uint32_t npops, double fst, double pmin,
                    double pmax, double miss_rate, uint8_t* payload, double* p_anc, int nthreads);
 
+/* Genotype likelihoods (host only), BEAGLE order: nsites x nsamples x 3
+ * float32 (GL of 0, 1, 2 copies of the modelled allele).  Genotypes as in
+ * pcaone_sim_bed, depth ~ Poisson(depth), per-read error err; each GL triple
+ * is normalised to sum 1 (depth 0 gives 1/3 each). */
+int pcaone_sim_gl(uint64_t seed, uint32_t nsamples, uint32_t nsites, uint32_t npops, double fst, double pmin,
+                  double pmax, double depth, double err, float* out, int nthreads);
 /* scRNA-like dense matrix (host only): ngenes rows of ncells float32 in
  * `out`.  Gene means base * LogNormal(0, sdlog), cell size factors
  * LogNormal(0, 0.3), counts ~ NegBinomial(mean, dispersion phi) drawn as

@@ -473,6 +473,87 @@ def emu_halko(codes: np.ndarray, k: int, os: int, maxp: int, tol: float, maxiter
     return op, it, diffs
 
 
+# ----------------------------------------------------------------------------
+# genotype likelihoods: emMAF_with_GL and PCAngsd (BEAGLE input)
+# ----------------------------------------------------------------------------
+def _posterior(P0, P1, pt):
+    """(p1 + 2 p2) / (p0 + p1 + p2) with the third GL = 1 - P0 - P1
+    (src/Utils.cpp:511-515, src/Data.cpp:262-267)."""
+    p0 = P0 * (1.0 - pt) * (1.0 - pt)
+    p1 = P1 * 2.0 * pt * (1.0 - pt)
+    p2 = (1.0 - P0 - P1) * pt * pt
+    return (p1 + 2.0 * p2) / (p0 + p1 + p2), p0, p1, p2
+
+
+def em_maf_with_gl(P0: np.ndarray, P1: np.ndarray, maxiter: int, tolmaf: float, f0: float = 0.25):
+    """emMAF_with_GL (src/Utils.cpp:501-531) from F = 0.25 (src/FileBeagle.cpp:127).
+    P0, P1: (N, M) -- the first two GLs of each sample and site.  Returns
+    (F, iterations)."""
+    n, m = P0.shape
+    F = np.full(m, f0)
+    it = 0
+    for it in range(1, maxiter + 1):
+        Ft = F.copy()
+        post, *_ = _posterior(P0, P1, F[None, :])
+        F = post.sum(axis=0) * (1.0 / (2.0 * n))
+        if math.sqrt(((F - Ft) ** 2).sum() / m) < tolmaf:
+            break
+    return F, it
+
+
+def gl_initial_E(P0, P1, F):
+    """Initial E of FileBeagle::read_all (src/FileBeagle.cpp:133-143): posterior
+    mean genotype at pt = F minus 2F; (N, M)."""
+    post, *_ = _posterior(P0, P1, F[None, :])
+    return post - 2.0 * F[None, :]
+
+
+def pcangsd_E(P0, P1, F, U, S, V):
+    """fit_with_pi, PCAngsd branch (src/Data.cpp:249-270)."""
+    pt = ((U * S) @ V.T + 2.0 * F[None, :]) / 2.0
+    pt = np.minimum(np.maximum(pt, 1e-4), 1.0 - 1e-4)
+    post, *_ = _posterior(P0, P1, pt)
+    return post - 2.0 * F[None, :]
+
+
+def pcangsd_standardize(P0, P1, F, U, S, V):
+    """pcangsd_standardize_E (src/Data.cpp:301-343): (E / sqrt(2F(1-F)), Dc)."""
+    pt = ((U * S) @ V.T + 2.0 * F[None, :]) / 2.0
+    pt = np.minimum(np.maximum(pt, 1e-4), 1.0 - 1e-4)
+    post, p0, p1, p2 = _posterior(P0, P1, pt)
+    E = post - 2.0 * F[None, :]
+    norm = np.sqrt(2.0 * F * (1.0 - F))
+    E = np.where(norm[None, :] > VAR_TOL, E / norm[None, :], E)
+    ps = p0 + p1 + p2
+    f2 = 2.0 * F[None, :]
+    tmp = (0.0 - f2) ** 2 * (p0 / ps) + (1.0 - f2) ** 2 * (p1 / ps) + (2.0 - f2) ** 2 * (p2 / ps)
+    Dc = (tmp / (2.0 * F * (1.0 - F))[None, :]).sum(axis=1)
+    return E, Dc
+
+
+def pcangsd_halko(P0, P1, k: int, os: int, maxp: int, tol: float, maxiter: int = 100, tolem: float = 1e-5,
+                  tolmaf: float = 1e-6, seed: int = 101, rand: int = 1):
+    """run_pca_with_halko's EM branch for PCAngsd on BEAGLE input with the plain
+    RSVD (src/Halko.cpp:294-331): returns (op of the last solve, F, EM
+    iterations, diffs, Dc, GRM)."""
+    n, m = P0.shape
+    F, _ = em_maf_with_gl(P0, P1, maxiter, tolmaf)
+    op = RsvdOracle(gl_initial_E(P0, P1, F), k, os, fancy=False, seed=seed, rand=rand)
+    op.compute_usv(maxp, tol)
+    diffs, it = [], 0
+    for it in range(1, maxiter + 1):
+        vpre = op.V.copy()
+        op.X = pcangsd_E(P0, P1, F, op.U, op.S, op.V)
+        op.compute_usv(maxp, tol)
+        diffs.append(1.0 - mev(op.V, vpre))
+        if diffs[-1] < tolem:
+            break
+    E, Dc = pcangsd_standardize(P0, P1, F, op.U, op.S, op.V)
+    C = E @ E.T / m
+    C[np.diag_indices(n)] = Dc / m
+    return op, F, it, diffs, Dc, C
+
+
 def principal_angles_max(A: np.ndarray, B: np.ndarray) -> float:
     """Largest principal angle (radians) between span(A) and span(B)."""
     qa, _ = np.linalg.qr(A)

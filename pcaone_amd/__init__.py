@@ -72,6 +72,8 @@ SIGNATURES = {
                                  C.c_double, C.c_double, _U8, _D, C.c_int]),
     "pcaone_sim_counts": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                     C.POINTER(C.c_float), C.c_int]),
+    "pcaone_sim_gl": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                C.c_double, C.c_double, C.c_double, C.POINTER(C.c_float), C.c_int]),
 }
 
 
@@ -118,6 +120,30 @@ def sim_counts(seed: int, ncells: int, ngenes: int, base: float = 0.05, sdlog: f
     if rc != 0:
         raise GpuError(rc, "pcaone_sim_counts failed")
     return out
+
+
+def sim_gl(seed: int, nsamples: int, nsites: int, npops: int = 4, fst: float = 0.1, pmin: float = 0.05,
+           pmax: float = 0.5, depth: float = 2.0, err: float = 0.01, nthreads: int = 0) -> np.ndarray:
+    """Genotype likelihoods, (nsites, nsamples, 3) float32 (host only)."""
+    out = np.empty((nsites, nsamples, 3), dtype=np.float32)
+    rc = lib().pcaone_sim_gl(seed, nsamples, nsites, npops, fst, pmin, pmax, depth, err, _p(out, C.c_float),
+                             nthreads)
+    if rc != 0:
+        raise GpuError(rc, "pcaone_sim_gl failed")
+    return out
+
+
+def write_beagle(path: str | os.PathLike, gl: np.ndarray):
+    """BEAGLE text (gzip) with exact float32 values: the reference parses each
+    field with strtod (src/Utils.cpp:377-405)."""
+    import gzip
+
+    m, n, _ = gl.shape
+    with gzip.open(path, "wt") as f:
+        f.write("marker\tallele1\tallele2\t" + "\t".join(f"Ind{j}\tInd{j}\tInd{j}" for j in range(n)) + "\n")
+        for s in range(m):
+            vals = "\t".join(repr(float(v)) for v in gl[s].reshape(-1))
+            f.write(f"1_{s + 1}\t0\t1\t{vals}\n")
 
 
 def write_bin(path: str | os.PathLike, featmajor: np.ndarray):

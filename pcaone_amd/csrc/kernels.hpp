@@ -177,6 +177,31 @@ void launch_dense_rows(const float* raw, uint32_t nsamples, uint32_t nfeat, cons
 void launch_dense_mean(const float* raw, uint32_t nsamples, uint32_t nfeat, double* mu, hipStream_t s);
 void launch_dense_transpose(const float* rows, uint32_t ldr, uint32_t nfeat, uint32_t bs, uint32_t nwin,
                             uint32_t nsamples, float* dt, uint32_t ldt, hipStream_t s);
+
+// ---- genotype likelihoods, PCAngsd (src/FileBeagle.cpp, src/Data.cpp:249-343)
+// P: [site][sample][2] FP32, the first two of each sample's three GLs.
+void launch_gl_em_maf(const float* P, uint32_t nsamples, uint32_t nsites, const double* Fin, double* Fout,
+                      double* dpart, hipStream_t s);
+uint32_t gl_em_maf_blocks(uint32_t nsites);
+struct GlE {
+  const float* P;
+  uint32_t nsamples, nsites;
+  const int32_t* perm;   // laid-out row -> site (null: identity)
+  const double* F;       // per site (original order)
+  const double* U;       // [nsamples][k] or null (initial E, prior = F)
+  const double* S;       // [k]
+  const double* V;       // [nsites][k], laid-out rows
+  uint32_t k;
+  int standardize;       // divide by sqrt(2F(1-F)) and emit Dc partials
+  float* out;            // [laid-out row][ld]
+  uint64_t ld;
+  uint32_t rows_per_block;
+  double* dpart;         // [gridDim.y][nsamples] Dc partials, or null
+};
+void launch_gl_E(const GlE& e, hipStream_t s);
+// GRM C = E^T E / nsites with diagonal Dc / nsites (E rows = sites, ld stride)
+void launch_grm(const float* E, uint64_t ld, uint32_t nsamples, uint32_t nsites, const double* Dc, double* C,
+                hipStream_t s);
 // ntiles: 16-column tiles of the output (FP32 arm) or of each half (split arm)
 void launch_dense_gemm(const DenseGemm& g, int arm, int ncols, int ntiles, hipStream_t s);
 

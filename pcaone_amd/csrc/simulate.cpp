@@ -196,6 +196,56 @@ extern "C" int pcaone_sim_counts(uint64_t seed, uint32_t ncells, uint32_t ngenes
   return 0;
 }
 
+// Genotype likelihoods (BEAGLE order, 3 per sample and site) from
+// Balding-Nichols genotypes: depth ~ Poisson(depth), each read carries one of
+// the genotype's two alleles, flipped with probability err; GL(g) =
+// prod_reads P(read | g), normalised to sum 1 (depth 0 gives 1/3 each).
+// out: nsites x nsamples x 3 float32; g counts copies of the allele whose
+// frequency is modelled (the third GL is that homozygote).
+extern "C" int pcaone_sim_gl(uint64_t seed, uint32_t nsamples, uint32_t nsites, uint32_t npops, double fst,
+                             double pmin, double pmax, double depth, double err, float* out, int nthreads) {
+  if (!out || nsamples == 0 || nsites == 0 || npops == 0 || npops > nsamples) return -1;
+  if (!(fst >= 0 && fst < 1) || !(pmin > 0 && pmax < 1 && pmin <= pmax) || !(depth >= 0) || !(err >= 0 && err < 0.5))
+    return -1;
+  unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  unsigned nt = nthreads > 0 ? (unsigned)nthreads : std::min(hw, 16u);
+  nt = std::max(1u, std::min<unsigned>(nt, nsites));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; ++t) {
+    pool.emplace_back([&, t]() {
+      const uint64_t lo = (uint64_t)nsites * t / nt, hi = (uint64_t)nsites * (t + 1) / nt;
+      std::vector<double> pk(npops);
+      for (uint64_t s = lo; s < hi; ++s) {
+        Xoshiro256 rng(snp_seed(seed ^ 0x5851F42D4C957F2Dull, s, 0));
+        const double p = pmin + (pmax - pmin) * rng.uniform();
+        for (uint32_t k = 0; k < npops; ++k) {
+          pk[k] = p;
+          if (fst > 0) pk[k] = rng.beta(p * (1.0 - fst) / fst, (1.0 - p) * (1.0 - fst) / fst);
+        }
+        float* row = out + s * (uint64_t)nsamples * 3;
+        for (uint32_t j = 0; j < nsamples; ++j) {
+          const double f = pk[(uint64_t)j * npops / nsamples];
+          const int g = (rng.uniform() < f) + (rng.uniform() < f);
+          const uint32_t d = poisson(rng, depth);
+          double L[3] = {1.0, 1.0, 1.0};
+          for (uint32_t r = 0; r < d; ++r) {
+            int b = rng.uniform() < g / 2.0;  // allele carried by the read
+            if (rng.uniform() < err) b ^= 1;
+            for (int gg = 0; gg < 3; ++gg) {
+              const double q = gg / 2.0;  // fraction of the modelled allele
+              L[gg] *= b ? q * (1 - err) + (1 - q) * err : (1 - q) * (1 - err) + q * err;
+            }
+          }
+          const double sum = L[0] + L[1] + L[2];
+          for (int gg = 0; gg < 3; ++gg) row[3 * j + gg] = (float)(L[gg] / sum);
+        }
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  return 0;
+}
+
 extern "C" int pcaone_sim_bed(uint64_t seed, uint32_t nsamples, uint32_t nsnps, uint32_t npops, double fst,
                               double pmin, double pmax, double miss_rate, uint8_t* payload, double* p_anc,
                               int nthreads) {

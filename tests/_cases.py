@@ -40,6 +40,10 @@ def oracle_matrix(payload, n, m, standardize=True):
 def oracle_run(z: dict):
     d = parse_args(z["args"])
     n, m = int(z["nsamples"]), int(z["nsnps"])
+    if "gl" in z:  # PCAngsd on BEAGLE likelihoods: the op of the last EM solve
+        P0, P1 = gl_planes(z["gl"])
+        orc, *_ = O.pcangsd_halko(P0, P1, d["k"], d["os"], d["p"], d["tol"], seed=d["seed"], rand=d["rand"])
+        return orc, None, d
     if "dense" in z:
         X = O.dense_centered(z["dense"])
         if d["svd"] == 0:  # IRAM: the converged top-k singular triplets
@@ -57,6 +61,13 @@ def oracle_run(z: dict):
                        perm=not d["S"])
     ep = orc.compute_usv(d["p"], d["tol"])
     return orc, ep, d
+
+
+def gl_planes(gl):
+    """(nsites, nsamples, 3) likelihoods -> the (nsamples, nsites) planes P0, P1
+    the reference keeps (src/FileBeagle.cpp:99-100)."""
+    return np.ascontiguousarray(gl[:, :, 0].T, dtype=np.float64), np.ascontiguousarray(gl[:, :, 1].T,
+                                                                                        dtype=np.float64)
 
 
 def eig_relerr(a, b) -> float:

@@ -51,6 +51,12 @@ CASES = {
 
 # dense float32 .bin inputs (scRNA-like log1p CPM): name -> (seed, ncells, ngenes, base)
 DENSE = {"b1": (21, 300, 410, 0.035)}
+# genotype likelihoods (BEAGLE): name -> (seed, nsamples, nsites, npops, fst, depth, err)
+GL = {"g1": (31, 120, 700, 4, 0.1, 2.0, 0.01)}
+GL_CASES = {
+    # PCAngsd EM; --svd 1 keeps the reference's in-core indices consistent
+    "gl_pcangsd_svd1": ("g1", "--svd 1 -k 3 --oversamples 7 -p 5 --tol-rsvd 0"),
+}
 DENSE_CASES = {
     "bin_svd1_p5": ("b1", "--svd 1 -k 5 --oversamples 5 -p 5 --tol-rsvd 0"),
     "bin_svd2_w8": ("b1", "--svd 2 -k 5 --oversamples 5 -w 8"),
@@ -86,6 +92,16 @@ def main():
             res = run_case(tmp, ds, args, case)
             np.savez_compressed(OUT / f"{case}.npz", payload=pay, nsamples=n, nsnps=m, args=args, **res)
             print(case, res["eigvals"])
+        for name, (seed, n, m, npops, fst, depth, err) in GL.items():
+            gl = pa.sim_gl(seed, n, m, npops, fst, 0.05, 0.5, depth, err)
+            pa.write_beagle(tmp / (name + ".beagle.gz"), gl)
+            for case, (ds, args) in GL_CASES.items():
+                if ds != name:
+                    continue
+                res = run_case(tmp, name + ".beagle.gz", args, case, flag="-G")
+                res["cov"] = np.loadtxt(tmp / (case + ".cov"))
+                np.savez_compressed(OUT / f"{case}.npz", gl=gl, nsamples=n, nsnps=m, args=args, **res)
+                print(case, res["eigvals"])
         for name, (seed, n, m, base) in DENSE.items():
             D = pa.sim_counts(seed, n, m, base=base)
             pa.write_bin(tmp / (name + ".bin"), D)

@@ -176,3 +176,69 @@ def test_fancy_schedule_closed_forms(bands):
             assert [a for _, a, _, _ in steps] == [1] * (bands // 2) + [2] * (bands // 2)
         if 0 < pi < lb:
             assert upd[0] == 2 ** (pi - 1) - 1  # the "adjacent" refresh
+
+
+# --- genotype likelihoods (PCAngsd on BEAGLE input) --------------------------
+
+def test_em_maf_with_gl_closed_forms():
+    rng = np.random.default_rng(3)
+    n, m = 40, 25
+    # certain genotypes: the posterior is the genotype whatever the prior, so
+    # one EM step lands on the dosage mean and the second step confirms it
+    g = rng.integers(0, 3, size=(n, m))
+    P0, P1 = (g == 0).astype(float), (g == 1).astype(float)
+    F, it = O.em_maf_with_gl(P0, P1, 100, 1e-12)
+    assert np.allclose(F, g.mean(axis=0) / 2.0, rtol=0, atol=1e-15) and it == 2
+    # flat likelihoods carry no information: F stays at its start value 0.25
+    flat = np.full((n, m), 1.0 / 3.0)
+    F, it = O.em_maf_with_gl(flat, flat, 100, 1e-12)
+    assert np.allclose(F, 0.25, rtol=0, atol=1e-15) and it == 1
+    # the EM fixed point: F = mean posterior(F) / 2 at convergence
+    P = rng.dirichlet([1.0, 1.0, 1.0], size=(n, m))
+    F, _ = O.em_maf_with_gl(P[..., 0], P[..., 1], 500, 1e-14)
+    post, *_ = O._posterior(P[..., 0], P[..., 1], F[None, :])
+    assert np.allclose(post.mean(axis=0) / 2.0, F, rtol=0, atol=1e-12)
+    # initial E is centred on 2F per site at that fixed point
+    E = O.gl_initial_E(P[..., 0], P[..., 1], F)
+    assert np.allclose(E.mean(axis=0), 0.0, atol=1e-12)
+
+
+def test_pcangsd_update_closed_forms():
+    rng = np.random.default_rng(4)
+    n, m, k = 30, 20, 2
+    P = rng.dirichlet([1.0, 1.0, 1.0], size=(n, m))
+    F = rng.uniform(0.1, 0.4, m)
+    U, S, V = rng.standard_normal((n, k)), np.abs(rng.standard_normal(k)), rng.standard_normal((m, k))
+    # zero singular values: the individual frequency prior falls back to F
+    E0 = O.pcangsd_E(P[..., 0], P[..., 1], F, U, np.zeros(k), V)
+    assert np.allclose(E0, O.gl_initial_E(P[..., 0], P[..., 1], F), rtol=0, atol=1e-15)
+    # huge loadings clamp the prior to [1e-4, 1 - 1e-4]: the posterior is the
+    # brute-force one at the clamped prior
+    E = O.pcangsd_E(P[..., 0], P[..., 1], F, U, S * 1e6, V)
+    pt = np.where((U * S) @ V.T > 0, 1 - 1e-4, 1e-4)
+    p = [P[..., 0] * (1 - pt) ** 2, P[..., 1] * 2 * pt * (1 - pt), P[..., 2] * pt ** 2]
+    want = (p[1] + 2 * p[2]) / (p[0] + p[1] + p[2]) - 2 * F[None, :]
+    assert np.allclose(E, want, rtol=1e-12, atol=1e-12)
+    # the GRM diagonal term: E[(g - 2F)^2] / (2F(1-F)) under the posterior
+    Es, Dc = O.pcangsd_standardize(P[..., 0], P[..., 1], F, U, S, V)
+    ptc = np.clip(((U * S) @ V.T + 2 * F[None, :]) / 2, 1e-4, 1 - 1e-4)
+    q = np.stack([P[..., 0] * (1 - ptc) ** 2, P[..., 1] * 2 * ptc * (1 - ptc), P[..., 2] * ptc ** 2])
+    q /= q.sum(axis=0)
+    var = sum(q[gi] * (gi - 2 * F[None, :]) ** 2 for gi in range(3))
+    assert np.allclose(Dc, (var / (2 * F * (1 - F))[None, :]).sum(axis=1), rtol=1e-12)
+    mean = q[1] + 2 * q[2]
+    assert np.allclose(Es, (mean - 2 * F[None, :]) / np.sqrt(2 * F * (1 - F))[None, :], rtol=1e-12, atol=1e-12)
+
+
+def test_pcangsd_grm_matches_reference():
+    z = load_golden(ROOT / "tests" / "golden" / "gl_pcangsd_svd1.npz")
+    from _cases import gl_planes
+    P0, P1 = gl_planes(z["gl"])
+    d = parse_args(z["args"])
+    op, F, it, diffs, Dc, C = O.pcangsd_halko(P0, P1, d["k"], d["os"], d["p"], d["tol"])
+    # the reference writes the GRM with 6 significant digits
+    assert np.max(np.abs(C - z["cov"])) < 1e-5 * np.max(np.abs(z["cov"]))
+    assert np.allclose(np.diag(C), Dc / P0.shape[1])
+    # planted mistake: the GRM without the Dc diagonal differs visibly
+    E, _ = O.pcangsd_standardize(P0, P1, F, op.U, op.S, op.V)
+    assert np.max(np.abs(np.diag(E @ E.T / P0.shape[1]) - np.diag(z["cov"]))) > 1e-3
