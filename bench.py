@@ -160,7 +160,8 @@ def main():
                 "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom,
                 "launches": cnt, "avg_launch_us": round(ms * 1e3 / cnt, 2),
                 "algorithmic_flops_per_launch": flops / cnt,
-                "kernel_ms_per_solve": {k: round(v[1], 3) for k, v in sorted(stats.items())}}
+                "kernel_ms_per_solve": {k: round(v[1], 3) for k, v in sorted(stats.items())},
+                "kernel_launches_per_solve": {k: v[0] for k, v in sorted(stats.items())}}
 
     bf = arms["bf16x2"]
     t_bf = float(np.sum(bf["times"]))

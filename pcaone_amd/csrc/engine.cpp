@@ -276,7 +276,12 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   bh_.reserve((uint64_t)nwin_ * bcols_ * kw_ * opsz_);
   PCAONE_HIP(hipMemsetAsync(bh_.p, 0, bh_.bytes(), stream_));
   G_.reserve((uint64_t)M_ * l_);
-  g2_.reserve((uint64_t)M_ * l_);
+  g2_.reserve((uint64_t)M_ * l_ * (prec == PREC_FP64 ? 8 : 4));
+  if (nnz_snp_ > 0) {
+    corrG_.reserve((uint64_t)M_ * l_);
+    corrH_.reserve((uint64_t)nwin_ * N_ * l_);
+    if (prec != PREC_FP64) omega32_.reserve((uint64_t)N_ * l_);
+  }
   S2_.reserve((uint64_t)nwin_ * l_);
   s2p_.reserve((uint64_t)nwin_ * finalize_chunks(bs_) * l_);
   for (auto* h : {&H1_, &H2_, &H_}) {
@@ -486,7 +491,16 @@ void Engine::gram_init(int precision, bool standardize, int ploidy) {
   mv_omega_.reserve(N_);
   mv_colsum_.reserve(1);
   mv_G_.reserve(M_);
-  mv_g2_.reserve(M_);
+  mv_g2_.reserve((uint64_t)M_ * (precision == PREC_FP64 ? 8 : 4));
+  b.g2_f32 = precision != PREC_FP64;
+  if (nnz_snp_ > 0) {
+    mv_corrG_.reserve(M_);
+    mv_corrH_.reserve((uint64_t)nwin_ * N_);
+    if (b.g2_f32) mv_omega32_.reserve(N_);
+    b.corrG = mv_corrG_.p;
+    b.corrH = mv_corrH_.p;
+    b.omega32 = mv_omega32_.p;
+  }
   mv_S2_.reserve(nwin_);
   mv_s2p_.reserve((uint64_t)nwin_ * finalize_chunks(bs_));
   mv_H_.reserve(N_);
@@ -529,6 +543,7 @@ void Engine::set_omega_vec(const double* dev_v, const double* host_v) {
   timed("omega_operand", [&] {
     launch_omega_operand(mv_omega_.p, N_, 1, mv_.prec, mv_bg_.p, mv_.bcols, mv_.lo_off, kg_, stream_);
   });
+  refresh_omega32(mv_);
   sync();  // cs lives on this stack frame
 }
 
@@ -624,6 +639,7 @@ void Engine::init_omega() {
   PCAONE_HIP(hipMemcpyAsync(colsum_.p, colsum0_.p, (size_t)l_ * 8, hipMemcpyDeviceToDevice, stream_));
   timed("omega_operand",
         [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, lo_off_, kg_, stream_); });
+  refresh_omega32(op_bufs());
 }
 
 // Omega <- Householder Q of H (H = H1 + H2 for winSVD), then flipOmg
@@ -631,28 +647,35 @@ void Engine::init_omega() {
 // R-diagonal signs reproduced, so Q equals Eigen's householderQ() columns.
 void Engine::omega_refresh(bool from_two, double* zero_acc) {
   const uint32_t l = l_;
-  const int nb = gram_blocks(N_ - l);
+  const int nb = gram_blocks(N_);
   timed("qr_gram", [&] {
     if (from_two)
-      launch_gram_partial(H1_.p, H2_.p, H_.p, N_, l, l, part_.p, nb, stream_);
+      launch_gram_partial(H1_.p, H2_.p, H_.p, N_, 0, l, part_.p, nb, stream_);
     else
-      launch_gram_partial(H_.p, nullptr, nullptr, N_, l, l, part_.p, nb, stream_);
+      launch_gram_partial(H_.p, nullptr, nullptr, N_, 0, l, part_.p, nb, stream_);
   });
   timed("qr_chol", [&] {
-    launch_chol(part_.p, nb, H_.p, l, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, sign_.p, status_.p,
+    launch_chol(part_.p, nb, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, status_.p,
                 stream_);
   });
-  const int nb2 = gram_blocks(N_);
-  timed("qr_apply", [&] { launch_apply_rinv(H_.p, R1i_.p, nullptr, Q_.p, N_, l, part_.p, nb2, stream_); });
+  timed("qr_apply", [&] { launch_apply_rinv(H_.p, R1i_.p, nullptr, Q_.p, N_, l, part_.p, nb, stream_); });
+  // second pass; the Householder signs come from the top block of Q R2^-1
   timed("qr_chol", [&] {
-    launch_chol(part_.p, nb2, nullptr, 0, l, R2_.p, R2i_.p, nullptr, nullptr, nullptr, nullptr, nullptr,
-                status_.p + 1, stream_);
+    launch_chol(part_.p, nb, l, R2_.p, R2i_.p, nullptr, nullptr, nullptr, nullptr, Q_.p, sign_.p, status_.p + 1,
+                stream_);
   });
-  timed("qr_apply", [&] { launch_apply_rinv(Q_.p, R2i_.p, sign_.p, omega_.p, N_, l, nullptr, nb2, stream_); });
-  timed("flip", [&] { launch_flip_stats(omega_.p, omg2_.p, N_, l, fpart_.p, nb2, stream_); });
-  timed("flip", [&] { launch_flip_apply(omega_.p, omg2_.p, N_, l, fpart_.p, nb2, colsum_.p, zero_acc, stream_); });
+  timed("qr_apply", [&] { launch_apply_rinv(Q_.p, R2i_.p, sign_.p, omega_.p, N_, l, nullptr, nb, stream_); });
+  timed("flip", [&] { launch_flip_stats(omega_.p, omg2_.p, N_, l, fpart_.p, nb, stream_); });
+  timed("flip", [&] { launch_flip_apply(omega_.p, omg2_.p, N_, l, fpart_.p, nb, colsum_.p, zero_acc, stream_); });
   timed("omega_operand",
         [&] { launch_omega_operand(omega_.p, N_, l, p_.precision, bg_.p, bcols_, lo_off_, kg_, stream_); });
+  refresh_omega32(op_bufs());
+}
+
+// the FP32 table the missing-entry gathers of the G contraction read
+void Engine::refresh_omega32(const SegBufs& b) {
+  if (!b.corrG || !b.g2_f32) return;
+  timed("omega_operand", [&] { launch_cast_f32(b.omega, b.omega32, (uint64_t)N_ * b.l, stream_); });
 }
 
 // G rows of windows [w0, w1) and their H contributions, routed per window.
@@ -673,6 +696,10 @@ SegBufs Engine::op_bufs() {
   b.colsum = colsum_.p;
   b.G = G_.p;
   b.g2 = g2_.p;
+  b.g2_f32 = p_.precision != PREC_FP64;
+  b.omega32 = omega32_.p;
+  b.corrG = nnz_snp_ > 0 ? corrG_.p : nullptr;
+  b.corrH = nnz_snp_ > 0 ? corrH_.p : nullptr;
   b.S2 = S2_.p;
   b.s2p = s2p_.p;
   b.bg = bg_.p;
@@ -730,6 +757,21 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
     g_nsplit = g.nsplit;
     timed("gemm_G", [&] { launch_dense_gemm(g, prec, (int)b.l, b.ntiles, stream_); });
   }
+  if (b.corrG) {
+    CsrGather cg{};
+    cg.ptr = miss_snp_ptr_.p;
+    cg.idx = miss_snp_idx_.p;
+    cg.table = b.g2_f32 ? static_cast<const void*>(b.omega32) : static_cast<const void*>(b.omega);
+    cg.table_f32 = b.g2_f32;
+    cg.l = b.l;
+    cg.in_base = r0;
+    cg.stride_in = 0;
+    cg.out_base = r0;
+    cg.per = rows;
+    cg.ngroups = 1;
+    cg.out = b.corrG;
+    timed("miss_gather", [&] { launch_csr_gather(cg, stream_); });
+  }
   FinalizeG fg = finalize_desc(w0, w1, b);
   fg.part = b.gpart;
   fg.nsplit = g_nsplit;
@@ -751,8 +793,10 @@ FinalizeG Engine::finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) cons
   fg.miss_ptr = miss_snp_ptr_.p;
   fg.miss_idx = miss_snp_idx_.p;
   fg.omega = b.omega;
+  fg.corr = b.corrG;
   fg.G = b.G;
   fg.g2 = b.g2;
+  fg.g2_f32 = b.g2_f32;
   fg.bprec = b.prec;
   fg.bh = b.bh;
   fg.bcols = b.bcols;
@@ -817,6 +861,21 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
     h_nsplit = h.nsplit;
     timed("gemm_H", [&] { launch_dense_gemm(h, prec, (int)b.l, b.ntiles, stream_); });
   }
+  if (b.corrH) {
+    CsrGather cg{};
+    cg.ptr = miss_win_ptr_.p;
+    cg.idx = miss_win_idx_.p;
+    cg.table = b.g2;
+    cg.table_f32 = b.g2_f32;
+    cg.l = b.l;
+    cg.in_base = (uint64_t)w0 * miss_rows_;
+    cg.stride_in = miss_rows_;
+    cg.out_base = (uint64_t)w0 * N_;
+    cg.per = N_;
+    cg.ngroups = nz;
+    cg.out = b.corrH;
+    timed("miss_gather", [&] { launch_csr_gather(cg, stream_); });
+  }
   FinalizeH fh{};
   fh.part = b.hpart;
   fh.part_is_f64 = b.part_f64;
@@ -833,7 +892,7 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
   fh.S2 = b.S2;
   fh.miss_ptr = miss_win_ptr_.p;
   fh.miss_idx = miss_win_idx_.p;
-  fh.g2 = b.g2;
+  fh.corr = b.corrH;
   fh.mval = b.mval;
   fh.mpos = miss_win_pos_.p;
   fh.G = b.G;
@@ -931,12 +990,12 @@ void Engine::usv_step() {
   const int nbG = gram_blocks(M_), nbN = gram_blocks(N_);
   timed("usv_gram", [&] { launch_gram_partial(G_.p, nullptr, nullptr, M_, 0, l, part_.p, nbG, stream_); });
   timed("usv_chol", [&] {
-    launch_chol(part_.p, nbG, nullptr, 0, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, nullptr, status_.p,
+    launch_chol(part_.p, nbG, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, status_.p,
                 stream_);
   });
   timed("usv_apply", [&] { launch_apply_rinv(G_.p, R1i_.p, nullptr, G_.p, M_, l, part_.p, nbG, stream_); });
   timed("usv_chol", [&] {
-    launch_chol(part_.p, nbG, nullptr, 0, l, R2_.p, R2i_.p, R1_.p, R1i_.p, Rt_.p, Rti_.p, nullptr, status_.p + 1,
+    launch_chol(part_.p, nbG, l, R2_.p, R2i_.p, R1_.p, R1i_.p, Rt_.p, Rti_.p, nullptr, nullptr, status_.p + 1,
                 stream_);
   });
   timed("usv_apply", [&] { launch_apply_rinv(G_.p, R2i_.p, nullptr, G_.p, M_, l, nullptr, nbG, stream_); });

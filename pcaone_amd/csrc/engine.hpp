@@ -59,8 +59,12 @@ struct SegBufs {
   bool part_f64 = false, split_cols = false;
   double hf = 0.5;
   const double* omega = nullptr;
+  float* omega32 = nullptr;  // FP32 copy of Omega: the missing-entry gather table of the non-FP64 arms
   const double* colsum = nullptr;
-  double *G = nullptr, *g2 = nullptr, *S2 = nullptr, *s2p = nullptr;
+  double *G = nullptr, *S2 = nullptr, *s2p = nullptr;
+  void* g2 = nullptr;        // 2 F d G, float when g2_f32
+  bool g2_f32 = false;
+  double *corrG = nullptr, *corrH = nullptr;  // missing-entry sums (null: nothing missing)
   unsigned char *bg = nullptr, *bh = nullptr, *gpart = nullptr, *hpart = nullptr;
   double *Ha = nullptr, *Hb = nullptr;
   const double* mval = nullptr;
@@ -110,6 +114,7 @@ class Engine {
     return (src_ == SRC_DENSE || prec == PREC_SPLIT_BF16) ? 256u : 512u;
   }
   SegBufs op_bufs();
+  void refresh_omega32(const SegBufs& b);
   FinalizeG finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) const;
   void seg_G(uint32_t w0, uint32_t w1, const SegBufs& b);
   void seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b);
@@ -163,7 +168,10 @@ class Engine {
   std::vector<double> omega_given_;
   DevBuf<double> omega_, omg2_, colsum_, omega0_, colsum0_;
   DevBuf<unsigned char> bg_, bh_;   // operands of the G and H contractions
-  DevBuf<double> G_, g2_, S2_, s2p_, H1_, H2_, H_;
+  DevBuf<double> G_, S2_, s2p_, H1_, H2_, H_;
+  DevBuf<unsigned char> g2_;
+  DevBuf<float> omega32_;
+  DevBuf<double> corrG_, corrH_;    // [M][l] and [nwin][N][l] missing-entry sums
   DevBuf<unsigned char> gpart_, hpart_;
   uint64_t bandsize_ = 1;
   // RsvdOpData::update / U, S, V of the last computeUSV (src/Halko.hpp:11-14)
@@ -178,7 +186,9 @@ class Engine {
   bool have_mv_ = false;
   int64_t nops_ = 0;
   SegBufs mv_;
-  DevBuf<double> mv_omega_, mv_colsum_, mv_G_, mv_g2_, mv_S2_, mv_s2p_, mv_H_;
+  DevBuf<double> mv_omega_, mv_colsum_, mv_G_, mv_S2_, mv_s2p_, mv_H_, mv_corrG_, mv_corrH_;
+  DevBuf<unsigned char> mv_g2_;
+  DevBuf<float> mv_omega32_;
   DevBuf<unsigned char> mv_bg_, mv_bh_, mv_gpart_, mv_hpart_;
 };
 

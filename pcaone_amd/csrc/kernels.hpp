@@ -1,6 +1,6 @@
 // Host-side launch wrappers for the gfx950 kernels.  Every wrapper enqueues on
 // the given stream and returns immediately; argument shapes are checked by the
-// callers in rsvd.cpp before anything is launched.
+// callers in engine.cpp before anything is launched.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -85,11 +85,13 @@ struct FinalizeG {
   const double* F;      // [M] allele frequency, laid-out order
   const double* d;      // [M] 1/sd or 1
   const double* colsum; // [l]  sum_j Omega[j][c]
-  const int64_t* miss_ptr;  // [M+1]
+  const int64_t* miss_ptr;  // [M+1]  (EMU values only)
   const int32_t* miss_idx;  // sample indices
   const double* omega;  // [N][l]
+  const double* corr;   // [M][l] sum of Omega over each row's missing samples, or null (none missing)
   double* G;            // [M][l]
-  double* g2;           // [M][l]  2 f d G
+  void* g2;             // [M][l]  2 f d G, float when g2_f32 else double
+  int g2_f32;
   // H operand, per window: [nwin][bcols][kw]
   int bprec;            // 0 fp64, 1 fp32, 2 bf16 split
   void* bh;
@@ -103,7 +105,7 @@ struct FinalizeG {
   uint32_t nsnps;
   double* s2p;          // [nwin][finalize_chunks(blocksize)][l] partial sums of g2
 };
-// grid = (chunks of 32 rows, windows of the segment)
+// grid = (chunks of 64 rows, windows of the segment)
 void launch_finalize_G(const FinalizeG& f, hipStream_t s);
 uint32_t finalize_chunks(uint32_t blocksize);
 // the H operand, g2 and its chunk sums from a given f.G (no contraction before)
@@ -138,9 +140,9 @@ struct FinalizeH {
   double* Ha;
   double* Hb;
   const double* S2;     // [nwin][l]
-  const int64_t* miss_ptr;  // [nwin*miss_rows+1]
+  const int64_t* miss_ptr;  // [nwin*miss_rows+1]  (EMU values only)
   const int32_t* miss_idx;  // SNP rows
-  const double* g2;     // [M][l]
+  const double* corr;   // [nwin][nsamples][l] sum of g2 over each (window, sample) missing list, or null
   // EMU: H_j += sum over missing (i, j) of val * d_i G_i
   const double* mval;   // SNP-CSR values or null
   const int64_t* mpos;  // window entry -> SNP-CSR position
@@ -149,6 +151,25 @@ struct FinalizeH {
   double hf;            // 0.5 for A1 dosages, 1 for dense values
 };
 void launch_finalize_H(const FinalizeH& f, hipStream_t s);
+
+// Sparse corrections for the missing genotypes: for each list row o of a
+// CSR (ptr, idx), out[o][c] = sum_p table[idx[p]][c] (FP64 sums of a float or
+// double table).  List rows come in ngroups groups of `per` rows: row o reads
+// the list at in_base + (o / per) * stride_in + o % per and writes out row
+// out_base + o.
+struct CsrGather {
+  const int64_t* ptr;
+  const int32_t* idx;
+  const void* table;
+  int table_f32;
+  uint32_t l;
+  uint64_t in_base, stride_in, out_base;
+  uint32_t per, ngroups;
+  double* out;
+};
+void launch_csr_gather(const CsrGather& g, hipStream_t s);
+// FP64 -> FP32 copy
+void launch_cast_f32(const double* src, float* dst, uint64_t n, hipStream_t s);
 
 // Omega (FP64 [N][l]) -> G-contraction operand [bcols][kpad] in the arm's type
 // (split: hi at row c, lo at row lo_off + c).
@@ -213,13 +234,14 @@ void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t s);
 int gram_blocks(uint64_t nrows);
 void launch_gram_partial(const double* A, const double* A2, double* Aout, uint64_t nrows, uint32_t row0,
                          uint32_t l, double* part, int nblocks, hipStream_t s);
-// Single workgroup: Gram = sum(part) (+ top^T top for rows [0,row0) of A),
-// R = chol(Gram) upper, Rinv = R^-1, optional Rprev: Rtot = R * Rprev and
-// Rtot_inv = Rprev_inv * Rinv.  sign (optional, l): Householder R-diagonal
-// signs of A reproduced from A's top rows and the bottom Gram.
-void launch_chol(const double* part, int nparts, const double* A, uint32_t row0, uint32_t l, double* R,
-                 double* Rinv, const double* Rprev, const double* Rprev_inv, double* Rtot, double* Rtot_inv,
-                 double* sign, int* status, hipStream_t s);
+// Single workgroup: Gram = sum(part), R = chol(Gram) upper, Rinv = R^-1,
+// optional Rprev: Rtot = R * Rprev and Rtot_inv = Rprev_inv * Rinv.  sign
+// (optional, l): Householder R-diagonal signs of the matrix being
+// orthogonalised, from the top l x l block Qtop of the matrix this Cholesky
+// factor is applied to (Qtop R^-1 is the top block of the orthonormal factor).
+void launch_chol(const double* part, int nparts, uint32_t l, double* R, double* Rinv, const double* Rprev,
+                 const double* Rprev_inv, double* Rtot, double* Rtot_inv, const double* Qtop, double* sign,
+                 int* status, hipStream_t s);
 // Q = A * Rinv (rows), optional per-column sign; optional partial Gram of Q.
 void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, double* Q, uint64_t nrows,
                        uint32_t l, double* gram_part, int nblocks, hipStream_t s);

@@ -11,6 +11,8 @@
 //   g' = d G, g'' = 2 F g', S'' = sum_i g''_i.
 // The dense parts sum_j y_ij Omega_j and sum_i y_ij g'_i run on the packed codes;
 // the missing entries (sparse) are corrected in the epilogues from CSR lists.
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -423,25 +425,77 @@ __device__ inline double load_part(const void* p, int is64, uint64_t i) {
   return is64 ? reinterpret_cast<const double*>(p)[i] : (double)reinterpret_cast<const float*>(p)[i];
 }
 
-// Grid (row chunk of 32, window); thread (c = t % 64, rs = t / 64) handles rows
+// Grid (row chunk of 64, window); thread (c = t % 64, rs = t / 64) handles rows
 // rs, rs + 4, ... of the chunk for column c.  Besides G, g'' and the H operand
 // it writes the chunk's partial column sums of g'' (s2p), which
-// k_window_colsum reduces into S''.
-constexpr uint32_t kFinRows = 32;
+// k_window_colsum reduces into S''.  The operand (column-major per window) is
+// staged through LDS so its stores run along the window's SNPs.
+constexpr uint32_t kFinRows = 64;
+constexpr uint32_t kTileLd = 65;
+
+// bh[w][c][k0 + q] <- tile[q][c] for q < nq, c < l (and the lo half for the
+// split arm); consecutive lanes take consecutive q.
+__device__ inline void store_operand_tile(const double* tile, uint32_t nq, uint32_t l, uint32_t w, uint32_t k0,
+                                          int bprec, void* bh, uint32_t bcols, uint32_t lo_off, uint64_t kw) {
+  for (uint32_t e = threadIdx.x; e < l * kFinRows; e += 256) {
+    const uint32_t c = e / kFinRows, q = e % kFinRows;
+    if (q >= nq) continue;
+    const double gp = tile[q * kTileLd + c];
+    const uint64_t o = ((uint64_t)w * bcols + c) * kw + k0 + q;
+    if (bprec == PREC_FP64) {
+      reinterpret_cast<double*>(bh)[o] = gp;
+    } else if (bprec == PREC_FP32) {
+      reinterpret_cast<float*>(bh)[o] = (float)gp;
+    } else {
+      const __bf16 hi = (__bf16)(float)gp;
+      const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
+      __bf16* b = reinterpret_cast<__bf16*>(bh);
+      b[o] = hi;
+      b[((uint64_t)w * bcols + lo_off + c) * kw + k0 + q] = lo;
+    }
+  }
+}
+
+// sum over a missing-entry list of v[idx[p] * l + c] (and of val[p] times it),
+// four independent loads at a time
+__device__ inline void miss_sums(const int64_t* ptr, const int32_t* idx, uint64_t li, const double* v, uint32_t l,
+                                 uint32_t c, const double* val, double& corr, double& corr_e) {
+  int64_t p = ptr[li];
+  const int64_t pe = ptr[li + 1];
+  for (; p + 4 <= pe; p += 4) {
+    const double o0 = v[(uint64_t)idx[p] * l + c], o1 = v[(uint64_t)idx[p + 1] * l + c];
+    const double o2 = v[(uint64_t)idx[p + 2] * l + c], o3 = v[(uint64_t)idx[p + 3] * l + c];
+    corr += (o0 + o1) + (o2 + o3);
+    if (val) corr_e += (val[p] * o0 + val[p + 1] * o1) + (val[p + 2] * o2 + val[p + 3] * o3);
+  }
+  for (; p < pe; ++p) {
+    const double o = v[(uint64_t)idx[p] * l + c];
+    corr += o;
+    if (val) corr_e += val[p] * o;
+  }
+}
+
+__device__ inline void store_g2(const FinalizeG& f, uint64_t i, double v) {
+  if (f.g2_f32)
+    reinterpret_cast<float*>(f.g2)[i] = (float)v;
+  else
+    reinterpret_cast<double*>(f.g2)[i] = v;
+}
 
 __global__ __launch_bounds__(256) void k_finalize_G(FinalizeG f) {
   __shared__ double sh[4][64];
+  __shared__ double tile[kFinRows * kTileLd];
   const uint32_t t = threadIdx.x, c = t & 63, rs = t >> 6;
   const uint32_t w = f.w0 + blockIdx.y, chunk = blockIdx.x;
   const uint32_t wstart = w * f.blocksize;
   const uint32_t wend = min(wstart + f.blocksize, f.nsnps);
   const uint32_t cb = wstart + chunk * kFinRows;
+  const uint32_t nq = cb < wend ? min(kFinRows, wend - cb) : 0;
   const uint64_t nrow = f.r1 - f.r0;
   double acc2 = 0.0;
   if (c < f.l) {
-    for (uint32_t q = rs; q < kFinRows; q += 4) {
+    for (uint32_t q = rs; q < nq; q += 4) {
       const uint32_t r = cb + q;
-      if (r >= wend) break;
       const uint32_t rr = r - f.r0;
       double raw = 0.0;
       for (uint32_t s = 0; s < f.nsplit; ++s) {
@@ -449,43 +503,27 @@ __global__ __launch_bounds__(256) void k_finalize_G(FinalizeG f) {
         raw += load_part(f.part, f.part_is_f64, base + c);
         if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
       }
-      double corr = 0.0, corr_e = 0.0;
+      const double corr = f.corr ? f.corr[(uint64_t)r * f.l + c] : 0.0;
+      double corr_e = 0.0;
       if (f.mval) {
-        for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p) {
-          const double om = f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
-          corr += om;
-          corr_e += f.mval[p] * om;
-        }
-      } else {
-        for (int64_t p = f.miss_ptr[r]; p < f.miss_ptr[r + 1]; ++p)
-          corr += f.omega[(uint64_t)f.miss_idx[p] * f.l + c];
+        double unused = 0.0;
+        miss_sums(f.miss_ptr, f.miss_idx, r, f.omega, f.l, c, f.mval, unused, corr_e);
       }
       const double Fi = f.F[r], di = f.d[r];
       const double Gv = di * (f.hf * (raw - 2.0 * Fi * (f.colsum[c] - corr)) + corr_e);
       f.G[(uint64_t)r * f.l + c] = Gv;
       const double gp = di * Gv;
       const double g2 = 2.0 * Fi * gp;
-      f.g2[(uint64_t)r * f.l + c] = g2;
+      store_g2(f, (uint64_t)r * f.l + c, g2);
       acc2 += g2;
-      const uint32_t k = r - wstart;
-      const uint64_t o = ((uint64_t)w * f.bcols + c) * f.kw + k;
-      if (f.bprec == PREC_FP64) {
-        reinterpret_cast<double*>(f.bh)[o] = gp;
-      } else if (f.bprec == PREC_FP32) {
-        reinterpret_cast<float*>(f.bh)[o] = (float)gp;
-      } else {
-        const __bf16 hi = (__bf16)(float)gp;
-        const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
-        __bf16* bh = reinterpret_cast<__bf16*>(f.bh);
-        bh[o] = hi;
-        bh[((uint64_t)w * f.bcols + f.lo_off + c) * f.kw + k] = lo;
-      }
+      tile[q * kTileLd + c] = gp;
     }
   }
   sh[rs][c] = acc2;
   __syncthreads();
   if (t < f.l)
     f.s2p[((uint64_t)w * gridDim.x + chunk) * f.l + t] = (sh[0][t] + sh[1][t]) + (sh[2][t] + sh[3][t]);
+  store_operand_tile(tile, nq, f.l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw);
 }
 
 void launch_finalize_G(const FinalizeG& f, hipStream_t s) {
@@ -502,39 +540,29 @@ uint32_t finalize_chunks(uint32_t blocksize) { return (uint32_t)ceil_div(blocksi
 // without a preceding X^T Omega (Gram mat-vec X^T X x starts here).
 __global__ __launch_bounds__(256) void k_operand_from_G(FinalizeG f) {
   __shared__ double sh[4][64];
+  __shared__ double tile[kFinRows * kTileLd];
   const uint32_t t = threadIdx.x, c = t & 63, rs = t >> 6;
   const uint32_t w = f.w0 + blockIdx.y, chunk = blockIdx.x;
   const uint32_t wstart = w * f.blocksize;
   const uint32_t wend = min(wstart + f.blocksize, f.nsnps);
   const uint32_t cb = wstart + chunk * kFinRows;
+  const uint32_t nq = cb < wend ? min(kFinRows, wend - cb) : 0;
   double acc2 = 0.0;
   if (c < f.l) {
-    for (uint32_t q = rs; q < kFinRows; q += 4) {
+    for (uint32_t q = rs; q < nq; q += 4) {
       const uint32_t r = cb + q;
-      if (r >= wend) break;
       const double gp = f.d[r] * f.G[(uint64_t)r * f.l + c];
       const double g2 = 2.0 * f.F[r] * gp;
-      f.g2[(uint64_t)r * f.l + c] = g2;
+      store_g2(f, (uint64_t)r * f.l + c, g2);
       acc2 += g2;
-      const uint32_t k = r - wstart;
-      const uint64_t o = ((uint64_t)w * f.bcols + c) * f.kw + k;
-      if (f.bprec == PREC_FP64) {
-        reinterpret_cast<double*>(f.bh)[o] = gp;
-      } else if (f.bprec == PREC_FP32) {
-        reinterpret_cast<float*>(f.bh)[o] = (float)gp;
-      } else {
-        const __bf16 hi = (__bf16)(float)gp;
-        const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
-        __bf16* bh = reinterpret_cast<__bf16*>(f.bh);
-        bh[o] = hi;
-        bh[((uint64_t)w * f.bcols + f.lo_off + c) * f.kw + k] = lo;
-      }
+      tile[q * kTileLd + c] = gp;
     }
   }
   sh[rs][c] = acc2;
   __syncthreads();
   if (t < f.l)
     f.s2p[((uint64_t)w * gridDim.x + chunk) * f.l + t] = (sh[0][t] + sh[1][t]) + (sh[2][t] + sh[3][t]);
+  store_operand_tile(tile, nq, f.l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw);
 }
 
 void launch_operand_from_G(const FinalizeG& f, hipStream_t s) {
@@ -544,27 +572,35 @@ void launch_operand_from_G(const FinalizeG& f, hipStream_t s) {
   hipLaunchKernelGGL(k_operand_from_G, grid, dim3(256), 0, s, f);
 }
 
-// S2[w][c] = sum of the chunk partials of window w (fixed order)
-__global__ void k_window_colsum(const double* __restrict__ s2p, uint32_t l, uint32_t nchunk, uint32_t w0,
-                                double* __restrict__ S2) {
-  const uint32_t w = w0 + blockIdx.x, c = threadIdx.x;
-  if (c >= l) return;
+// S2[w][c] = sum of the chunk partials of window w: four slices of chunks,
+// combined in a fixed order
+__global__ __launch_bounds__(256) void k_window_colsum(const double* __restrict__ s2p, uint32_t l, uint32_t nchunk,
+                                                       uint32_t w0, double* __restrict__ S2) {
+  __shared__ double sh[4][64];
+  const uint32_t w = w0 + blockIdx.x, c = threadIdx.x & 63, q0 = threadIdx.x >> 6;
   double tot = 0.0;
-  for (uint32_t q = 0; q < nchunk; ++q) tot += s2p[((uint64_t)w * nchunk + q) * l + c];
-  S2[(uint64_t)w * l + c] = tot;
+  if (c < l)
+    for (uint32_t q = q0; q < nchunk; q += 4) tot += s2p[((uint64_t)w * nchunk + q) * l + c];
+  sh[q0][c] = tot;
+  __syncthreads();
+  if (threadIdx.x < l) S2[(uint64_t)w * l + c] = (sh[0][c] + sh[1][c]) + (sh[2][c] + sh[3][c]);
 }
 
 void launch_window_colsum(const double* s2p, uint32_t l, uint32_t blocksize, uint32_t w0, uint32_t w1, double* S2,
                           hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= 64, -1, "l out of range");
   if (w1 <= w0) return;
-  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0), dim3(64), 0, s, s2p, l, finalize_chunks(blocksize), w0, S2);
+  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0), dim3(256), 0, s, s2p, l, finalize_chunks(blocksize), w0, S2);
 }
 
+// thread (sample j, column c); the windows' contributions are summed in
+// registers per destination and added once
 __global__ void k_finalize_H(FinalizeH f) {
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (uint64_t)f.nsamples * f.l) return;
   const uint32_t j = (uint32_t)(gid / f.l), c = (uint32_t)(gid % f.l);
+  double ha = 0.0, hb = 0.0;
+  bool any_a = false, any_b = false;
   for (uint32_t w = f.w0; w < f.w1; ++w) {
     const uint32_t wz = w - f.wpart0;
     double raw = 0.0;
@@ -573,21 +609,26 @@ __global__ void k_finalize_H(FinalizeH f) {
       raw += load_part(f.part, f.part_is_f64, base + c);
       if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
     }
-    const uint64_t li = (uint64_t)w * f.miss_rows + j;
-    double corr = 0.0, corr_e = 0.0;
+    const double corr = f.corr ? f.corr[((uint64_t)w * f.nsamples + j) * f.l + c] : 0.0;
+    double corr_e = 0.0;
     if (f.mval) {
+      const uint64_t li = (uint64_t)w * f.miss_rows + j;
       for (int64_t p = f.miss_ptr[li]; p < f.miss_ptr[li + 1]; ++p) {
         const uint64_t i = (uint64_t)f.miss_idx[p];
-        corr += f.g2[i * f.l + c];
         corr_e += f.mval[f.mpos[p]] * f.d[i] * f.G[i * f.l + c];
       }
-    } else {
-      for (int64_t p = f.miss_ptr[li]; p < f.miss_ptr[li + 1]; ++p) corr += f.g2[(uint64_t)f.miss_idx[p] * f.l + c];
     }
     const double h = f.hf * (raw - (f.S2[(uint64_t)w * f.l + c] - corr)) + corr_e;
-    double* dst = ((f.route_b >> (w - f.w0)) & 1ull) ? f.Hb : f.Ha;
-    dst[(uint64_t)j * f.l + c] += h;
+    if ((f.route_b >> (w - f.w0)) & 1ull) {
+      hb += h;
+      any_b = true;
+    } else {
+      ha += h;
+      any_a = true;
+    }
   }
+  if (any_a) f.Ha[(uint64_t)j * f.l + c] += ha;
+  if (any_b) f.Hb[(uint64_t)j * f.l + c] += hb;
 }
 
 void launch_finalize_H(const FinalizeH& f, hipStream_t s) {
@@ -648,6 +689,83 @@ void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin,
   const uint64_t rows = (uint64_t)nwin * npad;
   hipLaunchKernelGGL(k_map_missing, dim3((uint32_t)ceil_div(rows, 256)), dim3(256), 0, s, wptr, widx, nwin, npad,
                      nsamples, sptr, sidx, pos);
+}
+
+// One wave per list row.  The row's indices are loaded 64 at a time (one per
+// lane) and handed out with lane shuffles; lane group grp (lpg = ceil(l / 2)
+// lanes, two columns per lane) takes entries grp, grp + ng, ... so a wave has
+// 4 ng table rows in flight.  The groups' sums combine in a fixed order.
+template <class T>
+__global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lpg = (int)((g.l + 1) >> 1);
+  const int ng = 64 / lpg;
+  const int grp = lane / lpg, li = lane - grp * lpg;
+  const bool act = grp < ng;
+  const uint32_t c0 = 2 * li, c1 = 2 * li + 1;
+  const bool has0 = act && c0 < g.l, has1 = act && c1 < g.l;
+  const T* tab = reinterpret_cast<const T*>(g.table);
+  const uint64_t total = (uint64_t)g.per * g.ngroups;
+  for (uint64_t o = (uint64_t)blockIdx.x * 4 + wv; o < total; o += (uint64_t)gridDim.x * 4) {
+    const uint64_t q = o / g.per;
+    const uint64_t row = g.in_base + q * g.stride_in + (o - q * g.per);
+    const int64_t p0 = g.ptr[row], p1 = g.ptr[row + 1];
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t b = p0; b < p1; b += 64) {
+      const int cnt = (int)(p1 - b < 64 ? p1 - b : 64);
+      const int32_t mine = lane < cnt ? g.idx[b + lane] : 0;
+      for (int e0 = 0; e0 < cnt; e0 += 4 * ng) {
+        int j[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + u * ng + grp;
+          ok[u] = act && e < cnt;
+          j[u] = __shfl(mine, ok[u] ? e : 0);
+        }
+        double v0[4], v1[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint64_t base = (uint64_t)j[u] * g.l;
+          v0[u] = (ok[u] && has0) ? (double)tab[base + c0] : 0.0;
+          v1[u] = (ok[u] && has1) ? (double)tab[base + c1] : 0.0;
+        }
+        s0 += (v0[0] + v0[1]) + (v0[2] + v0[3]);
+        s1 += (v1[0] + v1[1]) + (v1[2] + v1[3]);
+      }
+    }
+    double t0 = s0, t1 = s1;
+    for (int gg = 1; gg < ng; ++gg) {
+      t0 += __shfl(s0, gg * lpg + li);
+      t1 += __shfl(s1, gg * lpg + li);
+    }
+    if (grp == 0) {
+      const uint64_t orow = g.out_base + o;
+      if (c0 < g.l) g.out[orow * g.l + c0] = t0;
+      if (c1 < g.l) g.out[orow * g.l + c1] = t1;
+    }
+  }
+}
+
+void launch_csr_gather(const CsrGather& g, hipStream_t s) {
+  PCAONE_REQUIRE(g.l >= 1 && g.l <= 64, -1, "gather: l must be in [1, 64]");
+  const uint64_t total = (uint64_t)g.per * g.ngroups;
+  if (!total) return;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(total, 4), 1u << 20);
+  if (g.table_f32)
+    hipLaunchKernelGGL(k_csr_gather<float>, dim3(blocks), dim3(256), 0, s, g);
+  else
+    hipLaunchKernelGGL(k_csr_gather<double>, dim3(blocks), dim3(256), 0, s, g);
+}
+
+__global__ void k_cast_f32(const double* __restrict__ src, float* __restrict__ dst, uint64_t n) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid < n) dst[gid] = (float)src[gid];
+}
+
+void launch_cast_f32(const double* src, float* dst, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_cast_f32, dim3((uint32_t)ceil_div(n, 256)), dim3(256), 0, s, src, dst, n);
 }
 
 __global__ void k_omega_operand(const double* __restrict__ omega, uint32_t n, uint32_t l, int bprec, void* bg,

@@ -14,252 +14,250 @@
 
 namespace pcaone {
 
-constexpr int kTile = 32;  // rows staged per LDS tile
 constexpr int kMaxL = 64;  // largest l the single-workgroup kernels hold in LDS
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4_f64: lane (k = lane / 16, i = lane % 16) supplies A[i][k]
+// and B[k][i]; accumulator element v of a lane holds D[mfma64_row(lane, v)][lane % 16].
+__device__ inline int mfma64_row(int lane, int v) { return (lane >> 4) + 4 * v; }
+__device__ inline f64x4 mfma64(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
 int gram_blocks(uint64_t nrows) {
   const uint64_t b = ceil_div(nrows, 64);
-  return (int)std::min<uint64_t>(std::max<uint64_t>(b, 1), 1024);
+  return (int)std::min<uint64_t>(std::max<uint64_t>(b, 1), 256);
 }
 
-// part[e] = sum_p part[p * n + e], in place, fixed order (deterministic)
-__global__ void k_reduce_parts(double* part, int nparts, uint32_t n) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  double s = 0.0;
-  for (int p = 0; p < nparts; ++p) s += part[(uint64_t)p * n + e];
-  part[e] = s;
+// part[e] = sum_p part[p * n + e] (e < n), in place.  Block b owns entries
+// [64 b, 64 b + 64); wave w sums parts w, w + 4, ... in four interleaved
+// chains, then the waves combine in a fixed order, so the result is
+// deterministic.
+__global__ __launch_bounds__(256) void k_reduce_parts(double* part, int nparts, uint32_t n) {
+  __shared__ double sh[4][64];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t e = blockIdx.x * 64 + lane;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (e < n) {
+    int p = (int)wv;
+    for (; p + 12 < nparts; p += 16) {
+      s0 += part[(uint64_t)p * n + e];
+      s1 += part[(uint64_t)(p + 4) * n + e];
+      s2 += part[(uint64_t)(p + 8) * n + e];
+      s3 += part[(uint64_t)(p + 12) * n + e];
+    }
+    for (; p < nparts; p += 4) s0 += part[(uint64_t)p * n + e];
+  }
+  sh[wv][lane] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (wv == 0 && e < n) part[e] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
 }
 
 static int reduce_parts(const double* part, int nparts, uint32_t n, hipStream_t s) {
   if (nparts <= 1) return nparts;
-  hipLaunchKernelGGL(k_reduce_parts, dim3((uint32_t)ceil_div(n, 256)), dim3(256), 0, s, const_cast<double*>(part),
+  hipLaunchKernelGGL(k_reduce_parts, dim3((uint32_t)ceil_div(n, 64)), dim3(256), 0, s, const_cast<double*>(part),
                      nparts, n);
   return 1;
 }
 
-// acc[e] += sum_r X[r][i(e)] * Y[r][j(e)] for the entries e = t + 256 q owned
-// by this thread; X and Y are LDS tiles [rows][ldx].
-template <int E>
-__device__ inline void tile_xty(const double* X, const double* Y, int rows, int ldx, int ncx, int ncy,
-                                double (&acc)[E]) {
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int e = t + 256 * q;
-    if (e < ncx * ncy) {
-      const int i = e / ncy, j = e % ncy;
-      double s = acc[q];
-      for (int r = 0; r < rows; ++r) s = fma(X[r * ldx + i], Y[r * ldx + j], s);
-      acc[q] = s;
+// The NB x NB tile accumulators of a wave, summed over the block's four waves
+// in a fixed order, written as the block's full l x l partial.  Tile (ca, cb)
+// is skipped when `upper` and ca > cb (then mirrored from (cb, ca)).
+template <int NB, int NP>
+__device__ inline void block_tiles_out(const f64x4 (&acc)[NP], bool upper, uint32_t l, double* red, double* out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int w = 0; w < 4; ++w) {
+    if (wv == w)
+      for (int p = 0; p < NP; ++p)
+        for (int v = 0; v < 4; ++v) {
+          const int i = p * 256 + v * 64 + lane;
+          red[i] = (w == 0) ? acc[p][v] : red[i] + acc[p][v];
+        }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < NP * 256; e += 256) {
+    const int p = e >> 8, v = (e >> 6) & 3, ln = e & 63;
+    int ca, cb;
+    if (upper) {  // p enumerates ca <= cb row by row
+      ca = 0;
+      int rem = p;
+      while (rem >= NB - ca) {
+        rem -= NB - ca;
+        ++ca;
+      }
+      cb = ca + rem;
+    } else {
+      ca = p / NB;
+      cb = p % NB;
+    }
+    const uint32_t r = ca * 16 + mfma64_row(ln, v), c = cb * 16 + (ln & 15);
+    if (upper && ca == cb && r > c) continue;  // the diagonal tile's upper half writes both
+    if (r < l && c < l) {
+      out[(uint64_t)r * l + c] = red[e];
+      if (upper) out[(uint64_t)c * l + r] = red[e];
     }
   }
 }
 
-template <int E>
-__device__ inline void store_acc(double* part, int ncx, int ncy, const double (&acc)[E]) {
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int e = t + 256 * q;
-    if (e < ncx * ncy) part[e] = acc[q];
-  }
-}
-
-static int entries_per_thread(int n) {
-  const int e = (n + 255) / 256;
-  int p = 1;
-  while (p < e) p <<= 1;
-  return p;
-}
-
 // ---------------------------------------------------------------------------
-// partial Gram, optionally forming A + A2 on the fly
+// partial Gram (FP64 MFMA), optionally forming A + A2 on the fly
 // ---------------------------------------------------------------------------
-template <int E>
+// Rows [row0, nrows) go to the waves four at a time (one MFMA K step); lane
+// (k = lane / 16, i = lane % 16) loads A[row k][16 cb + i] of every column
+// block cb, which is at once the A and the B operand of the tile products.
+template <int NB>
 __global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__ A, const double* __restrict__ A2,
                                                       double* __restrict__ Aout, uint64_t nrows, uint32_t row0,
                                                       uint32_t l, double* __restrict__ part) {
-  __shared__ double tile[kTile * kMaxL];
-  const uint64_t live = nrows - row0;
-  const uint64_t rpb = ceil_div(live, gridDim.x);
-  const uint64_t rs = row0 + blockIdx.x * rpb;
-  const uint64_t re = std::min<uint64_t>(rs + rpb, nrows);
-  double acc[E];
+  constexpr int NP = NB * (NB + 1) / 2;
+  __shared__ double red[NP * 256];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kr = lane >> 4, ci = lane & 15;
+  f64x4 acc[NP];
 #pragma unroll
-  for (int q = 0; q < E; ++q) acc[q] = 0.0;
-  for (uint64_t r0 = rs; r0 < re; r0 += kTile) {
-    const int rows = (int)std::min<uint64_t>(kTile, re - r0);
-    __syncthreads();
-    for (int it = threadIdx.x; it < rows * (int)l; it += 256) {
-      const uint64_t gi = (r0 + it / l) * l + it % l;
-      double v = A[gi];
-      if (A2) {
-        v += A2[gi];
-        Aout[gi] = v;
+  for (int p = 0; p < NP; ++p) acc[p] = f64x4{0.0, 0.0, 0.0, 0.0};
+  const uint64_t steps = ceil_div(nrows - row0, 4);
+  const uint64_t nw = (uint64_t)gridDim.x * 4, wid = (uint64_t)blockIdx.x * 4 + wv;
+  const uint64_t sps = ceil_div(steps, nw);
+  const uint64_t s0 = wid * sps, s1 = std::min<uint64_t>(s0 + sps, steps);
+  for (uint64_t st = s0; st < s1; ++st) {
+    const uint64_t r = row0 + st * 4 + kr;
+    double a[NB];
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) {
+      const uint32_t c = cb * 16 + ci;
+      double v = 0.0;
+      if (r < nrows && c < l) {
+        const uint64_t gi = r * l + c;
+        v = A[gi];
+        if (A2) {
+          v += A2[gi];
+          Aout[gi] = v;
+        }
       }
-      tile[it] = v;
+      a[cb] = v;
     }
-    __syncthreads();
-    tile_xty<E>(tile, tile, rows, l, l, l, acc);
+    int p = 0;
+#pragma unroll
+    for (int ca = 0; ca < NB; ++ca)
+#pragma unroll
+      for (int cb = ca; cb < NB; ++cb, ++p) acc[p] = mfma64(a[ca], a[cb], acc[p]);
   }
   // rows above row0 still need A + A2 written
   if (A2 && blockIdx.x == 0)
     for (uint32_t it = threadIdx.x; it < row0 * l; it += 256) Aout[it] = A[it] + A2[it];
-  store_acc<E>(part + (uint64_t)blockIdx.x * l * l, l, l, acc);
+  block_tiles_out<NB, NP>(acc, true, l, red, part + (uint64_t)blockIdx.x * l * l);
 }
 
 void launch_gram_partial(const double* A, const double* A2, double* Aout, uint64_t nrows, uint32_t row0,
                          uint32_t l, double* part, int nblocks, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
   PCAONE_REQUIRE(nrows > row0, -1, "Gram over an empty row range");
-  const int E = entries_per_thread((int)(l * l));
-  switch (E) {
-#define PCAONE_E(EPT) \
-  case EPT: hipLaunchKernelGGL(k_gram_partial<EPT>, dim3(nblocks), dim3(256), 0, s, A, A2, Aout, nrows, row0, l, part); break;
-    PCAONE_E(1) PCAONE_E(2) PCAONE_E(4) PCAONE_E(8) PCAONE_E(16)
-#undef PCAONE_E
+  switch ((l + 15) / 16) {
+#define PCAONE_NB(NBV) \
+  case NBV: hipLaunchKernelGGL(k_gram_partial<NBV>, dim3(nblocks), dim3(256), 0, s, A, A2, Aout, nrows, row0, l, part); break;
+    PCAONE_NB(1) PCAONE_NB(2) PCAONE_NB(3) PCAONE_NB(4)
+#undef PCAONE_NB
     default: throw Error(-1, "gram: unsupported l");
   }
 }
 
 // ---------------------------------------------------------------------------
-// single-workgroup Cholesky (+ Householder signs, inverse, products)
+// single-workgroup Cholesky (+ inverse, products, Householder signs)
 // ---------------------------------------------------------------------------
-// Upper Cholesky in place (a[l][l] in LDS, R^T R = a).  Returns via *bad the
-// first failing pivot (1-based) or 0.
-__device__ void chol_upper_lds(double* a, int l, int* bad) {
+// Upper Cholesky R^T R = a (a: l x l in LDS, destroyed) into out (row stride
+// ldo).  Step k writes row k of R and the rank-1 update of the trailing block
+// from the same unscaled row k of a, so one barrier per step suffices.  *bad
+// receives the first failing pivot (1-based).
+__device__ void chol_upper_lds(double* a, int l, double* out, int ldo, int* bad) {
   const int t = threadIdx.x;
   for (int k = 0; k < l; ++k) {
-    if (t == 0) {
-      double d = a[k * l + k];
-      if (!(d > 0.0) || !isfinite(d)) {
-        if (*bad == 0) *bad = k + 1;
-        d = 1e-300;
-      }
-      a[k * l + k] = sqrt(d);
+    double d = a[k * l + k];
+    if (!(d > 0.0) || !isfinite(d)) {
+      if (t == 0 && *bad == 0) *bad = k + 1;
+      d = 1e-300;
     }
-    __syncthreads();
-    const double dk = a[k * l + k];
-    for (int j = k + 1 + t; j < l; j += 256) a[k * l + j] /= dk;
-    __syncthreads();
-    const int m = l - k - 1;
-    for (int e = t; e < m * m; e += 256) {
-      const int i = k + 1 + e / m, j = k + 1 + e % m;
-      if (j >= i) a[i * l + j] -= a[k * l + i] * a[k * l + j];
+    const double sd = sqrt(d), isd = 1.0 / sd, id = isd * isd;
+    for (int e = t; e < l - k; e += 256) out[k * ldo + k + e] = (e == 0) ? sd : a[k * l + k + e] * isd;
+    // trailing update on a 16 x 16 thread grid, upper triangle only
+    for (int i = k + 1 + (t >> 4); i < l; i += 16) {
+      const double aki = a[k * l + i] * id;
+      for (int j = k + 1 + (t & 15); j < l; j += 16)
+        if (j >= i) a[i * l + j] -= aki * a[k * l + j];
     }
     __syncthreads();
   }
-  for (int e = t; e < l * l; e += 256)
-    if (e / l > e % l) a[e] = 0.0;
+  for (int e = t; e < l * l; e += 256) {
+    const int i = e / l, j = e % l;
+    if (i > j) out[i * ldo + j] = 0.0;
+  }
   __syncthreads();
 }
 
-// Householder QR of S (rows x l, LDS, row-major) keeping only the sign of each
-// R diagonal: beta_k = -sign(alpha_k) ||x_k|| as in LAPACK dlarfg and Eigen's
-// makeHouseholder (zero tail: beta_k = alpha_k).  Wave 0 forms the tail norm,
-// then thread j applies the reflection to column j (blockDim >= l).
-__device__ void householder_signs_lds(double* S, int rows, int l, double* red, double* sign) {
-  const int t = threadIdx.x;
-  for (int k = 0; k < l; ++k) {
-    if (t < 64) {
-      double p = 0.0;
-      for (int r = k + 1 + t; r < rows; r += 64) p += S[r * l + k] * S[r * l + k];
-      for (int off = 32; off > 0; off >>= 1) p += __shfl_down(p, off);
-      if (t == 0) {
-        const double alpha = S[k * l + k];
-        double beta, tau = 0.0, inv = 0.0;
-        if (p <= 2.2250738585072014e-308) {
-          beta = alpha;
-        } else {
-          beta = sqrt(alpha * alpha + p);
-          if (alpha >= 0.0) beta = -beta;
-          tau = (beta - alpha) / beta;
-          inv = 1.0 / (alpha - beta);
-        }
-        sign[k] = (beta < 0.0) ? -1.0 : 1.0;
-        red[0] = tau;
-        red[1] = inv;
-      }
-    }
-    __syncthreads();
-    const double tau = red[0], inv = red[1];
-    if (tau != 0.0 && t > k && t < l) {
-      // u = [1; x_tail * inv]: w = u^T S[:, j]; S[:, j] -= tau u w
-      const int j = t;
-      double w = S[k * l + j];
-      for (int r = k + 1; r < rows; ++r) w += S[r * l + k] * inv * S[r * l + j];
-      S[k * l + j] -= tau * w;
-      for (int r = k + 1; r < rows; ++r) S[r * l + j] -= tau * (S[r * l + k] * inv) * w;
-    }
-    __syncthreads();
-  }
-}
-
-// X = R^-1 (upper) in LDS, rows solved bottom-up, one column per thread.
+// X = R^-1 (R upper, both l x l in LDS).  Rows are solved bottom-up; four
+// lanes share each column's dot product.
 __device__ void upper_inverse_lds(const double* a, double* X, int l) {
   const int t = threadIdx.x;
   for (int e = t; e < l * l; e += 256) X[e] = (e / l == e % l) ? 1.0 / a[e] : 0.0;
   __syncthreads();
+  const int j = t >> 2, q = t & 3;
   for (int i = l - 2; i >= 0; --i) {
-    if (t > i && t < l) {
-      const int j = t;
-      double s = 0.0;
-      for (int p = i + 1; p <= j; ++p) s += a[i * l + p] * X[p * l + j];
-      X[i * l + j] = -s / a[i * l + i];
+    double s = 0.0;
+    if (j > i && j < l)
+      for (int p = i + 1 + q; p <= j; p += 4) s += a[i * l + p] * X[p * l + j];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q == 0 && j > i && j < l) X[i * l + j] = -s * X[i * l + i];
+    __syncthreads();
+  }
+}
+
+// Signs of the Householder R diagonal of a tall matrix whose CholeskyQR factor
+// Q (positive R diagonal) has top l x l block B: LU of B - S with
+// S_kk = -sign(pivot) chosen at each step (Householder reconstruction, Ballard
+// et al. 2014); Householder QR with beta_k = -sign(alpha_k) ||x_k||, as in
+// LAPACK dgeqrf and Eigen's HouseholderQR, has R = S R_chol.  (A reflector
+// with an exactly zero tail keeps beta_k = alpha_k in LAPACK; a matrix with at
+// least 2l rows and full column rank does not produce one in practice.)
+__device__ void householder_signs_lu(double* B, int l, double* sign) {
+  const int t = threadIdx.x;
+  for (int k = 0; k < l; ++k) {
+    const double bkk = B[k * l + k];
+    const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
+    const double ip = 1.0 / (bkk - sk);
+    if (t == 0) sign[k] = sk;
+    for (int i = k + 1 + (t >> 4); i < l; i += 16) {
+      const double lik = B[i * l + k] * ip;
+      for (int j = k + 1 + (t & 15); j < l; j += 16) B[i * l + j] -= lik * B[k * l + j];
     }
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(256) void k_chol(const double* __restrict__ part, int nparts,
-                                              const double* __restrict__ A, uint32_t row0, uint32_t l,
-                                              double* R, double* Rinv, const double* Rprev,
-                                              const double* Rprev_inv, double* Rtot, double* Rtot_inv,
-                                              double* sign, int* status) {
+// LDS: four l x l slots.
+static size_t chol_lds_bytes(uint32_t l) { return (size_t)4 * l * l * sizeof(double); }
+
+__global__ __launch_bounds__(256) void k_chol(const double* __restrict__ part, int nparts, uint32_t l, double* R,
+                                              double* Rinv, const double* Rprev, const double* Rprev_inv,
+                                              double* Rtot, double* Rtot_inv, const double* Qtop, double* sign,
+                                              int* status) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  double* a = reinterpret_cast<double*>(smem_raw);  // l*l
-  double* S = a + l * l;                             // 2l*l
-  __shared__ double red[4];
+  const int L = (int)l, t = threadIdx.x;
+  double* F = reinterpret_cast<double*>(smem_raw);  // Gram, then R^-1
+  double* a = F + L * L;                             // R
+  double* P = a + L * L;                             // Rprev, then Q's top block
+  double* Pi = P + L * L;                            // Rprev^-1, then the sign LU
   __shared__ int bad;
-  const int t = threadIdx.x;
-  const int L = (int)l;
   if (t == 0) bad = 0;
   for (int e = t; e < L * L; e += 256) {
     double v = 0.0;
     for (int p = 0; p < nparts; ++p) v += part[(uint64_t)p * L * L + e];
-    a[e] = v;
+    F[e] = v;
   }
   __syncthreads();
-  if (row0 > 0) {
-    if (sign) {
-      // a = Gram of rows >= row0; R_b = chol(a); S = [A_top; R_b]
-      chol_upper_lds(a, L, &bad);
-      for (int e = t; e < (int)row0 * L; e += 256) S[e] = A[e];
-      for (int e = t; e < L * L; e += 256) S[row0 * L + e] = a[e];
-      __syncthreads();
-      householder_signs_lds(S, (int)row0 + L, L, red, sign);
-      // full Gram = R_b^T R_b + A_top^T A_top
-      for (int e = t; e < L * L; e += 256) {
-        const int i = e / L, j = e % L;
-        double v = 0.0;
-        for (int p = 0; p <= min(i, j); ++p) v += a[p * L + i] * a[p * L + j];
-        S[e] = v;
-      }
-      __syncthreads();
-      for (int e = t; e < L * L; e += 256) a[e] = S[e];
-      __syncthreads();
-    }
-    for (int e = t; e < L * L; e += 256) {
-      const int i = e / L, j = e % L;
-      double v = a[e];
-      for (uint32_t r = 0; r < row0; ++r) v += A[(uint64_t)r * L + i] * A[(uint64_t)r * L + j];
-      a[e] = v;
-    }
-    __syncthreads();
-  }
-  chol_upper_lds(a, L, &bad);
-  double* X = S;  // R^-1
+  chol_upper_lds(F, L, a, L, &bad);
+  double* X = F;
   upper_inverse_lds(a, X, L);
   for (int e = t; e < L * L; e += 256) {
     R[e] = a[e];
@@ -267,76 +265,143 @@ __global__ __launch_bounds__(256) void k_chol(const double* __restrict__ part, i
   }
   if (Rprev) {
     for (int e = t; e < L * L; e += 256) {
+      P[e] = Rprev[e];
+      Pi[e] = Rprev_inv[e];
+    }
+    __syncthreads();
+    for (int e = t; e < L * L; e += 256) {
       const int i = e / L, j = e % L;
       double v = 0.0, w = 0.0;
       for (int p = 0; p < L; ++p) {
-        v += a[i * L + p] * Rprev[p * L + j];
-        w += Rprev_inv[i * L + p] * X[p * L + j];
+        v += a[i * L + p] * P[p * L + j];
+        w += Pi[i * L + p] * X[p * L + j];
       }
       Rtot[e] = v;
       Rtot_inv[e] = w;
     }
+    __syncthreads();
+  }
+  if (sign) {
+    // top block of this pass's Q = Qtop R^-1
+    for (int e = t; e < L * L; e += 256) P[e] = Qtop[e];
+    __syncthreads();
+    for (int e = t; e < L * L; e += 256) {
+      const int i = e / L, j = e % L;
+      double v = 0.0;
+      for (int p = 0; p <= j; ++p) v += P[i * L + p] * X[p * L + j];
+      Pi[e] = v;
+    }
+    __syncthreads();
+    householder_signs_lu(Pi, L, sign);
   }
   if (t == 0 && status) *status = bad;
 }
 
-void launch_chol(const double* part, int nparts, const double* A, uint32_t row0, uint32_t l, double* R,
-                 double* Rinv, const double* Rprev, const double* Rprev_inv, double* Rtot, double* Rtot_inv,
-                 double* sign, int* status, hipStream_t s) {
+void launch_chol(const double* part, int nparts, uint32_t l, double* R, double* Rinv, const double* Rprev,
+                 const double* Rprev_inv, double* Rtot, double* Rtot_inv, const double* Qtop, double* sign,
+                 int* status, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
+  PCAONE_REQUIRE(!sign || Qtop, -1, "chol: Householder signs need the top block");
   nparts = reduce_parts(part, nparts, l * l, s);
-  const size_t shm = (size_t)3 * l * l * sizeof(double);
-  hipLaunchKernelGGL(k_chol, dim3(1), dim3(256), shm, s, part, nparts, A, row0, l, R, Rinv, Rprev, Rprev_inv,
-                     Rtot, Rtot_inv, sign, status);
+  const size_t shm = chol_lds_bytes(l);
+  if (shm > 65536)
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chol),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  hipLaunchKernelGGL(k_chol, dim3(1), dim3(256), shm, s, part, nparts, l, R, Rinv, Rprev, Rprev_inv, Rtot, Rtot_inv,
+                     Qtop, sign, status);
 }
 
 // ---------------------------------------------------------------------------
-// Q = A * Rinv * diag(sign), plus the partial Gram of Q
+// Q = A * Rinv * diag(sign), plus the partial Gram of Q (FP64 MFMA)
 // ---------------------------------------------------------------------------
-template <int E>
+// Each wave takes 16-row tiles: the tile (16 l contiguous doubles) is staged
+// in LDS, Q's tile = A_tile * Rinv' accumulates over K steps of 4, and the
+// accumulator element v of every column block is itself a 4-row slab in the
+// MFMA operand layout, so Q^T Q accumulates from registers.  A and Q may alias
+// (each wave reads its tile before writing it).
+template <int NB>
 __global__ __launch_bounds__(256) void k_apply_rinv(const double* __restrict__ A, const double* __restrict__ Rinv,
-                                                    const double* __restrict__ sign, double* __restrict__ Q,
-                                                    uint64_t nrows, uint32_t l, double* __restrict__ part) {
-  __shared__ double ri[kMaxL * kMaxL];
-  __shared__ double at[kTile * kMaxL];
-  __shared__ double qt[kTile * kMaxL];
-  const int L = (int)l;
-  for (int e = threadIdx.x; e < L * L; e += 256) ri[e] = Rinv[e] * (sign ? sign[e % L] : 1.0);
-  const uint64_t rpb = ceil_div(nrows, gridDim.x);
-  const uint64_t rs = blockIdx.x * rpb;
-  const uint64_t re = std::min<uint64_t>(rs + rpb, nrows);
-  double acc[E];
+                                                    const double* __restrict__ sign, double* Q, uint64_t nrows,
+                                                    uint32_t l, double* __restrict__ part) {
+  constexpr int W = NB * 16, NP = NB * (NB + 1) / 2;
+  static_assert(NP * 256 <= 4 * 16 * (W + 1), "the Gram reduction reuses the A tiles");
+  __shared__ double ri[W * W];
+  __shared__ double at[4][16 * (W + 1)];
+  double* red = &at[0][0];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kr = lane >> 4, ci = lane & 15;
+  for (int e = threadIdx.x; e < W * W; e += 256) {
+    const uint32_t k = e / W, j = e % W;
+    ri[e] = (k < l && j < l) ? Rinv[k * l + j] * (sign ? sign[j] : 1.0) : 0.0;
+  }
+  for (int e = threadIdx.x; e < 4 * 16 * (W + 1); e += 256) (&at[0][0])[e] = 0.0;
+  f64x4 g[NP];
 #pragma unroll
-  for (int q = 0; q < E; ++q) acc[q] = 0.0;
-  for (uint64_t r0 = rs; r0 < re; r0 += kTile) {
-    const int rows = (int)std::min<uint64_t>(kTile, re - r0);
+  for (int p = 0; p < NP; ++p) g[p] = f64x4{0.0, 0.0, 0.0, 0.0};
+  const uint64_t tiles = ceil_div(nrows, 16);
+  const uint64_t tpb = ceil_div(tiles, gridDim.x);
+  const uint64_t t0 = blockIdx.x * tpb, t1 = std::min<uint64_t>(t0 + tpb, tiles);
+  const uint32_t ksteps = (l + 3) / 4;
+  double* my = at[wv];
+  for (uint64_t tb = t0; tb < t1; tb += 4) {  // every wave runs the same number of iterations
+    const uint64_t tile = tb + wv;
+    const bool live = tile < t1;
+    const uint64_t r0 = tile * 16;
     __syncthreads();
-    for (int it = threadIdx.x; it < rows * L; it += 256) at[it] = A[r0 * L + it];
-    __syncthreads();
-    for (int it = threadIdx.x; it < rows * L; it += 256) {
-      const int r = it / L, c = it % L;
-      double v = 0.0;
-      for (int p = 0; p <= c; ++p) v = fma(at[r * L + p], ri[p * L + c], v);
-      qt[it] = v;
-      Q[r0 * L + it] = v;
+    if (live) {
+      const uint64_t rows = std::min<uint64_t>(16, nrows - r0);
+      for (uint32_t e = lane; e < 16 * l; e += 64) {
+        const uint32_t i = e / l, k = e % l;
+        my[i * (W + 1) + k] = (i < rows) ? A[r0 * l + e] : 0.0;
+      }
     }
     __syncthreads();
-    if (part) tile_xty<E>(qt, qt, rows, L, L, L, acc);
+    if (!live) continue;
+    f64x4 acc[NB];
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) acc[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (uint32_t ks = 0; ks < ksteps; ++ks) {
+      const int k = ks * 4 + kr;
+      const double a = my[ci * (W + 1) + k];
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb) acc[cb] = mfma64(a, ri[k * W + cb * 16 + ci], acc[cb]);
+    }
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) {
+      const uint32_t c = cb * 16 + ci;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint64_t r = r0 + mfma64_row(lane, v);
+        if (r < nrows && c < l) Q[r * l + c] = acc[cb][v];
+      }
+    }
+    if (part) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int p = 0;
+#pragma unroll
+        for (int ca = 0; ca < NB; ++ca)
+#pragma unroll
+          for (int cb = ca; cb < NB; ++cb, ++p) g[p] = mfma64(acc[ca][v], acc[cb][v], g[p]);
+      }
+    }
   }
-  if (part) store_acc<E>(part + (uint64_t)blockIdx.x * L * L, L, L, acc);
+  if (part) {
+    __syncthreads();
+    block_tiles_out<NB, NP>(g, true, l, red, part + (uint64_t)blockIdx.x * l * l);
+  }
 }
 
 void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, double* Q, uint64_t nrows,
                        uint32_t l, double* gram_part, int nblocks, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
-  const int E = entries_per_thread((int)(l * l));
-  switch (E) {
-#define PCAONE_E(EPT)                                                                                        \
-  case EPT:                                                                                                \
-    hipLaunchKernelGGL(k_apply_rinv<EPT>, dim3(nblocks), dim3(256), 0, s, A, Rinv, sign, Q, nrows, l, gram_part); \
+  switch ((l + 15) / 16) {
+#define PCAONE_NB(NBV)                                                                                          \
+  case NBV:                                                                                                   \
+    hipLaunchKernelGGL(k_apply_rinv<NBV>, dim3(nblocks), dim3(256), 0, s, A, Rinv, sign, Q, nrows, l, gram_part); \
     break;
-    PCAONE_E(1) PCAONE_E(2) PCAONE_E(4) PCAONE_E(8) PCAONE_E(16)
-#undef PCAONE_E
+    PCAONE_NB(1) PCAONE_NB(2) PCAONE_NB(3) PCAONE_NB(4)
+#undef PCAONE_NB
     default: throw Error(-1, "apply: unsupported l");
   }
 }
@@ -411,6 +476,7 @@ void launch_flip_stats(const double* Q, const double* omg2, uint64_t nrows, uint
 
 void launch_flip_apply(double* Q, double* omg2, uint64_t nrows, uint32_t l, const double* part, int nblocks,
                        double* colsum, double* zero_acc, hipStream_t s) {
+  nblocks = reduce_parts(part, nblocks, 3 * l, s);
   const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(nrows * l, 256), 1024);
   hipLaunchKernelGGL(k_flip_apply, dim3(blocks), dim3(256), 0, s, Q, omg2, nrows, l, part, nblocks, colsum,
                      zero_acc);
@@ -439,21 +505,49 @@ void launch_rows_times_small(const double* A, uint64_t n, uint32_t l, const doub
                      kc, scale, C);
 }
 
-// Cyclic Jacobi with round-robin pair ordering: every round applies n/2
-// disjoint rotations at once (rows, then columns), Numerical-Recipes angle.
+// Cyclic Jacobi with round-robin pair ordering: each round applies n/2
+// disjoint rotations at once (Numerical-Recipes angle).  A thread owns a 2 x 2
+// block {p1, q1} x {p2, q2} of the matrix (rows, then columns rotated in
+// place) or a row pair of the eigenvector matrix, so a round needs two
+// barriers.  LDS: a and v, l x l each.
+__device__ inline void rr_pair(int t, int round, int n, int& p, int& q) {
+  // position k of the round-robin table: 0 stays, the rest rotate
+  auto pos = [&](int k) { return k == 0 ? 0 : 1 + (k - 1 + round) % (n - 1); };
+  p = pos(t);
+  q = pos(n - 1 - t);
+  if (p > q) {
+    const int x = p;
+    p = q;
+    q = x;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part, int nparts, uint32_t l,
                                                  double* __restrict__ evals, double* __restrict__ W,
                                                  double* __restrict__ sinv, uint32_t kc) {
-  __shared__ double a[kMaxL * kMaxL];
-  __shared__ double v[kMaxL * kMaxL];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int L = (int)l, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double* a = reinterpret_cast<double*>(smem_raw);
+  double* v = a + L * L;
   __shared__ double cs[kMaxL / 2 + 1][2];
   __shared__ int pr[kMaxL / 2 + 1][2];
-  __shared__ int pos[kMaxL + 1];
-  __shared__ double red[256];
+  __shared__ double red[4][2];
   __shared__ int order[kMaxL];
-  const int t = threadIdx.x;
-  const int L = (int)l;
   const int n = (L % 2) ? L + 1 : L;  // dummy index L when odd
+  const int h = n / 2;
+  // this thread's fixed work items: rotation blocks (k1, k2) of a, then
+  // (pair, row) items of v, packed as hi << 8 | lo
+  int asg[12];
+#pragma unroll
+  for (int u = 0; u < 12; ++u) {
+    const int e = t + 256 * u;
+    if (e < h * h)
+      asg[u] = ((e / h) << 8) | (e % h);
+    else if (e < h * h + h * L)
+      asg[u] = (((e - h * h) / L) << 8) | ((e - h * h) % L);
+    else
+      asg[u] = 0;
+  }
   for (int e = t; e < L * L; e += 256) {
     double s = 0.0;
     for (int p = 0; p < nparts; ++p) s += part[(uint64_t)p * L * L + e];
@@ -462,39 +556,28 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
   }
   __syncthreads();
   for (int sweep = 0; sweep < 40; ++sweep) {
-    // convergence check
     double off = 0.0, dg = 0.0;
     for (int e = t; e < L * L; e += 256) {
       const double x = a[e] * a[e];
       if (e / L == e % L) dg += x; else off += x;
     }
-    red[t] = off;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if (t < o) red[t] += red[t + o];
-      __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_down(off, o);
+      dg += __shfl_down(dg, o);
     }
-    const double offs = red[0];
-    __syncthreads();
-    red[t] = dg;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if (t < o) red[t] += red[t + o];
-      __syncthreads();
+    if (lane == 0) {
+      red[wv][0] = off;
+      red[wv][1] = dg;
     }
-    const double dgs = red[0];
     __syncthreads();
+    const double offs = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+    const double dgs = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
     // off-diagonal Frobenius norm below 1e-13 of the diagonal's: FP64 rounding level
     if (offs <= 1e-26 * dgs || offs == 0.0) break;
     for (int round = 0; round < n - 1; ++round) {
-      if (t == 0) {
-        pos[0] = 0;
-        for (int k = 1; k < n; ++k) pos[k] = 1 + (k - 1 + round) % (n - 1);
-      }
-      __syncthreads();
-      if (t < n / 2) {
-        int p = pos[t], q = pos[n - 1 - t];
-        if (p > q) { const int tmp = p; p = q; q = tmp; }
+      if (t < h) {
+        int p, q;
+        rr_pair(t, round, n, p, q);
         double c = 1.0, s = 0.0;
         if (q < L) {
           const double apq = a[p * L + q];
@@ -511,48 +594,51 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
         cs[t][0] = c;
         cs[t][1] = s;
       }
-      __syncthreads();
-      for (int e = t; e < (n / 2) * L; e += 256) {  // rows
-        const int k = e / L, j = e % L;
-        const int p = pr[k][0], q = pr[k][1];
-        if (q < L) {
-          const double c = cs[k][0], s = cs[k][1];
-          const double x = a[p * L + j], y = a[q * L + j];
-          a[p * L + j] = c * x - s * y;
-          a[q * L + j] = s * x + c * y;
-        }
-      }
-      __syncthreads();
-      for (int e = t; e < (n / 2) * L; e += 256) {  // columns of a and v
-        const int k = e / L, i = e % L;
-        const int p = pr[k][0], q = pr[k][1];
-        if (q < L) {
-          const double c = cs[k][0], s = cs[k][1];
-          double x = a[i * L + p], y = a[i * L + q];
-          a[i * L + p] = c * x - s * y;
-          a[i * L + q] = s * x + c * y;
-          x = v[i * L + p];
-          y = v[i * L + q];
-          v[i * L + p] = c * x - s * y;
-          v[i * L + q] = s * x + c * y;
+      __syncthreads();  // also orders the convergence reads of red before its next write
+#pragma unroll
+      for (int u = 0; u < 12; ++u) {
+        const int e = t + 256 * u;
+        if (e >= h * h + h * L) break;
+        if (e < h * h) {
+          const int k1 = asg[u] >> 8, k2 = asg[u] & 255;
+          const int p1 = pr[k1][0], q1 = pr[k1][1], p2 = pr[k2][0], q2 = pr[k2][1];
+          const double c1 = cs[k1][0], s1 = cs[k1][1], c2 = cs[k2][0], s2 = cs[k2][1];
+          const bool r2 = q1 < L, col2 = q2 < L;
+          // rows: (p1, q1) <- (c x - s y, s x + c y)
+          const double x0 = a[p1 * L + p2], y0 = r2 ? a[q1 * L + p2] : 0.0;
+          const double x1 = col2 ? a[p1 * L + q2] : 0.0, y1 = (r2 && col2) ? a[q1 * L + q2] : 0.0;
+          const double rp0 = c1 * x0 - s1 * y0, rq0 = s1 * x0 + c1 * y0;
+          const double rp1 = c1 * x1 - s1 * y1, rq1 = s1 * x1 + c1 * y1;
+          // columns: (p2, q2)
+          a[p1 * L + p2] = c2 * rp0 - s2 * rp1;
+          if (col2) a[p1 * L + q2] = s2 * rp0 + c2 * rp1;
+          if (r2) {
+            a[q1 * L + p2] = c2 * rq0 - s2 * rq1;
+            if (col2) a[q1 * L + q2] = s2 * rq0 + c2 * rq1;
+          }
+        } else {
+          const int k = asg[u] >> 8, i = asg[u] & 255;
+          const int p = pr[k][0], q = pr[k][1];
+          if (q < L) {
+            const double c = cs[k][0], s = cs[k][1];
+            const double x = v[i * L + p], y = v[i * L + q];
+            v[i * L + p] = c * x - s * y;
+            v[i * L + q] = s * x + c * y;
+          }
         }
       }
       __syncthreads();
     }
   }
-  if (t == 0) {  // descending order of the eigenvalues
-    uint64_t used = 0;
-    for (int i = 0; i < L; ++i) {
-      int best = -1;
-      double bv = 0.0;
-      for (int j = 0; j < L; ++j)
-        if (!((used >> j) & 1ull) && (best < 0 || a[j * L + j] > bv)) {
-          best = j;
-          bv = a[j * L + j];
-        }
-      used |= 1ull << best;
-      order[i] = best;
+  // descending order of the eigenvalues (ties by index): rank of each diagonal entry
+  if (t < L) {
+    const double di = a[t * L + t];
+    int rank = 0;
+    for (int j = 0; j < L; ++j) {
+      const double dj = a[j * L + j];
+      rank += (dj > di) || (dj == di && j < t);
     }
+    order[rank] = t;
   }
   __syncthreads();
   for (int e = t; e < L * L; e += 256) {
@@ -570,45 +656,56 @@ void launch_sym_eig(const double* part, int nparts, uint32_t l, double* evals, d
                     uint32_t kc, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
   nparts = reduce_parts(part, nparts, l * l, s);
-  hipLaunchKernelGGL(k_sym_eig, dim3(1), dim3(256), 0, s, part, nparts, l, evals, W, sinv, kc);
+  const size_t shm = (size_t)2 * l * l * sizeof(double);
+  if (shm > 65536)
+    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sym_eig),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  hipLaunchKernelGGL(k_sym_eig, dim3(1), dim3(256), shm, s, part, nparts, l, evals, W, sinv, kc);
 }
 
 // ---------------------------------------------------------------------------
 // mev: M = X^T Y (k x k), value = mean_i ||M[:, i]||
 // ---------------------------------------------------------------------------
-template <int E>
+template <int NB>
 __global__ __launch_bounds__(256) void k_xty_partial(const double* __restrict__ X, const double* __restrict__ Y,
                                                      uint64_t n, uint32_t k, double* __restrict__ part) {
-  __shared__ double xt[kTile * kMaxL];
-  __shared__ double yt[kTile * kMaxL];
-  const int K = (int)k;
-  const uint64_t rpb = ceil_div(n, gridDim.x);
-  const uint64_t rs = blockIdx.x * rpb, re = std::min<uint64_t>(rs + rpb, n);
-  double acc[E];
+  constexpr int NP = NB * NB;
+  __shared__ double red[NP * 256];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kr = lane >> 4, ci = lane & 15;
+  f64x4 acc[NP];
 #pragma unroll
-  for (int q = 0; q < E; ++q) acc[q] = 0.0;
-  for (uint64_t r0 = rs; r0 < re; r0 += kTile) {
-    const int rows = (int)std::min<uint64_t>(kTile, re - r0);
-    __syncthreads();
-    for (int it = threadIdx.x; it < rows * K; it += 256) {
-      xt[it] = X[r0 * K + it];
-      yt[it] = Y[r0 * K + it];
+  for (int p = 0; p < NP; ++p) acc[p] = f64x4{0.0, 0.0, 0.0, 0.0};
+  const uint64_t steps = ceil_div(n, 4);
+  const uint64_t nw = (uint64_t)gridDim.x * 4, wid = (uint64_t)blockIdx.x * 4 + wv;
+  const uint64_t sps = ceil_div(steps, nw);
+  const uint64_t s0 = wid * sps, s1 = std::min<uint64_t>(s0 + sps, steps);
+  for (uint64_t st = s0; st < s1; ++st) {
+    const uint64_t r = st * 4 + kr;
+    double x[NB], y[NB];
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) {
+      const uint32_t c = cb * 16 + ci;
+      const bool ok = r < n && c < k;
+      x[cb] = ok ? X[r * k + c] : 0.0;
+      y[cb] = ok ? Y[r * k + c] : 0.0;
     }
-    __syncthreads();
-    tile_xty<E>(xt, yt, rows, K, K, K, acc);
+#pragma unroll
+    for (int ca = 0; ca < NB; ++ca)
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb) acc[ca * NB + cb] = mfma64(x[ca], y[cb], acc[ca * NB + cb]);
   }
-  store_acc<E>(part + (uint64_t)blockIdx.x * K * K, K, K, acc);
+  block_tiles_out<NB, NP>(acc, false, k, red, part + (uint64_t)blockIdx.x * k * k);
 }
 
 void launch_xty_partial(const double* X, const double* Y, uint64_t n, uint32_t k, double* part, int nblocks,
                         hipStream_t s) {
   PCAONE_REQUIRE(k >= 1 && k <= kMaxL, -1, "k must be in [1, 64]");
-  const int E = entries_per_thread((int)(k * k));
-  switch (E) {
-#define PCAONE_E(EPT) \
-  case EPT: hipLaunchKernelGGL(k_xty_partial<EPT>, dim3(nblocks), dim3(256), 0, s, X, Y, n, k, part); break;
-    PCAONE_E(1) PCAONE_E(2) PCAONE_E(4) PCAONE_E(8) PCAONE_E(16)
-#undef PCAONE_E
+  switch ((k + 15) / 16) {
+#define PCAONE_NB(NBV) \
+  case NBV: hipLaunchKernelGGL(k_xty_partial<NBV>, dim3(nblocks), dim3(256), 0, s, X, Y, n, k, part); break;
+    PCAONE_NB(1) PCAONE_NB(2) PCAONE_NB(3) PCAONE_NB(4)
+#undef PCAONE_NB
     default: throw Error(-1, "xty: unsupported k");
   }
 }
