@@ -116,6 +116,36 @@ int pcaone_gpu_rsvd_set_flags(pcaone_gpu* h, int update, int standardize);
 int pcaone_gpu_rsvd_em(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
                        double* V, int* iters, double* diffs);
 
+/* ---- PCAngsd on genotype likelihoods ------------------------------------ */
+/* Replaces FileBeagle::read_all's likelihood parse (src/FileBeagle.cpp:99-100,
+ * 160-180): gl holds nsites x nsamples x per_site float32 likelihoods (the
+ * BEAGLE body, per_site = 3; or only the first two, per_site = 2).  Only the
+ * first two stay in HBM; the third is taken as 1 - P0 - P1. */
+int pcaone_gpu_load_gl(pcaone_gpu* h, const float* gl, uint32_t nsamples, uint32_t nsites, int per_site);
+/* emMAF_with_GL (src/Utils.cpp:501-531) from F = 0.25 (src/FileBeagle.cpp:127)
+ * until the RMS change of F is below tol (at most maxiter passes).  F (nsites,
+ * optional) and iters (optional) receive the result.  Required before
+ * pcaone_gpu_rsvd_init, which then builds the initial E of
+ * src/FileBeagle.cpp:133-143 in HBM. */
+int pcaone_gpu_gl_em_maf(pcaone_gpu* h, int maxiter, double tol, double* F, int* iters);
+/* Data::fit_with_pi's PCAngsd branch (src/Data.cpp:249-270) or, with
+ * standardize, Data::pcangsd_standardize_E (src/Data.cpp:301-343), from U
+ * (nsamples x k), S (k) and V (nsites x k, original site order), column-major.
+ * The result replaces the operator's matrix (as data->G in the reference).
+ * E (optional): nsites x nsamples float32, site-major; Dc (optional, only with
+ * standardize): the nsamples GRM diagonal sums. */
+int pcaone_gpu_pcangsd_update(pcaone_gpu* h, const double* U, const double* S, const double* V, int k,
+                              int standardize, float* E, double* Dc);
+/* run_pca_with_halko's EM branch for PCAngsd on BEAGLE input
+ * (src/Halko.cpp:294-331): a solve on the initial E, then E <- fit_with_pi(U,
+ * S, V) and a new solve until 1 - mev(V, Vpre) < tolem (at most maxiter); then
+ * pcangsd_standardize_E.  U, S, V: the last solve; iters, diffs (maxiter
+ * entries) optional; Dc (nsamples, optional); C (nsamples x nsamples,
+ * optional): the GRM E E^T / nsites with diagonal Dc / nsites written to the
+ * reference's .cov. */
+int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
+                       double* V, int* iters, double* diffs, double* Dc, double* C);
+
 /* ---- IRAM: the Spectra op and its driver ----------------------------------
  * Prepares the single-column Gram mat-vec on the loaded matrix (identity
  * SNP order; standardize / ploidy as for the RSVD operator).  Re-initialises

@@ -52,6 +52,11 @@ SIGNATURES = {
     "pcaone_gpu_allele_freq": (C.c_int, [C.c_void_p, _D, _I64]),
     "pcaone_gpu_load_dense_f32": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]),
     "pcaone_gpu_feature_means": (C.c_int, [C.c_void_p, _D]),
+    "pcaone_gpu_load_gl": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]),
+    "pcaone_gpu_gl_em_maf": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, C.POINTER(C.c_int)]),
+    "pcaone_gpu_pcangsd_update": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_int, C.c_int, C.POINTER(C.c_float), _D]),
+    "pcaone_gpu_pcangsd": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, _D, _D, _D,
+                                     C.POINTER(C.c_int), _D, _D, _D]),
     "pcaone_gpu_rsvd_init": (C.c_int, [C.c_void_p, C.POINTER(RsvdParams), _D, _I32]),
     "pcaone_gpu_rsvd_dims": (C.c_int, [C.c_void_p, _I64, _I64, _I64]),
     "pcaone_gpu_rsvd_perm": (C.c_int, [C.c_void_p, _I32]),
@@ -212,6 +217,49 @@ class Gpu:
         m, n = a.shape
         self._check(self._L.pcaone_gpu_load_dense_f32(self.h, _p(a, C.c_float), n, m))
         self.N, self.M = n, m
+
+    def load_gl(self, gl: np.ndarray):
+        """gl: (nsites, nsamples, 3 or 2) float32 genotype likelihoods (BEAGLE order)."""
+        a = np.ascontiguousarray(gl, dtype=np.float32)
+        m, n, per = a.shape
+        self._check(self._L.pcaone_gpu_load_gl(self.h, _p(a, C.c_float), n, m, per))
+        self.N, self.M = n, m
+
+    def gl_em_maf(self, maxiter: int = 100, tol: float = 1e-6):
+        """emMAF_with_GL; returns (F, iterations)."""
+        F = np.empty(self.M)
+        it = C.c_int(0)
+        self._check(self._L.pcaone_gpu_gl_em_maf(self.h, maxiter, tol, _p(F, C.c_double), C.byref(it)))
+        return F, it.value
+
+    def pcangsd_update(self, U, S, V, standardize: bool = False):
+        """fit_with_pi (PCAngsd) or pcangsd_standardize_E from U (N, k), S (k), V (M, k);
+        returns (E as (M, N) float32, Dc or None)."""
+        U = np.asfortranarray(U, dtype=np.float64)
+        V = np.asfortranarray(V, dtype=np.float64)
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        k = U.shape[1]
+        E = np.empty((self.M, self.N), dtype=np.float32)
+        Dc = np.empty(self.N) if standardize else None
+        self._check(self._L.pcaone_gpu_pcangsd_update(self.h, _p(U, C.c_double), _p(S, C.c_double),
+                                                       _p(V, C.c_double), k, int(standardize), _p(E, C.c_float),
+                                                       _p(Dc, C.c_double) if Dc is not None else None))
+        return E, Dc
+
+    def pcangsd(self, maxp: int, tol: float, maxiter: int = 100, tolem: float = 1e-5, grm: bool = True):
+        """PCAngsd EM (BEAGLE input); returns U, S, V, iterations, diffs, Dc, C (or None)."""
+        U = np.empty((self.N, self.k), order="F")
+        S = np.empty(self.k)
+        V = np.empty((self.M, self.k), order="F")
+        it = C.c_int(0)
+        diffs = np.zeros(max(maxiter, 1))
+        Dc = np.empty(self.N)
+        Cm = np.empty((self.N, self.N)) if grm else None
+        self._check(self._L.pcaone_gpu_pcangsd(self.h, maxp, tol, maxiter, tolem, _p(U, C.c_double),
+                                                _p(S, C.c_double), _p(V, C.c_double), C.byref(it),
+                                                _p(diffs, C.c_double), _p(Dc, C.c_double),
+                                                _p(Cm, C.c_double) if grm else None))
+        return U, S, V, it.value, diffs[:it.value], Dc, Cm
 
     def feature_means(self):
         mu = np.empty(self.M)

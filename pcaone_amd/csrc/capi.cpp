@@ -138,6 +138,24 @@ int pcaone_gpu_rsvd_em(pcaone_gpu* h, int maxp, double tol, int maxiter, double 
   return guarded(h, [&](pcaone::Engine& e) { e.rsvd_em(maxp, tol, maxiter, tolem, U, S, V, iters, diffs); });
 }
 
+int pcaone_gpu_load_gl(pcaone_gpu* h, const float* gl, uint32_t nsamples, uint32_t nsites, int per_site) {
+  return guarded(h, [&](pcaone::Engine& e) { e.load_gl(gl, nsamples, nsites, per_site); });
+}
+
+int pcaone_gpu_gl_em_maf(pcaone_gpu* h, int maxiter, double tol, double* F, int* iters) {
+  return guarded(h, [&](pcaone::Engine& e) { e.gl_em_maf(maxiter, tol, F, iters); });
+}
+
+int pcaone_gpu_pcangsd_update(pcaone_gpu* h, const double* U, const double* S, const double* V, int k,
+                              int standardize, float* E, double* Dc) {
+  return guarded(h, [&](pcaone::Engine& e) { e.pcangsd_update(U, S, V, k, standardize, E, Dc); });
+}
+
+int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
+                       double* V, int* iters, double* diffs, double* Dc, double* C) {
+  return guarded(h, [&](pcaone::Engine& e) { e.pcangsd(maxp, tol, maxiter, tolem, U, S, V, iters, diffs, Dc, C); });
+}
+
 int pcaone_gpu_gram_init(pcaone_gpu* h, int precision, int standardize, int ploidy) {
   return guarded(h, [&](pcaone::Engine& e) { e.gram_init(precision, standardize != 0, ploidy); });
 }

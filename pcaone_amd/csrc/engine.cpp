@@ -39,24 +39,6 @@ KernelTimer::~KernelTimer() {
   }
 }
 
-template <class F>
-void Engine::timed(const char* name, F&& launch) {
-  if (!timer.on) {
-    launch();
-    PCAONE_HIP(hipGetLastError());
-    return;
-  }
-  KernelTimer::Rec r{name, nullptr, nullptr};
-  PCAONE_HIP(hipEventCreate(&r.a));
-  PCAONE_HIP(hipEventCreate(&r.b));
-  PCAONE_HIP(hipEventRecord(r.a, stream_));
-  launch();
-  PCAONE_HIP(hipGetLastError());
-  PCAONE_HIP(hipEventRecord(r.b, stream_));
-  timer.pending.push_back(r);
-  if (timer.pending.size() > 4096) timer.resolve();
-}
-
 // ---------------------------------------------------------------------------
 // winSVD window schedule, src/Halko.cpp:159-223
 // ---------------------------------------------------------------------------
@@ -139,6 +121,7 @@ void Engine::load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps)
   }
   have_bed_ = true;
   src_ = SRC_BED;
+  gl_mode_ = have_glF_ = false;
 }
 
 // Replaces FileBin::read_all (src/FileBinary.cpp:21-30): nfeatures rows of
@@ -164,6 +147,7 @@ void Engine::load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfea
   counts_orig_.clear();
   have_bed_ = true;
   src_ = SRC_DENSE;
+  gl_mode_ = have_glF_ = false;
 }
 
 void Engine::feature_means(double* mu) const {
@@ -328,11 +312,19 @@ void Engine::build_layout() {
     d_perm_.reserve(M_);
     PCAONE_HIP(hipMemcpyAsync(d_perm_.p, perm_.data(), (size_t)M_ * 4, hipMemcpyHostToDevice, stream_));
     dns_.reserve((uint64_t)M_ * kg_);
-    timed("dense_layout", [&] { launch_dense_rows(rawf_.p, N_, M_, d_perm_.p, dns_.p, (uint32_t)kg_, stream_); });
     dnt_.reserve((uint64_t)nwin_ * N_ * kw_);
-    timed("dense_layout", [&] {
-      launch_dense_transpose(dns_.p, (uint32_t)kg_, M_, bs_, nwin_, N_, dnt_.p, (uint32_t)kw_, stream_);
-    });
+    if (gl_mode_) {
+      // PCAngsd: the initial E (posterior at pt = F) straight into the laid-out rows
+      PCAONE_REQUIRE(have_glF_, -3, "PCAngsd: estimate the allele frequencies (gl_em_maf) before rsvd_init");
+      PCAONE_HIP(hipMemsetAsync(dns_.p, 0, dns_.bytes(), stream_));
+      gl_fill_E(nullptr, nullptr, nullptr, 0, false, nullptr);
+    } else {
+      timed("dense_layout",
+            [&] { launch_dense_rows(rawf_.p, N_, M_, d_perm_.p, dns_.p, (uint32_t)kg_, stream_); });
+      timed("dense_layout", [&] {
+        launch_dense_transpose(dns_.p, (uint32_t)kg_, M_, bs_, nwin_, N_, dnt_.p, (uint32_t)kw_, stream_);
+      });
+    }
     F_lay_.resize(M_);
     for (uint32_t r = 0; r < M_; ++r) F_lay_[r] = F_orig_[perm_[r]];
     F_.reserve(M_);
@@ -1046,8 +1038,21 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
   if (!done) std::swap(Ucur_.p, Upre_.p);  // the last Ucur sits in Upre after the swap
   // V = Q U_B[:, :k] (rows in laid-out order), S = sqrt(eig(B B^T))[:k]
   timed("usv_rows", [&] { launch_rows_times_small(G_.p, M_, l_, W_.p, l_, k, nullptr, V_.p, stream_); });
-  std::vector<double> ev(l_), u, v;
+  std::vector<double> ev(l_);
   PCAONE_HIP(hipMemcpyAsync(ev.data(), evals_.p, l_ * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+  last_S_.assign(k, 0.0);
+  for (uint32_t c = 0; c < k; ++c) last_S_[c] = std::sqrt(std::max(ev[c], 0.0));
+  have_usv_ = true;
+  copy_usv(U, S, V);
+  if (epochs) *epochs = ep;
+}
+
+// U (N x k) and V (M x k, original feature order) column-major, S (k) of the
+// last computeUSV
+void Engine::copy_usv(double* U, double* S, double* V) {
+  const uint32_t k = (uint32_t)p_.k;
+  std::vector<double> u, v;
   if (U) {
     u.resize((size_t)N_ * k);
     PCAONE_HIP(hipMemcpyAsync(u.data(), Ucur_.p, u.size() * 8, hipMemcpyDeviceToHost, stream_));
@@ -1057,9 +1062,6 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
     PCAONE_HIP(hipMemcpyAsync(v.data(), V_.p, v.size() * 8, hipMemcpyDeviceToHost, stream_));
   }
   sync();
-  last_S_.assign(k, 0.0);
-  for (uint32_t c = 0; c < k; ++c) last_S_[c] = std::sqrt(std::max(ev[c], 0.0));
-  have_usv_ = true;
   if (S) std::memcpy(S, last_S_.data(), k * sizeof(double));
   if (U)
     for (uint32_t r = 0; r < N_; ++r)
@@ -1067,7 +1069,6 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
   if (V)
     for (uint32_t r = 0; r < M_; ++r)
       for (uint32_t c = 0; c < k; ++c) V[(size_t)c * M_ + perm_[r]] = v[(size_t)r * k + c];
-  if (epochs) *epochs = ep;
 }
 
 // The EM branch of run_pca_with_halko for --emu (src/Halko.cpp:294-336):

@@ -77,6 +77,13 @@ class Engine {
 
   void load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps);
   void load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfeatures);
+  // genotype likelihoods (PCAngsd): the E matrix is built in HBM from them
+  void load_gl(const float* gl, uint32_t nsamples, uint32_t nsites, int per_site);
+  void gl_em_maf(int maxiter, double tol, double* F, int* iters);
+  void pcangsd_update(const double* U, const double* S, const double* V, int k, int standardize, float* E,
+                      double* Dc);
+  void pcangsd(int maxp, double tol, int maxiter, double tolem, double* U, double* S, double* V, int* iters,
+               double* diffs, double* Dc, double* C);
   void allele_freq(double* F, int64_t* counts) const;
   void feature_means(double* mu) const;
 
@@ -122,6 +129,8 @@ class Engine {
   void set_omega_vec(const double* dev_v, const double* host_v);
   void apply_X(const double* x, double* v);
   void apply_Xt(const double* v, double* y);
+  void copy_usv(double* U, double* S, double* V);
+  void gl_fill_E(const double* U, const double* S, const double* V, uint32_t k, bool standardize, double* Dc);
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -137,6 +146,11 @@ class Engine {
   std::vector<double> mu_orig_;    // dense: per-feature mean (FileBin centring)
   std::vector<double> F_orig_;     // allele frequency, original order
   std::vector<int64_t> counts_orig_;
+  // genotype likelihoods: P [site][sample][2] FP32, F from emMAF_with_GL
+  bool gl_mode_ = false, have_glF_ = false;
+  DevBuf<float> glP_;
+  DevBuf<double> glF_, gl_part_, gl_Dc_;
+  std::vector<double> glF_host_;
 
   // ---- layout (depends on the permutation and the windowing) -------------
   bool have_op_ = false;
@@ -191,5 +205,25 @@ class Engine {
   DevBuf<float> mv_omega32_;
   DevBuf<unsigned char> mv_bg_, mv_bh_, mv_gpart_, mv_hpart_;
 };
+
+// Launch wrapper: with profiling on, HIP events on the handle's stream bracket
+// the launch and are resolved into per-name totals later.
+template <class F>
+void Engine::timed(const char* name, F&& launch) {
+  if (!timer.on) {
+    launch();
+    PCAONE_HIP(hipGetLastError());
+    return;
+  }
+  KernelTimer::Rec r{name, nullptr, nullptr};
+  PCAONE_HIP(hipEventCreate(&r.a));
+  PCAONE_HIP(hipEventCreate(&r.b));
+  PCAONE_HIP(hipEventRecord(r.a, stream_));
+  launch();
+  PCAONE_HIP(hipGetLastError());
+  PCAONE_HIP(hipEventRecord(r.b, stream_));
+  timer.pending.push_back(r);
+  if (timer.pending.size() > 4096) timer.resolve();
+}
 
 }  // namespace pcaone

@@ -220,6 +220,10 @@ struct GlE {
   double* dpart;         // [gridDim.y][nsamples] Dc partials, or null
 };
 void launch_gl_E(const GlE& e, hipStream_t s);
+// rows_per_block for launch_gl_E: at most 256 row blocks (Dc partials)
+uint32_t gl_rows_per_block(uint32_t nsites);
+// out[i] = sum_y part[y][i] (fixed order)
+void launch_sum_rows(const double* part, uint32_t ny, uint32_t n, double* out, hipStream_t s);
 // GRM C = E^T E / nsites with diagonal Dc / nsites (E rows = sites, ld stride)
 void launch_grm(const float* E, uint64_t ld, uint32_t nsamples, uint32_t nsites, const double* Dc, double* C,
                 hipStream_t s);

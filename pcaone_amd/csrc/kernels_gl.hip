@@ -11,6 +11,8 @@
 //                   E_ij = posterior(pt) - 2 F_j                 (src/Data.cpp:249-270)
 //   standardise:    E_ij / sqrt(2 F (1 - F)) and the GRM diagonal
 //                   Dc_i = sum_j sum_g (g - 2F)^2 post_g / (2F(1-F))  (src/Data.cpp:301-343)
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -59,28 +61,32 @@ uint32_t gl_em_maf_blocks(uint32_t nsites) { return (uint32_t)ceil_div(nsites, 4
 // E rows (laid-out row r holds site perm[r]) into out[r * ld + i]; with U == null
 // the prior is pt = F (initial E), else the PCAngsd update from U (N x k),
 // S (k), V (M x k, laid-out rows).  Optional standardisation and Dc partials
-// (dpart[blockIdx.y][i], rows of block y).
+// (dpart[blockIdx.y][i], rows of block y).  U_i S stays in registers (KMAX
+// bounds k at compile time).
+template <int KMAX>
 __global__ __launch_bounds__(256) void k_gl_E(GlE e) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t r0 = blockIdx.y * e.rows_per_block;
   const uint32_t r1 = min(r0 + e.rows_per_block, e.nsites);
   double dc = 0.0;
   if (i < e.nsamples) {
-    double us[64];
-    if (e.U)
-      for (uint32_t c = 0; c < e.k; ++c) us[c] = e.U[(uint64_t)i * e.k + c] * e.S[c];
+    double us[KMAX];
+#pragma unroll
+    for (int c = 0; c < KMAX; ++c) us[c] = (e.U && c < (int)e.k) ? e.U[(uint64_t)i * e.k + c] * e.S[c] : 0.0;
     for (uint32_t r = r0; r < r1; ++r) {
       const uint32_t site = e.perm ? (uint32_t)e.perm[r] : r;
-      const float* pr = e.P + (uint64_t)site * e.nsamples * 2;
+      const float2 pp = reinterpret_cast<const float2*>(e.P + (uint64_t)site * e.nsamples * 2)[i];
       const double F = e.F[site];
       double pt = F;
       if (e.U) {
         double dot = 0.0;
-        for (uint32_t c = 0; c < e.k; ++c) dot += us[c] * e.V[(uint64_t)r * e.k + c];
+#pragma unroll
+        for (int c = 0; c < KMAX; ++c)
+          if (c < (int)e.k) dot += us[c] * e.V[(uint64_t)r * e.k + c];
         pt = fmin(fmax((dot + 2.0 * F) / 2.0, 1e-4), 1.0 - 1e-4);
       }
       double p0, p1, p2;
-      const double post = gl_post(pr[2 * i], pr[2 * i + 1], pt, p0, p1, p2);
+      const double post = gl_post(pp.x, pp.y, pt, p0, p1, p2);
       double v = post - 2.0 * F;
       if (e.standardize) {
         const double norm = sqrt(2.0 * F * (1.0 - F));
@@ -99,7 +105,38 @@ __global__ __launch_bounds__(256) void k_gl_E(GlE e) {
 void launch_gl_E(const GlE& e, hipStream_t s) {
   PCAONE_REQUIRE(e.k <= 64, -1, "PCAngsd update supports k <= 64");
   dim3 grid((uint32_t)ceil_div(e.nsamples, 256), (uint32_t)ceil_div(e.nsites, e.rows_per_block));
-  hipLaunchKernelGGL(k_gl_E, grid, dim3(256), 0, s, e);
+  const uint32_t k = e.U ? e.k : 0;
+  if (k <= 8)
+    hipLaunchKernelGGL(k_gl_E<8>, grid, dim3(256), 0, s, e);
+  else if (k <= 16)
+    hipLaunchKernelGGL(k_gl_E<16>, grid, dim3(256), 0, s, e);
+  else if (k <= 32)
+    hipLaunchKernelGGL(k_gl_E<32>, grid, dim3(256), 0, s, e);
+  else
+    hipLaunchKernelGGL(k_gl_E<64>, grid, dim3(256), 0, s, e);
+}
+
+// out[i] = sum_y part[y][i] over ny rows, fixed order
+__global__ void k_sum_rows(const double* __restrict__ part, uint32_t ny, uint32_t n, double* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s0 = 0.0, s1 = 0.0;
+  uint32_t y = 0;
+  for (; y + 1 < ny; y += 2) {
+    s0 += part[(uint64_t)y * n + i];
+    s1 += part[(uint64_t)(y + 1) * n + i];
+  }
+  if (y < ny) s0 += part[(uint64_t)y * n + i];
+  out[i] = s0 + s1;
+}
+
+void launch_sum_rows(const double* part, uint32_t ny, uint32_t n, double* out, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_sum_rows, dim3((uint32_t)ceil_div(n, 256)), dim3(256), 0, s, part, ny, n, out);
+}
+
+uint32_t gl_rows_per_block(uint32_t nsites) {
+  return (uint32_t)std::max<uint64_t>(64, ceil_div(nsites, 256));
 }
 
 // C[a][b] = sum_r E[r][a] E[r][b] / M (FP64 accumulation of FP32 rows), with

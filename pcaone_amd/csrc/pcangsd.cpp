@@ -1,0 +1,197 @@
+// PCAngsd on genotype likelihoods (BEAGLE input): the likelihoods stay in HBM
+// as FP32, emMAF_with_GL runs on the device, and the E matrix the RSVD
+// operator contracts is regenerated in place (laid-out rows + windowed
+// transpose) at every EM iteration.  The operator runs on E exactly as on a
+// dense FP32 matrix, without centring (setFlags(false, false)).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "engine.hpp"
+
+namespace pcaone {
+
+// FileBeagle::read_all keeps the first two likelihoods of each sample
+// (src/FileBeagle.cpp:99-100, 160-180); gl is [site][sample][per_site].
+void Engine::load_gl(const float* gl, uint32_t nsamples, uint32_t nsites, int per_site) {
+  PCAONE_REQUIRE(gl && nsamples > 0 && nsites > 0, -1, "load_gl: empty input");
+  PCAONE_REQUIRE(per_site == 2 || per_site == 3, -1, "load_gl: per_site must be 2 or 3");
+  PCAONE_HIP(hipSetDevice(device_));
+  have_op_ = have_mv_ = false;
+  N_ = nsamples;
+  M_ = nsites;
+  std::vector<float> two;
+  const float* src = gl;
+  if (per_site == 3) {
+    two.resize((size_t)N_ * M_ * 2);
+    for (size_t e = 0; e < (size_t)N_ * M_; ++e) {
+      two[2 * e] = gl[3 * e];
+      two[2 * e + 1] = gl[3 * e + 1];
+    }
+    src = two.data();
+  }
+  glP_.reserve((uint64_t)N_ * M_ * 2);
+  PCAONE_HIP(hipMemcpyAsync(glP_.p, src, (uint64_t)N_ * M_ * 2 * sizeof(float), hipMemcpyHostToDevice, stream_));
+  sync();  // `two` goes out of scope
+  // the operator sees E uncentred: mean 0, F 0 for the epilogues
+  mu_orig_.assign(M_, 0.0);
+  F_orig_.assign(M_, 0.0);
+  counts_orig_.clear();
+  have_bed_ = true;
+  src_ = SRC_DENSE;
+  gl_mode_ = true;
+  have_glF_ = false;
+}
+
+// emMAF_with_GL (src/Utils.cpp:501-531) from F = 0.25 (src/FileBeagle.cpp:127):
+// F_j <- 1/(2N) sum_i posterior(F_j) until sqrt(sum (dF)^2 / M) < tol.
+void Engine::gl_em_maf(int maxiter, double tol, double* F, int* iters) {
+  PCAONE_REQUIRE(gl_mode_, -3, "gl_em_maf: no genotype likelihoods loaded");
+  PCAONE_REQUIRE(maxiter >= 1, -1, "gl_em_maf: maxiter must be >= 1");
+  PCAONE_HIP(hipSetDevice(device_));
+  have_op_ = false;
+  DevBuf<double> fa, fb, part;
+  fa.reserve(M_);
+  fb.reserve(M_);
+  const uint32_t nb = gl_em_maf_blocks(M_);
+  part.reserve(nb);
+  timed("gl_maf", [&] { launch_fill_f64(fa.p, M_, 0.25, stream_); });
+  std::vector<double> hp(nb);
+  int it = 0;
+  for (it = 1; it <= maxiter; ++it) {
+    timed("gl_maf", [&] { launch_gl_em_maf(glP_.p, N_, M_, fa.p, fb.p, part.p, stream_); });
+    PCAONE_HIP(hipMemcpyAsync(hp.data(), part.p, (size_t)nb * 8, hipMemcpyDeviceToHost, stream_));
+    sync();
+    double d2 = 0.0;
+    for (uint32_t b = 0; b < nb; ++b) d2 += hp[b];
+    std::swap(fa.p, fb.p);
+    if (std::sqrt(d2 / M_) < tol) break;
+  }
+  if (it > maxiter) it = maxiter;
+  glF_.reserve(M_);
+  PCAONE_HIP(hipMemcpyAsync(glF_.p, fa.p, (size_t)M_ * 8, hipMemcpyDeviceToDevice, stream_));
+  glF_host_.resize(M_);
+  PCAONE_HIP(hipMemcpyAsync(glF_host_.data(), fa.p, (size_t)M_ * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+  if (F) std::memcpy(F, glF_host_.data(), (size_t)M_ * 8);
+  if (iters) *iters = it;
+  have_glF_ = true;
+}
+
+// E into the laid-out rows (dns_) and the windowed transpose (dnt_): the
+// initial E when U == null, else the PCAngsd update from U (N x k row-major),
+// S (k) and V (M x k, laid-out rows), all device pointers.  With standardize
+// the rows are divided by sqrt(2F(1-F)) and Dc (device, N) receives the GRM
+// diagonal sums.
+void Engine::gl_fill_E(const double* U, const double* S, const double* V, uint32_t k, bool standardize,
+                       double* Dc) {
+  GlE e{};
+  e.P = glP_.p;
+  e.nsamples = N_;
+  e.nsites = M_;
+  e.perm = d_perm_.p;
+  e.F = glF_.p;
+  e.U = U;
+  e.S = S;
+  e.V = V;
+  e.k = k;
+  e.standardize = standardize ? 1 : 0;
+  e.out = dns_.p;
+  e.ld = kg_;
+  e.rows_per_block = gl_rows_per_block(M_);
+  const uint32_t ny = (uint32_t)ceil_div(M_, e.rows_per_block);
+  if (Dc) {
+    gl_part_.reserve((uint64_t)ny * N_);
+    e.dpart = gl_part_.p;
+  }
+  timed("gl_E", [&] { launch_gl_E(e, stream_); });
+  if (Dc) timed("gl_E", [&] { launch_sum_rows(gl_part_.p, ny, N_, Dc, stream_); });
+  timed("dense_layout", [&] {
+    launch_dense_transpose(dns_.p, (uint32_t)kg_, M_, bs_, nwin_, N_, dnt_.p, (uint32_t)kw_, stream_);
+  });
+}
+
+// fit_with_pi, PCAngsd branch (src/Data.cpp:249-270), or with standardize
+// pcangsd_standardize_E (src/Data.cpp:301-343), from host U (N x k), S (k),
+// V (M x k, original site order), all column-major.  Replaces the operator's
+// matrix like the reference's in-place update.  E (optional): M x N
+// site-major FP32; Dc (optional): N.
+void Engine::pcangsd_update(const double* U, const double* S, const double* V, int k, int standardize, float* E,
+                            double* Dc) {
+  PCAONE_REQUIRE(gl_mode_ && have_op_, -3, "pcangsd_update: load likelihoods, run gl_em_maf and rsvd_init first");
+  PCAONE_REQUIRE(U && S && V && k >= 1 && k <= 64, -1, "pcangsd_update: bad U / S / V / k");
+  PCAONE_HIP(hipSetDevice(device_));
+  std::vector<double> ur((size_t)N_ * k), vr((size_t)M_ * k);
+  for (uint32_t r = 0; r < N_; ++r)
+    for (int c = 0; c < k; ++c) ur[(size_t)r * k + c] = U[(size_t)c * N_ + r];
+  for (uint32_t r = 0; r < M_; ++r)
+    for (int c = 0; c < k; ++c) vr[(size_t)r * k + c] = V[(size_t)c * M_ + perm_[r]];
+  DevBuf<double> du, ds, dv;
+  du.reserve(ur.size());
+  ds.reserve(k);
+  dv.reserve(vr.size());
+  PCAONE_HIP(hipMemcpyAsync(du.p, ur.data(), ur.size() * 8, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(ds.p, S, (size_t)k * 8, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(dv.p, vr.data(), vr.size() * 8, hipMemcpyHostToDevice, stream_));
+  gl_Dc_.reserve(N_);
+  gl_fill_E(du.p, ds.p, dv.p, (uint32_t)k, standardize != 0, (standardize && Dc) ? gl_Dc_.p : nullptr);
+  std::vector<float> rows;
+  if (E) {
+    rows.resize((size_t)M_ * kg_);
+    PCAONE_HIP(hipMemcpyAsync(rows.data(), dns_.p, rows.size() * 4, hipMemcpyDeviceToHost, stream_));
+  }
+  if (standardize && Dc) PCAONE_HIP(hipMemcpyAsync(Dc, gl_Dc_.p, (size_t)N_ * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+  if (E)
+    for (uint32_t r = 0; r < M_; ++r)
+      std::memcpy(E + (size_t)perm_[r] * N_, rows.data() + (size_t)r * kg_, (size_t)N_ * 4);
+}
+
+// run_pca_with_halko's EM branch for PCAngsd (src/Halko.cpp:294-331): one
+// solve on the initial E, then E <- fit_with_pi(U, S, V) and a new solve until
+// 1 - mev(V, Vpre) < tolem; then pcangsd_standardize_E gives Dc and, when C is
+// given, the GRM C = E E^T / M with diagonal Dc / M (N x N, row-major).  U, S,
+// V are those of the last solve; the operator's matrix is left standardised.
+void Engine::pcangsd(int maxp, double tol, int maxiter, double tolem, double* U, double* S, double* V, int* iters,
+                     double* diffs, double* Dc, double* C) {
+  PCAONE_REQUIRE(gl_mode_ && have_op_, -3, "pcangsd: load likelihoods, run gl_em_maf and rsvd_init first");
+  PCAONE_REQUIRE(maxiter >= 0, -1, "pcangsd: maxiter must be >= 0");
+  PCAONE_HIP(hipSetDevice(device_));
+  const uint32_t k = (uint32_t)p_.k;
+  set_flags(false, false);
+  rsvd_usv(maxp, tol, nullptr, nullptr, nullptr, nullptr, nullptr);
+  int it = 0;
+  for (int i = 0; i < maxiter; ++i) {
+    PCAONE_HIP(hipMemcpyAsync(Vpre_.p, V_.p, (size_t)M_ * k * 8, hipMemcpyDeviceToDevice, stream_));
+    PCAONE_HIP(hipMemcpyAsync(Sdev_.p, last_S_.data(), (size_t)k * 8, hipMemcpyHostToDevice, stream_));
+    gl_fill_E(Ucur_.p, Sdev_.p, V_.p, k, false, nullptr);
+    rsvd_usv(maxp, tol, nullptr, nullptr, nullptr, nullptr, nullptr);
+    const int nb = gram_blocks(M_);
+    timed("mev", [&] { launch_xty_partial(V_.p, Vpre_.p, M_, k, part_.p, nb, stream_); });
+    timed("mev", [&] { launch_mev_finish(part_.p, nb, k, scal_.p, stream_); });
+    double m = 0.0;
+    PCAONE_HIP(hipMemcpyAsync(&m, scal_.p, 8, hipMemcpyDeviceToHost, stream_));
+    sync();
+    it = i + 1;
+    if (diffs) diffs[i] = 1.0 - m;
+    if (1.0 - m < tolem) break;
+  }
+  copy_usv(U, S, V);
+  if (Dc || C) {
+    gl_Dc_.reserve(N_);
+    PCAONE_HIP(hipMemcpyAsync(Sdev_.p, last_S_.data(), (size_t)k * 8, hipMemcpyHostToDevice, stream_));
+    gl_fill_E(Ucur_.p, Sdev_.p, V_.p, k, true, gl_Dc_.p);
+    if (Dc) PCAONE_HIP(hipMemcpyAsync(Dc, gl_Dc_.p, (size_t)N_ * 8, hipMemcpyDeviceToHost, stream_));
+    if (C) {
+      DevBuf<double> dc;
+      dc.reserve((uint64_t)N_ * N_);
+      timed("grm", [&] { launch_grm(dns_.p, kg_, N_, M_, gl_Dc_.p, dc.p, stream_); });
+      PCAONE_HIP(hipMemcpyAsync(C, dc.p, (size_t)N_ * N_ * 8, hipMemcpyDeviceToHost, stream_));
+      sync();  // dc is freed on return
+    }
+    sync();
+  }
+  if (iters) *iters = it;
+}
+
+}  // namespace pcaone
