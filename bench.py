@@ -34,6 +34,10 @@ METRIC = "PCA solve s + genotype GB/s: split-BF16 MFMA vs FP32 GPU at eig relerr
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per launch of k_packed_gemm_bf16 on this workload: rocprofv3
+# FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0) per launch over one split-BF16 solve,
+# the factors from a 512 MiB copy kernel (profiles/k_packed_gemm_bf16_c2_winsvd.md)
+TRAFFIC = 352.6e6
 
 
 def log(*a):
@@ -99,6 +103,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="2,000 x 20,000 quick check")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-snps", type=int, default=40000)
+    ap.add_argument("--arms", default="bf16x2,fp32,fp64",
+                    help="comma list of arms to run (profiling runs use bf16x2 alone); the JSON line needs all three")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design "
@@ -119,13 +125,19 @@ def main():
     pay = pa.sim_bed(wl["seed"], n, m, wl["npops"], wl["fst"], 0.05, 0.5, wl["miss"])
     log(f"generated {m} x {n} genotypes in {time.perf_counter() - t0:.1f} s")
     g = pa.Gpu(0)
-    g.load_bed(pay, n, m)
+    t0 = time.perf_counter()
+    g.load_bed(pay, n, m)  # host -> HBM over PCIe, plus the allele-frequency count
+    g.synchronize()
+    load_s = time.perf_counter() - t0
     bytes_per_epoch = m * ((n + 3) >> 2)
 
     arms = {}
+    want = set(args.arms.split(","))
     for name, prec, steps, warm in (("bf16x2", pa.PREC_SPLIT_BF16, args.steps, args.warmup),
                                     ("fp32", pa.PREC_FP32, max(1, args.steps), max(1, args.warmup)),
                                     ("fp64", pa.PREC_FP64, 1, 0)):
+        if name not in want:
+            continue
         for _ in range(warm):
             solve(g, pa, wl, prec)
         ts, res = [], None
@@ -136,6 +148,9 @@ def main():
         arms[name] = dict(times=ts, U=res[0], S=res[1], V=res[2], epochs=res[3])
         log(f"{name}: {np.mean(ts) * 1e3:.1f} ms/solve over {steps} solves, {res[3]} epochs")
 
+    if want != {"bf16x2", "fp32", "fp64"}:
+        log("partial run (--arms): no JSON line")
+        return
     ref = arms["fp64"]
     acc = {}
     npc = wl["npops"] - 1
@@ -151,13 +166,15 @@ def main():
     tprof, _, _, _, ep_prof = solve(g, pa, wl, pa.PREC_SPLIT_BF16)
     stats = g.kernel_stats()
     g.profiling(False)
-    gemm = {k: v for k, v in stats.items() if k in ("gemm_G", "gemm_H")}
-    dom = max(gemm, key=lambda k: gemm[k][1])
-    cnt, ms = gemm[dom]
-    flops = ep_prof * 2.0 * m * n * l  # algorithmic: one of the two 2*M*N*l products per epoch
+    # the dominant kernel is the split-BF16 contraction k_packed_gemm_bf16, launched
+    # for both products (gemm_G: X^T Omega per window segment, gemm_H: X G)
+    cnt = stats["gemm_G"][0] + stats["gemm_H"][0]
+    ms = stats["gemm_G"][1] + stats["gemm_H"][1]
+    flops = ep_prof * 4.0 * m * n * l  # algorithmic: two 2*M*N*l products per epoch
     achieved = flops / (ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": TRAFFIC,
+                "kernel": "k_packed_gemm_bf16 (gemm_G + gemm_H)",
                 "launches": cnt, "avg_launch_us": round(ms * 1e3 / cnt, 2),
                 "algorithmic_flops_per_launch": flops / cnt,
                 "kernel_ms_per_solve": {k: round(v[1], 3) for k, v in sorted(stats.items())},
@@ -183,6 +200,9 @@ def main():
                                                                       t_fp32 / 1e9, 3)},
         "fp64_arm": {"solve_s": round(float(np.mean(ref["times"])), 5)},
         "speedup_vs_fp32": round(t_fp32 / (t_bf / len(bf["times"])), 3),
+        "pcie_inclusive": {"load_bed_s": round(load_s, 4),
+                           "value_GBps": round(bf["epochs"] * bytes_per_epoch /
+                                               (t_bf / len(bf["times"]) + load_s) / 1e9, 3)},
         "accuracy_vs_fp64": acc,
         "roofline": roofline,
         "cpu_baseline": cpu,
