@@ -76,15 +76,23 @@ Engine::Engine(int device) : device_(device) {
   PCAONE_REQUIRE(device >= 0 && device < n, -1, "no such HIP device");
   PCAONE_HIP(hipSetDevice(device));
   PCAONE_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  PCAONE_HIP(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+  for (auto& e : ev_) PCAONE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (aux_) (void)hipStreamSynchronize(aux_);
   timer.resolve();
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (aux_) (void)hipStreamDestroy(aux_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+// The main stream waits for every aux-stream gather it consumes, so draining
+// it drains both.
 void Engine::sync() { PCAONE_HIP(hipStreamSynchronize(stream_)); }
 
 // ---------------------------------------------------------------------------
@@ -713,6 +721,25 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
   const uint32_t rows = r1 - r0;
   const uint32_t rpb = rows_per_block_(prec);
   uint32_t g_nsplit = 0;
+  if (b.corrG) {
+    CsrGather cg{};
+    cg.ptr = miss_snp_ptr_.p;
+    cg.idx = miss_snp_idx_.p;
+    cg.table = b.g2_f32 ? static_cast<const void*>(b.omega32) : static_cast<const void*>(b.omega);
+    cg.table_f32 = b.g2_f32;
+    cg.l = b.l;
+    cg.in_base = r0;
+    cg.stride_in = 0;
+    cg.out_base = r0;
+    cg.per = rows;
+    cg.ngroups = 1;
+    cg.out = b.corrG;
+    // Omega (and everything before it) is ready on the main stream
+    PCAONE_HIP(hipEventRecord(ev_[0], stream_));
+    PCAONE_HIP(hipStreamWaitEvent(aux_, ev_[0], 0));
+    timed_on(aux_, "miss_gather", [&] { launch_csr_gather(cg, aux_); });
+    PCAONE_HIP(hipEventRecord(ev_[1], aux_));
+  }
   if (!dense) {
     PackedGemm g{};
     g.A = bed_snp_.p + (uint64_t)r0 * row_bytes_;
@@ -749,21 +776,7 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
     g_nsplit = g.nsplit;
     timed("gemm_G", [&] { launch_dense_gemm(g, prec, (int)b.l, b.ntiles, stream_); });
   }
-  if (b.corrG) {
-    CsrGather cg{};
-    cg.ptr = miss_snp_ptr_.p;
-    cg.idx = miss_snp_idx_.p;
-    cg.table = b.g2_f32 ? static_cast<const void*>(b.omega32) : static_cast<const void*>(b.omega);
-    cg.table_f32 = b.g2_f32;
-    cg.l = b.l;
-    cg.in_base = r0;
-    cg.stride_in = 0;
-    cg.out_base = r0;
-    cg.per = rows;
-    cg.ngroups = 1;
-    cg.out = b.corrG;
-    timed("miss_gather", [&] { launch_csr_gather(cg, stream_); });
-  }
+  if (b.corrG) PCAONE_HIP(hipStreamWaitEvent(stream_, ev_[1], 0));
   FinalizeG fg = finalize_desc(w0, w1, b);
   fg.part = b.gpart;
   fg.nsplit = g_nsplit;
@@ -811,6 +824,25 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
   const uint32_t rpb = rows_per_block_(prec);
   const uint32_t nz = w1 - w0;
   uint32_t h_nsplit = 0;
+  if (b.corrH) {
+    CsrGather cg{};
+    cg.ptr = miss_win_ptr_.p;
+    cg.idx = miss_win_idx_.p;
+    cg.table = b.g2;
+    cg.table_f32 = b.g2_f32;
+    cg.l = b.l;
+    cg.in_base = (uint64_t)w0 * miss_rows_;
+    cg.stride_in = miss_rows_;
+    cg.out_base = (uint64_t)w0 * N_;
+    cg.per = N_;
+    cg.ngroups = nz;
+    cg.out = b.corrH;
+    // g'' of these windows is ready on the main stream
+    PCAONE_HIP(hipEventRecord(ev_[2], stream_));
+    PCAONE_HIP(hipStreamWaitEvent(aux_, ev_[2], 0));
+    timed_on(aux_, "miss_gather", [&] { launch_csr_gather(cg, aux_); });
+    PCAONE_HIP(hipEventRecord(ev_[3], aux_));
+  }
   if (!dense) {
     PackedGemm h{};
     h.A = bed_smp_.p + (uint64_t)w0 * npad_ * win_bytes_;
@@ -853,21 +885,7 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
     h_nsplit = h.nsplit;
     timed("gemm_H", [&] { launch_dense_gemm(h, prec, (int)b.l, b.ntiles, stream_); });
   }
-  if (b.corrH) {
-    CsrGather cg{};
-    cg.ptr = miss_win_ptr_.p;
-    cg.idx = miss_win_idx_.p;
-    cg.table = b.g2;
-    cg.table_f32 = b.g2_f32;
-    cg.l = b.l;
-    cg.in_base = (uint64_t)w0 * miss_rows_;
-    cg.stride_in = miss_rows_;
-    cg.out_base = (uint64_t)w0 * N_;
-    cg.per = N_;
-    cg.ngroups = nz;
-    cg.out = b.corrH;
-    timed("miss_gather", [&] { launch_csr_gather(cg, stream_); });
-  }
+  if (b.corrH) PCAONE_HIP(hipStreamWaitEvent(stream_, ev_[3], 0));
   FinalizeH fh{};
   fh.part = b.hpart;
   fh.part_is_f64 = b.part_f64;

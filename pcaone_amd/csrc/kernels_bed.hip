@@ -692,25 +692,27 @@ void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin,
 }
 
 // One wave per list row.  The row's indices are loaded 64 at a time (one per
-// lane) and handed out with lane shuffles; lane group grp (lpg = ceil(l / 2)
-// lanes, two columns per lane) takes entries grp, grp + ng, ... so a wave has
-// 4 ng table rows in flight.  The groups' sums combine in a fixed order.
-template <class T>
+// lane) and handed out with lane shuffles.  Lane group grp (lpg = ceil(l / V)
+// lanes, V consecutive columns per lane read as one vector load) takes
+// entries grp, grp + ng, ... so a wave has 4 ng table rows in flight.  The
+// groups' sums combine in a fixed order.
+template <class T, int V>
 __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
+  typedef T vec __attribute__((ext_vector_type(V)));
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int lpg = (int)((g.l + 1) >> 1);
+  const int lpg = (int)((g.l + V - 1) / V);
   const int ng = 64 / lpg;
   const int grp = lane / lpg, li = lane - grp * lpg;
-  const bool act = grp < ng;
-  const uint32_t c0 = 2 * li, c1 = 2 * li + 1;
-  const bool has0 = act && c0 < g.l, has1 = act && c1 < g.l;
+  const bool act = grp < ng && (uint32_t)(li * V) < g.l;
   const T* tab = reinterpret_cast<const T*>(g.table);
   const uint64_t total = (uint64_t)g.per * g.ngroups;
   for (uint64_t o = (uint64_t)blockIdx.x * 4 + wv; o < total; o += (uint64_t)gridDim.x * 4) {
     const uint64_t q = o / g.per;
     const uint64_t row = g.in_base + q * g.stride_in + (o - q * g.per);
     const int64_t p0 = g.ptr[row], p1 = g.ptr[row + 1];
-    double s0 = 0.0, s1 = 0.0;
+    double s[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) s[v] = 0.0;
     for (int64_t b = p0; b < p1; b += 64) {
       const int cnt = (int)(p1 - b < 64 ? p1 - b : 64);
       const int32_t mine = lane < cnt ? g.idx[b + lane] : 0;
@@ -721,28 +723,34 @@ __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
         for (int u = 0; u < 4; ++u) {
           const int e = e0 + u * ng + grp;
           ok[u] = act && e < cnt;
-          j[u] = __shfl(mine, ok[u] ? e : 0);
+          j[u] = __shfl(mine, (e < cnt) ? e : 0);
         }
-        double v0[4], v1[4];
+        vec x[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const uint64_t base = (uint64_t)j[u] * g.l;
-          v0[u] = (ok[u] && has0) ? (double)tab[base + c0] : 0.0;
-          v1[u] = (ok[u] && has1) ? (double)tab[base + c1] : 0.0;
+          if (ok[u]) {
+            x[u] = *reinterpret_cast<const vec*>(tab + (uint64_t)j[u] * g.l + li * V);
+          } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) x[u][v] = T(0);
+          }
         }
-        s0 += (v0[0] + v0[1]) + (v0[2] + v0[3]);
-        s1 += (v1[0] + v1[1]) + (v1[2] + v1[3]);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          s[v] += ((double)x[0][v] + (double)x[1][v]) + ((double)x[2][v] + (double)x[3][v]);
       }
     }
-    double t0 = s0, t1 = s1;
-    for (int gg = 1; gg < ng; ++gg) {
-      t0 += __shfl(s0, gg * lpg + li);
-      t1 += __shfl(s1, gg * lpg + li);
-    }
-    if (grp == 0) {
+    double t[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) t[v] = s[v];
+    for (int gg = 1; gg < ng; ++gg)
+#pragma unroll
+      for (int v = 0; v < V; ++v) t[v] += __shfl(s[v], gg * lpg + li);
+    if (grp == 0 && act) {
       const uint64_t orow = g.out_base + o;
-      if (c0 < g.l) g.out[orow * g.l + c0] = t0;
-      if (c1 < g.l) g.out[orow * g.l + c1] = t1;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        if ((uint32_t)(li * V + v) < g.l) g.out[orow * g.l + li * V + v] = t[v];
     }
   }
 }
@@ -752,10 +760,20 @@ void launch_csr_gather(const CsrGather& g, hipStream_t s) {
   const uint64_t total = (uint64_t)g.per * g.ngroups;
   if (!total) return;
   const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(total, 4), 1u << 20);
-  if (g.table_f32)
-    hipLaunchKernelGGL(k_csr_gather<float>, dim3(blocks), dim3(256), 0, s, g);
-  else
-    hipLaunchKernelGGL(k_csr_gather<double>, dim3(blocks), dim3(256), 0, s, g);
+  // 16-byte vector loads need rows that are whole vectors (tables are hipMalloc'd)
+  if (g.table_f32) {
+    if (g.l % 4 == 0)
+      hipLaunchKernelGGL((k_csr_gather<float, 4>), dim3(blocks), dim3(256), 0, s, g);
+    else if (g.l % 2 == 0)
+      hipLaunchKernelGGL((k_csr_gather<float, 2>), dim3(blocks), dim3(256), 0, s, g);
+    else
+      hipLaunchKernelGGL((k_csr_gather<float, 1>), dim3(blocks), dim3(256), 0, s, g);
+  } else {
+    if (g.l % 2 == 0)
+      hipLaunchKernelGGL((k_csr_gather<double, 2>), dim3(blocks), dim3(256), 0, s, g);
+    else
+      hipLaunchKernelGGL((k_csr_gather<double, 1>), dim3(blocks), dim3(256), 0, s, g);
+  }
 }
 
 __global__ void k_cast_f32(const double* __restrict__ src, float* __restrict__ dst, uint64_t n) {
