@@ -146,6 +146,22 @@ int pcaone_gpu_pcangsd_update(pcaone_gpu* h, const double* U, const double* S, c
 int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
                        double* V, int* iters, double* diffs, double* Dc, double* C);
 
+/* ---- LD r^2 (run_ld_stuff --print-r2 on a .residuals matrix) ------------ */
+/* divide_pos_by_window (src/LD.cpp:125-139) from the positions of nsnps SNPs
+ * and each chromosome's last SNP index (0-based, increasing, the last one
+ * nsnps - 1; SNPld::end_pos, src/LD.hpp:8-15).  ws and we (capacity nsnps)
+ * receive each window's lead SNP and its size (lead included); *nwin the
+ * number of windows.  Host only. */
+int pcaone_ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end, uint32_t nchr, int32_t ld_bp,
+                      int32_t* ws, int32_t* we, uint32_t* nwin);
+/* ld_r2_big (src/LD.cpp:419-442) on the matrix loaded by
+ * pcaone_gpu_load_dense_f32 (centred per feature as FileBin::read_all):
+ * r^2 of the pairs (ws[w], ws[w] + j), j = 1 .. we[w] - 1, window after
+ * window, the line order of the reference's .ld.gz.  With r2 == NULL only
+ * *npairs is set; otherwise r2 must hold capacity >= npairs doubles. */
+int pcaone_gpu_ld_r2(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2,
+                     uint64_t capacity, uint64_t* npairs);
+
 /* ---- IRAM: the Spectra op and its driver ----------------------------------
  * Prepares the single-column Gram mat-vec on the loaded matrix (identity
  * SNP order; standardize / ploidy as for the RSVD operator).  Re-initialises

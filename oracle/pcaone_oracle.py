@@ -20,6 +20,14 @@ the reference's behaviour, of:
 * ``mev``                                      (src/Utils.cpp:192-198)
 * output scaling ``eigvals = S^2/M*ploidy``    (src/Halko.cpp:338-343,
   src/Data.cpp:177-194)
+* EMU imputation and EM loop                   (src/Data.cpp:272-287,
+  src/Halko.cpp:294-336)
+* FileBin centring                             (src/FileBinary.cpp:21-30)
+* emMAF_with_GL, initial E, PCAngsd update, pcangsd_standardize_E, GRM
+                                               (src/Utils.cpp:501-531,
+  src/FileBeagle.cpp:121-143, src/Data.cpp:249-343, src/Halko.cpp:317-331)
+* LD windows and r^2                           (src/LD.cpp:20-23, 125-139,
+  419-442)
 
 The third-party pieces restated here come from libstdc++ (GCC 11.4, the
 toolchain the reference builds with): the Park-Miller minimal-standard LCG
@@ -552,6 +560,53 @@ def pcangsd_halko(P0, P1, k: int, os: int, maxp: int, tol: float, maxiter: int =
     C = E @ E.T / m
     C[np.diag_indices(n)] = Dc / m
     return op, F, it, diffs, Dc, C
+
+
+# ----------------------------------------------------------------------------
+# LD r^2 on a dense (residual) matrix
+# ----------------------------------------------------------------------------
+def ld_windows(pos, chr_end, ld_bp: int):
+    """divide_pos_by_window (src/LD.cpp:125-139), loop for loop: every SNP whose
+    position differs from its chromosome's last position leads a window of
+    the SNPs j >= i of that chromosome with pos[j] - pos[i] <= ld_bp.
+    chr_end: 0-based index of each chromosome's last SNP.  Returns (ws, we),
+    we counting the lead itself."""
+    pos = [int(p) for p in pos]
+    ws, we = [], []
+    c = 0
+    for i in range(len(pos)):
+        if pos[i] == pos[chr_end[c]]:
+            c += 1
+            continue
+        j = i
+        while j <= chr_end[c]:
+            if pos[j] - pos[i] > ld_bp:
+                break
+            j += 1
+        ws.append(i)
+        we.append(j - i)
+    return np.array(ws, dtype=np.int64), np.array(we, dtype=np.int64)
+
+
+def ld_r2(featmajor: np.ndarray, ws, we) -> np.ndarray:
+    """ld_r2_big (src/LD.cpp:419-442) on FileBin::read_all's matrix
+    (src/FileBinary.cpp:21-30: float32 -> double, each feature centred by its
+    mean): r = <x_i, x_k> (sds_i sds_k / (N - 1)), sds = 1 / sqrt(sum x^2 /
+    (N - 1)) (calc_sds, :20-23).  r^2 for the pairs (ws[w], ws[w] + j),
+    j = 1 .. we[w] - 1, window after window."""
+    X = featmajor.astype(np.float64)
+    X = X - X.mean(axis=1, keepdims=True)
+    n = X.shape[1]
+    df = 1.0 / (n - 1)
+    sds = 1.0 / np.sqrt((X * X).sum(axis=1) * df)
+    out = []
+    for i, e in zip(ws, we):
+        if e <= 1:
+            continue
+        k = np.arange(i + 1, i + e)
+        r = (X[k] @ X[i]) * (sds[i] * sds[k] * df)
+        out.append(r * r)
+    return np.concatenate(out) if out else np.zeros(0)
 
 
 def principal_angles_max(A: np.ndarray, B: np.ndarray) -> float:

@@ -52,6 +52,9 @@ SIGNATURES = {
     "pcaone_gpu_allele_freq": (C.c_int, [C.c_void_p, _D, _I64]),
     "pcaone_gpu_load_dense_f32": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]),
     "pcaone_gpu_feature_means": (C.c_int, [C.c_void_p, _D]),
+    "pcaone_ld_windows": (C.c_int, [_I32, C.c_uint32, _I32, C.c_uint32, C.c_int32, _I32, _I32,
+                                    C.POINTER(C.c_uint32)]),
+    "pcaone_gpu_ld_r2": (C.c_int, [C.c_void_p, _I32, _I32, C.c_uint32, _D, C.c_uint64, C.POINTER(C.c_uint64)]),
     "pcaone_gpu_load_gl": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]),
     "pcaone_gpu_gl_em_maf": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, C.POINTER(C.c_int)]),
     "pcaone_gpu_pcangsd_update": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_int, C.c_int, C.POINTER(C.c_float), _D]),
@@ -159,6 +162,20 @@ def write_bin(path: str | os.PathLike, featmajor: np.ndarray):
         f.write(a.tobytes())
 
 
+def ld_windows(pos: np.ndarray, chr_end: np.ndarray, ld_bp: int):
+    """divide_pos_by_window -> (ws, we) int32 arrays."""
+    pos = np.ascontiguousarray(pos, dtype=np.int32)
+    ce = np.ascontiguousarray(chr_end, dtype=np.int32)
+    ws = np.empty(len(pos), dtype=np.int32)
+    we = np.empty(len(pos), dtype=np.int32)
+    nw = C.c_uint32(0)
+    rc = lib().pcaone_ld_windows(_p(pos, C.c_int32), len(pos), _p(ce, C.c_int32), len(ce), int(ld_bp),
+                                 _p(ws, C.c_int32), _p(we, C.c_int32), C.byref(nw))
+    if rc != 0:
+        raise GpuError(rc, "pcaone_ld_windows failed")
+    return ws[:nw.value].copy(), we[:nw.value].copy()
+
+
 def fancy_plan(bands: int, pi: int, bandsize: int):
     """Host schedule of one winSVD computeGandH call -> (acc, update, zero, reset, new_bandsize)."""
     bs = C.c_uint64(bandsize)
@@ -260,6 +277,18 @@ class Gpu:
                                                 _p(diffs, C.c_double), _p(Dc, C.c_double),
                                                 _p(Cm, C.c_double) if grm else None))
         return U, S, V, it.value, diffs[:it.value], Dc, Cm
+
+    def ld_r2(self, ws: np.ndarray, we: np.ndarray) -> np.ndarray:
+        """r^2 of the window pairs on the loaded dense matrix, window after window."""
+        ws = np.ascontiguousarray(ws, dtype=np.int32)
+        we = np.ascontiguousarray(we, dtype=np.int32)
+        n = C.c_uint64(0)
+        self._check(self._L.pcaone_gpu_ld_r2(self.h, _p(ws, C.c_int32), _p(we, C.c_int32), len(ws), None, 0,
+                                              C.byref(n)))
+        out = np.empty(n.value)
+        self._check(self._L.pcaone_gpu_ld_r2(self.h, _p(ws, C.c_int32), _p(we, C.c_int32), len(ws),
+                                              _p(out, C.c_double), n.value, C.byref(n)))
+        return out
 
     def feature_means(self):
         mu = np.empty(self.M)

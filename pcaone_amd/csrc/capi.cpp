@@ -156,6 +156,27 @@ int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double 
   return guarded(h, [&](pcaone::Engine& e) { e.pcangsd(maxp, tol, maxiter, tolem, U, S, V, iters, diffs, Dc, C); });
 }
 
+int pcaone_ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end, uint32_t nchr, int32_t ld_bp,
+                      int32_t* ws, int32_t* we, uint32_t* nwin) {
+  try {
+    const uint32_t n = pcaone::ld_windows(pos, nsnps, chr_end, nchr, ld_bp, ws, we);
+    if (nwin) *nwin = n;
+    return PCAONE_OK;
+  } catch (const pcaone::Error& e) {
+    return e.code < 0 ? e.code : PCAONE_EINVAL;
+  } catch (...) {
+    return PCAONE_EINVAL;
+  }
+}
+
+int pcaone_gpu_ld_r2(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2,
+                     uint64_t capacity, uint64_t* npairs) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    const uint64_t n = e.ld_r2(ws, we, nwin, r2, capacity);
+    if (npairs) *npairs = n;
+  });
+}
+
 int pcaone_gpu_gram_init(pcaone_gpu* h, int precision, int standardize, int ploidy) {
   return guarded(h, [&](pcaone::Engine& e) { e.gram_init(precision, standardize != 0, ploidy); });
 }

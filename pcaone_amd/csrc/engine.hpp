@@ -71,6 +71,10 @@ struct SegBufs {
   const double* mval = nullptr;
 };
 
+// divide_pos_by_window (src/LD.cpp:125-139); returns the number of windows
+uint32_t ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end, uint32_t nchr, int32_t ld_bp,
+                    int32_t* ws, int32_t* we);
+
 class Engine {
  public:
   explicit Engine(int device);
@@ -85,6 +89,8 @@ class Engine {
                       double* Dc);
   void pcangsd(int maxp, double tol, int maxiter, double tolem, double* U, double* S, double* V, int* iters,
                double* diffs, double* Dc, double* C);
+  // LD r^2 over windows on the loaded dense matrix
+  uint64_t ld_r2(const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2, uint64_t capacity);
   void allele_freq(double* F, int64_t* counts) const;
   void feature_means(double* mu) const;
 

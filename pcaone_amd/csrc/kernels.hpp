@@ -224,6 +224,21 @@ void launch_gl_E(const GlE& e, hipStream_t s);
 uint32_t gl_rows_per_block(uint32_t nsites);
 // out[i] = sum_y part[y][i] (fixed order)
 void launch_sum_rows(const double* part, uint32_t ny, uint32_t n, double* out, hipStream_t s);
+// ---- LD r^2 on a dense feature-major matrix (src/LD.cpp:20-23, 419-442) -----
+void launch_ld_sds(const float* X, uint32_t n, uint32_t m, const double* mu, double* sds, hipStream_t s);
+struct LdTiles {
+  const float* X;          // [m][n] feature-major FP32
+  const double* mu;        // [m] feature means
+  const double* sds;       // [m] 1 / sd
+  uint32_t n, m;
+  const int2* tiles;       // (row block, column block) pairs, 64 x 64
+  const int32_t* win_of_row;  // lead row -> window, or -1
+  const int32_t* we;       // [nwin] window sizes (lead included)
+  const int64_t* off;      // [nwin] first output of each window
+  double* r2;              // window-ordered pair values
+};
+void launch_ld_tiles(const LdTiles& t, uint32_t ntiles, hipStream_t s);
+
 // GRM C = E^T E / nsites with diagonal Dc / nsites (E rows = sites, ld stride)
 void launch_grm(const float* E, uint64_t ld, uint32_t nsamples, uint32_t nsites, const double* Dc, double* C,
                 hipStream_t s);

@@ -1,0 +1,107 @@
+// LD r^2 (the --print-r2 path of run_ld_stuff on a .residuals matrix,
+// src/LD.cpp:450-470): windows from positions on the host, the band of the
+// Gram on the device.
+#include <algorithm>
+#include <cstring>
+
+#include "engine.hpp"
+
+namespace pcaone {
+
+// divide_pos_by_window (src/LD.cpp:125-139), step for step: a SNP leads a
+// window unless its position equals its chromosome's last position (which
+// also moves on to the next chromosome); the window holds the SNPs j >= i of
+// that chromosome with pos[j] - pos[i] <= ld_bp.
+uint32_t ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end, uint32_t nchr, int32_t ld_bp,
+                    int32_t* ws, int32_t* we) {
+  PCAONE_REQUIRE(pos && chr_end && ws && we && nsnps > 0 && nchr > 0, -1, "ld_windows: empty input");
+  for (uint32_t c = 0; c < nchr; ++c)
+    PCAONE_REQUIRE(chr_end[c] >= 0 && (uint32_t)chr_end[c] < nsnps && (c == 0 || chr_end[c] > chr_end[c - 1]), -1,
+                   "ld_windows: chromosome ends must increase within [0, nsnps)");
+  PCAONE_REQUIRE((uint32_t)chr_end[nchr - 1] == nsnps - 1, -1, "ld_windows: the last chromosome must end at nsnps-1");
+  uint32_t nwin = 0, c = 0;
+  for (uint32_t i = 0; i < nsnps; ++i) {
+    PCAONE_REQUIRE(c < nchr, -1, "ld_windows: positions run past the last chromosome");
+    if (pos[i] == pos[chr_end[c]]) {
+      ++c;
+      continue;
+    }
+    uint32_t j = i;
+    for (; j <= (uint32_t)chr_end[c]; ++j)
+      if ((int64_t)pos[j] - pos[i] > ld_bp) break;
+    ws[nwin] = (int32_t)i;
+    we[nwin] = (int32_t)(j - i);
+    ++nwin;
+  }
+  return nwin;
+}
+
+// ld_r2_big (src/LD.cpp:419-442) on FileBin::read_all's centred matrix: the
+// r^2 of pairs (ws[w], ws[w] + j), j = 1 .. we[w] - 1, window after window.
+// Returns the number of pairs (written when r2 != null and capacity allows).
+uint64_t Engine::ld_r2(const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2, uint64_t capacity) {
+  PCAONE_REQUIRE(have_bed_ && src_ == SRC_DENSE && !gl_mode_, -3, "ld_r2: load a dense (.residuals) matrix first");
+  PCAONE_REQUIRE(N_ >= 2, -1, "ld_r2: needs at least two samples");
+  PCAONE_REQUIRE(ws && we, -1, "ld_r2: null windows");
+  PCAONE_HIP(hipSetDevice(device_));
+  std::vector<int64_t> off(nwin);
+  std::vector<int32_t> win_of_row(M_, -1);
+  uint64_t total = 0;
+  for (uint32_t w = 0; w < nwin; ++w) {
+    PCAONE_REQUIRE(ws[w] >= 0 && (uint32_t)ws[w] < M_ && we[w] >= 1 && (uint64_t)ws[w] + we[w] <= M_, -1,
+                   "ld_r2: window out of range");
+    PCAONE_REQUIRE(w == 0 || ws[w] > ws[w - 1], -1, "ld_r2: lead SNPs must increase");
+    off[w] = (int64_t)total;
+    total += (uint64_t)(we[w] - 1);
+    win_of_row[ws[w]] = (int32_t)w;
+  }
+  if (!r2) return total;
+  PCAONE_REQUIRE(capacity >= total, -1, "ld_r2: output too small");
+  if (total == 0) return 0;
+  // tiles of the band: row block I, column blocks I .. (last column of I's windows) / 64
+  const uint32_t nb = (uint32_t)ceil_div(M_, 64);
+  std::vector<int32_t> last(nb, -1);
+  for (uint32_t w = 0; w < nwin; ++w) {
+    if (we[w] <= 1) continue;
+    const uint32_t I = (uint32_t)ws[w] / 64;
+    last[I] = std::max<int32_t>(last[I], ws[w] + we[w] - 1);
+  }
+  std::vector<int2> tiles;
+  for (uint32_t I = 0; I < nb; ++I)
+    if (last[I] >= 0)
+      for (uint32_t J = I; J <= (uint32_t)last[I] / 64; ++J) tiles.push_back(int2{(int)I, (int)J});
+  DevBuf<double> mu, sds, out;
+  DevBuf<int2> dt;
+  DevBuf<int32_t> dwin, dwe;
+  DevBuf<int64_t> doff;
+  mu.reserve(M_);
+  sds.reserve(M_);
+  out.reserve(total);
+  dt.reserve(tiles.size());
+  dwin.reserve(M_);
+  dwe.reserve(nwin);
+  doff.reserve(nwin);
+  PCAONE_HIP(hipMemcpyAsync(mu.p, mu_orig_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(dt.p, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(dwin.p, win_of_row.data(), (size_t)M_ * 4, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(dwe.p, we, (size_t)nwin * 4, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(doff.p, off.data(), (size_t)nwin * 8, hipMemcpyHostToDevice, stream_));
+  timed("ld_sds", [&] { launch_ld_sds(rawf_.p, N_, M_, mu.p, sds.p, stream_); });
+  LdTiles t{};
+  t.X = rawf_.p;
+  t.mu = mu.p;
+  t.sds = sds.p;
+  t.n = N_;
+  t.m = M_;
+  t.tiles = dt.p;
+  t.win_of_row = dwin.p;
+  t.we = dwe.p;
+  t.off = doff.p;
+  t.r2 = out.p;
+  timed("ld_tile", [&] { launch_ld_tiles(t, (uint32_t)tiles.size(), stream_); });
+  PCAONE_HIP(hipMemcpyAsync(r2, out.p, total * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+  return total;
+}
+
+}  // namespace pcaone

@@ -34,7 +34,8 @@ def gpu():
 
 
 def golden_cases():
-    return sorted(GOLDEN.glob("*.npz"))
+    """The PCA fixtures (LD fixtures, ld_*.npz, have their own tests)."""
+    return sorted(p for p in GOLDEN.glob("*.npz") if not p.name.startswith("ld_"))
 
 
 def load_golden(path):

@@ -5,10 +5,10 @@ reference sources by `make -f oracle/ref.mk`):
 
     python tests/golden/make_golden.py
 
-Inputs are seeded Balding-Nichols PLINK sets from pcaone_sim_bed; each case
-stores the .bed payload and the reference's printed outputs (.eigvals,
-.sigvals, .eigvecs, .loadings; 6 significant digits, src/Data.cpp:177-194)
-in one .npz.  The reference is run with -n 1 so its output does not depend on
+Inputs are seeded Balding-Nichols PLINK sets from pcaone_sim_bed (and dense,
+likelihood and LD inputs); each case stores its input and the reference's
+printed outputs (.eigvals, .sigvals, .eigvecs, .loadings with 6 significant
+digits, src/Data.cpp:177-194; .cov; .ld.gz with 6 decimals) in one .npz.  The reference is run with -n 1 so its output does not depend on
 OpenMP scheduling.
 """
 from __future__ import annotations
@@ -64,6 +64,57 @@ DENSE_CASES = {
 }
 
 
+# LD r^2 on a dense residual matrix (-B .residuals --match-bim .mbim --print-r2):
+# name -> (seed, nsamples, SNPs per chromosome, ld_bp)
+LD = {"ld_r2_bp2000": (41, 150, (300, 250, 351), 2000)}
+
+
+def ld_inputs(seed, n, per_chr):
+    """Residual-like float32 rows with LD decaying along the chromosome (an AR(1)
+    chain of SNPs) and sorted positions with 1..400 bp gaps (one duplicated
+    position inside chromosome 1)."""
+    rng = np.random.default_rng(seed)
+    m = int(sum(per_chr))
+    X = np.empty((m, n))
+    pos = np.empty(m, dtype=np.int64)
+    chr_end = np.cumsum(per_chr) - 1
+    start = 0
+    for c, cnt in enumerate(per_chr):
+        x = rng.standard_normal(n)
+        p = int(rng.integers(1, 1000))
+        for t in range(cnt):
+            rho = rng.uniform(0.3, 0.95)
+            x = rho * x + np.sqrt(1 - rho * rho) * rng.standard_normal(n)
+            X[start + t] = x
+            pos[start + t] = p
+            p += int(rng.integers(1, 400))
+        start += cnt
+    pos[10] = pos[9]  # duplicated position inside chromosome 1
+    X -= X.mean(axis=1, keepdims=True)
+    return X.astype(np.float32), pos, chr_end
+
+
+def run_ld(tmp: Path, name: str, X, pos, chr_end, ld_bp):
+    pa.write_bin(tmp / (name + ".residuals"), X)
+    chrom = np.searchsorted(chr_end, np.arange(X.shape[0])) + 1
+    with open(tmp / (name + ".mbim"), "w") as f:
+        for i in range(X.shape[0]):
+            f.write(f"{chrom[i]}\trs{i}\t0\t{pos[i]}\tA\tG\t0.25\n")
+    cmd = [str(REF), "-B", name + ".residuals", "--match-bim", name + ".mbim", "--print-r2", "--ld-bp",
+           str(ld_bp), "-o", name, "-n", "1", "-v", "0"]
+    subprocess.run(cmd, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    import gzip
+    ia, ib, r2 = [], [], []
+    with gzip.open(tmp / (name + ".ld.gz"), "rt") as f:
+        f.readline()
+        for line in f:
+            t = line.rstrip("\n").split("\t")
+            ia.append(int(t[2][2:]))
+            ib.append(int(t[5][2:]))
+            r2.append(float(t[6]))
+    return np.array(ia), np.array(ib), np.array(r2)
+
+
 def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
     cmd = [str(REF), flag, prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v", "0"]
     subprocess.run(cmd, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
@@ -102,6 +153,12 @@ def main():
                 res["cov"] = np.loadtxt(tmp / (case + ".cov"))
                 np.savez_compressed(OUT / f"{case}.npz", gl=gl, nsamples=n, nsnps=m, args=args, **res)
                 print(case, res["eigvals"])
+        for name, (seed, n, per_chr, ld_bp) in LD.items():
+            X, pos, chr_end = ld_inputs(seed, n, per_chr)
+            ia, ib, r2 = run_ld(tmp, name, X, pos, chr_end, ld_bp)
+            np.savez_compressed(OUT / f"{name}.npz", resid=X, pos=pos, chr_end=chr_end, ld_bp=ld_bp, snp_a=ia,
+                                snp_b=ib, r2=r2)
+            print(name, len(r2), "pairs")
         for name, (seed, n, m, base) in DENSE.items():
             D = pa.sim_counts(seed, n, m, base=base)
             pa.write_bin(tmp / (name + ".bin"), D)

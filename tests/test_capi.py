@@ -92,3 +92,15 @@ def test_host_window_schedule_rejects_bad_bands():
     for bands in (2, 6, 0):
         assert pa.lib().pcaone_fancy_plan(bands, 0, C.byref(bs), a.ctypes.data_as(P), a.ctypes.data_as(P),
                                           a.ctypes.data_as(P), None) == -1
+
+
+def test_ld_windows_match_oracle():
+    import pcaone_oracle as O
+    from conftest import GOLDEN, load_golden
+    z = load_golden(GOLDEN / "ld_r2_bp2000.npz")
+    for bp in (0, 1, 150, int(z["ld_bp"]), 10 ** 7):
+        ws, we = pa.ld_windows(z["pos"], z["chr_end"], bp)
+        wo, eo = O.ld_windows(z["pos"], z["chr_end"], bp)
+        assert np.array_equal(ws, wo) and np.array_equal(we, eo)
+    with pytest.raises(pa.GpuError):
+        pa.ld_windows(z["pos"], z["chr_end"][:-1], 100)  # the last chromosome must end at nsnps - 1

@@ -242,3 +242,52 @@ def test_pcangsd_grm_matches_reference():
     # planted mistake: the GRM without the Dc diagonal differs visibly
     E, _ = O.pcangsd_standardize(P0, P1, F, op.U, op.S, op.V)
     assert np.max(np.abs(np.diag(E @ E.T / P0.shape[1]) - np.diag(z["cov"]))) > 1e-3
+
+
+# --- LD windows and r^2 ------------------------------------------------------
+
+def _brute_windows(pos, chr_end, ld_bp):
+    """Pairs (i, j), i < j in one chromosome with pos[j] - pos[i] <= ld_bp, for
+    every i whose position differs from its chromosome's last one."""
+    pairs, start = [], 0
+    for e in chr_end:
+        for i in range(start, e + 1):
+            if pos[i] == pos[e]:
+                continue
+            pairs += [(i, j) for j in range(i + 1, e + 1) if pos[j] - pos[i] <= ld_bp]
+        start = e + 1
+    return pairs
+
+
+def test_ld_oracle_matches_reference():
+    z = load_golden(ROOT / "tests" / "golden" / "ld_r2_bp2000.npz")
+    ws, we = O.ld_windows(z["pos"], z["chr_end"], int(z["ld_bp"]))
+    ia = np.concatenate([np.full(e - 1, i) for i, e in zip(ws, we)])
+    ib = np.concatenate([np.arange(i + 1, i + e) for i, e in zip(ws, we)])
+    assert np.array_equal(ia, z["snp_a"]) and np.array_equal(ib, z["snp_b"])
+    r2 = O.ld_r2(z["resid"], ws, we)
+    assert np.abs(r2 - z["r2"]).max() <= 5.01e-7  # the reference prints 6 decimals
+    # brute-force windows
+    pairs = _brute_windows(z["pos"], z["chr_end"], int(z["ld_bp"]))
+    assert pairs == list(zip(ia.tolist(), ib.tolist()))
+    # planted mistake: uncentred rows
+    X = z["resid"] + 0.3
+    Xc = X.astype(np.float64)
+    sds = 1 / np.sqrt((Xc * Xc).sum(1) / (Xc.shape[1] - 1))
+    wrong = np.concatenate([((Xc[np.arange(i + 1, i + e)] @ Xc[i]) * sds[i] * sds[np.arange(i + 1, i + e)]
+                             / (Xc.shape[1] - 1)) ** 2 for i, e in zip(ws, we)])
+    assert np.abs(wrong - z["r2"]).max() > 1e-3
+
+
+def test_ld_r2_closed_forms():
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(50)
+    y = rng.standard_normal(50)
+    y -= (y - y.mean()) @ (x - x.mean()) / ((x - x.mean()) @ (x - x.mean())) * (x - x.mean())  # orthogonal
+    X = np.stack([x, 3 * x + 1, -x, y]).astype(np.float32)
+    r2 = O.ld_r2(X, np.array([0]), np.array([4]))
+    assert np.allclose(r2[:2], 1.0, atol=1e-6) and abs(r2[2]) < 1e-6
+    # a correlation with a known value: y = x + noise of equal variance -> r^2 = 1/2 in expectation
+    z = rng.standard_normal((2, 200000))
+    r2 = O.ld_r2(np.stack([z[0], z[0] + z[1]]).astype(np.float32), np.array([0]), np.array([2]))
+    assert abs(r2[0] - 0.5) < 0.01
