@@ -188,6 +188,15 @@ int pcaone_gpu_iram(pcaone_gpu* h, int k, int ncv, int maxit, double tol, double
  * refresh, 0 if none); *reset_both = both accumulators cleared first. */
 int pcaone_fancy_plan(int bands, int pi, uint64_t* bandsize, int* acc, int* update, int* zero, int* reset_both);
 
+/* ---- output files --------------------------------------------------------
+ * Data::write_eigs_files (src/Data.cpp:177-194), host only: prefix.sigvals
+ * ("#N,M" then S), prefix.eigvals (S^2 / nsnps * ploidy), prefix.eigvecs (U,
+ * nsamples x k) and, when V != NULL, prefix.loadings (V, nsnps x k); U and V
+ * column-major; 6 significant digits, tab-separated columns, one row per
+ * line, as Eigen::IOFormat(6, DontAlignCols, "\t", "\n"). */
+int pcaone_write_eigs(const char* prefix, uint32_t nsamples, uint32_t nsnps, int k, const double* S,
+                      const double* U, const double* V, int ploidy);
+
 /* ---- device timing ------------------------------------------------------
  * When on, every kernel the handle launches is bracketed by HIP events on the
  * handle's stream.  Stats are grouped by kernel family name. */

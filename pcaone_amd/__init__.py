@@ -55,6 +55,7 @@ SIGNATURES = {
     "pcaone_ld_windows": (C.c_int, [_I32, C.c_uint32, _I32, C.c_uint32, C.c_int32, _I32, _I32,
                                     C.POINTER(C.c_uint32)]),
     "pcaone_gpu_ld_r2": (C.c_int, [C.c_void_p, _I32, _I32, C.c_uint32, _D, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "pcaone_write_eigs": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, _D, _D, _D, C.c_int]),
     "pcaone_gpu_load_gl": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]),
     "pcaone_gpu_gl_em_maf": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, C.POINTER(C.c_int)]),
     "pcaone_gpu_pcangsd_update": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_int, C.c_int, C.POINTER(C.c_float), _D]),
@@ -174,6 +175,18 @@ def ld_windows(pos: np.ndarray, chr_end: np.ndarray, ld_bp: int):
     if rc != 0:
         raise GpuError(rc, "pcaone_ld_windows failed")
     return ws[:nw.value].copy(), we[:nw.value].copy()
+
+
+def write_eigs(prefix: str | os.PathLike, S: np.ndarray, U: np.ndarray, V: np.ndarray | None, nsnps: int,
+               ploidy: int = 2):
+    """The reference's .sigvals / .eigvals / .eigvecs (/ .loadings) files."""
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    U = np.asfortranarray(U, dtype=np.float64)
+    Vf = None if V is None else np.asfortranarray(V, dtype=np.float64)
+    rc = lib().pcaone_write_eigs(str(prefix).encode(), U.shape[0], nsnps, U.shape[1], _p(S, C.c_double),
+                                 _p(U, C.c_double), _p(Vf, C.c_double), ploidy)
+    if rc != 0:
+        raise GpuError(rc, "pcaone_write_eigs failed")
 
 
 def fancy_plan(bands: int, pi: int, bandsize: int):

@@ -3,6 +3,7 @@
 // std::runtime_error, src/Logger.hpp:96-105; the C++ wrapper on the other side
 // of this ABI turns the status back into an exception).
 #include <cstring>
+#include <fstream>
 #include <new>
 #include <string>
 
@@ -199,6 +200,30 @@ int pcaone_fancy_plan(int bands, int pi, uint64_t* bandsize, int* acc, int* upda
     zero[b] = pl.steps[b].zero;
   }
   if (reset_both) *reset_both = pl.reset_both ? 1 : 0;
+  return PCAONE_OK;
+}
+
+int pcaone_write_eigs(const char* prefix, uint32_t nsamples, uint32_t nsnps, int k, const double* S,
+                      const double* U, const double* V, int ploidy) {
+  if (!prefix || !S || !U || k < 1 || nsamples == 0 || nsnps == 0) return PCAONE_EINVAL;
+  const std::string p(prefix);
+  std::ofstream outs(p + ".sigvals"), oute(p + ".eigvals"), outu(p + ".eigvecs");
+  if (!outs || !oute || !outu) return PCAONE_EINVAL;
+  for (auto* o : {&outs, &oute, &outu}) o->precision(6);
+  outs << '#' << nsamples << ',' << nsnps << '\n';
+  for (int c = 0; c < k; ++c) outs << S[c] << '\n';
+  for (int c = 0; c < k; ++c) oute << S[c] * S[c] / (double)nsnps * ploidy << '\n';
+  auto rows = [k](std::ofstream& o, const double* A, uint64_t n) {
+    for (uint64_t r = 0; r < n; ++r)
+      for (int c = 0; c < k; ++c) o << A[(uint64_t)c * n + r] << (c + 1 < k ? '\t' : '\n');
+  };
+  rows(outu, U, nsamples);
+  if (V) {
+    std::ofstream outv(p + ".loadings");
+    if (!outv) return PCAONE_EINVAL;
+    outv.precision(6);
+    rows(outv, V, nsnps);
+  }
   return PCAONE_OK;
 }
 

@@ -125,6 +125,8 @@ def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
         res["sigvals"] = np.loadtxt(f, ndmin=1)
     res["eigvecs"] = np.loadtxt(tmp / (out + ".eigvecs"), ndmin=2)
     res["loadings"] = np.loadtxt(tmp / (out + ".loadings"), ndmin=2)
+    for ext in ("sigvals", "eigvals", "eigvecs", "loadings"):
+        res["text_" + ext] = np.array((tmp / (out + "." + ext)).read_text())
     return res
 
 

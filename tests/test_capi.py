@@ -104,3 +104,19 @@ def test_ld_windows_match_oracle():
         assert np.array_equal(ws, wo) and np.array_equal(we, eo)
     with pytest.raises(pa.GpuError):
         pa.ld_windows(z["pos"], z["chr_end"][:-1], 100)  # the last chromosome must end at nsnps - 1
+
+
+def test_write_eigs_reproduces_reference_files(tmp_path):
+    """The four text files, byte for byte, from the values the reference wrote
+    (already at 6 significant digits, so re-printing them is exact)."""
+    from conftest import GOLDEN, load_golden
+    z = load_golden(GOLDEN / "svd1_p7.npz")
+    m = int(z["nsnps"])
+    S = z["sigvals"]
+    pa.write_eigs(tmp_path / "o", S, z["eigvecs"], z["loadings"], m, 2)
+    for ext in ("sigvals", "eigvecs", "loadings"):
+        assert (tmp_path / ("o." + ext)).read_text() == str(z["text_" + ext]), ext
+    # eigvals are recomputed from S: equal to 6 significant digits
+    ev = np.loadtxt(tmp_path / "o.eigvals")
+    assert np.allclose(ev, z["eigvals"], rtol=1e-5)
+    assert (tmp_path / "o.eigvals").read_text().count("\n") == len(S)
