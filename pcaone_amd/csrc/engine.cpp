@@ -541,7 +541,8 @@ void Engine::set_omega_vec(const double* dev_v, const double* host_v) {
   for (uint32_t j = 0; j < N_; ++j) cs += host_v[j];
   PCAONE_HIP(hipMemcpyAsync(mv_colsum_.p, &cs, 8, hipMemcpyHostToDevice, stream_));
   timed("omega_operand", [&] {
-    launch_omega_operand(mv_omega_.p, N_, 1, mv_.prec, mv_bg_.p, mv_.bcols, mv_.lo_off, kg_, stream_);
+    launch_omega_operand(mv_omega_.p, N_, 1, mv_.prec, mv_bg_.p, mv_.bcols, mv_.lo_off, kg_, kperm_(mv_.prec),
+                         stream_);
   });
   refresh_omega32(mv_);
   sync();  // cs lives on this stack frame
@@ -638,7 +639,8 @@ void Engine::init_omega() {
   PCAONE_HIP(hipMemcpyAsync(omg2_.p, omega0_.p, nb, hipMemcpyDeviceToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(colsum_.p, colsum0_.p, (size_t)l_ * 8, hipMemcpyDeviceToDevice, stream_));
   timed("omega_operand",
-        [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, lo_off_, kg_, stream_); });
+        [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, lo_off_, kg_, kperm_(p_.precision),
+                                   stream_); });
   refresh_omega32(op_bufs());
 }
 
@@ -655,20 +657,22 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
       launch_gram_partial(H_.p, nullptr, nullptr, N_, 0, l, part_.p, nb, stream_);
   });
   timed("qr_chol", [&] {
-    launch_chol(part_.p, nb, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, status_.p,
-                stream_);
+    launch_chol(part_.p, nb, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, status_.p, stream_);
   });
   timed("qr_apply", [&] { launch_apply_rinv(H_.p, R1i_.p, nullptr, Q_.p, N_, l, part_.p, nb, stream_); });
-  // second pass; the Householder signs come from the top block of Q R2^-1
   timed("qr_chol", [&] {
-    launch_chol(part_.p, nb, l, R2_.p, R2i_.p, nullptr, nullptr, nullptr, nullptr, Q_.p, sign_.p, status_.p + 1,
-                stream_);
+    launch_chol(part_.p, nb, l, R2_.p, R2i_.p, nullptr, nullptr, nullptr, nullptr, status_.p + 1, stream_);
   });
-  timed("qr_apply", [&] { launch_apply_rinv(Q_.p, R2i_.p, sign_.p, omega_.p, N_, l, nullptr, nb, stream_); });
-  timed("flip", [&] { launch_flip_stats(omega_.p, omg2_.p, N_, l, fpart_.p, nb, stream_); });
-  timed("flip", [&] { launch_flip_apply(omega_.p, omg2_.p, N_, l, fpart_.p, nb, colsum_.p, zero_acc, stream_); });
+  timed("qr_apply", [&] { launch_apply_rinv(Q_.p, R2i_.p, nullptr, omega_.p, N_, l, nullptr, nb, stream_); });
+  // the Householder signs from the top block of the CholeskyQR2 factor, applied
+  // together with flipOmg
+  timed("qr_signs", [&] { launch_householder_signs(omega_.p, l, sign_.p, stream_); });
+  timed("flip", [&] { launch_flip_stats(omega_.p, omg2_.p, sign_.p, N_, l, fpart_.p, nb, stream_); });
+  timed("flip",
+        [&] { launch_flip_apply(omega_.p, omg2_.p, sign_.p, N_, l, fpart_.p, nb, colsum_.p, zero_acc, stream_); });
   timed("omega_operand",
-        [&] { launch_omega_operand(omega_.p, N_, l, p_.precision, bg_.p, bcols_, lo_off_, kg_, stream_); });
+        [&] { launch_omega_operand(omega_.p, N_, l, p_.precision, bg_.p, bcols_, lo_off_, kg_, kperm_(p_.precision),
+                                   stream_); });
   refresh_omega32(op_bufs());
 }
 
@@ -734,11 +738,14 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
     cg.per = rows;
     cg.ngroups = 1;
     cg.out = b.corrG;
-    // Omega (and everything before it) is ready on the main stream
+    // Omega (and everything before it) is ready on the main stream.  With
+    // profiling on the gathers stay on the main stream, so every kernel is
+    // timed alone.
+    hipStream_t gs = timer.on ? stream_ : aux_;
     PCAONE_HIP(hipEventRecord(ev_[0], stream_));
-    PCAONE_HIP(hipStreamWaitEvent(aux_, ev_[0], 0));
-    timed_on(aux_, "miss_gather", [&] { launch_csr_gather(cg, aux_); });
-    PCAONE_HIP(hipEventRecord(ev_[1], aux_));
+    PCAONE_HIP(hipStreamWaitEvent(gs, ev_[0], 0));
+    timed_on(gs, "miss_gather", [&] { launch_csr_gather(cg, gs); });
+    PCAONE_HIP(hipEventRecord(ev_[1], gs));
   }
   if (!dense) {
     PackedGemm g{};
@@ -814,6 +821,7 @@ FinalizeG Engine::finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) cons
   fg.w1 = w1;
   fg.nsnps = M_;
   fg.s2p = b.s2p;
+  fg.kperm = kperm_(b.prec);
   return fg;
 }
 
@@ -838,10 +846,11 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
     cg.ngroups = nz;
     cg.out = b.corrH;
     // g'' of these windows is ready on the main stream
+    hipStream_t gs = timer.on ? stream_ : aux_;
     PCAONE_HIP(hipEventRecord(ev_[2], stream_));
-    PCAONE_HIP(hipStreamWaitEvent(aux_, ev_[2], 0));
-    timed_on(aux_, "miss_gather", [&] { launch_csr_gather(cg, aux_); });
-    PCAONE_HIP(hipEventRecord(ev_[3], aux_));
+    PCAONE_HIP(hipStreamWaitEvent(gs, ev_[2], 0));
+    timed_on(gs, "miss_gather", [&] { launch_csr_gather(cg, gs); });
+    PCAONE_HIP(hipEventRecord(ev_[3], gs));
   }
   if (!dense) {
     PackedGemm h{};
@@ -1000,17 +1009,21 @@ void Engine::usv_step() {
   const int nbG = gram_blocks(M_), nbN = gram_blocks(N_);
   timed("usv_gram", [&] { launch_gram_partial(G_.p, nullptr, nullptr, M_, 0, l, part_.p, nbG, stream_); });
   timed("usv_chol", [&] {
-    launch_chol(part_.p, nbG, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, status_.p,
-                stream_);
+    launch_chol(part_.p, nbG, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, status_.p, stream_);
   });
   timed("usv_apply", [&] { launch_apply_rinv(G_.p, R1i_.p, nullptr, G_.p, M_, l, part_.p, nbG, stream_); });
   timed("usv_chol", [&] {
-    launch_chol(part_.p, nbG, l, R2_.p, R2i_.p, R1_.p, R1i_.p, Rt_.p, Rti_.p, nullptr, nullptr, status_.p + 1,
-                stream_);
+    launch_chol(part_.p, nbG, l, R2_.p, R2i_.p, R1_.p, R1i_.p, Rt_.p, Rti_.p, status_.p + 1, stream_);
   });
-  timed("usv_apply", [&] { launch_apply_rinv(G_.p, R2i_.p, nullptr, G_.p, M_, l, nullptr, nbG, stream_); });
+  // no Gram partials to collect: four times the workgroups for the streaming pass
+  const int nbA = (int)std::min<uint64_t>(ceil_div(M_, 64), 1024);
+  timed("usv_apply", [&] { launch_apply_rinv(G_.p, R2i_.p, nullptr, G_.p, M_, l, nullptr, nbA, stream_); });
   timed("usv_apply", [&] { launch_apply_rinv(H_.p, Rti_.p, nullptr, Bt_.p, N_, l, part_.p, nbN, stream_); });
-  timed("usv_eig", [&] { launch_sym_eig(part_.p, nbN, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, stream_); });
+  // later epochs start from the previous epoch's eigenvectors
+  timed("usv_eig", [&] {
+    launch_sym_eig(part_.p, nbN, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, have_W_ ? W_.p : nullptr, stream_);
+  });
+  have_W_ = true;
   timed("usv_rows", [&] {
     launch_rows_times_small(Bt_.p, N_, l, W_.p, l, (uint32_t)p_.k, sinv_.p, Ucur_.p, stream_);
   });
@@ -1022,6 +1035,7 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
   PCAONE_HIP(hipSetDevice(device_));
   const uint32_t k = (uint32_t)p_.k;
   int p = maxp, ep = 0;
+  have_W_ = false;
   if (diffs)
     for (int i = 0; i <= maxp; ++i) diffs[i] = 0.0;
   bool done = false;

@@ -131,6 +131,8 @@ class Engine {
   uint32_t rows_per_block_(int prec) const {
     return (src_ == SRC_DENSE || prec == PREC_SPLIT_BF16) ? 256u : 512u;
   }
+  // the split packed contraction reads its operands in the kperm16 K order
+  int kperm_(int prec) const { return (src_ == SRC_BED && prec == PREC_SPLIT_BF16) ? 1 : 0; }
   SegBufs op_bufs();
   void refresh_omega32(const SegBufs& b);
   FinalizeG finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) const;
@@ -205,6 +207,7 @@ class Engine {
   uint64_t bandsize_ = 1;
   // RsvdOpData::update / U, S, V of the last computeUSV (src/Halko.hpp:11-14)
   bool update_ = false, have_usv_ = false, have_vals_ = false;
+  bool have_W_ = false;  // W_ holds this solve's previous eigenvectors
   std::vector<double> last_S_;
   DevBuf<double> miss_val_, Sdev_, Vpre_;
   // linear algebra scratch

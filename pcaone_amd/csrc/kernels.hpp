@@ -104,6 +104,7 @@ struct FinalizeG {
   uint32_t w0, w1;      // windows of the segment ([r0, r1) are their rows)
   uint32_t nsnps;
   double* s2p;          // [nwin][finalize_chunks(blocksize)][l] partial sums of g2
+  int kperm;            // operand positions in the split packed contraction's K order
 };
 // grid = (chunks of 64 rows, windows of the segment)
 void launch_finalize_G(const FinalizeG& f, hipStream_t s);
@@ -173,8 +174,9 @@ void launch_cast_f32(const double* src, float* dst, uint64_t n, hipStream_t s);
 
 // Omega (FP64 [N][l]) -> G-contraction operand [bcols][kpad] in the arm's type
 // (split: hi at row c, lo at row lo_off + c).
+// kperm: positions in the split packed contraction's K order (kperm16)
 void launch_omega_operand(const double* omega, uint32_t n, uint32_t l, int bprec, void* bg, uint32_t bcols,
-                          uint32_t lo_off, uint64_t kpad, hipStream_t s);
+                          uint32_t lo_off, uint64_t kpad, int kperm, hipStream_t s);
 
 // ---- dense FP32 feature-major matrices (src/FileBinary.cpp:21-46) ----------
 struct DenseGemm {
@@ -254,28 +256,30 @@ int gram_blocks(uint64_t nrows);
 void launch_gram_partial(const double* A, const double* A2, double* Aout, uint64_t nrows, uint32_t row0,
                          uint32_t l, double* part, int nblocks, hipStream_t s);
 // Single workgroup: Gram = sum(part), R = chol(Gram) upper, Rinv = R^-1,
-// optional Rprev: Rtot = R * Rprev and Rtot_inv = Rprev_inv * Rinv.  sign
-// (optional, l): Householder R-diagonal signs of the matrix being
-// orthogonalised, from the top l x l block Qtop of the matrix this Cholesky
-// factor is applied to (Qtop R^-1 is the top block of the orthonormal factor).
+// optional Rprev: Rtot = R * Rprev and Rtot_inv = Rprev_inv * Rinv.
 void launch_chol(const double* part, int nparts, uint32_t l, double* R, double* Rinv, const double* Rprev,
-                 const double* Rprev_inv, double* Rtot, double* Rtot_inv, const double* Qtop, double* sign,
-                 int* status, hipStream_t s);
+                 const double* Rprev_inv, double* Rtot, double* Rtot_inv, int* status, hipStream_t s);
+// Householder R-diagonal signs of a tall matrix from the top l x l block B
+// (row-major) of its CholeskyQR factor: LU of B - S with S_kk = -sign(pivot)
+// (Householder reconstruction); R_householder = diag(sign) R_chol.
+void launch_householder_signs(const double* B, uint32_t l, double* sign, hipStream_t s);
 // Q = A * Rinv (rows), optional per-column sign; optional partial Gram of Q.
 void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, double* Q, uint64_t nrows,
                        uint32_t l, double* gram_part, int nblocks, hipStream_t s);
-// flipOmg against Omega2 + colsum: stats per block, then apply.
-void launch_flip_stats(const double* Q, const double* omg2, uint64_t nrows, uint32_t l, double* part,
-                       int nblocks, hipStream_t s);
-void launch_flip_apply(double* Q, double* omg2, uint64_t nrows, uint32_t l, const double* part, int nblocks,
-                       double* colsum, double* zero_acc, hipStream_t s);
+// flipOmg against Omega2 + colsum on diag(sign) Q (sign optional): stats per
+// block, then Q <- diag(flip * sign) Q, Omega2 <- Q.
+void launch_flip_stats(const double* Q, const double* omg2, const double* sign, uint64_t nrows, uint32_t l,
+                       double* part, int nblocks, hipStream_t s);
+void launch_flip_apply(double* Q, double* omg2, const double* sign, uint64_t nrows, uint32_t l, const double* part,
+                       int nblocks, double* colsum, double* zero_acc, hipStream_t s);
 // small dense: C[n][kc] = A[n][l] * W[l][ldw] (first kc columns) * diag(scale)
 void launch_rows_times_small(const double* A, uint64_t n, uint32_t l, const double* W, uint32_t ldw,
                              uint32_t kc, const double* scale, double* C, hipStream_t s);
 // symmetric eigen-decomposition of sum(part) (l x l) by cyclic Jacobi in one
 // workgroup; eigenvalues descending into evals, eigenvectors (columns) into W.
+// W0 (optional, may alias W): start from this basis (the previous eigenvectors).
 void launch_sym_eig(const double* part, int nparts, uint32_t l, double* evals, double* W, double* sinv,
-                    uint32_t kc, hipStream_t s);
+                    uint32_t kc, const double* W0, hipStream_t s);
 // mev partials: M = X^T Y (k x k) per block
 void launch_xty_partial(const double* X, const double* Y, uint64_t n, uint32_t k, double* part, int nblocks,
                         hipStream_t s);

@@ -242,6 +242,34 @@ __device__ inline bf16x8 decode8(uint32_t h) {
   return r;
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// K order of the split-BF16 packed contractions: within every aligned group of
+// 16 codes, operand position 4a + b holds code 4b + a (a 4 x 4 transpose, its
+// own inverse).  decode16 produces the A operand in this order straight from
+// one 32-bit word of packed codes, and the B operands are written in it.
+__host__ __device__ inline uint64_t kperm16(uint64_t k) {
+  return (k & ~(uint64_t)15) | ((k & 3) << 2) | ((k >> 2) & 3);
+}
+
+// 16 codes (one 32-bit word) -> the A operands of two MFMA K steps, in kperm16
+// order: spread word x_j = (w >> 2j) & 0x03030303 holds codes j, j+4, j+8,
+// j+12 one per byte; each becomes four BF16 dosages through the v_perm tables.
+__device__ inline void decode16(uint32_t w, bf16x8& a0, bf16x8& a1) {
+  constexpr uint32_t LUT_HI = 0x003F0040u;  // high bytes of bf16 {2.0, 0, 1.0, 0}
+  constexpr uint32_t LUT_LO = 0x00800000u;  // low bytes
+  uint32_t o[8];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t x = (w >> (2 * j)) & 0x03030303u;
+    const uint32_t y = x + 0x04040404u;
+    o[2 * j] = __builtin_amdgcn_perm(LUT_LO, LUT_HI, __builtin_amdgcn_perm(y, x, 0x01050004u));
+    o[2 * j + 1] = __builtin_amdgcn_perm(LUT_LO, LUT_HI, __builtin_amdgcn_perm(y, x, 0x03070206u));
+  }
+  __builtin_memcpy(&a0, o, sizeof(a0));
+  __builtin_memcpy(&a1, o + 4, sizeof(a1));
+}
+
 // Block = 4 waves, 256 rows (64 per wave = 4 fragments of 16).  K advances in
 // chunks of 256 codes (64 bytes per row); lane (r, g) holds bytes [16g, 16g+16)
 // of its row's chunk, i.e. codes [64g, 64g+64), and K-step s (0..7) feeds the
@@ -250,15 +278,20 @@ __device__ inline bf16x8 decode8(uint32_t h) {
 // distinct banks.
 constexpr int kBStride = 264;
 
-template <int NT>
-__global__ __launch_bounds__(256) void k_packed_gemm_bf16(PackedGemm g) {
+template <int NT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_packed_gemm_bf16(PackedGemm g) {
+  constexpr int NTH = 64 * WAVES;
+  // uint4 of the B chunk per thread; held in registers one chunk ahead when small
+  constexpr int BV = NT * 16 * 32 / NTH;
+  constexpr bool BPRE = BV <= 10;
+  static_assert(NT * 16 * 32 % NTH == 0, "B chunk must split evenly over the block");
   __shared__ __attribute__((aligned(16))) __bf16 bl[NT * 16 * kBStride];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t z = blockIdx.z, split = blockIdx.y;
   const uint8_t* A = g.A + (uint64_t)z * g.a_zstride;
   const __bf16* B = reinterpret_cast<const __bf16*>(g.B) + (uint64_t)z * g.b_zstride;
   float* out = reinterpret_cast<float*>(g.out) + (uint64_t)z * g.o_zstride;
-  const uint32_t row0 = blockIdx.x * 256 + wave * 64;
+  const uint32_t row0 = blockIdx.x * NTH + wave * 64;
   const uint32_t c0 = split * g.chunks_per_split;
   const uint32_t c1 = min(c0 + g.chunks_per_split, g.kchunks);
 
@@ -276,33 +309,73 @@ __global__ __launch_bounds__(256) void k_packed_gemm_bf16(PackedGemm g) {
     for (int t = 0; t < NT; ++t) acc[f][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   const int bn = lane & 15, bg = lane >> 4;
+  // the next chunk's A bytes (and B tile, when BPRE) travel in registers while
+  // the current chunk computes; B goes through LDS (two barriers per chunk)
+  uint4 an[4];
+  u32x4 bn4[BPRE ? BV : 1];
+#define PCAONE_LOAD_A(CH)                                                                                \
+  do {                                                                                                   \
+    _Pragma("unroll") for (int f = 0; f < 4; ++f) an[f] =                                                \
+        *reinterpret_cast<const uint4*>(arow[f] + (uint64_t)(CH) * 64);                                  \
+  } while (0)
+#define PCAONE_B_SRC(CH, V)                                                                              \
+  (*reinterpret_cast<const u32x4*>(B + (uint64_t)((threadIdx.x + NTH * (V)) >> 5) * g.ldb +              \
+                                   (uint64_t)(CH) * 256 + ((threadIdx.x + NTH * (V)) & 31) * 8))
+#define PCAONE_B_DST(V) \
+  (*reinterpret_cast<u32x4*>(bl + ((threadIdx.x + NTH * (V)) >> 5) * kBStride + ((threadIdx.x + NTH * (V)) & 31) * 8))
+  if (c0 < c1) {
+    PCAONE_LOAD_A(c0);
+    if constexpr (BPRE) {
+#pragma unroll
+      for (int v = 0; v < BV; ++v) bn4[v] = PCAONE_B_SRC(c0, v);
+    }
+  }
   for (uint32_t c = c0; c < c1; ++c) {
+    __syncthreads();
+    if constexpr (BPRE) {
+#pragma unroll
+      for (int v = 0; v < BV; ++v) PCAONE_B_DST(v) = bn4[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < BV; ++v) PCAONE_B_DST(v) = PCAONE_B_SRC(c, v);
+    }
     uint4 ar[4];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) ar[f] = *reinterpret_cast<const uint4*>(arow[f] + (uint64_t)c * 64);
+    for (int f = 0; f < 4; ++f) ar[f] = an[f];
     __syncthreads();
-    for (int it = threadIdx.x; it < NT * 16 * 32; it += 256) {
-      const int n = it >> 5, seg = it & 31;
-      const uint4 v = *reinterpret_cast<const uint4*>(B + (uint64_t)n * g.ldb + (uint64_t)c * 256 + seg * 8);
-      *reinterpret_cast<uint4*>(bl + n * kBStride + seg * 8) = v;
+    if (c + 1 < c1) {
+      PCAONE_LOAD_A(c + 1);
+      if constexpr (BPRE) {
+#pragma unroll
+        for (int v = 0; v < BV; ++v) bn4[v] = PCAONE_B_SRC(c + 1, v);
+      }
     }
-    __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      bf16x8 b[NT];
+    for (int q = 0; q < 4; ++q) {  // 32-bit word q of each fragment: K steps 2q, 2q + 1
+      bf16x8 b0[NT], b1[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
-        b[t] = *reinterpret_cast<const bf16x8*>(bl + (t * 16 + bn) * kBStride + 64 * bg + 8 * s);
+      for (int t = 0; t < NT; ++t) {
+        b0[t] = *reinterpret_cast<const bf16x8*>(bl + (t * 16 + bn) * kBStride + 64 * bg + 16 * q);
+        b1[t] = *reinterpret_cast<const bf16x8*>(bl + (t * 16 + bn) * kBStride + 64 * bg + 16 * q + 8);
+      }
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
-        const uint32_t w = (s < 2) ? ar[f].x : (s < 4) ? ar[f].y : (s < 6) ? ar[f].z : ar[f].w;
-        const bf16x8 a = decode8((s & 1) ? (w >> 16) : (w & 0xFFFFu));
+        const uint32_t w = (q == 0) ? ar[f].x : (q == 1) ? ar[f].y : (q == 2) ? ar[f].z : ar[f].w;
+        bf16x8 a0, a1;
+        decode16(w, a0, a1);
+        // NT independent accumulators between two updates of the same one
 #pragma unroll
         for (int t = 0; t < NT; ++t)
-          acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[f][t], 0, 0, 0);
+          acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0[t], acc[f][t], 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[t], acc[f][t], 0, 0, 0);
       }
     }
   }
+#undef PCAONE_LOAD_A
+#undef PCAONE_B_SRC
+#undef PCAONE_B_DST
   float* o = out + (uint64_t)split * g.nrows * g.ldo;
 #pragma unroll
   for (int f = 0; f < 4; ++f)
@@ -316,11 +389,127 @@ __global__ __launch_bounds__(256) void k_packed_gemm_bf16(PackedGemm g) {
     }
 }
 
+// Long row ranges: 8 waves x 64 rows per block, the B chunk streamed straight
+// from global memory into a double-buffered LDS tile (global_load_lds_dwordx4:
+// lane L of a load fills bytes [16 L, 16 L + 16) of a 1 KB span, two B rows),
+// so chunk c + 1 lands while chunk c computes and one barrier per chunk
+// suffices.  B rows are unpadded (512 bytes); the 16-byte segment s of row n
+// sits at slot s ^ (n & 7), which spreads the 16 rows a ds_read_b128 touches
+// over distinct banks.
+template <int NT>
+__global__ __launch_bounds__(512) void k_packed_gemm_bf16_stream(PackedGemm g) {
+  constexpr int ROWS_B = NT * 16;  // B rows per chunk (512 bytes each)
+  __shared__ __attribute__((aligned(16))) __bf16 bs[2][ROWS_B * 256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t z = blockIdx.z, split = blockIdx.y;
+  const uint8_t* A = g.A + (uint64_t)z * g.a_zstride;
+  const __bf16* B = reinterpret_cast<const __bf16*>(g.B) + (uint64_t)z * g.b_zstride;
+  float* out = reinterpret_cast<float*>(g.out) + (uint64_t)z * g.o_zstride;
+  const uint32_t row0 = blockIdx.x * 512 + wave * 64;
+  const uint32_t c0 = split * g.chunks_per_split;
+  const uint32_t c1 = min(c0 + g.chunks_per_split, g.kchunks);
+
+  const uint8_t* arow[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    uint32_t r = row0 + 16 * f + (lane & 15);
+    r = min(r, g.nrows - 1);
+    arow[f] = A + (uint64_t)r * g.lda + 16 * (lane >> 4);
+  }
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[f][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  // this lane's share of a B chunk: load j fills rows 2 i, 2 i + 1 (i = wave + 8 j)
+  const int half = lane >> 5, sp = lane & 31;
+#define PCAONE_ISSUE_B(CH, BUF)                                                                          \
+  do {                                                                                                   \
+    _Pragma("unroll") for (int j = 0; j < NT; ++j) {                                                     \
+      const int i = wave + 8 * j;                                                                        \
+      const int row = 2 * i + half;                                                                      \
+      const __bf16* src = B + (uint64_t)row * g.ldb + (uint64_t)(CH) * 256 + (sp ^ (row & 7)) * 8;       \
+      __builtin_amdgcn_global_load_lds((const void*)src,                                                 \
+                                       (__attribute__((address_space(3))) void*)(&bs[BUF][2 * i * 256]), \
+                                       16, 0, 0);                                                        \
+    }                                                                                                    \
+  } while (0)
+  uint4 an[4];
+  if (c0 < c1) {
+    PCAONE_ISSUE_B(c0, 0);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) an[f] = *reinterpret_cast<const uint4*>(arow[f] + (uint64_t)c0 * 64);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+  const int bn = lane & 15, bg = lane >> 4;
+  for (uint32_t c = c0; c < c1; ++c) {
+    const int cur = (int)((c - c0) & 1);
+    uint4 ar[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) ar[f] = an[f];
+    if (c + 1 < c1) {
+      PCAONE_ISSUE_B(c + 1, cur ^ 1);
+#pragma unroll
+      for (int f = 0; f < 4; ++f) an[f] = *reinterpret_cast<const uint4*>(arow[f] + (uint64_t)(c + 1) * 64);
+    }
+    const __bf16* bl = bs[cur];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {  // 32-bit word q of each fragment: K steps 2q, 2q + 1
+      const int sg = 8 * bg + 2 * q;  // 16-byte segment of positions 64 bg + 16 q
+      bf16x8 b0[NT], b1[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int row = t * 16 + bn;
+        b0[t] = *reinterpret_cast<const bf16x8*>(bl + row * 256 + ((sg ^ (row & 7)) * 8));
+        b1[t] = *reinterpret_cast<const bf16x8*>(bl + row * 256 + (((sg + 1) ^ (row & 7)) * 8));
+      }
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const uint32_t w = (q == 0) ? ar[f].x : (q == 1) ? ar[f].y : (q == 2) ? ar[f].z : ar[f].w;
+        bf16x8 a0, a1;
+        decode16(w, a0, a1);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0[t], acc[f][t], 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[t], acc[f][t], 0, 0, 0);
+      }
+    }
+    // this wave's loads of chunk c + 1 are in LDS; after the barrier every
+    // wave's are, and nobody reads buffer `cur` any more
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+#undef PCAONE_ISSUE_B
+  float* o = out + (uint64_t)split * g.nrows * g.ldo;
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t r = row0 + 16 * f + 4 * (lane >> 4) + i;
+      if (r < g.nrows) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) o[(uint64_t)r * g.ldo + t * 16 + (lane & 15)] = acc[f][t][i];
+      }
+    }
+}
+
+// streamed 8-wave blocks (512 rows, half the B traffic per row) for long row
+// ranges, 4-wave blocks where the rows would not fill the chip
 void launch_packed_gemm_bf16(const PackedGemm& g, int nt16, hipStream_t s) {
-  dim3 grid((uint32_t)ceil_div(g.nrows, 256), g.nsplit, g.nz);
+  const bool big = (uint64_t)g.nrows * g.nz >= 65536;
+  dim3 grid((uint32_t)ceil_div(g.nrows, big ? 512 : 256), g.nsplit, g.nz);
   switch (nt16) {
-#define PCAONE_NT(n) \
-  case n: hipLaunchKernelGGL(k_packed_gemm_bf16<n>, grid, dim3(256), 0, s, g); break;
+#define PCAONE_NT(n)                                                                     \
+  case n:                                                                                \
+    if (big)                                                                             \
+      hipLaunchKernelGGL((k_packed_gemm_bf16_stream<n>), grid, dim3(512), 0, s, g);      \
+    else                                                                                 \
+      hipLaunchKernelGGL((k_packed_gemm_bf16<n, 4>), grid, dim3(256), 0, s, g);          \
+    break;
     PCAONE_NT(1) PCAONE_NT(2) PCAONE_NT(3) PCAONE_NT(4) PCAONE_NT(5) PCAONE_NT(6) PCAONE_NT(7) PCAONE_NT(8)
 #undef PCAONE_NT
     default: throw Error(-1, "split-BF16 contraction supports at most 128 operand columns (l <= 64)");
@@ -436,12 +625,14 @@ constexpr uint32_t kTileLd = 65;
 // bh[w][c][k0 + q] <- tile[q][c] for q < nq, c < l (and the lo half for the
 // split arm); consecutive lanes take consecutive q.
 __device__ inline void store_operand_tile(const double* tile, uint32_t nq, uint32_t l, uint32_t w, uint32_t k0,
-                                          int bprec, void* bh, uint32_t bcols, uint32_t lo_off, uint64_t kw) {
-  for (uint32_t e = threadIdx.x; e < l * kFinRows; e += 256) {
+                                          int bprec, void* bh, uint32_t bcols, uint32_t lo_off, uint64_t kw,
+                                          int kperm) {
+  for (uint32_t e = threadIdx.x; e < l * kFinRows; e += blockDim.x) {
     const uint32_t c = e / kFinRows, q = e % kFinRows;
     if (q >= nq) continue;
     const double gp = tile[q * kTileLd + c];
-    const uint64_t o = ((uint64_t)w * bcols + c) * kw + k0 + q;
+    const uint64_t kk = kperm ? kperm16(k0 + q) : k0 + q;  // k0 is a multiple of 64
+    const uint64_t o = ((uint64_t)w * bcols + c) * kw + kk;
     if (bprec == PREC_FP64) {
       reinterpret_cast<double*>(bh)[o] = gp;
     } else if (bprec == PREC_FP32) {
@@ -451,7 +642,7 @@ __device__ inline void store_operand_tile(const double* tile, uint32_t nq, uint3
       const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
       __bf16* b = reinterpret_cast<__bf16*>(bh);
       b[o] = hi;
-      b[((uint64_t)w * bcols + lo_off + c) * kw + k0 + q] = lo;
+      b[((uint64_t)w * bcols + lo_off + c) * kw + kk] = lo;
     }
   }
 }
@@ -482,8 +673,9 @@ __device__ inline void store_g2(const FinalizeG& f, uint64_t i, double v) {
     reinterpret_cast<double*>(f.g2)[i] = v;
 }
 
-__global__ __launch_bounds__(256) void k_finalize_G(FinalizeG f) {
-  __shared__ double sh[4][64];
+// 16 waves: thread (c = t % 64, rs = t / 64) takes rows rs, rs + 16, ... of the chunk
+__global__ __launch_bounds__(1024) void k_finalize_G(FinalizeG f) {
+  __shared__ double sh[16][64];
   __shared__ double tile[kFinRows * kTileLd];
   const uint32_t t = threadIdx.x, c = t & 63, rs = t >> 6;
   const uint32_t w = f.w0 + blockIdx.y, chunk = blockIdx.x;
@@ -494,7 +686,7 @@ __global__ __launch_bounds__(256) void k_finalize_G(FinalizeG f) {
   const uint64_t nrow = f.r1 - f.r0;
   double acc2 = 0.0;
   if (c < f.l) {
-    for (uint32_t q = rs; q < nq; q += 4) {
+    for (uint32_t q = rs; q < nq; q += 16) {
       const uint32_t r = cb + q;
       const uint32_t rr = r - f.r0;
       double raw = 0.0;
@@ -521,16 +713,21 @@ __global__ __launch_bounds__(256) void k_finalize_G(FinalizeG f) {
   }
   sh[rs][c] = acc2;
   __syncthreads();
-  if (t < f.l)
-    f.s2p[((uint64_t)w * gridDim.x + chunk) * f.l + t] = (sh[0][t] + sh[1][t]) + (sh[2][t] + sh[3][t]);
-  store_operand_tile(tile, nq, f.l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw);
+  if (t < f.l) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = sh[2 * q][t] + sh[2 * q + 1][t];
+    f.s2p[((uint64_t)w * gridDim.x + chunk) * f.l + t] =
+        ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+  }
+  store_operand_tile(tile, nq, f.l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
 }
 
 void launch_finalize_G(const FinalizeG& f, hipStream_t s) {
   PCAONE_REQUIRE(f.l >= 1 && f.l <= 64, -1, "finalize: l must be in [1, 64]");
   if (f.w1 <= f.w0) return;
   dim3 grid((uint32_t)ceil_div(f.blocksize, kFinRows), f.w1 - f.w0);
-  hipLaunchKernelGGL(k_finalize_G, grid, dim3(256), 0, s, f);
+  hipLaunchKernelGGL(k_finalize_G, grid, dim3(1024), 0, s, f);
 }
 
 uint32_t finalize_chunks(uint32_t blocksize) { return (uint32_t)ceil_div(blocksize, kFinRows); }
@@ -562,7 +759,7 @@ __global__ __launch_bounds__(256) void k_operand_from_G(FinalizeG f) {
   __syncthreads();
   if (t < f.l)
     f.s2p[((uint64_t)w * gridDim.x + chunk) * f.l + t] = (sh[0][t] + sh[1][t]) + (sh[2][t] + sh[3][t]);
-  store_operand_tile(tile, nq, f.l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw);
+  store_operand_tile(tile, nq, f.l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
 }
 
 void launch_operand_from_G(const FinalizeG& f, hipStream_t s) {
@@ -787,11 +984,11 @@ void launch_cast_f32(const double* src, float* dst, uint64_t n, hipStream_t s) {
 }
 
 __global__ void k_omega_operand(const double* __restrict__ omega, uint32_t n, uint32_t l, int bprec, void* bg,
-                                uint32_t bcols, uint32_t lo_off, uint64_t kpad) {
+                                uint32_t bcols, uint32_t lo_off, uint64_t kpad, int kperm) {
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (uint64_t)bcols * kpad) return;
   const uint32_t col = (uint32_t)(gid / kpad);
-  const uint64_t k = gid % kpad;
+  const uint64_t k = kperm ? kperm16(gid % kpad) : gid % kpad;  // the sample this position holds
   if (bprec == PREC_SPLIT_BF16) {
     __bf16 v = (__bf16)0.0f;
     const bool is_hi = col < l, is_lo = col >= lo_off && col < lo_off + l;
@@ -811,10 +1008,11 @@ __global__ void k_omega_operand(const double* __restrict__ omega, uint32_t n, ui
 }
 
 void launch_omega_operand(const double* omega, uint32_t n, uint32_t l, int bprec, void* bg, uint32_t bcols,
-                          uint32_t lo_off, uint64_t kpad, hipStream_t s) {
+                          uint32_t lo_off, uint64_t kpad, int kperm, hipStream_t s) {
+  PCAONE_REQUIRE(!kperm || kpad % 16 == 0, -1, "omega operand: permuted K needs 16-code groups");
   const uint64_t total = (uint64_t)bcols * kpad;
   hipLaunchKernelGGL(k_omega_operand, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, omega, n, l, bprec,
-                     bg, bcols, lo_off, kpad);
+                     bg, bcols, lo_off, kpad, kperm);
 }
 
 __global__ void k_fill_f64(double* p, uint64_t n, double v) {

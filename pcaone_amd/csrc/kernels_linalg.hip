@@ -240,8 +240,7 @@ static size_t chol_lds_bytes(uint32_t l) { return (size_t)4 * l * l * sizeof(dou
 
 __global__ __launch_bounds__(256) void k_chol(const double* __restrict__ part, int nparts, uint32_t l, double* R,
                                               double* Rinv, const double* Rprev, const double* Rprev_inv,
-                                              double* Rtot, double* Rtot_inv, const double* Qtop, double* sign,
-                                              int* status) {
+                                              double* Rtot, double* Rtot_inv, int* status) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int L = (int)l, t = threadIdx.x;
   double* F = reinterpret_cast<double*>(smem_raw);  // Gram, then R^-1
@@ -281,34 +280,167 @@ __global__ __launch_bounds__(256) void k_chol(const double* __restrict__ part, i
     }
     __syncthreads();
   }
-  if (sign) {
-    // top block of this pass's Q = Qtop R^-1
-    for (int e = t; e < L * L; e += 256) P[e] = Qtop[e];
-    __syncthreads();
-    for (int e = t; e < L * L; e += 256) {
-      const int i = e / L, j = e % L;
-      double v = 0.0;
-      for (int p = 0; p <= j; ++p) v += P[i * L + p] * X[p * L + j];
-      Pi[e] = v;
-    }
-    __syncthreads();
-    householder_signs_lu(Pi, L, sign);
-  }
   if (t == 0 && status) *status = bad;
 }
 
-void launch_chol(const double* part, int nparts, uint32_t l, double* R, double* Rinv, const double* Rprev,
-                 const double* Rprev_inv, double* Rtot, double* Rtot_inv, const double* Qtop, double* sign,
-                 int* status, hipStream_t s) {
+// ---------------------------------------------------------------------------
+// the same factorisations on one wave: lane j holds column j in registers
+// ---------------------------------------------------------------------------
+__device__ inline double bcast(double v, int lane) {  // v of `lane`, to every lane (lane is uniform)
+  const int2 w = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, int2{__builtin_amdgcn_readlane(w.x, lane), __builtin_amdgcn_readlane(w.y, lane)});
+}
+
+// Rows and columns >= l are padded with the identity, so every step runs to
+// the compile-time bound LM and the padding factors into itself.
+template <int LM>
+__global__ __launch_bounds__(64) void k_chol_wave(const double* __restrict__ part, uint32_t l, double* R,
+                                                  double* Rinv, const double* Rprev, const double* Rprev_inv,
+                                                  double* Rtot, double* Rtot_inv, int* status) {
+  __shared__ double qs[LM * LM];
+  __shared__ double Rs[LM * LM];
+  const int j = threadIdx.x;
+  const int L = (int)l;
+  double a[LM], x[LM];
+#pragma unroll
+  for (int i = 0; i < LM; ++i) a[i] = (i < L && j < L) ? part[i * L + j] : (i == j ? 1.0 : 0.0);
+  int bad = 0;
+  // Cholesky R^T R = A: after step k lane j's a[k] is R[k][j]
+#pragma unroll
+  for (int k = 0; k < LM; ++k) {
+    double d = bcast(a[k], k);
+    if (!(d > 0.0) || !isfinite(d)) {
+      if (bad == 0) bad = k + 1;
+      d = 1e-300;
+    }
+    const double sd = sqrt(d), isd = 1.0 / sd;
+    const double rk = (j == k) ? sd : (j > k ? a[k] * isd : 0.0);
+    a[k] = (j >= k) ? rk : a[k];
+#pragma unroll
+    for (int i = k + 1; i < LM; ++i) {
+      const double rki = bcast(a[k], i);  // R[k][i]
+      a[i] = (i <= j) ? fma(-rki, rk, a[i]) : a[i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < LM; ++i) a[i] = (i <= j) ? a[i] : 0.0;  // R column j (upper)
+  // X = R^-1 by back substitution of R x = e_j
+#pragma unroll
+  for (int i = LM - 1; i >= 0; --i) {
+    double s = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int p = i + 1; p < LM; ++p) s = fma(-bcast(a[i], p), x[p], s);  // R[i][p] x_p
+    x[i] = s / bcast(a[i], i);
+  }
+  if (j < L)
+#pragma unroll
+    for (int i = 0; i < LM; ++i)
+      if (i < L) {
+        R[i * L + j] = a[i];
+        Rinv[i * L + j] = x[i];
+      }
+  // staged square matrices are padded with the identity up to LM
+  auto stage = [&](double* dst, const double* src) {
+    for (int e = j; e < LM * LM; e += 64) {
+      const int r = e / LM, c = e % LM;
+      dst[e] = (r < L && c < L) ? src[r * L + c] : (r == c ? 1.0 : 0.0);
+    }
+  };
+  if (Rprev) {
+    // Rtot = R Rprev, Rtot_inv = Rprev_inv X (column j of each)
+    if (j < LM)
+#pragma unroll
+      for (int i = 0; i < LM; ++i) Rs[i * LM + j] = a[i];
+    stage(qs, Rprev);
+    __syncthreads();
+    if (j < L)
+      for (int i = 0; i < L; ++i) {
+        double v = 0.0;
+        for (int p = 0; p < LM; ++p) v = fma(Rs[i * LM + p], qs[p * LM + j], v);
+        Rtot[i * L + j] = v;
+      }
+    __syncthreads();
+    stage(qs, Rprev_inv);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < LM; ++i) {
+      double w = 0.0;
+#pragma unroll
+      for (int p = 0; p < LM; ++p) w = fma(qs[i * LM + p], x[p], w);
+      if (i < L && j < L) Rtot_inv[i * L + j] = w;
+    }
+    __syncthreads();
+  }
+  if (j == 0 && status) *status = bad;
+}
+
+// The Householder R-diagonal signs from the top l x l block B (row-major,
+// row stride l) of the CholeskyQR factor, lane j holding column j.
+template <int LM>
+__global__ __launch_bounds__(64) void k_signs_wave(const double* __restrict__ B, uint32_t l, double* sign) {
+  const int j = threadIdx.x;
+  const int L = (int)l;
+  double a[LM];
+#pragma unroll
+  for (int i = 0; i < LM; ++i) a[i] = (i < L && j < L) ? B[i * L + j] : (i == j ? 1.0 : 0.0);
+#pragma unroll
+  for (int k = 0; k < LM; ++k) {
+    const double bkk = bcast(a[k], k);
+    const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
+    const double ip = 1.0 / (bkk - sk);
+    if (j == 0 && k < L) sign[k] = sk;
+#pragma unroll
+    for (int i = k + 1; i < LM; ++i) {
+      const double lik = bcast(a[i], k) * ip;  // B[i][k] / pivot
+      a[i] = (j > k) ? fma(-lik, a[k], a[i]) : a[i];
+    }
+  }
+}
+
+// l > 48: the same LU over a 256-thread workgroup in LDS
+__global__ __launch_bounds__(256) void k_signs_lds(const double* __restrict__ B, uint32_t l, double* sign) {
+  __shared__ double b[kMaxL * kMaxL];
+  for (uint32_t e = threadIdx.x; e < l * l; e += 256) b[e] = B[e];
+  __syncthreads();
+  householder_signs_lu(b, (int)l, sign);
+}
+
+void launch_householder_signs(const double* B, uint32_t l, double* sign, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
-  PCAONE_REQUIRE(!sign || Qtop, -1, "chol: Householder signs need the top block");
+  if (l <= 16)
+    hipLaunchKernelGGL(k_signs_wave<16>, dim3(1), dim3(64), 0, s, B, l, sign);
+  else if (l <= 32)
+    hipLaunchKernelGGL(k_signs_wave<32>, dim3(1), dim3(64), 0, s, B, l, sign);
+  else if (l <= 48)
+    hipLaunchKernelGGL(k_signs_wave<48>, dim3(1), dim3(64), 0, s, B, l, sign);
+  else
+    hipLaunchKernelGGL(k_signs_lds, dim3(1), dim3(256), 0, s, B, l, sign);
+}
+
+void launch_chol(const double* part, int nparts, uint32_t l, double* R, double* Rinv, const double* Rprev,
+                 const double* Rprev_inv, double* Rtot, double* Rtot_inv, int* status, hipStream_t s) {
+  PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
+  if (l <= 48) {
+    reduce_parts(part, nparts, l * l, s);
+#define PCAONE_CW(LMV)                                                                                    \
+  hipLaunchKernelGGL(k_chol_wave<LMV>, dim3(1), dim3(64), 0, s, part, l, R, Rinv, Rprev, Rprev_inv, Rtot, \
+                     Rtot_inv, status)
+    if (l <= 16)
+      PCAONE_CW(16);
+    else if (l <= 32)
+      PCAONE_CW(32);
+    else
+      PCAONE_CW(48);
+#undef PCAONE_CW
+    return;
+  }
   nparts = reduce_parts(part, nparts, l * l, s);
   const size_t shm = chol_lds_bytes(l);
   if (shm > 65536)
     PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chol),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   hipLaunchKernelGGL(k_chol, dim3(1), dim3(256), shm, s, part, nparts, l, R, Rinv, Rprev, Rprev_inv, Rtot, Rtot_inv,
-                     Qtop, sign, status);
+                     status);
 }
 
 // ---------------------------------------------------------------------------
@@ -410,15 +542,17 @@ void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, 
 // flipOmg
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_flip_stats(const double* __restrict__ Q, const double* __restrict__ omg2,
-                                                    uint64_t nrows, uint32_t l, double* __restrict__ part) {
+                                                    const double* __restrict__ sign, uint64_t nrows, uint32_t l,
+                                                    double* __restrict__ part) {
   __shared__ double sh[3][256];
   const uint64_t rpb = ceil_div(nrows, gridDim.x);
   const uint64_t rs = blockIdx.x * rpb, re = std::min<uint64_t>(rs + rpb, nrows);
   const uint32_t t = threadIdx.x, R = 256 / l, c = t % l, rr = t / l;
   double dm = 0.0, dp = 0.0, cs = 0.0;
+  const double sg = (sign && rr < R) ? sign[c] : 1.0;
   if (rr < R)
     for (uint64_t r = rs + rr; r < re; r += R) {
-      const double q = Q[r * l + c], o = omg2[r * l + c];
+      const double q = sg * Q[r * l + c], o = omg2[r * l + c];
       dm += fabs(o - q);
       dp += fabs(o + q);
       cs += q;
@@ -442,9 +576,9 @@ __global__ __launch_bounds__(256) void k_flip_stats(const double* __restrict__ Q
 }
 
 __global__ __launch_bounds__(256) void k_flip_apply(double* __restrict__ Q, double* __restrict__ omg2,
-                                                    uint64_t nrows, uint32_t l, const double* __restrict__ part,
-                                                    int nparts, double* __restrict__ colsum,
-                                                    double* __restrict__ zero_acc) {
+                                                    const double* __restrict__ sign, uint64_t nrows, uint32_t l,
+                                                    const double* __restrict__ part, int nparts,
+                                                    double* __restrict__ colsum, double* __restrict__ zero_acc) {
   __shared__ double flip[kMaxL];
   const uint32_t t = threadIdx.x;
   if (t < l) {
@@ -455,7 +589,7 @@ __global__ __launch_bounds__(256) void k_flip_apply(double* __restrict__ Q, doub
       d += part[(uint64_t)p * 3 * l + 2 * l + t];
     }
     const double f = (a > 2.0 * b) ? -1.0 : 1.0;
-    flip[t] = f;
+    flip[t] = f * (sign ? sign[t] : 1.0);  // d already holds the signed column sums
     if (blockIdx.x == 0) colsum[t] = f * d;
   }
   __syncthreads();
@@ -468,17 +602,17 @@ __global__ __launch_bounds__(256) void k_flip_apply(double* __restrict__ Q, doub
   }
 }
 
-void launch_flip_stats(const double* Q, const double* omg2, uint64_t nrows, uint32_t l, double* part,
-                       int nblocks, hipStream_t s) {
+void launch_flip_stats(const double* Q, const double* omg2, const double* sign, uint64_t nrows, uint32_t l,
+                       double* part, int nblocks, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
-  hipLaunchKernelGGL(k_flip_stats, dim3(nblocks), dim3(256), 0, s, Q, omg2, nrows, l, part);
+  hipLaunchKernelGGL(k_flip_stats, dim3(nblocks), dim3(256), 0, s, Q, omg2, sign, nrows, l, part);
 }
 
-void launch_flip_apply(double* Q, double* omg2, uint64_t nrows, uint32_t l, const double* part, int nblocks,
-                       double* colsum, double* zero_acc, hipStream_t s) {
+void launch_flip_apply(double* Q, double* omg2, const double* sign, uint64_t nrows, uint32_t l, const double* part,
+                       int nblocks, double* colsum, double* zero_acc, hipStream_t s) {
   nblocks = reduce_parts(part, nblocks, 3 * l, s);
   const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(nrows * l, 256), 1024);
-  hipLaunchKernelGGL(k_flip_apply, dim3(blocks), dim3(256), 0, s, Q, omg2, nrows, l, part, nblocks, colsum,
+  hipLaunchKernelGGL(k_flip_apply, dim3(blocks), dim3(256), 0, s, Q, omg2, sign, nrows, l, part, nblocks, colsum,
                      zero_acc);
 }
 
@@ -522,13 +656,16 @@ __device__ inline void rr_pair(int t, int round, int n, int& p, int& q) {
   }
 }
 
+// W0 (optional): a starting basis (the previous eigenvectors); the sweeps then
+// run on W0^T A W0 and the rotations accumulate onto W0.  W may alias W0.
 __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part, int nparts, uint32_t l,
-                                                 double* __restrict__ evals, double* __restrict__ W,
-                                                 double* __restrict__ sinv, uint32_t kc) {
+                                                 double* evals, double* W, double* sinv, uint32_t kc,
+                                                 const double* W0) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int L = (int)l, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double* a = reinterpret_cast<double*>(smem_raw);
   double* v = a + L * L;
+  double* tmp = v + L * L;
   __shared__ double cs[kMaxL / 2 + 1][2];
   __shared__ int pr[kMaxL / 2 + 1][2];
   __shared__ double red[4][2];
@@ -552,9 +689,31 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
     double s = 0.0;
     for (int p = 0; p < nparts; ++p) s += part[(uint64_t)p * L * L + e];
     a[e] = s;
-    v[e] = (e / L == e % L) ? 1.0 : 0.0;
+    v[e] = W0 ? W0[e] : ((e / L == e % L) ? 1.0 : 0.0);
   }
   __syncthreads();
+  if (W0) {
+    // a <- W0^T (a W0)
+    for (int e = t; e < L * L; e += 256) {
+      const int i = e / L, j = e % L;
+      double s = 0.0;
+      for (int p = 0; p < L; ++p) s = fma(a[i * L + p], v[p * L + j], s);
+      tmp[e] = s;
+    }
+    __syncthreads();
+    for (int e = t; e < L * L; e += 256) {
+      const int i = e / L, j = e % L;
+      double s = 0.0;
+      for (int p = 0; p < L; ++p) s = fma(v[p * L + i], tmp[p * L + j], s);
+      a[e] = s;
+    }
+    __syncthreads();
+    for (int e = t; e < L * L; e += 256) {  // exact symmetry for the rotations
+      const int i = e / L, j = e % L;
+      if (i > j) a[e] = a[j * L + i];
+    }
+    __syncthreads();
+  }
   for (int sweep = 0; sweep < 40; ++sweep) {
     double off = 0.0, dg = 0.0;
     for (int e = t; e < L * L; e += 256) {
@@ -653,14 +812,14 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
 }
 
 void launch_sym_eig(const double* part, int nparts, uint32_t l, double* evals, double* W, double* sinv,
-                    uint32_t kc, hipStream_t s) {
+                    uint32_t kc, const double* W0, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
   nparts = reduce_parts(part, nparts, l * l, s);
-  const size_t shm = (size_t)2 * l * l * sizeof(double);
+  const size_t shm = (size_t)3 * l * l * sizeof(double);
   if (shm > 65536)
     PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sym_eig),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-  hipLaunchKernelGGL(k_sym_eig, dim3(1), dim3(256), shm, s, part, nparts, l, evals, W, sinv, kc);
+  hipLaunchKernelGGL(k_sym_eig, dim3(1), dim3(256), shm, s, part, nparts, l, evals, W, sinv, kc, W0);
 }
 
 // ---------------------------------------------------------------------------
