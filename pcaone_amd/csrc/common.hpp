@@ -65,6 +65,14 @@ struct DevBuf {
 __host__ __device__ inline uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 __host__ __device__ inline uint64_t ceil_div(uint64_t x, uint64_t m) { return (x + m - 1) / m; }
 
+// K order of the split-BF16 packed contractions: within every aligned group of
+// 16 codes, operand position 4a + b holds code 4b + a (a 4 x 4 transpose, its
+// own inverse).  decode16 produces the A operand in this order straight from
+// one 32-bit word of packed codes, and the B operands are written in it.
+__host__ __device__ inline uint64_t kperm16(uint64_t k) {
+  return (k & ~(uint64_t)15) | ((k & 3) << 2) | ((k >> 2) & 3);
+}
+
 // Arithmetic arms of the packed-genotype contractions.
 enum Precision : int {
   PREC_FP64 = 0,        // decode -> double, FP64 FMA (oracle arm)

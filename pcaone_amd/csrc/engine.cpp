@@ -285,13 +285,11 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   gpart_.reserve(gmax * ldp * partsz);
   hpart_.reserve(hmax * ldp * partsz);
   // linear algebra scratch
-  part_.reserve((uint64_t)1024 * l_ * l_);
-  fpart_.reserve((uint64_t)1024 * 3 * l_);
-  for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &W_}) b->reserve((uint64_t)l_ * l_);
+  part_.reserve(std::max<uint64_t>((uint64_t)1024 * l_ * l_, qr_part_doubles(l_)));
+  for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &Rf_, &W_}) b->reserve((uint64_t)l_ * l_);
   sign_.reserve(l_);
   evals_.reserve(l_);
   sinv_.reserve(l_);
-  Q_.reserve((uint64_t)N_ * l_);
   Bt_.reserve((uint64_t)N_ * l_);
   Ucur_.reserve((uint64_t)N_ * p.k);
   Upre_.reserve((uint64_t)N_ * p.k);
@@ -645,35 +643,38 @@ void Engine::init_omega() {
 }
 
 // Omega <- Householder Q of H (H = H1 + H2 for winSVD), then flipOmg
-// (src/Halko.cpp:118-121, 213-216).  CholeskyQR2 in FP64 with the Householder
-// R-diagonal signs reproduced, so Q equals Eigen's householderQ() columns.
+// (src/Halko.cpp:118-121, 213-216): CholeskyQR2 in FP64 with the Householder
+// R-diagonal signs reproduced, so Q equals Eigen's householderQ() columns,
+// fused into four passes that also write the contraction operand and the
+// FP32 gather table.
 void Engine::omega_refresh(bool from_two, double* zero_acc) {
-  const uint32_t l = l_;
-  const int nb = gram_blocks(N_);
-  timed("qr_gram", [&] {
-    if (from_two)
-      launch_gram_partial(H1_.p, H2_.p, H_.p, N_, 0, l, part_.p, nb, stream_);
-    else
-      launch_gram_partial(H_.p, nullptr, nullptr, N_, 0, l, part_.p, nb, stream_);
-  });
-  timed("qr_chol", [&] {
-    launch_chol(part_.p, nb, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, status_.p, stream_);
-  });
-  timed("qr_apply", [&] { launch_apply_rinv(H_.p, R1i_.p, nullptr, Q_.p, N_, l, part_.p, nb, stream_); });
-  timed("qr_chol", [&] {
-    launch_chol(part_.p, nb, l, R2_.p, R2i_.p, nullptr, nullptr, nullptr, nullptr, status_.p + 1, stream_);
-  });
-  timed("qr_apply", [&] { launch_apply_rinv(Q_.p, R2i_.p, nullptr, omega_.p, N_, l, nullptr, nb, stream_); });
-  // the Householder signs from the top block of the CholeskyQR2 factor, applied
-  // together with flipOmg
-  timed("qr_signs", [&] { launch_householder_signs(omega_.p, l, sign_.p, stream_); });
-  timed("flip", [&] { launch_flip_stats(omega_.p, omg2_.p, sign_.p, N_, l, fpart_.p, nb, stream_); });
-  timed("flip",
-        [&] { launch_flip_apply(omega_.p, omg2_.p, sign_.p, N_, l, fpart_.p, nb, colsum_.p, zero_acc, stream_); });
-  timed("omega_operand",
-        [&] { launch_omega_operand(omega_.p, N_, l, p_.precision, bg_.p, bcols_, lo_off_, kg_, kperm_(p_.precision),
-                                   stream_); });
-  refresh_omega32(op_bufs());
+  const SegBufs b = op_bufs();
+  QrPass q{};
+  q.H1 = from_two ? H1_.p : H_.p;
+  q.H2 = from_two ? H2_.p : nullptr;
+  q.H = H_.p;
+  q.n = N_;
+  q.l = l_;
+  q.part = part_.p;
+  q.status = status_.p;
+  q.R1i = R1i_.p;
+  q.R2i = R2i_.p;
+  q.sign = sign_.p;
+  q.Rf = Rf_.p;
+  q.colsum = colsum_.p;
+  q.omega = omega_.p;
+  q.omg2 = omg2_.p;
+  q.zero_acc = zero_acc;
+  q.omega32 = (b.corrG && b.g2_f32) ? b.omega32 : nullptr;
+  q.bg = bg_.p;
+  q.bprec = p_.precision;
+  q.lo_off = lo_off_;
+  q.kpad = kg_;
+  q.kperm = kperm_(p_.precision);
+  timed("qr_gram_h", [&] { launch_qr_pass(q, 0, stream_); });
+  timed("qr_gram_q", [&] { launch_qr_pass(q, 1, stream_); });
+  timed("qr_flip", [&] { launch_qr_pass(q, 2, stream_); });
+  timed("qr_write", [&] { launch_qr_pass(q, 3, stream_); });
 }
 
 // the FP32 table the missing-entry gathers of the G contraction read
@@ -1084,23 +1085,19 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
 // last computeUSV
 void Engine::copy_usv(double* U, double* S, double* V) {
   const uint32_t k = (uint32_t)p_.k;
-  std::vector<double> u, v;
+  // the transposes and the un-shuffle of V's rows run on the device, so each
+  // output is one contiguous copy into the caller's buffer
+  if (U || V) out_.reserve((uint64_t)std::max(N_, M_) * k);
   if (U) {
-    u.resize((size_t)N_ * k);
-    PCAONE_HIP(hipMemcpyAsync(u.data(), Ucur_.p, u.size() * 8, hipMemcpyDeviceToHost, stream_));
+    timed("usv_out", [&] { launch_rows_to_cols(Ucur_.p, N_, k, nullptr, out_.p, stream_); });
+    PCAONE_HIP(hipMemcpyAsync(U, out_.p, (size_t)N_ * k * 8, hipMemcpyDeviceToHost, stream_));
   }
   if (V) {
-    v.resize((size_t)M_ * k);
-    PCAONE_HIP(hipMemcpyAsync(v.data(), V_.p, v.size() * 8, hipMemcpyDeviceToHost, stream_));
+    timed("usv_out", [&] { launch_rows_to_cols(V_.p, M_, k, d_perm_.p, out_.p, stream_); });
+    PCAONE_HIP(hipMemcpyAsync(V, out_.p, (size_t)M_ * k * 8, hipMemcpyDeviceToHost, stream_));
   }
   sync();
   if (S) std::memcpy(S, last_S_.data(), k * sizeof(double));
-  if (U)
-    for (uint32_t r = 0; r < N_; ++r)
-      for (uint32_t c = 0; c < k; ++c) U[(size_t)c * N_ + r] = u[(size_t)r * k + c];
-  if (V)
-    for (uint32_t r = 0; r < M_; ++r)
-      for (uint32_t c = 0; c < k; ++c) V[(size_t)c * M_ + perm_[r]] = v[(size_t)r * k + c];
 }
 
 // The EM branch of run_pca_with_halko for --emu (src/Halko.cpp:294-336):

@@ -211,8 +211,9 @@ class Engine {
   std::vector<double> last_S_;
   DevBuf<double> miss_val_, Sdev_, Vpre_;
   // linear algebra scratch
-  DevBuf<double> part_, fpart_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, sign_, Q_;
+  DevBuf<double> part_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, Rf_, sign_;
   DevBuf<double> Bt_, evals_, W_, sinv_, Ucur_, Upre_, V_, scal_;
+  DevBuf<double> out_;  // column-major U / V staged for the copy to the host
   DevBuf<int> status_;
   // Gram mat-vec state (one operand column)
   bool have_mv_ = false;

@@ -259,22 +259,46 @@ void launch_gram_partial(const double* A, const double* A2, double* Aout, uint64
 // optional Rprev: Rtot = R * Rprev and Rtot_inv = Rprev_inv * Rinv.
 void launch_chol(const double* part, int nparts, uint32_t l, double* R, double* Rinv, const double* Rprev,
                  const double* Rprev_inv, double* Rtot, double* Rtot_inv, int* status, hipStream_t s);
-// Householder R-diagonal signs of a tall matrix from the top l x l block B
-// (row-major) of its CholeskyQR factor: LU of B - S with S_kk = -sign(pivot)
-// (Householder reconstruction); R_householder = diag(sign) R_chol.
-void launch_householder_signs(const double* B, uint32_t l, double* sign, hipStream_t s);
 // Q = A * Rinv (rows), optional per-column sign; optional partial Gram of Q.
 void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, double* Q, uint64_t nrows,
                        uint32_t l, double* gram_part, int nblocks, hipStream_t s);
-// flipOmg against Omega2 + colsum on diag(sign) Q (sign optional): stats per
-// block, then Q <- diag(flip * sign) Q, Omega2 <- Q.
-void launch_flip_stats(const double* Q, const double* omg2, const double* sign, uint64_t nrows, uint32_t l,
-                       double* part, int nblocks, hipStream_t s);
-void launch_flip_apply(double* Q, double* omg2, const double* sign, uint64_t nrows, uint32_t l, const double* part,
-                       int nblocks, double* colsum, double* zero_acc, hipStream_t s);
+// ---- the winSVD Omega update in four passes (kernels_qr.hip) ----------------
+// Omega <- Householder-signed CholeskyQR2 Q of H (H = H1 + H2 when H2 is
+// given), flipped against the previous Omega (omg2), written to every consumer.
+struct QrPass {
+  const double* H1;  // pass 0 input (the only input when H2 is null)
+  const double* H2;  // optional second accumulator: H1 + H2 is written to H
+  double* H;         // H (passes 1..3 read it; must equal H1 when H2 is null)
+  uint64_t n;        // rows (samples), n >= 2 l
+  uint32_t l;        // columns, 1..64
+  double* part;      // qr_part_doubles(l) scratch, kept between the four passes
+  int nparts;        // blocks of the reduction passes (set by launch_qr_pass)
+  int* status;       // [2]: first failing pivot of the two factorisations (sticky, 0 = none)
+  double* R1i;       // l x l outputs: R1^-1, R2^-1, Householder signs, R2^-1 diag(sign flip)
+  double* R2i;
+  double* sign;
+  double* Rf;
+  double* colsum;    // l: column sums of the new Omega
+  double* omega;     // n x l FP64
+  double* omg2;      // n x l: previous Omega in, new Omega out
+  double* zero_acc;  // optional n x l accumulator to clear
+  float* omega32;    // optional n x l FP32 copy
+  void* bg;          // contraction operand [bcols][kpad] of bprec
+  int bprec;
+  uint32_t lo_off;   // split BF16: column of the low halves
+  uint64_t kpad;
+  int kperm;         // operand positions in kperm16 order
+};
+uint64_t qr_part_doubles(uint32_t l);
+// pass 0: H and R1^-1; 1: R2^-1 and signs; 2: flip and colsum; 3: the writes.
+void launch_qr_pass(const QrPass& q, int pass, hipStream_t s);
 // small dense: C[n][kc] = A[n][l] * W[l][ldw] (first kc columns) * diag(scale)
 void launch_rows_times_small(const double* A, uint64_t n, uint32_t l, const double* W, uint32_t ldw,
                              uint32_t kc, const double* scale, double* C, hipStream_t s);
+// out[c * n + perm[r]] = A[r * kc + c] (perm null: identity); perm must be a
+// permutation of [0, n)
+void launch_rows_to_cols(const double* A, uint64_t n, uint32_t kc, const int32_t* perm, double* out,
+                         hipStream_t s);
 // symmetric eigen-decomposition of sum(part) (l x l) by cyclic Jacobi in one
 // workgroup; eigenvalues descending into evals, eigenvectors (columns) into W.
 // W0 (optional, may alias W): start from this basis (the previous eigenvectors).

@@ -244,14 +244,6 @@ __device__ inline bf16x8 decode8(uint32_t h) {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// K order of the split-BF16 packed contractions: within every aligned group of
-// 16 codes, operand position 4a + b holds code 4b + a (a 4 x 4 transpose, its
-// own inverse).  decode16 produces the A operand in this order straight from
-// one 32-bit word of packed codes, and the B operands are written in it.
-__host__ __device__ inline uint64_t kperm16(uint64_t k) {
-  return (k & ~(uint64_t)15) | ((k & 3) << 2) | ((k >> 2) & 3);
-}
-
 // 16 codes (one 32-bit word) -> the A operands of two MFMA K steps, in kperm16
 // order: spread word x_j = (w >> 2j) & 0x03030303 holds codes j, j+4, j+8,
 // j+12 one per byte; each becomes four BF16 dosages through the v_perm tables.

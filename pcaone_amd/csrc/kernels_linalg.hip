@@ -1,9 +1,8 @@
 // FP64 tall-skinny linear algebra for the RSVD steps whose stability needs it:
-// CholeskyQR2 of the N x l / M x l blocks (Gram -> Cholesky -> triangular
-// inverse -> apply), the Householder R-diagonal signs that make the CholeskyQR
-// Q equal to the Householder Q the reference uses for Omega, the sign
-// stabiliser flipOmg (src/RSVD.hpp:81-90), the l x l symmetric eigensolve for
-// the SVD of B, and the mev convergence metric (src/Utils.cpp:192-198).
+// CholeskyQR2 of the M x l block of computeUSV (Gram -> Cholesky -> triangular
+// inverse -> apply), the l x l symmetric eigensolve for the SVD of B, and the
+// mev convergence metric (src/Utils.cpp:192-198).  The Omega update (its
+// CholeskyQR2, Householder signs and flipOmg) is fused in kernels_qr.hip.
 //
 // All reductions are two-level (per-block partials, then one reduction in a
 // fixed order), so every result is deterministic run to run.
@@ -213,28 +212,6 @@ __device__ void upper_inverse_lds(const double* a, double* X, int l) {
   }
 }
 
-// Signs of the Householder R diagonal of a tall matrix whose CholeskyQR factor
-// Q (positive R diagonal) has top l x l block B: LU of B - S with
-// S_kk = -sign(pivot) chosen at each step (Householder reconstruction, Ballard
-// et al. 2014); Householder QR with beta_k = -sign(alpha_k) ||x_k||, as in
-// LAPACK dgeqrf and Eigen's HouseholderQR, has R = S R_chol.  (A reflector
-// with an exactly zero tail keeps beta_k = alpha_k in LAPACK; a matrix with at
-// least 2l rows and full column rank does not produce one in practice.)
-__device__ void householder_signs_lu(double* B, int l, double* sign) {
-  const int t = threadIdx.x;
-  for (int k = 0; k < l; ++k) {
-    const double bkk = B[k * l + k];
-    const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
-    const double ip = 1.0 / (bkk - sk);
-    if (t == 0) sign[k] = sk;
-    for (int i = k + 1 + (t >> 4); i < l; i += 16) {
-      const double lik = B[i * l + k] * ip;
-      for (int j = k + 1 + (t & 15); j < l; j += 16) B[i * l + j] -= lik * B[k * l + j];
-    }
-    __syncthreads();
-  }
-}
-
 // LDS: four l x l slots.
 static size_t chol_lds_bytes(uint32_t l) { return (size_t)4 * l * l * sizeof(double); }
 
@@ -280,7 +257,7 @@ __global__ __launch_bounds__(256) void k_chol(const double* __restrict__ part, i
     }
     __syncthreads();
   }
-  if (t == 0 && status) *status = bad;
+  if (t == 0 && status && bad && *status == 0) *status = bad;  // sticky until the host reads it
 }
 
 // ---------------------------------------------------------------------------
@@ -371,50 +348,7 @@ __global__ __launch_bounds__(64) void k_chol_wave(const double* __restrict__ par
     }
     __syncthreads();
   }
-  if (j == 0 && status) *status = bad;
-}
-
-// The Householder R-diagonal signs from the top l x l block B (row-major,
-// row stride l) of the CholeskyQR factor, lane j holding column j.
-template <int LM>
-__global__ __launch_bounds__(64) void k_signs_wave(const double* __restrict__ B, uint32_t l, double* sign) {
-  const int j = threadIdx.x;
-  const int L = (int)l;
-  double a[LM];
-#pragma unroll
-  for (int i = 0; i < LM; ++i) a[i] = (i < L && j < L) ? B[i * L + j] : (i == j ? 1.0 : 0.0);
-#pragma unroll
-  for (int k = 0; k < LM; ++k) {
-    const double bkk = bcast(a[k], k);
-    const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
-    const double ip = 1.0 / (bkk - sk);
-    if (j == 0 && k < L) sign[k] = sk;
-#pragma unroll
-    for (int i = k + 1; i < LM; ++i) {
-      const double lik = bcast(a[i], k) * ip;  // B[i][k] / pivot
-      a[i] = (j > k) ? fma(-lik, a[k], a[i]) : a[i];
-    }
-  }
-}
-
-// l > 48: the same LU over a 256-thread workgroup in LDS
-__global__ __launch_bounds__(256) void k_signs_lds(const double* __restrict__ B, uint32_t l, double* sign) {
-  __shared__ double b[kMaxL * kMaxL];
-  for (uint32_t e = threadIdx.x; e < l * l; e += 256) b[e] = B[e];
-  __syncthreads();
-  householder_signs_lu(b, (int)l, sign);
-}
-
-void launch_householder_signs(const double* B, uint32_t l, double* sign, hipStream_t s) {
-  PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
-  if (l <= 16)
-    hipLaunchKernelGGL(k_signs_wave<16>, dim3(1), dim3(64), 0, s, B, l, sign);
-  else if (l <= 32)
-    hipLaunchKernelGGL(k_signs_wave<32>, dim3(1), dim3(64), 0, s, B, l, sign);
-  else if (l <= 48)
-    hipLaunchKernelGGL(k_signs_wave<48>, dim3(1), dim3(64), 0, s, B, l, sign);
-  else
-    hipLaunchKernelGGL(k_signs_lds, dim3(1), dim3(256), 0, s, B, l, sign);
+  if (j == 0 && status && bad && *status == 0) *status = bad;  // sticky until the host reads it
 }
 
 void launch_chol(const double* part, int nparts, uint32_t l, double* R, double* Rinv, const double* Rprev,
@@ -539,84 +473,6 @@ void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, 
 }
 
 // ---------------------------------------------------------------------------
-// flipOmg
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_flip_stats(const double* __restrict__ Q, const double* __restrict__ omg2,
-                                                    const double* __restrict__ sign, uint64_t nrows, uint32_t l,
-                                                    double* __restrict__ part) {
-  __shared__ double sh[3][256];
-  const uint64_t rpb = ceil_div(nrows, gridDim.x);
-  const uint64_t rs = blockIdx.x * rpb, re = std::min<uint64_t>(rs + rpb, nrows);
-  const uint32_t t = threadIdx.x, R = 256 / l, c = t % l, rr = t / l;
-  double dm = 0.0, dp = 0.0, cs = 0.0;
-  const double sg = (sign && rr < R) ? sign[c] : 1.0;
-  if (rr < R)
-    for (uint64_t r = rs + rr; r < re; r += R) {
-      const double q = sg * Q[r * l + c], o = omg2[r * l + c];
-      dm += fabs(o - q);
-      dp += fabs(o + q);
-      cs += q;
-    }
-  sh[0][t] = (rr < R) ? dm : 0.0;
-  sh[1][t] = (rr < R) ? dp : 0.0;
-  sh[2][t] = (rr < R) ? cs : 0.0;
-  __syncthreads();
-  if (t < l) {
-    double a = 0, b = 0, d = 0;
-    for (uint32_t q = 0; q < R; ++q) {
-      a += sh[0][q * l + t];
-      b += sh[1][q * l + t];
-      d += sh[2][q * l + t];
-    }
-    double* p = part + (uint64_t)blockIdx.x * 3 * l;
-    p[t] = a;
-    p[l + t] = b;
-    p[2 * l + t] = d;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_flip_apply(double* __restrict__ Q, double* __restrict__ omg2,
-                                                    const double* __restrict__ sign, uint64_t nrows, uint32_t l,
-                                                    const double* __restrict__ part, int nparts,
-                                                    double* __restrict__ colsum, double* __restrict__ zero_acc) {
-  __shared__ double flip[kMaxL];
-  const uint32_t t = threadIdx.x;
-  if (t < l) {
-    double a = 0, b = 0, d = 0;
-    for (int p = 0; p < nparts; ++p) {
-      a += part[(uint64_t)p * 3 * l + t];
-      b += part[(uint64_t)p * 3 * l + l + t];
-      d += part[(uint64_t)p * 3 * l + 2 * l + t];
-    }
-    const double f = (a > 2.0 * b) ? -1.0 : 1.0;
-    flip[t] = f * (sign ? sign[t] : 1.0);  // d already holds the signed column sums
-    if (blockIdx.x == 0) colsum[t] = f * d;
-  }
-  __syncthreads();
-  const uint64_t n = nrows * l;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + t; i < n; i += (uint64_t)gridDim.x * 256) {
-    const double q = Q[i] * flip[i % l];
-    Q[i] = q;
-    omg2[i] = q;
-    if (zero_acc) zero_acc[i] = 0.0;
-  }
-}
-
-void launch_flip_stats(const double* Q, const double* omg2, const double* sign, uint64_t nrows, uint32_t l,
-                       double* part, int nblocks, hipStream_t s) {
-  PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
-  hipLaunchKernelGGL(k_flip_stats, dim3(nblocks), dim3(256), 0, s, Q, omg2, sign, nrows, l, part);
-}
-
-void launch_flip_apply(double* Q, double* omg2, const double* sign, uint64_t nrows, uint32_t l, const double* part,
-                       int nblocks, double* colsum, double* zero_acc, hipStream_t s) {
-  nblocks = reduce_parts(part, nblocks, 3 * l, s);
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(nrows * l, 256), 1024);
-  hipLaunchKernelGGL(k_flip_apply, dim3(blocks), dim3(256), 0, s, Q, omg2, sign, nrows, l, part, nblocks, colsum,
-                     zero_acc);
-}
-
-// ---------------------------------------------------------------------------
 // small dense helpers
 // ---------------------------------------------------------------------------
 __global__ void k_rows_times_small(const double* __restrict__ A, uint64_t n, uint32_t l,
@@ -637,6 +493,24 @@ void launch_rows_times_small(const double* A, uint64_t n, uint32_t l, const doub
   if (!total) return;
   hipLaunchKernelGGL(k_rows_times_small, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, A, n, l, W, ldw,
                      kc, scale, C);
+}
+
+// out (column-major, n x kc) <- A (row-major, n x kc) with row r moved to
+// perm[r] (identity when perm is null): one thread per element, reads coalesced.
+__global__ void k_rows_to_cols(const double* __restrict__ A, uint64_t n, uint32_t kc,
+                               const int32_t* __restrict__ perm, double* __restrict__ out) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n * kc) return;
+  const uint64_t r = gid / kc;
+  const uint32_t c = (uint32_t)(gid % kc);
+  out[(uint64_t)c * n + (perm ? (uint64_t)perm[r] : r)] = A[gid];
+}
+
+void launch_rows_to_cols(const double* A, uint64_t n, uint32_t kc, const int32_t* perm, double* out,
+                         hipStream_t s) {
+  const uint64_t total = n * kc;
+  if (!total) return;
+  hipLaunchKernelGGL(k_rows_to_cols, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, A, n, kc, perm, out);
 }
 
 // Cyclic Jacobi with round-robin pair ordering: each round applies n/2

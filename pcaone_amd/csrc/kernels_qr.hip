@@ -1,0 +1,699 @@
+// The winSVD Omega update in four launches: Omega <- Householder Q of H
+// (H = H1 + H2), then flipOmg (src/Halko.cpp:213-216, src/RSVD.hpp:81-90 on
+// Eigen's HouseholderQR).  The Q is CholeskyQR2's, with the Householder
+// R-diagonal signs reproduced:
+//
+//   k_qr_gram_h  H = H1 + H2 (written); block partials of A1 = H^T H
+//   k_qr_gram_q  prologue: A1 = R1^T R1, R1^-1;  pass: Q1 = H R1^-1 on the fly,
+//                block partials of A2 = Q1^T Q1
+//   k_qr_flip    prologue: A2 = R2^T R2, R2^-1, the Householder signs of the top
+//                l x l block of Q1 R2^-1;  pass: Omega_u = Q1 R2^-1 on the fly,
+//                block partials of flipOmg's column statistics against the
+//                previous Omega
+//   k_qr_write   prologue: the flips, Rf = R2^-1 diag(sign flip);  pass:
+//                Omega = Q1 Rf into every consumer: the FP64 matrix, the copy
+//                kept for the next flip, the contraction operand (split BF16 /
+//                FP32 / FP64, K-permuted for the packed-genotype kernels), the
+//                FP32 table of the missing-entry gathers; the emptied H
+//                accumulator is zeroed.
+//
+// The tall N x l data is read once per pass.  Each prologue reduces the
+// previous pass's block partials in a fixed order (deterministic) and runs the
+// l x l step redundantly in every block (block 0 stores the results), so no
+// block ever waits for another inside a launch.
+#include <algorithm>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pcaone {
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 512;  // 8 waves per block for the reduction passes
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxBlocks = 64;
+
+// v_mfma_f64_16x16x4_f64: lane (k = lane / 16, i = lane % 16) supplies A[i][k]
+// and B[k][i]; accumulator element v holds D[lane / 16 + 4 v][lane % 16].
+__device__ inline f64x4 mfma64(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <int NB>
+struct Shape {
+  static constexpr int W = NB * 16;                 // padded l
+  static constexpr int NP = NB * (NB + 1) / 2;      // upper Gram tiles
+  static constexpr int GRAM = NP * 256;             // doubles per Gram partial
+  static constexpr int STAT = 3 * W;                // doubles per flip partial
+  static constexpr int SLOTS = (kWaves / 2) * GRAM; // block_sum_waves scratch
+};
+
+// The eight waves' accumulators summed into wave 0 (fixed tree order).
+template <int NP>
+__device__ inline void block_sum_waves(f64x4 (&g)[NP], double* slots) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int half = kWaves / 2; half >= 1; half >>= 1) {
+    __syncthreads();
+    if (wv >= half && wv < 2 * half)
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) slots[((wv - half) * NP + p) * 256 + v * 64 + lane] = g[p][v];
+    __syncthreads();
+    if (wv < half)
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) g[p][v] += slots[(wv * NP + p) * 256 + v * 64 + lane];
+  }
+}
+
+// Wave 0's Gram tiles to the block's partial.
+template <int NP>
+__device__ inline void store_gram(const f64x4 (&g)[NP], double* dst) {
+  const int lane = threadIdx.x & 63;
+  if ((threadIdx.x >> 6) == 0)
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dst[p * 256 + v * 64 + lane] = g[p][v];
+}
+
+// sum_b part[b * stride + e] for b < nparts (<= kMaxBlocks), in block order;
+// every load is issued before the first add.
+__device__ inline double sum_parts(const double* __restrict__ part, int nparts, uint64_t stride, uint64_t e) {
+  double v[kMaxBlocks];
+#pragma unroll
+  for (int b = 0; b < kMaxBlocks; ++b) v[b] = (b < nparts) ? part[(uint64_t)b * stride + e] : 0.0;
+  double s = 0.0;
+#pragma unroll
+  for (int b = 0; b < kMaxBlocks; ++b) s += v[b];
+  return s;
+}
+
+// The Gram of nparts block partials (tile p enumerates ca <= cb row by row;
+// element v * 64 + lane of a tile is D[lane / 16 + 4 v][lane % 16]) summed in
+// partial order into the upper triangle of a (W x W, row stride W).
+template <int NB>
+__device__ void gram_to_upper(const double* __restrict__ part, int nparts, uint32_t l, double* a) {
+  using S = Shape<NB>;
+  for (int e = threadIdx.x; e < S::GRAM; e += blockDim.x) {
+    const int p = e >> 8, v = (e >> 6) & 3, ln = e & 63;
+    int ca = 0, rem = p;
+    while (rem >= NB - ca) {
+      rem -= NB - ca;
+      ++ca;
+    }
+    const int cb = ca + rem;
+    const uint32_t r = ca * 16 + (ln >> 4) + 4 * v, c = cb * 16 + (ln & 15);
+    if (r >= l || c >= l || r > c) continue;
+    a[r * S::W + c] = sum_parts(part, nparts, S::GRAM, e);
+  }
+}
+
+// Upper Cholesky A = R^T R and R^-1 in l elimination steps: row operations
+// reduce A to U = D L^T and turn the identity E into L^-1, so
+// R^-1[i][j] = E[j][i] / sqrt(d_j).  a (upper triangle used), E and rinv are
+// W x W in LDS; rinv comes out zero padded.  Returns the first failing pivot
+// (1-based), 0 when none.
+template <int W>
+__device__ int chol_inverse(double* a, double* E, double* isd, uint32_t l, double* rinv) {
+  const uint32_t t = threadIdx.x, nt = blockDim.x;
+  for (uint32_t e = t; e < W * W; e += nt) E[e] = (e / W == e % W) ? 1.0 : 0.0;
+  int bad = 0;
+  for (uint32_t k = 0; k < l; ++k) {
+    __syncthreads();
+    double d = a[k * W + k];
+    if (!(d > 0.0) || !isfinite(d)) {
+      if (bad == 0) bad = (int)k + 1;
+      d = 1e-300;
+    }
+    const double inv = 1.0 / d;
+    const uint32_t items = (l - k - 1) * W;
+    for (uint32_t e = t; e < items; e += nt) {
+      const uint32_t i = k + 1 + e / W, j = e % W;
+      const double m = a[k * W + i] * inv;
+      if (j >= i && j < l)
+        a[i * W + j] = fma(-m, a[k * W + j], a[i * W + j]);
+      else if (j <= k)
+        E[i * W + j] = fma(-m, E[k * W + j], E[i * W + j]);
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = t; j < l; j += nt) {
+    double d = a[j * W + j];
+    if (!(d > 0.0) || !isfinite(d)) d = 1e-300;
+    isd[j] = 1.0 / sqrt(d);
+  }
+  __syncthreads();
+  for (uint32_t e = t; e < W * W; e += nt) {
+    const uint32_t i = e / W, j = e % W;
+    rinv[e] = (i <= j && j < l) ? E[j * W + i] * isd[j] : 0.0;
+  }
+  __syncthreads();
+  return bad;
+}
+
+__device__ inline double bcast(double v, int lane) {  // v of `lane` (uniform) in every lane
+  const int2 w = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, int2{__builtin_amdgcn_readlane(w.x, lane), __builtin_amdgcn_readlane(w.y, lane)});
+}
+
+// Step K of chol_inverse_wave; the step index is a template parameter so
+// every register index is a compile-time constant.  Rows and columns in
+// [l, R) hold the identity, which factors into itself, so every step runs.
+template <int WS, int R, int K>
+__device__ __attribute__((always_inline)) inline void chol_wave_step(double (&a)[R], double* E, int j, double* mrow,
+                                                                     int& bad, double& dj) {
+  if constexpr (K < R) {
+    double d = bcast(a[K], K);
+    const bool ok = d > 0.0 && isfinite(d);
+    bad = (bad == 0 && !ok) ? K + 1 : bad;
+    d = ok ? d : 1e-300;
+    dj = (j == K) ? d : dj;
+    if (j < R) mrow[j] = a[K] * (1.0 / d);  // lane i: m_i = A[k][i] / A[k][k]
+    __builtin_amdgcn_wave_barrier();
+    if (j < WS) {
+      // E[k][j] is zero for j > k, which leaves those columns unchanged;
+      // loads are grouped eight rows at a time (register pressure)
+      const double ek = E[K * WS + j];
+#pragma unroll
+      for (int i = K + 1; i < R; ++i) {
+        const double m = mrow[i];
+        a[i] = fma(-m, a[K], a[i]);
+        E[i * WS + j] = fma(-m, ek, E[i * WS + j]);
+        if ((i & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_sched_barrier(0);
+    chol_wave_step<WS, R, K + 1>(a, E, j, mrow, bad, dj);
+  }
+}
+
+// chol_inverse in one wave for l <= R (R <= 48 registers): lane j holds
+// column j of A in registers (upper part; the strict lower part is updated
+// too but never read) and updates column j of E in LDS.  Step k: every lane
+// publishes m_j = A[k][j] / A[k][k] (row k equals column k) through LDS, and
+// lane j applies row_i -= m_i row_k to its columns for every i > k (E's column
+// j is zero above row j).  Called by the whole wave; A, E and rinv are LDS
+// with row stride WS (rinv comes out zero padded to WS x WS); mrow is WS
+// doubles of LDS.
+template <int WS, int R>
+__device__ __attribute__((always_inline)) inline int chol_inverse_wave(const double* A, double* E, uint32_t l,
+                                                                       double* rinv, double* mrow) {
+  const int j = threadIdx.x & 63;
+  const int L = __builtin_amdgcn_readfirstlane((int)l);
+  double a[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) a[i] = (i <= j && j < L) ? A[i * WS + j] : (i == j ? 1.0 : 0.0);
+  if (j < WS)
+    for (int i = 0; i < WS; ++i) E[i * WS + j] = (i == j) ? 1.0 : 0.0;
+  __builtin_amdgcn_wave_barrier();
+  int bad = 0;
+  double dj = 1.0;
+  chol_wave_step<WS, R, 0>(a, E, j, mrow, bad, dj);
+  // R^-1[i][c] = E[c][i] / sqrt(d_c): lane i writes row i from column i of E
+  if (j < WS) mrow[j] = 1.0 / sqrt(dj);
+  __builtin_amdgcn_wave_barrier();
+  if (j < WS)
+    for (int c = 0; c < WS; ++c) rinv[j * WS + c] = (j <= c && c < L) ? E[c * WS + j] * mrow[c] : 0.0;
+  __builtin_amdgcn_wave_barrier();
+  return bad > L ? 0 : bad;
+}
+
+template <int R, int K>
+__device__ __attribute__((always_inline)) inline void signs_wave_step(double (&b)[R], int j, int L, double* sign) {
+  if constexpr (K < R) {
+    const double bkk = bcast(b[K], K);
+    const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
+    const double ip = 1.0 / (bkk - sk);
+    if (j == 0 && K < L) sign[K] = sk;
+#pragma unroll
+    for (int i = K + 1; i < R; ++i) {
+      const double lik = bcast(b[i], K) * ip;
+      b[i] = fma(-lik, b[K], b[i]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    signs_wave_step<R, K + 1>(b, j, L, sign);
+  }
+}
+
+// Householder R-diagonal signs of the top l x l block B (LDS, row stride WS,
+// l <= R) of a CholeskyQR factor, in one wave: LU of B - S with
+// S_kk = -sign(pivot) (Householder reconstruction, Ballard et al. 2014), lane
+// j holding column j.  Entries left of the pivot column are updated too but
+// never read again; steps past l only touch rows and columns past l.
+template <int WS, int R>
+__device__ __attribute__((always_inline)) inline void householder_signs_wave(const double* B, uint32_t l,
+                                                                             double* sign) {
+  const int j = threadIdx.x & 63;
+  double b[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) b[i] = (j < WS) ? B[i * WS + j] : 0.0;
+  signs_wave_step<R, 0>(b, j, __builtin_amdgcn_readfirstlane((int)l), sign);
+}
+
+// l rounded up to 8, the register count of the one-wave routines
+#define PCAONE_QR_BY_R(NB, l, CALL)           \
+  do {                                        \
+    const uint32_t r8_ = ((l) + 7) / 8;       \
+    if (r8_ <= (NB) * 2 - 1) {                \
+      constexpr int R = (NB) * 16 - 8;        \
+      CALL;                                   \
+    } else {                                  \
+      constexpr int R = (NB) * 16;            \
+      CALL;                                   \
+    }                                         \
+  } while (0)
+
+// R^-1 of the Gram in a (upper triangle, W x W LDS) into rinv (W x W LDS):
+// one wave for l <= 48, the whole block above.  Ends with a block barrier.
+template <int NB>
+__device__ __attribute__((always_inline)) inline int factor(double* a, double* E, double* isd, uint32_t l, double* rinv) {
+  constexpr int W = NB * 16;
+  if constexpr (NB <= 3) {
+    __shared__ int bad_sh;
+    __syncthreads();
+    if ((threadIdx.x >> 6) == 0) {
+      int bad = 0;
+      PCAONE_QR_BY_R(NB, l, bad = (chol_inverse_wave<W, R>(a, E, l, rinv, isd)));
+      if (threadIdx.x == 0) bad_sh = bad;
+    }
+    __syncthreads();
+    return bad_sh;
+  } else {
+    return chol_inverse<W>(a, E, isd, l, rinv);
+  }
+}
+
+__device__ inline void stage_padded(const double* __restrict__ src, uint32_t l, int W, double* dst) {
+  for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+    const uint32_t k = e / W, j = e % W;
+    dst[e] = (k < l && j < l) ? src[k * l + j] : 0.0;
+  }
+}
+
+__device__ inline void store_compact(const double* src, uint32_t l, int W, double* dst) {
+  for (uint32_t e = threadIdx.x; e < l * l; e += blockDim.x) dst[e] = src[(e / l) * W + e % l];
+}
+
+// Omega^T of one 16-row tile in registers: the B operand of step ks is
+// H[r0 + i][4 ks + k]; Q1^T = R1^-T H^T, then Omega^T = Rb^T Q1^T, whose
+// D layout (lane (k, i): Q1[r0 + i][16 cb + k + 4 v]) is already the B operand
+// of the second product.  om[cb][v] = Omega[r0 + lane % 16][16 cb + lane / 16 + 4 v].
+template <int NB>
+__device__ inline void tile_omega(const double* __restrict__ H, uint64_t n, uint32_t l, uint64_t r0,
+                                  const double* r1, const double* rb, f64x4 (&om)[NB]) {
+  constexpr int W = NB * 16;
+  const int lane = threadIdx.x & 63, kr = lane >> 4, ci = lane & 15;
+  const uint64_t row = r0 + ci;
+  double b[W / 4];
+#pragma unroll
+  for (int ks = 0; ks < W / 4; ++ks) {
+    const uint32_t k = ks * 4 + kr;
+    b[ks] = (row < n && k < l) ? H[row * l + k] : 0.0;
+  }
+  f64x4 qt[NB];
+#pragma unroll
+  for (int cb = 0; cb < NB; ++cb) qt[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < W / 4; ++ks) {
+    if (ks * 4 >= (int)l) break;
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) qt[cb] = mfma64(r1[(ks * 4 + kr) * W + cb * 16 + ci], b[ks], qt[cb]);
+  }
+#pragma unroll
+  for (int cb = 0; cb < NB; ++cb) om[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int ks = cb * 4 + v;
+      if (ks * 4 >= (int)l) break;
+#pragma unroll
+      for (int c2 = 0; c2 < NB; ++c2) om[c2] = mfma64(rb[(ks * 4 + kr) * W + c2 * 16 + ci], qt[cb][v], om[c2]);
+    }
+}
+
+// Tile range of this wave in a pass over ceil(n / 16) tiles; with skip0,
+// wave 0 of block 0 takes none (it has other work).
+__device__ inline void wave_tiles(uint64_t n, bool skip0, uint64_t& t0, uint64_t& t1) {
+  const uint64_t tiles = ceil_div(n, 16);
+  uint64_t nw = (uint64_t)gridDim.x * kWaves, wid = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (skip0) {
+    if (wid == 0) {
+      t0 = t1 = 0;
+      return;
+    }
+    --nw;
+    --wid;
+  }
+  const uint64_t tpw = ceil_div(tiles, nw);
+  t0 = std::min<uint64_t>(wid * tpw, tiles);
+  t1 = std::min<uint64_t>(t0 + tpw, tiles);
+}
+
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(kThreads) void k_qr_gram_h(QrPass q) {
+  using S = Shape<NB>;
+  __shared__ double slots[S::SLOTS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kr = lane >> 4, ci = lane & 15;
+  const uint32_t l = q.l;
+  f64x4 acc[S::NP];
+#pragma unroll
+  for (int p = 0; p < S::NP; ++p) acc[p] = f64x4{0.0, 0.0, 0.0, 0.0};
+  // 4-row K steps: lane (k, i) loads H[row k][16 cb + i], at once the A and
+  // the B operand of the tile products
+  const uint64_t steps = ceil_div(q.n, 4);
+  const uint64_t nw = (uint64_t)gridDim.x * kWaves, wid = (uint64_t)blockIdx.x * kWaves + wv;
+  const uint64_t sps = ceil_div(steps, nw);
+  const uint64_t s0 = std::min<uint64_t>(wid * sps, steps), s1 = std::min<uint64_t>(s0 + sps, steps);
+  for (uint64_t st0 = s0; st0 < s1; st0 += 4) {  // four K steps' loads in flight
+    double a[4][NB];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint64_t r = (st0 + u) * 4 + kr;
+      const bool live = st0 + u < s1 && r < q.n;
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb) {
+        const uint32_t c = cb * 16 + ci;
+        a[u][cb] = (live && c < l) ? q.H1[r * l + c] : 0.0;
+      }
+    }
+    if (q.H2) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint64_t r = (st0 + u) * 4 + kr;
+        const bool live = st0 + u < s1 && r < q.n;
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb) {
+          const uint32_t c = cb * 16 + ci;
+          if (live && c < l) {
+            a[u][cb] += q.H2[r * l + c];
+            q.H[r * l + c] = a[u][cb];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int p = 0;
+#pragma unroll
+      for (int ca = 0; ca < NB; ++ca)
+#pragma unroll
+        for (int cb = ca; cb < NB; ++cb, ++p) acc[p] = mfma64(a[u][ca], a[u][cb], acc[p]);
+    }
+  }
+  block_sum_waves<S::NP>(acc, slots);
+  store_gram<S::NP>(acc, q.part + (uint64_t)blockIdx.x * S::GRAM);
+}
+
+template <int NB>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_qr_gram_q(QrPass q) {
+  using S = Shape<NB>;
+  constexpr int W = S::W;
+  constexpr int PRO = 2 * W * W + W;  // a, E, isd
+  __shared__ double r1[W * W];
+  __shared__ double sh[S::SLOTS > PRO ? S::SLOTS : PRO];
+  const int lane = threadIdx.x & 63;
+  const int kr = lane >> 4, ci = lane & 15;
+  const uint32_t l = q.l;
+  // prologue: R1^-1 from the Gram of H
+  {
+    double* a = sh;
+    double* E = a + W * W;
+    double* isd = E + W * W;
+    gram_to_upper<NB>(q.part, q.nparts, l, a);
+    const int bad = factor<NB>(a, E, isd, l, r1);
+    if (blockIdx.x == 0) {
+      store_compact(r1, l, W, q.R1i);
+      if (threadIdx.x == 0 && bad && q.status[0] == 0) q.status[0] = bad;
+    }
+  }
+  f64x4 g[S::NP];
+#pragma unroll
+  for (int p = 0; p < S::NP; ++p) g[p] = f64x4{0.0, 0.0, 0.0, 0.0};
+  uint64_t t0, t1;
+  wave_tiles(q.n, false, t0, t1);
+  for (uint64_t tile = t0; tile < t1; ++tile) {
+    // Q1 tile = H tile R1^-1: lane (k, i) supplies H[r0 + i][4 ks + k]
+    const uint64_t row = tile * 16 + ci;
+    f64x4 acc[NB];
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) acc[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < W / 4; ++ks) {
+      if (ks * 4 >= (int)l) break;
+      const uint32_t k = ks * 4 + kr;
+      const double x = (row < q.n && k < l) ? q.H[row * l + k] : 0.0;
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb) acc[cb] = mfma64(x, r1[k * W + cb * 16 + ci], acc[cb]);
+    }
+    // accumulator element v is the 4-row slab (v, lane / 16) of the tile, in
+    // the operand layout of the Gram's K step
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      int p = 0;
+#pragma unroll
+      for (int ca = 0; ca < NB; ++ca)
+#pragma unroll
+        for (int cb = ca; cb < NB; ++cb, ++p) g[p] = mfma64(acc[ca][v], acc[cb][v], g[p]);
+    }
+  }
+  block_sum_waves<S::NP>(g, sh);
+  store_gram<S::NP>(g, q.part + (uint64_t)(kMaxBlocks + blockIdx.x) * S::GRAM);
+}
+
+// The top W rows of the unsigned Omega (row stride W) from its first tiles,
+// computed by `waves` waves of the block, the same way as every other row.
+template <int NB>
+__device__ void omega_top(const QrPass& q, const double* r1, const double* r2, int waves, double* bt) {
+  constexpr int W = NB * 16;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kr = lane >> 4, ci = lane & 15;
+  for (int tb = wv; tb < NB; tb += waves) {
+    f64x4 om[NB];
+    tile_omega<NB>(q.H, q.n, q.l, (uint64_t)tb * 16, r1, r2, om);
+#pragma unroll
+    for (int c2 = 0; c2 < NB; ++c2)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) bt[(tb * 16 + ci) * W + c2 * 16 + kr + 4 * v] = om[c2][v];
+  }
+}
+
+// LU of B - S for the signs with the whole block (l > 48), B in LDS (stride W).
+template <int W>
+__device__ void householder_signs_block(double* B, uint32_t l, double* sign) {
+  const uint32_t t = threadIdx.x, nt = blockDim.x;
+  for (uint32_t k = 0; k < l; ++k) {
+    __syncthreads();
+    const double bkk = B[k * W + k];
+    const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
+    const double ip = 1.0 / (bkk - sk);
+    if (t == 0) sign[k] = sk;
+    const uint32_t items = (l - k - 1) * W;
+    for (uint32_t e = t; e < items; e += nt) {
+      const uint32_t i = k + 1 + e / W, j = e % W;
+      if (j > k && j < l) B[i * W + j] = fma(-B[i * W + k] * ip, B[k * W + j], B[i * W + j]);
+    }
+  }
+  __syncthreads();
+}
+
+template <int NB>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_qr_flip(QrPass q) {
+  using S = Shape<NB>;
+  constexpr int W = S::W;
+  constexpr int PRO = 2 * W * W + W;
+  constexpr bool kWaveLU = NB <= 3;  // the sign LU runs in one wave beside the pass
+  __shared__ double r1[W * W], r2[W * W];
+  __shared__ double sh[PRO > kWaves * S::STAT ? PRO : kWaves * S::STAT];
+  __shared__ double btw[kWaveLU ? W * W : 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kr = lane >> 4, ci = lane & 15;
+  const uint32_t l = q.l, t = threadIdx.x, nt = blockDim.x;
+  // prologue: R2^-1 from the Gram of Q1
+  stage_padded(q.R1i, l, W, r1);
+  {
+    double* a = sh;
+    double* E = a + W * W;
+    double* isd = E + W * W;
+    gram_to_upper<NB>(q.part + (uint64_t)kMaxBlocks * S::GRAM, q.nparts, l, a);
+    const int bad = factor<NB>(a, E, isd, l, r2);
+    if (blockIdx.x == 0) {
+      store_compact(r2, l, W, q.R2i);
+      if (t == 0 && bad && q.status[1] == 0) q.status[1] = bad;
+    }
+  }
+  // the Householder signs of the unsigned Omega's top l x l block
+  bool lu_wave = false;
+  if constexpr (kWaveLU) {
+    lu_wave = blockIdx.x == 0 && wv == 0;
+    if (blockIdx.x == 0) {  // waves 0 .. NB-1 compute the top tiles; wave 0 then runs the LU
+      omega_top<NB>(q, r1, r2, NB, btw);
+      __syncthreads();
+    }
+    if (lu_wave) {
+      PCAONE_QR_BY_R(NB, l, (householder_signs_wave<W, R>(btw, l, q.sign)));
+    }
+  } else {
+    if (blockIdx.x == 0) {
+      omega_top<NB>(q, r1, r2, kWaves, sh);
+      householder_signs_block<W>(sh, l, q.sign);
+    }
+  }
+  double dm[NB][4], dp[NB][4], cs[NB][4];
+#pragma unroll
+  for (int c2 = 0; c2 < NB; ++c2)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dm[c2][v] = dp[c2][v] = cs[c2][v] = 0.0;
+  uint64_t t0, t1;
+  wave_tiles(q.n, kWaveLU, t0, t1);
+  for (uint64_t tile = t0; tile < t1; ++tile) {
+    f64x4 om[NB];
+    tile_omega<NB>(q.H, q.n, l, tile * 16, r1, r2, om);
+    const uint64_t row = tile * 16 + ci;
+#pragma unroll
+    for (int c2 = 0; c2 < NB; ++c2)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint32_t c = c2 * 16 + kr + 4 * v;
+        if (row < q.n && c < l) {
+          const double x = om[c2][v], o = q.omg2[row * l + c];
+          dm[c2][v] += fabs(o - x);
+          dp[c2][v] += fabs(o + x);
+          cs[c2][v] += x;
+        }
+      }
+  }
+  // rows are spread over lane % 16: fold them, then the waves
+#pragma unroll
+  for (int c2 = 0; c2 < NB; ++c2)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        dm[c2][v] += __shfl_xor(dm[c2][v], off);
+        dp[c2][v] += __shfl_xor(dp[c2][v], off);
+        cs[c2][v] += __shfl_xor(cs[c2][v], off);
+      }
+  double* red = sh;
+  if (ci == 0)
+#pragma unroll
+    for (int c2 = 0; c2 < NB; ++c2)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int c = c2 * 16 + kr + 4 * v;
+        red[wv * S::STAT + c] = dm[c2][v];
+        red[wv * S::STAT + W + c] = dp[c2][v];
+        red[wv * S::STAT + 2 * W + c] = cs[c2][v];
+      }
+  __syncthreads();
+  double* dst = q.part + (uint64_t)2 * kMaxBlocks * S::GRAM + (uint64_t)blockIdx.x * S::STAT;
+  for (int e = t; e < S::STAT; e += nt) {
+    double s = 0.0;
+    for (int w = 0; w < kWaves; ++w) s += red[w * S::STAT + e];
+    dst[e] = s;
+  }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_qr_write(QrPass q) {
+  using S = Shape<NB>;
+  constexpr int W = S::W;
+  __shared__ double r1[W * W], rf[W * W];
+  __shared__ double fs[W];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kr = lane >> 4, ci = lane & 15;
+  const uint32_t l = q.l, t = threadIdx.x;
+  stage_padded(q.R1i, l, W, r1);
+  stage_padded(q.R2i, l, W, rf);
+  // flipOmg on the signed Omega (the sign of a column swaps its two
+  // distances): the statistics partials summed in block order
+  __shared__ double stat[S::STAT];
+  if (t < S::STAT) stat[t] = sum_parts(q.part + (uint64_t)2 * kMaxBlocks * S::GRAM, q.nparts, S::STAT, t);
+  __syncthreads();
+  if (t < l) {
+    const double s = q.sign[t];
+    const double a = (s > 0.0) ? stat[t] : stat[W + t];
+    const double b = (s > 0.0) ? stat[W + t] : stat[t];
+    const double f = (a > 2.0 * b) ? -1.0 : 1.0;
+    fs[t] = f * s;
+    if (blockIdx.x == 0) q.colsum[t] = f * s * stat[2 * W + t];
+  }
+  __syncthreads();
+  for (int e = t; e < W * W; e += blockDim.x) rf[e] *= (e % W < (int)l) ? fs[e % W] : 0.0;
+  __syncthreads();
+  const uint64_t tiles = ceil_div(q.n, 16);
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + wv; tile < tiles; tile += (uint64_t)gridDim.x * 4) {
+    f64x4 om[NB];
+    tile_omega<NB>(q.H, q.n, l, tile * 16, r1, rf, om);
+    const uint64_t row = tile * 16 + ci;
+    if (row >= q.n) continue;
+    const uint64_t kpos = q.kperm ? kperm16(row) : row;
+#pragma unroll
+    for (int c2 = 0; c2 < NB; ++c2)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint32_t c = c2 * 16 + kr + 4 * v;
+        if (c >= l) continue;
+        const double x = om[c2][v];
+        const uint64_t gi = row * l + c;
+        q.omega[gi] = x;
+        q.omg2[gi] = x;
+        if (q.zero_acc) q.zero_acc[gi] = 0.0;
+        if (q.omega32) q.omega32[gi] = (float)x;
+        if (q.bprec == PREC_SPLIT_BF16) {
+          const __bf16 hi = (__bf16)(float)x;
+          reinterpret_cast<__bf16*>(q.bg)[(uint64_t)c * q.kpad + kpos] = hi;
+          reinterpret_cast<__bf16*>(q.bg)[(uint64_t)(q.lo_off + c) * q.kpad + kpos] =
+              (__bf16)(float)(x - (double)(float)hi);
+        } else if (q.bprec == PREC_FP32) {
+          reinterpret_cast<float*>(q.bg)[(uint64_t)c * q.kpad + kpos] = (float)x;
+        } else {
+          reinterpret_cast<double*>(q.bg)[(uint64_t)c * q.kpad + kpos] = x;
+        }
+      }
+  }
+}
+
+template <int NB>
+void launch_nb(QrPass q, int pass, hipStream_t s) {
+  const uint64_t tiles = ceil_div(q.n, 16);
+  // reduction passes: about one tile per wave, at most kMaxBlocks partials
+  q.nparts = (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(tiles, kWaves), 1), kMaxBlocks);
+  const uint32_t nwrite = (uint32_t)std::max<uint64_t>(ceil_div(tiles, 4), 1);
+  switch (pass) {
+    case 0: hipLaunchKernelGGL(k_qr_gram_h<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q); break;
+    case 1: hipLaunchKernelGGL(k_qr_gram_q<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q); break;
+    case 2: hipLaunchKernelGGL(k_qr_flip<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q); break;
+    default: hipLaunchKernelGGL(k_qr_write<NB>, dim3(nwrite), dim3(256), 0, s, q); break;
+  }
+}
+
+}  // namespace
+
+uint64_t qr_part_doubles(uint32_t l) {
+  const uint64_t nb = ceil_div(l, 16), np = nb * (nb + 1) / 2;
+  return (uint64_t)kMaxBlocks * (2 * np * 256 + 3 * nb * 16);
+}
+
+void launch_qr_pass(const QrPass& q, int pass, hipStream_t s) {
+  PCAONE_REQUIRE(q.l >= 1 && q.l <= 64, -1, "l must be in [1, 64]");
+  PCAONE_REQUIRE(q.n >= 2ull * q.l, -1, "the Omega update needs at least 2 l rows");
+  PCAONE_REQUIRE(!q.kperm || q.kpad % 16 == 0, -1, "omega operand: permuted K needs 16-code groups");
+  switch ((q.l + 15) / 16) {
+    case 1: launch_nb<1>(q, pass, s); break;
+    case 2: launch_nb<2>(q, pass, s); break;
+    case 3: launch_nb<3>(q, pass, s); break;
+    default: launch_nb<4>(q, pass, s); break;
+  }
+}
+
+}  // namespace pcaone
