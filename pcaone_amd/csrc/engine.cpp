@@ -285,7 +285,9 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   gpart_.reserve(gmax * ldp * partsz);
   hpart_.reserve(hmax * ldp * partsz);
   // linear algebra scratch
-  part_.reserve(std::max<uint64_t>((uint64_t)1024 * l_ * l_, qr_part_doubles(l_)));
+  part_.reserve(std::max<uint64_t>({(uint64_t)1024 * l_ * l_, qr_part_doubles(l_), usv_part_doubles(l_)}));
+  gred_.reserve(usv_gram_doubles(l_));
+  bbt_.reserve((uint64_t)l_ * l_);
   for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &Rf_, &W_}) b->reserve((uint64_t)l_ * l_);
   sign_.reserve(l_);
   evals_.reserve(l_);
@@ -1003,26 +1005,32 @@ void Engine::check_status() {
   }
 }
 
-// One pass of computeUSV's linear algebra (src/Halko.cpp:53-68): Q of G by
-// CholeskyQR2, B^T = H R^-1, B B^T = U_B S^2 U_B^T, Ucur = B^T U_B S^-1.
+// One pass of computeUSV's linear algebra (src/Halko.cpp:53-68): the
+// CholeskyQR2 factors of G (R = R2 R1; Q = G R^-1 is never stored: only the
+// last pass needs it, for V), B^T = H R^-1, B B^T = U_B S^2 U_B^T,
+// Ucur = B^T U_B S^-1.
 void Engine::usv_step() {
   const uint32_t l = l_;
-  const int nbG = gram_blocks(M_), nbN = gram_blocks(N_);
-  timed("usv_gram", [&] { launch_gram_partial(G_.p, nullptr, nullptr, M_, 0, l, part_.p, nbG, stream_); });
-  timed("usv_chol", [&] {
-    launch_chol(part_.p, nbG, l, R1_.p, R1i_.p, nullptr, nullptr, nullptr, nullptr, status_.p, stream_);
-  });
-  timed("usv_apply", [&] { launch_apply_rinv(G_.p, R1i_.p, nullptr, G_.p, M_, l, part_.p, nbG, stream_); });
-  timed("usv_chol", [&] {
-    launch_chol(part_.p, nbG, l, R2_.p, R2i_.p, R1_.p, R1i_.p, Rt_.p, Rti_.p, status_.p + 1, stream_);
-  });
-  // no Gram partials to collect: four times the workgroups for the streaming pass
-  const int nbA = (int)std::min<uint64_t>(ceil_div(M_, 64), 1024);
-  timed("usv_apply", [&] { launch_apply_rinv(G_.p, R2i_.p, nullptr, G_.p, M_, l, nullptr, nbA, stream_); });
-  timed("usv_apply", [&] { launch_apply_rinv(H_.p, Rti_.p, nullptr, Bt_.p, N_, l, part_.p, nbN, stream_); });
+  UsvPass u{};
+  u.G = G_.p;
+  u.m = M_;
+  u.H = H_.p;
+  u.n = N_;
+  u.l = l;
+  u.part = part_.p;
+  u.gram = gred_.p;
+  u.bbt = bbt_.p;
+  u.R1i = R1i_.p;
+  u.R2i = R2i_.p;
+  u.Rti = Rti_.p;
+  u.Bt = Bt_.p;
+  u.status = status_.p;
+  timed("usv_gram", [&] { launch_usv_qr(u, 0, stream_); });
+  timed("usv_q1", [&] { launch_usv_qr(u, 1, stream_); });
+  timed("usv_bt", [&] { launch_usv_qr(u, 2, stream_); });
   // later epochs start from the previous epoch's eigenvectors
   timed("usv_eig", [&] {
-    launch_sym_eig(part_.p, nbN, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, have_W_ ? W_.p : nullptr, stream_);
+    launch_sym_eig(bbt_.p, 1, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, have_W_ ? W_.p : nullptr, stream_);
   });
   have_W_ = true;
   timed("usv_rows", [&] {
@@ -1069,8 +1077,9 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
     }
   }
   if (!done) std::swap(Ucur_.p, Upre_.p);  // the last Ucur sits in Upre after the swap
-  // V = Q U_B[:, :k] (rows in laid-out order), S = sqrt(eig(B B^T))[:k]
-  timed("usv_rows", [&] { launch_rows_times_small(G_.p, M_, l_, W_.p, l_, k, nullptr, V_.p, stream_); });
+  // V = Q U_B[:, :k] = G (R^-1 U_B[:, :k]) (rows in laid-out order), S = sqrt(eig(B B^T))[:k]
+  timed("usv_rows", [&] { launch_rows_times_small(Rti_.p, l_, l_, W_.p, l_, k, nullptr, Rt_.p, stream_); });
+  timed("usv_rows", [&] { launch_rows_times_small(G_.p, M_, l_, Rt_.p, k, k, nullptr, V_.p, stream_); });
   std::vector<double> ev(l_);
   PCAONE_HIP(hipMemcpyAsync(ev.data(), evals_.p, l_ * 8, hipMemcpyDeviceToHost, stream_));
   sync();

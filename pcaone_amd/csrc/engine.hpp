@@ -212,6 +212,7 @@ class Engine {
   DevBuf<double> miss_val_, Sdev_, Vpre_;
   // linear algebra scratch
   DevBuf<double> part_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, Rf_, sign_;
+  DevBuf<double> gred_, bbt_;  // computeUSV: reduced Gram between passes, B B^T
   DevBuf<double> Bt_, evals_, W_, sinv_, Ucur_, Upre_, V_, scal_;
   DevBuf<double> out_;  // column-major U / V staged for the copy to the host
   DevBuf<int> status_;

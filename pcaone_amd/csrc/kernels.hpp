@@ -250,18 +250,8 @@ void launch_dense_gemm(const DenseGemm& g, int arm, int ncols, int ntiles, hipSt
 void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t s);
 
 // ---- FP64 tall-skinny linear algebra --------------------------------------
-// partial Gram: part[b] = sum_{rows of block b} A[r]^T A[r], rows [row0, nrows)
-// of A (+ A2 when non-null, in which case Aout = A + A2 is stored).
+// workgroups of the partial-product passes over nrows rows (the mev products)
 int gram_blocks(uint64_t nrows);
-void launch_gram_partial(const double* A, const double* A2, double* Aout, uint64_t nrows, uint32_t row0,
-                         uint32_t l, double* part, int nblocks, hipStream_t s);
-// Single workgroup: Gram = sum(part), R = chol(Gram) upper, Rinv = R^-1,
-// optional Rprev: Rtot = R * Rprev and Rtot_inv = Rprev_inv * Rinv.
-void launch_chol(const double* part, int nparts, uint32_t l, double* R, double* Rinv, const double* Rprev,
-                 const double* Rprev_inv, double* Rtot, double* Rtot_inv, int* status, hipStream_t s);
-// Q = A * Rinv (rows), optional per-column sign; optional partial Gram of Q.
-void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, double* Q, uint64_t nrows,
-                       uint32_t l, double* gram_part, int nblocks, hipStream_t s);
 // ---- the winSVD Omega update in four passes (kernels_qr.hip) ----------------
 // Omega <- Householder-signed CholeskyQR2 Q of H (H = H1 + H2 when H2 is
 // given), flipped against the previous Omega (omg2), written to every consumer.
@@ -292,6 +282,27 @@ struct QrPass {
 uint64_t qr_part_doubles(uint32_t l);
 // pass 0: H and R1^-1; 1: R2^-1 and signs; 2: flip and colsum; 3: the writes.
 void launch_qr_pass(const QrPass& q, int pass, hipStream_t s);
+// computeUSV's linear algebra without materialising Q: CholeskyQR2 factors of
+// G (M x l) and B^T = H (R2 R1)^-1 with B B^T for the eigensolve.
+struct UsvPass {
+  const double* G;  // M x l
+  uint64_t m;
+  const double* H;  // N x l
+  uint64_t n;
+  uint32_t l;
+  double* part;     // usv_part_doubles(l) scratch
+  double* gram;     // usv_gram_doubles(l): reduced Gram between the steps
+  double* bbt;      // l x l: B B^T (step 2 output, row-major, symmetric)
+  double* R1i;      // l x l outputs: R1^-1, R2^-1, Rt^-1 = R1^-1 R2^-1
+  double* R2i;
+  double* Rti;
+  double* Bt;       // N x l output: H Rt^-1
+  int* status;      // [2]: first failing pivot of the two factorisations (sticky)
+};
+uint64_t usv_part_doubles(uint32_t l);
+uint64_t usv_gram_doubles(uint32_t l);
+// step 0: Gram of G; 1: R1^-1 and the Gram of G R1^-1; 2: R2^-1, Rt^-1, B^T, B B^T.
+void launch_usv_qr(const UsvPass& u, int step, hipStream_t s);
 // small dense: C[n][kc] = A[n][l] * W[l][ldw] (first kc columns) * diag(scale)
 void launch_rows_times_small(const double* A, uint64_t n, uint32_t l, const double* W, uint32_t ldw,
                              uint32_t kc, const double* scale, double* C, hipStream_t s);

@@ -1,8 +1,7 @@
-// FP64 tall-skinny linear algebra for the RSVD steps whose stability needs it:
-// CholeskyQR2 of the M x l block of computeUSV (Gram -> Cholesky -> triangular
-// inverse -> apply), the l x l symmetric eigensolve for the SVD of B, and the
-// mev convergence metric (src/Utils.cpp:192-198).  The Omega update (its
-// CholeskyQR2, Householder signs and flipOmg) is fused in kernels_qr.hip.
+// FP64 small dense linear algebra of the RSVD steps: the l x l symmetric
+// eigensolve for the SVD of B, the mev convergence metric
+// (src/Utils.cpp:192-198) and row-block products.  The CholeskyQR2 passes
+// (Omega update and computeUSV) are in kernels_qr.hip.
 //
 // All reductions are two-level (per-block partials, then one reduction in a
 // fixed order), so every result is deterministic run to run.
@@ -96,379 +95,6 @@ __device__ inline void block_tiles_out(const f64x4 (&acc)[NP], bool upper, uint3
       out[(uint64_t)r * l + c] = red[e];
       if (upper) out[(uint64_t)c * l + r] = red[e];
     }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// partial Gram (FP64 MFMA), optionally forming A + A2 on the fly
-// ---------------------------------------------------------------------------
-// Rows [row0, nrows) go to the waves four at a time (one MFMA K step); lane
-// (k = lane / 16, i = lane % 16) loads A[row k][16 cb + i] of every column
-// block cb, which is at once the A and the B operand of the tile products.
-template <int NB>
-__global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__ A, const double* __restrict__ A2,
-                                                      double* __restrict__ Aout, uint64_t nrows, uint32_t row0,
-                                                      uint32_t l, double* __restrict__ part) {
-  constexpr int NP = NB * (NB + 1) / 2;
-  __shared__ double red[NP * 256];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int kr = lane >> 4, ci = lane & 15;
-  f64x4 acc[NP];
-#pragma unroll
-  for (int p = 0; p < NP; ++p) acc[p] = f64x4{0.0, 0.0, 0.0, 0.0};
-  const uint64_t steps = ceil_div(nrows - row0, 4);
-  const uint64_t nw = (uint64_t)gridDim.x * 4, wid = (uint64_t)blockIdx.x * 4 + wv;
-  const uint64_t sps = ceil_div(steps, nw);
-  const uint64_t s0 = wid * sps, s1 = std::min<uint64_t>(s0 + sps, steps);
-  for (uint64_t st = s0; st < s1; ++st) {
-    const uint64_t r = row0 + st * 4 + kr;
-    double a[NB];
-#pragma unroll
-    for (int cb = 0; cb < NB; ++cb) {
-      const uint32_t c = cb * 16 + ci;
-      double v = 0.0;
-      if (r < nrows && c < l) {
-        const uint64_t gi = r * l + c;
-        v = A[gi];
-        if (A2) {
-          v += A2[gi];
-          Aout[gi] = v;
-        }
-      }
-      a[cb] = v;
-    }
-    int p = 0;
-#pragma unroll
-    for (int ca = 0; ca < NB; ++ca)
-#pragma unroll
-      for (int cb = ca; cb < NB; ++cb, ++p) acc[p] = mfma64(a[ca], a[cb], acc[p]);
-  }
-  // rows above row0 still need A + A2 written
-  if (A2 && blockIdx.x == 0)
-    for (uint32_t it = threadIdx.x; it < row0 * l; it += 256) Aout[it] = A[it] + A2[it];
-  block_tiles_out<NB, NP>(acc, true, l, red, part + (uint64_t)blockIdx.x * l * l);
-}
-
-void launch_gram_partial(const double* A, const double* A2, double* Aout, uint64_t nrows, uint32_t row0,
-                         uint32_t l, double* part, int nblocks, hipStream_t s) {
-  PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
-  PCAONE_REQUIRE(nrows > row0, -1, "Gram over an empty row range");
-  switch ((l + 15) / 16) {
-#define PCAONE_NB(NBV) \
-  case NBV: hipLaunchKernelGGL(k_gram_partial<NBV>, dim3(nblocks), dim3(256), 0, s, A, A2, Aout, nrows, row0, l, part); break;
-    PCAONE_NB(1) PCAONE_NB(2) PCAONE_NB(3) PCAONE_NB(4)
-#undef PCAONE_NB
-    default: throw Error(-1, "gram: unsupported l");
-  }
-}
-
-// ---------------------------------------------------------------------------
-// single-workgroup Cholesky (+ inverse, products, Householder signs)
-// ---------------------------------------------------------------------------
-// Upper Cholesky R^T R = a (a: l x l in LDS, destroyed) into out (row stride
-// ldo).  Step k writes row k of R and the rank-1 update of the trailing block
-// from the same unscaled row k of a, so one barrier per step suffices.  *bad
-// receives the first failing pivot (1-based).
-__device__ void chol_upper_lds(double* a, int l, double* out, int ldo, int* bad) {
-  const int t = threadIdx.x;
-  for (int k = 0; k < l; ++k) {
-    double d = a[k * l + k];
-    if (!(d > 0.0) || !isfinite(d)) {
-      if (t == 0 && *bad == 0) *bad = k + 1;
-      d = 1e-300;
-    }
-    const double sd = sqrt(d), isd = 1.0 / sd, id = isd * isd;
-    for (int e = t; e < l - k; e += 256) out[k * ldo + k + e] = (e == 0) ? sd : a[k * l + k + e] * isd;
-    // trailing update on a 16 x 16 thread grid, upper triangle only
-    for (int i = k + 1 + (t >> 4); i < l; i += 16) {
-      const double aki = a[k * l + i] * id;
-      for (int j = k + 1 + (t & 15); j < l; j += 16)
-        if (j >= i) a[i * l + j] -= aki * a[k * l + j];
-    }
-    __syncthreads();
-  }
-  for (int e = t; e < l * l; e += 256) {
-    const int i = e / l, j = e % l;
-    if (i > j) out[i * ldo + j] = 0.0;
-  }
-  __syncthreads();
-}
-
-// X = R^-1 (R upper, both l x l in LDS).  Rows are solved bottom-up; four
-// lanes share each column's dot product.
-__device__ void upper_inverse_lds(const double* a, double* X, int l) {
-  const int t = threadIdx.x;
-  for (int e = t; e < l * l; e += 256) X[e] = (e / l == e % l) ? 1.0 / a[e] : 0.0;
-  __syncthreads();
-  const int j = t >> 2, q = t & 3;
-  for (int i = l - 2; i >= 0; --i) {
-    double s = 0.0;
-    if (j > i && j < l)
-      for (int p = i + 1 + q; p <= j; p += 4) s += a[i * l + p] * X[p * l + j];
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    if (q == 0 && j > i && j < l) X[i * l + j] = -s * X[i * l + i];
-    __syncthreads();
-  }
-}
-
-// LDS: four l x l slots.
-static size_t chol_lds_bytes(uint32_t l) { return (size_t)4 * l * l * sizeof(double); }
-
-__global__ __launch_bounds__(256) void k_chol(const double* __restrict__ part, int nparts, uint32_t l, double* R,
-                                              double* Rinv, const double* Rprev, const double* Rprev_inv,
-                                              double* Rtot, double* Rtot_inv, int* status) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  const int L = (int)l, t = threadIdx.x;
-  double* F = reinterpret_cast<double*>(smem_raw);  // Gram, then R^-1
-  double* a = F + L * L;                             // R
-  double* P = a + L * L;                             // Rprev, then Q's top block
-  double* Pi = P + L * L;                            // Rprev^-1, then the sign LU
-  __shared__ int bad;
-  if (t == 0) bad = 0;
-  for (int e = t; e < L * L; e += 256) {
-    double v = 0.0;
-    for (int p = 0; p < nparts; ++p) v += part[(uint64_t)p * L * L + e];
-    F[e] = v;
-  }
-  __syncthreads();
-  chol_upper_lds(F, L, a, L, &bad);
-  double* X = F;
-  upper_inverse_lds(a, X, L);
-  for (int e = t; e < L * L; e += 256) {
-    R[e] = a[e];
-    Rinv[e] = X[e];
-  }
-  if (Rprev) {
-    for (int e = t; e < L * L; e += 256) {
-      P[e] = Rprev[e];
-      Pi[e] = Rprev_inv[e];
-    }
-    __syncthreads();
-    for (int e = t; e < L * L; e += 256) {
-      const int i = e / L, j = e % L;
-      double v = 0.0, w = 0.0;
-      for (int p = 0; p < L; ++p) {
-        v += a[i * L + p] * P[p * L + j];
-        w += Pi[i * L + p] * X[p * L + j];
-      }
-      Rtot[e] = v;
-      Rtot_inv[e] = w;
-    }
-    __syncthreads();
-  }
-  if (t == 0 && status && bad && *status == 0) *status = bad;  // sticky until the host reads it
-}
-
-// ---------------------------------------------------------------------------
-// the same factorisations on one wave: lane j holds column j in registers
-// ---------------------------------------------------------------------------
-__device__ inline double bcast(double v, int lane) {  // v of `lane`, to every lane (lane is uniform)
-  const int2 w = __builtin_bit_cast(int2, v);
-  return __builtin_bit_cast(double, int2{__builtin_amdgcn_readlane(w.x, lane), __builtin_amdgcn_readlane(w.y, lane)});
-}
-
-// Rows and columns >= l are padded with the identity, so every step runs to
-// the compile-time bound LM and the padding factors into itself.
-template <int LM>
-__global__ __launch_bounds__(64) void k_chol_wave(const double* __restrict__ part, uint32_t l, double* R,
-                                                  double* Rinv, const double* Rprev, const double* Rprev_inv,
-                                                  double* Rtot, double* Rtot_inv, int* status) {
-  __shared__ double qs[LM * LM];
-  __shared__ double Rs[LM * LM];
-  const int j = threadIdx.x;
-  const int L = (int)l;
-  double a[LM], x[LM];
-#pragma unroll
-  for (int i = 0; i < LM; ++i) a[i] = (i < L && j < L) ? part[i * L + j] : (i == j ? 1.0 : 0.0);
-  int bad = 0;
-  // Cholesky R^T R = A: after step k lane j's a[k] is R[k][j]
-#pragma unroll
-  for (int k = 0; k < LM; ++k) {
-    double d = bcast(a[k], k);
-    if (!(d > 0.0) || !isfinite(d)) {
-      if (bad == 0) bad = k + 1;
-      d = 1e-300;
-    }
-    const double sd = sqrt(d), isd = 1.0 / sd;
-    const double rk = (j == k) ? sd : (j > k ? a[k] * isd : 0.0);
-    a[k] = (j >= k) ? rk : a[k];
-#pragma unroll
-    for (int i = k + 1; i < LM; ++i) {
-      const double rki = bcast(a[k], i);  // R[k][i]
-      a[i] = (i <= j) ? fma(-rki, rk, a[i]) : a[i];
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < LM; ++i) a[i] = (i <= j) ? a[i] : 0.0;  // R column j (upper)
-  // X = R^-1 by back substitution of R x = e_j
-#pragma unroll
-  for (int i = LM - 1; i >= 0; --i) {
-    double s = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-    for (int p = i + 1; p < LM; ++p) s = fma(-bcast(a[i], p), x[p], s);  // R[i][p] x_p
-    x[i] = s / bcast(a[i], i);
-  }
-  if (j < L)
-#pragma unroll
-    for (int i = 0; i < LM; ++i)
-      if (i < L) {
-        R[i * L + j] = a[i];
-        Rinv[i * L + j] = x[i];
-      }
-  // staged square matrices are padded with the identity up to LM
-  auto stage = [&](double* dst, const double* src) {
-    for (int e = j; e < LM * LM; e += 64) {
-      const int r = e / LM, c = e % LM;
-      dst[e] = (r < L && c < L) ? src[r * L + c] : (r == c ? 1.0 : 0.0);
-    }
-  };
-  if (Rprev) {
-    // Rtot = R Rprev, Rtot_inv = Rprev_inv X (column j of each)
-    if (j < LM)
-#pragma unroll
-      for (int i = 0; i < LM; ++i) Rs[i * LM + j] = a[i];
-    stage(qs, Rprev);
-    __syncthreads();
-    if (j < L)
-      for (int i = 0; i < L; ++i) {
-        double v = 0.0;
-        for (int p = 0; p < LM; ++p) v = fma(Rs[i * LM + p], qs[p * LM + j], v);
-        Rtot[i * L + j] = v;
-      }
-    __syncthreads();
-    stage(qs, Rprev_inv);
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < LM; ++i) {
-      double w = 0.0;
-#pragma unroll
-      for (int p = 0; p < LM; ++p) w = fma(qs[i * LM + p], x[p], w);
-      if (i < L && j < L) Rtot_inv[i * L + j] = w;
-    }
-    __syncthreads();
-  }
-  if (j == 0 && status && bad && *status == 0) *status = bad;  // sticky until the host reads it
-}
-
-void launch_chol(const double* part, int nparts, uint32_t l, double* R, double* Rinv, const double* Rprev,
-                 const double* Rprev_inv, double* Rtot, double* Rtot_inv, int* status, hipStream_t s) {
-  PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
-  if (l <= 48) {
-    reduce_parts(part, nparts, l * l, s);
-#define PCAONE_CW(LMV)                                                                                    \
-  hipLaunchKernelGGL(k_chol_wave<LMV>, dim3(1), dim3(64), 0, s, part, l, R, Rinv, Rprev, Rprev_inv, Rtot, \
-                     Rtot_inv, status)
-    if (l <= 16)
-      PCAONE_CW(16);
-    else if (l <= 32)
-      PCAONE_CW(32);
-    else
-      PCAONE_CW(48);
-#undef PCAONE_CW
-    return;
-  }
-  nparts = reduce_parts(part, nparts, l * l, s);
-  const size_t shm = chol_lds_bytes(l);
-  if (shm > 65536)
-    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chol),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-  hipLaunchKernelGGL(k_chol, dim3(1), dim3(256), shm, s, part, nparts, l, R, Rinv, Rprev, Rprev_inv, Rtot, Rtot_inv,
-                     status);
-}
-
-// ---------------------------------------------------------------------------
-// Q = A * Rinv * diag(sign), plus the partial Gram of Q (FP64 MFMA)
-// ---------------------------------------------------------------------------
-// Each wave takes 16-row tiles: the tile (16 l contiguous doubles) is staged
-// in LDS, Q's tile = A_tile * Rinv' accumulates over K steps of 4, and the
-// accumulator element v of every column block is itself a 4-row slab in the
-// MFMA operand layout, so Q^T Q accumulates from registers.  A and Q may alias
-// (each wave reads its tile before writing it).
-template <int NB>
-__global__ __launch_bounds__(256) void k_apply_rinv(const double* __restrict__ A, const double* __restrict__ Rinv,
-                                                    const double* __restrict__ sign, double* Q, uint64_t nrows,
-                                                    uint32_t l, double* __restrict__ part) {
-  constexpr int W = NB * 16, NP = NB * (NB + 1) / 2;
-  static_assert(NP * 256 <= 4 * 16 * (W + 1), "the Gram reduction reuses the A tiles");
-  __shared__ double ri[W * W];
-  __shared__ double at[4][16 * (W + 1)];
-  double* red = &at[0][0];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int kr = lane >> 4, ci = lane & 15;
-  for (int e = threadIdx.x; e < W * W; e += 256) {
-    const uint32_t k = e / W, j = e % W;
-    ri[e] = (k < l && j < l) ? Rinv[k * l + j] * (sign ? sign[j] : 1.0) : 0.0;
-  }
-  for (int e = threadIdx.x; e < 4 * 16 * (W + 1); e += 256) (&at[0][0])[e] = 0.0;
-  f64x4 g[NP];
-#pragma unroll
-  for (int p = 0; p < NP; ++p) g[p] = f64x4{0.0, 0.0, 0.0, 0.0};
-  const uint64_t tiles = ceil_div(nrows, 16);
-  const uint64_t tpb = ceil_div(tiles, gridDim.x);
-  const uint64_t t0 = blockIdx.x * tpb, t1 = std::min<uint64_t>(t0 + tpb, tiles);
-  const uint32_t ksteps = (l + 3) / 4;
-  double* my = at[wv];
-  for (uint64_t tb = t0; tb < t1; tb += 4) {  // every wave runs the same number of iterations
-    const uint64_t tile = tb + wv;
-    const bool live = tile < t1;
-    const uint64_t r0 = tile * 16;
-    __syncthreads();
-    if (live) {
-      const uint64_t rows = std::min<uint64_t>(16, nrows - r0);
-      for (uint32_t e = lane; e < 16 * l; e += 64) {
-        const uint32_t i = e / l, k = e % l;
-        my[i * (W + 1) + k] = (i < rows) ? A[r0 * l + e] : 0.0;
-      }
-    }
-    __syncthreads();
-    if (!live) continue;
-    f64x4 acc[NB];
-#pragma unroll
-    for (int cb = 0; cb < NB; ++cb) acc[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
-    for (uint32_t ks = 0; ks < ksteps; ++ks) {
-      const int k = ks * 4 + kr;
-      const double a = my[ci * (W + 1) + k];
-#pragma unroll
-      for (int cb = 0; cb < NB; ++cb) acc[cb] = mfma64(a, ri[k * W + cb * 16 + ci], acc[cb]);
-    }
-#pragma unroll
-    for (int cb = 0; cb < NB; ++cb) {
-      const uint32_t c = cb * 16 + ci;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const uint64_t r = r0 + mfma64_row(lane, v);
-        if (r < nrows && c < l) Q[r * l + c] = acc[cb][v];
-      }
-    }
-    if (part) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        int p = 0;
-#pragma unroll
-        for (int ca = 0; ca < NB; ++ca)
-#pragma unroll
-          for (int cb = ca; cb < NB; ++cb, ++p) g[p] = mfma64(acc[ca][v], acc[cb][v], g[p]);
-      }
-    }
-  }
-  if (part) {
-    __syncthreads();
-    block_tiles_out<NB, NP>(g, true, l, red, part + (uint64_t)blockIdx.x * l * l);
-  }
-}
-
-void launch_apply_rinv(const double* A, const double* Rinv, const double* sign, double* Q, uint64_t nrows,
-                       uint32_t l, double* gram_part, int nblocks, hipStream_t s) {
-  PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
-  switch ((l + 15) / 16) {
-#define PCAONE_NB(NBV)                                                                                          \
-  case NBV:                                                                                                   \
-    hipLaunchKernelGGL(k_apply_rinv<NBV>, dim3(nblocks), dim3(256), 0, s, A, Rinv, sign, Q, nrows, l, gram_part); \
-    break;
-    PCAONE_NB(1) PCAONE_NB(2) PCAONE_NB(3) PCAONE_NB(4)
-#undef PCAONE_NB
-    default: throw Error(-1, "apply: unsupported l");
   }
 }
 

@@ -162,132 +162,181 @@ __device__ inline double bcast(double v, int lane) {  // v of `lane` (uniform) i
   return __builtin_bit_cast(double, int2{__builtin_amdgcn_readlane(w.x, lane), __builtin_amdgcn_readlane(w.y, lane)});
 }
 
-// Step K of chol_inverse_wave; the step index is a template parameter so
-// every register index is a compile-time constant.  Rows and columns in
-// [l, R) hold the identity, which factors into itself, so every step runs.
-template <int WS, int R, int K>
-__device__ __attribute__((always_inline)) inline void chol_wave_step(double (&a)[R], double* E, int j, double* mrow,
-                                                                     int& bad, double& dj) {
-  if constexpr (K < R) {
-    double d = bcast(a[K], K);
+// The one-wave l x l factorisations use the wave as an 8 x 8 lane grid: lane
+// (p, q) = (lane / 8, lane % 8) holds entry (8 u + p, 8 v + q) of the
+// matrix as x[u][v], u, v < V (V = R / 8, l <= R).  Step k publishes row k
+// (and for the LU column k) through LDS; every lane then updates its own
+// entries, so a step costs a few LDS operations and at most V^2 FMAs per lane.
+// The step index is a template parameter, so all register indices and the
+// block ranges a step touches are compile-time constants.
+
+// Step K of chol_inverse_2d.  Rows and columns in [l, 8 V) hold the identity,
+// which factors into itself.  Only the upper triangle of A is kept exact
+// (entries below it are updated too but never read); E is unit lower
+// triangular, so row K of E is zero past column K.
+template <int V, int K>
+__device__ __attribute__((always_inline)) inline void chol2d_step(double (&a)[V][V], double (&e)[V][V], int p, int q,
+                                                                  double* rowA, double* rowE, double* isd, int& bad) {
+  if constexpr (K < 8 * V) {
+    constexpr int uk = K / 8, pk = K % 8;
+    double d = bcast(a[uk][uk], 9 * pk);  // lane (pk, pk) holds A[K][K]
     const bool ok = d > 0.0 && isfinite(d);
     bad = (bad == 0 && !ok) ? K + 1 : bad;
     d = ok ? d : 1e-300;
-    dj = (j == K) ? d : dj;
-    if (j < R) mrow[j] = a[K] * (1.0 / d);  // lane i: m_i = A[k][i] / A[k][k]
-    __builtin_amdgcn_wave_barrier();
-    if (j < WS) {
-      // E[k][j] is zero for j > k, which leaves those columns unchanged;
-      // loads are grouped eight rows at a time (register pressure)
-      const double ek = E[K * WS + j];
+    const double inv = 1.0 / d;
+    if (p == pk) {
 #pragma unroll
-      for (int i = K + 1; i < R; ++i) {
-        const double m = mrow[i];
-        a[i] = fma(-m, a[K], a[i]);
-        E[i * WS + j] = fma(-m, ek, E[i * WS + j]);
-        if ((i & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      for (int v = 0; v < V; ++v) {
+        rowA[8 * v + q] = a[uk][v];
+        rowE[8 * v + q] = e[uk][v];
       }
     }
+    if (p == 0 && q == 0) isd[K] = 1.0 / sqrt(d);
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_sched_barrier(0);
-    chol_wave_step<WS, R, K + 1>(a, E, j, mrow, bad, dj);
+    double ak[V], ek[V], m[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      ak[v] = (v >= uk) ? rowA[8 * v + q] : 0.0;  // A[K][j], j = 8 v + q (j >= i > K needs v >= uk)
+      ek[v] = (v <= uk) ? rowE[8 * v + q] : 0.0;  // E[K][j], zero past j = K
+      m[v] = (v >= uk) ? rowA[8 * v + p] * inv : 0.0;  // A[K][i] / A[K][K], i = 8 v + p
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = uk; u < V; ++u) {
+      const bool live = u > uk || p > pk;  // row 8 u + p below the pivot
+      const double mu = live ? m[u] : 0.0;
+#pragma unroll
+      for (int v = u; v < V; ++v) a[u][v] = fma(-mu, ak[v], a[u][v]);
+#pragma unroll
+      for (int v = 0; v <= uk; ++v) e[u][v] = fma(-mu, ek[v], e[u][v]);
+    }
+    // the step's updates complete here (no sinking into later steps: registers)
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+#pragma unroll
+      for (int v = 0; v < V; ++v) asm volatile("" : "+v"(a[u][v]), "+v"(e[u][v]));
+    chol2d_step<V, K + 1>(a, e, p, q, rowA, rowE, isd, bad);
   }
 }
 
-// chol_inverse in one wave for l <= R (R <= 48 registers): lane j holds
-// column j of A in registers (upper part; the strict lower part is updated
-// too but never read) and updates column j of E in LDS.  Step k: every lane
-// publishes m_j = A[k][j] / A[k][k] (row k equals column k) through LDS, and
-// lane j applies row_i -= m_i row_k to its columns for every i > k (E's column
-// j is zero above row j).  Called by the whole wave; A, E and rinv are LDS
-// with row stride WS (rinv comes out zero padded to WS x WS); mrow is WS
-// doubles of LDS.
-template <int WS, int R>
-__device__ __attribute__((always_inline)) inline int chol_inverse_wave(const double* A, double* E, uint32_t l,
-                                                                       double* rinv, double* mrow) {
-  const int j = threadIdx.x & 63;
+// R^-1 of the Cholesky factor of A (upper triangle of a W x W LDS matrix,
+// l <= 8 V) by elimination: row operations reduce A to U = D L^T and turn the
+// identity E into L^-1, so R^-1[i][c] = E[c][i] / sqrt(d_c).  Called by one
+// whole wave; rinv (W x W LDS, row stride W) comes out zero padded; scratch is
+// 3 * 8 V doubles of LDS.  Returns the first failing pivot (1-based), 0 if none.
+template <int W, int V>
+__device__ __attribute__((always_inline)) inline int chol_inverse_2d(const double* A, uint32_t l, double* rinv,
+                                                                     double* scratch) {
+  const int lane = threadIdx.x & 63, p = lane >> 3, q = lane & 7;
   const int L = __builtin_amdgcn_readfirstlane((int)l);
-  double a[R];
+  double a[V][V], e[V][V];
 #pragma unroll
-  for (int i = 0; i < R; ++i) a[i] = (i <= j && j < L) ? A[i * WS + j] : (i == j ? 1.0 : 0.0);
-  if (j < WS)
-    for (int i = 0; i < WS; ++i) E[i * WS + j] = (i == j) ? 1.0 : 0.0;
-  __builtin_amdgcn_wave_barrier();
+  for (int u = 0; u < V; ++u)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int r = 8 * u + p, c = 8 * v + q;
+      a[u][v] = (r <= c && c < L) ? A[r * W + c] : (r == c ? 1.0 : 0.0);
+      e[u][v] = (r == c) ? 1.0 : 0.0;
+    }
+  double* rowA = scratch;
+  double* rowE = scratch + 8 * V;
+  double* isd = scratch + 16 * V;
   int bad = 0;
-  double dj = 1.0;
-  chol_wave_step<WS, R, 0>(a, E, j, mrow, bad, dj);
-  // R^-1[i][c] = E[c][i] / sqrt(d_c): lane i writes row i from column i of E
-  if (j < WS) mrow[j] = 1.0 / sqrt(dj);
+  chol2d_step<V, 0>(a, e, p, q, rowA, rowE, isd, bad);
   __builtin_amdgcn_wave_barrier();
-  if (j < WS)
-    for (int c = 0; c < WS; ++c) rinv[j * WS + c] = (j <= c && c < L) ? E[c * WS + j] * mrow[c] : 0.0;
+  for (int t = lane; t < W * W; t += 64) rinv[t] = 0.0;
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int u = 0; u < V; ++u)
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int c = 8 * u + p, i = 8 * v + q;  // e[u][v] = E[c][i]
+      if (i <= c && c < L) rinv[i * W + c] = e[u][v] * isd[c];
+    }
   __builtin_amdgcn_wave_barrier();
   return bad > L ? 0 : bad;
 }
 
-template <int R, int K>
-__device__ __attribute__((always_inline)) inline void signs_wave_step(double (&b)[R], int j, int L, double* sign) {
-  if constexpr (K < R) {
-    const double bkk = bcast(b[K], K);
+// Step K of householder_signs_2d: LU of B - S with S_KK = -sign(pivot).
+// Rows and columns at or left of K are dead after step K, so updates over
+// whole blocks are harmless.
+template <int V, int K>
+__device__ __attribute__((always_inline)) inline void signs2d_step(double (&b)[V][V], int p, int q, int L,
+                                                                   double* colK, double* rowK, double* sign) {
+  if constexpr (K < 8 * V) {
+    constexpr int kb = K / 8, kc = K % 8;
+    const double bkk = bcast(b[kb][kb], 9 * kc);
     const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
     const double ip = 1.0 / (bkk - sk);
-    if (j == 0 && K < L) sign[K] = sk;
+    if (p == 0 && q == 0 && K < L) sign[K] = sk;
+    if (q == kc)
 #pragma unroll
-    for (int i = K + 1; i < R; ++i) {
-      const double lik = bcast(b[i], K) * ip;
-      b[i] = fma(-lik, b[K], b[i]);
+      for (int u = 0; u < V; ++u) colK[8 * u + p] = b[u][kb];
+    if (p == kc)
+#pragma unroll
+      for (int v = 0; v < V; ++v) rowK[8 * v + q] = b[kb][v];
+    __builtin_amdgcn_wave_barrier();
+    double lm[V], rk[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      lm[u] = (u >= kb) ? colK[8 * u + p] * ip : 0.0;  // B[i][K] / pivot, i = 8 u + p
+      rk[u] = (u >= kb) ? rowK[8 * u + q] : 0.0;       // B[K][j], j = 8 u + q
     }
-    __builtin_amdgcn_sched_barrier(0);
-    signs_wave_step<R, K + 1>(b, j, L, sign);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = kb; u < V; ++u)
+#pragma unroll
+      for (int v = kb; v < V; ++v) b[u][v] = fma(-lm[u], rk[v], b[u][v]);
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+#pragma unroll
+      for (int v = 0; v < V; ++v) asm volatile("" : "+v"(b[u][v]));
+    signs2d_step<V, K + 1>(b, p, q, L, colK, rowK, sign);
   }
 }
 
-// Householder R-diagonal signs of the top l x l block B (LDS, row stride WS,
-// l <= R) of a CholeskyQR factor, in one wave: LU of B - S with
-// S_kk = -sign(pivot) (Householder reconstruction, Ballard et al. 2014), lane
-// j holding column j.  Entries left of the pivot column are updated too but
-// never read again; steps past l only touch rows and columns past l.
-template <int WS, int R>
-__device__ __attribute__((always_inline)) inline void householder_signs_wave(const double* B, uint32_t l,
-                                                                             double* sign) {
-  const int j = threadIdx.x & 63;
-  double b[R];
+// Householder R-diagonal signs of the top l x l block B (LDS, row stride W,
+// l <= 8 V) of a CholeskyQR factor, in one wave: LU of B - S with
+// S_kk = -sign(pivot) (Householder reconstruction, Ballard et al. 2014).
+// Rows and columns past l only touch entries past l.  scratch: 16 V doubles.
+template <int W, int V>
+__device__ __attribute__((always_inline)) inline void householder_signs_2d(const double* B, uint32_t l, double* sign,
+                                                                          double* scratch) {
+  const int lane = threadIdx.x & 63, p = lane >> 3, q = lane & 7;
+  double b[V][V];
 #pragma unroll
-  for (int i = 0; i < R; ++i) b[i] = (j < WS) ? B[i * WS + j] : 0.0;
-  signs_wave_step<R, 0>(b, j, __builtin_amdgcn_readfirstlane((int)l), sign);
+  for (int u = 0; u < V; ++u)
+#pragma unroll
+    for (int v = 0; v < V; ++v) b[u][v] = B[(8 * u + p) * W + 8 * v + q];
+  signs2d_step<V, 0>(b, p, q, __builtin_amdgcn_readfirstlane((int)l), scratch, scratch + 8 * V, sign);
 }
 
-// l rounded up to 8, the register count of the one-wave routines
-#define PCAONE_QR_BY_R(NB, l, CALL)           \
-  do {                                        \
-    const uint32_t r8_ = ((l) + 7) / 8;       \
-    if (r8_ <= (NB) * 2 - 1) {                \
-      constexpr int R = (NB) * 16 - 8;        \
-      CALL;                                   \
-    } else {                                  \
-      constexpr int R = (NB) * 16;            \
-      CALL;                                   \
-    }                                         \
-  } while (0)
+// The one-wave routines hold 8 V = 16 NB - 8 rows and columns: l up to 8,
+// 24, 40 or 56 for NB = 1 .. 4 (the register budget of two waves per SIMD);
+// larger l of an NB falls back to the whole-block versions.
+template <int NB>
+__device__ __host__ constexpr int wave_v() { return NB <= 3 ? 2 * NB - 1 : 0; }
 
 // R^-1 of the Gram in a (upper triangle, W x W LDS) into rinv (W x W LDS):
-// one wave for l <= 48, the whole block above.  Ends with a block barrier.
+// one wave when l fits wave_v, the whole block above.  Ends with a block
+// barrier.
 template <int NB>
-__device__ __attribute__((always_inline)) inline int factor(double* a, double* E, double* isd, uint32_t l, double* rinv) {
-  constexpr int W = NB * 16;
-  if constexpr (NB <= 3) {
-    __shared__ int bad_sh;
-    __syncthreads();
-    if ((threadIdx.x >> 6) == 0) {
-      int bad = 0;
-      PCAONE_QR_BY_R(NB, l, bad = (chol_inverse_wave<W, R>(a, E, l, rinv, isd)));
-      if (threadIdx.x == 0) bad_sh = bad;
+__device__ __attribute__((always_inline)) inline int factor(double* a, double* E, double* isd, uint32_t l,
+                                                            double* rinv) {
+  constexpr int W = NB * 16, V = wave_v<NB>();
+  if constexpr (V > 0) {
+    if (l <= 8 * V) {
+      __shared__ int bad_sh;
+      __syncthreads();
+      if ((threadIdx.x >> 6) == 0) {
+        const int bad = chol_inverse_2d<W, V>(a, l, rinv, E);
+        if (threadIdx.x == 0) bad_sh = bad;
+      }
+      __syncthreads();
+      return bad_sh;
     }
-    __syncthreads();
-    return bad_sh;
-  } else {
-    return chol_inverse<W>(a, E, isd, l, rinv);
   }
+  return chol_inverse<W>(a, E, isd, l, rinv);
 }
 
 __device__ inline void stage_padded(const double* __restrict__ src, uint32_t l, int W, double* dst) {
@@ -358,19 +407,22 @@ __device__ inline void wave_tiles(uint64_t n, bool skip0, uint64_t& t0, uint64_t
 }
 
 // ---------------------------------------------------------------------------
+// Gram partial of the rows of A1 (+ A2, written to Aout) over this block's
+// share of the 4-row K steps, summed over the block's waves into dst.
 template <int NB>
-__global__ __launch_bounds__(kThreads) void k_qr_gram_h(QrPass q) {
+__device__ __attribute__((always_inline)) inline void gram_rows_pass(const double* __restrict__ A1,
+                                                                     const double* __restrict__ A2, double* Aout,
+                                                                     uint64_t n, uint32_t l, double* slots,
+                                                                     double* dst) {
   using S = Shape<NB>;
-  __shared__ double slots[S::SLOTS];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int kr = lane >> 4, ci = lane & 15;
-  const uint32_t l = q.l;
   f64x4 acc[S::NP];
 #pragma unroll
   for (int p = 0; p < S::NP; ++p) acc[p] = f64x4{0.0, 0.0, 0.0, 0.0};
-  // 4-row K steps: lane (k, i) loads H[row k][16 cb + i], at once the A and
+  // 4-row K steps: lane (k, i) loads A[row k][16 cb + i], at once the A and
   // the B operand of the tile products
-  const uint64_t steps = ceil_div(q.n, 4);
+  const uint64_t steps = ceil_div(n, 4);
   const uint64_t nw = (uint64_t)gridDim.x * kWaves, wid = (uint64_t)blockIdx.x * kWaves + wv;
   const uint64_t sps = ceil_div(steps, nw);
   const uint64_t s0 = std::min<uint64_t>(wid * sps, steps), s1 = std::min<uint64_t>(s0 + sps, steps);
@@ -379,24 +431,24 @@ __global__ __launch_bounds__(kThreads) void k_qr_gram_h(QrPass q) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint64_t r = (st0 + u) * 4 + kr;
-      const bool live = st0 + u < s1 && r < q.n;
+      const bool live = st0 + u < s1 && r < n;
 #pragma unroll
       for (int cb = 0; cb < NB; ++cb) {
         const uint32_t c = cb * 16 + ci;
-        a[u][cb] = (live && c < l) ? q.H1[r * l + c] : 0.0;
+        a[u][cb] = (live && c < l) ? A1[r * l + c] : 0.0;
       }
     }
-    if (q.H2) {
+    if (A2) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint64_t r = (st0 + u) * 4 + kr;
-        const bool live = st0 + u < s1 && r < q.n;
+        const bool live = st0 + u < s1 && r < n;
 #pragma unroll
         for (int cb = 0; cb < NB; ++cb) {
           const uint32_t c = cb * 16 + ci;
           if (live && c < l) {
-            a[u][cb] += q.H2[r * l + c];
-            q.H[r * l + c] = a[u][cb];
+            a[u][cb] += A2[r * l + c];
+            Aout[r * l + c] = a[u][cb];
           }
         }
       }
@@ -411,7 +463,72 @@ __global__ __launch_bounds__(kThreads) void k_qr_gram_h(QrPass q) {
     }
   }
   block_sum_waves<S::NP>(acc, slots);
-  store_gram<S::NP>(acc, q.part + (uint64_t)blockIdx.x * S::GRAM);
+  store_gram<S::NP>(acc, dst);
+}
+
+// Q = A Rinv (rows of A in 16-row tiles, Rinv a zero-padded W x W LDS matrix)
+// over this block's share of the tiles, written to Qout when given, and the
+// Gram partial of Q summed over the block's waves into dst.
+template <int NB>
+__device__ __attribute__((always_inline)) inline void apply_gram_pass(const double* __restrict__ A, uint64_t n,
+                                                                      uint32_t l, const double* rinv, double* Qout,
+                                                                      double* slots, double* dst) {
+  using S = Shape<NB>;
+  constexpr int W = S::W;
+  const int lane = threadIdx.x & 63;
+  const int kr = lane >> 4, ci = lane & 15;
+  f64x4 g[S::NP];
+#pragma unroll
+  for (int p = 0; p < S::NP; ++p) g[p] = f64x4{0.0, 0.0, 0.0, 0.0};
+  uint64_t t0, t1;
+  wave_tiles(n, false, t0, t1);
+  for (uint64_t tile = t0; tile < t1; ++tile) {
+    // Q tile = A tile Rinv: lane (k, i) supplies A[r0 + i][4 ks + k]
+    const uint64_t row = tile * 16 + ci;
+    double x[W / 4];
+#pragma unroll
+    for (int ks = 0; ks < W / 4; ++ks) {
+      const uint32_t k = ks * 4 + kr;
+      x[ks] = (row < n && k < l) ? A[row * l + k] : 0.0;
+    }
+    f64x4 acc[NB];
+#pragma unroll
+    for (int cb = 0; cb < NB; ++cb) acc[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < W / 4; ++ks) {
+      if (ks * 4 >= (int)l) break;
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb) acc[cb] = mfma64(x[ks], rinv[(ks * 4 + kr) * W + cb * 16 + ci], acc[cb]);
+    }
+    if (Qout)
+#pragma unroll
+      for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const uint64_t r = tile * 16 + kr + 4 * v;
+          const uint32_t c = cb * 16 + ci;
+          if (r < n && c < l) Qout[r * l + c] = acc[cb][v];
+        }
+    // accumulator element v is the 4-row slab (v, lane / 16) of the tile, in
+    // the operand layout of the Gram's K step
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      int p = 0;
+#pragma unroll
+      for (int ca = 0; ca < NB; ++ca)
+#pragma unroll
+        for (int cb = ca; cb < NB; ++cb, ++p) g[p] = mfma64(acc[ca][v], acc[cb][v], g[p]);
+    }
+  }
+  block_sum_waves<S::NP>(g, slots);
+  store_gram<S::NP>(g, dst);
+}
+
+template <int NB>
+__global__ __launch_bounds__(kThreads) void k_qr_gram_h(QrPass q) {
+  using S = Shape<NB>;
+  __shared__ double slots[S::SLOTS];
+  gram_rows_pass<NB>(q.H1, q.H2, q.H, q.n, q.l, slots, q.part + (uint64_t)blockIdx.x * S::GRAM);
 }
 
 template <int NB>
@@ -436,38 +553,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
       if (threadIdx.x == 0 && bad && q.status[0] == 0) q.status[0] = bad;
     }
   }
-  f64x4 g[S::NP];
-#pragma unroll
-  for (int p = 0; p < S::NP; ++p) g[p] = f64x4{0.0, 0.0, 0.0, 0.0};
-  uint64_t t0, t1;
-  wave_tiles(q.n, false, t0, t1);
-  for (uint64_t tile = t0; tile < t1; ++tile) {
-    // Q1 tile = H tile R1^-1: lane (k, i) supplies H[r0 + i][4 ks + k]
-    const uint64_t row = tile * 16 + ci;
-    f64x4 acc[NB];
-#pragma unroll
-    for (int cb = 0; cb < NB; ++cb) acc[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int ks = 0; ks < W / 4; ++ks) {
-      if (ks * 4 >= (int)l) break;
-      const uint32_t k = ks * 4 + kr;
-      const double x = (row < q.n && k < l) ? q.H[row * l + k] : 0.0;
-#pragma unroll
-      for (int cb = 0; cb < NB; ++cb) acc[cb] = mfma64(x, r1[k * W + cb * 16 + ci], acc[cb]);
-    }
-    // accumulator element v is the 4-row slab (v, lane / 16) of the tile, in
-    // the operand layout of the Gram's K step
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      int p = 0;
-#pragma unroll
-      for (int ca = 0; ca < NB; ++ca)
-#pragma unroll
-        for (int cb = ca; cb < NB; ++cb, ++p) g[p] = mfma64(acc[ca][v], acc[cb][v], g[p]);
-    }
-  }
-  block_sum_waves<S::NP>(g, sh);
-  store_gram<S::NP>(g, q.part + (uint64_t)(kMaxBlocks + blockIdx.x) * S::GRAM);
+  apply_gram_pass<NB>(q.H, q.n, l, r1, nullptr, sh, q.part + (uint64_t)(kMaxBlocks + blockIdx.x) * S::GRAM);
 }
 
 // The top W rows of the unsigned Omega (row stride W) from its first tiles,
@@ -511,10 +597,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
   using S = Shape<NB>;
   constexpr int W = S::W;
   constexpr int PRO = 2 * W * W + W;
-  constexpr bool kWaveLU = NB <= 3;  // the sign LU runs in one wave beside the pass
   __shared__ double r1[W * W], r2[W * W];
   __shared__ double sh[PRO > kWaves * S::STAT ? PRO : kWaves * S::STAT];
-  __shared__ double btw[kWaveLU ? W * W : 1];
+  // the top block of Omega for the one-wave sign LU and its row / column exchange
+  __shared__ double btw[wave_v<NB>() > 0 ? W * W : 1];
+  __shared__ double lus[wave_v<NB>() > 0 ? 2 * W : 1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int kr = lane >> 4, ci = lane & 15;
   const uint32_t l = q.l, t = threadIdx.x, nt = blockDim.x;
@@ -531,19 +618,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
       if (t == 0 && bad && q.status[1] == 0) q.status[1] = bad;
     }
   }
-  // the Householder signs of the unsigned Omega's top l x l block
-  bool lu_wave = false;
-  if constexpr (kWaveLU) {
-    lu_wave = blockIdx.x == 0 && wv == 0;
-    if (blockIdx.x == 0) {  // waves 0 .. NB-1 compute the top tiles; wave 0 then runs the LU
-      omega_top<NB>(q, r1, r2, NB, btw);
-      __syncthreads();
+  // the Householder signs of the unsigned Omega's top l x l block: one wave
+  // beside the pass when l fits wave_v, else the whole of block 0 before it
+  constexpr int V = wave_v<NB>();
+  const bool wave_lu = l <= 8 * V;
+  if (blockIdx.x == 0) {
+    if constexpr (V > 0) {
+      if (wave_lu) {  // waves 0 .. NB-1 compute the top tiles; wave 0 then runs the LU
+        omega_top<NB>(q, r1, r2, NB, btw);
+        __syncthreads();
+        if (wv == 0) householder_signs_2d<W, V>(btw, l, q.sign, lus);
+      }
     }
-    if (lu_wave) {
-      PCAONE_QR_BY_R(NB, l, (householder_signs_wave<W, R>(btw, l, q.sign)));
-    }
-  } else {
-    if (blockIdx.x == 0) {
+    if (!wave_lu) {
       omega_top<NB>(q, r1, r2, kWaves, sh);
       householder_signs_block<W>(sh, l, q.sign);
     }
@@ -554,7 +641,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
 #pragma unroll
     for (int v = 0; v < 4; ++v) dm[c2][v] = dp[c2][v] = cs[c2][v] = 0.0;
   uint64_t t0, t1;
-  wave_tiles(q.n, kWaveLU, t0, t1);
+  wave_tiles(q.n, wave_lu, t0, t1);
   for (uint64_t tile = t0; tile < t1; ++tile) {
     f64x4 om[NB];
     tile_omega<NB>(q.H, q.n, l, tile * 16, r1, r2, om);
@@ -677,11 +764,168 @@ void launch_nb(QrPass q, int pass, hipStream_t s) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// computeUSV's CholeskyQR2 of G (M x l) and B^T = H R^-1 (src/Halko.cpp:53-68)
+// ---------------------------------------------------------------------------
+constexpr int kUsvBlocks = 256;  // one block per CU for the M-row passes
+
+// out = sum over nparts Gram partials (tile layout, in block order); with
+// square, out is the full symmetric l x l matrix (row-major) instead.
+template <int NB>
+__global__ __launch_bounds__(256) void k_gram_sum(const double* __restrict__ part, int nparts, uint32_t l,
+                                                  int square, double* __restrict__ out) {
+  using S = Shape<NB>;
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (e < S::GRAM)
+    for (int b0 = wv * 16; b0 < nparts; b0 += 64) {  // wave w takes batches 4 j + w of 16 partials
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = (b0 + u < nparts) ? part[(uint64_t)(b0 + u) * S::GRAM + e] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) s += v[u];
+    }
+  red[wv][lane] = s;
+  __syncthreads();
+  if (wv != 0 || e >= S::GRAM) return;
+  const double tot = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+  if (!square) {
+    out[e] = tot;
+    return;
+  }
+  const int p = e >> 8, v = (e >> 6) & 3, ln = e & 63;
+  int ca = 0, rem = p;
+  while (rem >= NB - ca) {
+    rem -= NB - ca;
+    ++ca;
+  }
+  const int cb = ca + rem;
+  const uint32_t r = ca * 16 + (ln >> 4) + 4 * v, c = cb * 16 + (ln & 15);
+  if (r < l && c < l && r <= c) {
+    out[r * l + c] = tot;
+    out[c * l + r] = tot;
+  }
+}
+
+// Gram partials of G's rows.
+template <int NB>
+__global__ __launch_bounds__(kThreads) void k_usv_gram(UsvPass u) {
+  using S = Shape<NB>;
+  __shared__ double slots[S::SLOTS];
+  gram_rows_pass<NB>(u.G, nullptr, nullptr, u.m, u.l, slots, u.part + (uint64_t)blockIdx.x * S::GRAM);
+}
+
+// prologue: R1^-1 from the reduced Gram of G; pass: Gram partials of
+// Q1 = G R1^-1 (Q1 is not stored).
+template <int NB>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_usv_q1(UsvPass u) {
+  using S = Shape<NB>;
+  constexpr int W = S::W;
+  constexpr int PRO = 2 * W * W + W;
+  __shared__ double r1[W * W];
+  __shared__ double sh[S::SLOTS > PRO ? S::SLOTS : PRO];
+  const uint32_t l = u.l;
+  {
+    double* a = sh;
+    double* E = a + W * W;
+    double* isd = E + W * W;
+    gram_to_upper<NB>(u.gram, 1, l, a);
+    const int bad = factor<NB>(a, E, isd, l, r1);
+    if (blockIdx.x == 0) {
+      store_compact(r1, l, W, u.R1i);
+      if (threadIdx.x == 0 && bad && u.status[0] == 0) u.status[0] = bad;
+    }
+  }
+  apply_gram_pass<NB>(u.G, u.m, l, r1, nullptr, sh, u.part + (uint64_t)blockIdx.x * S::GRAM);
+}
+
+// prologue: R2^-1 from the reduced Gram of Q1 and Rt^-1 = R1^-1 R2^-1 (the
+// inverse of R = R2 R1); pass: B^T = H Rt^-1 (written) and its Gram partials.
+template <int NB>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_usv_bt(UsvPass u) {
+  using S = Shape<NB>;
+  constexpr int W = S::W;
+  constexpr int PRO = 2 * W * W + W;
+  __shared__ double r1[W * W], rt[W * W];
+  __shared__ double sh[S::SLOTS > PRO ? S::SLOTS : PRO];
+  const uint32_t l = u.l, t = threadIdx.x;
+  stage_padded(u.R1i, l, W, r1);
+  {
+    double* a = sh;
+    double* E = a + W * W;
+    double* isd = E + W * W;
+    gram_to_upper<NB>(u.gram, 1, l, a);
+    const int bad = factor<NB>(a, E, isd, l, rt);  // rt holds R2^-1 for now
+    if (blockIdx.x == 0) {
+      store_compact(rt, l, W, u.R2i);
+      if (t == 0 && bad && u.status[1] == 0) u.status[1] = bad;
+    }
+    // Rt^-1 = R1^-1 R2^-1 (both upper triangular) into a, then back to rt
+    for (uint32_t e = t; e < W * W; e += blockDim.x) {
+      const uint32_t i = e / W, j = e % W;
+      double v = 0.0;
+      for (uint32_t k = i; k <= j && k < l; ++k) v = fma(r1[i * W + k], rt[k * W + j], v);
+      a[e] = v;
+    }
+    __syncthreads();
+    for (uint32_t e = t; e < W * W; e += blockDim.x) rt[e] = a[e];
+    __syncthreads();
+    if (blockIdx.x == 0) store_compact(rt, l, W, u.Rti);
+  }
+  apply_gram_pass<NB>(u.H, u.n, l, rt, u.Bt, sh, u.part + (uint64_t)blockIdx.x * S::GRAM);
+}
+
+template <int NB>
+void launch_usv_nb(UsvPass u, hipStream_t s, int step) {
+  using S = Shape<NB>;
+  const uint32_t mt = (uint32_t)ceil_div(u.m, 16), nt = (uint32_t)ceil_div(u.n, 16);
+  const int gm = (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(mt, kWaves), 1), kUsvBlocks);
+  const int gn = (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(nt, kWaves), 1), kMaxBlocks);
+  const dim3 sum_grid((uint32_t)ceil_div(S::GRAM, 64));
+  switch (step) {
+    case 0:  // Gram of G, reduced
+      hipLaunchKernelGGL(k_usv_gram<NB>, dim3(gm), dim3(kThreads), 0, s, u);
+      hipLaunchKernelGGL(k_gram_sum<NB>, sum_grid, dim3(256), 0, s, u.part, gm, u.l, 0, u.gram);
+      break;
+    case 1:  // R1^-1, Gram of Q1, reduced
+      hipLaunchKernelGGL(k_usv_q1<NB>, dim3(gm), dim3(kThreads), 0, s, u);
+      hipLaunchKernelGGL(k_gram_sum<NB>, sum_grid, dim3(256), 0, s, u.part, gm, u.l, 0, u.gram);
+      break;
+    default:  // R2^-1, Rt^-1, B^T and B B^T (square)
+      hipLaunchKernelGGL(k_usv_bt<NB>, dim3(gn), dim3(kThreads), 0, s, u);
+      hipLaunchKernelGGL(k_gram_sum<NB>, sum_grid, dim3(256), 0, s, u.part, gn, u.l, 1, u.bbt);
+      break;
+  }
+}
+
 }  // namespace
 
 uint64_t qr_part_doubles(uint32_t l) {
   const uint64_t nb = ceil_div(l, 16), np = nb * (nb + 1) / 2;
   return (uint64_t)kMaxBlocks * (2 * np * 256 + 3 * nb * 16);
+}
+
+uint64_t usv_part_doubles(uint32_t l) {
+  const uint64_t nb = ceil_div(l, 16), np = nb * (nb + 1) / 2;
+  return (uint64_t)kUsvBlocks * np * 256;
+}
+
+uint64_t usv_gram_doubles(uint32_t l) {
+  const uint64_t nb = ceil_div(l, 16), np = nb * (nb + 1) / 2;
+  return np * 256;
+}
+
+void launch_usv_qr(const UsvPass& u, int step, hipStream_t s) {
+  PCAONE_REQUIRE(u.l >= 1 && u.l <= 64, -1, "l must be in [1, 64]");
+  PCAONE_REQUIRE(u.m >= u.l && u.n >= 1, -1, "computeUSV: G needs at least l rows");
+  switch ((u.l + 15) / 16) {
+    case 1: launch_usv_nb<1>(u, s, step); break;
+    case 2: launch_usv_nb<2>(u, s, step); break;
+    case 3: launch_usv_nb<3>(u, s, step); break;
+    default: launch_usv_nb<4>(u, s, step); break;
+  }
 }
 
 void launch_qr_pass(const QrPass& q, int pass, hipStream_t s) {
