@@ -515,6 +515,7 @@ void Engine::gram_init(int precision, bool standardize, int ploidy) {
   b.g2 = mv_g2_.p;
   b.S2 = mv_S2_.p;
   b.s2p = mv_s2p_.p;
+  b.affine = src_ == SRC_BED && precision == PREC_SPLIT_BF16;
   b.bg = mv_bg_.p;
   b.bh = mv_bh_.p;
   b.gpart = mv_gpart_.p;
@@ -709,6 +710,7 @@ SegBufs Engine::op_bufs() {
   b.corrH = nnz_snp_ > 0 ? corrH_.p : nullptr;
   b.S2 = S2_.p;
   b.s2p = s2p_.p;
+  b.affine = src_ == SRC_BED && p_.precision == PREC_SPLIT_BF16;
   b.bg = bg_.p;
   b.bh = bh_.p;
   b.gpart = gpart_.p;
@@ -824,6 +826,7 @@ FinalizeG Engine::finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) cons
   fg.w1 = w1;
   fg.nsnps = M_;
   fg.s2p = b.s2p;
+  fg.affine = b.affine ? 1 : 0;
   fg.kperm = kperm_(b.prec);
   return fg;
 }
@@ -912,6 +915,7 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
   fh.Ha = b.Ha;
   fh.Hb = b.Hb;
   fh.S2 = b.S2;
+  fh.affine = b.affine ? 1 : 0;
   fh.miss_ptr = miss_win_ptr_.p;
   fh.miss_idx = miss_win_idx_.p;
   fh.corr = b.corrH;

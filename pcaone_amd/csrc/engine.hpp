@@ -63,6 +63,7 @@ struct SegBufs {
   float* omega32 = nullptr;  // FP32 copy of Omega: the missing-entry gather table of the non-FP64 arms
   const double* colsum = nullptr;
   double *G = nullptr, *S2 = nullptr, *s2p = nullptr;
+  bool affine = false;  // the packed split-BF16 contraction runs on 2 - 2 y
   void* g2 = nullptr;        // 2 F d G, float when g2_f32
   bool g2_f32 = false;
   double *corrG = nullptr, *corrH = nullptr;  // missing-entry sums (null: nothing missing)

@@ -65,7 +65,9 @@ struct PackedGemm {
   uint64_t o_zstride;  // elements between z-slices
   uint32_t nz;
 };
-// split-BF16 MFMA: B is bf16 with nt16*16 columns ([hi | lo | 0-pad]); out fp32
+// split-BF16 MFMA: B is bf16 with nt16*16 columns ([hi | lo | 0-pad]); out fp32.
+// The A operand is y' = 2 - 2 y (y the dosage), so out = sum_k y'(row,k) B[col][k]
+// and X B = colsum(B) - out / 2 (the finalize kernels' `affine` flag).
 void launch_packed_gemm_bf16(const PackedGemm& g, int nt16, hipStream_t s);
 // FP32 / FP64 vector-FMA: B has `ncols` columns of float / double; out same type
 void launch_packed_gemm_f32(const PackedGemm& g, int ncols, hipStream_t s);
@@ -103,8 +105,9 @@ struct FinalizeG {
   double hf;            // 0.5 for A1 dosages (BED2GENO = y/2), 1 for dense values
   uint32_t w0, w1;      // windows of the segment ([r0, r1) are their rows)
   uint32_t nsnps;
-  double* s2p;          // [nwin][finalize_chunks(blocksize)][l] partial sums of g2
+  double* s2p;          // [nwin][finalize_chunks(blocksize)][l] partial sums of g2 (affine: of g2 - g')
   int kperm;            // operand positions in the split packed contraction's K order
+  int affine;           // part holds the 2 - 2 y contraction: X Omega = colsum - part / 2
 };
 // grid = (chunks of 64 rows, windows of the segment)
 void launch_finalize_G(const FinalizeG& f, hipStream_t s);
@@ -140,7 +143,8 @@ struct FinalizeH {
   uint64_t route_b;     // bit (w - w0) set: window w adds into Hb, else Ha
   double* Ha;
   double* Hb;
-  const double* S2;     // [nwin][l]
+  const double* S2;     // [nwin][l] window sums of g2 (affine: of g2 - g')
+  int affine;           // part holds the 2 - 2 y contraction: X^T g' = sum g' - part / 2
   const int64_t* miss_ptr;  // [nwin*miss_rows+1]  (EMU values only)
   const int32_t* miss_idx;  // SNP rows
   const double* corr;   // [nwin][nsamples][l] sum of g2 over each (window, sample) missing list, or null

@@ -245,18 +245,23 @@ __device__ inline bf16x8 decode8(uint32_t h) {
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // 16 codes (one 32-bit word) -> the A operands of two MFMA K steps, in kperm16
-// order: spread word x_j = (w >> 2j) & 0x03030303 holds codes j, j+4, j+8,
-// j+12 one per byte; each becomes four BF16 dosages through the v_perm tables.
+// order.  The operand is not the dosage y in {2, 1, 0} but y' = 2 - 2 y in
+// {-2, 0, 2}: every such BF16 value has a zero low byte, so a code becomes its
+// BF16 value with one table lookup (v_perm over the high bytes) and one v_perm
+// that interleaves zero bytes.  The epilogues recover X B = colsum(B) - X' B / 2
+// exactly (their `affine` flag).  Spread word x_j = (w >> 2j) & 0x03030303 holds
+// codes j, j+4, j+8, j+12, one per byte.
 __device__ inline void decode16(uint32_t w, bf16x8& a0, bf16x8& a1) {
-  constexpr uint32_t LUT_HI = 0x003F0040u;  // high bytes of bf16 {2.0, 0, 1.0, 0}
-  constexpr uint32_t LUT_LO = 0x00800000u;  // low bytes
+  // high bytes of bf16(2 - 2 y) by code: 00 (y 2) -> -2, 01 (missing, y 0) -> 2,
+  // 10 (y 1) -> 0, 11 (y 0) -> 2
+  constexpr uint32_t LUT = 0x400040C0u;
   uint32_t o[8];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const uint32_t x = (w >> (2 * j)) & 0x03030303u;
-    const uint32_t y = x + 0x04040404u;
-    o[2 * j] = __builtin_amdgcn_perm(LUT_LO, LUT_HI, __builtin_amdgcn_perm(y, x, 0x01050004u));
-    o[2 * j + 1] = __builtin_amdgcn_perm(LUT_LO, LUT_HI, __builtin_amdgcn_perm(y, x, 0x03070206u));
+    const uint32_t h = __builtin_amdgcn_perm(0u, LUT, x);   // high bytes of codes j, j+4, j+8, j+12
+    o[2 * j] = __builtin_amdgcn_perm(0u, h, 0x010c000cu);       // [0, h0, 0, h1]
+    o[2 * j + 1] = __builtin_amdgcn_perm(0u, h, 0x030c020cu);   // [0, h2, 0, h3]
   }
   __builtin_memcpy(&a0, o, sizeof(a0));
   __builtin_memcpy(&a1, o + 4, sizeof(a1));
@@ -687,6 +692,7 @@ __global__ __launch_bounds__(1024) void k_finalize_G(FinalizeG f) {
         raw += load_part(f.part, f.part_is_f64, base + c);
         if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
       }
+      if (f.affine) raw = f.colsum[c] - 0.5 * raw;  // the contraction ran on 2 - 2 y
       const double corr = f.corr ? f.corr[(uint64_t)r * f.l + c] : 0.0;
       double corr_e = 0.0;
       if (f.mval) {
@@ -699,7 +705,7 @@ __global__ __launch_bounds__(1024) void k_finalize_G(FinalizeG f) {
       const double gp = di * Gv;
       const double g2 = 2.0 * Fi * gp;
       store_g2(f, (uint64_t)r * f.l + c, g2);
-      acc2 += g2;
+      acc2 += f.affine ? g2 - gp : g2;
       tile[q * kTileLd + c] = gp;
     }
   }
@@ -743,7 +749,7 @@ __global__ __launch_bounds__(256) void k_operand_from_G(FinalizeG f) {
       const double gp = f.d[r] * f.G[(uint64_t)r * f.l + c];
       const double g2 = 2.0 * f.F[r] * gp;
       store_g2(f, (uint64_t)r * f.l + c, g2);
-      acc2 += g2;
+      acc2 += f.affine ? g2 - gp : g2;
       tile[q * kTileLd + c] = gp;
     }
   }
@@ -798,6 +804,9 @@ __global__ void k_finalize_H(FinalizeH f) {
       raw += load_part(f.part, f.part_is_f64, base + c);
       if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
     }
+    // the contraction ran on 2 - 2 y: X^T g' = sum g' - part / 2, and S2 already
+    // holds sum (g'' - g') for this layout
+    if (f.affine) raw = -0.5 * raw;
     const double corr = f.corr ? f.corr[((uint64_t)w * f.nsamples + j) * f.l + c] : 0.0;
     double corr_e = 0.0;
     if (f.mval) {
