@@ -35,9 +35,10 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of k_packed_gemm_bf16 on this workload: rocprofv3
-# FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0) per launch over one split-BF16 solve,
-# the factors from a 512 MiB copy kernel (profiles/k_packed_gemm_bf16_c2_winsvd.md)
-TRAFFIC = 352.6e6
+# FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0) averaged over the 280 launches of one
+# split-BF16 solve, the factors from a 512 MiB copy kernel
+# (profiles/k_packed_gemm_bf16_c2_winsvd.md)
+TRAFFIC = 313.6e6
 
 
 def log(*a):
@@ -148,7 +149,34 @@ def main():
         arms[name] = dict(times=ts, U=res[0], S=res[1], V=res[2], epochs=res[3])
         log(f"{name}: {np.mean(ts) * 1e3:.1f} ms/solve over {steps} solves, {res[3]} epochs")
 
+    roofline = None
+    if "bf16x2" in want:
+        # roofline of the dominant kernel, timed with HIP events on the handle's stream
+        # (one extra split-BF16 solve, missing-entry gathers serialized so every
+        # kernel is timed alone)
+        g.profiling(True)
+        g.reset_stats()
+        tprof, _, _, _, ep_prof = solve(g, pa, wl, pa.PREC_SPLIT_BF16)
+        stats = g.kernel_stats()
+        g.profiling(False)
+        # the dominant kernel is the split-BF16 contraction k_packed_gemm_bf16, launched
+        # for both products (gemm_G: X^T Omega per window segment, gemm_H: X G)
+        cnt = stats["gemm_G"][0] + stats["gemm_H"][0]
+        ms = stats["gemm_G"][1] + stats["gemm_H"][1]
+        flops = ep_prof * 4.0 * m * n * l  # algorithmic: two 2*M*N*l products per epoch
+        achieved = flops / (ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": TRAFFIC,
+                    "kernel": "k_packed_gemm_bf16 (gemm_G + gemm_H)",
+                    "launches": cnt, "avg_launch_us": round(ms * 1e3 / cnt, 2),
+                    "algorithmic_flops_per_launch": flops / cnt,
+                    "kernel_ms_per_solve": {k: round(v[1], 3) for k, v in sorted(stats.items())},
+                    "kernel_launches_per_solve": {k: v[0] for k, v in sorted(stats.items())}}
+
     if want != {"bf16x2", "fp32", "fp64"}:
+        if roofline:
+            log(f"timing solve: contraction {roofline['launches']} launches, avg {roofline['avg_launch_us']} us, "
+                f"{roofline['achieved']} TFLOP/s")
         log("partial run (--arms): no JSON line")
         return
     ref = arms["fp64"]
@@ -160,25 +188,6 @@ def main():
         acc[name] = {"eig_relerr_max": float(np.max(np.abs(ev - ev0) / ev0)),
                      f"max_angle_top{npc}_rad": principal_angle(a["U"][:, :npc], ref["U"][:, :npc]),
                      "epochs": a["epochs"]}
-    # roofline of the dominant kernel, timed with HIP events on the handle's stream
-    g.profiling(True)
-    g.reset_stats()
-    tprof, _, _, _, ep_prof = solve(g, pa, wl, pa.PREC_SPLIT_BF16)
-    stats = g.kernel_stats()
-    g.profiling(False)
-    # the dominant kernel is the split-BF16 contraction k_packed_gemm_bf16, launched
-    # for both products (gemm_G: X^T Omega per window segment, gemm_H: X G)
-    cnt = stats["gemm_G"][0] + stats["gemm_H"][0]
-    ms = stats["gemm_G"][1] + stats["gemm_H"][1]
-    flops = ep_prof * 4.0 * m * n * l  # algorithmic: two 2*M*N*l products per epoch
-    achieved = flops / (ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": TRAFFIC,
-                "kernel": "k_packed_gemm_bf16 (gemm_G + gemm_H)",
-                "launches": cnt, "avg_launch_us": round(ms * 1e3 / cnt, 2),
-                "algorithmic_flops_per_launch": flops / cnt,
-                "kernel_ms_per_solve": {k: round(v[1], 3) for k, v in sorted(stats.items())},
-                "kernel_launches_per_solve": {k: v[0] for k, v in sorted(stats.items())}}
 
     bf = arms["bf16x2"]
     t_bf = float(np.sum(bf["times"]))
