@@ -611,35 +611,48 @@ __device__ inline double load_part(const void* p, int is64, uint64_t i) {
   return is64 ? reinterpret_cast<const double*>(p)[i] : (double)reinterpret_cast<const float*>(p)[i];
 }
 
-// Grid (row chunk of 64, window); thread (c = t % 64, rs = t / 64) handles rows
-// rs, rs + 4, ... of the chunk for column c.  Besides G, g'' and the H operand
-// it writes the chunk's partial column sums of g'' (s2p), which
+// Grid (row chunk of 64, window).  Besides G, g'' and the H operand the
+// epilogue writes the chunk's partial column sums of g'' (s2p), which
 // k_window_colsum reduces into S''.  The operand (column-major per window) is
 // staged through LDS so its stores run along the window's SNPs.
 constexpr uint32_t kFinRows = 64;
 constexpr uint32_t kTileLd = 65;
 
-// bh[w][c][k0 + q] <- tile[q][c] for q < nq, c < l (and the lo half for the
-// split arm); consecutive lanes take consecutive q.
+// bh[w][c][k0 + p] for the operand positions p < 64 of a chunk, from
+// tile[q][c] (q < nq; rows past nq are window padding and get 0).  Each item
+// is four consecutive positions of one column, stored as one 8-byte (BF16
+// halves), 16-byte (FP32) or 32-byte (FP64) write; with kperm, position p
+// holds row kperm16(p) (a 4 x 4 transpose inside each 16-position group).
 __device__ inline void store_operand_tile(const double* tile, uint32_t nq, uint32_t l, uint32_t w, uint32_t k0,
                                           int bprec, void* bh, uint32_t bcols, uint32_t lo_off, uint64_t kw,
                                           int kperm) {
-  for (uint32_t e = threadIdx.x; e < l * kFinRows; e += blockDim.x) {
-    const uint32_t c = e / kFinRows, q = e % kFinRows;
-    if (q >= nq) continue;
-    const double gp = tile[q * kTileLd + c];
-    const uint64_t kk = kperm ? kperm16(k0 + q) : k0 + q;  // k0 is a multiple of 64
-    const uint64_t o = ((uint64_t)w * bcols + c) * kw + kk;
+  for (uint32_t it = threadIdx.x; it < l * (kFinRows / 4); it += blockDim.x) {
+    const uint32_t c = it / (kFinRows / 4), p0 = 4 * (it % (kFinRows / 4));
+    double v[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const uint32_t q = kperm ? (uint32_t)kperm16(p0 + b) : p0 + b;
+      v[b] = q < nq ? tile[q * kTileLd + c] : 0.0;
+    }
+    const uint64_t o = ((uint64_t)w * bcols + c) * kw + k0 + p0;  // k0 is a multiple of 64
     if (bprec == PREC_FP64) {
-      reinterpret_cast<double*>(bh)[o] = gp;
+      *reinterpret_cast<double4*>(reinterpret_cast<double*>(bh) + o) = make_double4(v[0], v[1], v[2], v[3]);
     } else if (bprec == PREC_FP32) {
-      reinterpret_cast<float*>(bh)[o] = (float)gp;
+      *reinterpret_cast<float4*>(reinterpret_cast<float*>(bh) + o) =
+          make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
     } else {
-      const __bf16 hi = (__bf16)(float)gp;
-      const __bf16 lo = (__bf16)(float)(gp - (double)(float)hi);
-      __bf16* b = reinterpret_cast<__bf16*>(bh);
-      b[o] = hi;
-      b[((uint64_t)w * bcols + lo_off + c) * kw + kk] = lo;
+      __bf16 hi[4], lo[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        hi[b] = (__bf16)(float)v[b];
+        lo[b] = (__bf16)(float)(v[b] - (double)(float)hi[b]);
+      }
+      __bf16* bb = reinterpret_cast<__bf16*>(bh);
+      uint2 h2, l2;
+      __builtin_memcpy(&h2, hi, 8);
+      __builtin_memcpy(&l2, lo, 8);
+      *reinterpret_cast<uint2*>(bb + o) = h2;
+      *reinterpret_cast<uint2*>(bb + ((uint64_t)w * bcols + lo_off + c) * kw + k0 + p0) = l2;
     }
   }
 }
@@ -670,62 +683,123 @@ __device__ inline void store_g2(const FinalizeG& f, uint64_t i, double v) {
     reinterpret_cast<double*>(f.g2)[i] = v;
 }
 
-// 16 waves: thread (c = t % 64, rs = t / 64) takes rows rs, rs + 16, ... of the chunk
-__global__ __launch_bounds__(1024) void k_finalize_G(FinalizeG f) {
-  __shared__ double sh[16][64];
-  __shared__ double tile[kFinRows * kTileLd];
-  const uint32_t t = threadIdx.x, c = t & 63, rs = t >> 6;
+// 1024 threads over the chunk's rows x l columns in flat (row, column) order,
+// so every lane has an element and the partial, correction, G and g''
+// accesses run along rows; each thread's elements (at most 4 for l <= 64)
+// have their loads in flight together, four split-K partials at a time.
+constexpr int kFinThreads = 1024;
+constexpr int kFinEl = kFinRows * 64 / kFinThreads;
+
+template <typename P>
+__global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
+  __shared__ double tile[kFinRows * kTileLd];  // g' = d G, the H operand
+  __shared__ double acct[kFinRows * kTileLd];  // the column-sum term (g'', or g'' - g' when affine)
+  __shared__ double sh[kFinThreads / 64][64];
+  const uint32_t t = threadIdx.x, l = f.l;
   const uint32_t w = f.w0 + blockIdx.y, chunk = blockIdx.x;
   const uint32_t wstart = w * f.blocksize;
   const uint32_t wend = min(wstart + f.blocksize, f.nsnps);
   const uint32_t cb = wstart + chunk * kFinRows;
   const uint32_t nq = cb < wend ? min(kFinRows, wend - cb) : 0;
+  const uint32_t n = nq * l;
   const uint64_t nrow = f.r1 - f.r0;
-  double acc2 = 0.0;
-  if (c < f.l) {
-    for (uint32_t q = rs; q < nq; q += 16) {
-      const uint32_t r = cb + q;
-      const uint32_t rr = r - f.r0;
-      double raw = 0.0;
-      for (uint32_t s = 0; s < f.nsplit; ++s) {
-        const uint64_t base = ((uint64_t)s * nrow + rr) * f.ldp;
-        raw += load_part(f.part, f.part_is_f64, base + c);
-        if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
-      }
-      if (f.affine) raw = f.colsum[c] - 0.5 * raw;  // the contraction ran on 2 - 2 y
-      const double corr = f.corr ? f.corr[(uint64_t)r * f.l + c] : 0.0;
-      double corr_e = 0.0;
-      if (f.mval) {
-        double unused = 0.0;
-        miss_sums(f.miss_ptr, f.miss_idx, r, f.omega, f.l, c, f.mval, unused, corr_e);
-      }
-      const double Fi = f.F[r], di = f.d[r];
-      const double Gv = di * (f.hf * (raw - 2.0 * Fi * (f.colsum[c] - corr)) + corr_e);
-      f.G[(uint64_t)r * f.l + c] = Gv;
-      const double gp = di * Gv;
-      const double g2 = 2.0 * Fi * gp;
-      store_g2(f, (uint64_t)r * f.l + c, g2);
-      acc2 += f.affine ? g2 - gp : g2;
-      tile[q * kTileLd + c] = gp;
-    }
+  const P* __restrict__ part = reinterpret_cast<const P*>(f.part);
+  const double* __restrict__ colsum = f.colsum;
+  const double* __restrict__ corrp = f.corr;
+  const double* __restrict__ F = f.F;
+  const double* __restrict__ d = f.d;
+  const uint64_t sstride = nrow * f.ldp;
+  // Element offsets inside one split's partial (32 bits: nrow * ldp < 2^32 is
+  // checked at launch).  Every load is unconditional, from a row and split
+  // clamped into range, and its value is dropped afterwards when out of range:
+  // the loads then issue together instead of one branch at a time.
+  uint32_t qe[kFinEl], ce[kFinEl], off[kFinEl];
+  double raw[kFinEl], cs[kFinEl], corr[kFinEl], Fi[kFinEl], di[kFinEl];
+  const uint32_t rlast = f.r1 - 1;
+#pragma unroll
+  for (int u = 0; u < kFinEl; ++u) {
+    const uint32_t e = t + kFinThreads * u;
+    const bool ok = e < n;
+    qe[u] = ok ? e / l : 0;
+    ce[u] = ok ? e - qe[u] * l : 0;
+    const uint32_t r = min(cb + qe[u], rlast);
+    off[u] = (r - f.r0) * (uint32_t)f.ldp + ce[u];
+    raw[u] = 0.0;
+    cs[u] = colsum[ce[u]];
+    Fi[u] = F[r];
+    di[u] = d[r];
   }
-  sh[rs][c] = acc2;
+  if (corrp) {
+#pragma unroll
+    for (int u = 0; u < kFinEl; ++u) corr[u] = corrp[(uint64_t)min(cb + qe[u], rlast) * l + ce[u]];
+  } else {
+#pragma unroll
+    for (int u = 0; u < kFinEl; ++u) corr[u] = 0.0;
+  }
+  const uint32_t lo = f.split_cols ? l : 0;
+  for (uint32_t s0 = 0; s0 < f.nsplit; s0 += 4) {
+    P x[4][kFinEl][2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const P* __restrict__ ps = part + (uint64_t)min(s0 + j, f.nsplit - 1) * sstride;
+#pragma unroll
+      for (int u = 0; u < kFinEl; ++u) {
+        x[j][u][0] = ps[off[u]];
+        x[j][u][1] = ps[off[u] + lo];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (s0 + j < f.nsplit)
+#pragma unroll
+        for (int u = 0; u < kFinEl; ++u) raw[u] += (double)x[j][u][0] + (lo ? (double)x[j][u][1] : 0.0);
+  }
+#pragma unroll
+  for (int u = 0; u < kFinEl; ++u) {
+    if (t + kFinThreads * u >= n) continue;
+    const uint32_t q = qe[u], c = ce[u], r = cb + q;
+    const double rv = f.affine ? cs[u] - 0.5 * raw[u] : raw[u];  // the contraction ran on 2 - 2 y
+    double corr_e = 0.0;
+    if (f.mval) {
+      double unused = 0.0;
+      miss_sums(f.miss_ptr, f.miss_idx, r, f.omega, l, c, f.mval, unused, corr_e);
+    }
+    const double Gv = di[u] * (f.hf * (rv - 2.0 * Fi[u] * (cs[u] - corr[u])) + corr_e);
+    f.G[(uint64_t)r * l + c] = Gv;
+    const double gp = di[u] * Gv;
+    const double g2 = 2.0 * Fi[u] * gp;
+    store_g2(f, (uint64_t)r * l + c, g2);
+    tile[q * kTileLd + c] = gp;
+    acct[q * kTileLd + c] = f.affine ? g2 - gp : g2;
+  }
   __syncthreads();
-  if (t < f.l) {
+  {  // column sums of the chunk: 16 row slices, then the slices in a fixed order
+    const uint32_t c = t & 63, sl = t >> 6;
+    double a = 0.0;
+    if (c < l)
+      for (uint32_t q = sl; q < nq; q += kFinThreads / 64) a += acct[q * kTileLd + c];
+    sh[sl][c] = a;
+  }
+  __syncthreads();
+  if (t < l) {
     double v[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = sh[2 * q][t] + sh[2 * q + 1][t];
-    f.s2p[((uint64_t)w * gridDim.x + chunk) * f.l + t] =
-        ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    for (int i = 0; i < 8; ++i) v[i] = sh[2 * i][t] + sh[2 * i + 1][t];
+    f.s2p[((uint64_t)w * gridDim.x + chunk) * l + t] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
   }
-  store_operand_tile(tile, nq, f.l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
+  store_operand_tile(tile, nq, l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
 }
 
 void launch_finalize_G(const FinalizeG& f, hipStream_t s) {
   PCAONE_REQUIRE(f.l >= 1 && f.l <= 64, -1, "finalize: l must be in [1, 64]");
+  PCAONE_REQUIRE(f.kw % kFinRows == 0, -1, "finalize: window operand rows must be whole chunks");
+  PCAONE_REQUIRE((uint64_t)(f.r1 - f.r0) * f.ldp < (1ull << 32), -1, "finalize: partial block too large");
   if (f.w1 <= f.w0) return;
   dim3 grid((uint32_t)ceil_div(f.blocksize, kFinRows), f.w1 - f.w0);
-  hipLaunchKernelGGL(k_finalize_G, grid, dim3(1024), 0, s, f);
+  if (f.part_is_f64)
+    hipLaunchKernelGGL(k_finalize_G<double>, grid, dim3(kFinThreads), 0, s, f);
+  else
+    hipLaunchKernelGGL(k_finalize_G<float>, grid, dim3(kFinThreads), 0, s, f);
 }
 
 uint32_t finalize_chunks(uint32_t blocksize) { return (uint32_t)ceil_div(blocksize, kFinRows); }
@@ -767,25 +841,40 @@ void launch_operand_from_G(const FinalizeG& f, hipStream_t s) {
   hipLaunchKernelGGL(k_operand_from_G, grid, dim3(256), 0, s, f);
 }
 
-// S2[w][c] = sum of the chunk partials of window w: four slices of chunks,
-// combined in a fixed order
-__global__ __launch_bounds__(256) void k_window_colsum(const double* __restrict__ s2p, uint32_t l, uint32_t nchunk,
-                                                       uint32_t w0, double* __restrict__ S2) {
-  __shared__ double sh[4][64];
-  const uint32_t w = w0 + blockIdx.x, c = threadIdx.x & 63, q0 = threadIdx.x >> 6;
+// S2[w][c] = sum of the chunk partials of window w: 16 slices of chunks (all
+// of a slice's loads in flight), combined in a fixed order
+__global__ __launch_bounds__(1024) void k_window_colsum(const double* __restrict__ s2p, uint32_t l, uint32_t nchunk,
+                                                        uint32_t w0, double* __restrict__ S2) {
+  constexpr int kSlices = 16, kPer = 8;
+  __shared__ double sh[kSlices][64];
+  const uint32_t w = w0 + blockIdx.x, c = threadIdx.x & 63, sl = threadIdx.x >> 6;
   double tot = 0.0;
   if (c < l)
-    for (uint32_t q = q0; q < nchunk; q += 4) tot += s2p[((uint64_t)w * nchunk + q) * l + c];
-  sh[q0][c] = tot;
+    for (uint32_t q0 = sl; q0 < nchunk; q0 += kSlices * kPer) {
+      double v[kPer];
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) {
+        const uint32_t q = q0 + kSlices * i;
+        v[i] = q < nchunk ? s2p[((uint64_t)w * nchunk + q) * l + c] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) tot += v[i];
+    }
+  sh[sl][c] = tot;
   __syncthreads();
-  if (threadIdx.x < l) S2[(uint64_t)w * l + c] = (sh[0][c] + sh[1][c]) + (sh[2][c] + sh[3][c]);
+  if (threadIdx.x < l) {
+    double v[kSlices / 2];
+#pragma unroll
+    for (int i = 0; i < kSlices / 2; ++i) v[i] = sh[2 * i][c] + sh[2 * i + 1][c];
+    S2[(uint64_t)w * l + c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+  }
 }
 
 void launch_window_colsum(const double* s2p, uint32_t l, uint32_t blocksize, uint32_t w0, uint32_t w1, double* S2,
                           hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= 64, -1, "l out of range");
   if (w1 <= w0) return;
-  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0), dim3(256), 0, s, s2p, l, finalize_chunks(blocksize), w0, S2);
+  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0), dim3(1024), 0, s, s2p, l, finalize_chunks(blocksize), w0, S2);
 }
 
 // thread (sample j, column c); the windows' contributions are summed in

@@ -157,6 +157,15 @@ __device__ int chol_inverse(double* a, double* E, double* isd, uint32_t l, doubl
   return bad;
 }
 
+// 1 / d for a normal d: the hardware reciprocal estimate (about 23 bits) and
+// two Newton steps r <- r + r (1 - d r), which leave the error at FP64
+// rounding; a short dependent chain where the IEEE division sequence is long.
+__device__ inline double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
 __device__ inline double bcast(double v, int lane) {  // v of `lane` (uniform) in every lane
   const int2 w = __builtin_bit_cast(int2, v);
   return __builtin_bit_cast(double, int2{__builtin_amdgcn_readlane(w.x, lane), __builtin_amdgcn_readlane(w.y, lane)});
@@ -176,14 +185,14 @@ __device__ inline double bcast(double v, int lane) {  // v of `lane` (uniform) i
 // triangular, so row K of E is zero past column K.
 template <int V, int K>
 __device__ __attribute__((always_inline)) inline void chol2d_step(double (&a)[V][V], double (&e)[V][V], int p, int q,
-                                                                  double* rowA, double* rowE, double* isd, int& bad) {
+                                                                  double* rowA, double* rowE, int& bad) {
   if constexpr (K < 8 * V) {
     constexpr int uk = K / 8, pk = K % 8;
     double d = bcast(a[uk][uk], 9 * pk);  // lane (pk, pk) holds A[K][K]
     const bool ok = d > 0.0 && isfinite(d);
     bad = (bad == 0 && !ok) ? K + 1 : bad;
     d = ok ? d : 1e-300;
-    const double inv = 1.0 / d;
+    const double inv = rcp_nr(d);
     if (p == pk) {
 #pragma unroll
       for (int v = 0; v < V; ++v) {
@@ -191,7 +200,6 @@ __device__ __attribute__((always_inline)) inline void chol2d_step(double (&a)[V]
         rowE[8 * v + q] = e[uk][v];
       }
     }
-    if (p == 0 && q == 0) isd[K] = 1.0 / sqrt(d);
     __builtin_amdgcn_wave_barrier();
     double ak[V], ek[V], m[V];
 #pragma unroll
@@ -215,7 +223,7 @@ __device__ __attribute__((always_inline)) inline void chol2d_step(double (&a)[V]
     for (int u = 0; u < V; ++u)
 #pragma unroll
       for (int v = 0; v < V; ++v) asm volatile("" : "+v"(a[u][v]), "+v"(e[u][v]));
-    chol2d_step<V, K + 1>(a, e, p, q, rowA, rowE, isd, bad);
+    chol2d_step<V, K + 1>(a, e, p, q, rowA, rowE, bad);
   }
 }
 
@@ -242,7 +250,15 @@ __device__ __attribute__((always_inline)) inline int chol_inverse_2d(const doubl
   double* rowE = scratch + 8 * V;
   double* isd = scratch + 16 * V;
   int bad = 0;
-  chol2d_step<V, 0>(a, e, p, q, rowA, rowE, isd, bad);
+  chol2d_step<V, 0>(a, e, p, q, rowA, rowE, bad);
+  // Row K is final after step K - 1, so the diagonal now holds every pivot:
+  // lane (p, p) takes 1 / sqrt of pivots 8 u + p.
+  if (p == q)
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const double d = a[u][u];
+      isd[8 * u + p] = 1.0 / sqrt((d > 0.0 && isfinite(d)) ? d : 1e-300);
+    }
   __builtin_amdgcn_wave_barrier();
   for (int t = lane; t < W * W; t += 64) rinv[t] = 0.0;
   __builtin_amdgcn_wave_barrier();
@@ -267,7 +283,7 @@ __device__ __attribute__((always_inline)) inline void signs2d_step(double (&b)[V
     constexpr int kb = K / 8, kc = K % 8;
     const double bkk = bcast(b[kb][kb], 9 * kc);
     const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
-    const double ip = 1.0 / (bkk - sk);
+    const double ip = rcp_nr(bkk - sk);
     if (p == 0 && q == 0 && K < L) sign[K] = sk;
     if (q == kc)
 #pragma unroll
