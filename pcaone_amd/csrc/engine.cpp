@@ -76,23 +76,15 @@ Engine::Engine(int device) : device_(device) {
   PCAONE_REQUIRE(device >= 0 && device < n, -1, "no such HIP device");
   PCAONE_HIP(hipSetDevice(device));
   PCAONE_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  PCAONE_HIP(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
-  for (auto& e : ev_) PCAONE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  if (aux_) (void)hipStreamSynchronize(aux_);
   timer.resolve();
-  for (auto& e : ev_)
-    if (e) (void)hipEventDestroy(e);
-  if (aux_) (void)hipStreamDestroy(aux_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-// The main stream waits for every aux-stream gather it consumes, so draining
-// it drains both.
 void Engine::sync() { PCAONE_HIP(hipStreamSynchronize(stream_)); }
 
 // ---------------------------------------------------------------------------
@@ -287,6 +279,7 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   // linear algebra scratch
   part_.reserve(std::max<uint64_t>({(uint64_t)1024 * l_ * l_, qr_part_doubles(l_), usv_part_doubles(l_)}));
   gred_.reserve(usv_gram_doubles(l_));
+  qred_.reserve(usv_gram_doubles(l_));  // also holds the 3 x 16 ceil(l / 16) flip statistics
   bbt_.reserve((uint64_t)l_ * l_);
   for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &Rf_, &W_}) b->reserve((uint64_t)l_ * l_);
   sign_.reserve(l_);
@@ -659,6 +652,7 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
   q.n = N_;
   q.l = l_;
   q.part = part_.p;
+  q.gram = qred_.p;
   q.status = status_.p;
   q.R1i = R1i_.p;
   q.R2i = R2i_.p;
@@ -743,14 +737,10 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
     cg.per = rows;
     cg.ngroups = 1;
     cg.out = b.corrG;
-    // Omega (and everything before it) is ready on the main stream.  With
-    // profiling on the gathers stay on the main stream, so every kernel is
-    // timed alone.
-    hipStream_t gs = timer.on ? stream_ : aux_;
-    PCAONE_HIP(hipEventRecord(ev_[0], stream_));
-    PCAONE_HIP(hipStreamWaitEvent(gs, ev_[0], 0));
-    timed_on(gs, "miss_gather", [&] { launch_csr_gather(cg, gs); });
-    PCAONE_HIP(hipEventRecord(ev_[1], gs));
+    // In line before the contraction: run beside it on a second stream, the
+    // gather's random table reads evict the contraction's operand tiles from
+    // L2 and cost it more time than the gather takes alone.
+    timed("miss_gather", [&] { launch_csr_gather(cg, stream_); });
   }
   if (!dense) {
     PackedGemm g{};
@@ -788,7 +778,6 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
     g_nsplit = g.nsplit;
     timed("gemm_G", [&] { launch_dense_gemm(g, prec, (int)b.l, b.ntiles, stream_); });
   }
-  if (b.corrG) PCAONE_HIP(hipStreamWaitEvent(stream_, ev_[1], 0));
   FinalizeG fg = finalize_desc(w0, w1, b);
   fg.part = b.gpart;
   fg.nsplit = g_nsplit;
@@ -851,12 +840,7 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
     cg.per = N_;
     cg.ngroups = nz;
     cg.out = b.corrH;
-    // g'' of these windows is ready on the main stream
-    hipStream_t gs = timer.on ? stream_ : aux_;
-    PCAONE_HIP(hipEventRecord(ev_[2], stream_));
-    PCAONE_HIP(hipStreamWaitEvent(gs, ev_[2], 0));
-    timed_on(gs, "miss_gather", [&] { launch_csr_gather(cg, gs); });
-    PCAONE_HIP(hipEventRecord(ev_[3], gs));
+    timed("miss_gather", [&] { launch_csr_gather(cg, stream_); });
   }
   if (!dense) {
     PackedGemm h{};
@@ -900,7 +884,6 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
     h_nsplit = h.nsplit;
     timed("gemm_H", [&] { launch_dense_gemm(h, prec, (int)b.l, b.ntiles, stream_); });
   }
-  if (b.corrH) PCAONE_HIP(hipStreamWaitEvent(stream_, ev_[3], 0));
   FinalizeH fh{};
   fh.part = b.hpart;
   fh.part_is_f64 = b.part_f64;

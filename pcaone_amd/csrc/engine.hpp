@@ -147,11 +147,7 @@ class Engine {
   void gl_fill_E(const double* U, const double* S, const double* V, uint32_t k, bool standardize, double* Dc);
 
   int device_;
-  hipStream_t stream_ = nullptr;
-  // second stream for the missing-entry gathers, which overlap the
-  // contractions; both at the default priority
-  hipStream_t aux_ = nullptr;
-  hipEvent_t ev_[4] = {};
+  hipStream_t stream_ = nullptr;  // every launch, in order, at the default priority
 
   // ---- data --------------------------------------------------------------
   enum Source { SRC_NONE, SRC_BED, SRC_DENSE };
@@ -214,6 +210,7 @@ class Engine {
   // linear algebra scratch
   DevBuf<double> part_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, Rf_, sign_;
   DevBuf<double> gred_, bbt_;  // computeUSV: reduced Gram between passes, B B^T
+  DevBuf<double> qred_;        // Omega update: reduced partials between passes
   DevBuf<double> Bt_, evals_, W_, sinv_, Ucur_, Upre_, V_, scal_;
   DevBuf<double> out_;  // column-major U / V staged for the copy to the host
   DevBuf<int> status_;

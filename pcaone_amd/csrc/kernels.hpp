@@ -266,6 +266,7 @@ struct QrPass {
   uint64_t n;        // rows (samples), n >= 2 l
   uint32_t l;        // columns, 1..64
   double* part;      // qr_part_doubles(l) scratch, kept between the four passes
+  double* gram;      // usv_gram_doubles(l): the reduced partials handed to the next pass
   int nparts;        // blocks of the reduction passes (set by launch_qr_pass)
   int* status;       // [2]: first failing pivot of the two factorisations (sticky, 0 = none)
   double* R1i;       // l x l outputs: R1^-1, R2^-1, Householder signs, R2^-1 diag(sign flip)

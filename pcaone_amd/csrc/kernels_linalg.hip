@@ -140,10 +140,14 @@ void launch_rows_to_cols(const double* A, uint64_t n, uint32_t kc, const int32_t
 }
 
 // Cyclic Jacobi with round-robin pair ordering: each round applies n/2
-// disjoint rotations at once (Numerical-Recipes angle).  A thread owns a 2 x 2
-// block {p1, q1} x {p2, q2} of the matrix (rows, then columns rotated in
-// place) or a row pair of the eigenvector matrix, so a round needs two
-// barriers.  LDS: a and v, l x l each.
+// disjoint rotations at once (Numerical-Recipes angle).  A thread owns 2 x 2
+// blocks {p1, q1} x {p2, q2} of the matrix (rows, then columns rotated in
+// place) or row pairs of the eigenvector matrix; a round needs two barriers,
+// and each thread's items load everything they read before the first store.
+// LDS: a, v and tmp, W x W each (W = 16 NB, zero padded past l, so an odd
+// l's dummy index l is an empty row and column and its rotation the identity)
+// with the odd row stride W + 1, so a wave's walk down a column (the
+// eigenvector items) or along a row touches distinct banks.
 __device__ inline void rr_pair(int t, int round, int n, int& p, int& q) {
   // position k of the round-robin table: 0 stays, the rest rotate
   auto pos = [&](int k) { return k == 0 ? 0 : 1 + (k - 1 + round) % (n - 1); };
@@ -156,69 +160,100 @@ __device__ inline void rr_pair(int t, int round, int n, int& p, int& q) {
   }
 }
 
+// 1 / x and 1 / sqrt(x) for a normal x > 0: the hardware estimates (about 23
+// bits) and two Newton steps each, to FP64 rounding.
+__device__ inline double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ inline double rsqrt_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  r = r * fma(-hx * r, r, 1.5);
+  return r * fma(-hx * r, r, 1.5);
+}
+
+// C = X^T Y for W x W LDS matrices (W = 16 NB, row stride LD) on FP64 MFMA:
+// wave w takes the output tiles w, w + 8, ...
+template <int NB, int LD>
+__device__ inline void xty_lds(const double* X, const double* Y, double* C) {
+  constexpr int W = 16 * NB;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kr = lane >> 4, ci = lane & 15;
+  for (int tt = wv; tt < NB * NB; tt += (int)(blockDim.x >> 6)) {
+    const int ti = tt / NB, tj = tt % NB;
+    f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < W / 4; ++ks)
+      acc = mfma64(X[(4 * ks + kr) * LD + 16 * ti + ci], Y[(4 * ks + kr) * LD + 16 * tj + ci], acc);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) C[(16 * ti + kr + 4 * v) * LD + 16 * tj + ci] = acc[v];
+  }
+}
+
+constexpr int kEigThreads = 512;
+
 // W0 (optional): a starting basis (the previous eigenvectors); the sweeps then
 // run on W0^T A W0 and the rotations accumulate onto W0.  W may alias W0.
-__global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part, int nparts, uint32_t l,
-                                                 double* evals, double* W, double* sinv, uint32_t kc,
-                                                 const double* W0) {
+template <int NB>
+__global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restrict__ part, int nparts, uint32_t l,
+                                                         double* evals, double* W, double* sinv, uint32_t kc,
+                                                         const double* W0) {
+  constexpr int WD = 16 * NB, LD = WD + 1;
+  // items per thread and round: h^2 blocks of a plus h l row pairs of v, h <= 8 NB
+  constexpr int U = (192 * NB * NB + kEigThreads - 1) / kEigThreads;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int L = (int)l, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double* a = reinterpret_cast<double*>(smem_raw);
-  double* v = a + L * L;
-  double* tmp = v + L * L;
-  __shared__ double cs[kMaxL / 2 + 1][2];
-  __shared__ int pr[kMaxL / 2 + 1][2];
-  __shared__ double red[4][2];
-  __shared__ int order[kMaxL];
-  const int n = (L % 2) ? L + 1 : L;  // dummy index L when odd
+  double* v = a + WD * LD;
+  double* tmp = v + WD * LD;
+  __shared__ double cs[8 * NB][2];
+  __shared__ int pr[8 * NB][2];
+  __shared__ double red[kEigThreads / 64][2];
+  __shared__ int order[WD];
+  const int n = (L % 2) ? L + 1 : L;  // dummy index L when odd (a zero row of the padding)
   const int h = n / 2;
-  // this thread's fixed work items: rotation blocks (k1, k2) of a, then
-  // (pair, row) items of v, packed as hi << 8 | lo
-  int asg[12];
+  // this thread's fixed items: rotation blocks (k1, k2) of a, then (pair,
+  // row) items of v, packed as kind << 16 | hi << 8 | lo (kind 0: none)
+  int asg[U];
 #pragma unroll
-  for (int u = 0; u < 12; ++u) {
-    const int e = t + 256 * u;
+  for (int u = 0; u < U; ++u) {
+    const int e = t + kEigThreads * u;
     if (e < h * h)
-      asg[u] = ((e / h) << 8) | (e % h);
+      asg[u] = (1 << 16) | ((e / h) << 8) | (e % h);
     else if (e < h * h + h * L)
-      asg[u] = (((e - h * h) / L) << 8) | ((e - h * h) % L);
+      asg[u] = (2 << 16) | (((e - h * h) / L) << 8) | ((e - h * h) % L);
     else
       asg[u] = 0;
   }
-  for (int e = t; e < L * L; e += 256) {
+  for (int e = t; e < WD * WD; e += kEigThreads) {
+    const int i = e / WD, j = e % WD;
+    const bool in = i < L && j < L;
     double s = 0.0;
-    for (int p = 0; p < nparts; ++p) s += part[(uint64_t)p * L * L + e];
-    a[e] = s;
-    v[e] = W0 ? W0[e] : ((e / L == e % L) ? 1.0 : 0.0);
+    if (in)
+      for (int p = 0; p < nparts; ++p) s += part[(uint64_t)p * L * L + i * L + j];
+    a[i * LD + j] = s;
+    v[i * LD + j] = in ? (W0 ? W0[i * L + j] : (i == j ? 1.0 : 0.0)) : 0.0;
   }
   __syncthreads();
   if (W0) {
-    // a <- W0^T (a W0)
-    for (int e = t; e < L * L; e += 256) {
-      const int i = e / L, j = e % L;
-      double s = 0.0;
-      for (int p = 0; p < L; ++p) s = fma(a[i * L + p], v[p * L + j], s);
-      tmp[e] = s;
-    }
+    xty_lds<NB, LD>(a, v, tmp);  // a W0 (a is symmetric)
     __syncthreads();
-    for (int e = t; e < L * L; e += 256) {
-      const int i = e / L, j = e % L;
-      double s = 0.0;
-      for (int p = 0; p < L; ++p) s = fma(v[p * L + i], tmp[p * L + j], s);
-      a[e] = s;
-    }
+    xty_lds<NB, LD>(v, tmp, a);  // W0^T a W0
     __syncthreads();
-    for (int e = t; e < L * L; e += 256) {  // exact symmetry for the rotations
-      const int i = e / L, j = e % L;
-      if (i > j) a[e] = a[j * L + i];
+    for (int e = t; e < WD * WD; e += kEigThreads) {  // exact symmetry for the rotations
+      const int i = e / WD, j = e % WD;
+      if (i > j) a[i * LD + j] = a[j * LD + i];
     }
     __syncthreads();
   }
   for (int sweep = 0; sweep < 40; ++sweep) {
     double off = 0.0, dg = 0.0;
-    for (int e = t; e < L * L; e += 256) {
-      const double x = a[e] * a[e];
-      if (e / L == e % L) dg += x; else off += x;
+    for (int e = t; e < WD * WD; e += kEigThreads) {
+      const int i = e / WD, j = e % WD;
+      const double x = a[i * LD + j] * a[i * LD + j];
+      if (i == j) dg += x; else off += x;
     }
     for (int o = 32; o > 0; o >>= 1) {
       off += __shfl_down(off, o);
@@ -229,8 +264,12 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
       red[wv][1] = dg;
     }
     __syncthreads();
-    const double offs = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
-    const double dgs = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+    double offs = 0.0, dgs = 0.0;
+#pragma unroll
+    for (int w = 0; w < kEigThreads / 64; ++w) {
+      offs += red[w][0];
+      dgs += red[w][1];
+    }
     // off-diagonal Frobenius norm below 1e-13 of the diagonal's: FP64 rounding level
     if (offs <= 1e-26 * dgs || offs == 0.0) break;
     for (int round = 0; round < n - 1; ++round) {
@@ -238,15 +277,16 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
         int p, q;
         rr_pair(t, round, n, p, q);
         double c = 1.0, s = 0.0;
-        if (q < L) {
-          const double apq = a[p * L + q];
-          if (fabs(apq) > 1e-300) {
-            const double theta = (a[q * L + q] - a[p * L + p]) / (2.0 * apq);
-            double tt = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
-            if (theta < 0.0) tt = -tt;
-            c = 1.0 / sqrt(tt * tt + 1.0);
-            s = tt * c;
-          }
+        const double apq = a[p * LD + q];
+        if (q < L && fabs(apq) > 1e-300) {
+          const double theta = (a[q * LD + q] - a[p * LD + p]) * rcp_nr(2.0 * apq);
+          const double th2 = fma(theta, theta, 1.0);
+          // sqrt(theta^2 + 1) as x rsqrt(x); past 1e150 the rotation is the identity
+          const double root = th2 < 1e300 ? th2 * rsqrt_nr(th2) : 1e300;
+          double tt = rcp_nr(fabs(theta) + root);
+          if (theta < 0.0) tt = -tt;
+          c = rsqrt_nr(fma(tt, tt, 1.0));
+          s = tt * c;
         }
         pr[t][0] = p;
         pr[t][1] = q;
@@ -254,36 +294,54 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
         cs[t][1] = s;
       }
       __syncthreads();  // also orders the convergence reads of red before its next write
+      int i0[U], i1[U], i2[U], i3[U];
+      double c1[U], s1[U], c2[U], s2[U];
 #pragma unroll
-      for (int u = 0; u < 12; ++u) {
-        const int e = t + 256 * u;
-        if (e >= h * h + h * L) break;
-        if (e < h * h) {
-          const int k1 = asg[u] >> 8, k2 = asg[u] & 255;
-          const int p1 = pr[k1][0], q1 = pr[k1][1], p2 = pr[k2][0], q2 = pr[k2][1];
-          const double c1 = cs[k1][0], s1 = cs[k1][1], c2 = cs[k2][0], s2 = cs[k2][1];
-          const bool r2 = q1 < L, col2 = q2 < L;
-          // rows: (p1, q1) <- (c x - s y, s x + c y)
-          const double x0 = a[p1 * L + p2], y0 = r2 ? a[q1 * L + p2] : 0.0;
-          const double x1 = col2 ? a[p1 * L + q2] : 0.0, y1 = (r2 && col2) ? a[q1 * L + q2] : 0.0;
-          const double rp0 = c1 * x0 - s1 * y0, rq0 = s1 * x0 + c1 * y0;
-          const double rp1 = c1 * x1 - s1 * y1, rq1 = s1 * x1 + c1 * y1;
-          // columns: (p2, q2)
-          a[p1 * L + p2] = c2 * rp0 - s2 * rp1;
-          if (col2) a[p1 * L + q2] = s2 * rp0 + c2 * rp1;
-          if (r2) {
-            a[q1 * L + p2] = c2 * rq0 - s2 * rq1;
-            if (col2) a[q1 * L + q2] = s2 * rq0 + c2 * rq1;
-          }
-        } else {
-          const int k = asg[u] >> 8, i = asg[u] & 255;
-          const int p = pr[k][0], q = pr[k][1];
-          if (q < L) {
-            const double c = cs[k][0], s = cs[k][1];
-            const double x = v[i * L + p], y = v[i * L + q];
-            v[i * L + p] = c * x - s * y;
-            v[i * L + q] = s * x + c * y;
-          }
+      for (int u = 0; u < U; ++u) {
+        const int kind = asg[u] >> 16, hi = (asg[u] >> 8) & 255, lo = asg[u] & 255;
+        const int k1 = hi, k2 = kind == 1 ? lo : 0;
+        const int p1 = pr[k1][0], q1 = pr[k1][1], p2 = pr[k2][0], q2 = pr[k2][1];
+        c1[u] = cs[k1][0];
+        s1[u] = cs[k1][1];
+        c2[u] = cs[k2][0];
+        s2[u] = cs[k2][1];
+        if (kind == 1) {  // a entries (p1, p2), (q1, p2), (p1, q2), (q1, q2)
+          i0[u] = p1 * LD + p2;
+          i1[u] = q1 * LD + p2;
+          i2[u] = p1 * LD + q2;
+          i3[u] = q1 * LD + q2;
+        } else {  // v entries (row, p1), (row, q1)
+          i0[u] = lo * LD + p1;
+          i1[u] = lo * LD + q1;
+          i2[u] = i0[u];
+          i3[u] = i1[u];
+        }
+      }
+      double x0[U], y0[U], x1[U], y1[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        x0[u] = y0[u] = x1[u] = y1[u] = 0.0;
+        if (asg[u] == 0) continue;  // whole waves past the items skip their loads
+        const double* m = (asg[u] >> 16) == 2 ? v : a;
+        x0[u] = m[i0[u]];
+        y0[u] = m[i1[u]];
+        x1[u] = m[i2[u]];
+        y1[u] = m[i3[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kind = asg[u] >> 16;
+        if (kind == 1) {
+          // rows: (p1, q1) <- (c x - s y, s x + c y), then columns (p2, q2)
+          const double rp0 = c1[u] * x0[u] - s1[u] * y0[u], rq0 = s1[u] * x0[u] + c1[u] * y0[u];
+          const double rp1 = c1[u] * x1[u] - s1[u] * y1[u], rq1 = s1[u] * x1[u] + c1[u] * y1[u];
+          a[i0[u]] = c2[u] * rp0 - s2[u] * rp1;
+          a[i2[u]] = s2[u] * rp0 + c2[u] * rp1;
+          a[i1[u]] = c2[u] * rq0 - s2[u] * rq1;
+          a[i3[u]] = s2[u] * rq0 + c2[u] * rq1;
+        } else if (kind == 2) {
+          v[i0[u]] = c1[u] * x0[u] - s1[u] * y0[u];
+          v[i1[u]] = s1[u] * x0[u] + c1[u] * y0[u];
         }
       }
       __syncthreads();
@@ -291,21 +349,21 @@ __global__ __launch_bounds__(256) void k_sym_eig(const double* __restrict__ part
   }
   // descending order of the eigenvalues (ties by index): rank of each diagonal entry
   if (t < L) {
-    const double di = a[t * L + t];
+    const double di = a[t * LD + t];
     int rank = 0;
     for (int j = 0; j < L; ++j) {
-      const double dj = a[j * L + j];
+      const double dj = a[j * LD + j];
       rank += (dj > di) || (dj == di && j < t);
     }
     order[rank] = t;
   }
   __syncthreads();
-  for (int e = t; e < L * L; e += 256) {
+  for (int e = t; e < L * L; e += kEigThreads) {
     const int i = e / L, j = e % L;
-    W[e] = v[i * L + order[j]];
+    W[e] = v[i * LD + order[j]];
   }
   if (t < L) {
-    const double ev = a[order[t] * L + order[t]];
+    const double ev = a[order[t] * LD + order[t]];
     evals[t] = ev;
     if (t < (int)kc) sinv[t] = 1.0 / sqrt(fmax(ev, 1e-300));
   }
@@ -315,11 +373,20 @@ void launch_sym_eig(const double* part, int nparts, uint32_t l, double* evals, d
                     uint32_t kc, const double* W0, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kMaxL, -1, "l must be in [1, 64]");
   nparts = reduce_parts(part, nparts, l * l, s);
-  const size_t shm = (size_t)3 * l * l * sizeof(double);
-  if (shm > 65536)
-    PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sym_eig),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-  hipLaunchKernelGGL(k_sym_eig, dim3(1), dim3(256), shm, s, part, nparts, l, evals, W, sinv, kc, W0);
+  const uint32_t nb = (l + 15) / 16;
+  const size_t shm = (size_t)3 * (16 * nb) * (16 * nb + 1) * sizeof(double);
+  switch (nb) {
+#define PCAONE_EIG(NB)                                                                                         \
+  case NB:                                                                                                     \
+    if (shm > 65536)                                                                                           \
+      PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_sym_eig<NB>),                             \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));                   \
+    hipLaunchKernelGGL(k_sym_eig<NB>, dim3(1), dim3(kEigThreads), shm, s, part, nparts, l, evals, W, sinv, kc, \
+                       W0);                                                                                    \
+    break;
+    PCAONE_EIG(1) PCAONE_EIG(2) PCAONE_EIG(3) PCAONE_EIG(4)
+#undef PCAONE_EIG
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -17,10 +17,11 @@
 //                FP32 table of the missing-entry gathers; the emptied H
 //                accumulator is zeroed.
 //
-// The tall N x l data is read once per pass.  Each prologue reduces the
-// previous pass's block partials in a fixed order (deterministic) and runs the
-// l x l step redundantly in every block (block 0 stores the results), so no
-// block ever waits for another inside a launch.
+// The tall N x l data is read once per pass.  A small launch after each of
+// the first three passes sums its block partials in a fixed order
+// (deterministic); each prologue reads that sum and runs the l x l step
+// redundantly in every block (block 0 stores the results), so no block ever
+// waits for another inside a launch.
 #include <algorithm>
 
 #include "common.hpp"
@@ -82,23 +83,11 @@ __device__ inline void store_gram(const f64x4 (&g)[NP], double* dst) {
       for (int v = 0; v < 4; ++v) dst[p * 256 + v * 64 + lane] = g[p][v];
 }
 
-// sum_b part[b * stride + e] for b < nparts (<= kMaxBlocks), in block order;
-// every load is issued before the first add.
-__device__ inline double sum_parts(const double* __restrict__ part, int nparts, uint64_t stride, uint64_t e) {
-  double v[kMaxBlocks];
-#pragma unroll
-  for (int b = 0; b < kMaxBlocks; ++b) v[b] = (b < nparts) ? part[(uint64_t)b * stride + e] : 0.0;
-  double s = 0.0;
-#pragma unroll
-  for (int b = 0; b < kMaxBlocks; ++b) s += v[b];
-  return s;
-}
-
-// The Gram of nparts block partials (tile p enumerates ca <= cb row by row;
-// element v * 64 + lane of a tile is D[lane / 16 + 4 v][lane % 16]) summed in
-// partial order into the upper triangle of a (W x W, row stride W).
+// A reduced Gram (tile p enumerates ca <= cb row by row; element v * 64 +
+// lane of a tile is D[lane / 16 + 4 v][lane % 16]) into the upper triangle of
+// a (W x W, row stride W).
 template <int NB>
-__device__ void gram_to_upper(const double* __restrict__ part, int nparts, uint32_t l, double* a) {
+__device__ void gram_to_upper(const double* __restrict__ gram, uint32_t l, double* a) {
   using S = Shape<NB>;
   for (int e = threadIdx.x; e < S::GRAM; e += blockDim.x) {
     const int p = e >> 8, v = (e >> 6) & 3, ln = e & 63;
@@ -110,7 +99,7 @@ __device__ void gram_to_upper(const double* __restrict__ part, int nparts, uint3
     const int cb = ca + rem;
     const uint32_t r = ca * 16 + (ln >> 4) + 4 * v, c = cb * 16 + (ln & 15);
     if (r >= l || c >= l || r > c) continue;
-    a[r * S::W + c] = sum_parts(part, nparts, S::GRAM, e);
+    a[r * S::W + c] = gram[e];
   }
 }
 
@@ -366,22 +355,53 @@ __device__ inline void store_compact(const double* src, uint32_t l, int W, doubl
   for (uint32_t e = threadIdx.x; e < l * l; e += blockDim.x) dst[e] = src[(e / l) * W + e % l];
 }
 
-// Omega^T of one 16-row tile in registers: the B operand of step ks is
-// H[r0 + i][4 ks + k]; Q1^T = R1^-T H^T, then Omega^T = Rb^T Q1^T, whose
-// D layout (lane (k, i): Q1[r0 + i][16 cb + k + 4 v]) is already the B operand
-// of the second product.  om[cb][v] = Omega[r0 + lane % 16][16 cb + lane / 16 + 4 v].
+// The 16-row tile `tile` of an n x l row-major matrix in MFMA operand
+// layout: lane (k, i) = (lane / 16, lane % 16) gets A[16 tile + i][4 ks + k] in
+// x[ks], zero outside the matrix.  The loads read clamped indices, so they are
+// unconditional and issue together.
 template <int NB>
-__device__ inline void tile_omega(const double* __restrict__ H, uint64_t n, uint32_t l, uint64_t r0,
-                                  const double* r1, const double* rb, f64x4 (&om)[NB]) {
+__device__ inline void load_tile(const double* __restrict__ A, uint64_t n, uint32_t l, uint64_t tile,
+                                 double (&x)[NB * 4]) {
+  const int lane = threadIdx.x & 63, kr = lane >> 4, ci = lane & 15;
+  const uint64_t row = tile * 16 + ci;
+  const uint64_t rc = row < n ? row : n - 1;
+#pragma unroll
+  for (int ks = 0; ks < NB * 4; ++ks) {
+    const uint32_t k = ks * 4 + kr;
+    const double v = A[rc * l + (k < l ? k : l - 1)];
+    x[ks] = (row < n && k < l) ? v : 0.0;
+  }
+}
+
+// The previous Omega at the positions tile_omega's om[c2][v] takes for tile
+// `tile`: o[c2][v] = P[16 tile + lane % 16][16 c2 + lane / 16 + 4 v] (zero
+// outside; clamped, unconditional loads).
+template <int NB>
+__device__ inline void load_prev(const double* __restrict__ P, uint64_t n, uint32_t l, uint64_t tile,
+                                 double (&o)[NB][4]) {
+  const int lane = threadIdx.x & 63, kr = lane >> 4, ci = lane & 15;
+  const uint64_t row = tile * 16 + ci;
+  const uint64_t rc = row < n ? row : n - 1;
+#pragma unroll
+  for (int c2 = 0; c2 < NB; ++c2)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t c = c2 * 16 + kr + 4 * v;
+      const double y = P[rc * l + (c < l ? c : l - 1)];
+      o[c2][v] = (row < n && c < l) ? y : 0.0;
+    }
+}
+
+// Omega^T of one 16-row tile in registers from the tile of H in b (load_tile
+// layout, which is the B operand of step ks): Q1^T = R1^-T H^T, then
+// Omega^T = Rb^T Q1^T, whose D layout (lane (k, i): Q1[r0 + i][16 cb + k + 4 v])
+// is already the B operand of the second product.
+// om[cb][v] = Omega[r0 + lane % 16][16 cb + lane / 16 + 4 v].
+template <int NB>
+__device__ inline void tile_omega(const double (&b)[NB * 4], uint32_t l, const double* r1, const double* rb,
+                                  f64x4 (&om)[NB]) {
   constexpr int W = NB * 16;
   const int lane = threadIdx.x & 63, kr = lane >> 4, ci = lane & 15;
-  const uint64_t row = r0 + ci;
-  double b[W / 4];
-#pragma unroll
-  for (int ks = 0; ks < W / 4; ++ks) {
-    const uint32_t k = ks * 4 + kr;
-    b[ks] = (row < n && k < l) ? H[row * l + k] : 0.0;
-  }
   f64x4 qt[NB];
 #pragma unroll
   for (int cb = 0; cb < NB; ++cb) qt[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
@@ -483,12 +503,16 @@ __device__ __attribute__((always_inline)) inline void gram_rows_pass(const doubl
 }
 
 // Q = A Rinv (rows of A in 16-row tiles, Rinv a zero-padded W x W LDS matrix)
-// over this block's share of the tiles, written to Qout when given, and the
-// Gram partial of Q summed over the block's waves into dst.
-template <int NB>
+// over the wave's tiles [t0, t1), written to Qout when given, and the Gram
+// partial of Q summed over the block's waves into dst.  With AHEAD (for waves
+// with a tile or two), x holds tile t0 (load_tile) on entry and each next
+// tile is loaded while the current computes; without it (long ranges, where
+// the registers buy occupancy instead) each tile is loaded in turn.
+template <int NB, bool AHEAD>
 __device__ __attribute__((always_inline)) inline void apply_gram_pass(const double* __restrict__ A, uint64_t n,
                                                                       uint32_t l, const double* rinv, double* Qout,
-                                                                      double* slots, double* dst) {
+                                                                      double* slots, double* dst, uint64_t t0,
+                                                                      uint64_t t1, double (&x)[NB * 4]) {
   using S = Shape<NB>;
   constexpr int W = S::W;
   const int lane = threadIdx.x & 63;
@@ -496,17 +520,14 @@ __device__ __attribute__((always_inline)) inline void apply_gram_pass(const doub
   f64x4 g[S::NP];
 #pragma unroll
   for (int p = 0; p < S::NP; ++p) g[p] = f64x4{0.0, 0.0, 0.0, 0.0};
-  uint64_t t0, t1;
-  wave_tiles(n, false, t0, t1);
   for (uint64_t tile = t0; tile < t1; ++tile) {
-    // Q tile = A tile Rinv: lane (k, i) supplies A[r0 + i][4 ks + k]
-    const uint64_t row = tile * 16 + ci;
-    double x[W / 4];
-#pragma unroll
-    for (int ks = 0; ks < W / 4; ++ks) {
-      const uint32_t k = ks * 4 + kr;
-      x[ks] = (row < n && k < l) ? A[row * l + k] : 0.0;
+    double xn[AHEAD ? W / 4 : 1];
+    if constexpr (AHEAD) {
+      if (tile + 1 < t1) load_tile<NB>(A, n, l, tile + 1, xn);
+    } else {
+      load_tile<NB>(A, n, l, tile, x);
     }
+    // Q tile = A tile Rinv: lane (k, i) supplies A[r0 + i][4 ks + k]
     f64x4 acc[NB];
 #pragma unroll
     for (int cb = 0; cb < NB; ++cb) acc[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
@@ -535,6 +556,11 @@ __device__ __attribute__((always_inline)) inline void apply_gram_pass(const doub
 #pragma unroll
         for (int cb = ca; cb < NB; ++cb, ++p) g[p] = mfma64(acc[ca][v], acc[cb][v], g[p]);
     }
+    if constexpr (AHEAD) {
+      if (tile + 1 < t1)
+#pragma unroll
+        for (int ks = 0; ks < W / 4; ++ks) x[ks] = xn[ks];
+    }
   }
   block_sum_waves<S::NP>(g, slots);
   store_gram<S::NP>(g, dst);
@@ -554,22 +580,26 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
   constexpr int PRO = 2 * W * W + W;  // a, E, isd
   __shared__ double r1[W * W];
   __shared__ double sh[S::SLOTS > PRO ? S::SLOTS : PRO];
-  const int lane = threadIdx.x & 63;
-  const int kr = lane >> 4, ci = lane & 15;
   const uint32_t l = q.l;
+  // the wave's first tile of H is in flight during the prologue
+  uint64_t t0, t1;
+  wave_tiles(q.n, false, t0, t1);
+  double x[NB * 4];
+  if (t0 < t1) load_tile<NB>(q.H, q.n, l, t0, x);
   // prologue: R1^-1 from the Gram of H
   {
     double* a = sh;
     double* E = a + W * W;
     double* isd = E + W * W;
-    gram_to_upper<NB>(q.part, q.nparts, l, a);
+    gram_to_upper<NB>(q.gram, l, a);
     const int bad = factor<NB>(a, E, isd, l, r1);
     if (blockIdx.x == 0) {
       store_compact(r1, l, W, q.R1i);
       if (threadIdx.x == 0 && bad && q.status[0] == 0) q.status[0] = bad;
     }
   }
-  apply_gram_pass<NB>(q.H, q.n, l, r1, nullptr, sh, q.part + (uint64_t)(kMaxBlocks + blockIdx.x) * S::GRAM);
+  apply_gram_pass<NB, true>(q.H, q.n, l, r1, nullptr, sh, q.part + (uint64_t)(kMaxBlocks + blockIdx.x) * S::GRAM, t0,
+                            t1, x);
 }
 
 // The top W rows of the unsigned Omega (row stride W) from its first tiles,
@@ -580,8 +610,10 @@ __device__ void omega_top(const QrPass& q, const double* r1, const double* r2, i
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int kr = lane >> 4, ci = lane & 15;
   for (int tb = wv; tb < NB; tb += waves) {
+    double b[NB * 4];
+    load_tile<NB>(q.H, q.n, q.l, tb, b);
     f64x4 om[NB];
-    tile_omega<NB>(q.H, q.n, q.l, (uint64_t)tb * 16, r1, r2, om);
+    tile_omega<NB>(b, q.l, r1, r2, om);
 #pragma unroll
     for (int c2 = 0; c2 < NB; ++c2)
 #pragma unroll
@@ -621,13 +653,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int kr = lane >> 4, ci = lane & 15;
   const uint32_t l = q.l, t = threadIdx.x, nt = blockDim.x;
+  constexpr int V = wave_v<NB>();
+  const bool wave_lu = l <= 8 * V;
+  // the wave's first tile of H is in flight during the prologue (for l <= 48;
+  // above, the registers go to the prologue)
+  constexpr bool kPre = NB <= 3;
+  uint64_t t0, t1;
+  wave_tiles(q.n, wave_lu, t0, t1);
+  double x[NB * 4], o[NB][4];
+  if (kPre && t0 < t1) load_tile<NB>(q.H, q.n, l, t0, x);
   // prologue: R2^-1 from the Gram of Q1
   stage_padded(q.R1i, l, W, r1);
   {
     double* a = sh;
     double* E = a + W * W;
     double* isd = E + W * W;
-    gram_to_upper<NB>(q.part + (uint64_t)kMaxBlocks * S::GRAM, q.nparts, l, a);
+    gram_to_upper<NB>(q.gram, l, a);
     const int bad = factor<NB>(a, E, isd, l, r2);
     if (blockIdx.x == 0) {
       store_compact(r2, l, W, q.R2i);
@@ -636,8 +677,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
   }
   // the Householder signs of the unsigned Omega's top l x l block: one wave
   // beside the pass when l fits wave_v, else the whole of block 0 before it
-  constexpr int V = wave_v<NB>();
-  const bool wave_lu = l <= 8 * V;
   if (blockIdx.x == 0) {
     if constexpr (V > 0) {
       if (wave_lu) {  // waves 0 .. NB-1 compute the top tiles; wave 0 then runs the LU
@@ -656,11 +695,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
   for (int c2 = 0; c2 < NB; ++c2)
 #pragma unroll
     for (int v = 0; v < 4; ++v) dm[c2][v] = dp[c2][v] = cs[c2][v] = 0.0;
-  uint64_t t0, t1;
-  wave_tiles(q.n, wave_lu, t0, t1);
   for (uint64_t tile = t0; tile < t1; ++tile) {
+    if (!kPre || tile != t0) load_tile<NB>(q.H, q.n, l, tile, x);
+    if (kPre) load_prev<NB>(q.omg2, q.n, l, tile, o);  // waited for after the products
     f64x4 om[NB];
-    tile_omega<NB>(q.H, q.n, l, tile * 16, r1, r2, om);
+    tile_omega<NB>(x, l, r1, r2, om);
+    if (!kPre) load_prev<NB>(q.omg2, q.n, l, tile, o);
     const uint64_t row = tile * 16 + ci;
 #pragma unroll
     for (int c2 = 0; c2 < NB; ++c2)
@@ -668,10 +708,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
       for (int v = 0; v < 4; ++v) {
         const uint32_t c = c2 * 16 + kr + 4 * v;
         if (row < q.n && c < l) {
-          const double x = om[c2][v], o = q.omg2[row * l + c];
-          dm[c2][v] += fabs(o - x);
-          dp[c2][v] += fabs(o + x);
-          cs[c2][v] += x;
+          const double y = om[c2][v], p = o[c2][v];
+          dm[c2][v] += fabs(p - y);
+          dp[c2][v] += fabs(p + y);
+          cs[c2][v] += y;
         }
       }
   }
@@ -715,12 +755,17 @@ __global__ __launch_bounds__(256) void k_qr_write(QrPass q) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int kr = lane >> 4, ci = lane & 15;
   const uint32_t l = q.l, t = threadIdx.x;
+  // the wave's tile of H is in flight during the prologue
+  const uint64_t tiles = ceil_div(q.n, 16);
+  uint64_t tile = (uint64_t)blockIdx.x * 4 + wv;
+  double x[NB * 4];
+  if (tile < tiles) load_tile<NB>(q.H, q.n, l, tile, x);
   stage_padded(q.R1i, l, W, r1);
   stage_padded(q.R2i, l, W, rf);
   // flipOmg on the signed Omega (the sign of a column swaps its two
   // distances): the statistics partials summed in block order
   __shared__ double stat[S::STAT];
-  if (t < S::STAT) stat[t] = sum_parts(q.part + (uint64_t)2 * kMaxBlocks * S::GRAM, q.nparts, S::STAT, t);
+  if (t < S::STAT) stat[t] = q.gram[t];
   __syncthreads();
   if (t < l) {
     const double s = q.sign[t];
@@ -733,19 +778,18 @@ __global__ __launch_bounds__(256) void k_qr_write(QrPass q) {
   __syncthreads();
   for (int e = t; e < W * W; e += blockDim.x) rf[e] *= (e % W < (int)l) ? fs[e % W] : 0.0;
   __syncthreads();
-  const uint64_t tiles = ceil_div(q.n, 16);
-  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + wv; tile < tiles; tile += (uint64_t)gridDim.x * 4) {
+  for (; tile < tiles; tile += (uint64_t)gridDim.x * 4) {
     f64x4 om[NB];
-    tile_omega<NB>(q.H, q.n, l, tile * 16, r1, rf, om);
+    tile_omega<NB>(x, l, r1, rf, om);
+    if (tile + (uint64_t)gridDim.x * 4 < tiles) load_tile<NB>(q.H, q.n, l, tile + (uint64_t)gridDim.x * 4, x);
     const uint64_t row = tile * 16 + ci;
-    if (row >= q.n) continue;
     const uint64_t kpos = q.kperm ? kperm16(row) : row;
 #pragma unroll
     for (int c2 = 0; c2 < NB; ++c2)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const uint32_t c = c2 * 16 + kr + 4 * v;
-        if (c >= l) continue;
+        if (c >= l || row >= q.n) continue;
         const double x = om[c2][v];
         const uint64_t gi = row * l + c;
         q.omega[gi] = x;
@@ -766,46 +810,28 @@ __global__ __launch_bounds__(256) void k_qr_write(QrPass q) {
   }
 }
 
+// out = the sum of nparts partials of count doubles (stride apart), in block
+// order; with square (Gram partials in tile layout), out is the full symmetric
+// l x l matrix (row-major) instead.  Each thread has 16 partials' loads in
+// flight per batch.
 template <int NB>
-void launch_nb(QrPass q, int pass, hipStream_t s) {
-  const uint64_t tiles = ceil_div(q.n, 16);
-  // reduction passes: about one tile per wave, at most kMaxBlocks partials
-  q.nparts = (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(tiles, kWaves), 1), kMaxBlocks);
-  const uint32_t nwrite = (uint32_t)std::max<uint64_t>(ceil_div(tiles, 4), 1);
-  switch (pass) {
-    case 0: hipLaunchKernelGGL(k_qr_gram_h<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q); break;
-    case 1: hipLaunchKernelGGL(k_qr_gram_q<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q); break;
-    case 2: hipLaunchKernelGGL(k_qr_flip<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q); break;
-    default: hipLaunchKernelGGL(k_qr_write<NB>, dim3(nwrite), dim3(256), 0, s, q); break;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// computeUSV's CholeskyQR2 of G (M x l) and B^T = H R^-1 (src/Halko.cpp:53-68)
-// ---------------------------------------------------------------------------
-constexpr int kUsvBlocks = 256;  // one block per CU for the M-row passes
-
-// out = sum over nparts Gram partials (tile layout, in block order); with
-// square, out is the full symmetric l x l matrix (row-major) instead.
-template <int NB>
-__global__ __launch_bounds__(256) void k_gram_sum(const double* __restrict__ part, int nparts, uint32_t l,
-                                                  int square, double* __restrict__ out) {
-  using S = Shape<NB>;
+__global__ __launch_bounds__(256) void k_gram_sum(const double* __restrict__ part, int nparts, uint64_t stride,
+                                                  int count, uint32_t l, int square, double* __restrict__ out) {
   __shared__ double red[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int e = blockIdx.x * 64 + lane;
   double s = 0.0;
-  if (e < S::GRAM)
+  if (e < count)
     for (int b0 = wv * 16; b0 < nparts; b0 += 64) {  // wave w takes batches 4 j + w of 16 partials
       double v[16];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) v[u] = (b0 + u < nparts) ? part[(uint64_t)(b0 + u) * S::GRAM + e] : 0.0;
+      for (int u = 0; u < 16; ++u) v[u] = (b0 + u < nparts) ? part[(uint64_t)(b0 + u) * stride + e] : 0.0;
 #pragma unroll
       for (int u = 0; u < 16; ++u) s += v[u];
     }
   red[wv][lane] = s;
   __syncthreads();
-  if (wv != 0 || e >= S::GRAM) return;
+  if (wv != 0 || e >= count) return;
   const double tot = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
   if (!square) {
     out[e] = tot;
@@ -825,6 +851,43 @@ __global__ __launch_bounds__(256) void k_gram_sum(const double* __restrict__ par
   }
 }
 
+template <int NB>
+void launch_parts_sum(const double* part, int nparts, uint64_t stride, int count, uint32_t l, int square,
+                      double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gram_sum<NB>, dim3((uint32_t)ceil_div(count, 64)), dim3(256), 0, s, part, nparts, stride,
+                     count, l, square, out);
+}
+
+template <int NB>
+void launch_nb(QrPass q, int pass, hipStream_t s) {
+  const uint64_t tiles = ceil_div(q.n, 16);
+  // reduction passes: about one tile per wave, at most kMaxBlocks partials
+  q.nparts = (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(tiles, kWaves), 1), kMaxBlocks);
+  const uint32_t nwrite = (uint32_t)std::max<uint64_t>(ceil_div(tiles, 4), 1);
+  using S = Shape<NB>;
+  switch (pass) {
+    case 0:
+      hipLaunchKernelGGL(k_qr_gram_h<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q);
+      launch_parts_sum<NB>(q.part, q.nparts, S::GRAM, S::GRAM, q.l, 0, q.gram, s);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_qr_gram_q<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q);
+      launch_parts_sum<NB>(q.part + (uint64_t)kMaxBlocks * S::GRAM, q.nparts, S::GRAM, S::GRAM, q.l, 0, q.gram, s);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_qr_flip<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q);
+      launch_parts_sum<NB>(q.part + (uint64_t)2 * kMaxBlocks * S::GRAM, q.nparts, S::STAT, S::STAT, q.l, 0, q.gram,
+                           s);
+      break;
+    default: hipLaunchKernelGGL(k_qr_write<NB>, dim3(nwrite), dim3(256), 0, s, q); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// computeUSV's CholeskyQR2 of G (M x l) and B^T = H R^-1 (src/Halko.cpp:53-68)
+// ---------------------------------------------------------------------------
+constexpr int kUsvBlocks = 256;  // one block per CU for the M-row passes
+
 // Gram partials of G's rows.
 template <int NB>
 __global__ __launch_bounds__(kThreads) void k_usv_gram(UsvPass u) {
@@ -843,18 +906,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
   __shared__ double r1[W * W];
   __shared__ double sh[S::SLOTS > PRO ? S::SLOTS : PRO];
   const uint32_t l = u.l;
+  uint64_t t0, t1;
+  wave_tiles(u.m, false, t0, t1);
+  double x[NB * 4];
   {
     double* a = sh;
     double* E = a + W * W;
     double* isd = E + W * W;
-    gram_to_upper<NB>(u.gram, 1, l, a);
+    gram_to_upper<NB>(u.gram, l, a);
     const int bad = factor<NB>(a, E, isd, l, r1);
     if (blockIdx.x == 0) {
       store_compact(r1, l, W, u.R1i);
       if (threadIdx.x == 0 && bad && u.status[0] == 0) u.status[0] = bad;
     }
   }
-  apply_gram_pass<NB>(u.G, u.m, l, r1, nullptr, sh, u.part + (uint64_t)blockIdx.x * S::GRAM);
+  apply_gram_pass<NB, false>(u.G, u.m, l, r1, nullptr, sh, u.part + (uint64_t)blockIdx.x * S::GRAM, t0, t1, x);
 }
 
 // prologue: R2^-1 from the reduced Gram of Q1 and Rt^-1 = R1^-1 R2^-1 (the
@@ -867,12 +933,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
   __shared__ double r1[W * W], rt[W * W];
   __shared__ double sh[S::SLOTS > PRO ? S::SLOTS : PRO];
   const uint32_t l = u.l, t = threadIdx.x;
+  uint64_t t0, t1;
+  wave_tiles(u.n, false, t0, t1);
+  double x[NB * 4];
   stage_padded(u.R1i, l, W, r1);
   {
     double* a = sh;
     double* E = a + W * W;
     double* isd = E + W * W;
-    gram_to_upper<NB>(u.gram, 1, l, a);
+    gram_to_upper<NB>(u.gram, l, a);
     const int bad = factor<NB>(a, E, isd, l, rt);  // rt holds R2^-1 for now
     if (blockIdx.x == 0) {
       store_compact(rt, l, W, u.R2i);
@@ -890,7 +959,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
     __syncthreads();
     if (blockIdx.x == 0) store_compact(rt, l, W, u.Rti);
   }
-  apply_gram_pass<NB>(u.H, u.n, l, rt, u.Bt, sh, u.part + (uint64_t)blockIdx.x * S::GRAM);
+  apply_gram_pass<NB, false>(u.H, u.n, l, rt, u.Bt, sh, u.part + (uint64_t)blockIdx.x * S::GRAM, t0, t1, x);
 }
 
 template <int NB>
@@ -899,19 +968,18 @@ void launch_usv_nb(UsvPass u, hipStream_t s, int step) {
   const uint32_t mt = (uint32_t)ceil_div(u.m, 16), nt = (uint32_t)ceil_div(u.n, 16);
   const int gm = (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(mt, kWaves), 1), kUsvBlocks);
   const int gn = (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(nt, kWaves), 1), kMaxBlocks);
-  const dim3 sum_grid((uint32_t)ceil_div(S::GRAM, 64));
   switch (step) {
     case 0:  // Gram of G, reduced
       hipLaunchKernelGGL(k_usv_gram<NB>, dim3(gm), dim3(kThreads), 0, s, u);
-      hipLaunchKernelGGL(k_gram_sum<NB>, sum_grid, dim3(256), 0, s, u.part, gm, u.l, 0, u.gram);
+      launch_parts_sum<NB>(u.part, gm, S::GRAM, S::GRAM, u.l, 0, u.gram, s);
       break;
     case 1:  // R1^-1, Gram of Q1, reduced
       hipLaunchKernelGGL(k_usv_q1<NB>, dim3(gm), dim3(kThreads), 0, s, u);
-      hipLaunchKernelGGL(k_gram_sum<NB>, sum_grid, dim3(256), 0, s, u.part, gm, u.l, 0, u.gram);
+      launch_parts_sum<NB>(u.part, gm, S::GRAM, S::GRAM, u.l, 0, u.gram, s);
       break;
     default:  // R2^-1, Rt^-1, B^T and B B^T (square)
       hipLaunchKernelGGL(k_usv_bt<NB>, dim3(gn), dim3(kThreads), 0, s, u);
-      hipLaunchKernelGGL(k_gram_sum<NB>, sum_grid, dim3(256), 0, s, u.part, gn, u.l, 1, u.bbt);
+      launch_parts_sum<NB>(u.part, gn, S::GRAM, S::GRAM, u.l, 1, u.bbt, s);
       break;
   }
 }
