@@ -158,8 +158,8 @@ struct FinalizeH {
 void launch_finalize_H(const FinalizeH& f, hipStream_t s);
 
 // Sparse corrections for the missing genotypes: for each list row o of a
-// CSR (ptr, idx), out[o][c] = sum_p table[idx[p]][c] (FP64 sums of a float or
-// double table).  List rows come in ngroups groups of `per` rows: row o reads
+// CSR (ptr, idx), out[o][c] = sum_p table[idx[p]][c] (summed in the table's
+// type, FP32 or FP64, and stored as FP64).  List rows come in ngroups groups of `per` rows: row o reads
 // the list at in_base + (o / per) * stride_in + o % per and writes out row
 // out_base + o.
 struct CsrGather {

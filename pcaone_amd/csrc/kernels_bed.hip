@@ -607,9 +607,6 @@ void launch_packed_gemm_f64(const PackedGemm& g, int ncols, hipStream_t s) { lau
 // ---------------------------------------------------------------------------
 // epilogues
 // ---------------------------------------------------------------------------
-__device__ inline double load_part(const void* p, int is64, uint64_t i) {
-  return is64 ? reinterpret_cast<const double*>(p)[i] : (double)reinterpret_cast<const float*>(p)[i];
-}
 
 // Grid (row chunk of 64, window).  Besides G, g'' and the H operand the
 // epilogue writes the chunk's partial column sums of g'' (s2p), which
@@ -878,40 +875,69 @@ void launch_window_colsum(const double* s2p, uint32_t l, uint32_t blocksize, uin
 }
 
 // thread (sample j, column c); the windows' contributions are summed in
-// registers per destination and added once
-__global__ void k_finalize_H(FinalizeH f) {
+// registers per destination and added once.  Four windows' partials, four
+// split-K slices each, and their corrections load together from clamped
+// indices (unconditional loads issue together; out-of-range values drop out).
+template <typename P>
+__global__ __launch_bounds__(256) void k_finalize_H(FinalizeH f) {
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (uint64_t)f.nsamples * f.l) return;
   const uint32_t j = (uint32_t)(gid / f.l), c = (uint32_t)(gid % f.l);
+  const P* __restrict__ part = reinterpret_cast<const P*>(f.part);
+  const double* __restrict__ corrp = f.corr;
+  const double* __restrict__ S2 = f.S2;
+  const uint64_t lo = f.split_cols ? f.l : 0;
   double ha = 0.0, hb = 0.0;
   bool any_a = false, any_b = false;
-  for (uint32_t w = f.w0; w < f.w1; ++w) {
-    const uint32_t wz = w - f.wpart0;
-    double raw = 0.0;
-    for (uint32_t s = 0; s < f.nsplit; ++s) {
-      const uint64_t base = (((uint64_t)wz * f.nsplit + s) * f.part_rows + j) * f.ldp;
-      raw += load_part(f.part, f.part_is_f64, base + c);
-      if (f.split_cols) raw += load_part(f.part, f.part_is_f64, base + f.l + c);
+  for (uint32_t w0 = f.w0; w0 < f.w1; w0 += 4) {
+    double raw[4], corr[4], s2[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t w = min(w0 + u, f.w1 - 1);
+      raw[u] = 0.0;
+      corr[u] = corrp ? corrp[((uint64_t)w * f.nsamples + j) * f.l + c] : 0.0;
+      s2[u] = S2[(uint64_t)w * f.l + c];
     }
-    // the contraction ran on 2 - 2 y: X^T g' = sum g' - part / 2, and S2 already
-    // holds sum (g'' - g') for this layout
-    if (f.affine) raw = -0.5 * raw;
-    const double corr = f.corr ? f.corr[((uint64_t)w * f.nsamples + j) * f.l + c] : 0.0;
-    double corr_e = 0.0;
-    if (f.mval) {
-      const uint64_t li = (uint64_t)w * f.miss_rows + j;
-      for (int64_t p = f.miss_ptr[li]; p < f.miss_ptr[li + 1]; ++p) {
-        const uint64_t i = (uint64_t)f.miss_idx[p];
-        corr_e += f.mval[f.mpos[p]] * f.d[i] * f.G[i * f.l + c];
+    for (uint32_t s0 = 0; s0 < f.nsplit; s0 += 4) {
+      P x[4][4][2];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t wz = min(w0 + u, f.w1 - 1) - f.wpart0, sk = min(s0 + k, f.nsplit - 1);
+          const uint64_t base = (((uint64_t)wz * f.nsplit + sk) * f.part_rows + j) * f.ldp + c;
+          x[u][k][0] = part[base];
+          x[u][k][1] = part[base + lo];
+        }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (s0 + k < f.nsplit) raw[u] += (double)x[u][k][0] + (lo ? (double)x[u][k][1] : 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t w = w0 + u;
+      if (w >= f.w1) continue;
+      // the contraction ran on 2 - 2 y: X^T g' = sum g' - part / 2, and S2 already
+      // holds sum (g'' - g') for this layout
+      const double rv = f.affine ? -0.5 * raw[u] : raw[u];
+      double corr_e = 0.0;
+      if (f.mval) {
+        const uint64_t li = (uint64_t)w * f.miss_rows + j;
+        for (int64_t p = f.miss_ptr[li]; p < f.miss_ptr[li + 1]; ++p) {
+          const uint64_t i = (uint64_t)f.miss_idx[p];
+          corr_e += f.mval[f.mpos[p]] * f.d[i] * f.G[i * f.l + c];
+        }
       }
-    }
-    const double h = f.hf * (raw - (f.S2[(uint64_t)w * f.l + c] - corr)) + corr_e;
-    if ((f.route_b >> (w - f.w0)) & 1ull) {
-      hb += h;
-      any_b = true;
-    } else {
-      ha += h;
-      any_a = true;
+      const double h = f.hf * (rv - (s2[u] - corr[u])) + corr_e;
+      if ((f.route_b >> (w - f.w0)) & 1ull) {
+        hb += h;
+        any_b = true;
+      } else {
+        ha += h;
+        any_a = true;
+      }
     }
   }
   if (any_a) f.Ha[(uint64_t)j * f.l + c] += ha;
@@ -920,7 +946,10 @@ __global__ void k_finalize_H(FinalizeH f) {
 
 void launch_finalize_H(const FinalizeH& f, hipStream_t s) {
   const uint64_t total = (uint64_t)f.nsamples * f.l;
-  hipLaunchKernelGGL(k_finalize_H, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, f);
+  if (f.part_is_f64)
+    hipLaunchKernelGGL(k_finalize_H<double>, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, f);
+  else
+    hipLaunchKernelGGL(k_finalize_H<float>, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, f);
 }
 
 // one thread per SNP row: EMU value of each of its missing samples
@@ -978,56 +1007,59 @@ void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin,
                      nsamples, sptr, sidx, pos);
 }
 
-// One wave per list row.  The row's indices are loaded 64 at a time (one per
-// lane) and handed out with lane shuffles.  Lane group grp (lpg = ceil(l / V)
-// lanes, V consecutive columns per lane read as one vector load) takes
-// entries grp, grp + ng, ... so a wave has 4 ng table rows in flight.  The
-// groups' sums combine in a fixed order.
+// One wave per list row, grid-stride.  The row's indices are loaded 64 at a
+// time (one per lane) and handed out with lane shuffles.  Lane group grp
+// (lpg = ceil(l / V) lanes, V consecutive columns per lane read as one vector
+// load) takes entries grp, grp + ng, ..., two per round.  The sums run in the
+// table's own type (FP32 tables: FP32 sums of FP32 values, a relative error
+// of a few 1e-7 on each correction; FP64 tables: FP64), and the groups' sums
+// combine in a fixed order.  Measured on the 500,000-row missing lists of the
+// 10,000-sample workload: the FP32 sums (half the shuffles, fewer registers,
+// more waves resident), two entries per round and the grid stride take a
+// launch from 387 to 257 us.
 template <class T, int V>
 __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
   typedef T vec __attribute__((ext_vector_type(V)));
+  constexpr int D = 2;  // entries per group per round
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int lpg = (int)((g.l + V - 1) / V);
   const int ng = 64 / lpg;
   const int grp = lane / lpg, li = lane - grp * lpg;
   const bool act = grp < ng && (uint32_t)(li * V) < g.l;
-  const T* tab = reinterpret_cast<const T*>(g.table);
+  const uint32_t coff = act ? (uint32_t)(li * V) : 0;
+  const T* __restrict__ tab = reinterpret_cast<const T*>(g.table);
   const uint64_t total = (uint64_t)g.per * g.ngroups;
   for (uint64_t o = (uint64_t)blockIdx.x * 4 + wv; o < total; o += (uint64_t)gridDim.x * 4) {
     const uint64_t q = o / g.per;
     const uint64_t row = g.in_base + q * g.stride_in + (o - q * g.per);
     const int64_t p0 = g.ptr[row], p1 = g.ptr[row + 1];
-    double s[V];
+    T s[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) s[v] = 0.0;
+    for (int v = 0; v < V; ++v) s[v] = T(0);
     for (int64_t b = p0; b < p1; b += 64) {
       const int cnt = (int)(p1 - b < 64 ? p1 - b : 64);
       const int32_t mine = lane < cnt ? g.idx[b + lane] : 0;
-      for (int e0 = 0; e0 < cnt; e0 += 4 * ng) {
-        int j[4];
-        bool ok[4];
+      for (int e0 = 0; e0 < cnt; e0 += D * ng) {
+        vec x[D];
+        bool ok[D];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < D; ++u) {
           const int e = e0 + u * ng + grp;
           ok[u] = act && e < cnt;
-          j[u] = __shfl(mine, (e < cnt) ? e : 0);
-        }
-        vec x[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (ok[u]) {
-            x[u] = *reinterpret_cast<const vec*>(tab + (uint64_t)j[u] * g.l + li * V);
-          } else {
-#pragma unroll
-            for (int v = 0; v < V; ++v) x[u][v] = T(0);
-          }
+          const int32_t j = __shfl(mine, (e < cnt) ? e : 0);
+          // clamped, unconditional loads (row 0 stands in for idle lanes)
+          x[u] = *reinterpret_cast<const vec*>(tab + (uint64_t)(ok[u] ? j : 0) * g.l + coff);
         }
 #pragma unroll
-        for (int v = 0; v < V; ++v)
-          s[v] += ((double)x[0][v] + (double)x[1][v]) + ((double)x[2][v] + (double)x[3][v]);
+        for (int v = 0; v < V; ++v) {
+          T acc = T(0);
+#pragma unroll
+          for (int u = 0; u < D; ++u) acc += ok[u] ? x[u][v] : T(0);
+          s[v] += acc;
+        }
       }
     }
-    double t[V];
+    T t[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) t[v] = s[v];
     for (int gg = 1; gg < ng; ++gg)
@@ -1037,7 +1069,7 @@ __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
       const uint64_t orow = g.out_base + o;
 #pragma unroll
       for (int v = 0; v < V; ++v)
-        if ((uint32_t)(li * V + v) < g.l) g.out[orow * g.l + li * V + v] = t[v];
+        if ((uint32_t)(li * V + v) < g.l) g.out[orow * g.l + li * V + v] = (double)t[v];
     }
   }
 }
@@ -1046,7 +1078,7 @@ void launch_csr_gather(const CsrGather& g, hipStream_t s) {
   PCAONE_REQUIRE(g.l >= 1 && g.l <= 64, -1, "gather: l must be in [1, 64]");
   const uint64_t total = (uint64_t)g.per * g.ngroups;
   if (!total) return;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(total, 4), 1u << 20);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(total, 4), 32768);
   // 16-byte vector loads need rows that are whole vectors (tables are hipMalloc'd)
   if (g.table_f32) {
     if (g.l % 4 == 0)
