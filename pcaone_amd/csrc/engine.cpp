@@ -235,10 +235,14 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   part_f64_ = prec == PREC_FP64;
   const size_t partsz = part_f64_ ? 8 : 4;
   if (!dense) {
-    // packed: split operand = [hi | lo | 0-pad] in nt16 tiles, partials keep both halves
+    // packed: split operand = [hi | lo | 0-pad] in nt16 tiles; the partials
+    // keep both halves, or hi + lo per column where the contraction folds
+    // them (l % 16 in {0, 8}; the streamed kernel, launches of 65536 rows or
+    // more)
     nt16_ = (int)ceil_div(2 * l_, 16);
     bcols_ = prec == PREC_SPLIT_BF16 ? (uint32_t)nt16_ * 16 : l_;
     lo_off_ = l_;
+    fold_ = (prec == PREC_SPLIT_BF16 && (l_ % 16 == 0 || l_ % 16 == 8)) ? l_ : 0;
     ldp_ = prec == PREC_SPLIT_BF16 ? (uint64_t)nt16_ * 16 : l_;
     split_cols_ = prec == PREC_SPLIT_BF16;
     hf_ = 0.5;
@@ -246,6 +250,7 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
     // dense: FP32 arm in 16-column tiles; split arm = hi tiles | lo tiles, the
     // three products land in one accumulator per tile
     ntiles_ = (int)ceil_div(l_, 16);
+    fold_ = 0;
     bcols_ = prec == PREC_FP64 ? l_ : prec == PREC_FP32 ? (uint32_t)ntiles_ * 16 : 2u * ntiles_ * 16;
     lo_off_ = (uint32_t)ntiles_ * 16;
     ldp_ = prec == PREC_FP64 ? l_ : (uint64_t)ntiles_ * 16;
@@ -693,6 +698,7 @@ SegBufs Engine::op_bufs() {
   b.opsz = opsz_;
   b.part_f64 = part_f64_;
   b.split_cols = split_cols_;
+  b.fold = fold_;
   b.hf = hf_;
   b.omega = omega_.p;
   b.colsum = colsum_.p;
@@ -723,7 +729,7 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
   const uint32_t r0 = w0 * bs_, r1 = std::min<uint32_t>(w1 * bs_, M_);
   const uint32_t rows = r1 - r0;
   const uint32_t rpb = rows_per_block_(prec);
-  uint32_t g_nsplit = 0;
+  uint32_t g_nsplit = 0, g_fold = 0;
   if (b.corrG) {
     CsrGather cg{};
     cg.ptr = miss_snp_ptr_.p;
@@ -752,9 +758,11 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
     g.B = b.bg;
     g.ldb = kg_;
     g.out = b.gpart;
-    g.ldo = b.ldp;
     g.nz = 1;
+    g.fold = packed_bf16_streams(rows, 1) ? b.fold : 0;
+    g.ldo = g.fold ? ceil_div(g.fold, 16) * 16 : b.ldp;
     g_nsplit = g.nsplit;
+    g_fold = g.fold;
     timed("gemm_G", [&] {
       if (prec == PREC_SPLIT_BF16)
         launch_packed_gemm_bf16(g, b.nt16, stream_);
@@ -781,6 +789,10 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
   FinalizeG fg = finalize_desc(w0, w1, b);
   fg.part = b.gpart;
   fg.nsplit = g_nsplit;
+  if (g_fold) {  // hi + lo already summed per column
+    fg.ldp = ceil_div(g_fold, 16) * 16;
+    fg.split_cols = 0;
+  }
   timed("finalize_G", [&] { launch_finalize_G(fg, stream_); });
   timed("window_colsum", [&] { launch_window_colsum(b.s2p, b.l, bs_, w0, w1, b.S2, stream_); });
 }
@@ -826,7 +838,7 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
   const bool dense = src_ == SRC_DENSE;
   const uint32_t rpb = rows_per_block_(prec);
   const uint32_t nz = w1 - w0;
-  uint32_t h_nsplit = 0;
+  uint32_t h_nsplit = 0, h_fold = 0;
   if (b.corrH) {
     CsrGather cg{};
     cg.ptr = miss_win_ptr_.p;
@@ -854,10 +866,12 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
     h.ldb = kw_;
     h.b_zstride = (uint64_t)b.bcols * kw_;
     h.out = b.hpart;
-    h.ldo = b.ldp;
-    h.o_zstride = (uint64_t)h.nsplit * N_ * b.ldp;
     h.nz = nz;
+    h.fold = packed_bf16_streams(N_, nz) ? b.fold : 0;
+    h.ldo = h.fold ? ceil_div(h.fold, 16) * 16 : b.ldp;
+    h.o_zstride = (uint64_t)h.nsplit * N_ * h.ldo;
     h_nsplit = h.nsplit;
+    h_fold = h.fold;
     timed("gemm_H", [&] {
       if (prec == PREC_SPLIT_BF16)
         launch_packed_gemm_bf16(h, b.nt16, stream_);
@@ -888,10 +902,10 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
   fh.part = b.hpart;
   fh.part_is_f64 = b.part_f64;
   fh.nsplit = h_nsplit;
-  fh.ldp = b.ldp;
+  fh.ldp = h_fold ? ceil_div(h_fold, 16) * 16 : b.ldp;  // folded: hi + lo already summed per column
   fh.part_rows = N_;
   fh.miss_rows = miss_rows_;
-  fh.split_cols = b.split_cols;
+  fh.split_cols = h_fold ? 0 : b.split_cols;
   fh.nsamples = N_;
   fh.l = b.l;
   fh.wpart0 = w0;

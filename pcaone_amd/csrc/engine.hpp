@@ -58,6 +58,7 @@ struct SegBufs {
   uint64_t ldp = 0;
   size_t opsz = 8;
   bool part_f64 = false, split_cols = false;
+  uint32_t fold = 0;  // split contraction folds lo onto hi columns (PackedGemm::fold)
   double hf = 0.5;
   const double* omega = nullptr;
   float* omega32 = nullptr;  // FP32 copy of Omega: the missing-entry gather table of the non-FP64 arms
@@ -189,6 +190,7 @@ class Engine {
   uint32_t lo_off_ = 0; // split operands: row of the lo part of column c is lo_off_ + c
   uint64_t ldp_ = 0;    // columns of the partial products
   bool part_f64_ = false, split_cols_ = false;
+  uint32_t fold_ = 0;
   int ntiles_ = 0;      // dense MFMA: 16-column output tiles
   uint64_t kg_ = 0;     // K extent of the G contraction (padded samples)
   uint32_t miss_rows_ = 0;

@@ -64,11 +64,16 @@ struct PackedGemm {
   uint64_t ldo;
   uint64_t o_zstride;  // elements between z-slices
   uint32_t nz;
+  uint32_t fold;       // split-BF16, streamed kernel only: out column c < fold = column c +
+                       // column fold + c (hi + lo; fold % 16 in {0, 8}; ldo >= round_up(fold, 16));
+                       // 0: every column stored
 };
 // split-BF16 MFMA: B is bf16 with nt16*16 columns ([hi | lo | 0-pad]); out fp32.
 // The A operand is y' = 2 - 2 y (y the dosage), so out = sum_k y'(row,k) B[col][k]
 // and X B = colsum(B) - out / 2 (the finalize kernels' `affine` flag).
 void launch_packed_gemm_bf16(const PackedGemm& g, int nt16, hipStream_t s);
+// whether a launch of nrows x nz rows runs the streamed kernel (the one that folds)
+bool packed_bf16_streams(uint64_t nrows, uint32_t nz);
 // FP32 / FP64 vector-FMA: B has `ncols` columns of float / double; out same type
 void launch_packed_gemm_f32(const PackedGemm& g, int ncols, hipStream_t s);
 void launch_packed_gemm_f64(const PackedGemm& g, int ncols, hipStream_t s);

@@ -273,6 +273,70 @@ __device__ inline void decode16(uint32_t w, bf16x8& a0, bf16x8& a1) {
 // MFMA with codes 64g+8s..64g+8s+7.  The B operand is staged per chunk in LDS,
 // rows padded to 264 bf16 so the 16 column-lanes of a ds_read_b128 hit
 // distinct banks.
+// Accumulators to the split-K partial: every column, or with g.fold the hi and
+// lo columns of each output column summed (the lo column of c sits fold
+// columns on, i.e. in tile (c + fold) / 16 at lane (c + fold) % 16: for
+// fold % 16 == 8 a rotation by 8 lanes inside each 16-lane DPP row).
+__device__ inline float ror8_row(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x128, 0xf, 0xf, false));
+}
+
+// The folded store with the tile offset TB = fold / 16 and HALF = fold % 16
+// == 8 as constants, so every accumulator index is static (a runtime index
+// would move the accumulators to scratch memory).
+template <int NT, int TB, bool HALF>
+__device__ inline void store_acc_fold(const PackedGemm& g, float* o, uint32_t row0, const f32x4 (&acc)[4][NT]) {
+  constexpr int NTO = HALF ? TB + 1 : TB;  // output tiles: ceil(fold / 16)
+  const int lane = threadIdx.x & 63, L = lane & 15;
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t r = row0 + 16 * f + 4 * (lane >> 4) + i;
+#pragma unroll
+      for (int t = 0; t < NTO && t < NT; ++t) {
+        const int t1 = t + TB < NT ? t + TB : NT - 1, t2 = t + TB + 1 < NT ? t + TB + 1 : NT - 1;
+        float lo;
+        if constexpr (HALF) {  // every lane of the wave runs the rotations
+          const float a = ror8_row(acc[f][t1][i]), b = ror8_row(acc[f][t2][i]);
+          lo = L < 8 ? a : b;
+        } else {
+          lo = acc[f][t1][i];
+        }
+        if (r < g.nrows) o[(uint64_t)r * g.ldo + t * 16 + L] = acc[f][t][i] + lo;
+      }
+    }
+}
+
+template <int NT, bool FOLD>
+__device__ inline void store_acc(const PackedGemm& g, float* o, uint32_t row0, const f32x4 (&acc)[4][NT]) {
+  const int lane = threadIdx.x & 63, L = lane & 15;
+  if constexpr (FOLD) {
+    if (g.fold) {
+      switch (g.fold >> 3) {  // fold / 8: TB = fold / 16, HALF = odd
+#define PCAONE_FOLD(K)                                                                  \
+  case K:                                                                               \
+    if constexpr (K / 2 + (K & 1) <= NT) store_acc_fold<NT, K / 2, (K & 1) != 0>(g, o, row0, acc); \
+    return;
+        PCAONE_FOLD(1) PCAONE_FOLD(2) PCAONE_FOLD(3) PCAONE_FOLD(4) PCAONE_FOLD(5) PCAONE_FOLD(6) PCAONE_FOLD(7)
+        PCAONE_FOLD(8)
+#undef PCAONE_FOLD
+        default: break;
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t r = row0 + 16 * f + 4 * (lane >> 4) + i;
+      if (r < g.nrows) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) o[(uint64_t)r * g.ldo + t * 16 + L] = acc[f][t][i];
+      }
+    }
+}
+
 constexpr int kBStride = 264;
 
 template <int NT, int WAVES>
@@ -373,17 +437,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_packed_gemm_bf16(PackedGemm g) {
 #undef PCAONE_LOAD_A
 #undef PCAONE_B_SRC
 #undef PCAONE_B_DST
-  float* o = out + (uint64_t)split * g.nrows * g.ldo;
-#pragma unroll
-  for (int f = 0; f < 4; ++f)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const uint32_t r = row0 + 16 * f + 4 * (lane >> 4) + i;
-      if (r < g.nrows) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) o[(uint64_t)r * g.ldo + t * 16 + (lane & 15)] = acc[f][t][i];
-      }
-    }
+  store_acc<NT, false>(g, out + (uint64_t)split * g.nrows * g.ldo, row0, acc);
 }
 
 // Long row ranges: 8 waves x 64 rows per block, the B chunk streamed straight
@@ -481,23 +535,18 @@ __global__ __launch_bounds__(512) void k_packed_gemm_bf16_stream(PackedGemm g) {
     __syncthreads();
   }
 #undef PCAONE_ISSUE_B
-  float* o = out + (uint64_t)split * g.nrows * g.ldo;
-#pragma unroll
-  for (int f = 0; f < 4; ++f)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const uint32_t r = row0 + 16 * f + 4 * (lane >> 4) + i;
-      if (r < g.nrows) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) o[(uint64_t)r * g.ldo + t * 16 + (lane & 15)] = acc[f][t][i];
-      }
-    }
+  store_acc<NT, true>(g, out + (uint64_t)split * g.nrows * g.ldo, row0, acc);
 }
 
 // streamed 8-wave blocks (512 rows, half the B traffic per row) for long row
 // ranges, 4-wave blocks where the rows would not fill the chip
+bool packed_bf16_streams(uint64_t nrows, uint32_t nz) { return nrows * nz >= 65536; }
+
 void launch_packed_gemm_bf16(const PackedGemm& g, int nt16, hipStream_t s) {
-  const bool big = (uint64_t)g.nrows * g.nz >= 65536;
+  const bool big = packed_bf16_streams(g.nrows, g.nz);
+  PCAONE_REQUIRE(!g.fold || (big && (g.fold % 16 == 0 || g.fold % 16 == 8) && 2 * g.fold <= 16u * (uint32_t)nt16 &&
+                             g.ldo >= (g.fold + 15) / 16 * 16),
+                 -1, "split contraction: fold needs the streamed kernel and a multiple of 8 within the operand");
   dim3 grid((uint32_t)ceil_div(g.nrows, big ? 512 : 256), g.nsplit, g.nz);
   switch (nt16) {
 #define PCAONE_NT(n)                                                                     \
