@@ -38,7 +38,7 @@ FP32_PEAK_TFLOPS = 157.3
 # FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0) averaged over the 280 launches of one
 # split-BF16 solve, the factors from a 512 MiB copy kernel
 # (profiles/k_packed_gemm_bf16_c2_winsvd.md)
-TRAFFIC = 313.6e6
+TRAFFIC = 303.8e6
 
 
 def log(*a):
@@ -152,8 +152,7 @@ def main():
     roofline = None
     if "bf16x2" in want:
         # roofline of the dominant kernel, timed with HIP events on the handle's stream
-        # (one extra split-BF16 solve, missing-entry gathers serialized so every
-        # kernel is timed alone)
+        # (one extra split-BF16 solve with an event pair around every launch)
         g.profiling(True)
         g.reset_stats()
         tprof, _, _, _, ep_prof = solve(g, pa, wl, pa.PREC_SPLIT_BF16)
