@@ -9,6 +9,7 @@ missing.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 from pathlib import Path
 
@@ -48,6 +49,8 @@ SIGNATURES = {
     "pcaone_gpu_destroy": (None, [C.c_void_p]),
     "pcaone_gpu_last_error": (C.c_char_p, [C.c_void_p]),
     "pcaone_gpu_device_count": (C.c_int, []),
+    "pcaone_gpu_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "pcaone_gpu_host_free": (C.c_int, [C.c_void_p]),
     "pcaone_gpu_load_bed": (C.c_int, [C.c_void_p, _U8, C.c_uint32, C.c_uint32]),
     "pcaone_gpu_allele_freq": (C.c_int, [C.c_void_p, _D, _I64]),
     "pcaone_gpu_load_dense_f32": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]),
@@ -106,6 +109,44 @@ def _p(a: np.ndarray | None, ctype):
 
 def device_count() -> int:
     return lib().pcaone_gpu_device_count()
+
+
+class _PinnedPool:
+    """Page-locked host buffers for result arrays, reused by byte size: a
+    buffer goes back to the pool when the last array viewing it is collected,
+    so repeated solves neither re-pin memory nor fault pageable pages in."""
+
+    def __init__(self, keep: int = 4):
+        self.free: dict[int, list[int]] = {}
+        self.keep = keep
+
+    def array(self, shape, order="C"):
+        n = int(np.prod(shape))
+        nbytes = max(n, 1) * 8
+        stack = self.free.get(nbytes)
+        if stack:
+            ptr = stack.pop()
+        else:
+            out = C.c_void_p()
+            if lib().pcaone_gpu_host_alloc(nbytes, C.byref(out)) != 0 or not out.value:
+                return np.empty(shape, order=order)  # pageable: same results, slower copy-out
+            ptr = out.value
+        flat = np.ctypeslib.as_array((C.c_double * max(n, 1)).from_address(ptr))
+        weakref.finalize(flat, self._release, ptr, nbytes)
+        a = flat[:n]
+        if order == "F":
+            return a.reshape(tuple(reversed(shape))).T
+        return a.reshape(shape)
+
+    def _release(self, ptr: int, nbytes: int):
+        stack = self.free.setdefault(nbytes, [])
+        if len(stack) < self.keep:
+            stack.append(ptr)
+        elif _lib is not None:
+            _lib.pcaone_gpu_host_free(ptr)
+
+
+_pinned = _PinnedPool()
 
 
 def sim_bed(seed: int, nsamples: int, nsnps: int, npops: int = 3, fst: float = 0.1, pmin: float = 0.05,
@@ -337,9 +378,9 @@ class Gpu:
         return G, H
 
     def usv(self, maxp: int, tol: float):
-        U = np.empty((self.N, self.k), order="F")
+        U = _pinned.array((self.N, self.k), order="F")
         S = np.empty(self.k)
-        V = np.empty((self.M, self.k), order="F")
+        V = _pinned.array((self.M, self.k), order="F")
         ep = C.c_int(0)
         diffs = np.zeros(maxp + 1)
         self._check(self._L.pcaone_gpu_rsvd_usv(self.h, maxp, tol, _p(U, C.c_double), _p(S, C.c_double),
@@ -351,9 +392,9 @@ class Gpu:
 
     def em(self, maxp: int, tol: float, maxiter: int = 100, tolem: float = 1e-5):
         """EMU EM-PCA (--emu); returns U, S, V, iterations, per-iteration diffs."""
-        U = np.empty((self.N, self.k), order="F")
+        U = _pinned.array((self.N, self.k), order="F")
         S = np.empty(self.k)
-        V = np.empty((self.M, self.k), order="F")
+        V = _pinned.array((self.M, self.k), order="F")
         it = C.c_int(0)
         diffs = np.zeros(max(maxiter, 1))
         self._check(self._L.pcaone_gpu_rsvd_em(self.h, maxp, tol, maxiter, tolem, _p(U, C.c_double),

@@ -46,6 +46,18 @@ int pcaone_gpu_device_count(void) {
   return n;
 }
 
+int pcaone_gpu_host_alloc(size_t bytes, void** out) {
+  if (!out) return PCAONE_EINVAL;
+  *out = nullptr;
+  if (bytes == 0) return 0;
+  return hipHostMalloc(out, bytes, hipHostMallocDefault) == hipSuccess ? 0 : -2;
+}
+
+int pcaone_gpu_host_free(void* p) {
+  if (!p) return 0;
+  return hipHostFree(p) == hipSuccess ? 0 : -2;
+}
+
 int pcaone_gpu_create(int device, pcaone_gpu** out) {
   if (!out) return PCAONE_EINVAL;
   *out = nullptr;
