@@ -524,8 +524,14 @@ __device__ __attribute__((always_inline)) inline void apply_gram_pass(const doub
     double xn[AHEAD ? W / 4 : 1];
     if constexpr (AHEAD) {
       if (tile + 1 < t1) load_tile<NB>(A, n, l, tile + 1, xn);
-    } else {
-      load_tile<NB>(A, n, l, tile, x);
+    } else {  // predicated loads (no clamped extra loads for the padding columns)
+      const int lane = threadIdx.x & 63;
+      const uint64_t row = tile * 16 + (lane & 15);
+#pragma unroll
+      for (int ks = 0; ks < W / 4; ++ks) {
+        const uint32_t k = ks * 4 + (lane >> 4);
+        x[ks] = (row < n && k < l) ? A[row * l + k] : 0.0;
+      }
     }
     // Q tile = A tile Rinv: lane (k, i) supplies A[r0 + i][4 ks + k]
     f64x4 acc[NB];
