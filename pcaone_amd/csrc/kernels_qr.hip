@@ -357,19 +357,17 @@ __device__ inline void store_compact(const double* src, uint32_t l, int W, doubl
 
 // The 16-row tile `tile` of an n x l row-major matrix in MFMA operand
 // layout: lane (k, i) = (lane / 16, lane % 16) gets A[16 tile + i][4 ks + k] in
-// x[ks], zero outside the matrix.  The loads read clamped indices, so they are
-// unconditional and issue together.
+// x[ks], zero outside the matrix (predicated loads: none for the padding
+// columns).
 template <int NB>
 __device__ inline void load_tile(const double* __restrict__ A, uint64_t n, uint32_t l, uint64_t tile,
                                  double (&x)[NB * 4]) {
   const int lane = threadIdx.x & 63, kr = lane >> 4, ci = lane & 15;
   const uint64_t row = tile * 16 + ci;
-  const uint64_t rc = row < n ? row : n - 1;
 #pragma unroll
   for (int ks = 0; ks < NB * 4; ++ks) {
     const uint32_t k = ks * 4 + kr;
-    const double v = A[rc * l + (k < l ? k : l - 1)];
-    x[ks] = (row < n && k < l) ? v : 0.0;
+    x[ks] = (row < n && k < l) ? A[row * l + k] : 0.0;
   }
 }
 
@@ -524,14 +522,8 @@ __device__ __attribute__((always_inline)) inline void apply_gram_pass(const doub
     double xn[AHEAD ? W / 4 : 1];
     if constexpr (AHEAD) {
       if (tile + 1 < t1) load_tile<NB>(A, n, l, tile + 1, xn);
-    } else {  // predicated loads (no clamped extra loads for the padding columns)
-      const int lane = threadIdx.x & 63;
-      const uint64_t row = tile * 16 + (lane & 15);
-#pragma unroll
-      for (int ks = 0; ks < W / 4; ++ks) {
-        const uint32_t k = ks * 4 + (lane >> 4);
-        x[ks] = (row < n && k < l) ? A[row * l + k] : 0.0;
-      }
+    } else {
+      load_tile<NB>(A, n, l, tile, x);
     }
     // Q tile = A tile Rinv: lane (k, i) supplies A[r0 + i][4 ks + k]
     f64x4 acc[NB];
