@@ -259,7 +259,6 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   }
   const uint64_t ldp = ldp_;
   omega_.reserve((uint64_t)N_ * l_);
-  omg2_.reserve((uint64_t)N_ * l_);
   colsum_.reserve(l_);
   bg_.reserve((uint64_t)bcols_ * kg_ * opsz_);
   bh_.reserve((uint64_t)nwin_ * bcols_ * kw_ * opsz_);
@@ -635,7 +634,6 @@ void Engine::draw_omega() {
 void Engine::init_omega() {
   const size_t nb = (size_t)N_ * l_ * 8;
   PCAONE_HIP(hipMemcpyAsync(omega_.p, omega0_.p, nb, hipMemcpyDeviceToDevice, stream_));
-  PCAONE_HIP(hipMemcpyAsync(omg2_.p, omega0_.p, nb, hipMemcpyDeviceToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(colsum_.p, colsum0_.p, (size_t)l_ * 8, hipMemcpyDeviceToDevice, stream_));
   timed("omega_operand",
         [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, lo_off_, kg_, kperm_(p_.precision),
@@ -665,7 +663,7 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
   q.Rf = Rf_.p;
   q.colsum = colsum_.p;
   q.omega = omega_.p;
-  q.omg2 = omg2_.p;
+  q.omg2 = omega_.p;  // the previous Omega for flipOmg is the one being replaced
   q.zero_acc = zero_acc;
   q.omega32 = (b.corrG && b.g2_f32) ? b.omega32 : nullptr;
   q.bg = bg_.p;

@@ -196,7 +196,7 @@ class Engine {
   uint32_t miss_rows_ = 0;
   double hf_ = 0.5;
   std::vector<double> omega_given_;
-  DevBuf<double> omega_, omg2_, colsum_, omega0_, colsum0_;
+  DevBuf<double> omega_, colsum_, omega0_, colsum0_;
   DevBuf<unsigned char> bg_, bh_;   // operands of the G and H contractions
   DevBuf<double> G_, S2_, s2p_, H1_, H2_, H_;
   DevBuf<unsigned char> g2_;

@@ -280,7 +280,8 @@ struct QrPass {
   double* Rf;
   double* colsum;    // l: column sums of the new Omega
   double* omega;     // n x l FP64
-  double* omg2;      // n x l: previous Omega in, new Omega out
+  double* omg2;      // n x l: previous Omega in (new Omega out unless it is `omega` itself:
+                     // the flip pass reads it before the write pass overwrites omega)
   double* zero_acc;  // optional n x l accumulator to clear
   float* omega32;    // optional n x l FP32 copy
   void* bg;          // contraction operand [bcols][kpad] of bprec

@@ -791,7 +791,7 @@ __global__ __launch_bounds__(256) void k_qr_write(QrPass q) {
         const double x = om[c2][v];
         const uint64_t gi = row * l + c;
         q.omega[gi] = x;
-        q.omg2[gi] = x;
+        if (q.omg2 != q.omega) q.omg2[gi] = x;
         if (q.zero_acc) q.zero_acc[gi] = 0.0;
         if (q.omega32) q.omega32[gi] = (float)x;
         if (q.bprec == PREC_SPLIT_BF16) {
