@@ -266,8 +266,9 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   G_.reserve((uint64_t)M_ * l_);
   g2_.reserve((uint64_t)M_ * l_ * (prec == PREC_FP64 ? 8 : 4));
   if (nnz_snp_ > 0) {
-    corrG_.reserve((uint64_t)M_ * l_);
-    corrH_.reserve((uint64_t)nwin_ * N_ * l_);
+    const uint64_t csz = prec == PREC_FP64 ? 8 : 4;  // the partials' type
+    corrG_.reserve((uint64_t)M_ * l_ * csz);
+    corrH_.reserve((uint64_t)nwin_ * N_ * l_ * csz);
     if (prec != PREC_FP64) omega32_.reserve((uint64_t)N_ * l_);
   }
   S2_.reserve((uint64_t)nwin_ * l_);
@@ -491,8 +492,8 @@ void Engine::gram_init(int precision, bool standardize, int ploidy) {
   mv_g2_.reserve((uint64_t)M_ * (precision == PREC_FP64 ? 8 : 4));
   b.g2_f32 = precision != PREC_FP64;
   if (nnz_snp_ > 0) {
-    mv_corrG_.reserve(M_);
-    mv_corrH_.reserve((uint64_t)nwin_ * N_);
+    mv_corrG_.reserve((uint64_t)M_ * partsz);
+    mv_corrH_.reserve((uint64_t)nwin_ * N_ * partsz);
     if (b.g2_f32) mv_omega32_.reserve(N_);
     b.corrG = mv_corrG_.p;
     b.corrH = mv_corrH_.p;

@@ -67,7 +67,7 @@ struct SegBufs {
   bool affine = false;  // the packed split-BF16 contraction runs on 2 - 2 y
   void* g2 = nullptr;        // 2 F d G, float when g2_f32
   bool g2_f32 = false;
-  double *corrG = nullptr, *corrH = nullptr;  // missing-entry sums (null: nothing missing)
+  void *corrG = nullptr, *corrH = nullptr;  // missing-entry sums in the partials' type (null: nothing missing)
   unsigned char *bg = nullptr, *bh = nullptr, *gpart = nullptr, *hpart = nullptr;
   double *Ha = nullptr, *Hb = nullptr;
   const double* mval = nullptr;
@@ -201,7 +201,7 @@ class Engine {
   DevBuf<double> G_, S2_, s2p_, H1_, H2_, H_;
   DevBuf<unsigned char> g2_;
   DevBuf<float> omega32_;
-  DevBuf<double> corrG_, corrH_;    // [M][l] and [nwin][N][l] missing-entry sums
+  DevBuf<unsigned char> corrG_, corrH_;  // [M][l] and [nwin][N][l] missing-entry sums (partials' type)
   DevBuf<unsigned char> gpart_, hpart_;
   uint64_t bandsize_ = 1;
   // RsvdOpData::update / U, S, V of the last computeUSV (src/Halko.hpp:11-14)
@@ -220,7 +220,8 @@ class Engine {
   bool have_mv_ = false;
   int64_t nops_ = 0;
   SegBufs mv_;
-  DevBuf<double> mv_omega_, mv_colsum_, mv_G_, mv_S2_, mv_s2p_, mv_H_, mv_corrG_, mv_corrH_;
+  DevBuf<double> mv_omega_, mv_colsum_, mv_G_, mv_S2_, mv_s2p_, mv_H_;
+  DevBuf<unsigned char> mv_corrG_, mv_corrH_;
   DevBuf<unsigned char> mv_g2_;
   DevBuf<float> mv_omega32_;
   DevBuf<unsigned char> mv_bg_, mv_bh_, mv_gpart_, mv_hpart_;

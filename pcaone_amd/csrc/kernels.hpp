@@ -95,7 +95,8 @@ struct FinalizeG {
   const int64_t* miss_ptr;  // [M+1]  (EMU values only)
   const int32_t* miss_idx;  // sample indices
   const double* omega;  // [N][l]
-  const double* corr;   // [M][l] sum of Omega over each row's missing samples, or null (none missing)
+  const void* corr;     // [M][l] sum of Omega over each row's missing samples, in the partials' type, or
+                        // null (none missing)
   double* G;            // [M][l]
   void* g2;             // [M][l]  2 f d G, float when g2_f32 else double
   int g2_f32;
@@ -152,7 +153,8 @@ struct FinalizeH {
   int affine;           // part holds the 2 - 2 y contraction: X^T g' = sum g' - part / 2
   const int64_t* miss_ptr;  // [nwin*miss_rows+1]  (EMU values only)
   const int32_t* miss_idx;  // SNP rows
-  const double* corr;   // [nwin][nsamples][l] sum of g2 over each (window, sample) missing list, or null
+  const void* corr;     // [nwin][nsamples][l] sum of g2 over each (window, sample) missing list, in the
+                        // partials' type, or null
   // EMU: H_j += sum over missing (i, j) of val * d_i G_i
   const double* mval;   // SNP-CSR values or null
   const int64_t* mpos;  // window entry -> SNP-CSR position
@@ -163,8 +165,8 @@ struct FinalizeH {
 void launch_finalize_H(const FinalizeH& f, hipStream_t s);
 
 // Sparse corrections for the missing genotypes: for each list row o of a
-// CSR (ptr, idx), out[o][c] = sum_p table[idx[p]][c] (summed in the table's
-// type, FP32 or FP64, and stored as FP64).  List rows come in ngroups groups of `per` rows: row o reads
+// CSR (ptr, idx), out[o][c] = sum_p table[idx[p]][c] (summed and stored in
+// the table's type, FP32 or FP64).  List rows come in ngroups groups of `per` rows: row o reads
 // the list at in_base + (o / per) * stride_in + o % per and writes out row
 // out_base + o.
 struct CsrGather {
@@ -175,7 +177,7 @@ struct CsrGather {
   uint32_t l;
   uint64_t in_base, stride_in, out_base;
   uint32_t per, ngroups;
-  double* out;
+  void* out;  // [.][l] of the table's type
 };
 void launch_csr_gather(const CsrGather& g, hipStream_t s);
 // FP64 -> FP32 copy

@@ -751,7 +751,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
   const uint64_t nrow = f.r1 - f.r0;
   const P* __restrict__ part = reinterpret_cast<const P*>(f.part);
   const double* __restrict__ colsum = f.colsum;
-  const double* __restrict__ corrp = f.corr;
+  const P* __restrict__ corrp = reinterpret_cast<const P*>(f.corr);
   const double* __restrict__ F = f.F;
   const double* __restrict__ d = f.d;
   const uint64_t sstride = nrow * f.ldp;
@@ -760,7 +760,8 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
   // clamped into range, and its value is dropped afterwards when out of range:
   // the loads then issue together instead of one branch at a time.
   uint32_t qe[kFinEl], ce[kFinEl], off[kFinEl];
-  double raw[kFinEl], cs[kFinEl], corr[kFinEl], Fi[kFinEl], di[kFinEl];
+  double raw[kFinEl], cs[kFinEl], Fi[kFinEl], di[kFinEl];
+  P corr[kFinEl];  // widened where used
   const uint32_t rlast = f.r1 - 1;
 #pragma unroll
   for (int u = 0; u < kFinEl; ++u) {
@@ -780,7 +781,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
     for (int u = 0; u < kFinEl; ++u) corr[u] = corrp[(uint64_t)min(cb + qe[u], rlast) * l + ce[u]];
   } else {
 #pragma unroll
-    for (int u = 0; u < kFinEl; ++u) corr[u] = 0.0;
+    for (int u = 0; u < kFinEl; ++u) corr[u] = P(0);
   }
   const uint32_t lo = f.split_cols ? l : 0;
   for (uint32_t s0 = 0; s0 < f.nsplit; s0 += 4) {
@@ -810,7 +811,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
       double unused = 0.0;
       miss_sums(f.miss_ptr, f.miss_idx, r, f.omega, l, c, f.mval, unused, corr_e);
     }
-    const double Gv = di[u] * (f.hf * (rv - 2.0 * Fi[u] * (cs[u] - corr[u])) + corr_e);
+    const double Gv = di[u] * (f.hf * (rv - 2.0 * Fi[u] * (cs[u] - (double)corr[u])) + corr_e);
     f.G[(uint64_t)r * l + c] = Gv;
     const double gp = di[u] * Gv;
     const double g2 = 2.0 * Fi[u] * gp;
@@ -933,19 +934,26 @@ __global__ __launch_bounds__(256) void k_finalize_H(FinalizeH f) {
   if (gid >= (uint64_t)f.nsamples * f.l) return;
   const uint32_t j = (uint32_t)(gid / f.l), c = (uint32_t)(gid % f.l);
   const P* __restrict__ part = reinterpret_cast<const P*>(f.part);
-  const double* __restrict__ corrp = f.corr;
+  const P* __restrict__ corrp = reinterpret_cast<const P*>(f.corr);
   const double* __restrict__ S2 = f.S2;
   const uint64_t lo = f.split_cols ? f.l : 0;
   double ha = 0.0, hb = 0.0;
   bool any_a = false, any_b = false;
   for (uint32_t w0 = f.w0; w0 < f.w1; w0 += 4) {
-    double raw[4], corr[4], s2[4];
+    double raw[4], s2[4];
+    P corr[4];  // widened where used: a conversion here would wait for each load in turn
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t w = min(w0 + u, f.w1 - 1);
       raw[u] = 0.0;
-      corr[u] = corrp ? corrp[((uint64_t)w * f.nsamples + j) * f.l + c] : 0.0;
       s2[u] = S2[(uint64_t)w * f.l + c];
+    }
+    if (corrp) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) corr[u] = corrp[((uint64_t)min(w0 + u, f.w1 - 1) * f.nsamples + j) * f.l + c];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) corr[u] = P(0);
     }
     for (uint32_t s0 = 0; s0 < f.nsplit; s0 += 4) {
       P x[4][4][2];
@@ -979,7 +987,7 @@ __global__ __launch_bounds__(256) void k_finalize_H(FinalizeH f) {
           corr_e += f.mval[f.mpos[p]] * f.d[i] * f.G[i * f.l + c];
         }
       }
-      const double h = f.hf * (rv - (s2[u] - corr[u])) + corr_e;
+      const double h = f.hf * (rv - (s2[u] - (double)corr[u])) + corr_e;
       if ((f.route_b >> (w - f.w0)) & 1ull) {
         hb += h;
         any_b = true;
@@ -1061,8 +1069,9 @@ void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin,
 // (lpg = ceil(l / V) lanes, V consecutive columns per lane read as one vector
 // load) takes entries grp, grp + ng, ..., two per round.  The sums run in the
 // table's own type (FP32 tables: FP32 sums of FP32 values, a relative error
-// of a few 1e-7 on each correction; FP64 tables: FP64), and the groups' sums
-// combine in a fixed order.  Measured on the 500,000-row missing lists of the
+// of a few 1e-7 on each correction; FP64 tables: FP64), the groups' sums
+// combine in a fixed order, and the row is stored in that type (one vector
+// store per lane), which is the type the epilogues' partials have.  Measured on the 500,000-row missing lists of the
 // 10,000-sample workload: the FP32 sums (half the shuffles, fewer registers,
 // more waves resident), two entries per round and the grid stride take a
 // launch from 387 to 257 us.
@@ -1114,11 +1123,11 @@ __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
     for (int gg = 1; gg < ng; ++gg)
 #pragma unroll
       for (int v = 0; v < V; ++v) t[v] += __shfl(s[v], gg * lpg + li);
-    if (grp == 0 && act) {
-      const uint64_t orow = g.out_base + o;
+    if (grp == 0 && act) {  // act: the lane's V columns are inside the row (l % V == 0)
+      vec y;
 #pragma unroll
-      for (int v = 0; v < V; ++v)
-        if ((uint32_t)(li * V + v) < g.l) g.out[orow * g.l + li * V + v] = (double)t[v];
+      for (int v = 0; v < V; ++v) y[v] = t[v];
+      *reinterpret_cast<vec*>(reinterpret_cast<T*>(g.out) + (g.out_base + o) * g.l + coff) = y;
     }
   }
 }
