@@ -131,6 +131,15 @@ int pcaone_gpu_rsvd_em(pcaone_gpu* h, int maxp, double tol, int maxiter, double 
  * BEAGLE body, per_site = 3; or only the first two, per_site = 2).  Only the
  * first two stay in HBM; the third is taken as 1 - P0 - P1. */
 int pcaone_gpu_load_gl(pcaone_gpu* h, const float* gl, uint32_t nsamples, uint32_t nsites, int per_site);
+/* --pcangsd on in-core PLINK input (FileBed::read_all, src/FilePlink.cpp:68,
+ * 89-114): the genotypes of pcaone_gpu_load_bed become the hard-call
+ * likelihoods RR (1, 0), AA (0, 0), RA (0, 1), missing (0.333333, 0.333333);
+ * F stays the genotype-count allele frequency (no emMAF_with_GL for PLINK,
+ * src/FilePlink.cpp:127-130).  pcaone_gpu_rsvd_init then builds the initial E
+ * as the centred genotype BED2GENO - F (missing 0) and pcaone_gpu_pcangsd runs
+ * the EM (pass Dc = C = NULL: the GRM step is for BEAGLE input only,
+ * src/Halko.cpp:317). */
+int pcaone_gpu_gl_from_bed(pcaone_gpu* h);
 /* emMAF_with_GL (src/Utils.cpp:501-531) from F = 0.25 (src/FileBeagle.cpp:127)
  * until the RMS change of F is below tol (at most maxiter passes).  F (nsites,
  * optional) and iters (optional) receive the result.  Required before

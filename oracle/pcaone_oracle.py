@@ -26,6 +26,8 @@ the reference's behaviour, of:
 * emMAF_with_GL, initial E, PCAngsd update, pcangsd_standardize_E, GRM
                                                (src/Utils.cpp:501-531,
   src/FileBeagle.cpp:121-143, src/Data.cpp:249-343, src/Halko.cpp:317-331)
+* PCAngsd on PLINK hard calls                  (src/FilePlink.cpp:89-104,
+  src/Halko.cpp:294-316)
 * LD windows and r^2                           (src/LD.cpp:20-23, 125-139,
   419-442)
 
@@ -560,6 +562,39 @@ def pcangsd_halko(P0, P1, k: int, os: int, maxp: int, tol: float, maxiter: int =
     C = E @ E.T / m
     C[np.diag_indices(n)] = Dc / m
     return op, F, it, diffs, Dc, C
+
+
+def plink_hard_call_gl(codes: np.ndarray):
+    """The two GLs FileBed::read_all keeps per sample for --pcangsd on PLINK
+    input (src/FilePlink.cpp:89-104): code 3 (RR) (1, 0), code 0 (AA) (0, 0),
+    code 2 (RA) (0, 1), missing (0.333333, 0.333333).  Returns (P0, P1), (N, M)."""
+    conds = [codes == 3, codes == 0, codes == 2]
+    P0 = np.select(conds, [1.0, 0.0, 0.0], 0.333333)
+    P1 = np.select(conds, [0.0, 0.0, 1.0], 0.333333)
+    return np.ascontiguousarray(P0.T), np.ascontiguousarray(P1.T)
+
+
+def pcangsd_plink(codes: np.ndarray, k: int, os: int, maxp: int, tol: float, maxiter: int = 100,
+                  tolem: float = 1e-5, seed: int = 101, rand: int = 1):
+    """run_pca_with_halko's EM branch for --pcangsd on in-core PLINK input with
+    the plain RSVD (src/Halko.cpp:294-316): the first solve on the centred
+    genotypes (src/FilePlink.cpp:106-114), then E <- fit_with_pi's PCAngsd
+    update (src/Data.cpp:249-270) on the hard-call GLs with the PLINK allele
+    frequencies, until 1 - mev(V, Vpre) < tolem.  No GRM: that step is for
+    BEAGLE input only (src/Halko.cpp:317).  Returns (op, F, iterations, diffs)."""
+    F, _ = allele_freq(codes)
+    P0, P1 = plink_hard_call_gl(codes)
+    op = RsvdOracle(centered_matrix(codes, F), k, os, fancy=False, seed=seed, rand=rand)
+    op.compute_usv(maxp, tol)
+    diffs, it = [], 0
+    for it in range(1, maxiter + 1):
+        vpre = op.V.copy()
+        op.X = pcangsd_E(P0, P1, F, op.U, op.S, op.V)
+        op.compute_usv(maxp, tol)
+        diffs.append(1.0 - mev(op.V, vpre))
+        if diffs[-1] < tolem:
+            break
+    return op, F, it, diffs
 
 
 # ----------------------------------------------------------------------------

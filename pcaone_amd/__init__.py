@@ -61,6 +61,7 @@ SIGNATURES = {
     "pcaone_write_eigs": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, _D, _D, _D, C.c_int]),
     "pcaone_gpu_load_gl": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]),
     "pcaone_gpu_gl_em_maf": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, C.POINTER(C.c_int)]),
+    "pcaone_gpu_gl_from_bed": (C.c_int, [C.c_void_p]),
     "pcaone_gpu_pcangsd_update": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_int, C.c_int, C.POINTER(C.c_float), _D]),
     "pcaone_gpu_pcangsd": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, _D, _D, _D,
                                      C.POINTER(C.c_int), _D, _D, _D]),
@@ -296,6 +297,14 @@ class Gpu:
         self._check(self._L.pcaone_gpu_load_gl(self.h, _p(a, C.c_float), n, m, per))
         self.N, self.M = n, m
 
+    def gl_from_bed(self):
+        """--pcangsd on the loaded PLINK set: its codes become hard-call
+        likelihoods, F the genotype-count frequencies (returned), and
+        rsvd_init then builds the centred-genotype initial E."""
+        F = self.allele_freq()[0]
+        self._check(self._L.pcaone_gpu_gl_from_bed(self.h))
+        return F
+
     def gl_em_maf(self, maxiter: int = 100, tol: float = 1e-6):
         """emMAF_with_GL; returns (F, iterations)."""
         F = np.empty(self.M)
@@ -317,18 +326,22 @@ class Gpu:
                                                        _p(Dc, C.c_double) if Dc is not None else None))
         return E, Dc
 
-    def pcangsd(self, maxp: int, tol: float, maxiter: int = 100, tolem: float = 1e-5, grm: bool = True):
-        """PCAngsd EM (BEAGLE input); returns U, S, V, iterations, diffs, Dc, C (or None)."""
+    def pcangsd(self, maxp: int, tol: float, maxiter: int = 100, tolem: float = 1e-5, grm: bool = True,
+                dc: bool = True):
+        """PCAngsd EM; returns U, S, V, iterations, diffs, Dc (or None), C (or None).
+        BEAGLE input: dc and grm give pcangsd_standardize_E's Dc and the GRM;
+        PLINK input (gl_from_bed) passes dc=False, grm=False as the reference
+        stops after the EM there."""
         U = np.empty((self.N, self.k), order="F")
         S = np.empty(self.k)
         V = np.empty((self.M, self.k), order="F")
         it = C.c_int(0)
         diffs = np.zeros(max(maxiter, 1))
-        Dc = np.empty(self.N)
+        Dc = np.empty(self.N) if dc else None
         Cm = np.empty((self.N, self.N)) if grm else None
         self._check(self._L.pcaone_gpu_pcangsd(self.h, maxp, tol, maxiter, tolem, _p(U, C.c_double),
                                                 _p(S, C.c_double), _p(V, C.c_double), C.byref(it),
-                                                _p(diffs, C.c_double), _p(Dc, C.c_double),
+                                                _p(diffs, C.c_double), _p(Dc, C.c_double) if dc else None,
                                                 _p(Cm, C.c_double) if grm else None))
         return U, S, V, it.value, diffs[:it.value], Dc, Cm
 

@@ -155,6 +155,10 @@ int pcaone_gpu_load_gl(pcaone_gpu* h, const float* gl, uint32_t nsamples, uint32
   return guarded(h, [&](pcaone::Engine& e) { e.load_gl(gl, nsamples, nsites, per_site); });
 }
 
+int pcaone_gpu_gl_from_bed(pcaone_gpu* h) {
+  return guarded(h, [&](pcaone::Engine& e) { e.gl_from_bed(); });
+}
+
 int pcaone_gpu_gl_em_maf(pcaone_gpu* h, int maxiter, double tol, double* F, int* iters) {
   return guarded(h, [&](pcaone::Engine& e) { e.gl_em_maf(maxiter, tol, F, iters); });
 }

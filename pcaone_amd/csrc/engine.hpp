@@ -86,6 +86,8 @@ class Engine {
   void load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfeatures);
   // genotype likelihoods (PCAngsd): the E matrix is built in HBM from them
   void load_gl(const float* gl, uint32_t nsamples, uint32_t nsites, int per_site);
+  // PCAngsd on PLINK input: the loaded genotypes become hard-call likelihoods
+  void gl_from_bed();
   void gl_em_maf(int maxiter, double tol, double* F, int* iters);
   void pcangsd_update(const double* U, const double* S, const double* V, int k, int standardize, float* E,
                       double* Dc);
@@ -163,6 +165,7 @@ class Engine {
   std::vector<int64_t> counts_orig_;
   // genotype likelihoods: P [site][sample][2] FP32, F from emMAF_with_GL
   bool gl_mode_ = false, have_glF_ = false;
+  bool gl_bed_ = false;  // likelihoods are the hard calls of a loaded PLINK set
   DevBuf<float> glP_;
   DevBuf<double> glF_, gl_part_, gl_Dc_;
   std::vector<double> glF_host_;

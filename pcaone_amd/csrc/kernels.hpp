@@ -227,12 +227,15 @@ struct GlE {
   const double* V;       // [nsites][k], laid-out rows
   uint32_t k;
   int standardize;       // divide by sqrt(2F(1-F)) and emit Dc partials
+  int bed_init;          // U == null and P holds PLINK hard calls: E = dosage / 2 - F, missing 0
   float* out;            // [laid-out row][ld]
   uint64_t ld;
   uint32_t rows_per_block;
   double* dpart;         // [gridDim.y][nsamples] Dc partials, or null
 };
 void launch_gl_E(const GlE& e, hipStream_t s);
+// PLINK hard calls -> the two likelihoods per sample, P [site][sample][2] FP32
+void launch_bed_to_gl(const uint8_t* raw, uint32_t nsamples, uint32_t nsites, float* P, hipStream_t s);
 // rows_per_block for launch_gl_E: at most 256 row blocks (Dc partials)
 uint32_t gl_rows_per_block(uint32_t nsites);
 // out[i] = sum_y part[y][i] (fixed order)

@@ -11,6 +11,11 @@
 //                   E_ij = posterior(pt) - 2 F_j                 (src/Data.cpp:249-270)
 //   standardise:    E_ij / sqrt(2 F (1 - F)) and the GRM diagonal
 //                   Dc_i = sum_j sum_g (g - 2F)^2 post_g / (2F(1-F))  (src/Data.cpp:301-343)
+//
+// PLINK input with --pcangsd: the likelihoods are the hard calls of the
+// packed codes (src/FilePlink.cpp:89-104), F the genotype-count frequency and
+// the initial E the centred genotype BED2GENO - F, missing 0
+// (src/FilePlink.cpp:106-114), in half-dosage units as the reference has it.
 #include <algorithm>
 
 #include "common.hpp"
@@ -58,6 +63,31 @@ void launch_gl_em_maf(const float* P, uint32_t nsamples, uint32_t nsites, const 
 
 uint32_t gl_em_maf_blocks(uint32_t nsites) { return (uint32_t)ceil_div(nsites, 4); }
 
+// FileBed::read_all's two likelihoods per sample for --pcangsd
+// (src/FilePlink.cpp:89-104): code 3 (RR) (1, 0), code 0 (AA) (0, 0), code 2
+// (RA) (0, 1), missing (0.333333, 0.333333).  raw: nsites rows of
+// ceil(nsamples / 4) bytes, sample j in bits 2 (j % 4) of byte j / 4.
+__global__ __launch_bounds__(256) void k_bed_to_gl(const uint8_t* __restrict__ raw, uint32_t nsamples,
+                                                   uint32_t nsites, float2* __restrict__ P) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)nsamples * nsites) return;
+  const uint64_t site = gid / nsamples;
+  const uint32_t j = (uint32_t)(gid - site * nsamples);
+  const uint32_t nb = (nsamples + 3) >> 2;
+  const uint32_t code = (raw[site * nb + (j >> 2)] >> (2 * (j & 3))) & 3u;
+  float2 p;
+  p.x = code == 3 ? 1.0f : (code == 1 ? 0.333333f : 0.0f);
+  p.y = code == 2 ? 1.0f : (code == 1 ? 0.333333f : 0.0f);
+  P[gid] = p;
+}
+
+void launch_bed_to_gl(const uint8_t* raw, uint32_t nsamples, uint32_t nsites, float* P, hipStream_t s) {
+  const uint64_t total = (uint64_t)nsamples * nsites;
+  if (!total) return;
+  hipLaunchKernelGGL(k_bed_to_gl, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, raw, nsamples, nsites,
+                     reinterpret_cast<float2*>(P));
+}
+
 // E rows (laid-out row r holds site perm[r]) into out[r * ld + i]; with U == null
 // the prior is pt = F (initial E), else the PCAngsd update from U (N x k),
 // S (k), V (M x k, laid-out rows).  Optional standardisation and Dc partials
@@ -88,6 +118,11 @@ __global__ __launch_bounds__(256) void k_gl_E(GlE e) {
       double p0, p1, p2;
       const double post = gl_post(pp.x, pp.y, pt, p0, p1, p2);
       double v = post - 2.0 * F;
+      if (!e.U && e.bed_init) {
+        // hard calls: dosage P1 + 2 (1 - P0 - P1); missing (both 0.333333) is 0
+        const bool miss = pp.x > 0.3f && pp.y > 0.3f;
+        v = miss ? 0.0 : 0.5 * ((double)pp.y + 2.0 * (1.0 - (double)pp.x - (double)pp.y)) - F;
+      }
       if (e.standardize) {
         const double norm = sqrt(2.0 * F * (1.0 - F));
         if (norm > 1e-9) v /= norm;

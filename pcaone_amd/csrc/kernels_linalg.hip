@@ -141,16 +141,20 @@ void launch_rows_to_cols(const double* A, uint64_t n, uint32_t kc, const int32_t
 
 // Cyclic Jacobi with round-robin pair ordering: each round applies n/2
 // disjoint rotations at once (Numerical-Recipes angle).  A thread owns 2 x 2
-// blocks {p1, q1} x {p2, q2} of the matrix (rows, then columns rotated in
-// place) or row pairs of the eigenvector matrix; a round needs two barriers,
-// and each thread's items load everything they read before the first store.
-// LDS: a, v and tmp, W x W each (W = 16 NB, zero padded past l, so an odd
+// blocks {p1, q1} x {p2, q2} of the matrix (rows, then columns rotated from
+// one buffer into the other) or row pairs of the eigenvector matrix (in
+// place).  Every wave computes the round's angles itself, so a round needs
+// one barrier.  LDS: a, v and tmp, W x W each (W = 16 NB, zero padded past l, so an odd
 // l's dummy index l is an empty row and column and its rotation the identity)
 // with the odd row stride W + 1, so a wave's walk down a column (the
 // eigenvector items) or along a row touches distinct banks.
 __device__ inline void rr_pair(int t, int round, int n, int& p, int& q) {
-  // position k of the round-robin table: 0 stays, the rest rotate
-  auto pos = [&](int k) { return k == 0 ? 0 : 1 + (k - 1 + round) % (n - 1); };
+  // position k of the round-robin table: 0 stays, the rest rotate; k - 1 and
+  // round are both below n - 1, so the wrap is one conditional subtraction
+  auto pos = [&](int k) {
+    const int x = k - 1 + round;
+    return k == 0 ? 0 : 1 + (x >= n - 1 ? x - (n - 1) : x);
+  };
   p = pos(t);
   q = pos(n - 1 - t);
   if (p > q) {
@@ -208,8 +212,7 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restric
   double* a = reinterpret_cast<double*>(smem_raw);
   double* v = a + WD * LD;
   double* tmp = v + WD * LD;
-  __shared__ double cs[8 * NB][2];
-  __shared__ int pr[8 * NB][2];
+  __shared__ __attribute__((aligned(16))) double csw[kEigThreads / 64][8 * NB][2];  // per wave: (c, s) of each pair
   __shared__ double red[kEigThreads / 64][2];
   __shared__ int order[WD];
   const int n = (L % 2) ? L + 1 : L;  // dummy index L when odd (a zero row of the padding)
@@ -248,6 +251,11 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restric
     }
     __syncthreads();
   }
+  // the rounds write the next matrix into tmp (an) and swap; what they never
+  // write there (past the n x n block) stays zero, as in a
+  double* an = tmp;
+  for (int e = t; e < WD * LD; e += kEigThreads) an[e] = 0.0;
+  __syncthreads();
   for (int sweep = 0; sweep < 40; ++sweep) {
     double off = 0.0, dg = 0.0;
     for (int e = t; e < WD * WD; e += kEigThreads) {
@@ -273,38 +281,16 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restric
     // off-diagonal Frobenius norm below 1e-13 of the diagonal's: FP64 rounding level
     if (offs <= 1e-26 * dgs || offs == 0.0) break;
     for (int round = 0; round < n - 1; ++round) {
-      if (t < h) {
-        int p, q;
-        rr_pair(t, round, n, p, q);
-        double c = 1.0, s = 0.0;
-        const double apq = a[p * LD + q];
-        if (q < L && fabs(apq) > 1e-300) {
-          const double theta = (a[q * LD + q] - a[p * LD + p]) * rcp_nr(2.0 * apq);
-          const double th2 = fma(theta, theta, 1.0);
-          // sqrt(theta^2 + 1) as x rsqrt(x); past 1e150 the rotation is the identity
-          const double root = th2 < 1e300 ? th2 * rsqrt_nr(th2) : 1e300;
-          double tt = rcp_nr(fabs(theta) + root);
-          if (theta < 0.0) tt = -tt;
-          c = rsqrt_nr(fma(tt, tt, 1.0));
-          s = tt * c;
-        }
-        pr[t][0] = p;
-        pr[t][1] = q;
-        cs[t][0] = c;
-        cs[t][1] = s;
-      }
-      __syncthreads();  // also orders the convergence reads of red before its next write
+      // Items read this round's a and write the next (an): every read of the
+      // round, the angle inputs included, sees the matrix the round started
+      // with, so one barrier per round suffices.
       int i0[U], i1[U], i2[U], i3[U];
-      double c1[U], s1[U], c2[U], s2[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int kind = asg[u] >> 16, hi = (asg[u] >> 8) & 255, lo = asg[u] & 255;
-        const int k1 = hi, k2 = kind == 1 ? lo : 0;
-        const int p1 = pr[k1][0], q1 = pr[k1][1], p2 = pr[k2][0], q2 = pr[k2][1];
-        c1[u] = cs[k1][0];
-        s1[u] = cs[k1][1];
-        c2[u] = cs[k2][0];
-        s2[u] = cs[k2][1];
+        int p1, q1, p2, q2;
+        rr_pair(hi, round, n, p1, q1);
+        rr_pair(kind == 1 ? lo : 0, round, n, p2, q2);
         if (kind == 1) {  // a entries (p1, p2), (q1, p2), (p1, q2), (q1, q2)
           i0[u] = p1 * LD + p2;
           i1[u] = q1 * LD + p2;
@@ -328,23 +314,51 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restric
         x1[u] = m[i2[u]];
         y1[u] = m[i3[u]];
       }
+      // every wave computes the round's h angles itself (lanes 0 .. h-1) and
+      // hands them to its other lanes through its own LDS slots: LDS accesses
+      // of one wave complete in order, so no barrier is needed
+      if (lane < h) {
+        int p, q;
+        rr_pair(lane, round, n, p, q);
+        double c = 1.0, sn = 0.0;
+        const double apq = a[p * LD + q];
+        if (q < L && fabs(apq) > 1e-300) {
+          const double theta = (a[q * LD + q] - a[p * LD + p]) * rcp_nr(2.0 * apq);
+          const double th2 = fma(theta, theta, 1.0);
+          // sqrt(theta^2 + 1) as x rsqrt(x); past 1e150 the rotation is the identity
+          const double root = th2 < 1e300 ? th2 * rsqrt_nr(th2) : 1e300;
+          double tt = rcp_nr(fabs(theta) + root);
+          if (theta < 0.0) tt = -tt;
+          c = rsqrt_nr(fma(tt, tt, 1.0));
+          sn = tt * c;
+        }
+        csw[wv][lane][0] = c;
+        csw[wv][lane][1] = sn;
+      }
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int kind = asg[u] >> 16;
+        const int kind = asg[u] >> 16, hi = (asg[u] >> 8) & 255, lo = asg[u] & 255;
+        if (kind == 0) continue;
+        const double c1 = csw[wv][hi][0], s1 = csw[wv][hi][1];
         if (kind == 1) {
+          const double c2 = csw[wv][lo][0], s2 = csw[wv][lo][1];
           // rows: (p1, q1) <- (c x - s y, s x + c y), then columns (p2, q2)
-          const double rp0 = c1[u] * x0[u] - s1[u] * y0[u], rq0 = s1[u] * x0[u] + c1[u] * y0[u];
-          const double rp1 = c1[u] * x1[u] - s1[u] * y1[u], rq1 = s1[u] * x1[u] + c1[u] * y1[u];
-          a[i0[u]] = c2[u] * rp0 - s2[u] * rp1;
-          a[i2[u]] = s2[u] * rp0 + c2[u] * rp1;
-          a[i1[u]] = c2[u] * rq0 - s2[u] * rq1;
-          a[i3[u]] = s2[u] * rq0 + c2[u] * rq1;
-        } else if (kind == 2) {
-          v[i0[u]] = c1[u] * x0[u] - s1[u] * y0[u];
-          v[i1[u]] = s1[u] * x0[u] + c1[u] * y0[u];
+          const double rp0 = c1 * x0[u] - s1 * y0[u], rq0 = s1 * x0[u] + c1 * y0[u];
+          const double rp1 = c1 * x1[u] - s1 * y1[u], rq1 = s1 * x1[u] + c1 * y1[u];
+          an[i0[u]] = c2 * rp0 - s2 * rp1;
+          an[i2[u]] = s2 * rp0 + c2 * rp1;
+          an[i1[u]] = c2 * rq0 - s2 * rq1;
+          an[i3[u]] = s2 * rq0 + c2 * rq1;
+        } else {
+          v[i0[u]] = c1 * x0[u] - s1 * y0[u];
+          v[i1[u]] = s1 * x0[u] + c1 * y0[u];
         }
       }
-      __syncthreads();
+      __syncthreads();  // also orders the convergence reads of red before its next write
+      double* const sw = a;
+      a = an;
+      an = sw;
     }
   }
   // descending order of the eigenvalues (ties by index): rank of each diagonal entry

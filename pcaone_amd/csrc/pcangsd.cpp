@@ -40,7 +40,30 @@ void Engine::load_gl(const float* gl, uint32_t nsamples, uint32_t nsites, int pe
   have_bed_ = true;
   src_ = SRC_DENSE;
   gl_mode_ = true;
-  have_glF_ = false;
+  have_glF_ = gl_bed_ = false;
+}
+
+// --pcangsd on in-core PLINK input (src/FilePlink.cpp:68, 89-114;
+// src/Data.cpp:249-270): the likelihoods are the hard calls of the loaded
+// codes, F stays the genotype-count frequency (the reference's emMAF_with_GL
+// call is commented out there, src/FilePlink.cpp:127-130) and the initial E
+// is the centred genotype.  The operator then runs on E as for BEAGLE input.
+void Engine::gl_from_bed() {
+  PCAONE_REQUIRE(have_bed_ && src_ == SRC_BED, -3, "gl_from_bed: load PLINK genotypes first");
+  PCAONE_HIP(hipSetDevice(device_));
+  have_op_ = have_mv_ = false;
+  glP_.reserve((uint64_t)N_ * M_ * 2);
+  timed("gl_from_bed", [&] { launch_bed_to_gl(raw_.p, N_, M_, glP_.p, stream_); });
+  glF_.reserve(M_);
+  glF_host_ = F_orig_;
+  PCAONE_HIP(hipMemcpyAsync(glF_.p, glF_host_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  sync();
+  // the operator sees E uncentred: mean 0, F 0 for the epilogues
+  mu_orig_.assign(M_, 0.0);
+  F_orig_.assign(M_, 0.0);
+  counts_orig_.clear();
+  src_ = SRC_DENSE;
+  gl_mode_ = have_glF_ = gl_bed_ = true;
 }
 
 // emMAF_with_GL (src/Utils.cpp:501-531) from F = 0.25 (src/FileBeagle.cpp:127):
@@ -96,6 +119,7 @@ void Engine::gl_fill_E(const double* U, const double* S, const double* V, uint32
   e.V = V;
   e.k = k;
   e.standardize = standardize ? 1 : 0;
+  e.bed_init = (gl_bed_ && !U) ? 1 : 0;
   e.out = dns_.p;
   e.ld = kg_;
   e.rows_per_block = gl_rows_per_block(M_);

@@ -13,14 +13,14 @@ def parse_args(args) -> dict:
     (src/Cmd.hpp:28-50) and oversamples = max(os, k) (src/Cmd.cpp:190)."""
     a = shlex.split(str(args))
     d = {"svd": 2, "k": 10, "os": 10, "p": 20, "tol": 1e-4, "seed": 101, "rand": 1, "w": 64, "S": False,
-         "scale": 1, "m": 0.0, "emu": False}
+         "scale": 1, "m": 0.0, "emu": False, "pcangsd": False}
     i = 0
     flags = {"--svd": ("svd", int), "-k": ("k", int), "--oversamples": ("os", int), "-p": ("p", int),
              "--tol-rsvd": ("tol", float), "--seed": ("seed", int), "--rand": ("rand", int), "-w": ("w", int),
              "--scale": ("scale", int), "-m": ("m", float)}
     while i < len(a):
-        if a[i] in ("-S", "--emu"):
-            d["S" if a[i] == "-S" else "emu"] = True
+        if a[i] in ("-S", "--emu", "--pcangsd"):
+            d[{"-S": "S", "--emu": "emu", "--pcangsd": "pcangsd"}[a[i]]] = True
             i += 1
         else:
             key, typ = flags[a[i]]
@@ -52,6 +52,10 @@ def oracle_run(z: dict):
         orc = O.RsvdOracle(X, d["k"], d["os"], fancy=d["svd"] == 2, bands=d["w"], seed=d["seed"], rand=d["rand"],
                            perm=not d["S"])
         return orc, orc.compute_usv(d["p"], d["tol"]), d
+    if d["pcangsd"]:  # PCAngsd on PLINK hard calls: the op of the last EM solve
+        codes = O.decode_codes(z["payload"], n, m)
+        orc, *_ = O.pcangsd_plink(codes, d["k"], d["os"], d["p"], d["tol"], seed=d["seed"], rand=d["rand"])
+        return orc, None, d
     if d["emu"]:
         codes = O.decode_codes(z["payload"], n, m)
         orc, _, _ = O.emu_halko(codes, d["k"], d["os"], d["p"], d["tol"], seed=d["seed"], rand=d["rand"])

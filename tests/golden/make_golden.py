@@ -3,7 +3,7 @@
 Run in the build container only (it needs oracle/_ref/PCAone, built from the
 reference sources by `make -f oracle/ref.mk`):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 
 Inputs are seeded Balding-Nichols PLINK sets from pcaone_sim_bed (and dense,
 likelihood and LD inputs); each case stores its input and the reference's
@@ -13,6 +13,7 @@ OpenMP scheduling.
 """
 from __future__ import annotations
 
+import re
 import subprocess
 import sys
 import tempfile
@@ -47,6 +48,8 @@ CASES = {
     # EMU through the out-of-core reader (-m): the consistent EM path of the
     # reference (the in-core one indexes a permutation that --svd 1 never sets)
     "emu_svd1": ("d3", "--svd 1 -m 0.0005 --emu -k 3 --oversamples 7 -p 5 --tol-rsvd 0"),
+    # PCAngsd on in-core PLINK hard calls (--svd 1: no shuffle, consistent indices)
+    "plink_pcangsd_svd1": ("d3", "--svd 1 --pcangsd -k 3 --oversamples 7 -p 5 --tol-rsvd 0"),
 }
 
 # dense float32 .bin inputs (scRNA-like log1p CPM): name -> (seed, ncells, ngenes, base)
@@ -127,12 +130,22 @@ def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
     res["loadings"] = np.loadtxt(tmp / (out + ".loadings"), ndmin=2)
     for ext in ("sigvals", "eigvals", "eigvecs", "loadings"):
         res["text_" + ext] = np.array((tmp / (out + "." + ext)).read_text())
+    if "--pcangsd" in args.split():
+        # the EM trajectory the reference logs (6 decimals), src/Halko.cpp:309-310
+        log = (tmp / (out + ".log")).read_text()
+        res["em_diffs"] = np.array([float(x) for x in re.findall(
+            r"individual allele frequencies estimated iter =\s*\d+, diff =\s*([0-9.eE+-]+)", log)])
     return res
 
 
 def main():
     if not REF.exists():
         raise SystemExit(f"{REF} missing: make -f oracle/ref.mk")
+    only = set(sys.argv[1:])  # case names to (re)write; none: all of them
+
+    def want(name):
+        return not only or name in only
+
     with tempfile.TemporaryDirectory() as td:
         tmp = Path(td)
         payloads = {}
@@ -141,6 +154,8 @@ def main():
             pa.write_plink(tmp / name, pay, n, m)
             payloads[name] = (pay, n, m)
         for case, (ds, args) in CASES.items():
+            if not want(case):
+                continue
             pay, n, m = payloads[ds]
             res = run_case(tmp, ds, args, case)
             np.savez_compressed(OUT / f"{case}.npz", payload=pay, nsamples=n, nsnps=m, args=args, **res)
@@ -149,13 +164,15 @@ def main():
             gl = pa.sim_gl(seed, n, m, npops, fst, 0.05, 0.5, depth, err)
             pa.write_beagle(tmp / (name + ".beagle.gz"), gl)
             for case, (ds, args) in GL_CASES.items():
-                if ds != name:
+                if ds != name or not want(case):
                     continue
                 res = run_case(tmp, name + ".beagle.gz", args, case, flag="-G")
                 res["cov"] = np.loadtxt(tmp / (case + ".cov"))
                 np.savez_compressed(OUT / f"{case}.npz", gl=gl, nsamples=n, nsnps=m, args=args, **res)
                 print(case, res["eigvals"])
         for name, (seed, n, per_chr, ld_bp) in LD.items():
+            if not want(name):
+                continue
             X, pos, chr_end = ld_inputs(seed, n, per_chr)
             ia, ib, r2 = run_ld(tmp, name, X, pos, chr_end, ld_bp)
             np.savez_compressed(OUT / f"{name}.npz", resid=X, pos=pos, chr_end=chr_end, ld_bp=ld_bp, snp_a=ia,
@@ -165,7 +182,7 @@ def main():
             D = pa.sim_counts(seed, n, m, base=base)
             pa.write_bin(tmp / (name + ".bin"), D)
             for case, (ds, args) in DENSE_CASES.items():
-                if ds != name:
+                if ds != name or not want(case):
                     continue
                 res = run_case(tmp, name + ".bin", args, case, flag="-B")
                 np.savez_compressed(OUT / f"{case}.npz", dense=D, nsamples=n, nsnps=m, args=args, **res)

@@ -2,7 +2,8 @@
 emMAF_with_GL, the initial E, the fit_with_pi PCAngsd update and
 pcangsd_standardize_E against the oracle on the same inputs, and the whole EM
 (plus the GRM written to .cov) against the reference CLI's BEAGLE run
-(tests/golden/gl_pcangsd_svd1.npz).
+(tests/golden/gl_pcangsd_svd1.npz); --pcangsd on PLINK hard calls against the
+reference CLI's in-core PLINK run (tests/golden/plink_pcangsd_svd1.npz).
 
 Tolerances: F and Dc within FP64 rounding (sums in a different order); E is
 stored FP32, so within 2^-23 of the largest entry; eigenvalues and the GRM as
@@ -90,6 +91,64 @@ def test_pcangsd_matches_reference_cli(gpu, arm):
     if arm == "fp64":
         assert np.allclose(diffs, dfo, rtol=1e-6, atol=1e-12)
         assert eig_relerr(ev, op.eigvals()) < 1e-9
+
+
+@pytest.mark.parametrize("arm", list(ARMS))
+def test_pcangsd_plink_matches_reference_cli(gpu, arm):
+    """--pcangsd on in-core PLINK input: hard-call likelihoods, genotype-count
+    F, centred-genotype initial E, then the EM (no GRM step)."""
+    z = load_golden(GOLDEN / "plink_pcangsd_svd1.npz")
+    d = parse_args(z["args"])
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    gpu.load_bed(z["payload"], n, m)
+    F = gpu.gl_from_bed()
+    codes = O.decode_codes(z["payload"], n, m)
+    Fo, _ = O.allele_freq(codes)
+    assert np.array_equal(F, Fo)
+    gpu.rsvd_init(d["k"], d["os"], variant=d["svd"], precision=ARMS[arm], seed=d["seed"], rand=d["rand"])
+    U, S, V, it, diffs, Dc, Cm = gpu.pcangsd(d["p"], d["tol"], 100, 1e-5, grm=False, dc=False)
+    assert Dc is None and Cm is None
+    ev = S ** 2 / m * 2
+    assert eig_relerr(ev, z["eigvals"]) < max(5e-6, EIG_TOL[arm])
+    assert O.principal_angles_max(U, z["eigvecs"]) < 1e-4
+    assert O.principal_angles_max(V, z["loadings"]) < 1e-4
+    op, _, ito, dfo = O.pcangsd_plink(codes, d["k"], d["os"], d["p"], d["tol"])
+    assert it == ito
+    if arm == "fp64":
+        # the device holds the missing entries' 0.333333 as FP32 (1.4e-8 off)
+        # and E as FP32: the trajectory agrees to about 1e-8
+        assert np.allclose(diffs, dfo, rtol=1e-4, atol=1e-8)
+        assert np.allclose(diffs, z["em_diffs"], rtol=1e-4, atol=1e-8)  # the reference's log
+        assert eig_relerr(ev, op.eigvals()) < 1e-7
+
+
+def test_pcangsd_plink_initial_E_and_update(gpu):
+    """The hard-call likelihoods through the update kernel: the initial E is the
+    centred genotype (half-dosage units, missing 0), and the update equals the
+    oracle's fit_with_pi on the hard calls."""
+    pay = pa.sim_bed(17, 66, 301, 3, 0.1, 0.05, 0.5, 0.05)
+    n, m, k = 66, 301, 3
+    gpu.load_bed(pay, n, m)
+    gpu.gl_from_bed()
+    gpu.rsvd_init(k, 5, variant=pa.RSVD_NORMAL, precision=pa.PREC_FP64)
+    codes = O.decode_codes(pay, n, m)
+    F, _ = O.allele_freq(codes)
+    P0, P1 = O.plink_hard_call_gl(codes)
+    rng = np.random.default_rng(8)
+    U = np.linalg.qr(rng.standard_normal((n, k)))[0]
+    V = np.linalg.qr(rng.standard_normal((m, k)))[0]
+    S = np.array([9.0, 5.0, 2.0])
+    E, _ = gpu.pcangsd_update(U, S, V)
+    want = O.pcangsd_E(P0, P1, F, U, S, V).T
+    # P is FP32 on the device (0.333333 rounds), E is stored FP32
+    assert np.abs(E - want).max() < 1e-6 * np.abs(want).max()
+    # the first solve runs on the centred genotype: its top singular values match
+    # the oracle's on BED2GENO - F
+    gpu.rsvd_init(k, 5, variant=pa.RSVD_NORMAL, precision=pa.PREC_FP64)
+    U0, S0, V0, _, _ = gpu.usv(7, 0.0)
+    orc = O.RsvdOracle(O.centered_matrix(codes, F), k, 5, fancy=False)
+    orc.compute_usv(7, 0.0)
+    assert eig_relerr(S0, orc.S[:k]) < 1e-6  # E is FP32 on the device
 
 
 def test_pcangsd_errors(gpu):

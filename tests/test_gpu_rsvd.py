@@ -84,7 +84,7 @@ def test_usv_matches_reference_cli(gpu, case, arm):
     n, m = int(z["nsamples"]), int(z["nsnps"])
     if d["svd"] == 0:
         pytest.skip("IRAM cases are covered by the Gram mat-vec tests")
-    if "gl" in z:
+    if "gl" in z or d["pcangsd"]:
         pytest.skip("PCAngsd cases are covered by the genotype-likelihood tests")
     if "dense" in z:
         gpu.load_dense(z["dense"])

@@ -244,6 +244,30 @@ def test_pcangsd_grm_matches_reference():
     assert np.max(np.abs(np.diag(E @ E.T / P0.shape[1]) - np.diag(z["cov"]))) > 1e-3
 
 
+def test_pcangsd_plink_planted_mistakes(monkeypatch):
+    """The PLINK PCAngsd fixture pins the oracle: the EM trajectory the
+    reference logs (6 significant digits) and its outputs.  An initial E in
+    full-dosage units moves the first EM diff; missing hard calls read as AA
+    (0, 0) instead of thirds move the eigenvalues."""
+    z = load_golden(ROOT / "tests" / "golden" / "plink_pcangsd_svd1.npz")
+    d = parse_args(z["args"])
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    codes = O.decode_codes(z["payload"], n, m)
+    op, _, it, diffs = O.pcangsd_plink(codes, d["k"], d["os"], d["p"], d["tol"])
+    assert eig_relerr(op.eigvals(), z["eigvals"]) < 5e-6
+    assert it == len(z["em_diffs"])
+    assert np.allclose(diffs, z["em_diffs"], rtol=1e-5, atol=0)
+    centred = O.centered_matrix
+    monkeypatch.setattr(O, "centered_matrix", lambda c, F: 2.0 * centred(c, F))
+    _, _, _, bad = O.pcangsd_plink(codes, d["k"], d["os"], d["p"], d["tol"])
+    assert abs(bad[0] / z["em_diffs"][0] - 1) > 1e-2
+    monkeypatch.setattr(O, "centered_matrix", centred)
+    hard = O.plink_hard_call_gl
+    monkeypatch.setattr(O, "plink_hard_call_gl", lambda c: tuple(np.where((c == 1).T, 0.0, p) for p in hard(c)))
+    bad, *_ = O.pcangsd_plink(codes, d["k"], d["os"], d["p"], d["tol"])
+    assert eig_relerr(bad.eigvals(), z["eigvals"]) > 1e-4
+
+
 # --- LD windows and r^2 ------------------------------------------------------
 
 def _brute_windows(pos, chr_end, ld_bp):
