@@ -172,9 +172,18 @@ int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double 
  * number of windows.  Host only. */
 int pcaone_ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end, uint32_t nchr, int32_t ld_bp,
                       int32_t* ws, int32_t* we, uint32_t* nwin);
+/* run_ld_stuff's --USV branch for PLINK input (src/Main.cpp:64-75,
+ * src/LD.cpp:460-466): after pcaone_gpu_load_bed, the LD matrix becomes
+ * (I - U U^T) G with G = BED2GENO - F (missing 0, FileBed::read_all,
+ * src/FilePlink.cpp:106-114) and U the nsamples x k matrix read_usv returns
+ * for the .eigvecs (src/Utils.cpp:255-285; column-major, 1 <= k <= 64: the
+ * caller passes that Mat2D's data as it is).  It stays in HBM in FP64,
+ * not re-centred, and the following pcaone_gpu_ld_r2 calls run on it (until
+ * the next load). */
+int pcaone_gpu_ld_usv(pcaone_gpu* h, const double* U, uint32_t k);
 /* ld_r2_big (src/LD.cpp:419-442) on the matrix loaded by
- * pcaone_gpu_load_dense_f32 (centred per feature as FileBin::read_all):
- * r^2 of the pairs (ws[w], ws[w] + j), j = 1 .. we[w] - 1, window after
+ * pcaone_gpu_load_dense_f32 (centred per feature as FileBin::read_all), or on
+ * the --USV-adjusted genotypes of pcaone_gpu_ld_usv: r^2 of the pairs (ws[w], ws[w] + j), j = 1 .. we[w] - 1, window after
  * window, the line order of the reference's .ld.gz.  With r2 == NULL only
  * *npairs is set; otherwise r2 must hold capacity >= npairs doubles. */
 int pcaone_gpu_ld_r2(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2,

@@ -28,8 +28,8 @@ the reference's behaviour, of:
   src/FileBeagle.cpp:121-143, src/Data.cpp:249-343, src/Halko.cpp:317-331)
 * PCAngsd on PLINK hard calls                  (src/FilePlink.cpp:89-104,
   src/Halko.cpp:294-316)
-* LD windows and r^2                           (src/LD.cpp:20-23, 125-139,
-  419-442)
+* LD windows and r^2, the --USV adjustment    (src/LD.cpp:20-23, 125-139,
+  419-442, 460-466)
 
 The third-party pieces restated here come from libstdc++ (GCC 11.4, the
 toolchain the reference builds with): the Park-Miller minimal-standard LCG
@@ -623,14 +623,36 @@ def ld_windows(pos, chr_end, ld_bp: int):
     return np.array(ws, dtype=np.int64), np.array(we, dtype=np.int64)
 
 
-def ld_r2(featmajor: np.ndarray, ws, we) -> np.ndarray:
+def read_usv(table: np.ndarray) -> np.ndarray:
+    """The matrix read_usv (src/Utils.cpp:255-285) returns for a text table
+    (rows x k): it collects the values row after row and maps that buffer as a
+    column-major rows x k matrix, so element (r, c) is the table's value at
+    row-major position c * rows + r."""
+    rows, k = table.shape
+    return np.ascontiguousarray(np.asarray(table, dtype=np.float64).ravel().reshape(k, rows).T)
+
+
+def ld_usv_matrix(codes: np.ndarray, U: np.ndarray) -> np.ndarray:
+    """run_ld_stuff's --USV branch for PLINK input (src/LD.cpp:460-466):
+    FileBed::read_all's centred genotypes G (N x M, missing 0) adjusted as
+    (I - U U^T) G with U the matrix read_usv returns (N x k), not re-centred;
+    returned feature-major (M x N)."""
+    F, _ = allele_freq(codes)
+    G = centered_matrix(codes, F)
+    n = G.shape[0]
+    return np.ascontiguousarray(((np.eye(n) - U @ U.T) @ G).T)
+
+
+def ld_r2(featmajor: np.ndarray, ws, we, centre: bool = True) -> np.ndarray:
     """ld_r2_big (src/LD.cpp:419-442) on FileBin::read_all's matrix
     (src/FileBinary.cpp:21-30: float32 -> double, each feature centred by its
-    mean): r = <x_i, x_k> (sds_i sds_k / (N - 1)), sds = 1 / sqrt(sum x^2 /
-    (N - 1)) (calc_sds, :20-23).  r^2 for the pairs (ws[w], ws[w] + j),
-    j = 1 .. we[w] - 1, window after window."""
+    mean; centre=False for the --USV matrix, used as it is): r = <x_i, x_k>
+    (sds_i sds_k / (N - 1)), sds = 1 / sqrt(sum x^2 / (N - 1)) (calc_sds,
+    :20-23).  r^2 for the pairs (ws[w], ws[w] + j), j = 1 .. we[w] - 1, window
+    after window."""
     X = featmajor.astype(np.float64)
-    X = X - X.mean(axis=1, keepdims=True)
+    if centre:
+        X = X - X.mean(axis=1, keepdims=True)
     n = X.shape[1]
     df = 1.0 / (n - 1)
     sds = 1.0 / np.sqrt((X * X).sum(axis=1) * df)

@@ -58,6 +58,7 @@ SIGNATURES = {
     "pcaone_ld_windows": (C.c_int, [_I32, C.c_uint32, _I32, C.c_uint32, C.c_int32, _I32, _I32,
                                     C.POINTER(C.c_uint32)]),
     "pcaone_gpu_ld_r2": (C.c_int, [C.c_void_p, _I32, _I32, C.c_uint32, _D, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "pcaone_gpu_ld_usv": (C.c_int, [C.c_void_p, _D, C.c_uint32]),
     "pcaone_write_eigs": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, _D, _D, _D, C.c_int]),
     "pcaone_gpu_load_gl": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]),
     "pcaone_gpu_gl_em_maf": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, C.POINTER(C.c_int)]),
@@ -344,6 +345,13 @@ class Gpu:
                                                 _p(diffs, C.c_double), _p(Dc, C.c_double) if dc else None,
                                                 _p(Cm, C.c_double) if grm else None))
         return U, S, V, it.value, diffs[:it.value], Dc, Cm
+
+    def ld_usv(self, U: np.ndarray):
+        """--USV on the loaded PLINK set: the LD matrix becomes (I - U U^T) G,
+        U the (N, k) matrix applied (the reference applies what its read_usv
+        makes of the .eigvecs table)."""
+        U = np.asfortranarray(U, dtype=np.float64)
+        self._check(self._L.pcaone_gpu_ld_usv(self.h, _p(U, C.c_double), U.shape[1]))
 
     def ld_r2(self, ws: np.ndarray, we: np.ndarray) -> np.ndarray:
         """r^2 of the window pairs on the loaded dense matrix, window after window."""

@@ -186,6 +186,10 @@ int pcaone_ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end
   }
 }
 
+int pcaone_gpu_ld_usv(pcaone_gpu* h, const double* U, uint32_t k) {
+  return guarded(h, [&](pcaone::Engine& e) { e.ld_usv(U, k); });
+}
+
 int pcaone_gpu_ld_r2(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2,
                      uint64_t capacity, uint64_t* npairs) {
   return guarded(h, [&](pcaone::Engine& e) {

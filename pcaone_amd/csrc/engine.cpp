@@ -121,7 +121,7 @@ void Engine::load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps)
   }
   have_bed_ = true;
   src_ = SRC_BED;
-  gl_mode_ = have_glF_ = gl_bed_ = false;
+  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = false;
 }
 
 // Replaces FileBin::read_all (src/FileBinary.cpp:21-30): nfeatures rows of
@@ -147,7 +147,7 @@ void Engine::load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfea
   counts_orig_.clear();
   have_bed_ = true;
   src_ = SRC_DENSE;
-  gl_mode_ = have_glF_ = gl_bed_ = false;
+  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = false;
 }
 
 void Engine::feature_means(double* mu) const {

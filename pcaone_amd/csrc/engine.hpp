@@ -93,8 +93,10 @@ class Engine {
                       double* Dc);
   void pcangsd(int maxp, double tol, int maxiter, double tolem, double* U, double* S, double* V, int* iters,
                double* diffs, double* Dc, double* C);
-  // LD r^2 over windows on the loaded dense matrix
+  // LD r^2 over windows on the loaded dense matrix, or on the --USV-adjusted genotypes
   uint64_t ld_r2(const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2, uint64_t capacity);
+  // --USV on PLINK input: the LD matrix becomes (I - U U^T)(BED2GENO - F)
+  void ld_usv(const double* U, uint32_t k);
   void allele_freq(double* F, int64_t* counts) const;
   void feature_means(double* mu) const;
 
@@ -169,6 +171,9 @@ class Engine {
   DevBuf<float> glP_;
   DevBuf<double> glF_, gl_part_, gl_Dc_;
   std::vector<double> glF_host_;
+  // --USV LD input: (I - U U^T) G, [SNP][sample] FP64, original SNP order
+  bool have_resid_ = false;
+  DevBuf<double> resid_;
 
   // ---- layout (depends on the permutation and the windowing) -------------
   bool have_op_ = false;

@@ -241,9 +241,11 @@ uint32_t gl_rows_per_block(uint32_t nsites);
 // out[i] = sum_y part[y][i] (fixed order)
 void launch_sum_rows(const double* part, uint32_t ny, uint32_t n, double* out, hipStream_t s);
 // ---- LD r^2 on a dense feature-major matrix (src/LD.cpp:20-23, 419-442) -----
-void launch_ld_sds(const float* X, uint32_t n, uint32_t m, const double* mu, double* sds, hipStream_t s);
+void launch_ld_sds(const void* X, int x_f64, uint32_t n, uint32_t m, const double* mu, double* sds,
+                   hipStream_t s);
 struct LdTiles {
-  const float* X;          // [m][n] feature-major FP32
+  const void* X;           // [m][n] feature-major, FP32 or (x_f64) FP64
+  int x_f64;
   const double* mu;        // [m] feature means
   const double* sds;       // [m] 1 / sd
   uint32_t n, m;
@@ -254,6 +256,11 @@ struct LdTiles {
   double* r2;              // window-ordered pair values
 };
 void launch_ld_tiles(const LdTiles& t, uint32_t ntiles, hipStream_t s);
+// --USV on PLINK input: R = G - U (U^T G), G = BED2GENO - F (missing 0), from
+// the packed codes raw (m rows of ceil(n / 4) bytes), F (m), U ([n][k]); W
+// ([m][k]) receives U^T G, R ([m][n], FP64) the adjusted genotypes.
+void launch_usv_residual(const uint8_t* raw, uint32_t n, uint32_t m, const double* F, const double* U, uint32_t k,
+                         double* W, double* R, hipStream_t s);
 
 // GRM C = E^T E / nsites with diagonal Dc / nsites (E rows = sites, ld stride)
 void launch_grm(const float* E, uint64_t ld, uint32_t nsamples, uint32_t nsites, const double* Dc, double* C,

@@ -1,6 +1,6 @@
-// LD r^2 (the --print-r2 path of run_ld_stuff on a .residuals matrix,
-// src/LD.cpp:450-470): windows from positions on the host, the band of the
-// Gram on the device.
+// LD r^2 (the --print-r2 path of run_ld_stuff on a .residuals matrix, or on
+// PLINK input with --USV, src/LD.cpp:450-470): windows from positions on the
+// host, the band of the Gram on the device.
 #include <algorithm>
 #include <cstring>
 
@@ -36,11 +36,39 @@ uint32_t ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end, 
   return nwin;
 }
 
-// ld_r2_big (src/LD.cpp:419-442) on FileBin::read_all's centred matrix: the
-// r^2 of pairs (ws[w], ws[w] + j), j = 1 .. we[w] - 1, window after window.
-// Returns the number of pairs (written when r2 != null and capacity allows).
+// run_ld_stuff's --USV branch for PLINK input (src/Main.cpp:64-75,
+// src/LD.cpp:460-466): FileBed::read_all's centred genotypes G = BED2GENO - F
+// (missing 0, no MAF filter at the default --maf 0) become (I - U U^T) G, U
+// the N x k left singular vectors as read from .eigvecs (column-major here).
+// The result stays in HBM in FP64, as the reference's data->G, and is not
+// re-centred; ld_r2 then runs on it.
+void Engine::ld_usv(const double* U, uint32_t k) {
+  PCAONE_REQUIRE(have_bed_ && src_ == SRC_BED && !gl_mode_, -3, "ld_usv: load PLINK genotypes first");
+  PCAONE_REQUIRE(U && k >= 1 && k <= 64, -1, "ld_usv: U must be nsamples x k with 1 <= k <= 64");
+  PCAONE_HIP(hipSetDevice(device_));
+  std::vector<double> ur((size_t)N_ * k);
+  for (uint32_t j = 0; j < N_; ++j)
+    for (uint32_t c = 0; c < k; ++c) ur[(size_t)j * k + c] = U[(size_t)c * N_ + j];
+  DevBuf<double> du, df, dw;
+  du.reserve(ur.size());
+  df.reserve(M_);
+  dw.reserve((uint64_t)M_ * k);
+  resid_.reserve((uint64_t)M_ * N_);
+  PCAONE_HIP(hipMemcpyAsync(du.p, ur.data(), ur.size() * 8, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(df.p, F_orig_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  timed("ld_usv", [&] { launch_usv_residual(raw_.p, N_, M_, df.p, du.p, k, dw.p, resid_.p, stream_); });
+  sync();  // the staging buffers are freed on return
+  have_resid_ = true;
+}
+
+// ld_r2_big (src/LD.cpp:419-442) on FileBin::read_all's centred matrix (or the
+// --USV-adjusted genotypes after ld_usv): the r^2 of pairs (ws[w], ws[w] + j),
+// j = 1 .. we[w] - 1, window after window.  Returns the number of pairs
+// (written when r2 != null and capacity allows).
 uint64_t Engine::ld_r2(const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2, uint64_t capacity) {
-  PCAONE_REQUIRE(have_bed_ && src_ == SRC_DENSE && !gl_mode_, -3, "ld_r2: load a dense (.residuals) matrix first");
+  const bool resid = have_resid_;
+  PCAONE_REQUIRE(have_bed_ && ((src_ == SRC_DENSE && !gl_mode_) || resid), -3,
+                 "ld_r2: load a dense (.residuals) matrix, or PLINK genotypes and ld_usv, first");
   PCAONE_REQUIRE(N_ >= 2, -1, "ld_r2: needs at least two samples");
   PCAONE_REQUIRE(ws && we, -1, "ld_r2: null windows");
   PCAONE_HIP(hipSetDevice(device_));
@@ -81,14 +109,19 @@ uint64_t Engine::ld_r2(const int32_t* ws, const int32_t* we, uint32_t nwin, doub
   dwin.reserve(M_);
   dwe.reserve(nwin);
   doff.reserve(nwin);
-  PCAONE_HIP(hipMemcpyAsync(mu.p, mu_orig_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+  if (resid)  // the adjusted genotypes are used as they are
+    timed("ld_sds", [&] { launch_fill_f64(mu.p, M_, 0.0, stream_); });
+  else
+    PCAONE_HIP(hipMemcpyAsync(mu.p, mu_orig_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(dt.p, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(dwin.p, win_of_row.data(), (size_t)M_ * 4, hipMemcpyHostToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(dwe.p, we, (size_t)nwin * 4, hipMemcpyHostToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(doff.p, off.data(), (size_t)nwin * 8, hipMemcpyHostToDevice, stream_));
-  timed("ld_sds", [&] { launch_ld_sds(rawf_.p, N_, M_, mu.p, sds.p, stream_); });
+  const void* X = resid ? static_cast<const void*>(resid_.p) : static_cast<const void*>(rawf_.p);
+  timed("ld_sds", [&] { launch_ld_sds(X, resid ? 1 : 0, N_, M_, mu.p, sds.p, stream_); });
   LdTiles t{};
-  t.X = rawf_.p;
+  t.X = X;
+  t.x_f64 = resid ? 1 : 0;
   t.mu = mu.p;
   t.sds = sds.p;
   t.n = N_;

@@ -70,6 +70,9 @@ DENSE_CASES = {
 # LD r^2 on a dense residual matrix (-B .residuals --match-bim .mbim --print-r2):
 # name -> (seed, nsamples, SNPs per chromosome, ld_bp)
 LD = {"ld_r2_bp2000": (41, 150, (300, 250, 351), 2000)}
+# LD with --USV on PLINK input (a PCA run for .eigvecs, then -b --USV --print-r2):
+# name -> (seed, nsamples, SNPs per chromosome, ld_bp, k)
+LD_USV = {"ld_usv_bp3000": (51, 160, (400, 350, 450), 3000, 3)}
 
 
 def ld_inputs(seed, n, per_chr):
@@ -116,6 +119,43 @@ def run_ld(tmp: Path, name: str, X, pos, chr_end, ld_bp):
             ib.append(int(t[5][2:]))
             r2.append(float(t[6]))
     return np.array(ia), np.array(ib), np.array(r2)
+
+
+def read_ld_gz(path: Path):
+    import gzip
+    ia, ib, r2 = [], [], []
+    with gzip.open(path, "rt") as f:
+        f.readline()
+        for line in f:
+            t = line.rstrip("\n").split("\t")
+            ia.append(int(t[2][2:]))
+            ib.append(int(t[5][2:]))
+            r2.append(float(t[6]))
+    return np.array(ia), np.array(ib), np.array(r2)
+
+
+def run_ld_usv(tmp: Path, name: str, seed, n, per_chr, ld_bp, k):
+    """Genotypes with 2 % missing on len(per_chr) chromosomes (sorted positions,
+    1..400 bp gaps, one duplicated position), the reference's PCA for U, then
+    its --USV LD."""
+    m = int(sum(per_chr))
+    pay = pa.sim_bed(seed, n, m, 3, 0.1, 0.05, 0.5, 0.02)
+    rng = np.random.default_rng(seed)
+    chrom = np.repeat(np.arange(1, len(per_chr) + 1), per_chr)
+    pos = np.concatenate([np.cumsum(rng.integers(1, 400, c)) for c in per_chr]).astype(np.int64)
+    pos[10] = pos[9]
+    pa.write_plink(tmp / name, pay, n, m, chrom, pos)
+    pca = [str(REF), "-b", name, "--svd", "1", "-k", str(k), "-p", "7", "--tol-rsvd", "0", "-V", "-o",
+           name + "_pca", "-n", "1", "-v", "0"]  # -V also writes the .mbim that --USV reads
+    subprocess.run(pca, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    ld = [str(REF), "-b", name, "--USV", name + "_pca", "--print-r2", "--ld-bp", str(ld_bp), "-o", name, "-n", "1",
+          "-v", "0"]
+    subprocess.run(ld, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    U = np.loadtxt(tmp / (name + "_pca.eigvecs"), ndmin=2)
+    ia, ib, r2 = read_ld_gz(tmp / (name + ".ld.gz"))
+    chr_end = np.cumsum(per_chr) - 1
+    return dict(payload=pay, nsamples=n, nsnps=m, pos=pos, chr_end=chr_end, ld_bp=ld_bp, U=U, snp_a=ia, snp_b=ib,
+                r2=r2)
 
 
 def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
@@ -178,6 +218,12 @@ def main():
             np.savez_compressed(OUT / f"{name}.npz", resid=X, pos=pos, chr_end=chr_end, ld_bp=ld_bp, snp_a=ia,
                                 snp_b=ib, r2=r2)
             print(name, len(r2), "pairs")
+        for name, (seed, n, per_chr, ld_bp, k) in LD_USV.items():
+            if not want(name):
+                continue
+            res = run_ld_usv(tmp, name, seed, n, per_chr, ld_bp, k)
+            np.savez_compressed(OUT / f"{name}.npz", **res)
+            print(name, len(res["r2"]), "pairs")
         for name, (seed, n, m, base) in DENSE.items():
             D = pa.sim_counts(seed, n, m, base=base)
             pa.write_bin(tmp / (name + ".bin"), D)

@@ -1,7 +1,8 @@
 """GPU parity of LD r^2 (the --print-r2 path on a .residuals matrix) through the
 C ABI (tests marked `gpu`): against the oracle on the same inputs (the FP64
 MFMA band Gram differs from the oracle's FP64 dots only in summation order:
-1e-12 absolute on r^2) and against the reference CLI's .ld.gz (6 decimals)."""
+1e-12 absolute on r^2) and against the reference CLI's .ld.gz (6 decimals);
+the --USV path on PLINK input the same way (tests/golden/ld_usv_bp3000.npz)."""
 from __future__ import annotations
 
 import numpy as np
@@ -40,6 +41,50 @@ def test_ld_r2_matches_oracle(gpu, n, m, bp):
     assert r2.shape == want.shape
     if len(want):
         assert np.abs(r2 - want).max() < 1e-12
+
+
+def test_ld_usv_matches_reference_cli(gpu):
+    """--USV on PLINK input: (I - U U^T) G in FP64 on the device, r^2 against
+    the reference CLI's .ld.gz (6 decimals) and the oracle."""
+    z = load_golden(GOLDEN / "ld_usv_bp3000.npz")
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    ws, we = pa.ld_windows(z["pos"], z["chr_end"], int(z["ld_bp"]))
+    U = O.read_usv(z["U"])  # the matrix the reference applies (read_usv's column-major map)
+    gpu.load_bed(z["payload"], n, m)
+    gpu.ld_usv(U)
+    r2 = gpu.ld_r2(ws, we)
+    assert r2.shape == z["r2"].shape
+    assert np.abs(r2 - z["r2"]).max() <= 5.01e-7
+    R = O.ld_usv_matrix(O.decode_codes(z["payload"], n, m), U)
+    assert np.abs(r2 - O.ld_r2(R, ws, we, centre=False)).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,m,k", [(3, 70, 1), (65, 130, 5), (257, 300, 33)])
+def test_ld_usv_matches_oracle(gpu, n, m, k):
+    """Sample and SNP counts across the 64-row and 256-thread tile edges, k on
+    each register bound; a random (not orthonormal) U."""
+    pay = pa.sim_bed(n + m, n, m, 2, 0.1, 0.05, 0.5, 0.05)
+    rng = np.random.default_rng(k)
+    U = rng.standard_normal((n, k)) / np.sqrt(n)
+    pos = np.cumsum(rng.integers(0, 30, m)).astype(np.int32)
+    chr_end = np.array([m // 2, m - 1], dtype=np.int32)
+    ws, we = pa.ld_windows(pos, chr_end, 200)
+    gpu.load_bed(pay, n, m)
+    gpu.ld_usv(U)
+    r2 = gpu.ld_r2(ws, we)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        want = O.ld_r2(O.ld_usv_matrix(O.decode_codes(pay, n, m), U), ws, we, centre=False)
+    assert r2.shape == want.shape
+    # a monomorphic SNP (likely at N = 3) has a zero row: 0 * inf = NaN there, as
+    # in the reference's 0 * (1 / 0)
+    assert np.array_equal(np.isnan(r2), np.isnan(want))
+    ok = ~np.isnan(want)
+    if ok.any():
+        assert np.abs(r2[ok] - want[ok]).max() < 1e-10
+    # a new load drops the adjusted matrix: ld_r2 on PLINK genotypes alone is an error
+    gpu.load_bed(pay, n, m)
+    with pytest.raises(pa.GpuError):
+        gpu.ld_r2(ws, we)
 
 
 def test_ld_r2_errors(gpu):

@@ -303,6 +303,31 @@ def test_ld_oracle_matches_reference():
     assert np.abs(wrong - z["r2"]).max() > 1e-3
 
 
+def test_ld_usv_oracle_matches_reference():
+    """--USV on PLINK input: the reference's PCA gives the .eigvecs table (6
+    digits), its LD run the r^2 of (I - U U^T) G with U as read_usv maps that
+    table."""
+    z = load_golden(ROOT / "tests" / "golden" / "ld_usv_bp3000.npz")
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    codes = O.decode_codes(z["payload"], n, m)
+    ws, we = O.ld_windows(z["pos"], z["chr_end"], int(z["ld_bp"]))
+    ia = np.concatenate([np.full(e - 1, i) for i, e in zip(ws, we)])
+    ib = np.concatenate([np.arange(i + 1, i + e) for i, e in zip(ws, we)])
+    assert np.array_equal(ia, z["snp_a"]) and np.array_equal(ib, z["snp_b"])
+    U = O.read_usv(z["U"])
+    R = O.ld_usv_matrix(codes, U)
+    r2 = O.ld_r2(R, ws, we, centre=False)
+    assert np.abs(r2 - z["r2"]).max() <= 5.01e-7  # the reference prints 6 decimals
+    # closed form: the product as two thin products
+    G = O.centered_matrix(codes, O.allele_freq(codes)[0])
+    assert np.allclose(R.T, G - U @ (U.T @ G), rtol=0, atol=1e-12)
+    # planted mistakes: no adjustment, the table taken row-major (U as printed),
+    # re-centred rows
+    assert np.abs(O.ld_r2(G.T, ws, we, centre=False) - z["r2"]).max() > 1e-3
+    assert np.abs(O.ld_r2(O.ld_usv_matrix(codes, z["U"]), ws, we, centre=False) - z["r2"]).max() > 1e-3
+    assert np.abs(O.ld_r2(R, ws, we, centre=True) - z["r2"]).max() > 1e-5
+
+
 def test_ld_r2_closed_forms():
     rng = np.random.default_rng(9)
     x = rng.standard_normal(50)
