@@ -184,6 +184,38 @@ static uint32_t split_for(uint64_t blocks, uint32_t kchunks, uint32_t& cps) {
   return (uint32_t)ceil_div(kchunks, cps);
 }
 
+// Split-K factor of the split-BF16 packed contraction: `blocks` row blocks (all
+// windows) over kchunks K chunks with `slots` blocks resident on the chip at
+// once.  The split (at most 8) minimises rounds of resident blocks times the
+// chunks each block runs plus one chunk's worth for its epilogue, the smallest
+// split on a tie.  (Counting only "enough blocks" put 320 blocks on 256 slots
+// for the 16-window H product: two rounds of whole-K blocks.)
+static uint32_t split_rounds(uint64_t blocks, uint32_t kchunks, uint32_t slots, uint32_t& cps) {
+  uint32_t best = 1;
+  uint64_t best_t = ~0ull;
+  for (uint32_t s = 1; s <= std::min<uint32_t>(kchunks, 8); ++s) {
+    const uint32_t c = (uint32_t)ceil_div(kchunks, s), ns = (uint32_t)ceil_div(kchunks, c);
+    const uint64_t t = ceil_div(blocks * ns, slots) * (uint64_t)(c + 1);
+    if (t < best_t) {
+      best_t = t;
+      best = ns;
+    }
+  }
+  cps = (uint32_t)ceil_div(kchunks, best);
+  return (uint32_t)ceil_div(kchunks, cps);
+}
+
+// split-K of one packed contraction launch: rows per window group, nz groups
+uint32_t Engine::packed_split(int prec, uint64_t rows, uint32_t nz, uint32_t kchunks, uint32_t& cps) const {
+  if (prec == PREC_SPLIT_BF16) {
+    // the streamed kernel: 512-row blocks, one resident per CU; the 4-wave one:
+    // 256-row blocks, two per CU
+    const bool big = packed_bf16_streams(rows, nz);
+    return split_rounds(ceil_div(rows, big ? 512 : 256) * nz, kchunks, big ? 256 : 512, cps);
+  }
+  return split_for(ceil_div(rows, rows_per_block_(prec)) * nz, kchunks, cps);
+}
+
 void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* perm) {
   PCAONE_REQUIRE(have_bed_, -3, "rsvd_init: no data loaded");
   PCAONE_HIP(hipSetDevice(device_));
@@ -436,9 +468,9 @@ void Engine::partial_rows(int prec, uint64_t& gmax, uint64_t& hmax) const {
   for (uint32_t nw = 1; nw <= nwin_; ++nw) {
     const uint64_t rows = std::min<uint64_t>((uint64_t)nw * bs_, M_);
     uint32_t cps;
-    const uint32_t sg = split_for(ceil_div(rows, rpb), kcg, cps);
+    const uint32_t sg = dense ? split_for(ceil_div(rows, rpb), kcg, cps) : packed_split(prec, rows, 1, kcg, cps);
     gmax = std::max<uint64_t>(gmax, sg * rows);
-    const uint32_t sh = split_for(ceil_div(N_, rpb) * nw, kch, cps);
+    const uint32_t sh = dense ? split_for(ceil_div(N_, rpb) * nw, kch, cps) : packed_split(prec, N_, nw, kch, cps);
     hmax = std::max<uint64_t>(hmax, (uint64_t)sh * N_ * nw);
   }
 }
@@ -753,7 +785,7 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
     g.lda = row_bytes_;
     g.nrows = rows;
     g.kchunks = row_bytes_ / 64;
-    g.nsplit = split_for(ceil_div(rows, rpb), g.kchunks, g.chunks_per_split);
+    g.nsplit = packed_split(prec, rows, 1, g.kchunks, g.chunks_per_split);
     g.B = b.bg;
     g.ldb = kg_;
     g.out = b.gpart;
@@ -860,7 +892,7 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
     h.a_zstride = (uint64_t)npad_ * win_bytes_;
     h.nrows = N_;
     h.kchunks = win_bytes_ / 64;
-    h.nsplit = split_for(ceil_div(N_, rpb) * nz, h.kchunks, h.chunks_per_split);
+    h.nsplit = packed_split(prec, N_, nz, h.kchunks, h.chunks_per_split);
     h.B = b.bh + (uint64_t)w0 * b.bcols * kw_ * b.opsz;
     h.ldb = kw_;
     h.b_zstride = (uint64_t)b.bcols * kw_;

@@ -145,6 +145,7 @@ class Engine {
   void seg_G(uint32_t w0, uint32_t w1, const SegBufs& b);
   void seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b);
   void partial_rows(int prec, uint64_t& gmax, uint64_t& hmax) const;
+  uint32_t packed_split(int prec, uint64_t rows, uint32_t nz, uint32_t kchunks, uint32_t& cps) const;
   void set_omega_vec(const double* dev_v, const double* host_v);
   void apply_X(const double* x, double* v);
   void apply_Xt(const double* v, double* y);
