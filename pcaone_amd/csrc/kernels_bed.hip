@@ -538,9 +538,11 @@ __global__ __launch_bounds__(512) void k_packed_gemm_bf16_stream(PackedGemm g) {
   store_acc<NT, true>(g, out + (uint64_t)split * g.nrows * g.ldo, row0, acc);
 }
 
-// streamed 8-wave blocks (512 rows, half the B traffic per row) for long row
-// ranges, 4-wave blocks where the rows would not fill the chip
-bool packed_bf16_streams(uint64_t nrows, uint32_t nz) { return nrows * nz >= 65536; }
+// Streamed 8-wave blocks (512 rows, half the B traffic per row) wherever the
+// rows fill a few blocks; measured on the 10,000 x 500,000 winSVD, they beat
+// the 4-wave blocks from one window's H product (10,000 rows, 25 -> 21 us) up,
+// with the split-K chosen by resident rounds.  4-wave blocks for smaller ones.
+bool packed_bf16_streams(uint64_t nrows, uint32_t nz) { return nrows * nz >= 4096; }
 
 void launch_packed_gemm_bf16(const PackedGemm& g, int nt16, hipStream_t s) {
   const bool big = packed_bf16_streams(g.nrows, g.nz);
