@@ -213,6 +213,7 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restric
   double* v = a + WD * LD;
   double* tmp = v + WD * LD;
   __shared__ __attribute__((aligned(16))) double csw[kEigThreads / 64][8 * NB][2];  // per wave: (c, s) of each pair
+  __shared__ unsigned char rrp[16 * NB][8 * NB][2];  // [round][pair] -> (p, q), p < q
   __shared__ double red[kEigThreads / 64][2];
   __shared__ int order[WD];
   const int n = (L % 2) ? L + 1 : L;  // dummy index L when odd (a zero row of the padding)
@@ -238,6 +239,13 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restric
       for (int p = 0; p < nparts; ++p) s += part[(uint64_t)p * L * L + i * L + j];
     a[i * LD + j] = s;
     v[i * LD + j] = in ? (W0 ? W0[i * L + j] : (i == j ? 1.0 : 0.0)) : 0.0;
+  }
+  // the round-robin pairs of every round, looked up instead of recomputed per item
+  for (int e = t; e < (n - 1) * h; e += kEigThreads) {
+    int p, q;
+    rr_pair(e % h, e / h, n, p, q);
+    rrp[e / h][e % h][0] = (unsigned char)p;
+    rrp[e / h][e % h][1] = (unsigned char)q;
   }
   __syncthreads();
   if (W0) {
@@ -288,9 +296,11 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restric
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int kind = asg[u] >> 16, hi = (asg[u] >> 8) & 255, lo = asg[u] & 255;
-        int p1, q1, p2, q2;
-        rr_pair(hi, round, n, p1, q1);
-        rr_pair(kind == 1 ? lo : 0, round, n, p2, q2);
+        i0[u] = i1[u] = i2[u] = i3[u] = 0;
+        if (kind == 0) continue;  // whole waves past the items skip the lookups
+        const int p1 = rrp[round][hi][0], q1 = rrp[round][hi][1];
+        const int k2 = kind == 1 ? lo : 0;
+        const int p2 = rrp[round][k2][0], q2 = rrp[round][k2][1];
         if (kind == 1) {  // a entries (p1, p2), (q1, p2), (p1, q2), (q1, q2)
           i0[u] = p1 * LD + p2;
           i1[u] = q1 * LD + p2;
@@ -318,8 +328,7 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restric
       // hands them to its other lanes through its own LDS slots: LDS accesses
       // of one wave complete in order, so no barrier is needed
       if (lane < h) {
-        int p, q;
-        rr_pair(lane, round, n, p, q);
+        const int p = rrp[round][lane][0], q = rrp[round][lane][1];
         double c = 1.0, sn = 0.0;
         const double apq = a[p * LD + q];
         if (q < L && fabs(apq) > 1e-300) {
