@@ -659,11 +659,11 @@ void launch_packed_gemm_f64(const PackedGemm& g, int ncols, hipStream_t s) { lau
 // epilogues
 // ---------------------------------------------------------------------------
 
-// Grid (row chunk of 64, window).  Besides G, g'' and the H operand the
+// Grid (row chunk of 32, window).  Besides G, g'' and the H operand the
 // epilogue writes the chunk's partial column sums of g'' (s2p), which
 // k_window_colsum reduces into S''.  The operand (column-major per window) is
 // staged through LDS so its stores run along the window's SNPs.
-constexpr uint32_t kFinRows = 64;
+constexpr uint32_t kFinRows = 32;  // two 512-thread finalize blocks fit one CU
 constexpr uint32_t kTileLd = 65;
 
 // bh[w][c][k0 + p] for the operand positions p < 64 of a chunk, from
@@ -682,7 +682,7 @@ __device__ inline void store_operand_tile(const double* tile, uint32_t nq, uint3
       const uint32_t q = kperm ? (uint32_t)kperm16(p0 + b) : p0 + b;
       v[b] = q < nq ? tile[q * kTileLd + c] : 0.0;
     }
-    const uint64_t o = ((uint64_t)w * bcols + c) * kw + k0 + p0;  // k0 is a multiple of 64
+    const uint64_t o = ((uint64_t)w * bcols + c) * kw + k0 + p0;  // k0 is a multiple of kFinRows
     if (bprec == PREC_FP64) {
       *reinterpret_cast<double4*>(reinterpret_cast<double*>(bh) + o) = make_double4(v[0], v[1], v[2], v[3]);
     } else if (bprec == PREC_FP32) {
@@ -735,8 +735,9 @@ __device__ inline void store_g2(const FinalizeG& f, uint64_t i, double v) {
 // so every lane has an element and the partial, correction, G and g''
 // accesses run along rows; each thread's elements (at most 4 for l <= 64)
 // have their loads in flight together, four split-K partials at a time.
-constexpr int kFinThreads = 1024;
+constexpr int kFinThreads = 512;
 constexpr int kFinEl = kFinRows * 64 / kFinThreads;
+static_assert(kFinThreads / 64 == 8, "finalize_G sums its column slices eight at a time");
 
 template <typename P>
 __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
@@ -822,7 +823,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
     acct[q * kTileLd + c] = f.affine ? g2 - gp : g2;
   }
   __syncthreads();
-  {  // column sums of the chunk: 16 row slices, then the slices in a fixed order
+  {  // column sums of the chunk: 8 row slices, then the slices in a fixed order
     const uint32_t c = t & 63, sl = t >> 6;
     double a = 0.0;
     if (c < l)
@@ -833,7 +834,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
   if (t < l) {
     double v[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = sh[2 * i][t] + sh[2 * i + 1][t];
+    for (int i = 0; i < 8; ++i) v[i] = sh[i][t];
     f.s2p[((uint64_t)w * gridDim.x + chunk) * l + t] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
   }
   store_operand_tile(tile, nq, l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
