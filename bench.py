@@ -38,7 +38,7 @@ FP32_PEAK_TFLOPS = 157.3
 # FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0) averaged over the 280 launches of one
 # split-BF16 solve, the factors from a 512 MiB copy kernel
 # (profiles/k_packed_gemm_bf16_c2_winsvd.md)
-TRAFFIC = 303.8e6
+TRAFFIC = 291.3e6
 
 
 def log(*a):
