@@ -928,10 +928,12 @@ void launch_window_colsum(const double* s2p, uint32_t l, uint32_t blocksize, uin
 }
 
 // thread (sample j, column c); the windows' contributions are summed in
-// registers per destination and added once.  Four windows' partials, four
-// split-K slices each, and their corrections load together from clamped
-// indices (unconditional loads issue together; out-of-range values drop out).
-template <typename P>
+// registers per destination and added once.  Four windows' partials, KS
+// split-K slices each (KS = 1 when the launch has one slice, so no load is
+// repeated), and their corrections load together from clamped indices
+// (unconditional loads issue together; out-of-range values drop out).  LO:
+// the partials hold separate hi and lo columns.
+template <typename P, int KS, bool LO>
 __global__ __launch_bounds__(256) void k_finalize_H(FinalizeH f) {
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (uint64_t)f.nsamples * f.l) return;
@@ -939,7 +941,7 @@ __global__ __launch_bounds__(256) void k_finalize_H(FinalizeH f) {
   const P* __restrict__ part = reinterpret_cast<const P*>(f.part);
   const P* __restrict__ corrp = reinterpret_cast<const P*>(f.corr);
   const double* __restrict__ S2 = f.S2;
-  const uint64_t lo = f.split_cols ? f.l : 0;
+  const uint64_t lo = LO ? f.l : 0;
   double ha = 0.0, hb = 0.0;
   bool any_a = false, any_b = false;
   for (uint32_t w0 = f.w0; w0 < f.w1; w0 += 4) {
@@ -958,22 +960,22 @@ __global__ __launch_bounds__(256) void k_finalize_H(FinalizeH f) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) corr[u] = P(0);
     }
-    for (uint32_t s0 = 0; s0 < f.nsplit; s0 += 4) {
-      P x[4][4][2];
+    for (uint32_t s0 = 0; s0 < f.nsplit; s0 += KS) {
+      P x[4][KS], y[4][KS];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < KS; ++k) {
           const uint32_t wz = min(w0 + u, f.w1 - 1) - f.wpart0, sk = min(s0 + k, f.nsplit - 1);
           const uint64_t base = (((uint64_t)wz * f.nsplit + sk) * f.part_rows + j) * f.ldp + c;
-          x[u][k][0] = part[base];
-          x[u][k][1] = part[base + lo];
+          x[u][k] = part[base];
+          y[u][k] = LO ? part[base + lo] : P(0);
         }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (s0 + k < f.nsplit) raw[u] += (double)x[u][k][0] + (lo ? (double)x[u][k][1] : 0.0);
+        for (int k = 0; k < KS; ++k)
+          if (s0 + k < f.nsplit) raw[u] += (double)x[u][k] + (LO ? (double)y[u][k] : 0.0);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -1004,12 +1006,26 @@ __global__ __launch_bounds__(256) void k_finalize_H(FinalizeH f) {
   if (any_b) f.Hb[(uint64_t)j * f.l + c] += hb;
 }
 
+template <typename P>
+void launch_finalize_H_t(const FinalizeH& f, dim3 grid, hipStream_t s) {
+  const bool one = f.nsplit == 1, lo = f.split_cols != 0;
+  if (one && !lo)
+    hipLaunchKernelGGL((k_finalize_H<P, 1, false>), grid, dim3(256), 0, s, f);
+  else if (one)
+    hipLaunchKernelGGL((k_finalize_H<P, 1, true>), grid, dim3(256), 0, s, f);
+  else if (!lo)
+    hipLaunchKernelGGL((k_finalize_H<P, 4, false>), grid, dim3(256), 0, s, f);
+  else
+    hipLaunchKernelGGL((k_finalize_H<P, 4, true>), grid, dim3(256), 0, s, f);
+}
+
 void launch_finalize_H(const FinalizeH& f, hipStream_t s) {
   const uint64_t total = (uint64_t)f.nsamples * f.l;
+  const dim3 grid((uint32_t)ceil_div(total, 256));
   if (f.part_is_f64)
-    hipLaunchKernelGGL(k_finalize_H<double>, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, f);
+    launch_finalize_H_t<double>(f, grid, s);
   else
-    hipLaunchKernelGGL(k_finalize_H<float>, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, f);
+    launch_finalize_H_t<float>(f, grid, s);
 }
 
 // one thread per SNP row: EMU value of each of its missing samples
