@@ -739,7 +739,9 @@ constexpr int kFinThreads = 512;
 constexpr int kFinEl = kFinRows * 64 / kFinThreads;
 static_assert(kFinThreads / 64 == 8, "finalize_G sums its column slices eight at a time");
 
-template <typename P>
+// KS split-K slices per round (1 when the launch has one, so no load repeats);
+// LO: the partials hold separate hi and lo columns.
+template <typename P, int KS, bool LO>
 __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
   __shared__ double tile[kFinRows * kTileLd];  // g' = d G, the H operand
   __shared__ double acct[kFinRows * kTileLd];  // the column-sum term (g'', or g'' - g' when affine)
@@ -786,23 +788,23 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
 #pragma unroll
     for (int u = 0; u < kFinEl; ++u) corr[u] = P(0);
   }
-  const uint32_t lo = f.split_cols ? l : 0;
-  for (uint32_t s0 = 0; s0 < f.nsplit; s0 += 4) {
-    P x[4][kFinEl][2];
+  const uint32_t lo = LO ? l : 0;
+  for (uint32_t s0 = 0; s0 < f.nsplit; s0 += KS) {
+    P x[KS][kFinEl], y[KS][kFinEl];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < KS; ++j) {
       const P* __restrict__ ps = part + (uint64_t)min(s0 + j, f.nsplit - 1) * sstride;
 #pragma unroll
       for (int u = 0; u < kFinEl; ++u) {
-        x[j][u][0] = ps[off[u]];
-        x[j][u][1] = ps[off[u] + lo];
+        x[j][u] = ps[off[u]];
+        y[j][u] = LO ? ps[off[u] + lo] : P(0);
       }
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < KS; ++j)
       if (s0 + j < f.nsplit)
 #pragma unroll
-        for (int u = 0; u < kFinEl; ++u) raw[u] += (double)x[j][u][0] + (lo ? (double)x[j][u][1] : 0.0);
+        for (int u = 0; u < kFinEl; ++u) raw[u] += (double)x[j][u] + (LO ? (double)y[j][u] : 0.0);
   }
 #pragma unroll
   for (int u = 0; u < kFinEl; ++u) {
@@ -840,6 +842,19 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
   store_operand_tile(tile, nq, l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
 }
 
+template <typename P>
+void launch_finalize_G_t(const FinalizeG& f, dim3 grid, hipStream_t s) {
+  const bool one = f.nsplit == 1, lo = f.split_cols != 0;
+  if (one && !lo)
+    hipLaunchKernelGGL((k_finalize_G<P, 1, false>), grid, dim3(kFinThreads), 0, s, f);
+  else if (one)
+    hipLaunchKernelGGL((k_finalize_G<P, 1, true>), grid, dim3(kFinThreads), 0, s, f);
+  else if (!lo)
+    hipLaunchKernelGGL((k_finalize_G<P, 4, false>), grid, dim3(kFinThreads), 0, s, f);
+  else
+    hipLaunchKernelGGL((k_finalize_G<P, 4, true>), grid, dim3(kFinThreads), 0, s, f);
+}
+
 void launch_finalize_G(const FinalizeG& f, hipStream_t s) {
   PCAONE_REQUIRE(f.l >= 1 && f.l <= 64, -1, "finalize: l must be in [1, 64]");
   PCAONE_REQUIRE(f.kw % kFinRows == 0, -1, "finalize: window operand rows must be whole chunks");
@@ -847,9 +862,9 @@ void launch_finalize_G(const FinalizeG& f, hipStream_t s) {
   if (f.w1 <= f.w0) return;
   dim3 grid((uint32_t)ceil_div(f.blocksize, kFinRows), f.w1 - f.w0);
   if (f.part_is_f64)
-    hipLaunchKernelGGL(k_finalize_G<double>, grid, dim3(kFinThreads), 0, s, f);
+    launch_finalize_G_t<double>(f, grid, s);
   else
-    hipLaunchKernelGGL(k_finalize_G<float>, grid, dim3(kFinThreads), 0, s, f);
+    launch_finalize_G_t<float>(f, grid, s);
 }
 
 uint32_t finalize_chunks(uint32_t blocksize) { return (uint32_t)ceil_div(blocksize, kFinRows); }
