@@ -219,4 +219,10 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    try:
+        main()
+    finally:
+        # release the handle and the page-locked buffers while the HIP runtime
+        # is fully alive, never from finalizers during interpreter shutdown
+        if "pcaone_amd" in sys.modules:
+            sys.modules["pcaone_amd"].shutdown()

@@ -79,7 +79,8 @@ typedef struct {
   int variant;     /* PCAONE_RSVD_NORMAL | PCAONE_RSVD_FANCY                        */
   int k;           /* -k                                                           */
   int oversamples; /* --oversamples, after Param's max(os, k) (src/Cmd.cpp:190)     */
-  int bands;       /* -w/--batches: power of two >= 4 (src/Cmd.cpp:213-214)        */
+  int bands;       /* -w/--batches: even, >= 4 (src/Cmd.cpp:213-214); read by the
+                      winSVD variant only (the plain variant tiles its own)     */
   int precision;   /* PCAONE_PREC_*                                                */
   int seed;        /* --seed (src/Cmd.hpp:86)                                      */
   int rand;        /* --rand: 1 Gaussian, 0 uniform(-1, 1) (src/Cmd.hpp:50)        */
@@ -204,7 +205,9 @@ int pcaone_gpu_gram_matvec(pcaone_gpu* h, int side, const double* x, double* y);
  * triplets by a restarted FP64 Lanczos on X^T X (nsamples > nfeatures) or
  * X X^T, ncv basis vectors, at most maxit restarts, Ritz residual tolerance
  * tol (--ncv, --imaxiter, --itol).  U (nsamples x k), S (k), V (nfeatures x k)
- * column-major; nconv = converged pairs, nops = mat-vec count. */
+ * column-major; nconv = converged pairs, nops = mat-vec count.  Fewer than k
+ * converged pairs is an error (PCAONE_ENUM, "the nconv is not equal to k.",
+ * src/Arnoldi.cpp:64); so is a zero singular value among the top k. */
 int pcaone_gpu_iram(pcaone_gpu* h, int k, int ncv, int maxit, double tol, double* U, double* S, double* V,
                     int* nconv, int* nops);
 

@@ -9,8 +9,9 @@ missing.
 from __future__ import annotations
 
 import ctypes as C
-import weakref
 import os
+import sys
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -116,11 +117,16 @@ def device_count() -> int:
 class _PinnedPool:
     """Page-locked host buffers for result arrays, reused by byte size: a
     buffer goes back to the pool when the last array viewing it is collected,
-    so repeated solves neither re-pin memory nor fault pageable pages in."""
+    so repeated solves neither re-pin memory nor fault pageable pages in.
+
+    Nothing here calls HIP while the interpreter shuts down: the finalizers
+    are not run at exit, and buffers still pooled or viewed then are left to
+    the process exit.  ``shutdown()`` frees the pooled ones explicitly."""
 
     def __init__(self, keep: int = 4):
         self.free: dict[int, list[int]] = {}
         self.keep = keep
+        self.closed = False
 
     def array(self, shape, order="C"):
         n = int(np.prod(shape))
@@ -134,21 +140,45 @@ class _PinnedPool:
                 return np.empty(shape, order=order)  # pageable: same results, slower copy-out
             ptr = out.value
         flat = np.ctypeslib.as_array((C.c_double * max(n, 1)).from_address(ptr))
-        weakref.finalize(flat, self._release, ptr, nbytes)
+        fin = weakref.finalize(flat, self._release, ptr, nbytes)
+        fin.atexit = False
         a = flat[:n]
         if order == "F":
             return a.reshape(tuple(reversed(shape))).T
         return a.reshape(shape)
 
     def _release(self, ptr: int, nbytes: int):
+        if self.closed or sys.is_finalizing():
+            return
         stack = self.free.setdefault(nbytes, [])
         if len(stack) < self.keep:
             stack.append(ptr)
         elif _lib is not None:
             _lib.pcaone_gpu_host_free(ptr)
 
+    def drain(self):
+        """Free every pooled buffer now (arrays still alive keep theirs)."""
+        self.closed = True
+        if _lib is not None:
+            for stack in self.free.values():
+                for ptr in stack:
+                    _lib.pcaone_gpu_host_free(ptr)
+        self.free.clear()
+
 
 _pinned = _PinnedPool()
+_handles: "weakref.WeakSet[Gpu]" = weakref.WeakSet()
+
+
+def shutdown():
+    """Close every open handle and free the pooled page-locked buffers, while
+    the HIP runtime is still fully alive.  Call it before the process ends
+    (the benchmark and the test session do); without it the memory is simply
+    returned by the process exit, and no HIP call runs during interpreter
+    shutdown."""
+    for g in list(_handles):
+        g.close()
+    _pinned.drain()
 
 
 def sim_bed(seed: int, nsamples: int, nsnps: int, npops: int = 3, fst: float = 0.1, pmin: float = 0.05,
@@ -257,13 +287,18 @@ class Gpu:
             raise GpuError(rc, f"pcaone_gpu_create({device}) failed")
         self.h = h
         self.N = self.M = 0
+        _handles.add(self)
 
     def close(self):
         if getattr(self, "h", None):
             self._L.pcaone_gpu_destroy(self.h)
             self.h = None
 
-    __del__ = close
+    def __del__(self):
+        # never tear the handle down from a finalizer during interpreter
+        # shutdown: the process exit releases its device memory
+        if not sys.is_finalizing():
+            self.close()
 
     def __enter__(self):
         return self

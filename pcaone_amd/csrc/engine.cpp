@@ -223,8 +223,8 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   PCAONE_REQUIRE(p.variant == 1 || p.variant == 2, -1, "rsvd_init: variant must be 1 (normal) or 2 (fancy)");
   PCAONE_REQUIRE(p.precision >= 0 && p.precision <= 2, -1, "rsvd_init: bad precision");
   PCAONE_REQUIRE(p.k >= 1 && p.os >= 0, -1, "rsvd_init: bad k / oversamples");
-  PCAONE_REQUIRE(p.bands >= 4 && (p.bands & (p.bands - 1)) == 0, -1,
-                 "the -w/--batches must be a power of 2 and the minimun is 4.");
+  // the reference's rule (src/Cmd.cpp:213-214): at least 4 and even
+  PCAONE_REQUIRE(p.bands >= 4 && p.bands % 2 == 0, -1, "the -w/--batches must be a power of 2 and the minimun is 4.");
   PCAONE_REQUIRE(p.ploidy == 1 || p.ploidy == 2, -1, "rsvd_init: ploidy must be 1 or 2");
   const uint32_t l = (uint32_t)(p.k + p.os);
   PCAONE_REQUIRE(l <= 64, -1, "rsvd_init: k + oversamples must be <= 64");
@@ -233,8 +233,14 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   if (src_ == SRC_DENSE) p_.standardize = 0;  // dense inputs are only centred (setFlags(false, false))
   l_ = l;
   // windows: winSVD uses blocksize = ceil(M / bands) (src/Halko.cpp:178); the
-  // plain RSVD uses the same cut only as a tiling of the H contraction.
-  bs_ = (uint32_t)ceil_div(M_, (uint64_t)p.bands);
+  // plain RSVD never reads -w (src/Halko.cpp:97-151) and cuts the SNPs into
+  // its own windows, only as a tiling of the H contraction: 64, fewer when
+  // there are fewer SNPs.
+  if (p.variant == 1) {
+    p_.bands = 64;
+    while (p_.bands > 4 && (uint64_t)p_.bands > M_) p_.bands /= 2;
+  }
+  bs_ = (uint32_t)ceil_div(M_, (uint64_t)p_.bands);
   nwin_ = (uint32_t)ceil_div(M_, bs_);
   if (p.variant == 2)
     PCAONE_REQUIRE(nwin_ == (uint32_t)p.bands, -1,
@@ -968,7 +974,7 @@ void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route
   seg_H(w0, w1, route, b);
 }
 
-void Engine::rsvd_gandh(int pi, double* Gout, double* Hout) {
+void Engine::rsvd_gandh(int pi, double* Gout, double* Hout, bool check) {
   PCAONE_REQUIRE(have_op_, -3, "computeGandH: operator not initialised");
   PCAONE_REQUIRE(pi >= 0, -1, "computeGandH: pi must be >= 0");
   PCAONE_HIP(hipSetDevice(device_));
@@ -981,12 +987,15 @@ void Engine::rsvd_gandh(int pi, double* Gout, double* Hout) {
     });
     have_vals_ = true;
   }
+  bool refreshed = false;
   if (p_.variant == 1) {
     // NormalRsvdOpData in-core, src/Halko.cpp:105-126
-    if (pi == 0)
+    if (pi == 0) {
       init_omega();
-    else
+    } else {
       omega_refresh(false, nullptr);
+      refreshed = true;
+    }
     timed("fill", [&] { launch_fill_f64(H_.p, (uint64_t)N_ * l_, 0.0, stream_); });
     run_segment(0, nwin_, std::vector<int>(nwin_, 1));
   } else {
@@ -1005,9 +1014,13 @@ void Engine::rsvd_gandh(int pi, double* Gout, double* Hout) {
       if (!st.update && b + 1 < nwin_) continue;
       run_segment(start, b + 1, route);
       start = b + 1;
-      if (st.update) omega_refresh(true, st.zero == 1 ? H1_.p : H2_.p);
+      if (st.update) {
+        omega_refresh(true, st.zero == 1 ? H1_.p : H2_.p);
+        refreshed = true;
+      }
     }
   }
+  if (check && refreshed) check_status();
   if (Gout || Hout) {
     sync();
     std::vector<double> tmp;
@@ -1081,7 +1094,7 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
     for (int i = 0; i <= maxp; ++i) diffs[i] = 0.0;
   bool done = false;
   for (int pi = 0; pi <= p; ++pi) {
-    rsvd_gandh(pi, nullptr, nullptr);
+    rsvd_gandh(pi, nullptr, nullptr, false);
     ++ep;
     usv_step();
     check_status();

@@ -101,7 +101,10 @@ class Engine {
   void feature_means(double* mu) const;
 
   void rsvd_init(const RsvdParams& p, const double* omega, const int32_t* perm);
-  void rsvd_gandh(int pi, double* G, double* H);
+  // computeGandH; `check` reads the CholeskyQR breakdown flag back when an
+  // Omega refresh ran (the public entry point), so a breakdown is reported by
+  // the call that caused it (computeUSV checks once per epoch instead)
+  void rsvd_gandh(int pi, double* G, double* H, bool check = true);
   void rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int* epochs, double* diffs);
   void rsvd_perm(int32_t* perm) const;
   void set_flags(bool update, bool standardize);

@@ -143,6 +143,7 @@ void Engine::eigs(int side, int k, int ncv, int maxit, double tol, double* evals
         for (uint64_t q = 0; q < n; ++q) ui[q] += c * vp[q];
       }
     std::copy(U.begin(), U.end(), V.begin());
+    PCAONE_REQUIRE(rnorm > 0.0, -4, "eigs: Krylov space exhausted at restart (zero residual)");
     for (uint64_t q = 0; q < n; ++q) vec(keep)[q] = r[q] / rnorm;
     std::fill(T.begin(), T.end(), 0.0);
     for (int i = 0; i < keep; ++i) T[(size_t)i * ncv + i] = theta[i];
@@ -170,9 +171,16 @@ void Engine::iram(int k, int ncv, int maxit, double tol, double* U, double* S, d
   const int side = (N_ > M_) ? 1 : 0;
   const uint64_t n = side ? M_ : N_;
   std::vector<double> lam(k), vecs((size_t)n * k);
-  eigs(side, k, ncv, maxit, tol, lam.data(), vecs.data(), nconv, nops);
+  int nc = 0;
+  eigs(side, k, ncv, maxit, tol, lam.data(), vecs.data(), &nc, nops);
+  if (nconv) *nconv = nc;
+  // run_pca_with_arnoldi stops here (src/Arnoldi.cpp:64)
+  PCAONE_REQUIRE(nc == k, -4, "the nconv is not equal to k.");
   std::vector<double> s(k);
-  for (int i = 0; i < k; ++i) s[i] = std::sqrt(std::max(lam[i], 0.0));
+  for (int i = 0; i < k; ++i) {
+    s[i] = std::sqrt(std::max(lam[i], 0.0));
+    PCAONE_REQUIRE(s[i] > 0.0, -4, "iram: zero singular value among the top k (rank-deficient matrix)");
+  }
   if (S) std::memcpy(S, s.data(), k * sizeof(double));
   std::vector<double> tmp(side ? N_ : M_);
   for (int i = 0; i < k; ++i) {

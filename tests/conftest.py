@@ -31,6 +31,7 @@ def gpu():
     g = pa.Gpu(0)
     yield g
     g.close()
+    pa.shutdown()
 
 
 def golden_cases():

@@ -223,11 +223,36 @@ def test_errors_are_reported(gpu):
     with pytest.raises(pa.GpuError):
         gpu.rsvd_init(20, 10, bands=4)       # 2 (k + os) > N
     with pytest.raises(pa.GpuError):
-        gpu.rsvd_init(2, 2, bands=6)         # bands not a power of two
+        gpu.rsvd_init(2, 2, bands=5)         # bands odd (src/Cmd.cpp:213)
+    with pytest.raises(pa.GpuError):
+        gpu.rsvd_init(2, 2, bands=2)         # bands below 4
     with pytest.raises(pa.GpuError):
         gpu.rsvd_init(2, 2, bands=512)       # more windows than SNPs
     with pytest.raises(pa.GpuError):
         gpu.rsvd_init(40, 40, bands=4)       # l > 64
+
+
+@pytest.mark.parametrize("bands", [6, 12])
+def test_even_bands_match_oracle(gpu, bands):
+    """winSVD with an even window count that is not a power of two, which the
+    reference accepts (src/Cmd.cpp:213): schedule, epochs and results equal
+    the oracle's; the plain variant ignores -w."""
+    n, m = 200, 2400
+    pay = _payload(61, n, m)
+    gpu.load_bed(pay, n, m)
+    X = oracle_matrix(pay, n, m)
+    gpu.rsvd_init(4, 6, variant=pa.RSVD_FANCY, bands=bands, precision=pa.PREC_FP64)
+    U, S, V, ep, diffs = gpu.usv(10, 1e-4)
+    orc = O.RsvdOracle(X, 4, 6, fancy=True, bands=bands)
+    epo = orc.compute_usv(10, 1e-4)
+    assert ep == epo
+    assert eig_relerr(S, orc.S) < 1e-9
+    assert O.principal_angles_max(V, orc.loadings_unpermuted()) < 1e-6
+    gpu.rsvd_init(4, 6, variant=pa.RSVD_NORMAL, bands=bands, precision=pa.PREC_FP64)
+    U1, S1, _, _, _ = gpu.usv(4, 0.0)
+    orc = O.RsvdOracle(X, 4, 6, fancy=False)
+    orc.compute_usv(4, 0.0)
+    assert eig_relerr(S1, orc.S) < 1e-9
 
 
 def test_split_bf16_vs_fp64_at_scale(gpu):
