@@ -227,7 +227,8 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   PCAONE_REQUIRE(p.bands >= 4 && p.bands % 2 == 0, -1, "the -w/--batches must be a power of 2 and the minimun is 4.");
   PCAONE_REQUIRE(p.ploidy == 1 || p.ploidy == 2, -1, "rsvd_init: ploidy must be 1 or 2");
   const uint32_t l = (uint32_t)(p.k + p.os);
-  PCAONE_REQUIRE(l <= 64, -1, "rsvd_init: k + oversamples must be <= 64");
+  PCAONE_REQUIRE(l <= kWideMaxL, -1, "rsvd_init: k + oversamples must be <= 128");
+  PCAONE_REQUIRE(p.k <= 64, -1, "rsvd_init: k must be <= 64");
   PCAONE_REQUIRE(l <= N_ && l <= M_, -1, "rsvd_init: k + oversamples exceeds the matrix dimensions");
   p_ = p;
   if (src_ == SRC_DENSE) p_.standardize = 0;  // dense inputs are only centred (setFlags(false, false))
@@ -280,7 +281,7 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
     nt16_ = (int)ceil_div(2 * l_, 16);
     bcols_ = prec == PREC_SPLIT_BF16 ? (uint32_t)nt16_ * 16 : l_;
     lo_off_ = l_;
-    fold_ = (prec == PREC_SPLIT_BF16 && (l_ % 16 == 0 || l_ % 16 == 8)) ? l_ : 0;
+    fold_ = (prec == PREC_SPLIT_BF16 && l_ <= 64 && (l_ % 16 == 0 || l_ % 16 == 8)) ? l_ : 0;
     ldp_ = prec == PREC_SPLIT_BF16 ? (uint64_t)nt16_ * 16 : l_;
     split_cols_ = prec == PREC_SPLIT_BF16;
     hf_ = 0.5;
@@ -320,9 +321,18 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   gpart_.reserve(gmax * ldp * partsz);
   hpart_.reserve(hmax * ldp * partsz);
   // linear algebra scratch
-  part_.reserve(std::max<uint64_t>({(uint64_t)1024 * l_ * l_, qr_part_doubles(l_), usv_part_doubles(l_)}));
-  gred_.reserve(usv_gram_doubles(l_));
-  qred_.reserve(usv_gram_doubles(l_));  // also holds the 3 x 16 ceil(l / 16) flip statistics
+  wide_ = l_ > 64;
+  part_.reserve(std::max<uint64_t>({(uint64_t)1024 * l_ * l_, qr_part_doubles(l_), usv_part_doubles(l_),
+                                    wide_ ? wide_part_doubles() : 0}));
+  // reduced Grams between passes: tile layout (l <= 64) or full l x l (wide);
+  // qred_ also holds the flip statistics
+  const uint64_t red = std::max<uint64_t>({usv_gram_doubles(l_), (uint64_t)l_ * l_, 3ull * kWideMaxL});
+  gred_.reserve(red);
+  qred_.reserve(red);
+  if (wide_) {
+    wq_.reserve((uint64_t)std::max(N_, M_) * l_);
+    eigtmp_.reserve((uint64_t)l_ * l_);
+  }
   bbt_.reserve((uint64_t)l_ * l_);
   for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &Rf_, &W_}) b->reserve((uint64_t)l_ * l_);
   sign_.reserve(l_);
@@ -686,6 +696,10 @@ void Engine::init_omega() {
 // fused into four passes that also write the contraction operand and the
 // FP32 gather table.
 void Engine::omega_refresh(bool from_two, double* zero_acc) {
+  if (wide_) {
+    omega_refresh_wide(from_two, zero_acc);
+    return;
+  }
   const SegBufs b = op_bufs();
   QrPass q{};
   q.H1 = from_two ? H1_.p : H_.p;
@@ -714,6 +728,43 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
   timed("qr_gram_q", [&] { launch_qr_pass(q, 1, stream_); });
   timed("qr_flip", [&] { launch_qr_pass(q, 2, stream_); });
   timed("qr_write", [&] { launch_qr_pass(q, 3, stream_); });
+}
+
+// The same refresh for l > 64 in general form: Gram of H (= H1 + H2, written),
+// R1^-1, Q1 = H R1^-1 (stored), Gram of Q1, R2^-1 with the Householder signs of
+// the top block of Q1 R2^-1, Omega_u = Q1 R2^-1 with flipOmg's statistics
+// against the previous Omega, then the signed, flipped Omega into every
+// consumer.
+void Engine::omega_refresh_wide(bool from_two, double* zero_acc) {
+  const SegBufs b = op_bufs();
+  const double* H1 = from_two ? H1_.p : H_.p;
+  const double* H2 = from_two ? H2_.p : nullptr;
+  timed("qr_gram_h", [&] { launch_wide_gram(H1, H2, H_.p, N_, l_, part_.p, qred_.p, stream_); });
+  timed("qr_chol", [&] { launch_wide_chol(qred_.p, l_, R1i_.p, status_.p, stream_); });
+  WideApply a1{H_.p, N_, l_, R1i_.p, wq_.p, nullptr, nullptr};
+  timed("qr_apply", [&] { launch_wide_apply(a1, nullptr, stream_); });
+  timed("qr_gram_q", [&] { launch_wide_gram(wq_.p, nullptr, nullptr, N_, l_, part_.p, qred_.p, stream_); });
+  timed("qr_chol", [&] {
+    launch_wide_r2(qred_.p, l_, R1i_.p, R2i_.p, Rti_.p, wq_.p, sign_.p, status_.p + 1, stream_);
+  });
+  WideApply a2{wq_.p, N_, l_, R2i_.p, Bt_.p, omega_.p, part_.p};
+  timed("qr_flip", [&] { launch_wide_apply(a2, qred_.p, stream_); });
+  WideWrite w{};
+  w.omu = Bt_.p;
+  w.n = N_;
+  w.l = l_;
+  w.stat = qred_.p;
+  w.sign = sign_.p;
+  w.omega = omega_.p;
+  w.colsum = colsum_.p;
+  w.zero_acc = zero_acc;
+  w.omega32 = (b.corrG && b.g2_f32) ? b.omega32 : nullptr;
+  w.bg = bg_.p;
+  w.bprec = p_.precision;
+  w.lo_off = lo_off_;
+  w.kpad = kg_;
+  w.kperm = kperm_(p_.precision);
+  timed("qr_write", [&] { launch_wide_write(w, stream_); });
 }
 
 // the FP32 table the missing-entry gathers of the G contraction read
@@ -1055,6 +1106,10 @@ void Engine::check_status() {
 // last pass needs it, for V), B^T = H R^-1, B B^T = U_B S^2 U_B^T,
 // Ucur = B^T U_B S^-1.
 void Engine::usv_step() {
+  if (wide_) {
+    usv_step_wide();
+    return;
+  }
   const uint32_t l = l_;
   UsvPass u{};
   u.G = G_.p;
@@ -1076,6 +1131,31 @@ void Engine::usv_step() {
   // later epochs start from the previous epoch's eigenvectors
   timed("usv_eig", [&] {
     launch_sym_eig(bbt_.p, 1, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, have_W_ ? W_.p : nullptr, stream_);
+  });
+  have_W_ = true;
+  timed("usv_rows", [&] {
+    launch_rows_times_small(Bt_.p, N_, l, W_.p, l, (uint32_t)p_.k, sinv_.p, Ucur_.p, stream_);
+  });
+}
+
+// usv_step for l > 64: CholeskyQR2 factors of G (Q1 = G R1^-1 stored),
+// B^T = H Rt^-1, the Gram B B^T and its eigensolve, Ucur = B^T U_B S^-1.
+void Engine::usv_step_wide() {
+  const uint32_t l = l_;
+  timed("usv_gram", [&] { launch_wide_gram(G_.p, nullptr, nullptr, M_, l, part_.p, gred_.p, stream_); });
+  timed("usv_chol", [&] { launch_wide_chol(gred_.p, l, R1i_.p, status_.p, stream_); });
+  WideApply a1{G_.p, M_, l, R1i_.p, wq_.p, nullptr, nullptr};
+  timed("usv_q1", [&] { launch_wide_apply(a1, nullptr, stream_); });
+  timed("usv_gram", [&] { launch_wide_gram(wq_.p, nullptr, nullptr, M_, l, part_.p, gred_.p, stream_); });
+  timed("usv_chol", [&] {
+    launch_wide_r2(gred_.p, l, R1i_.p, R2i_.p, Rti_.p, nullptr, nullptr, status_.p + 1, stream_);
+  });
+  WideApply a2{H_.p, N_, l, Rti_.p, Bt_.p, nullptr, nullptr};
+  timed("usv_bt", [&] { launch_wide_apply(a2, nullptr, stream_); });
+  timed("usv_gram", [&] { launch_wide_gram(Bt_.p, nullptr, nullptr, N_, l, part_.p, bbt_.p, stream_); });
+  timed("usv_eig", [&] {
+    launch_sym_eig_wide(bbt_.p, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, have_W_ ? W_.p : nullptr, eigtmp_.p,
+                        stream_);
   });
   have_W_ = true;
   timed("usv_rows", [&] {

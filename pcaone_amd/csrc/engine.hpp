@@ -133,8 +133,10 @@ class Engine {
   void draw_omega();
   void init_omega();
   void omega_refresh(bool from_two, double* zero_acc);
+  void omega_refresh_wide(bool from_two, double* zero_acc);  // l > 64
   void run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route);
   void usv_step();
+  void usv_step_wide();  // l > 64
   void check_status();
   void upload_scale();
   uint32_t rows_per_block_(int prec) const {
@@ -227,6 +229,10 @@ class Engine {
   DevBuf<double> qred_;        // Omega update: reduced partials between passes
   DevBuf<double> Bt_, evals_, W_, sinv_, Ucur_, Upre_, V_, scal_;
   DevBuf<double> out_;  // column-major U / V staged for the copy to the host
+  // l > 64: the tall-skinny steps run in general form (kernels_wide.hip)
+  bool wide_ = false;
+  DevBuf<double> wq_;      // max(N, M) x l: Q1 of the current factorisation
+  DevBuf<double> eigtmp_;  // l x l eigensolve scratch
   DevBuf<int> status_;
   // Gram mat-vec state (one operand column)
   bool have_mv_ = false;

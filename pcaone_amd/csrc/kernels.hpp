@@ -174,7 +174,8 @@ struct CsrGather {
   const int32_t* idx;
   const void* table;
   int table_f32;
-  uint32_t l;
+  uint32_t l;                // columns of the table and of out (row stride)
+  uint32_t col0, ncols;      // set by the launcher: the column window of one launch
   uint64_t in_base, stride_in, out_base;
   uint32_t per, ngroups;
   void* out;  // [.][l] of the table's type
@@ -338,6 +339,57 @@ void launch_rows_to_cols(const double* A, uint64_t n, uint32_t kc, const int32_t
 // W0 (optional, may alias W): start from this basis (the previous eigenvectors).
 void launch_sym_eig(const double* part, int nparts, uint32_t l, double* evals, double* W, double* sinv,
                     uint32_t kc, const double* W0, hipStream_t s);
+// ---- the same steps for l in (64, kWideMaxL] (kernels_wide.hip) -----------
+constexpr uint32_t kWideMaxL = 128;
+// scratch (doubles) the wide passes use for their partials
+uint64_t wide_part_doubles();
+int wide_gram_blocks(uint64_t n);
+// gram (l x l, full symmetric, row-major) = (A1 + A2)^T (A1 + A2) over n rows
+// of row-major n x l matrices; with A2 the sum is also written to Aout
+void launch_wide_gram(const double* A1, const double* A2, double* Aout, uint64_t n, uint32_t l, double* part,
+                      double* gram, hipStream_t s);
+// out = A S (A: n x l, S: l x l, row-major); with prev, flipOmg's column
+// statistics of out against prev (sum |prev - out|, sum |prev + out|, sum out,
+// 3 x kWideMaxL doubles) land in stat
+struct WideApply {
+  const double* A;
+  uint64_t n;
+  uint32_t l;
+  const double* S;
+  double* out;
+  const double* prev;  // optional
+  double* stat_part;   // block partials of the statistics (with prev)
+};
+void launch_wide_apply(const WideApply& w, double* stat, hipStream_t s);
+// R^-1 (l x l, upper) of the Cholesky factor of gram; first failing pivot into
+// *status when it is still 0
+void launch_wide_chol(const double* gram, uint32_t l, double* rinv, int* status, hipStream_t s);
+// R2^-1 from gram, Rt^-1 = R1^-1 R2^-1 and, with q1 (the first l rows of Q1),
+// the Householder signs of the top l x l block of Q1 R2^-1
+void launch_wide_r2(const double* gram, uint32_t l, const double* r1i, double* r2i, double* rti, const double* q1,
+                    double* sign, int* status, hipStream_t s);
+// Omega = omu diag(flip * sign) written to every consumer (as k_qr_write)
+struct WideWrite {
+  const double* omu;   // n x l unsigned Omega (Q1 R2^-1)
+  uint64_t n;
+  uint32_t l;
+  const double* stat;  // 3 x kWideMaxL flip statistics of omu against the previous Omega
+  const double* sign;  // l Householder signs
+  double* omega;
+  double* colsum;
+  double* zero_acc;    // optional accumulator to clear
+  float* omega32;      // optional FP32 copy
+  void* bg;
+  int bprec;
+  uint32_t lo_off;
+  uint64_t kpad;
+  int kperm;
+};
+void launch_wide_write(const WideWrite& q, hipStream_t s);
+// Jacobi eigensolve of the full symmetric l x l A (as launch_sym_eig); tmp: l x l scratch
+void launch_sym_eig_wide(const double* A, uint32_t l, double* evals, double* W, double* sinv, uint32_t kc,
+                         const double* W0, double* tmp, hipStream_t s);
+
 // mev partials: M = X^T Y (k x k) per block
 void launch_xty_partial(const double* X, const double* Y, uint64_t n, uint32_t k, double* part, int nblocks,
                         hipStream_t s);

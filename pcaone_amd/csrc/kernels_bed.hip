@@ -544,23 +544,40 @@ __global__ __launch_bounds__(512) void k_packed_gemm_bf16_stream(PackedGemm g) {
 // with the split-K chosen by resident rounds.  4-wave blocks for smaller ones.
 bool packed_bf16_streams(uint64_t nrows, uint32_t nz) { return nrows * nz >= 4096; }
 
+// One launch holds at most 8 column tiles in its accumulators (128 FP32
+// registers per lane at two waves per SIMD); wider operands (l > 64) run as
+// column blocks of at most 8 tiles, each reading the packed rows again, with B
+// and the output offset to the block's first column.
+constexpr int kMaxTilesPerLaunch = 8;
+
 void launch_packed_gemm_bf16(const PackedGemm& g, int nt16, hipStream_t s) {
   const bool big = packed_bf16_streams(g.nrows, g.nz);
-  PCAONE_REQUIRE(!g.fold || (big && (g.fold % 16 == 0 || g.fold % 16 == 8) && 2 * g.fold <= 16u * (uint32_t)nt16 &&
-                             g.ldo >= (g.fold + 15) / 16 * 16),
+  PCAONE_REQUIRE(nt16 >= 1 && nt16 <= 2 * kMaxTilesPerLaunch, -1,
+                 "split-BF16 contraction supports at most 256 operand columns (l <= 128)");
+  PCAONE_REQUIRE(!g.fold || (big && nt16 <= kMaxTilesPerLaunch && (g.fold % 16 == 0 || g.fold % 16 == 8) &&
+                             2 * g.fold <= 16u * (uint32_t)nt16 && g.ldo >= (g.fold + 15) / 16 * 16),
                  -1, "split contraction: fold needs the streamed kernel and a multiple of 8 within the operand");
+  PCAONE_REQUIRE(g.ldo >= 16ull * (uint64_t)nt16 || g.fold, -1, "split contraction: output rows too short");
   dim3 grid((uint32_t)ceil_div(g.nrows, big ? 512 : 256), g.nsplit, g.nz);
-  switch (nt16) {
+  const int nblk = (nt16 + kMaxTilesPerLaunch - 1) / kMaxTilesPerLaunch;
+  const int per = (nt16 + nblk - 1) / nblk;
+  for (int t0 = 0; t0 < nt16; t0 += per) {
+    PackedGemm gb = g;
+    const int nt = std::min(per, nt16 - t0);
+    gb.B = reinterpret_cast<const __bf16*>(g.B) + (uint64_t)t0 * 16 * g.ldb;
+    gb.out = reinterpret_cast<float*>(g.out) + (uint64_t)t0 * 16;
+    switch (nt) {
 #define PCAONE_NT(n)                                                                     \
   case n:                                                                                \
     if (big)                                                                             \
-      hipLaunchKernelGGL((k_packed_gemm_bf16_stream<n>), grid, dim3(512), 0, s, g);      \
+      hipLaunchKernelGGL((k_packed_gemm_bf16_stream<n>), grid, dim3(512), 0, s, gb);     \
     else                                                                                 \
-      hipLaunchKernelGGL((k_packed_gemm_bf16<n, 4>), grid, dim3(256), 0, s, g);          \
+      hipLaunchKernelGGL((k_packed_gemm_bf16<n, 4>), grid, dim3(256), 0, s, gb);         \
     break;
-    PCAONE_NT(1) PCAONE_NT(2) PCAONE_NT(3) PCAONE_NT(4) PCAONE_NT(5) PCAONE_NT(6) PCAONE_NT(7) PCAONE_NT(8)
+      PCAONE_NT(1) PCAONE_NT(2) PCAONE_NT(3) PCAONE_NT(4) PCAONE_NT(5) PCAONE_NT(6) PCAONE_NT(7) PCAONE_NT(8)
 #undef PCAONE_NT
-    default: throw Error(-1, "split-BF16 contraction supports at most 128 operand columns (l <= 64)");
+      default: throw Error(-1, "split-BF16 contraction: bad column block");
+    }
   }
 }
 
@@ -664,13 +681,13 @@ void launch_packed_gemm_f64(const PackedGemm& g, int ncols, hipStream_t s) { lau
 // k_window_colsum reduces into S''.  The operand (column-major per window) is
 // staged through LDS so its stores run along the window's SNPs.
 constexpr uint32_t kFinRows = 32;  // two 512-thread finalize blocks fit one CU
-constexpr uint32_t kTileLd = 65;
 
 // bh[w][c][k0 + p] for the operand positions p < 64 of a chunk, from
 // tile[q][c] (q < nq; rows past nq are window padding and get 0).  Each item
 // is four consecutive positions of one column, stored as one 8-byte (BF16
 // halves), 16-byte (FP32) or 32-byte (FP64) write; with kperm, position p
 // holds row kperm16(p) (a 4 x 4 transpose inside each 16-position group).
+template <int LD>
 __device__ inline void store_operand_tile(const double* tile, uint32_t nq, uint32_t l, uint32_t w, uint32_t k0,
                                           int bprec, void* bh, uint32_t bcols, uint32_t lo_off, uint64_t kw,
                                           int kperm) {
@@ -680,7 +697,7 @@ __device__ inline void store_operand_tile(const double* tile, uint32_t nq, uint3
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const uint32_t q = kperm ? (uint32_t)kperm16(p0 + b) : p0 + b;
-      v[b] = q < nq ? tile[q * kTileLd + c] : 0.0;
+      v[b] = q < nq ? tile[q * LD + c] : 0.0;
     }
     const uint64_t o = ((uint64_t)w * bcols + c) * kw + k0 + p0;  // k0 is a multiple of kFinRows
     if (bprec == PREC_FP64) {
@@ -731,21 +748,23 @@ __device__ inline void store_g2(const FinalizeG& f, uint64_t i, double v) {
     reinterpret_cast<double*>(f.g2)[i] = v;
 }
 
-// 1024 threads over the chunk's rows x l columns in flat (row, column) order,
+// 512 threads over the chunk's rows x l columns in flat (row, column) order,
 // so every lane has an element and the partial, correction, G and g''
-// accesses run along rows; each thread's elements (at most 4 for l <= 64)
-// have their loads in flight together, four split-K partials at a time.
+// accesses run along rows; each thread's elements (at most 4 for l <= 64, 8
+// for l <= 128: LW is the width class) have their loads in flight together,
+// four split-K partials at a time.
 constexpr int kFinThreads = 512;
-constexpr int kFinEl = kFinRows * 64 / kFinThreads;
 static_assert(kFinThreads / 64 == 8, "finalize_G sums its column slices eight at a time");
 
 // KS split-K slices per round (1 when the launch has one, so no load repeats);
 // LO: the partials hold separate hi and lo columns.
-template <typename P, int KS, bool LO>
+template <typename P, int KS, bool LO, int LW>
 __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
+  constexpr int kTileLd = LW + 1;
+  constexpr int kFinEl = kFinRows * LW / kFinThreads;
   __shared__ double tile[kFinRows * kTileLd];  // g' = d G, the H operand
   __shared__ double acct[kFinRows * kTileLd];  // the column-sum term (g'', or g'' - g' when affine)
-  __shared__ double sh[kFinThreads / 64][64];
+  __shared__ double sh[kFinThreads / 64][LW];
   const uint32_t t = threadIdx.x, l = f.l;
   const uint32_t w = f.w0 + blockIdx.y, chunk = blockIdx.x;
   const uint32_t wstart = w * f.blocksize;
@@ -826,11 +845,13 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
   }
   __syncthreads();
   {  // column sums of the chunk: 8 row slices, then the slices in a fixed order
-    const uint32_t c = t & 63, sl = t >> 6;
-    double a = 0.0;
-    if (c < l)
-      for (uint32_t q = sl; q < nq; q += kFinThreads / 64) a += acct[q * kTileLd + c];
-    sh[sl][c] = a;
+    const uint32_t sl = t >> 6;
+    for (uint32_t c = t & 63; c < (uint32_t)LW; c += 64) {
+      double a = 0.0;
+      if (c < l)
+        for (uint32_t q = sl; q < nq; q += kFinThreads / 64) a += acct[q * kTileLd + c];
+      sh[sl][c] = a;
+    }
   }
   __syncthreads();
   if (t < l) {
@@ -839,32 +860,39 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
     for (int i = 0; i < 8; ++i) v[i] = sh[i][t];
     f.s2p[((uint64_t)w * gridDim.x + chunk) * l + t] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
   }
-  store_operand_tile(tile, nq, l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
+  store_operand_tile<kTileLd>(tile, nq, l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
 }
 
-template <typename P>
+template <typename P, int LW>
 void launch_finalize_G_t(const FinalizeG& f, dim3 grid, hipStream_t s) {
   const bool one = f.nsplit == 1, lo = f.split_cols != 0;
   if (one && !lo)
-    hipLaunchKernelGGL((k_finalize_G<P, 1, false>), grid, dim3(kFinThreads), 0, s, f);
+    hipLaunchKernelGGL((k_finalize_G<P, 1, false, LW>), grid, dim3(kFinThreads), 0, s, f);
   else if (one)
-    hipLaunchKernelGGL((k_finalize_G<P, 1, true>), grid, dim3(kFinThreads), 0, s, f);
+    hipLaunchKernelGGL((k_finalize_G<P, 1, true, LW>), grid, dim3(kFinThreads), 0, s, f);
   else if (!lo)
-    hipLaunchKernelGGL((k_finalize_G<P, 4, false>), grid, dim3(kFinThreads), 0, s, f);
+    hipLaunchKernelGGL((k_finalize_G<P, 4, false, LW>), grid, dim3(kFinThreads), 0, s, f);
   else
-    hipLaunchKernelGGL((k_finalize_G<P, 4, true>), grid, dim3(kFinThreads), 0, s, f);
+    hipLaunchKernelGGL((k_finalize_G<P, 4, true, LW>), grid, dim3(kFinThreads), 0, s, f);
 }
 
 void launch_finalize_G(const FinalizeG& f, hipStream_t s) {
-  PCAONE_REQUIRE(f.l >= 1 && f.l <= 64, -1, "finalize: l must be in [1, 64]");
+  PCAONE_REQUIRE(f.l >= 1 && f.l <= 128, -1, "finalize: l must be in [1, 128]");
   PCAONE_REQUIRE(f.kw % kFinRows == 0, -1, "finalize: window operand rows must be whole chunks");
   PCAONE_REQUIRE((uint64_t)(f.r1 - f.r0) * f.ldp < (1ull << 32), -1, "finalize: partial block too large");
   if (f.w1 <= f.w0) return;
   dim3 grid((uint32_t)ceil_div(f.blocksize, kFinRows), f.w1 - f.w0);
-  if (f.part_is_f64)
-    launch_finalize_G_t<double>(f, grid, s);
-  else
-    launch_finalize_G_t<float>(f, grid, s);
+  if (f.l <= 64) {
+    if (f.part_is_f64)
+      launch_finalize_G_t<double, 64>(f, grid, s);
+    else
+      launch_finalize_G_t<float, 64>(f, grid, s);
+  } else {
+    if (f.part_is_f64)
+      launch_finalize_G_t<double, 128>(f, grid, s);
+    else
+      launch_finalize_G_t<float, 128>(f, grid, s);
+  }
 }
 
 uint32_t finalize_chunks(uint32_t blocksize) { return (uint32_t)ceil_div(blocksize, kFinRows); }
@@ -872,38 +900,45 @@ uint32_t finalize_chunks(uint32_t blocksize) { return (uint32_t)ceil_div(blocksi
 // H-contraction operand from a given G (rows [r0, r1) of windows [w0, w1)): the
 // output half of finalize_G for an externally supplied G, so X G can run
 // without a preceding X^T Omega (Gram mat-vec X^T X x starts here).
+template <int LW>
 __global__ __launch_bounds__(256) void k_operand_from_G(FinalizeG f) {
-  __shared__ double sh[4][64];
+  constexpr int kTileLd = LW + 1;
+  __shared__ double sh[4][LW];
   __shared__ double tile[kFinRows * kTileLd];
-  const uint32_t t = threadIdx.x, c = t & 63, rs = t >> 6;
+  const uint32_t t = threadIdx.x, rs = t >> 6;
   const uint32_t w = f.w0 + blockIdx.y, chunk = blockIdx.x;
   const uint32_t wstart = w * f.blocksize;
   const uint32_t wend = min(wstart + f.blocksize, f.nsnps);
   const uint32_t cb = wstart + chunk * kFinRows;
   const uint32_t nq = cb < wend ? min(kFinRows, wend - cb) : 0;
-  double acc2 = 0.0;
-  if (c < f.l) {
-    for (uint32_t q = rs; q < nq; q += 4) {
-      const uint32_t r = cb + q;
-      const double gp = f.d[r] * f.G[(uint64_t)r * f.l + c];
-      const double g2 = 2.0 * f.F[r] * gp;
-      store_g2(f, (uint64_t)r * f.l + c, g2);
-      acc2 += f.affine ? g2 - gp : g2;
-      tile[q * kTileLd + c] = gp;
+  for (uint32_t c = t & 63; c < (uint32_t)LW; c += 64) {
+    double acc2 = 0.0;
+    if (c < f.l) {
+      for (uint32_t q = rs; q < nq; q += 4) {
+        const uint32_t r = cb + q;
+        const double gp = f.d[r] * f.G[(uint64_t)r * f.l + c];
+        const double g2 = 2.0 * f.F[r] * gp;
+        store_g2(f, (uint64_t)r * f.l + c, g2);
+        acc2 += f.affine ? g2 - gp : g2;
+        tile[q * kTileLd + c] = gp;
+      }
     }
+    sh[rs][c] = acc2;
   }
-  sh[rs][c] = acc2;
   __syncthreads();
   if (t < f.l)
     f.s2p[((uint64_t)w * gridDim.x + chunk) * f.l + t] = (sh[0][t] + sh[1][t]) + (sh[2][t] + sh[3][t]);
-  store_operand_tile(tile, nq, f.l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
+  store_operand_tile<kTileLd>(tile, nq, f.l, w, cb - wstart, f.bprec, f.bh, f.bcols, f.lo_off, f.kw, f.kperm);
 }
 
 void launch_operand_from_G(const FinalizeG& f, hipStream_t s) {
-  PCAONE_REQUIRE(f.l >= 1 && f.l <= 64, -1, "operand: l must be in [1, 64]");
+  PCAONE_REQUIRE(f.l >= 1 && f.l <= 128, -1, "operand: l must be in [1, 128]");
   if (f.w1 <= f.w0) return;
   dim3 grid((uint32_t)ceil_div(f.blocksize, kFinRows), f.w1 - f.w0);
-  hipLaunchKernelGGL(k_operand_from_G, grid, dim3(256), 0, s, f);
+  if (f.l <= 64)
+    hipLaunchKernelGGL(k_operand_from_G<64>, grid, dim3(256), 0, s, f);
+  else
+    hipLaunchKernelGGL(k_operand_from_G<128>, grid, dim3(256), 0, s, f);
 }
 
 // S2[w][c] = sum of the chunk partials of window w: 16 slices of chunks (all
@@ -912,7 +947,7 @@ __global__ __launch_bounds__(1024) void k_window_colsum(const double* __restrict
                                                         uint32_t w0, double* __restrict__ S2) {
   constexpr int kSlices = 16, kPer = 8;
   __shared__ double sh[kSlices][64];
-  const uint32_t w = w0 + blockIdx.x, c = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const uint32_t w = w0 + blockIdx.x, cl = threadIdx.x & 63, c = blockIdx.y * 64 + cl, sl = threadIdx.x >> 6;
   double tot = 0.0;
   if (c < l)
     for (uint32_t q0 = sl; q0 < nchunk; q0 += kSlices * kPer) {
@@ -925,21 +960,22 @@ __global__ __launch_bounds__(1024) void k_window_colsum(const double* __restrict
 #pragma unroll
       for (int i = 0; i < kPer; ++i) tot += v[i];
     }
-  sh[sl][c] = tot;
+  sh[sl][cl] = tot;
   __syncthreads();
-  if (threadIdx.x < l) {
+  if (threadIdx.x < 64 && c < l) {
     double v[kSlices / 2];
 #pragma unroll
-    for (int i = 0; i < kSlices / 2; ++i) v[i] = sh[2 * i][c] + sh[2 * i + 1][c];
+    for (int i = 0; i < kSlices / 2; ++i) v[i] = sh[2 * i][cl] + sh[2 * i + 1][cl];
     S2[(uint64_t)w * l + c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
   }
 }
 
 void launch_window_colsum(const double* s2p, uint32_t l, uint32_t blocksize, uint32_t w0, uint32_t w1, double* S2,
                           hipStream_t s) {
-  PCAONE_REQUIRE(l >= 1 && l <= 64, -1, "l out of range");
+  PCAONE_REQUIRE(l >= 1 && l <= 128, -1, "l out of range");
   if (w1 <= w0) return;
-  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0), dim3(1024), 0, s, s2p, l, finalize_chunks(blocksize), w0, S2);
+  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0, (uint32_t)ceil_div(l, 64)), dim3(1024), 0, s, s2p, l,
+                     finalize_chunks(blocksize), w0, S2);
 }
 
 // thread (sample j, column c); the windows' contributions are summed in
@@ -1050,20 +1086,18 @@ __global__ void k_emu_update(const int64_t* __restrict__ ptr, const int32_t* __r
                              double* __restrict__ val) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nsnps) return;
-  double vs[64];
-  for (uint32_t c = 0; c < k; ++c) vs[c] = S[c] * V[i * k + c];
   const double lo = -F[i], hi = 1.0 - F[i];
   for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) {
     const uint64_t j = (uint64_t)idx[p];
     double e = 0.0;
-    for (uint32_t c = 0; c < k; ++c) e += U[j * k + c] * vs[c];
+    for (uint32_t c = 0; c < k; ++c) e += U[j * k + c] * (S[c] * V[i * k + c]);
     val[p] = fmin(fmax(e, lo), hi);
   }
 }
 
 void launch_emu_update(const int64_t* miss_ptr, const int32_t* miss_idx, uint32_t nsnps, const double* U,
                        const double* S, const double* V, uint32_t k, const double* F, double* val, hipStream_t s) {
-  PCAONE_REQUIRE(k >= 1 && k <= 64, -1, "EMU update supports k <= 64");
+  PCAONE_REQUIRE(k >= 1 && k <= 128, -1, "EMU update supports k <= 128");
   hipLaunchKernelGGL(k_emu_update, dim3((uint32_t)ceil_div(nsnps, 256)), dim3(256), 0, s, miss_ptr, miss_idx,
                      nsnps, U, S, V, k, F, val);
 }
@@ -1114,11 +1148,11 @@ __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
   typedef T vec __attribute__((ext_vector_type(V)));
   constexpr int D = 2;  // entries per group per round
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int lpg = (int)((g.l + V - 1) / V);
+  const int lpg = (int)((g.ncols + V - 1) / V);
   const int ng = 64 / lpg;
   const int grp = lane / lpg, li = lane - grp * lpg;
-  const bool act = grp < ng && (uint32_t)(li * V) < g.l;
-  const uint32_t coff = act ? (uint32_t)(li * V) : 0;
+  const bool act = grp < ng && (uint32_t)(li * V) < g.ncols;
+  const uint32_t coff = act ? g.col0 + (uint32_t)(li * V) : g.col0;
   const T* __restrict__ tab = reinterpret_cast<const T*>(g.table);
   const uint64_t total = (uint64_t)g.per * g.ngroups;
   for (uint64_t o = (uint64_t)blockIdx.x * 4 + wv; o < total; o += (uint64_t)gridDim.x * 4) {
@@ -1166,24 +1200,33 @@ __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
   }
 }
 
-void launch_csr_gather(const CsrGather& g, hipStream_t s) {
-  PCAONE_REQUIRE(g.l >= 1 && g.l <= 64, -1, "gather: l must be in [1, 64]");
-  const uint64_t total = (uint64_t)g.per * g.ngroups;
+void launch_csr_gather(const CsrGather& g0, hipStream_t s) {
+  PCAONE_REQUIRE(g0.l >= 1 && g0.l <= 128, -1, "gather: l must be in [1, 128]");
+  const uint64_t total = (uint64_t)g0.per * g0.ngroups;
   if (!total) return;
   const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(total, 4), 32768);
-  // 16-byte vector loads need rows that are whole vectors (tables are hipMalloc'd)
-  if (g.table_f32) {
-    if (g.l % 4 == 0)
-      hipLaunchKernelGGL((k_csr_gather<float, 4>), dim3(blocks), dim3(256), 0, s, g);
-    else if (g.l % 2 == 0)
-      hipLaunchKernelGGL((k_csr_gather<float, 2>), dim3(blocks), dim3(256), 0, s, g);
-    else
-      hipLaunchKernelGGL((k_csr_gather<float, 1>), dim3(blocks), dim3(256), 0, s, g);
-  } else {
-    if (g.l % 2 == 0)
-      hipLaunchKernelGGL((k_csr_gather<double, 2>), dim3(blocks), dim3(256), 0, s, g);
-    else
-      hipLaunchKernelGGL((k_csr_gather<double, 1>), dim3(blocks), dim3(256), 0, s, g);
+  // 16-byte vector loads need rows that are whole vectors (tables are hipMalloc'd);
+  // a lane group spans at most the 64 lanes of a wave, so rows of single
+  // columns wider than 64 run as two column windows
+  const int V = g0.table_f32 ? (g0.l % 4 == 0 ? 4 : g0.l % 2 == 0 ? 2 : 1) : (g0.l % 2 == 0 ? 2 : 1);
+  const uint32_t wmax = 64u * (uint32_t)V;
+  for (uint32_t c0 = 0; c0 < g0.l; c0 += wmax) {
+    CsrGather g = g0;
+    g.col0 = c0;
+    g.ncols = std::min(wmax, g0.l - c0);
+    if (g.table_f32) {
+      if (V == 4)
+        hipLaunchKernelGGL((k_csr_gather<float, 4>), dim3(blocks), dim3(256), 0, s, g);
+      else if (V == 2)
+        hipLaunchKernelGGL((k_csr_gather<float, 2>), dim3(blocks), dim3(256), 0, s, g);
+      else
+        hipLaunchKernelGGL((k_csr_gather<float, 1>), dim3(blocks), dim3(256), 0, s, g);
+    } else {
+      if (V == 2)
+        hipLaunchKernelGGL((k_csr_gather<double, 2>), dim3(blocks), dim3(256), 0, s, g);
+      else
+        hipLaunchKernelGGL((k_csr_gather<double, 1>), dim3(blocks), dim3(256), 0, s, g);
+    }
   }
 }
 

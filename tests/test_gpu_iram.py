@@ -73,3 +73,19 @@ def test_iram_matches_reference_cli(gpu):
     m = int(z["nsnps"])
     assert eig_relerr(S ** 2 / m * 2, z["eigvals"]) < 5e-6  # 6 printed digits
     assert O.principal_angles_max(U[:, :2], z["eigvecs"][:, :2]) < 1e-4
+
+
+def test_iram_k50_ncv101_matches_exact_svd(gpu):
+    """The C4 IRAM setting (top-50, ncv = 2 k + 1 = 101, src/Cmd.cpp:189) on a
+    dense centred matrix: eigenvalues and the leading subspace against exact
+    FP64 SVD."""
+    D = pa.sim_counts(31, 1200, 700, base=0.2)
+    gpu.load_dense(D)
+    gpu.gram_init(pa.PREC_FP64)
+    U, S, V, nconv, nops = gpu.iram(50, ncv=101, tol=1e-8)
+    assert nconv == 50
+    X = O.dense_centered(D)
+    u, s, vt = np.linalg.svd(X, full_matrices=False)
+    assert eig_relerr(S ** 2, s[:50] ** 2) < 1e-9
+    assert O.principal_angles_max(U[:, :5], u[:, :5]) < 1e-6
+    assert np.allclose(U.T @ U, np.eye(50), atol=1e-8)

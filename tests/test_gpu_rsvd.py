@@ -217,6 +217,167 @@ def test_explicit_omega_and_permutation(gpu):
     assert O.principal_angles_max(V, orc.loadings_unpermuted()) < 1e-6
 
 
+# l = k + os across the contraction's tile counts: folded hi + lo epilogue
+# (l % 16 in {0, 8}, l <= 64), the one-wave and whole-block l x l steps, and
+# the general path above 64 columns (column-blocked contraction, wide
+# linear algebra).  M = 6000 SNPs in 64 windows and N = 300 samples make the
+# 64-window H products (N x 64 rows) and the long G segments run the streamed
+# kernel, the short ones the 4-wave kernel.
+L_SWEEP = [16, 32, 40, 48, 64, 100, 128]
+_sweep_cache = {}
+
+
+def _sweep_oracle(l):
+    if l not in _sweep_cache:
+        n, m = 300, 6000
+        pay = _payload(90 + l, n, m, npops=3, fst=0.1, miss=0.01)
+        X = oracle_matrix(pay, n, m)
+        k = l // 2
+        orc = O.RsvdOracle(X, k, l - k, fancy=True, bands=64)
+        gh = [tuple(x.copy() for x in orc.compute_gandh(pi)) for pi in range(3)]
+        orc2 = O.RsvdOracle(X, k, l - k, fancy=True, bands=64)
+        epo = orc2.compute_usv(7, 1e-4)
+        _sweep_cache[l] = (pay, n, m, k, gh, orc2, epo)
+    return _sweep_cache[l]
+
+
+# Entrywise G / H after Omega refreshes are compared in the FP64 arm only.
+# Householder's R-diagonal sign (beta = -sign(pivot)) and flipOmg are
+# discontinuous: a column whose pivot is near zero changes sign under any
+# perturbation above FP64 rounding, and for noise columns (uncorrelated with
+# the previous Omega) flipOmg does not undo it.  Emulating only the BF16 hi + lo
+# rounding of the operands inside the FP64 oracle (N = 300, M = 6000, 64
+# windows) already moves max |G| entries by 4e-4 at l = 64 and by O(0.1) at
+# l >= 100 after one epoch, so the FP32 and split arms are held to the exact
+# error bound of one refresh-free product (test_split_error_bound) and, end
+# to end, to eigenvalues and subspaces (test_usv_l_sweep).
+
+
+@pytest.mark.parametrize("l", L_SWEEP)
+def test_gandh_l_sweep(gpu, l):
+    pay, n, m, k, gh, _, _ = _sweep_oracle(l)
+    gpu.load_bed(pay, n, m)
+    gpu.rsvd_init(k, l - k, variant=pa.RSVD_FANCY, bands=64, precision=pa.PREC_FP64)
+    for pi, (Go, Ho) in enumerate(gh):
+        G, H = gpu.gandh(pi)
+        eg = np.abs(G - Go).max() / np.abs(Go).max()
+        eh = np.abs(H - Ho).max() / np.abs(Ho).max()
+        assert eg < 1e-10 and eh < 1e-10, (l, pi, eg, eh)  # FP64 rounding through cond(H) <= 50 refreshes
+
+
+U16, U24 = 2.0 ** -16, 2.0 ** -24
+
+
+@pytest.mark.parametrize("arm", ["fp32", "bf16x2"])
+@pytest.mark.parametrize("source", ["bed", "dense"])
+@pytest.mark.parametrize("l", [20, 40, 64, 100])
+def test_split_error_bound(gpu, l, source, arm):
+    """G = X^T Omega and H = X G of the first plain-RSVD call against an
+    entrywise error bound derived from the arithmetic, not fitted:
+
+    * an FP64 value split into BF16 hi + lo keeps it to 2^-16 relative
+      (each rounding to BF16 is within 2^-8 of its input); FP32 rounding is
+      2^-24 relative;
+    * packed genotypes enter as y' = 2 - 2 y (|y'| <= 2, exact), and
+      G_i = d_i / 2 (colsum - sum_j y'_ij Omega_j / 2 - ...): the contraction
+      error is d_i / 4 times that of sum_j y'_ij Omega_j;
+    * dense values are split too and three of the four hi/lo products are
+      kept: |ab - (ahbh + ahbl + albh)| <= 3 * 2^-16 |a| |b|;
+    * FP32 accumulation over K terms adds at most K * 2^-24 of sum |terms|;
+    * H = X G carries X's application to G's error plus its own operand and
+      accumulation error on the operand d G.
+    A dropped lo half (2^-8 per entry) breaks the bound by two orders of
+    magnitude."""
+    k = l // 2
+    if source == "bed":
+        n, m = 300, 5000
+        pay = _payload(300 + l, n, m, miss=0.01)
+        gpu.load_bed(pay, n, m)
+        codes = O.decode_codes(pay, n, m)
+        F, _ = O.allele_freq(codes)
+        X = oracle_matrix(pay, n, m)
+        sd = np.sqrt(2 * F * (1 - F))
+        d = np.where(sd > 1e-9, 1.0 / sd, 1.0)
+        A = np.full((n, m), 2.0) * d / 4          # G side: |y'| d / 4 <= 2 d / 4
+        Ah = np.full((n, m), 0.5)                  # H side: |y'| / 4 on the operand d G
+        opnd = U16 if arm == "bf16x2" else U24
+        xop = 0.0                                  # the genotypes are exact
+    else:
+        n, m = 360, 3000
+        D = pa.sim_counts(40 + l, n, m, base=0.1)
+        gpu.load_dense(D)
+        X = O.dense_centered(D)
+        A = np.abs(D.T.astype(np.float64))
+        Ah = A
+        d = np.ones(m)
+        opnd = U16 if arm == "bf16x2" else U24
+        xop = 2 * U16 if arm == "bf16x2" else 0.0  # the split of D itself (FP32 D is exact in FP32)
+    gpu.rsvd_init(k, l - k, variant=pa.RSVD_NORMAL, precision=ARMS[arm])
+    G, H = gpu.gandh(0)
+    orc = O.RsvdOracle(X, k, l - k, fancy=False)
+    Go, Ho = orc.compute_gandh(0)
+    Om = np.abs(orc.omg)
+    eG = (opnd + xop + n * U24) * (A.T @ Om) * 1.01 + 1e-12 * np.abs(Go).max()
+    assert np.all(np.abs(G - Go) <= eG), float(np.max(np.abs(G - Go) / eG))
+    gp = np.abs(d[:, None] * G) if source == "bed" else np.abs(G)
+    eH = (np.abs(X) @ eG + (opnd + xop + m * U24) * (Ah @ gp)) * 1.01 + 1e-12 * np.abs(Ho).max()
+    assert np.all(np.abs(H - Ho) <= eH), float(np.max(np.abs(H - Ho) / eH))
+    # the bound is not vacuous: it sits well below max |entry|
+    assert eG.max() < 1e-2 * np.abs(Go).max() and eH.max() < 1e-2 * np.abs(Ho).max()
+
+
+@pytest.mark.parametrize("arm", list(ARMS))
+@pytest.mark.parametrize("l", L_SWEEP)
+def test_usv_l_sweep(gpu, l, arm):
+    pay, n, m, k, _, orc, epo = _sweep_oracle(l)
+    gpu.load_bed(pay, n, m)
+    gpu.rsvd_init(k, l - k, variant=pa.RSVD_FANCY, bands=64, precision=ARMS[arm])
+    U, S, V, ep, diffs = gpu.usv(7, 1e-4)
+    # all k in the FP64 arm; in the others the structured PCs (npops - 1 = 2):
+    # the unconverged estimates of the noise bulk after 7 epochs depend on the
+    # sign decisions above (at l = 100: 9e-5 apart in the split arm, the two
+    # structured eigenvalues 1e-8)
+    top = k if arm == "fp64" else 2
+    assert eig_relerr(S[:top] ** 2, orc.S[:top] ** 2) < EIG_TOL[arm]
+    assert O.principal_angles_max(U[:, :2], orc.U[:, :2]) < (1e-7 if arm == "fp64" else 1e-4)
+    assert np.allclose(U.T @ U, np.eye(k), atol=1e-9)
+    assert np.allclose(V.T @ V, np.eye(k), atol=1e-9)
+    if arm == "fp64":
+        assert ep == epo
+        n_d = min(len(orc.diffs), 7)
+        assert np.allclose(diffs[1:1 + n_d], orc.diffs[:n_d], rtol=1e-7, atol=1e-11)
+
+
+@pytest.mark.parametrize("arm", list(ARMS))
+@pytest.mark.parametrize("variant", [pa.RSVD_NORMAL, pa.RSVD_FANCY])
+def test_dense_l100_matches_oracle(gpu, arm, variant):
+    """Dense input at the C4 operator width (k = 50, l = 100): G / H of the
+    first epochs and the converged eigenvalues against the oracle."""
+    D = pa.sim_counts(12, 420, 2600, base=0.1)
+    gpu.load_dense(D)
+    X = O.dense_centered(D)
+    gpu.rsvd_init(50, 50, variant=variant, bands=16, precision=ARMS[arm])
+    orc = O.RsvdOracle(X, 50, 50, fancy=variant == pa.RSVD_FANCY, bands=16)
+    tol = {"fp64": 1e-10, "fp32": 3e-5, "bf16x2": 1e-4}[arm]
+    # entrywise only where it is well posed (see test_gandh_l_sweep): every
+    # epoch in the FP64 arm, before the first Omega refresh in the others
+    npi = 3 if arm == "fp64" else (1 if variant == pa.RSVD_NORMAL else 0)
+    for pi in range(npi):
+        G, H = gpu.gandh(pi)
+        Go, Ho = orc.compute_gandh(pi)
+        eg = np.abs(G - Go).max() / np.abs(Go).max()
+        eh = np.abs(H - Ho).max() / np.abs(Ho).max()
+        assert eg < tol and eh < tol, (pi, eg, eh)
+    gpu.rsvd_init(50, 50, variant=variant, bands=16, precision=ARMS[arm])
+    orc = O.RsvdOracle(X, 50, 50, fancy=variant == pa.RSVD_FANCY, bands=16)
+    U, S, V, ep, _ = gpu.usv(6, 1e-5)
+    epo = orc.compute_usv(6, 1e-5)
+    top = 50 if arm == "fp64" else 3
+    assert eig_relerr(S[:top] ** 2, orc.S[:top] ** 2) < EIG_TOL[arm]
+    if arm == "fp64":
+        assert ep == epo
+
+
 def test_errors_are_reported(gpu):
     pay = _payload(2, 40, 400)
     gpu.load_bed(pay, 40, 400)
@@ -228,8 +389,11 @@ def test_errors_are_reported(gpu):
         gpu.rsvd_init(2, 2, bands=2)         # bands below 4
     with pytest.raises(pa.GpuError):
         gpu.rsvd_init(2, 2, bands=512)       # more windows than SNPs
+    gpu.load_bed(_payload(2, 400, 400), 400, 400)
     with pytest.raises(pa.GpuError):
-        gpu.rsvd_init(40, 40, bands=4)       # l > 64
+        gpu.rsvd_init(60, 69, bands=4)       # l > 128
+    with pytest.raises(pa.GpuError):
+        gpu.rsvd_init(65, 0, bands=4)        # k > 64
 
 
 @pytest.mark.parametrize("bands", [6, 12])
