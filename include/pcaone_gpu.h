@@ -190,6 +190,26 @@ int pcaone_gpu_ld_usv(pcaone_gpu* h, const double* U, uint32_t k);
 int pcaone_gpu_ld_r2(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2,
                      uint64_t capacity, uint64_t* npairs);
 
+/* LD r^2 straight from the genotypes of pcaone_gpu_load_bed, as the
+ * reference's two-step route gives it: a run with --ld --ld-stats 1 writes the
+ * centred genotypes BED2GENO - F (missing 0, unstandardised, each SNP
+ * re-centred by its mean; Data::write_residuals, src/Data.cpp:196-221) to
+ * .residuals, and -B .residuals --print-r2 runs ld_r2_big on them
+ * (src/LD.cpp:419-442).  Here no matrix is materialised: each window pair's
+ * integer sums over samples (S_yy = sum y_a y_b, S_yw = sum y_a w_b,
+ * S_wy = sum w_a y_b, S_ww = sum w_a w_b; y the A1 dosage 0/1/2 with missing 0,
+ * w the non-missing mask) come exactly from int8 MFMA with int32
+ * accumulation, and r^2 follows from them in FP64.  Pairs and their order as
+ * for pcaone_gpu_ld_r2 (windows from pcaone_ld_windows).  With r2 == NULL only
+ * *npairs is set.  sums (optional, npairs x 4 int32, row per pair in the
+ * order above) receives the integer sums.  tile_config (optional): on input a
+ * tile-shape index >= 0 to use, or < 0 / NULL to autotune (every candidate
+ * timed on a prefix of the windows, the choice kept per window-size class of
+ * the handle); on return the index used.  A monomorphic or all-missing SNP
+ * has r^2 NaN, as the reference's 0 * (1 / 0). */
+int pcaone_gpu_ld_r2_geno(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2,
+                          uint64_t capacity, uint64_t* npairs, int32_t* sums, int* tile_config);
+
 /* ---- IRAM: the Spectra op and its driver ----------------------------------
  * Prepares the single-column Gram mat-vec on the loaded matrix (identity
  * SNP order; standardize / ploidy as for the RSVD operator).  Re-initialises

@@ -30,6 +30,9 @@ the reference's behaviour, of:
   src/Halko.cpp:294-316)
 * LD windows and r^2, the --USV adjustment    (src/LD.cpp:20-23, 125-139,
   419-442, 460-466)
+* the --ld-stats 1 residual matrix of PLINK input and its r^2 from integer
+  sums                                         (src/Data.cpp:196-221,
+  src/LD.cpp:419-442)
 
 The third-party pieces restated here come from libstdc++ (GCC 11.4, the
 toolchain the reference builds with): the Park-Miller minimal-standard LCG
@@ -664,6 +667,62 @@ def ld_r2(featmajor: np.ndarray, ws, we, centre: bool = True) -> np.ndarray:
         r = (X[k] @ X[i]) * (sds[i] * sds[k] * df)
         out.append(r * r)
     return np.concatenate(out) if out else np.zeros(0)
+
+
+def ld_stats1_matrix(codes: np.ndarray) -> np.ndarray:
+    """The matrix Data::write_residuals writes with --ld-stats 1 for in-core
+    PLINK input (src/Data.cpp:196-221): FileBed::read_all's G = BED2GENO - F
+    with missing 0 (unstandardised: --ld sets setFlags(false, false),
+    src/Halko.cpp:284-288), each SNP centred by its mean over samples;
+    returned feature-major (M x N) in FP64 (the reference then casts to
+    float32)."""
+    F, _ = allele_freq(codes)
+    G = centered_matrix(codes, F)
+    G = G - G.mean(axis=0, keepdims=True)
+    return np.ascontiguousarray(G.T)
+
+
+def ld_geno_sums(codes: np.ndarray, ws, we) -> np.ndarray:
+    """The integer sums behind the --ld-stats 1 r^2 of each window pair (i, k),
+    window after window: with y the A1 dosage (0, 1, 2; missing 0) and w the
+    non-missing mask, (sum y_i y_k, sum y_i w_k, sum w_i y_k, sum w_i w_k) over
+    samples, as int64 (npairs x 4).  Brute force: one dot product per pair."""
+    y = np.where(codes == 0, 2, np.where(codes == 2, 1, 0)).astype(np.int64)  # M x N
+    w = (codes != BED_MISSING_CODE).astype(np.int64)
+    out = []
+    for i, e in zip(ws, we):
+        for k in range(i + 1, i + e):
+            out.append((int(y[i] @ y[k]), int(y[i] @ w[k]), int(w[i] @ y[k]), int(w[i] @ w[k])))
+    return np.array(out, dtype=np.int64).reshape(-1, 4)
+
+
+def ld_geno_r2(codes: np.ndarray, ws, we, sums: np.ndarray | None = None) -> np.ndarray:
+    """r^2 of ld_r2_big (src/LD.cpp:419-442) on ld_stats1_matrix, from the
+    integer sums in closed form.  With a = w (y / 2 - F) (the centred,
+    missing-zero genotype), N mu = sum a = sum y / 2 - F n and
+        sum a_i a_k = S_yy / 4 - F_k S_yw / 2 - F_i S_wy / 2 + F_i F_k S_ww,
+        cov = sum a_i a_k - (N mu_i)(N mu_k) / N,
+        ss_i = sum y_i^2 / 4 - F_i sum y_i + F_i^2 n_i - (N mu_i)^2 / N,
+    r^2 = cov^2 / (ss_i ss_k) (the (N - 1) factors of calc_cor cancel)."""
+    N = codes.shape[1]
+    F, _ = allele_freq(codes)
+    y = np.where(codes == 0, 2, np.where(codes == 2, 1, 0)).astype(np.int64)
+    w = (codes != BED_MISSING_CODE).astype(np.int64)
+    sy, syy, nn = y.sum(axis=1), (y * y).sum(axis=1), w.sum(axis=1)
+    mun = 0.5 * sy - F * nn
+    ss = 0.25 * syy - F * sy + F * F * nn - mun * mun / N
+    if sums is None:
+        sums = ld_geno_sums(codes, ws, we)
+    ia = np.concatenate([np.full(e - 1, i) for i, e in zip(ws, we) if e > 1]) if len(sums) else np.zeros(0, int)
+    ka = np.concatenate([np.arange(i + 1, i + e) for i, e in zip(ws, we) if e > 1]) if len(sums) else np.zeros(0, int)
+    s = sums.astype(np.float64)
+    cov = (0.25 * s[:, 0] - 0.5 * F[ka] * s[:, 1] - 0.5 * F[ia] * s[:, 2] + F[ia] * F[ka] * s[:, 3]
+           - mun[ia] * mun[ka] / N)
+    # a SNP without variation (monomorphic or all missing; ss is then exactly
+    # 0) has an all-zero residual column and r = 0 * (1 / 0) = NaN in the
+    # reference (calc_sds, ld_r2_big: src/LD.cpp:20-23, 427)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where((ss[ia] > 0) & (ss[ka] > 0), cov * cov / (ss[ia] * ss[ka]), np.nan)
 
 
 def principal_angles_max(A: np.ndarray, B: np.ndarray) -> float:

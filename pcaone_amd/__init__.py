@@ -60,6 +60,8 @@ SIGNATURES = {
                                     C.POINTER(C.c_uint32)]),
     "pcaone_gpu_ld_r2": (C.c_int, [C.c_void_p, _I32, _I32, C.c_uint32, _D, C.c_uint64, C.POINTER(C.c_uint64)]),
     "pcaone_gpu_ld_usv": (C.c_int, [C.c_void_p, _D, C.c_uint32]),
+    "pcaone_gpu_ld_r2_geno": (C.c_int, [C.c_void_p, _I32, _I32, C.c_uint32, _D, C.c_uint64, C.POINTER(C.c_uint64),
+                                        _I32, C.POINTER(C.c_int)]),
     "pcaone_write_eigs": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, _D, _D, _D, C.c_int]),
     "pcaone_gpu_load_gl": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]),
     "pcaone_gpu_gl_em_maf": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, C.POINTER(C.c_int)]),
@@ -399,6 +401,23 @@ class Gpu:
         self._check(self._L.pcaone_gpu_ld_r2(self.h, _p(ws, C.c_int32), _p(we, C.c_int32), len(ws),
                                               _p(out, C.c_double), n.value, C.byref(n)))
         return out
+
+    def ld_r2_geno(self, ws: np.ndarray, we: np.ndarray, sums: bool = False, config: int = -1):
+        """r^2 of the window pairs straight from the loaded PLINK genotypes
+        (the --ld --ld-stats 1 route), window after window; returns (r2,
+        integer sums (npairs, 4) or None, tile configuration used)."""
+        ws = np.ascontiguousarray(ws, dtype=np.int32)
+        we = np.ascontiguousarray(we, dtype=np.int32)
+        n = C.c_uint64(0)
+        self._check(self._L.pcaone_gpu_ld_r2_geno(self.h, _p(ws, C.c_int32), _p(we, C.c_int32), len(ws), None, 0,
+                                                   C.byref(n), None, None))
+        out = np.empty(n.value)
+        sm = np.empty((n.value, 4), dtype=np.int32) if sums else None
+        cfg = C.c_int(config)
+        self._check(self._L.pcaone_gpu_ld_r2_geno(self.h, _p(ws, C.c_int32), _p(we, C.c_int32), len(ws),
+                                                   _p(out, C.c_double), n.value, C.byref(n),
+                                                   _p(sm, C.c_int32) if sums else None, C.byref(cfg)))
+        return out, sm, cfg.value
 
     def feature_means(self):
         mu = np.empty(self.M)

@@ -76,6 +76,7 @@ Engine::Engine(int device) : device_(device) {
   PCAONE_REQUIRE(device >= 0 && device < n, -1, "no such HIP device");
   PCAONE_HIP(hipSetDevice(device));
   PCAONE_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  PCAONE_HIP(hipDeviceGetAttribute(&ncu_, hipDeviceAttributeMultiprocessorCount, device));
 }
 
 Engine::~Engine() {
@@ -121,7 +122,7 @@ void Engine::load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps)
   }
   have_bed_ = true;
   src_ = SRC_BED;
-  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = false;
+  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = have_ldg_ = false;
 }
 
 // Replaces FileBin::read_all (src/FileBinary.cpp:21-30): nfeatures rows of
@@ -147,7 +148,7 @@ void Engine::load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfea
   counts_orig_.clear();
   have_bed_ = true;
   src_ = SRC_DENSE;
-  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = false;
+  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = have_ldg_ = false;
 }
 
 void Engine::feature_means(double* mu) const {

@@ -97,6 +97,11 @@ class Engine {
   uint64_t ld_r2(const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2, uint64_t capacity);
   // --USV on PLINK input: the LD matrix becomes (I - U U^T)(BED2GENO - F)
   void ld_usv(const double* U, uint32_t k);
+  // LD r^2 on the loaded PLINK genotypes as the --ld --ld-stats 1 route gives
+  // it, from exact integer sums (int8 MFMA); config < 0 autotunes the tile
+  // shape (the choice is written back)
+  uint64_t ld_r2_geno(const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2, uint64_t capacity,
+                      int32_t* sums, int* config);
   void allele_freq(double* F, int64_t* counts) const;
   void feature_means(double* mu) const;
 
@@ -180,6 +185,15 @@ class Engine {
   // --USV LD input: (I - U U^T) G, [SNP][sample] FP64, original SNP order
   bool have_resid_ = false;
   DevBuf<double> resid_;
+  // integer LD on the packed genotypes: identity SNP-major copy and per-SNP
+  // terms, built on the first call after a load; tuned tile shape per
+  // (window-size class)
+  bool have_ldg_ = false;
+  uint32_t ldg_row_bytes_ = 0;
+  DevBuf<uint8_t> ldg_bed_;
+  DevBuf<double> ldg_F_, ldg_mun_, ldg_ss_;
+  std::map<int, int> ldg_tuned_;
+  int ncu_ = 0;
 
   // ---- layout (depends on the permutation and the windowing) -------------
   bool have_op_ = false;

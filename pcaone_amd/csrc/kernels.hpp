@@ -257,6 +257,32 @@ struct LdTiles {
   double* r2;              // window-ordered pair values
 };
 void launch_ld_tiles(const LdTiles& t, uint32_t ntiles, hipStream_t s);
+// ---- LD r^2 from packed genotypes with int8 MFMA (kernels_ld_geno.hip) -----
+// Band tiles (row block x column block of SNPs) of the window pairs; r^2 of
+// pair (a, b) lands at off[w] + b - a - 1 (w the window led by a), the four
+// integer sums (S_yy, S_yw, S_wy, S_ww) at the same index of `sums` when given.
+struct LdGeno {
+  const uint8_t* bed;        // [nsnps][row_bytes] SNP-major packed codes, padding codes 01
+  uint32_t row_bytes;        // multiple of 64
+  uint32_t nsnps;
+  const int2* tiles;         // (row block, column block) in units of the tile shape
+  uint32_t ntiles;
+  const double* F;           // allele frequency
+  const double* mun;         // sum y / 2 - F n: N times the mean of the centred genotype
+  const double* ss;          // sum of squares of the re-centred genotype
+  double inv_n;              // 1 / N
+  const int32_t* win_of_row; // lead SNP -> window, or -1
+  const int32_t* we;         // window sizes (lead included)
+  const int64_t* off;        // first output of each window
+  double* r2;
+  int32_t* sums;             // optional, [npairs][4]
+};
+int ld_geno_nconfigs();
+void ld_geno_tile(int config, int* tm, int* tn);
+// persistent grid for a tile list on ncu compute units (a multiple of 8)
+uint32_t ld_geno_grid(int config, uint32_t ntiles, uint32_t ncu);
+void launch_ld_geno(const LdGeno& g, int config, uint32_t grid, hipStream_t s);
+
 // --USV on PLINK input: R = G - U (U^T G), G = BED2GENO - F (missing 0), from
 // the packed codes raw (m rows of ceil(n / 4) bytes), F (m), U ([n][k]); W
 // ([m][k]) receives U^T G, R ([m][n], FP64) the adjusted genotypes.

@@ -2,6 +2,7 @@
 // PLINK input with --USV, src/LD.cpp:450-470): windows from positions on the
 // host, the band of the Gram on the device.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "engine.hpp"
@@ -133,6 +134,173 @@ uint64_t Engine::ld_r2(const int32_t* ws, const int32_t* we, uint32_t nwin, doub
   t.r2 = out.p;
   timed("ld_tile", [&] { launch_ld_tiles(t, (uint32_t)tiles.size(), stream_); });
   PCAONE_HIP(hipMemcpyAsync(r2, out.p, total * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+  return total;
+}
+
+// Window pairs -> output offsets and the lead-SNP map; returns the pair count.
+static uint64_t window_offsets(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m,
+                               std::vector<int64_t>& off, std::vector<int32_t>& win_of_row) {
+  off.assign(nwin, 0);
+  win_of_row.assign(m, -1);
+  uint64_t total = 0;
+  for (uint32_t w = 0; w < nwin; ++w) {
+    PCAONE_REQUIRE(ws[w] >= 0 && (uint32_t)ws[w] < m && we[w] >= 1 && (uint64_t)ws[w] + we[w] <= m, -1,
+                   "ld_r2: window out of range");
+    PCAONE_REQUIRE(w == 0 || ws[w] > ws[w - 1], -1, "ld_r2: lead SNPs must increase");
+    off[w] = (int64_t)total;
+    total += (uint64_t)(we[w] - 1);
+    win_of_row[ws[w]] = (int32_t)w;
+  }
+  return total;
+}
+
+// Band tiles of the windows [0, nwin): row block I (tm SNPs) against the
+// column blocks (tn SNPs) from its own first SNP to the last SNP its windows
+// reach.
+static std::vector<int2> band_tiles(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m, int tm,
+                                    int tn) {
+  const uint32_t nb = (uint32_t)ceil_div(m, (uint64_t)tm);
+  std::vector<int64_t> last(nb, -1);
+  for (uint32_t w = 0; w < nwin; ++w)
+    if (we[w] > 1) {
+      const uint32_t I = (uint32_t)ws[w] / tm;
+      last[I] = std::max<int64_t>(last[I], (int64_t)ws[w] + we[w] - 1);
+    }
+  std::vector<int2> tiles;
+  for (uint32_t I = 0; I < nb; ++I)
+    if (last[I] >= 0)
+      for (int64_t J = (int64_t)I * tm / tn; J <= last[I] / tn; ++J) tiles.push_back(int2{(int)I, (int)J});
+  return tiles;
+}
+
+// ld_r2_big (src/LD.cpp:419-442) on the matrix Data::write_residuals writes
+// for PLINK input with --ld --ld-stats 1 (src/Data.cpp:196-221): G = BED2GENO
+// - F (missing 0, not standardised), each SNP re-centred by its mean.  The r^2
+// of each window pair is computed from the exact integer sums of the packed
+// genotypes (int8 MFMA) and the per-SNP terms below, in FP64; the reference
+// computes the same quantity on a float32 copy of G.
+uint64_t Engine::ld_r2_geno(const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2, uint64_t capacity,
+                            int32_t* sums, int* config) {
+  PCAONE_REQUIRE(have_bed_ && src_ == SRC_BED && !gl_mode_, -3, "ld_r2_geno: load PLINK genotypes first");
+  PCAONE_REQUIRE(N_ >= 2, -1, "ld_r2_geno: needs at least two samples");
+  PCAONE_REQUIRE(ws && we, -1, "ld_r2_geno: null windows");
+  PCAONE_HIP(hipSetDevice(device_));
+  std::vector<int64_t> off;
+  std::vector<int32_t> win_of_row;
+  const uint64_t total = window_offsets(ws, we, nwin, M_, off, win_of_row);
+  if (!r2) return total;
+  PCAONE_REQUIRE(capacity >= total, -1, "ld_r2_geno: output too small");
+  if (total == 0) return 0;
+  if (!have_ldg_) {
+    // identity SNP-major layout (64-byte rows, padding codes missing) and the
+    // per-SNP terms from the exact code counts of load_bed
+    ldg_row_bytes_ = (uint32_t)round_up((N_ + 3) >> 2, 64);
+    ldg_bed_.reserve((uint64_t)ldg_row_bytes_ * M_);
+    timed("ld_geno_layout",
+          [&] { launch_bed_to_snpmajor(raw_.p, N_, M_, nullptr, ldg_bed_.p, ldg_row_bytes_, stream_); });
+    std::vector<double> mun(M_), ss(M_);
+    for (uint32_t i = 0; i < M_; ++i) {
+      const int64_t n00 = counts_orig_[4 * (size_t)i], n10 = counts_orig_[4 * (size_t)i + 2],
+                    n11 = counts_orig_[4 * (size_t)i + 3];
+      const double F = F_orig_[i], sy = (double)(2 * n00 + n10), syy = (double)(4 * n00 + n10),
+                   n = (double)(n00 + n10 + n11);
+      mun[i] = 0.5 * sy - F * n;
+      ss[i] = 0.25 * syy - F * sy + F * F * n - mun[i] * mun[i] / (double)N_;
+    }
+    ldg_F_.reserve(M_);
+    ldg_mun_.reserve(M_);
+    ldg_ss_.reserve(M_);
+    PCAONE_HIP(hipMemcpyAsync(ldg_F_.p, F_orig_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+    PCAONE_HIP(hipMemcpyAsync(ldg_mun_.p, mun.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+    PCAONE_HIP(hipMemcpyAsync(ldg_ss_.p, ss.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+    sync();
+    have_ldg_ = true;
+  }
+  DevBuf<int32_t> dwin, dwe;
+  DevBuf<int64_t> doff;
+  DevBuf<double> out;
+  DevBuf<int32_t> dsums;
+  dwin.reserve(M_);
+  dwe.reserve(nwin);
+  doff.reserve(nwin);
+  out.reserve(total);
+  if (sums) dsums.reserve(total * 4);
+  PCAONE_HIP(hipMemcpyAsync(dwin.p, win_of_row.data(), (size_t)M_ * 4, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(dwe.p, we, (size_t)nwin * 4, hipMemcpyHostToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(doff.p, off.data(), (size_t)nwin * 8, hipMemcpyHostToDevice, stream_));
+  LdGeno g{};
+  g.bed = ldg_bed_.p;
+  g.row_bytes = ldg_row_bytes_;
+  g.nsnps = M_;
+  g.F = ldg_F_.p;
+  g.mun = ldg_mun_.p;
+  g.ss = ldg_ss_.p;
+  g.inv_n = 1.0 / (double)N_;
+  g.win_of_row = dwin.p;
+  g.we = dwe.p;
+  g.off = doff.p;
+  g.r2 = out.p;
+  g.sums = sums ? dsums.p : nullptr;
+  DevBuf<int2> dt;
+  uint32_t grid = 0;
+  // tile list of a configuration over the windows [0, nw), uploaded
+  auto prepare = [&](int cfg, uint32_t nw) {
+    int tm, tn;
+    ld_geno_tile(cfg, &tm, &tn);
+    const std::vector<int2> tiles = band_tiles(ws, we, nw, M_, tm, tn);
+    dt.reserve(std::max<size_t>(tiles.size(), 1));
+    PCAONE_HIP(hipMemcpyAsync(dt.p, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice, stream_));
+    g.tiles = dt.p;
+    g.ntiles = (uint32_t)tiles.size();
+    grid = ld_geno_grid(cfg, g.ntiles, (uint32_t)ncu_);
+    sync();  // `tiles` goes out of scope
+  };
+  int cfg = config ? *config : -1;
+  PCAONE_REQUIRE(cfg < ld_geno_nconfigs(), -1, "ld_r2_geno: no such tile configuration");
+  if (cfg < 0) {
+    // tile shape by window-size class (log2 of the mean pairs per window):
+    // every candidate timed on a prefix of the windows holding about 4M pairs
+    const int cls = (int)std::floor(std::log2(std::max(1.0, (double)total / nwin)));
+    auto it = ldg_tuned_.find(cls);
+    if (it != ldg_tuned_.end()) {
+      cfg = it->second;
+    } else {
+      uint32_t nw = nwin;
+      for (uint32_t w = 0; w < nwin; ++w)
+        if ((uint64_t)off[w] >= (4ull << 20)) {
+          nw = w;
+          break;
+        }
+      float best = 0.f;
+      hipEvent_t e0, e1;
+      PCAONE_HIP(hipEventCreate(&e0));
+      PCAONE_HIP(hipEventCreate(&e1));
+      for (int c = 0; c < ld_geno_nconfigs(); ++c) {
+        prepare(c, nw);
+        launch_ld_geno(g, c, grid, stream_);  // code load and caches
+        PCAONE_HIP(hipEventRecord(e0, stream_));
+        launch_ld_geno(g, c, grid, stream_);
+        PCAONE_HIP(hipGetLastError());
+        PCAONE_HIP(hipEventRecord(e1, stream_));
+        PCAONE_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PCAONE_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (cfg < 0 || ms < best) {
+          best = ms;
+          cfg = c;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      ldg_tuned_[cls] = cfg;
+    }
+  }
+  if (config) *config = cfg;
+  prepare(cfg, nwin);
+  timed("ld_geno", [&] { launch_ld_geno(g, cfg, grid, stream_); });
+  PCAONE_HIP(hipMemcpyAsync(r2, out.p, total * 8, hipMemcpyDeviceToHost, stream_));
+  if (sums) PCAONE_HIP(hipMemcpyAsync(sums, dsums.p, total * 16, hipMemcpyDeviceToHost, stream_));
   sync();
   return total;
 }

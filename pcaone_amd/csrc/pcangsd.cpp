@@ -40,7 +40,7 @@ void Engine::load_gl(const float* gl, uint32_t nsamples, uint32_t nsites, int pe
   have_bed_ = true;
   src_ = SRC_DENSE;
   gl_mode_ = true;
-  have_glF_ = gl_bed_ = have_resid_ = false;
+  have_glF_ = gl_bed_ = have_resid_ = have_ldg_ = false;
 }
 
 // --pcangsd on in-core PLINK input (src/FilePlink.cpp:68, 89-114;
@@ -64,7 +64,7 @@ void Engine::gl_from_bed() {
   counts_orig_.clear();
   src_ = SRC_DENSE;
   gl_mode_ = have_glF_ = gl_bed_ = true;
-  have_resid_ = false;
+  have_resid_ = have_ldg_ = false;
 }
 
 // emMAF_with_GL (src/Utils.cpp:501-531) from F = 0.25 (src/FileBeagle.cpp:127):

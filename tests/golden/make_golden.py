@@ -158,6 +158,40 @@ def run_ld_usv(tmp: Path, name: str, seed, n, per_chr, ld_bp, k):
                 r2=r2)
 
 
+# LD on PLINK input through the reference's two steps: a PCA run with --ld
+# --ld-stats 1 writes the centred genotypes (.residuals, .mbim), then -B
+# .residuals --print-r2.  name -> (seed, nsamples, SNPs per chromosome, ld_bp)
+LD_STATS1 = {"ld_stats1_bp3000": (61, 203, (420, 380, 500), 3000)}
+
+
+def run_ld_stats1(tmp: Path, name: str, seed, n, per_chr, ld_bp):
+    """Genotypes with 2 % missing (N % 4 == 3), one monomorphic SNP (its r^2 is
+    NaN: sd = 0), sorted positions with 1..400 bp gaps and one duplicated
+    position on len(per_chr) chromosomes."""
+    m = int(sum(per_chr))
+    pay = pa.sim_bed(seed, n, m, 3, 0.1, 0.05, 0.5, 0.02).reshape(m, -1).copy()
+    pay[25, :] = 0x00  # every sample hom A1
+    pay = pay.reshape(-1)
+    rng = np.random.default_rng(seed)
+    chrom = np.repeat(np.arange(1, len(per_chr) + 1), per_chr)
+    pos = np.concatenate([np.cumsum(rng.integers(1, 400, c)) for c in per_chr]).astype(np.int64)
+    pos[40] = pos[39]
+    pa.write_plink(tmp / name, pay, n, m, chrom, pos)
+    res = [str(REF), "-b", name, "--svd", "1", "-k", "2", "-p", "3", "--ld", "--ld-stats", "1", "-o",
+           name + "_res", "-n", "1", "-v", "0"]
+    subprocess.run(res, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    ld = [str(REF), "-B", name + "_res.residuals", "--match-bim", name + "_res.mbim", "--print-r2", "--ld-bp",
+          str(ld_bp), "-o", name, "-n", "1", "-v", "0"]
+    subprocess.run(ld, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    raw = np.fromfile(tmp / (name + "_res.residuals"), dtype=np.uint32, count=2)
+    resid = np.fromfile(tmp / (name + "_res.residuals"), dtype=np.float32, offset=8).reshape(int(raw[0]),
+                                                                                           int(raw[1]))
+    ia, ib, r2 = read_ld_gz(tmp / (name + ".ld.gz"))
+    chr_end = np.cumsum(per_chr) - 1
+    return dict(payload=pay, nsamples=n, nsnps=m, pos=pos, chr_end=chr_end, ld_bp=ld_bp, resid=resid, snp_a=ia,
+                snp_b=ib, r2=r2)
+
+
 def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
     cmd = [str(REF), flag, prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v", "0"]
     subprocess.run(cmd, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
@@ -224,6 +258,12 @@ def main():
             res = run_ld_usv(tmp, name, seed, n, per_chr, ld_bp, k)
             np.savez_compressed(OUT / f"{name}.npz", **res)
             print(name, len(res["r2"]), "pairs")
+        for name, (seed, n, per_chr, ld_bp) in LD_STATS1.items():
+            if not want(name):
+                continue
+            res = run_ld_stats1(tmp, name, seed, n, per_chr, ld_bp)
+            np.savez_compressed(OUT / f"{name}.npz", **res)
+            print(name, len(res["r2"]), "pairs", int(np.isnan(res["r2"]).sum()), "NaN")
         for name, (seed, n, m, base) in DENSE.items():
             D = pa.sim_counts(seed, n, m, base=base)
             pa.write_bin(tmp / (name + ".bin"), D)

@@ -10,6 +10,7 @@ import pytest
 
 import pcaone_amd as pa
 import pcaone_oracle as O
+from _cases import geno_sums_fast
 from conftest import GOLDEN, load_golden
 
 pytestmark = pytest.mark.gpu
@@ -94,3 +95,94 @@ def test_ld_r2_errors(gpu):
         gpu.ld_r2(np.array([3, 2]), np.array([2, 2]))  # leads must increase
     with pytest.raises(pa.GpuError):
         gpu.ld_r2(np.array([8]), np.array([5]))  # window past the last SNP
+
+
+# ---- LD straight from the packed genotypes (the --ld --ld-stats 1 route) ----
+def _ld_geno_case():
+    z = load_golden(GOLDEN / "ld_stats1_bp3000.npz")
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    codes = O.decode_codes(z["payload"], n, m)
+    ws, we = pa.ld_windows(z["pos"], z["chr_end"], int(z["ld_bp"]))
+    return z, n, m, codes, ws, we
+
+
+@pytest.mark.parametrize("config", list(range(5)) + [-1])
+def test_ld_geno_matches_reference_cli(gpu, config):
+    """Every tile configuration (and the autotuner): integer sums bit-exact
+    against the oracle's per-pair brute force, r^2 equal to the oracle's FP64
+    closed form and within 1e-6 of the reference CLI's .ld.gz (6 printed
+    decimals on float32 residuals), NaN where the reference prints nan."""
+    z, n, m, codes, ws, we = _ld_geno_case()
+    gpu.load_bed(z["payload"], n, m)
+    r2, sums, used = gpu.ld_r2_geno(ws, we, sums=True, config=config)
+    assert used >= 0 and (config < 0 or used == config)
+    want = O.ld_geno_sums(codes, ws, we)
+    assert sums.shape == want.shape and np.array_equal(sums.astype(np.int64), want)
+    nan = np.isnan(z["r2"])
+    assert np.array_equal(np.isnan(r2), nan)
+    assert np.abs(r2[~nan] - z["r2"][~nan]).max() <= 1e-6
+    ro = O.ld_geno_r2(codes, ws, we, want)
+    assert np.abs(r2[~nan] - ro[~nan]).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,m,bp,miss", [(2, 70, 300, 0.0), (3, 129, 200, 0.2), (67, 500, 1000, 0.05),
+                                         (301, 1100, 20, 0.01), (1001, 700, 10 ** 6, 0.02)])
+def test_ld_geno_sums_bit_exact(gpu, n, m, bp, miss):
+    """N below one 64-sample step and N % 4 in {1, 2, 3}; SNP counts off the
+    32 / 64 / 128 tile edges; windows of one pair up to whole chromosomes;
+    missing rates 0 to 20 %; repeated positions."""
+    pay = pa.sim_bed(n * 7 + m, n, m, 2, 0.1, 0.05, 0.5, miss)
+    rng = np.random.default_rng(m)
+    chr_end = np.array(sorted({m // 3, m - 1}), dtype=np.int32)
+    gaps = rng.integers(0, 12, m)
+    gaps[chr_end] = np.maximum(gaps[chr_end], 1)  # a chromosome's last position is its own (src/LD.cpp:130)
+    pos = np.cumsum(gaps).astype(np.int32)
+    ws, we = pa.ld_windows(pos, chr_end, bp)
+    gpu.load_bed(pay, n, m)
+    codes = O.decode_codes(pay, n, m)
+    want = geno_sums_fast(codes, ws, we)
+    for cfg in range(5):
+        r2, sums, _ = gpu.ld_r2_geno(ws, we, sums=True, config=cfg)
+        assert np.array_equal(sums.astype(np.int64), want), cfg
+    ro = O.ld_geno_r2(codes, ws, we, want)
+    ok = ~np.isnan(ro)
+    assert np.array_equal(np.isnan(r2), ~ok)
+    if ok.any():
+        assert np.abs(r2[ok] - ro[ok]).max() < 1e-12
+
+
+def test_ld_geno_windows_200_to_3000(gpu):
+    """Uneven windows of about 200 to 2,500 SNPs (SNP density alternating 15x
+    every 2,500 SNPs along two chromosomes; 7.8M pairs) in one launch: sums
+    bit-exact, r^2 to the oracle."""
+    n, m = 256, 10000
+    pay = pa.sim_bed(77, n, m, 3, 0.1, 0.05, 0.5, 0.005)
+    rng = np.random.default_rng(5)
+    gaps = np.where((np.arange(m) // 2500) % 2 == 0, rng.integers(1, 20, m), rng.integers(60, 240, m))
+    pos = np.cumsum(gaps).astype(np.int32)
+    chr_end = np.array([m // 2 - 1, m - 1], dtype=np.int32)
+    ws, we = pa.ld_windows(pos, chr_end, 30000)
+    assert we.max() >= 2500 and np.percentile(we, 5) >= 190
+    gpu.load_bed(pay, n, m)
+    r2, sums, cfg = gpu.ld_r2_geno(ws, we, sums=True)
+    codes = O.decode_codes(pay, n, m)
+    want = geno_sums_fast(codes, ws, we)
+    assert np.array_equal(sums.astype(np.int64), want)
+    ro = O.ld_geno_r2(codes, ws, we, want)
+    ok = ~np.isnan(ro)
+    assert np.abs(r2[ok] - ro[ok]).max() < 1e-12
+    # r^2 is a squared correlation
+    assert np.all((r2[ok] >= 0) & (r2[ok] <= 1 + 1e-12))
+
+
+def test_ld_geno_errors(gpu):
+    X = np.ones((10, 5), dtype=np.float32)
+    gpu.load_dense(X)
+    with pytest.raises(pa.GpuError):
+        gpu.ld_r2_geno(np.array([0]), np.array([3]))  # no genotypes loaded
+    pay = pa.sim_bed(1, 20, 50, 1, 0.1, 0.05, 0.5, 0.0)
+    gpu.load_bed(pay, 20, 50)
+    with pytest.raises(pa.GpuError):
+        gpu.ld_r2_geno(np.array([3, 2]), np.array([2, 2]))  # leads must increase
+    with pytest.raises(pa.GpuError):
+        gpu.ld_r2_geno(np.array([0]), np.array([3]), config=99)  # no such tile shape

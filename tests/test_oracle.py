@@ -17,8 +17,9 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+import pcaone_amd as pa
 import pcaone_oracle as O
-from _cases import eig_relerr, oracle_matrix, oracle_run, parse_args
+from _cases import eig_relerr, geno_sums_fast, oracle_matrix, oracle_run, parse_args
 from conftest import ROOT, golden_cases, load_golden
 
 BED2GENO_SPEC = {0b00: 1.0, 0b10: 0.5, 0b11: 0.0}  # A1 dosage / 2; 01 = missing
@@ -340,3 +341,64 @@ def test_ld_r2_closed_forms():
     z = rng.standard_normal((2, 200000))
     r2 = O.ld_r2(np.stack([z[0], z[0] + z[1]]).astype(np.float32), np.array([0]), np.array([2]))
     assert abs(r2[0] - 0.5) < 0.01
+
+
+def _pairs(ws, we):
+    ia = np.concatenate([np.full(e - 1, i) for i, e in zip(ws, we) if e > 1])
+    ib = np.concatenate([np.arange(i + 1, i + e) for i, e in zip(ws, we) if e > 1])
+    return ia, ib
+
+
+def test_ld_geno_oracle_matches_reference():
+    """--ld --ld-stats 1 on PLINK input, then --print-r2 on its .residuals
+    (tests/golden/ld_stats1_bp3000.npz): the window pairs, the residual matrix
+    itself and the r^2 from the integer sums."""
+    z = load_golden(ROOT / "tests" / "golden" / "ld_stats1_bp3000.npz")
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    codes = O.decode_codes(z["payload"], n, m)
+    ws, we = O.ld_windows(z["pos"], z["chr_end"], int(z["ld_bp"]))
+    ia, ib = _pairs(ws, we)
+    assert np.array_equal(ia, z["snp_a"]) and np.array_equal(ib, z["snp_b"])
+    R = O.ld_stats1_matrix(codes)
+    assert np.abs(R - z["resid"].astype(np.float64)).max() < 1e-6  # the file holds float32
+    assert np.abs(R.astype(np.float32) - z["resid"]).max() < 1e-14
+    r2 = O.ld_geno_r2(codes, ws, we)
+    nan = np.isnan(z["r2"])
+    assert nan.any() and np.array_equal(np.isnan(r2), nan)  # the monomorphic SNP: sd = 0
+    # 6 printed decimals (5e-7) plus the reference's float32 residuals (|dr^2| <= 5e-7)
+    assert np.abs(r2[~nan] - z["r2"][~nan]).max() <= 1e-6
+    # planted mistakes: missing read as hom A2 (w = 1), F from all samples, no re-centring term
+    y = np.where(codes == 0, 2, np.where(codes == 2, 1, 0))
+    wrong = np.where(codes == 1, 3, codes).astype(np.uint8)  # missing -> code 11 (y 0, counted)
+    bad = O.ld_geno_r2(wrong, ws, we)
+    assert np.nanmax(np.abs(bad - z["r2"])) > 1e-3
+    Fall = y.sum(axis=1) / (2.0 * n)
+    Xall = np.where(codes == 1, 0.0, y / 2.0 - Fall[:, None])
+    assert np.nanmax(np.abs(O.ld_r2(Xall, ws, we, centre=False) - z["r2"])) > 1e-4
+
+
+def test_ld_geno_sums_brute_force():
+    """Integer sums two ways (per-pair dot products, and whole Gram matrices
+    of y and the mask), on N % 4 in {1, 2, 3} and heavy missingness."""
+    for n, m, miss in ((9, 40, 0.3), (34, 70, 0.05), (51, 33, 0.0)):
+        pay = pa.sim_bed(n * m, n, m, 2, 0.1, 0.05, 0.5, miss)
+        codes = O.decode_codes(pay, n, m)
+        pos = np.arange(m) * 10
+        ws, we = O.ld_windows(pos, np.array([m - 1]), 95)
+        s = O.ld_geno_sums(codes, ws, we)
+        y = np.where(codes == 0, 2, np.where(codes == 2, 1, 0)).astype(np.int64)
+        w = (codes != 1).astype(np.int64)
+        ia, ib = _pairs(ws, we)
+        want = np.stack([(y @ y.T)[ia, ib], (y @ w.T)[ia, ib], (w @ y.T)[ia, ib], (w @ w.T)[ia, ib]], axis=1)
+        assert np.array_equal(s, want)
+        assert np.array_equal(geno_sums_fast(codes, ws, we), s)
+        # tiny explicit loop for the first pair
+        i, k = ia[0], ib[0]
+        yy = sum(int(y[i, j]) * int(y[k, j]) for j in range(n))
+        ww = sum(int(codes[i, j] != 1) * int(codes[k, j] != 1) for j in range(n))
+        assert s[0, 0] == yy and s[0, 3] == ww
+        # the closed-form r^2 equals the FP64 residual-matrix route
+        r2 = O.ld_geno_r2(codes, ws, we, s)
+        ref = O.ld_r2(O.ld_stats1_matrix(codes), ws, we)
+        ok = ~np.isnan(ref)
+        assert np.array_equal(np.isnan(r2), np.isnan(ref)) and np.abs(r2[ok] - ref[ok]).max() < 1e-12
