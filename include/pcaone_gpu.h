@@ -210,6 +210,25 @@ int pcaone_gpu_ld_r2(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32
 int pcaone_gpu_ld_r2_geno(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2,
                           uint64_t capacity, uint64_t* npairs, int32_t* sums, int* tile_config);
 
+/* Allocation-free variant of pcaone_gpu_ld_r2_geno.  It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It allocates no device memory: the window metadata, the tile
+ * list and the r^2 / sums staging live in one fixed array of
+ * PCAONE_LD_SCRATCH_BYTES (32 MiB) per device, static device memory held from
+ * the library's load until the process ends.  A call whose windows need more
+ * (about 8 bytes per pair, 24 with sums, plus 4 per SNP from the first to the
+ * last lead) returns PCAONE_EINVAL: pass the windows in batches (leads
+ * increasing within a batch; results equal the full call's pairs of those
+ * windows).  Needs the handle's LD layout, built by one pcaone_gpu_ld_r2_geno
+ * call after the load.  tile_config < 0 takes the shape the handle tuned for
+ * this window-size class, or the 64 x 64 default; no timing runs.  All calls
+ * on one device share its scratch and each leaves its contents behind: calls
+ * that overlap on one device race. */
+#define PCAONE_LD_SCRATCH_BYTES (32ull << 20)
+int pcaone_gpu_ld_r2_geno_noalloc(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2,
+                                  uint64_t capacity, uint64_t* npairs, int32_t* sums, int tile_config);
+
 /* ---- IRAM: the Spectra op and its driver ----------------------------------
  * Prepares the single-column Gram mat-vec on the loaded matrix (identity
  * SNP order; standardize / ploidy as for the RSVD operator).  Re-initialises

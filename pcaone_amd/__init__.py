@@ -62,6 +62,8 @@ SIGNATURES = {
     "pcaone_gpu_ld_usv": (C.c_int, [C.c_void_p, _D, C.c_uint32]),
     "pcaone_gpu_ld_r2_geno": (C.c_int, [C.c_void_p, _I32, _I32, C.c_uint32, _D, C.c_uint64, C.POINTER(C.c_uint64),
                                         _I32, C.POINTER(C.c_int)]),
+    "pcaone_gpu_ld_r2_geno_noalloc": (C.c_int, [C.c_void_p, _I32, _I32, C.c_uint32, _D, C.c_uint64,
+                                                C.POINTER(C.c_uint64), _I32, C.c_int]),
     "pcaone_write_eigs": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, _D, _D, _D, C.c_int]),
     "pcaone_gpu_load_gl": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]),
     "pcaone_gpu_gl_em_maf": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, C.POINTER(C.c_int)]),
@@ -418,6 +420,21 @@ class Gpu:
                                                    _p(out, C.c_double), n.value, C.byref(n),
                                                    _p(sm, C.c_int32) if sums else None, C.byref(cfg)))
         return out, sm, cfg.value
+
+    def ld_r2_geno_noalloc(self, ws: np.ndarray, we: np.ndarray, sums: bool = False, config: int = -1):
+        """ld_r2_geno through the allocation-free entry point (the handle's LD
+        layout must exist); returns (r2, integer sums or None)."""
+        ws = np.ascontiguousarray(ws, dtype=np.int32)
+        we = np.ascontiguousarray(we, dtype=np.int32)
+        n = C.c_uint64(0)
+        self._check(self._L.pcaone_gpu_ld_r2_geno_noalloc(self.h, _p(ws, C.c_int32), _p(we, C.c_int32), len(ws),
+                                                           None, 0, C.byref(n), None, config))
+        out = np.empty(n.value)
+        sm = np.empty((n.value, 4), dtype=np.int32) if sums else None
+        self._check(self._L.pcaone_gpu_ld_r2_geno_noalloc(self.h, _p(ws, C.c_int32), _p(we, C.c_int32), len(ws),
+                                                           _p(out, C.c_double), n.value, C.byref(n),
+                                                           _p(sm, C.c_int32) if sums else None, config))
+        return out, sm
 
     def feature_means(self):
         mu = np.empty(self.M)

@@ -206,6 +206,15 @@ int pcaone_gpu_ld_r2_geno(pcaone_gpu* h, const int32_t* ws, const int32_t* we, u
   });
 }
 
+int pcaone_gpu_ld_r2_geno_noalloc(pcaone_gpu* h, const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2,
+                                  uint64_t capacity, uint64_t* npairs, int32_t* sums, int tile_config) {
+  static_assert(PCAONE_LD_SCRATCH_BYTES == pcaone::kLdScratchBytes, "scratch size of the header and the library");
+  return guarded(h, [&](pcaone::Engine& e) {
+    const uint64_t n = e.ld_r2_geno_noalloc(ws, we, nwin, r2, capacity, sums, tile_config);
+    if (npairs) *npairs = n;
+  });
+}
+
 int pcaone_gpu_gram_init(pcaone_gpu* h, int precision, int standardize, int ploidy) {
   return guarded(h, [&](pcaone::Engine& e) { e.gram_init(precision, standardize != 0, ploidy); });
 }

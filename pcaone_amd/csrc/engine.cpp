@@ -77,6 +77,7 @@ Engine::Engine(int device) : device_(device) {
   PCAONE_HIP(hipSetDevice(device));
   PCAONE_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   PCAONE_HIP(hipDeviceGetAttribute(&ncu_, hipDeviceAttributeMultiprocessorCount, device));
+  (void)ld_scratch(device);  // the allocation-free entry point's scratch address, once per device
 }
 
 Engine::~Engine() {

@@ -73,6 +73,17 @@ struct SegBufs {
   const double* mval = nullptr;
 };
 
+// window pairs -> output offsets (from 0) and the lead-SNP map over [0, m);
+// returns the pair count
+uint64_t window_offsets(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m, std::vector<int64_t>& off,
+                        std::vector<int32_t>& win_of_row);
+// band tiles (row block of tm SNPs, column block of tn SNPs) of the windows
+std::vector<int2> band_tiles(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m, int tm, int tn);
+// the per-device scratch of the allocation-free LD entry point (address taken
+// once per device); kLdScratchBytes is its fixed size
+constexpr uint64_t kLdScratchBytes = 32ull << 20;
+void* ld_scratch(int device);
+
 // divide_pos_by_window (src/LD.cpp:125-139); returns the number of windows
 uint32_t ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end, uint32_t nchr, int32_t ld_bp,
                     int32_t* ws, int32_t* we);
@@ -102,6 +113,10 @@ class Engine {
   // shape (the choice is written back)
   uint64_t ld_r2_geno(const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2, uint64_t capacity,
                       int32_t* sums, int* config);
+  // the same without device allocation: metadata, tiles and result staging in
+  // the per-device scratch (needs the layout of an earlier ld_r2_geno)
+  uint64_t ld_r2_geno_noalloc(const int32_t* ws, const int32_t* we, uint32_t nwin, double* r2, uint64_t capacity,
+                              int32_t* sums, int config);
   void allele_freq(double* F, int64_t* counts) const;
   void feature_means(double* mu) const;
 

@@ -271,7 +271,8 @@ struct LdGeno {
   const double* mun;         // sum y / 2 - F n: N times the mean of the centred genotype
   const double* ss;          // sum of squares of the re-centred genotype
   double inv_n;              // 1 / N
-  const int32_t* win_of_row; // lead SNP -> window, or -1
+  const int32_t* win_of_row; // lead SNP win_lo + r -> window (r < win_rows), or -1
+  uint32_t win_lo, win_rows;
   const int32_t* we;         // window sizes (lead included)
   const int64_t* off;        // first output of each window
   double* r2;

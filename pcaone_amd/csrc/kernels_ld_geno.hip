@@ -162,8 +162,8 @@ __global__ __launch_bounds__(64 * WR * WC) void k_ld_geno(LdGeno g) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const uint32_t a = a0 + (uint32_t)(wr * S::FA + fa) * 8 + 4 * lg + v;
-          if (a >= b) continue;
-          const int32_t w = g.win_of_row[a];
+          if (a >= b || a < g.win_lo || a - g.win_lo >= g.win_rows) continue;
+          const int32_t w = g.win_of_row[a - g.win_lo];
           if (w < 0) continue;
           const uint32_t d = b - a;
           if (d >= (uint32_t)g.we[w]) continue;

@@ -139,7 +139,7 @@ uint64_t Engine::ld_r2(const int32_t* ws, const int32_t* we, uint32_t nwin, doub
 }
 
 // Window pairs -> output offsets and the lead-SNP map; returns the pair count.
-static uint64_t window_offsets(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m,
+uint64_t window_offsets(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m,
                                std::vector<int64_t>& off, std::vector<int32_t>& win_of_row) {
   off.assign(nwin, 0);
   win_of_row.assign(m, -1);
@@ -158,7 +158,7 @@ static uint64_t window_offsets(const int32_t* ws, const int32_t* we, uint32_t nw
 // Band tiles of the windows [0, nwin): row block I (tm SNPs) against the
 // column blocks (tn SNPs) from its own first SNP to the last SNP its windows
 // reach.
-static std::vector<int2> band_tiles(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m, int tm,
+std::vector<int2> band_tiles(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m, int tm,
                                     int tn) {
   const uint32_t nb = (uint32_t)ceil_div(m, (uint64_t)tm);
   std::vector<int64_t> last(nb, -1);
@@ -238,6 +238,8 @@ uint64_t Engine::ld_r2_geno(const int32_t* ws, const int32_t* we, uint32_t nwin,
   g.ss = ldg_ss_.p;
   g.inv_n = 1.0 / (double)N_;
   g.win_of_row = dwin.p;
+  g.win_lo = 0;
+  g.win_rows = M_;
   g.we = dwe.p;
   g.off = doff.p;
   g.r2 = out.p;
