@@ -1,23 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark of the PCAone randomized-SVD hot path on one MI355X.
 
-Workload (a "step" = one full winSVD solve, computeUSV from the first
+Headline workload (a "step" = one full winSVD solve, computeUSV from the first
 computeGandH to U, S, V on the host): 10,000 samples x 500,000 SNPs, 5
 Balding-Nichols populations (Fst 0.05, ancestral p ~ U(0.05, 0.5)), 0.5 %
 missing, packed .bed resident in HBM, k = 20, l = 40, 64 windows, tol 1e-4,
-at most 20 epochs.  Operator set-up (SNP shuffle, HBM layout, missing lists)
-is the analogue of the reference's data loading and is outside the timed
-region; so is data generation.
+at most 20 epochs.  Operator set-up (SNP shuffle, HBM layout, missing lists,
+Omega draw) is the analogue of the reference's data loading; it is outside
+the timed region and reported beside it (`init_s`).
 
 Arms: split-BF16 MFMA (the measured value), plain FP32 (the drop-in it is
-compared with) and FP64 (accuracy reference for both).  Prints ONE JSON line
-on stdout; progress goes to stderr.
+compared with) and FP64 (accuracy reference for both).
+
+The other workloads of the same hot path are measured in the same run and
+reported under "configs" (each with its own metric, roofline and CPU baseline):
+  C5  LD r^2 straight from packed genotypes, 20,000 x 1,000,000 SNPs over 22
+      chromosomes, 1 Mb windows (integer sums by int8 MFMA)
+  C4  dense scRNA-like 200,000 cells x 20,000 genes FP32: RSVD k = 50 (l = 100)
+      and IRAM top-50 (ncv = 101)
+  C3  PCAngsd EM on genotype likelihoods, 5,000 x 200,000 sites, k = 4
+Prints ONE JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
 import argparse
+import gzip
 import json
 import os
+import platform
 import re
 import subprocess
 import sys
@@ -33,12 +43,16 @@ sys.path.insert(0, str(ROOT))
 METRIC = "PCA solve s + genotype GB/s: split-BF16 MFMA vs FP32 GPU at eig relerr≤1e-5"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+INT8_PEAK_TOPS = 5000.0
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per launch of k_packed_gemm_bf16 on this workload: rocprofv3
+# HBM bytes per launch of k_packed_gemm_bf16 on the headline workload: rocprofv3
 # FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0) averaged over the 280 launches of one
 # split-BF16 solve, the factors from a 512 MiB copy kernel
 # (profiles/k_packed_gemm_bf16_c2_winsvd.md)
 TRAFFIC = 291.3e6
+REF = ROOT / "oracle" / "_ref" / "PCAone"
+# worker threads of the CPU baselines: the GPU host's CPU share for one GPU
+CPU_THREADS = 16
 
 
 def log(*a):
@@ -52,71 +66,96 @@ def principal_angle(A, B):
     return float(np.arccos(np.clip(s.min(), -1, 1)))
 
 
-def solve(g, pa, wl, prec):
+def cpu_facts():
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count()
+    return {"threads_used": CPU_THREADS, "os_cpu_count": os.cpu_count(), "affinity": aff, "model": model}
+
+
+def run_ref(args, cwd):
+    """The reference CLI (built from its sources by oracle/ref.mk); returns
+    (wall s, solve s = wall - logged reading time, log text) or None."""
+    if not REF.exists():
+        return None
+    out = None
+    for i, a in enumerate(args):
+        if a == "-o":
+            out = args[i + 1]
+    t0 = time.perf_counter()
+    r = subprocess.run([str(REF), *args], cwd=cwd, capture_output=True, text=True)
+    wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        log("reference CLI failed:", " ".join(args), r.stderr[-400:])
+        return None
+    text = (Path(cwd) / (out + ".log")).read_text() if out and (Path(cwd) / (out + ".log")).exists() else ""
+    rd = re.search(r"total elapsed reading time:\s*([0-9.eE+-]+)", text)
+    tot = re.search(r"total elapsed wall time:\s*([0-9.eE+-]+)", text)
+    total_s = float(tot.group(1)) if tot else wall
+    read_s = float(rd.group(1)) if rd else 0.0
+    return wall, max(total_s - read_s, 1e-9), text
+
+
+# ---------------------------------------------------------------------------
+# C2: winSVD on packed genotypes (headline)
+# ---------------------------------------------------------------------------
+def c2_solve(g, pa, wl, prec):
+    t0 = time.perf_counter()
     g.rsvd_init(wl["k"], wl["os"], variant=pa.RSVD_FANCY, bands=wl["bands"], precision=prec, seed=101)
     g.synchronize()
-    t0 = time.perf_counter()
+    t1 = time.perf_counter()
     U, S, V, ep, diffs = g.usv(wl["maxp"], wl["tol"])
     g.synchronize()
-    return time.perf_counter() - t0, U, S, V, ep
+    return time.perf_counter() - t1, t1 - t0, U, S, V, ep
 
 
-def cpu_baseline(pa, wl, msample, cores):
-    """Reference CLI (oracle/_ref/PCAone, built from the reference sources) on
-    the first `msample` SNPs of the same simulation, same winSVD settings."""
-    exe = ROOT / "oracle" / "_ref" / "PCAone"
+def c2_cpu_baseline(g, pa, wl, msample, cores):
+    """Reference CLI on the first `msample` SNPs of the same simulation, same
+    winSVD settings; then the GPU's split-BF16 solve of that same sample
+    against the reference's printed eigenvalues and PCs."""
     n = wl["N"]
     pay = pa.sim_bed(wl["seed"], n, msample, wl["npops"], wl["fst"], 0.05, 0.5, wl["miss"])
     with tempfile.TemporaryDirectory() as td:
-        pre = Path(td) / "sample"
-        pa.write_plink(pre, pay, n, msample)
-        if not exe.exists():
-            return None
-        cmd = [str(exe), "-b", str(pre), "--svd", "2", "-k", str(wl["k"]), "--oversamples", str(wl["os"]), "-w",
-               str(wl["bands"]), "-p", str(wl["maxp"]), "--tol-rsvd", str(wl["tol"]), "-n", str(cores), "-o",
-               str(Path(td) / "out"), "-v", "1"]
-        t0 = time.perf_counter()
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        wall = time.perf_counter() - t0
-        if r.returncode != 0:
-            log("reference CLI failed:", r.stderr[-500:])
-            return None
-        text = (Path(td) / "out.log").read_text()
-    rd = re.search(r"total elapsed reading time:\s*([0-9.eE+-]+)", text)
+        pa.write_plink(Path(td) / "sample", pay, n, msample)
+        res = run_ref(["-b", "sample", "--svd", "2", "-k", str(wl["k"]), "--oversamples", str(wl["os"]), "-w",
+                       str(wl["bands"]), "-p", str(wl["maxp"]), "--tol-rsvd", str(wl["tol"]), "-n", str(cores), "-o",
+                       "out", "-v", "1"], td)
+        if res is None:
+            return None, None
+        wall, solve_s, text = res
+        ev_ref = np.loadtxt(Path(td) / "out.eigvals", ndmin=1)
+        U_ref = np.loadtxt(Path(td) / "out.eigvecs", ndmin=2)
     ep = re.search(r"stops at epoch =\s*(\d+)", text)
-    tot = re.search(r"total elapsed wall time:\s*([0-9.eE+-]+)", text)
-    read_s = float(rd.group(1)) if rd else 0.0
-    total_s = float(tot.group(1)) if tot else wall
     epochs = int(ep.group(1)) if ep else 7
-    solve_s = max(total_s - read_s, 1e-9)
     gbytes = epochs * msample * ((n + 3) >> 2) / 1e9
-    return {"value": round(gbytes / solve_s, 4), "unit": "GB/s", "cores": cores, "kind": "reference",
-            "sample": f"reference PCAone CLI (-n {cores}) winSVD -k {wl['k']} --oversamples {wl['os']} "
-                      f"-w {wl['bands']} on the first {msample} SNPs x {n} samples of the same simulation; "
-                      f"solve {solve_s:.2f} s = wall {total_s:.2f} s - reading {read_s:.2f} s, {epochs} epochs"}
+    cpu = {"value": round(gbytes / solve_s, 4), "unit": "GB/s", "cores": cores, "kind": "reference",
+           "sample": f"reference PCAone CLI (-n {cores}) winSVD -k {wl['k']} --oversamples {wl['os']} "
+                     f"-w {wl['bands']} on the first {msample} SNPs x {n} samples of the same simulation; "
+                     f"solve {solve_s:.2f} s = wall {wall:.2f} s - reading; {epochs} epochs",
+           "cpu": cpu_facts()}
+    # the GPU on the same sample
+    g.load_bed(pay, n, msample)
+    g.rsvd_init(wl["k"], wl["os"], variant=pa.RSVD_FANCY, bands=wl["bands"], precision=pa.PREC_SPLIT_BF16, seed=101)
+    U, S, V, ep_gpu, _ = g.usv(wl["maxp"], wl["tol"])
+    ev = S ** 2 / msample * 2
+    npc = wl["npops"] - 1
+    acc = {"sample": f"first {msample} SNPs", "epochs_gpu": ep_gpu, "epochs_ref": epochs,
+           "eig_relerr_max_vs_printed": float(np.max(np.abs(ev - ev_ref) / ev_ref)),
+           f"max_angle_top{npc}_rad": principal_angle(U[:, :npc], U_ref[:, :npc]),
+           "note": "the reference prints 6 significant digits (5e-6 relative)"}
+    return cpu, acc
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--small", action="store_true", help="2,000 x 20,000 quick check")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-snps", type=int, default=40000)
-    ap.add_argument("--arms", default="bf16x2,fp32,fp64",
-                    help="comma list of arms to run (profiling runs use bf16x2 alone); the JSON line needs all three")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design "
-              "in DESIGN.md, not part of this build.", file=sys.stderr)
-        sys.exit(2)
-
-    import pcaone_amd as pa
-
-    if pa.device_count() < 1:
-        log("no HIP device visible")
-        sys.exit(1)
+def run_c2(g, pa, args):
     wl = dict(N=10000, M=500000, npops=5, fst=0.05, miss=0.005, k=20, os=20, bands=64, maxp=20, tol=1e-4,
               seed=2026)
     if args.small:
@@ -124,8 +163,7 @@ def main():
     n, m, l = wl["N"], wl["M"], wl["k"] + wl["os"]
     t0 = time.perf_counter()
     pay = pa.sim_bed(wl["seed"], n, m, wl["npops"], wl["fst"], 0.05, 0.5, wl["miss"])
-    log(f"generated {m} x {n} genotypes in {time.perf_counter() - t0:.1f} s")
-    g = pa.Gpu(0)
+    log(f"C2: generated {m} x {n} genotypes in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     g.load_bed(pay, n, m)  # host -> HBM over PCIe, plus the allele-frequency count
     g.synchronize()
@@ -140,14 +178,16 @@ def main():
         if name not in want:
             continue
         for _ in range(warm):
-            solve(g, pa, wl, prec)
-        ts, res = [], None
+            c2_solve(g, pa, wl, prec)
+        ts, inits, res = [], [], None
         for _ in range(steps):
-            t, U, S, V, ep = solve(g, pa, wl, prec)
+            t, ti, U, S, V, ep = c2_solve(g, pa, wl, prec)
             ts.append(t)
+            inits.append(ti)
             res = (U, S, V, ep)
-        arms[name] = dict(times=ts, U=res[0], S=res[1], V=res[2], epochs=res[3])
-        log(f"{name}: {np.mean(ts) * 1e3:.1f} ms/solve over {steps} solves, {res[3]} epochs")
+        arms[name] = dict(times=ts, init=inits, U=res[0], S=res[1], V=res[2], epochs=res[3])
+        log(f"C2 {name}: {np.mean(ts) * 1e3:.1f} ms/solve over {steps} solves (init {np.mean(inits) * 1e3:.1f} ms), "
+            f"{res[3]} epochs")
 
     roofline = None
     if "bf16x2" in want:
@@ -155,11 +195,9 @@ def main():
         # (one extra split-BF16 solve with an event pair around every launch)
         g.profiling(True)
         g.reset_stats()
-        tprof, _, _, _, ep_prof = solve(g, pa, wl, pa.PREC_SPLIT_BF16)
+        _, _, _, _, _, ep_prof = c2_solve(g, pa, wl, pa.PREC_SPLIT_BF16)
         stats = g.kernel_stats()
         g.profiling(False)
-        # the dominant kernel is the split-BF16 contraction k_packed_gemm_bf16, launched
-        # for both products (gemm_G: X^T Omega per window segment, gemm_H: X G)
         cnt = stats["gemm_G"][0] + stats["gemm_H"][0]
         ms = stats["gemm_G"][1] + stats["gemm_H"][1]
         flops = ep_prof * 4.0 * m * n * l  # algorithmic: two 2*M*N*l products per epoch
@@ -171,13 +209,11 @@ def main():
                     "algorithmic_flops_per_launch": flops / cnt,
                     "kernel_ms_per_solve": {k: round(v[1], 3) for k, v in sorted(stats.items())},
                     "kernel_launches_per_solve": {k: v[0] for k, v in sorted(stats.items())}}
-
-    if want != {"bf16x2", "fp32", "fp64"}:
-        if roofline:
-            log(f"timing solve: contraction {roofline['launches']} launches, avg {roofline['avg_launch_us']} us, "
+        if want != {"bf16x2", "fp32", "fp64"}:
+            log(f"C2 timing solve: contraction {cnt} launches, avg {roofline['avg_launch_us']} us, "
                 f"{roofline['achieved']} TFLOP/s")
-        log("partial run (--arms): no JSON line")
-        return
+    if want != {"bf16x2", "fp32", "fp64"}:
+        return None
     ref = arms["fp64"]
     acc = {}
     npc = wl["npops"] - 1
@@ -187,35 +223,331 @@ def main():
         acc[name] = {"eig_relerr_max": float(np.max(np.abs(ev - ev0) / ev0)),
                      f"max_angle_top{npc}_rad": principal_angle(a["U"][:, :npc], ref["U"][:, :npc]),
                      "epochs": a["epochs"]}
-
     bf = arms["bf16x2"]
-    t_bf = float(np.sum(bf["times"]))
-    gbs = bf["epochs"] * bytes_per_epoch * len(bf["times"]) / t_bf / 1e9
+    t_bf = float(np.mean(bf["times"]))
+    init_s = float(np.mean(bf["init"]))
+    gbs = bf["epochs"] * bytes_per_epoch / t_bf / 1e9
     t_fp32 = float(np.mean(arms["fp32"]["times"]))
-    cores = min(os.cpu_count() or 1, 16)
-    cpu = None if args.no_cpu else cpu_baseline(pa, wl, min(args.cpu_snps, m), cores)
-    out = {
+    cpu, acc_ref = (None, None) if args.no_cpu else c2_cpu_baseline(g, pa, wl, min(args.cpu_snps, m), CPU_THREADS)
+    if acc_ref:
+        acc["bf16x2_vs_reference_cli_on_cpu_sample"] = acc_ref
+    return {
         "metric": METRIC, "value": round(gbs, 3), "unit": "GB/s", "n_gpus": 1, "steps": len(bf["times"]),
-        "warmup": args.warmup, "ms_per_step": round(t_bf / len(bf["times"]) * 1e3, 3), "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": round(t_bf * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
         "config": {"workload": f"winSVD {n} samples x {m} SNPs packed .bed in HBM, {wl['npops']} Balding-Nichols "
                                f"pops Fst {wl['fst']}, {wl['miss'] * 100:g}% missing, k={wl['k']} l={l}, "
                                f"{wl['bands']} windows, tol {wl['tol']}",
                    "precision": "genotypes exact in BF16 x operand split into BF16 hi+lo, FP32 MFMA accumulation;"
                                 " FP64 CholeskyQR2, Householder signs, l x l eigensolve",
-                   "epochs": bf["epochs"], "solve_s": round(t_bf / len(bf["times"]), 5)},
+                   "epochs": bf["epochs"], "solve_s": round(t_bf, 5)},
+        "init_s": round(init_s, 5),
+        "value_incl_init_GBps": round(bf["epochs"] * bytes_per_epoch / (t_bf + init_s) / 1e9, 3),
         "fp32_arm": {"solve_s": round(t_fp32, 5), "value_GBps": round(arms["fp32"]["epochs"] * bytes_per_epoch /
                                                                       t_fp32 / 1e9, 3)},
         "fp64_arm": {"solve_s": round(float(np.mean(ref["times"])), 5)},
-        "speedup_vs_fp32": round(t_fp32 / (t_bf / len(bf["times"])), 3),
+        "speedup_vs_fp32": round(t_fp32 / t_bf, 3),
         "pcie_inclusive": {"load_bed_s": round(load_s, 4),
-                           "value_GBps": round(bf["epochs"] * bytes_per_epoch /
-                                               (t_bf / len(bf["times"]) + load_s) / 1e9, 3)},
+                           "value_GBps": round(bf["epochs"] * bytes_per_epoch / (t_bf + load_s) / 1e9, 3)},
         "accuracy_vs_fp64": acc,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# C5: LD r^2 straight from the packed genotypes
+# ---------------------------------------------------------------------------
+def c5_positions(M, nchr, seed):
+    """Positions as a Poisson process whose mean spacing changes every 2,000
+    SNPs (uniform in 333 .. 5,667 bp, mean 3 kb), so 1 Mb windows hold about
+    200 to 3,000 SNPs; nchr chromosomes of equal SNP counts."""
+    rng = np.random.default_rng(seed)
+    nreg = M // 2000 + 1
+    rate = rng.uniform(333, 5667, nreg)
+    gaps = rng.exponential(np.repeat(rate, 2000)[:M]).astype(np.int64) + 1
+    per = np.full(nchr, M // nchr)
+    per[-1] += M - per.sum()
+    pos = np.empty(M, np.int64)
+    s = 0
+    for c in range(nchr):
+        pos[s:s + per[c]] = np.cumsum(gaps[s:s + per[c]])
+        s += per[c]
+    return pos.astype(np.int32), (np.cumsum(per) - 1).astype(np.int32)
+
+
+def c5_cpu_baseline(pa, pay_rows, n, pos, chr_end, nsnp, cores):
+    """The reference's own two steps on the first `nsnp` SNPs of chromosome 1:
+    PCA with --ld --ld-stats 1 (writes the .residuals), then --print-r2 on them
+    (ld_r2_big: single-threaded, text and gzip per pair)."""
+    pay = np.ascontiguousarray(pay_rows[:nsnp]).reshape(-1)
+    with tempfile.TemporaryDirectory() as td:
+        pa.write_plink(Path(td) / "s", pay, n, nsnp, np.ones(nsnp, dtype=np.int64), pos[:nsnp].astype(np.int64))
+        a = run_ref(["-b", "s", "--svd", "1", "-k", "2", "-p", "3", "--ld", "--ld-stats", "1", "-o", "res", "-n",
+                     str(cores), "-v", "0"], td)
+        b = run_ref(["-B", "res.residuals", "--match-bim", "res.mbim", "--print-r2", "--ld-bp", "1000000", "-o", "ld",
+                     "-n", str(cores), "-v", "1"], td)
+        if a is None or b is None:
+            return None
+        with gzip.open(Path(td) / "ld.ld.gz", "rt") as f:
+            npairs = sum(1 for _ in f) - 1
+    wall, solve_s, _ = b
+    return {"value": round(npairs / solve_s, 1), "unit": "pairs/s", "cores": 1, "kind": "reference",
+            "sample": f"reference CLI --print-r2 on its own --ld --ld-stats 1 .residuals of the first {nsnp} SNPs of "
+                      f"chromosome 1 x {n} samples ({npairs} pairs, ld_r2_big single-threaded: {solve_s:.2f} s = wall "
+                      f"- reading); per-pair rate, which scales linearly to the whole genome",
+            "cpu": cpu_facts()}
+
+
+def run_c5(g, pa, args):
+    N, M = (20000, 1000000) if not args.small else (2000, 50000)
+    t0 = time.perf_counter()
+    pay = pa.sim_bed(2027, N, M, 5, 0.05, 0.05, 0.5, 0.005)
+    pos, chr_end = c5_positions(M, 22, 7)
+    ws, we = pa.ld_windows(pos, chr_end, 1000000)
+    P = int((we - 1).sum())
+    log(f"C5: generated {M} x {N} genotypes, {len(ws)} windows, {P} pairs in {time.perf_counter() - t0:.1f} s")
+    g.load_bed(pay, N, M)
+    g.synchronize()
+    r2, _, cfg = g.ld_r2_geno(ws, we)  # builds the layout, autotunes the tile shape
+    log(f"C5: tuned tile configuration {cfg}")
+    g.profiling(True)
+    walls, kms = [], []
+    for _ in range(max(1, args.steps)):
+        g.reset_stats()
+        t0 = time.perf_counter()
+        r2, _, _ = g.ld_r2_geno(ws, we, config=cfg)
+        walls.append(time.perf_counter() - t0)
+        kms.append(g.kernel_stats()["ld_geno"][1])
+    g.profiling(False)
+    ms = float(np.mean(kms))
+    wall = float(np.mean(walls))
+    ops = 4 * 2.0 * N * P  # the four integer Grams per pair
+    achieved = ops / (ms * 1e-3) / 1e12
+    ok = ~np.isnan(r2)
+    out = {"metric": "LD r^2 pairs/s (int8 MFMA integer sums, FP64 r^2), r^2 resident in HBM",
+           "value": round(P / (ms * 1e-3), 1), "unit": "pairs/s", "higher_is_better": True, "dtype": "int8",
+           "config": {"workload": f"LD r^2, {N} samples x {M} SNPs packed .bed, 22 chromosomes, positions a "
+                                  "Poisson process with mean spacing changing every 2,000 SNPs (333..5,667 bp), "
+                                  "--ld-bp 1000000, 0.5% missing, --ld-stats 1 semantics",
+                      "pairs": P, "windows": len(ws), "window_snps_p1_p50_p99": [int(x) for x in
+                                                                             np.percentile(we, [1, 50, 99])],
+                      "tile_config": cfg},
+           "ms_per_step": round(ms, 3), "steps": len(kms),
+           "pcie_inclusive": {"call_s": round(wall, 4), "pairs_per_s": round(P / wall, 1),
+                              "note": "the whole C-ABI call, r^2 copied to host memory (8 B per pair)"},
+           "int_ops_per_s": achieved * 1e12, "packed_GBps_read_once": round(M * ((N + 3) >> 2) / (ms * 1e-3) / 1e9, 2),
+           "r2_range": [float(r2[ok].min()), float(r2[ok].max())],
+           "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                        "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
+                        "kernel": "k_ld_geno (one persistent launch)",
+                        "algorithmic_ops_per_launch": ops, "avg_launch_us": round(ms * 1e3, 1)}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = c5_cpu_baseline(pa, pay.reshape(M, -1), N, pos, chr_end, 1000, CPU_THREADS)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# C4: dense scRNA-like matrix, RSVD k = 50 and IRAM top-50
+# ---------------------------------------------------------------------------
+def run_c4(g, pa, args):
+    ncells, ngenes = (200000, 20000) if not args.small else (20000, 2048)
+    t0 = time.perf_counter()
+    D = pa.sim_counts(44, ncells, ngenes, base=0.03, sdlog=1.5, phi=0.2, nthreads=CPU_THREADS)
+    zeros = float((D[:200] == 0).mean())
+    log(f"C4: generated {ngenes} x {ncells} counts ({zeros * 100:.1f}% zeros) in {time.perf_counter() - t0:.1f} s")
+    g.load_dense(D)
+    g.synchronize()
+    m, n = ngenes, ncells
+    res = {}
+    for name, prec in (("bf16x2", pa.PREC_SPLIT_BF16), ("fp32", pa.PREC_FP32), ("fp64", pa.PREC_FP64)):
+        steps = 1 if name == "fp64" else max(1, args.steps)
+        g.rsvd_init(50, 50, variant=pa.RSVD_FANCY, bands=64, precision=prec)
+        g.usv(20, 1e-4)  # warm-up
+        ts = []
+        for _ in range(steps):
+            g.rsvd_init(50, 50, variant=pa.RSVD_FANCY, bands=64, precision=prec)
+            g.synchronize()
+            t0 = time.perf_counter()
+            U, S, V, ep, _ = g.usv(20, 1e-4)
+            ts.append(time.perf_counter() - t0)
+        res[name] = (float(np.mean(ts)), U, S, ep)
+        log(f"C4 RSVD {name}: {res[name][0] * 1e3:.1f} ms, {ep} epochs")
+    # roofline of the split arm's dominant kernel (the dense bf16x3 contraction: HBM-bound)
+    g.rsvd_init(50, 50, variant=pa.RSVD_FANCY, bands=64, precision=pa.PREC_SPLIT_BF16)
+    g.profiling(True)
+    g.reset_stats()
+    _, _, _, ep, _ = g.usv(20, 1e-4)
+    st = g.kernel_stats()
+    g.profiling(False)
+    cms = st["gemm_G"][1] + st["gemm_H"][1]
+    cnt = st["gemm_G"][0] + st["gemm_H"][0]
+    gbytes = ep * 2 * 4.0 * m * n / 1e9  # each epoch reads the FP32 matrix once per product
+    t_bf, U, S, ep_bf = res["bf16x2"]
+    S64, U64 = res["fp64"][2], res["fp64"][1]
+    ev, ev0 = S ** 2, S64 ** 2
+    out = {"metric": "dense PCA solve s (RSVD k=50 split-BF16 3-product MFMA; IRAM top-50)", "value": round(t_bf, 4),
+           "unit": "s", "higher_is_better": False, "dtype": "bf16x3",
+           "config": {"workload": f"{ncells} cells x {ngenes} genes log1p(CPM) FP32 in HBM, centred per gene, "
+                                  f"NB(mean LogNormal(0,1.5) x 0.03, dispersion 0.2) counts, {zeros * 100:.1f}% zeros;"
+                                  " winSVD k=50 l=100 64 windows tol 1e-4", "epochs": ep_bf},
+           "fp32_arm_s": round(res["fp32"][0], 4), "fp64_arm_s": round(res["fp64"][0], 4),
+           "speedup_vs_fp32": round(res["fp32"][0] / t_bf, 3),
+           "accuracy_vs_fp64": {"eig_relerr_max_top10": float(np.max(np.abs(ev[:10] - ev0[:10]) / ev0[:10])),
+                                "eig_relerr_max_top50": float(np.max(np.abs(ev - ev0) / ev0)),
+                                "max_angle_top5_rad": principal_angle(U[:, :5], U64[:, :5])},
+           "roofline": {"bound": "hbm", "achieved": round(gbytes / (cms * 1e-3), 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(gbytes / (cms * 1e-3) / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "k_dense_gemm_bf16x3 (gemm_G + gemm_H)", "launches": cnt,
+                        "avg_launch_us": round(cms * 1e3 / cnt, 2),
+                        "kernel_ms_per_solve": {k: round(v[1], 3) for k, v in sorted(st.items())}}}
+    # IRAM top-50 (Spectra op on the GPU, restarted Lanczos on the host), FP64 arm
+    g.gram_init(pa.PREC_FP64)
+    t0 = time.perf_counter()
+    Ui, Si, Vi, nconv, nops = g.iram(50, ncv=101, maxit=1000, tol=1e-6)
+    t_iram = time.perf_counter() - t0
+    g.gram_init(pa.PREC_SPLIT_BF16)
+    t0 = time.perf_counter()
+    Ub, Sb, Vb, nconvb, nopsb = g.iram(50, ncv=101, maxit=1000, tol=1e-6)
+    t_iramb = time.perf_counter() - t0
+    log(f"C4 IRAM: fp64 {t_iram:.2f} s ({nops} ops), bf16x2 {t_iramb:.2f} s ({nopsb} ops)")
+    out["iram"] = {"fp64_s": round(t_iram, 3), "fp64_nops": nops, "bf16x2_s": round(t_iramb, 3), "bf16x2_nops": nopsb,
+                   "perform_op_GBps_fp64": round(nops * 2 * 4.0 * m * n / t_iram / 1e9, 1),
+                   "eig_relerr_top50_iram_vs_rsvd_fp64": float(np.max(np.abs(Si ** 2 - ev0) / Si ** 2)),
+                   "eig_relerr_top50_bf16x2_vs_fp64": float(np.max(np.abs(Sb ** 2 - Si ** 2) / Si ** 2))}
+    if not args.no_cpu:
+        out["cpu_baseline"] = c4_cpu_baseline(pa, D, CPU_THREADS)
+    return out
+
+
+def c4_cpu_baseline(pa, D, cores):
+    """Reference CLI RSVD k=50 (winSVD, -w 64) on the first 10,000 cells of the
+    same matrix (all genes), as .bin input: elements x epochs per second."""
+    ncell = min(10000, D.shape[1])
+    sub = np.ascontiguousarray(D[:, :ncell])
+    with tempfile.TemporaryDirectory() as td:
+        pa.write_bin(Path(td) / "d.bin", sub)
+        r = run_ref(["-B", "d.bin", "--svd", "2", "-k", "50", "-w", "64", "-p", "20", "--tol-rsvd", "1e-4", "-n",
+                     str(cores), "-o", "out", "-v", "1"], td)
+    if r is None:
+        return None
+    wall, solve_s, text = r
+    ep = re.search(r"stops at epoch =\s*(\d+)", text)
+    epochs = int(ep.group(1)) if ep else 7
+    gbytes = epochs * 2 * 4.0 * sub.size / 1e9
+    return {"value": round(gbytes / solve_s, 3), "unit": "GB/s (FP32 matrix bytes per product)", "cores": cores,
+            "kind": "reference",
+            "sample": f"reference CLI (-n {cores}) winSVD -k 50 -w 64 on {sub.shape[0]} genes x {ncell} cells of the "
+                      f"same matrix: solve {solve_s:.2f} s = wall {wall:.2f} s - reading, {epochs} epochs",
+            "cpu": cpu_facts()}
+
+
+# ---------------------------------------------------------------------------
+# C3: PCAngsd EM on genotype likelihoods
+# ---------------------------------------------------------------------------
+def run_c3(g, pa, args):
+    n, m = (5000, 200000) if not args.small else (1000, 20000)
+    t0 = time.perf_counter()
+    gl = pa.sim_gl(33, n, m, 4, 0.1, 0.05, 0.5, 2.0, 0.01, nthreads=CPU_THREADS)
+    log(f"C3: generated {m} x {n} likelihoods in {time.perf_counter() - t0:.1f} s")
+    g.load_gl(gl)
+    out = {}
+    res = {}
+    for name, prec in (("bf16x2", pa.PREC_SPLIT_BF16), ("fp32", pa.PREC_FP32), ("fp64", pa.PREC_FP64)):
+        g.profiling(name == "bf16x2")
+        g.reset_stats()
+        t0 = time.perf_counter()
+        F, it_maf = g.gl_em_maf(100, 1e-6)
+        g.rsvd_init(4, 10, variant=pa.RSVD_NORMAL, precision=prec)
+        U, S, V, it, diffs, Dc, Cm = g.pcangsd(20, 1e-4, 100, 1e-5, grm=False, dc=True)
+        g.synchronize()
+        res[name] = (time.perf_counter() - t0, S, U, it)
+        if name == "bf16x2":
+            st = g.kernel_stats()
+        log(f"C3 {name}: {res[name][0]:.2f} s, {it} EM iterations")
+    g.profiling(False)
+    t_bf, S, U, it = res["bf16x2"]
+    S64, U64 = res["fp64"][1], res["fp64"][2]
+    ev, ev0 = S ** 2, S64 ** 2
+    em = st.get("gl_E", (0, 0.0))
+    # the E update reads the two FP32 likelihoods and writes E (FP32, laid-out rows): 12 B per element
+    ebytes = em[0] * 12.0 * n * m / 1e9 if em[0] else 0.0
+    out = {"metric": "PCAngsd EM solve s (emMAF + EM iterations, split-BF16 3-product MFMA RSVD per iteration)",
+           "value": round(t_bf, 3), "unit": "s", "higher_is_better": False, "dtype": "bf16x3",
+           "config": {"workload": f"{n} samples x {m} sites, 4 pops Fst 0.1, depth Poisson(2), 1% error, FP32 GLs; "
+                                  "--svd 1 k=4 l=14, tol-em 1e-5, tol-maf 1e-6", "em_iterations": it},
+           "fp32_arm_s": round(res["fp32"][0], 3), "fp64_arm_s": round(res["fp64"][0], 3),
+           "accuracy_vs_fp64": {"eig_relerr_max": float(np.max(np.abs(ev - ev0) / ev0)),
+                                "max_angle_top3_rad": principal_angle(U[:, :3], U64[:, :3]),
+                                "em_iterations_fp64": res["fp64"][3]},
+           "kernel_ms": {k: round(v[1], 2) for k, v in sorted(st.items())}}
+    if em[0]:
+        out["roofline"] = {"bound": "hbm", "achieved": round(ebytes / (em[1] * 1e-3), 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(ebytes / (em[1] * 1e-3) / HBM_PEAK_GBS, 4), "traffic": None,
+                           "kernel": "k_gl_E (E update, FP32 likelihoods in, FP32 E out in two layouts)",
+                           "launches": em[0], "avg_launch_us": round(em[1] * 1e3 / em[0], 1)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = c3_cpu_baseline(pa, gl, CPU_THREADS)
+    return out
+
+
+def c3_cpu_baseline(pa, gl, cores):
+    """Reference CLI PCAngsd (-G BEAGLE, --svd 1 -k 4) on the first 400 sites of
+    the same likelihoods: site x sample x EM iterations per second."""
+    ms = min(400, gl.shape[0])
+    sub = gl[:ms]
+    with tempfile.TemporaryDirectory() as td:
+        pa.write_beagle(Path(td) / "g.beagle.gz", sub)
+        r = run_ref(["-G", "g.beagle.gz", "--svd", "1", "-k", "4", "-p", "20", "--tol-rsvd", "1e-4", "-n", str(cores),
+                     "-o", "out", "-v", "1"], td)
+    if r is None:
+        return None
+    wall, solve_s, text = r
+    its = len(re.findall(r"individual allele frequencies estimated", text)) or 1
+    rate = ms * gl.shape[1] * its / solve_s
+    return {"value": round(rate, 1), "unit": "site-samples x EM iterations / s", "cores": cores, "kind": "reference",
+            "sample": f"reference CLI (-n {cores}) -G --svd 1 -k 4 on the first {ms} sites x {gl.shape[1]} samples: "
+                      f"solve {solve_s:.2f} s = wall {wall:.2f} s - reading, {its} EM iterations",
+            "cpu": cpu_facts()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--small", action="store_true", help="small shapes for a quick check")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-snps", type=int, default=40000)
+    ap.add_argument("--arms", default="bf16x2,fp32,fp64",
+                    help="C2 arms to run (profiling runs use bf16x2 alone); the JSON line needs all three")
+    ap.add_argument("--configs", default="c2,c5,c4,c3", help="workloads to run; the JSON line needs c2")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design "
+              "in DESIGN.md, not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import pcaone_amd as pa
+
+    if pa.device_count() < 1:
+        log("no HIP device visible")
+        sys.exit(1)
+    g = pa.Gpu(0)
+    todo = args.configs.split(",")
+    head = run_c2(g, pa, args) if "c2" in todo else None
+    configs = {}
+    for name, fn in (("c5", run_c5), ("c4", run_c4), ("c3", run_c3)):
+        if name in todo:
+            configs[name.upper()] = fn(g, pa, args)
+            log(f"{name.upper()}: " + json.dumps({k: v for k, v in configs[name.upper()].items()
+                                                  if k in ("value", "unit", "roofline")}))
+    g.close()
+    if head is None:
+        log("partial run (--configs / --arms): no JSON line")
+        return
+    head["configs"] = configs
+    print(json.dumps(head), flush=True)
 
 
 if __name__ == "__main__":

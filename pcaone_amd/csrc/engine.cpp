@@ -324,6 +324,7 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   hpart_.reserve(hmax * ldp * partsz);
   // linear algebra scratch
   wide_ = l_ > 64;
+  robust_ = false;
   part_.reserve(std::max<uint64_t>({(uint64_t)1024 * l_ * l_, qr_part_doubles(l_), usv_part_doubles(l_),
                                     wide_ ? wide_part_doubles() : 0}));
   // reduced Grams between passes: tile layout (l <= 64) or full l x l (wide);
@@ -335,6 +336,8 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
     wq_.reserve((uint64_t)std::max(N_, M_) * l_);
     eigtmp_.reserve((uint64_t)l_ * l_);
   }
+  snapH1_.reserve((uint64_t)N_ * l_);
+  snapH2_.reserve((uint64_t)N_ * l_);
   bbt_.reserve((uint64_t)l_ * l_);
   for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &Rf_, &W_}) b->reserve((uint64_t)l_ * l_);
   sign_.reserve(l_);
@@ -698,7 +701,7 @@ void Engine::init_omega() {
 // fused into four passes that also write the contraction operand and the
 // FP32 gather table.
 void Engine::omega_refresh(bool from_two, double* zero_acc) {
-  if (wide_) {
+  if (wide_ || robust_) {
     omega_refresh_wide(from_two, zero_acc);
     return;
   }
@@ -741,9 +744,20 @@ void Engine::omega_refresh_wide(bool from_two, double* zero_acc) {
   const SegBufs b = op_bufs();
   const double* H1 = from_two ? H1_.p : H_.p;
   const double* H2 = from_two ? H2_.p : nullptr;
+  if (!wq_.p) wq_.reserve((uint64_t)std::max(N_, M_) * l_);  // robust mode at l <= 64
+  if (part_.n < wide_part_doubles()) part_.reserve(wide_part_doubles());
   timed("qr_gram_h", [&] { launch_wide_gram(H1, H2, H_.p, N_, l_, part_.p, qred_.p, stream_); });
+  const double* Hq = H_.p;  // the matrix orthogonalised (H itself stays for computeUSV)
+  if (robust_) {
+    hrep_.reserve((uint64_t)N_ * l_);
+    rmask_.reserve(l_);
+    rscale_.reserve(1);
+    timed("qr_repair", [&] { launch_wide_repair(H_.p, qred_.p, N_, l_, rmask_.p, rscale_.p, hrep_.p, stream_); });
+    timed("qr_gram_h", [&] { launch_wide_gram(hrep_.p, nullptr, nullptr, N_, l_, part_.p, qred_.p, stream_); });
+    Hq = hrep_.p;
+  }
   timed("qr_chol", [&] { launch_wide_chol(qred_.p, l_, R1i_.p, status_.p, stream_); });
-  WideApply a1{H_.p, N_, l_, R1i_.p, wq_.p, nullptr, nullptr};
+  WideApply a1{Hq, N_, l_, R1i_.p, wq_.p, nullptr, nullptr};
   timed("qr_apply", [&] { launch_wide_apply(a1, nullptr, stream_); });
   timed("qr_gram_q", [&] { launch_wide_gram(wq_.p, nullptr, nullptr, N_, l_, part_.p, qred_.p, stream_); });
   timed("qr_chol", [&] {
@@ -1092,14 +1106,14 @@ void Engine::rsvd_gandh(int pi, double* Gout, double* Hout, bool check) {
   }
 }
 
-void Engine::check_status() {
+void Engine::check_status(const char* context) {
   int st[2] = {0, 0};
   PCAONE_HIP(hipMemcpyAsync(st, status_.p, sizeof(st), hipMemcpyDeviceToHost, stream_));
   sync();
   if (st[0] || st[1]) {
     PCAONE_HIP(hipMemsetAsync(status_.p, 0, 2 * sizeof(int), stream_));
     throw Error(-4, "CholeskyQR breakdown (pivot " + std::to_string(st[0] ? st[0] : st[1]) +
-                        "): the sketch is numerically rank deficient");
+                        "): the sketch is numerically rank deficient" + (context ? context : ""));
   }
 }
 
@@ -1165,21 +1179,21 @@ void Engine::usv_step_wide() {
   });
 }
 
-void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int* epochs, double* diffs) {
-  PCAONE_REQUIRE(have_op_, -3, "computeUSV: operator not initialised");
-  PCAONE_REQUIRE(maxp >= 0, -1, "computeUSV: p must be >= 0");
-  PCAONE_HIP(hipSetDevice(device_));
+// The epochs of computeUSV (src/Halko.cpp:51-93); `done` is false when the
+// loop ran out of epochs (the last Ucur then sits in Upre).
+int Engine::usv_epochs(int maxp, double tol, double* diffs, bool& done) {
   const uint32_t k = (uint32_t)p_.k;
   int p = maxp, ep = 0;
   have_W_ = false;
   if (diffs)
     for (int i = 0; i <= maxp; ++i) diffs[i] = 0.0;
-  bool done = false;
+  done = false;
   for (int pi = 0; pi <= p; ++pi) {
     rsvd_gandh(pi, nullptr, nullptr, false);
     ++ep;
     usv_step();
-    check_status();
+    check_status(robust_ ? " even with the rank repair of the Omega refresh: the matrix has rank below k + oversamples"
+                         : nullptr);
     if (pi > 0) {
       const int nbN = gram_blocks(N_);
       timed("mev", [&] { launch_xty_partial(Ucur_.p, Upre_.p, N_, k, part_.p, nbN, stream_); });
@@ -1202,6 +1216,44 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
     } else {
       std::swap(Ucur_.p, Upre_.p);
     }
+  }
+  return ep;
+}
+
+void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int* epochs, double* diffs) {
+  PCAONE_REQUIRE(have_op_, -3, "computeUSV: operator not initialised");
+  PCAONE_REQUIRE(maxp >= 0, -1, "computeUSV: p must be >= 0");
+  PCAONE_HIP(hipSetDevice(device_));
+  const uint32_t k = (uint32_t)p_.k;
+  // the winSVD accumulators carry over between solves (the reference resets
+  // them only when 2^pi >= bands): keep them for a restart
+  const size_t hb = (size_t)N_ * l_ * 8;
+  PCAONE_HIP(hipMemcpyAsync(snapH1_.p, H1_.p, hb, hipMemcpyDeviceToDevice, stream_));
+  PCAONE_HIP(hipMemcpyAsync(snapH2_.p, H2_.p, hb, hipMemcpyDeviceToDevice, stream_));
+  bool done = false;
+  int ep = 0;
+  try {
+    ep = usv_epochs(maxp, tol, diffs, done);
+  } catch (const Error& e) {
+    if (e.code != -4 || robust_) throw;
+    // A partial H of lower rank than l (a window of fewer SNPs than l, or a
+    // rank-deficient matrix): Householder QR, as the reference uses, carries
+    // on with trailing columns set by rounding; CholeskyQR breaks down.  The
+    // solve restarts with the rank repair in every refresh, from the same
+    // state (Omega is redrawn at pi == 0; the EMU values of this solve are
+    // kept).
+    robust_ = true;
+    PCAONE_HIP(hipMemcpyAsync(H1_.p, snapH1_.p, hb, hipMemcpyDeviceToDevice, stream_));
+    PCAONE_HIP(hipMemcpyAsync(H2_.p, snapH2_.p, hb, hipMemcpyDeviceToDevice, stream_));
+    const bool upd = update_;
+    update_ = false;
+    try {
+      ep = usv_epochs(maxp, tol, diffs, done);
+    } catch (...) {
+      update_ = upd;
+      throw;
+    }
+    update_ = upd;
   }
   if (!done) std::swap(Ucur_.p, Upre_.p);  // the last Ucur sits in Upre after the swap
   // V = Q U_B[:, :k] = G (R^-1 U_B[:, :k]) (rows in laid-out order), S = sqrt(eig(B B^T))[:k]

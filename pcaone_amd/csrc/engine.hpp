@@ -153,11 +153,13 @@ class Engine {
   void draw_omega();
   void init_omega();
   void omega_refresh(bool from_two, double* zero_acc);
-  void omega_refresh_wide(bool from_two, double* zero_acc);  // l > 64
+  void omega_refresh_wide(bool from_two, double* zero_acc);  // l > 64, and every l in robust mode
   void run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route);
   void usv_step();
   void usv_step_wide();  // l > 64
-  void check_status();
+  // computeUSV's epochs; returns the number of computeGandH calls
+  int usv_epochs(int maxp, double tol, double* diffs, bool& done);
+  void check_status(const char* context = nullptr);
   void upload_scale();
   uint32_t rows_per_block_(int prec) const {
     return (src_ == SRC_DENSE || prec == PREC_SPLIT_BF16) ? 256u : 512u;
@@ -262,6 +264,14 @@ class Engine {
   bool wide_ = false;
   DevBuf<double> wq_;      // max(N, M) x l: Q1 of the current factorisation
   DevBuf<double> eigtmp_;  // l x l eigensolve scratch
+  // Robust mode (after a CholeskyQR breakdown of an Omega refresh, the solve
+  // restarts in it): every refresh marks the columns of H numerically in the
+  // span of the earlier ones and orthogonalises a copy of H with those columns
+  // replaced by pseudo-random ones.  The accumulators are kept as they were at
+  // the start of the solve, for the restart.
+  bool robust_ = false;
+  DevBuf<double> hrep_, snapH1_, snapH2_, rscale_;
+  DevBuf<int> rmask_;
   DevBuf<int> status_;
   // Gram mat-vec state (one operand column)
   bool have_mv_ = false;

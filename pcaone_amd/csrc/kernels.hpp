@@ -413,6 +413,13 @@ struct WideWrite {
   int kperm;
 };
 void launch_wide_write(const WideWrite& q, hipStream_t s);
+// Rank repair for the robust Omega refresh: from the Gram of the n x l A, mark
+// the columns whose Cholesky pivot is below 1e-12 of their squared norm
+// (numerically in the span of the earlier kept columns) and write out = A
+// with those columns replaced by deterministic pseudo-random normal columns
+// (RMS entry that of A); mask (l ints) and scale (1 double) are scratch
+void launch_wide_repair(const double* A, const double* gram, uint64_t n, uint32_t l, int* mask, double* scale,
+                        double* out, hipStream_t s);
 // Jacobi eigensolve of the full symmetric l x l A (as launch_sym_eig); tmp: l x l scratch
 void launch_sym_eig_wide(const double* A, uint32_t l, double* evals, double* W, double* sinv, uint32_t kc,
                          const double* W0, double* tmp, hipStream_t s);

@@ -139,6 +139,9 @@ void launch_rows_to_cols(const double* A, uint64_t n, uint32_t kc, const int32_t
   hipLaunchKernelGGL(k_rows_to_cols, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, A, n, kc, perm, out);
 }
 
+// sort key of an eigenvalue: NaN below every number
+__device__ inline double eig_key(double x) { return x == x ? x : -__builtin_huge_val(); }
+
 // Cyclic Jacobi with round-robin pair ordering: each round applies n/2
 // disjoint rotations at once (Numerical-Recipes angle).  A thread owns 2 x 2
 // blocks {p1, q1} x {p2, q2} of the matrix (rows, then columns rotated from
@@ -371,11 +374,16 @@ __global__ __launch_bounds__(kEigThreads) void k_sym_eig(const double* __restric
     }
   }
   // descending order of the eigenvalues (ties by index): rank of each diagonal entry
+  // (a total order even when a breakdown upstream left NaNs on the diagonal:
+  // NaN ranks last, ties by index, so order is a permutation of 0 .. L-1;
+  // every entry holds a valid index before the ranks land)
+  if (t < L) order[t] = t;
+  __syncthreads();
   if (t < L) {
-    const double di = a[t * LD + t];
+    const double di = eig_key(a[t * LD + t]);
     int rank = 0;
     for (int j = 0; j < L; ++j) {
-      const double dj = a[j * LD + j];
+      const double dj = eig_key(a[j * LD + j]);
       rank += (dj > di) || (dj == di && j < t);
     }
     order[rank] = t;

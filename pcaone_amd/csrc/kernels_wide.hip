@@ -355,6 +355,64 @@ __global__ __launch_bounds__(kSmallThreads) void k_wide_r2(const double* __restr
 }
 
 // ---------------------------------------------------------------------------
+// Rank repair of a Gram (robust refresh): column k is (numerically) in the
+// span of the columns before it when its Cholesky pivot falls below tau times
+// its own squared norm (the pivot is the squared distance of the column from
+// that span).  Such columns are marked and dropped from the elimination, so
+// later columns are judged against the kept ones only.  scale receives the
+// RMS entry of the n x l matrix, sqrt(trace / (n l)).
+__global__ __launch_bounds__(kSmallThreads) void k_wide_deficient(const double* __restrict__ gram, uint32_t l,
+                                                                 uint64_t n, double tau, int* __restrict__ mask,
+                                                                 double* __restrict__ scale) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  double* a = reinterpret_cast<double*>(smem_raw);
+  const uint32_t t = threadIdx.x, nt = blockDim.x;
+  load_gram_lds(gram, l, a);
+  for (uint32_t k = 0; k < l; ++k) {
+    __syncthreads();
+    const double d = a[k * kSm + k];
+    const bool keep = d > tau * gram[(uint64_t)k * l + k] && d > 0.0 && isfinite(d);
+    if (t == 0) mask[k] = keep ? 0 : 1;
+    if (!keep) continue;  // uniform: every thread read the same d
+    const double inv = 1.0 / d;
+    const uint32_t items = (l - k - 1) * l;
+    for (uint32_t e = t; e < items; e += nt) {
+      const uint32_t i = k + 1 + e / l, j = e % l;
+      if (j >= i) a[i * kSm + j] = fma(-a[k * kSm + i] * inv, a[k * kSm + j], a[i * kSm + j]);
+    }
+  }
+  if (t == 0) {
+    double tr = 0.0;
+    for (uint32_t c = 0; c < l; ++c) tr += gram[(uint64_t)c * l + c];
+    *scale = sqrt(fmax(tr, 1e-300) / ((double)n * (double)l));
+  }
+}
+
+// out = A with every marked column replaced by a deterministic pseudo-random
+// normal column of RMS entry *scale (SplitMix64 of (row, column), Box-Muller)
+__device__ inline uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_fill_deficient(const double* __restrict__ A, uint64_t n, uint32_t l,
+                                                       const int* __restrict__ mask, const double* __restrict__ scale,
+                                                       double* __restrict__ out) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n * l) return;
+  const uint32_t c = (uint32_t)(gid % l);
+  if (!mask[c]) {
+    out[gid] = A[gid];
+    return;
+  }
+  const uint64_t h = mix64(gid * 0x2545F4914F6CDD1Dull + 0x5851F42D4C957F2Dull);
+  const double u1 = ((double)(h >> 11) + 0.5) * 0x1.0p-53, u2 = (double)(mix64(h) >> 11) * 0x1.0p-53;
+  out[gid] = *scale * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// ---------------------------------------------------------------------------
 constexpr int kWrRows = 64;
 
 __global__ __launch_bounds__(256) void k_wide_write(WideWrite q) {
@@ -410,6 +468,9 @@ __global__ __launch_bounds__(256) void k_wide_write(WideWrite q) {
 }
 
 // ---------------------------------------------------------------------------
+// sort key of an eigenvalue: NaN below every number
+__device__ inline double eig_key(double x) { return x == x ? x : -__builtin_huge_val(); }
+
 // Cyclic Jacobi with round-robin pair ordering (n/2 disjoint rotations per
 // round, Numerical-Recipes angle), the symmetric matrix in LDS and the
 // eigenvectors in global memory (W).  A round: every wave computes the
@@ -543,11 +604,16 @@ __global__ __launch_bounds__(kSmallThreads) void k_sym_eig_wide(const double* __
     }
   }
   // descending eigenvalues (ties by index)
+  // (a total order even when a breakdown upstream left NaNs on the diagonal:
+  // NaN ranks last, ties by index, so order is a permutation of 0 .. L-1;
+  // every entry holds a valid index before the ranks land)
+  if (t < L) order[t] = t;
+  __syncthreads();
   if (t < L) {
-    const double di = a[t * kSm + t];
+    const double di = eig_key(a[t * kSm + t]);
     int rank = 0;
     for (int j = 0; j < L; ++j) {
-      const double dj = a[j * kSm + j];
+      const double dj = eig_key(a[j * kSm + j]);
       rank += (dj > di) || (dj == di && j < t);
     }
     order[rank] = t;
@@ -614,6 +680,15 @@ void launch_wide_write(const WideWrite& q, hipStream_t s) {
   PCAONE_REQUIRE(!q.kperm || q.kpad % 16 == 0, -1, "omega operand: permuted K needs 16-code groups");
   const uint32_t nb = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ceil_div(q.n, kWrRows), 1), 1024);
   hipLaunchKernelGGL(k_wide_write, dim3(nb), dim3(256), 0, s, q);
+}
+
+void launch_wide_repair(const double* A, const double* gram, uint64_t n, uint32_t l, int* mask, double* scale,
+                        double* out, hipStream_t s) {
+  PCAONE_REQUIRE(l >= 1 && l <= kWideMaxL, -1, "rank repair: l must be in [1, 128]");
+  big_lds(k_wide_deficient, kSmallLds);
+  hipLaunchKernelGGL(k_wide_deficient, dim3(1), dim3(kSmallThreads), kSmallLds, s, gram, l, n, 1e-12, mask, scale);
+  hipLaunchKernelGGL(k_fill_deficient, dim3((uint32_t)ceil_div(n * l, 256)), dim3(256), 0, s, A, n, l, mask, scale,
+                     out);
 }
 
 void launch_sym_eig_wide(const double* A, uint32_t l, double* evals, double* W, double* sinv, uint32_t kc,

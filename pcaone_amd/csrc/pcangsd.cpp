@@ -130,7 +130,7 @@ void Engine::gl_fill_E(const double* U, const double* S, const double* V, uint32
     e.dpart = gl_part_.p;
   }
   timed("gl_E", [&] { launch_gl_E(e, stream_); });
-  if (Dc) timed("gl_E", [&] { launch_sum_rows(gl_part_.p, ny, N_, Dc, stream_); });
+  if (Dc) timed("gl_E_dc", [&] { launch_sum_rows(gl_part_.p, ny, N_, Dc, stream_); });
   timed("dense_layout", [&] {
     launch_dense_transpose(dns_.p, (uint32_t)kg_, M_, bs_, nwin_, N_, dnt_.p, (uint32_t)kw_, stream_);
   });

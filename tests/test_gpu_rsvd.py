@@ -437,3 +437,68 @@ def test_split_bf16_vs_fp64_at_scale(gpu):
     assert np.allclose(U.T @ U, np.eye(10), atol=1e-9)
     assert np.allclose(V.T @ V, np.eye(10), atol=1e-9)
     assert np.all(np.diff(S) <= 0)
+
+
+@pytest.mark.parametrize("arm", list(ARMS))
+@pytest.mark.parametrize("source", ["bed", "dense"])
+def test_rank_deficient_windows(gpu, arm, source):
+    """winSVD windows of fewer SNPs than l: the first Omega refreshes see a
+    partial H of rank below l, where CholeskyQR breaks down and the reference's
+    Householder QR fills the trailing columns from rounding.  The solve
+    restarts with the rank repair; its top eigenvalues match exact SVD and
+    the oracle's Householder run."""
+    if source == "bed":
+        n, m, k = 300, 2048, 32               # 32 SNPs per window, l = 64
+        pay = _payload(81, n, m, npops=3, fst=0.1, miss=0.01)
+        gpu.load_bed(pay, n, m)
+        X = oracle_matrix(pay, n, m)
+    else:
+        n, m, k = 3000, 640, 50               # 10 genes per window, l = 100
+        D = pa.sim_counts(82, n, m, base=0.1)
+        gpu.load_dense(D)
+        X = O.dense_centered(D)
+    gpu.rsvd_init(k, k, variant=pa.RSVD_FANCY, bands=64, precision=ARMS[arm])
+    U, S, V, ep, _ = gpu.usv(20, 1e-7)
+    s = np.linalg.svd(X, compute_uv=False)
+    orc = O.RsvdOracle(X, k, k, fancy=True, bands=64)
+    orc.compute_usv(20, 1e-7)
+    top = 5
+    # as close to exact SVD as the reference's own algorithm gets (the oracle's
+    # Householder run), within the arm's arithmetic
+    e_orc = eig_relerr(orc.S[:top] ** 2, s[:top] ** 2)
+    assert e_orc < 1e-5
+    assert eig_relerr(S[:top] ** 2, s[:top] ** 2) < max(10 * e_orc, 1e-9 if arm == "fp64" else 1e-5)
+    assert np.allclose(U.T @ U, np.eye(k), atol=1e-8)
+
+
+@pytest.mark.parametrize("arm", list(ARMS))
+def test_matrix_rank_below_l(gpu, arm):
+    """A matrix of rank below k + oversamples (20 distinct samples, each five
+    times: rank <= 20, l = 40).  G and H are singular; the reference's
+    Householder QR returns trailing columns set by rounding, CholeskyQR2 (with
+    the rank repair when a pivot fails) an orthonormal completion of its own.
+    The eigenvalues within the rank match exact SVD as closely as the
+    oracle's Householder run does."""
+    base = O.decode_codes(_payload(83, 20, 3000), 20, 3000)
+    codes = np.repeat(base, 5, axis=1)        # 100 samples
+    n, m = codes.shape[1], codes.shape[0]
+    b = (n + 3) // 4
+    pay = np.zeros((m, b), dtype=np.uint8)
+    for j in range(n):
+        pay[:, j >> 2] |= (codes[:, j] << (2 * (j & 3))).astype(np.uint8)
+    pay = pay.reshape(-1)
+    gpu.load_bed(pay, n, m)
+    X = oracle_matrix(pay, n, m)
+    s = np.linalg.svd(X, compute_uv=False)
+    r = int(np.sum(s > 1e-9 * s[0]))
+    assert r <= 20
+    gpu.rsvd_init(20, 20, variant=pa.RSVD_FANCY, bands=8, precision=ARMS[arm])
+    U, S, V, ep, _ = gpu.usv(10, 1e-6)
+    orc = O.RsvdOracle(X, 20, 20, fancy=True, bands=8)
+    orc.compute_usv(10, 1e-6)
+    top = min(r, 10)
+    e_orc = eig_relerr(orc.S[:top] ** 2, s[:top] ** 2)
+    e_gpu = eig_relerr(S[:top] ** 2, s[:top] ** 2)
+    assert e_gpu < max(10 * e_orc, 1e-8 if arm == "fp64" else 1e-5), (e_gpu, e_orc)
+    assert np.all(np.isfinite(U)) and np.all(np.isfinite(V))
+    assert np.allclose(U[:, :top].T @ U[:, :top], np.eye(top), atol=1e-6)
