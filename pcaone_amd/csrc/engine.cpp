@@ -178,10 +178,13 @@ void Engine::rsvd_perm(int32_t* perm) const {
 // ---------------------------------------------------------------------------
 // operator set-up
 // ---------------------------------------------------------------------------
-// split-K factor: enough workgroups to cover the 256 CUs twice, few partials
+// split-K factor: enough workgroups to cover the 256 CUs twice, at least
+// four K chunks per slice.  A window segment of a few hundred rows against a
+// long K (200,000 samples) needs a deep split: capped at 8, the dense G
+// product of one 313-row window ran 16 blocks and read 0.8 TB/s.
 static uint32_t split_for(uint64_t blocks, uint32_t kchunks, uint32_t& cps) {
   uint64_t want = ceil_div(512, std::max<uint64_t>(blocks, 1));
-  want = std::min<uint64_t>(std::max<uint64_t>(want, 1), std::min<uint32_t>(kchunks, 8));
+  want = std::min<uint64_t>(std::max<uint64_t>(want, 1), std::min<uint32_t>(std::max<uint32_t>(kchunks / 4, 1), 128));
   cps = (uint32_t)ceil_div(kchunks, want);
   return (uint32_t)ceil_div(kchunks, cps);
 }
@@ -1122,6 +1125,10 @@ void Engine::check_status(const char* context) {
 // last pass needs it, for V), B^T = H R^-1, B B^T = U_B S^2 U_B^T,
 // Ucur = B^T U_B S^-1.
 void Engine::usv_step() {
+  if (robust_) {
+    usv_step_robust();
+    return;
+  }
   if (wide_) {
     usv_step_wide();
     return;
@@ -1192,8 +1199,7 @@ int Engine::usv_epochs(int maxp, double tol, double* diffs, bool& done) {
     rsvd_gandh(pi, nullptr, nullptr, false);
     ++ep;
     usv_step();
-    check_status(robust_ ? " even with the rank repair of the Omega refresh: the matrix has rank below k + oversamples"
-                         : nullptr);
+    check_status(robust_ ? " even in the rank-robust mode" : nullptr);
     if (pi > 0) {
       const int nbN = gram_blocks(N_);
       timed("mev", [&] { launch_xty_partial(Ucur_.p, Upre_.p, N_, k, part_.p, nbN, stream_); });
@@ -1220,6 +1226,36 @@ int Engine::usv_epochs(int maxp, double tol, double* diffs, bool& done) {
   return ep;
 }
 
+// usv_step in robust mode, for a G of numerical rank below l (the data
+// matrix itself of rank below l, where Householder QR -- the reference's --
+// fills Q's trailing columns from rounding and CholeskyQR breaks down): the
+// eigenpairs of G^T G give T = W_r Lambda_r^-1/2 over its numerical range
+// (eigenvalues above 1e-12 of the largest), so Q = G T has orthonormal
+// columns and B^T = X^T Q = H T; the rest is usv_step's: B B^T, its
+// eigenvectors, Ucur = B^T U_B S^-1, and at the end V = G (T U_B).
+void Engine::usv_step_robust() {
+  const uint32_t l = l_;
+  if (part_.n < wide_part_doubles()) part_.reserve(wide_part_doubles());
+  if (!eigtmp_.p) eigtmp_.reserve((uint64_t)l * l);
+  Wg_.reserve((uint64_t)l * l);
+  timed("usv_gram", [&] { launch_wide_gram(G_.p, nullptr, nullptr, M_, l, part_.p, gred_.p, stream_); });
+  timed("usv_eig", [&] {
+    launch_sym_eig_wide(gred_.p, l, evals_.p, Wg_.p, sinv_.p, 0, nullptr, eigtmp_.p, stream_);
+  });
+  timed("usv_rows", [&] { launch_whiten(Wg_.p, evals_.p, l, 1e-12, Rti_.p, stream_); });
+  WideApply a2{H_.p, N_, l, Rti_.p, Bt_.p, nullptr, nullptr};
+  timed("usv_bt", [&] { launch_wide_apply(a2, nullptr, stream_); });
+  timed("usv_gram", [&] { launch_wide_gram(Bt_.p, nullptr, nullptr, N_, l, part_.p, bbt_.p, stream_); });
+  timed("usv_eig", [&] {
+    launch_sym_eig_wide(bbt_.p, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, have_W_ ? W_.p : nullptr, eigtmp_.p,
+                        stream_);
+  });
+  have_W_ = true;
+  timed("usv_rows", [&] {
+    launch_rows_times_small(Bt_.p, N_, l, W_.p, l, (uint32_t)p_.k, sinv_.p, Ucur_.p, stream_);
+  });
+}
+
 void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int* epochs, double* diffs) {
   PCAONE_REQUIRE(have_op_, -3, "computeUSV: operator not initialised");
   PCAONE_REQUIRE(maxp >= 0, -1, "computeUSV: p must be >= 0");
@@ -1236,12 +1272,12 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
     ep = usv_epochs(maxp, tol, diffs, done);
   } catch (const Error& e) {
     if (e.code != -4 || robust_) throw;
-    // A partial H of lower rank than l (a window of fewer SNPs than l, or a
-    // rank-deficient matrix): Householder QR, as the reference uses, carries
-    // on with trailing columns set by rounding; CholeskyQR breaks down.  The
-    // solve restarts with the rank repair in every refresh, from the same
-    // state (Omega is redrawn at pi == 0; the EMU values of this solve are
-    // kept).
+    // A partial H or a G of lower rank than l (a window of fewer SNPs than l,
+    // or a data matrix of rank below l): Householder QR, as the reference
+    // uses, carries on with trailing columns set by rounding; CholeskyQR
+    // breaks down.  The solve restarts in the rank-robust mode (rank repair
+    // in every refresh, G through its Gram eigenpairs) from the same state
+    // (Omega is redrawn at pi == 0; the EMU values of this solve are kept).
     robust_ = true;
     PCAONE_HIP(hipMemcpyAsync(H1_.p, snapH1_.p, hb, hipMemcpyDeviceToDevice, stream_));
     PCAONE_HIP(hipMemcpyAsync(H2_.p, snapH2_.p, hb, hipMemcpyDeviceToDevice, stream_));

@@ -157,6 +157,7 @@ class Engine {
   void run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route);
   void usv_step();
   void usv_step_wide();  // l > 64
+  void usv_step_robust();  // robust mode: G orthogonalised through its Gram eigenpairs
   // computeUSV's epochs; returns the number of computeGandH calls
   int usv_epochs(int maxp, double tol, double* diffs, bool& done);
   void check_status(const char* context = nullptr);
@@ -270,7 +271,7 @@ class Engine {
   // replaced by pseudo-random ones.  The accumulators are kept as they were at
   // the start of the solve, for the restart.
   bool robust_ = false;
-  DevBuf<double> hrep_, snapH1_, snapH2_, rscale_;
+  DevBuf<double> hrep_, snapH1_, snapH2_, rscale_, Wg_;
   DevBuf<int> rmask_;
   DevBuf<int> status_;
   // Gram mat-vec state (one operand column)

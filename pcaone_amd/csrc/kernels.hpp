@@ -420,6 +420,10 @@ void launch_wide_write(const WideWrite& q, hipStream_t s);
 // (RMS entry that of A); mask (l ints) and scale (1 double) are scratch
 void launch_wide_repair(const double* A, const double* gram, uint64_t n, uint32_t l, int* mask, double* scale,
                         double* out, hipStream_t s);
+// T = W diag(evals)^-1/2 over the eigenpairs above tau * evals[0] (descending
+// evals), zero columns for the rest: G T has orthonormal columns spanning the
+// numerical range of G when W, evals are the eigenpairs of G^T G
+void launch_whiten(const double* W, const double* evals, uint32_t l, double tau, double* T, hipStream_t s);
 // Jacobi eigensolve of the full symmetric l x l A (as launch_sym_eig); tmp: l x l scratch
 void launch_sym_eig_wide(const double* A, uint32_t l, double* evals, double* W, double* sinv, uint32_t kc,
                          const double* W0, double* tmp, hipStream_t s);

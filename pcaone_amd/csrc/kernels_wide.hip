@@ -691,6 +691,19 @@ void launch_wide_repair(const double* A, const double* gram, uint64_t n, uint32_
                      out);
 }
 
+__global__ void k_whiten(const double* __restrict__ W, const double* __restrict__ evals, uint32_t l, double tau,
+                         double* __restrict__ T) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= l * l) return;
+  const uint32_t j = e % l;
+  const double ev = evals[j], top = evals[0];
+  T[e] = (ev > tau * top && ev > 0.0) ? W[e] / sqrt(ev) : 0.0;
+}
+
+void launch_whiten(const double* W, const double* evals, uint32_t l, double tau, double* T, hipStream_t s) {
+  hipLaunchKernelGGL(k_whiten, dim3((uint32_t)ceil_div((uint64_t)l * l, 256)), dim3(256), 0, s, W, evals, l, tau, T);
+}
+
 void launch_sym_eig_wide(const double* A, uint32_t l, double* evals, double* W, double* sinv, uint32_t kc,
                          const double* W0, double* tmp, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kWideMaxL, -1, "wide eigensolve: l must be in [1, 128]");
