@@ -50,6 +50,14 @@ FP32_PEAK_TFLOPS = 157.3
 # split-BF16 solve, the factors from a 512 MiB copy kernel
 # (profiles/k_packed_gemm_bf16_c2_winsvd.md)
 TRAFFIC = 291.3e6
+# the same measure for the other workloads' dominant kernels (same calibration):
+# k_ld_geno, the one full-call launch (profiles/k_ld_geno_c5_ld_r2.md)
+TRAFFIC_C5 = 26.03e9
+# k_dense_gemm_bf16x3, averaged over the 280 launches of a solve
+# (profiles/k_dense_gemm_bf16x3_c4_dense_rsvd.md)
+TRAFFIC_C4 = 3.089e9
+# k_gl_E, one launch per EM iteration (profiles/k_gl_E_c3_pcangsd.md)
+TRAFFIC_C3 = 12.49e9
 REF = ROOT / "oracle" / "_ref" / "PCAone"
 # worker threads of the CPU baselines: the GPU host's CPU share for one GPU
 CPU_THREADS = 16
@@ -341,7 +349,8 @@ def run_c5(g, pa, args):
            "int_ops_per_s": achieved * 1e12, "packed_GBps_read_once": round(M * ((N + 3) >> 2) / (ms * 1e-3) / 1e9, 2),
            "r2_range": [float(r2[ok].min()), float(r2[ok].max())],
            "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                        "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
+                        "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                        "traffic": None if args.small else TRAFFIC_C5,
                         "kernel": "k_ld_geno (one persistent launch)",
                         "algorithmic_ops_per_launch": ops, "avg_launch_us": round(ms * 1e3, 1)}}
     if not args.no_cpu:
@@ -399,10 +408,15 @@ def run_c4(g, pa, args):
                                 "eig_relerr_max_top50": float(np.max(np.abs(ev - ev0) / ev0)),
                                 "max_angle_top5_rad": principal_angle(U[:, :5], U64[:, :5])},
            "roofline": {"bound": "hbm", "achieved": round(gbytes / (cms * 1e-3), 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(gbytes / (cms * 1e-3) / HBM_PEAK_GBS, 4), "traffic": None,
+                        "unit": "GB/s", "frac": round(gbytes / (cms * 1e-3) / HBM_PEAK_GBS, 4),
+                        "traffic": None if args.small else TRAFFIC_C4,
                         "kernel": "k_dense_gemm_bf16x3 (gemm_G + gemm_H)", "launches": cnt,
                         "avg_launch_us": round(cms * 1e3 / cnt, 2),
                         "kernel_ms_per_solve": {k: round(v[1], 3) for k, v in sorted(st.items())}}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = c4_cpu_baseline(pa, D, CPU_THREADS)
+    if args.no_iram:
+        return out
     # IRAM top-50 (Spectra op on the GPU, restarted Lanczos on the host): the
     # FP64 arm converges to itol 1e-6 and is the ground truth for every arm
     iram = {}
@@ -427,8 +441,6 @@ def run_c4(g, pa, args):
                 "this flat spectrum in every arm, so the split and FP64 RSVD arms differ by their unconverged "
                 "trajectories (Householder signs and flipOmg are discontinuous), not by their arithmetic",
         **{f"rsvd_{a}_top{t}": rel(res[a][2][:t] ** 2, evi[:t]) for a in ("bf16x2", "fp32", "fp64") for t in (5, 50)}}
-    if not args.no_cpu:
-        out["cpu_baseline"] = c4_cpu_baseline(pa, D, CPU_THREADS)
     return out
 
 
@@ -496,8 +508,9 @@ def run_c3(g, pa, args):
            "kernel_ms": {k: round(v[1], 2) for k, v in sorted(st.items())}}
     if em[0]:
         out["roofline"] = {"bound": "hbm", "achieved": round(ebytes / (em[1] * 1e-3), 1), "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": round(ebytes / (em[1] * 1e-3) / HBM_PEAK_GBS, 4), "traffic": None,
-                           "kernel": "k_gl_E (E update, FP32 likelihoods in, FP32 E out in two layouts)",
+                           "unit": "GB/s", "frac": round(ebytes / (em[1] * 1e-3) / HBM_PEAK_GBS, 4),
+                           "traffic": None if args.small else TRAFFIC_C3,
+                           "kernel": "k_gl_E (E update: two FP32 likelihoods in, FP32 E out)",
                            "launches": em[0], "avg_launch_us": round(em[1] * 1e3 / em[0], 1)}
     if not args.no_cpu:
         out["cpu_baseline"] = c3_cpu_baseline(pa, gl, CPU_THREADS)
@@ -539,6 +552,7 @@ def main():
     ap.add_argument("--arms", default="bf16x2,fp32,fp64",
                     help="C2 arms to run (profiling runs use bf16x2 alone); the JSON line needs all three")
     ap.add_argument("--configs", default="c2,c5,c4,c3", help="workloads to run; the JSON line needs c2")
+    ap.add_argument("--no-iram", action="store_true", help="C4 without the IRAM runs (short profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design "
