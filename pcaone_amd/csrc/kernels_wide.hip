@@ -628,6 +628,207 @@ __global__ __launch_bounds__(kSmallThreads) void k_sym_eig_wide(const double* __
   }
 }
 
+// The same Jacobi with the eigenvectors in LDS as well, for l up to about
+// 115: the symmetric matrix is kept as its packed upper triangle (row i holds
+// columns i .. l-1) and W as l rows of stride l | 1.  Only the 2 x 2 blocks
+// (k1, k2) with k1 <= k2 of a round are rotated (the others are their
+// transposes); the diagonal block keeps its one off-diagonal entry.  A round:
+// lanes 0 .. h-1 of wave 0 compute the angles and pairs, a barrier, every
+// thread rotates its blocks and its W row pairs in place, a barrier.  The
+// starting basis W0, when given, enters as W0^T A W0 by FP64 MFMA on the
+// global copies of A and W0 (tmp holds A W0).
+__device__ inline int pk_idx(int i, int j, int L) {  // packed upper index of (min, max)
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  return a * L - ((a * (a - 1)) >> 1) + (b - a);
+}
+
+// C = X^T Y for l x l row-major X, Y in global memory, FP64 MFMA: every
+// wave takes output tiles w, w + nwaves, ...; emit(row, col, value) per entry
+template <class F>
+__device__ inline void xty_global(const double* __restrict__ X, const double* __restrict__ Y, int L, F emit) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+  const int kr = lane >> 4, ci = lane & 15, nb = (L + 15) / 16;
+  for (int tt = wv; tt < nb * nb; tt += nw) {
+    const int ti = tt / nb, tj = tt % nb;
+    const int cx = 16 * ti + ci, cy = 16 * tj + ci;
+    f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < L; k0 += 4) {
+      const int k = k0 + kr;
+      const double x = (k < L && cx < L) ? X[k * L + cx] : 0.0;
+      const double y = (k < L && cy < L) ? Y[k * L + cy] : 0.0;
+      acc = mfma64(x, y, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * ti + kr + 4 * v;
+      if (r < L && cy < L) emit(r, cy, acc[v]);
+    }
+  }
+}
+
+constexpr int kEigLwItems = (64 * 65 / 2 + 64 * 128 + kSmallThreads - 1) / kSmallThreads;  // per thread and round
+
+size_t eig_lw_bytes(uint32_t l) {
+  const size_t L = l, lw = L | 1;
+  return (L * (L + 1) / 2 + L * lw) * sizeof(double);
+}
+
+__global__ __launch_bounds__(kSmallThreads) void k_sym_eig_lw(const double* __restrict__ A, uint32_t l,
+                                                             double* evals, double* W, double* sinv, uint32_t kc,
+                                                             const double* W0, double* tmp) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int L = (int)l, LW = L | 1, t = threadIdx.x, lane = t & 63, wv = t >> 6, nt = blockDim.x;
+  double* a = reinterpret_cast<double*>(smem_raw);  // packed upper triangle
+  double* v = a + L * (L + 1) / 2;                  // W, L rows of stride LW
+  __shared__ double csw[64][2];
+  __shared__ int pq[64][2];
+  __shared__ double red[kEigWaves][2];
+  __shared__ int order[WW];
+  const int n = (L % 2) ? L + 1 : L;  // dummy index L when odd: a zero row and column
+  const int h = n / 2, nblk = h * (h + 1) / 2;
+  // this thread's items of every round: triangle blocks (k1 <= k2), then (pair, row) of W
+  int asg[kEigLwItems];
+#pragma unroll
+  for (int u = 0; u < kEigLwItems; ++u) {
+    const int e = t + kSmallThreads * u;
+    asg[u] = -1;
+    if (e < nblk) {
+      int k1 = 0, rem = e;
+      while (rem >= h - k1) {
+        rem -= h - k1;
+        ++k1;
+      }
+      asg[u] = (1 << 16) | (k1 << 8) | (k1 + rem);
+    } else if (e < nblk + h * L) {
+      const int f = e - nblk;
+      asg[u] = (2 << 16) | ((f / L) << 8) | (f % L);
+    }
+  }
+  if (W0) {
+    // a <- W0^T A W0 (upper triangle), v <- W0
+    xty_global(A, W0, L, [&](int r, int c, double x) { tmp[r * L + c] = x; });  // A^T W0 = A W0
+    __syncthreads();
+    xty_global(W0, tmp, L, [&](int r, int c, double x) {
+      if (r <= c) a[pk_idx(r, c, L)] = x;
+    });
+    for (int e = t; e < L * L; e += nt) v[(e / L) * LW + e % L] = W0[e];
+  } else {
+    for (int e = t; e < L * L; e += nt) {
+      const int i = e / L, j = e % L;
+      if (i <= j) a[pk_idx(i, j, L)] = A[e];
+      v[i * LW + j] = (i == j) ? 1.0 : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    double off = 0.0, dg = 0.0;
+    for (int e = t; e < L * L; e += nt) {
+      const int i = e / L, j = e % L;
+      if (j < i) continue;
+      const double y = a[pk_idx(i, j, L)], x = y * y;
+      if (i == j) dg += x; else off += 2.0 * x;  // both triangles, as the full-matrix test
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_down(off, o);
+      dg += __shfl_down(dg, o);
+    }
+    if (lane == 0) {
+      red[wv][0] = off;
+      red[wv][1] = dg;
+    }
+    __syncthreads();
+    double offs = 0.0, dgs = 0.0;
+    for (int q = 0; q < kEigWaves; ++q) {
+      offs += red[q][0];
+      dgs += red[q][1];
+    }
+    __syncthreads();  // red is rewritten by the next sweep's check
+    if (offs <= 1e-26 * dgs || offs == 0.0) break;
+    for (int round = 0; round < n - 1; ++round) {
+      if (t < h) {
+        int p, q;
+        rr_pair(t, round, n, p, q);
+        double c = 1.0, sn = 0.0;
+        if (q < L) {
+          const double apq = a[pk_idx(p, q, L)];
+          if (fabs(apq) > 1e-300) {
+            const double theta = (a[pk_idx(q, q, L)] - a[pk_idx(p, p, L)]) / (2.0 * apq);
+            double tt = 1.0 / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
+            if (theta < 0.0) tt = -tt;
+            c = 1.0 / sqrt(fma(tt, tt, 1.0));
+            sn = tt * c;
+          }
+        }
+        csw[t][0] = c;
+        csw[t][1] = sn;
+        pq[t][0] = p;
+        pq[t][1] = q;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < kEigLwItems; ++u) {
+        if (asg[u] < 0) continue;
+        const int kind = asg[u] >> 16, k1 = (asg[u] >> 8) & 255, lo = asg[u] & 255;
+        const int p1 = pq[k1][0], q1 = pq[k1][1];
+        const double c1 = csw[k1][0], s1 = csw[k1][1];
+        if (kind == 1) {
+          const int k2 = lo, p2 = pq[k2][0], q2 = pq[k2][1];
+          const double c2 = csw[k2][0], s2 = csw[k2][1];
+          if (k1 == k2) {  // the diagonal block {p, q}: p < q
+            if (q1 >= L) continue;
+            const int ip = pk_idx(p1, p1, L), iq = pk_idx(q1, q1, L), ipq = pk_idx(p1, q1, L);
+            const double x0 = a[ip], y0 = a[ipq], x1 = a[ipq], y1 = a[iq];
+            const double rp0 = c1 * x0 - s1 * y0, rq0 = s1 * x0 + c1 * y0;
+            const double rp1 = c1 * x1 - s1 * y1, rq1 = s1 * x1 + c1 * y1;
+            a[ip] = c2 * rp0 - s2 * rp1;
+            a[ipq] = s2 * rp0 + c2 * rp1;
+            a[iq] = s2 * rq0 + c2 * rq1;
+          } else {  // rows {p1, q1} x columns {p2, q2}; the dummy index holds zeros
+            const bool vq1 = q1 < L, vq2 = q2 < L;
+            const int i00 = pk_idx(p1, p2, L);
+            const int i10 = vq1 ? pk_idx(q1, p2, L) : 0;
+            const int i01 = vq2 ? pk_idx(p1, q2, L) : 0;
+            const int i11 = (vq1 && vq2) ? pk_idx(q1, q2, L) : 0;
+            const double x0 = a[i00], y0 = vq1 ? a[i10] : 0.0;
+            const double x1 = vq2 ? a[i01] : 0.0, y1 = (vq1 && vq2) ? a[i11] : 0.0;
+            const double rp0 = c1 * x0 - s1 * y0, rq0 = s1 * x0 + c1 * y0;
+            const double rp1 = c1 * x1 - s1 * y1, rq1 = s1 * x1 + c1 * y1;
+            a[i00] = c2 * rp0 - s2 * rp1;
+            if (vq2) a[i01] = s2 * rp0 + c2 * rp1;
+            if (vq1) a[i10] = c2 * rq0 - s2 * rq1;
+            if (vq1 && vq2) a[i11] = s2 * rq0 + c2 * rq1;
+          }
+        } else {  // W entries (row, p1), (row, q1)
+          if (q1 >= L) continue;
+          const double x = v[lo * LW + p1], y = v[lo * LW + q1];
+          v[lo * LW + p1] = c1 * x - s1 * y;
+          v[lo * LW + q1] = s1 * x + c1 * y;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // descending eigenvalues (ties by index; NaN last): a permutation of 0 .. L-1
+  if (t < L) order[t] = t;
+  __syncthreads();
+  if (t < L) {
+    const double di = eig_key(a[pk_idx(t, t, L)]);
+    int rank = 0;
+    for (int j = 0; j < L; ++j) {
+      const double dj = eig_key(a[pk_idx(j, j, L)]);
+      rank += (dj > di) || (dj == di && j < t);
+    }
+    order[rank] = t;
+  }
+  __syncthreads();
+  for (int e = t; e < L * L; e += nt) W[e] = v[(e / L) * LW + order[e % L]];
+  if (t < L) {
+    const double ev = a[pk_idx(order[t], order[t], L)];
+    evals[t] = ev;
+    if (t < (int)kc) sinv[t] = 1.0 / sqrt(fmax(ev, 1e-300));
+  }
+}
+
 template <class K>
 void big_lds(K kern, size_t bytes) {
   PCAONE_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -707,6 +908,14 @@ void launch_whiten(const double* W, const double* evals, uint32_t l, double tau,
 void launch_sym_eig_wide(const double* A, uint32_t l, double* evals, double* W, double* sinv, uint32_t kc,
                          const double* W0, double* tmp, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kWideMaxL, -1, "wide eigensolve: l must be in [1, 128]");
+  // the matrix and the eigenvectors both in LDS while they fit beside the
+  // kernel's static arrays (l up to about 115), else the eigenvectors in global memory
+  if (eig_lw_bytes(l) + 4096 <= 160 * 1024) {
+    big_lds(k_sym_eig_lw, eig_lw_bytes(l));
+    hipLaunchKernelGGL(k_sym_eig_lw, dim3(1), dim3(kSmallThreads), eig_lw_bytes(l), s, A, l, evals, W, sinv, kc, W0,
+                       tmp);
+    return;
+  }
   big_lds(k_sym_eig_wide, kEigLds);
   hipLaunchKernelGGL(k_sym_eig_wide, dim3(1), dim3(kSmallThreads), kEigLds, s, A, l, evals, W, sinv, kc, W0, tmp);
 }
