@@ -257,6 +257,18 @@ int pcaone_gpu_iram(pcaone_gpu* h, int k, int ncv, int maxit, double tol, double
  * refresh, 0 if none); *reset_both = both accumulators cleared first. */
 int pcaone_fancy_plan(int bands, int pi, uint64_t* bandsize, int* acc, int* update, int* zero, int* reset_both);
 
+/* The .ld.gz file of --print-r2: ld_r2_big / ld_r2_small (src/LD.cpp:388-442)
+ * write a header line "CHR_A\tBP_A\tSNP_A\tCHR_B\tBP_B\tSNP_B\tR2", then one line
+ * per pair of the windows (ws, we as for pcaone_ld_windows, r2 in the order of
+ * pcaone_gpu_ld_r2 / pcaone_gpu_ld_r2_geno): the .bim fields (chromosome, bp,
+ * id) of both SNPs and std::to_string(r2).  Same text; the file is a series of
+ * gzip members (level: zlib's -1..9; the reference uses -1, the default)
+ * formatted and compressed by nthreads host threads.  The .bim must hold at
+ * least the SNPs the windows reach; npairs must equal the windows' pair count.
+ * Host only. */
+int pcaone_write_ld_gz(const char* fileout, const char* filebim, const int32_t* ws, const int32_t* we, uint32_t nwin,
+                       const double* r2, uint64_t npairs, int level, int nthreads);
+
 /* ---- output files --------------------------------------------------------
  * Data::write_eigs_files (src/Data.cpp:177-194), host only: prefix.sigvals
  * ("#N,M" then S), prefix.eigvals (S^2 / nsnps * ploidy), prefix.eigvecs (U,

@@ -65,6 +65,8 @@ SIGNATURES = {
     "pcaone_gpu_ld_r2_geno_noalloc": (C.c_int, [C.c_void_p, _I32, _I32, C.c_uint32, _D, C.c_uint64,
                                                 C.POINTER(C.c_uint64), _I32, C.c_int]),
     "pcaone_write_eigs": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, _D, _D, _D, C.c_int]),
+    "pcaone_write_ld_gz": (C.c_int, [C.c_char_p, C.c_char_p, _I32, _I32, C.c_uint32, _D, C.c_uint64, C.c_int,
+                                     C.c_int]),
     "pcaone_gpu_load_gl": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]),
     "pcaone_gpu_gl_em_maf": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, C.POINTER(C.c_int)]),
     "pcaone_gpu_gl_from_bed": (C.c_int, [C.c_void_p]),
@@ -264,6 +266,19 @@ def write_eigs(prefix: str | os.PathLike, S: np.ndarray, U: np.ndarray, V: np.nd
                                  _p(U, C.c_double), _p(Vf, C.c_double), ploidy)
     if rc != 0:
         raise GpuError(rc, "pcaone_write_eigs failed")
+
+
+def write_ld_gz(path: str | os.PathLike, bim: str | os.PathLike, ws: np.ndarray, we: np.ndarray, r2: np.ndarray,
+                level: int = 1, nthreads: int = 16):
+    """The reference's --print-r2 .ld.gz (header, then bim fields of both SNPs and
+    r2 per window pair), formatted and compressed on nthreads host threads."""
+    ws = np.ascontiguousarray(ws, dtype=np.int32)
+    we = np.ascontiguousarray(we, dtype=np.int32)
+    r2 = np.ascontiguousarray(r2, dtype=np.float64)
+    rc = lib().pcaone_write_ld_gz(str(path).encode(), str(bim).encode(), _p(ws, C.c_int32), _p(we, C.c_int32),
+                                  len(ws), _p(r2, C.c_double), len(r2), int(level), int(nthreads))
+    if rc != 0:
+        raise GpuError(rc, "pcaone_write_ld_gz failed")
 
 
 def fancy_plan(bands: int, pi: int, bandsize: int):

@@ -186,6 +186,18 @@ int pcaone_ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end
   }
 }
 
+int pcaone_write_ld_gz(const char* fileout, const char* filebim, const int32_t* ws, const int32_t* we, uint32_t nwin,
+                       const double* r2, uint64_t npairs, int level, int nthreads) {
+  try {
+    pcaone::write_ld_gz(fileout, filebim, ws, we, nwin, r2, npairs, level, nthreads);
+    return PCAONE_OK;
+  } catch (const pcaone::Error& e) {
+    return e.code < 0 ? e.code : PCAONE_EINVAL;
+  } catch (...) {
+    return PCAONE_EINVAL;
+  }
+}
+
 int pcaone_gpu_ld_usv(pcaone_gpu* h, const double* U, uint32_t k) {
   return guarded(h, [&](pcaone::Engine& e) { e.ld_usv(U, k); });
 }

@@ -78,6 +78,9 @@ struct SegBufs {
 uint64_t window_offsets(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m, std::vector<int64_t>& off,
                         std::vector<int32_t>& win_of_row);
 // band tiles (row block of tm SNPs, column block of tn SNPs) of the windows
+// the .ld.gz text of the window pairs (ld_write.cpp)
+void write_ld_gz(const char* fileout, const char* filebim, const int32_t* ws, const int32_t* we, uint32_t nwin,
+                 const double* r2, uint64_t npairs, int level, int nthreads);
 std::vector<int2> band_tiles(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m, int tm, int tn);
 // the per-device scratch of the allocation-free LD entry point (address taken
 // once per device); kLdScratchBytes is its fixed size

@@ -188,8 +188,12 @@ def run_ld_stats1(tmp: Path, name: str, seed, n, per_chr, ld_bp):
                                                                                            int(raw[1]))
     ia, ib, r2 = read_ld_gz(tmp / (name + ".ld.gz"))
     chr_end = np.cumsum(per_chr) - 1
+    # the file bytes as well: the .ld.gz as the reference wrote it and the
+    # .mbim its SNP fields come from (the text of both is the fixture)
+    ldgz = np.frombuffer((tmp / (name + ".ld.gz")).read_bytes(), dtype=np.uint8)
+    mbim = np.frombuffer((tmp / (name + "_res.mbim")).read_bytes(), dtype=np.uint8)
     return dict(payload=pay, nsamples=n, nsnps=m, pos=pos, chr_end=chr_end, ld_bp=ld_bp, resid=resid, snp_a=ia,
-                snp_b=ib, r2=r2)
+                snp_b=ib, r2=r2, ldgz=ldgz, mbim=mbim)
 
 
 def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):

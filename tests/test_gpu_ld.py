@@ -125,6 +125,33 @@ def test_ld_geno_matches_reference_cli(gpu, config):
     assert np.abs(r2[~nan] - ro[~nan]).max() < 1e-12
 
 
+def test_ld_geno_written_as_reference_text(gpu, tmp_path):
+    """GPU r^2 through pcaone_write_ld_gz against the reference CLI's .ld.gz:
+    every line's six SNP fields equal, NaN lines equal, the printed r^2 within
+    one unit of the 6th decimal (the reference's r^2 comes from float32
+    residuals, so a printed digit may round the other way)."""
+    import gzip
+
+    z, n, m, codes, ws, we = _ld_geno_case()
+    gpu.load_bed(z["payload"], n, m)
+    r2, _, _ = gpu.ld_r2_geno(ws, we)
+    (tmp_path / "g.mbim").write_bytes(z["mbim"].tobytes())
+    pa.write_ld_gz(tmp_path / "g.ld.gz", tmp_path / "g.mbim", ws, we, r2)
+    got = gzip.decompress((tmp_path / "g.ld.gz").read_bytes()).splitlines()
+    ref = gzip.decompress(z["ldgz"].tobytes()).splitlines()
+    assert len(got) == len(ref) and got[0] == ref[0]
+    same = 0
+    for a, b in zip(got[1:], ref[1:]):
+        fa, fb = a.split(b"\t"), b.split(b"\t")
+        assert fa[:6] == fb[:6]
+        if fb[6] == b"-nan":
+            assert fa[6] == b"-nan"
+        else:
+            assert abs(float(fa[6]) - float(fb[6])) <= 1.0000001e-6
+        same += a == b
+    assert same >= 0.95 * (len(ref) - 1)
+
+
 @pytest.mark.parametrize("n,m,bp,miss", [(2, 70, 300, 0.0), (3, 129, 200, 0.2), (67, 500, 1000, 0.05),
                                          (301, 1100, 20, 0.01), (1001, 700, 10 ** 6, 0.02)])
 def test_ld_geno_sums_bit_exact(gpu, n, m, bp, miss):
