@@ -165,6 +165,23 @@ int pcaone_gpu_pcangsd_update(pcaone_gpu* h, const double* U, const double* S, c
 int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
                        double* V, int* iters, double* diffs, double* Dc, double* C);
 
+/* The GRM's eigenvectors for .eigvecs2: the PCAngsd branch of the EM driver
+ * decomposes C with Eigen::JacobiSVD and writes its left singular vectors
+ * (src/Halko.cpp:317-331; src/Arnoldi.cpp:94-103).  C is symmetric, so these
+ * are its eigenvectors ordered by |eigenvalue| descending; column signs are
+ * the solver's own.  Runs on the C kept by the last pcaone_gpu_pcangsd call
+ * that was given C (FP64, on the device).  evals (nsamples, optional): the
+ * singular values |lambda|; U2 (nsamples x nsamples row-major, optional): row i
+ * = sample i, column j = vector j. */
+int pcaone_gpu_grm_eig(pcaone_gpu* h, double* evals, double* U2);
+
+/* write_eigvecs2_beagle (src/Utils.cpp:426-438), host only: "#FID\tIID\tPC1 ..
+ * PCn", then per sample (names from the BEAGLE header, plain or gzipped, as
+ * parse_beagle_samples, src/Utils.cpp:407-424) "name\tname\t" and row i of U2
+ * as Eigen prints a row: 6 significant digits, right-aligned to the row's
+ * widest value, separated by spaces. */
+int pcaone_write_eigvecs2(const char* fileout, const char* filebeagle, uint32_t n, const double* U2);
+
 /* ---- LD r^2 (run_ld_stuff --print-r2 on a .residuals matrix) ------------ */
 /* divide_pos_by_window (src/LD.cpp:125-139) from the positions of nsnps SNPs
  * and each chromosome's last SNP index (0-based, increasing, the last one

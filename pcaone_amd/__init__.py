@@ -65,6 +65,8 @@ SIGNATURES = {
     "pcaone_gpu_ld_r2_geno_noalloc": (C.c_int, [C.c_void_p, _I32, _I32, C.c_uint32, _D, C.c_uint64,
                                                 C.POINTER(C.c_uint64), _I32, C.c_int]),
     "pcaone_write_eigs": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, _D, _D, _D, C.c_int]),
+    "pcaone_gpu_grm_eig": (C.c_int, [C.c_void_p, _D, _D]),
+    "pcaone_write_eigvecs2": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, _D]),
     "pcaone_write_ld_gz": (C.c_int, [C.c_char_p, C.c_char_p, _I32, _I32, C.c_uint32, _D, C.c_uint64, C.c_int,
                                      C.c_int]),
     "pcaone_gpu_load_gl": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_int]),
@@ -281,6 +283,17 @@ def write_ld_gz(path: str | os.PathLike, bim: str | os.PathLike, ws: np.ndarray,
         raise GpuError(rc, "pcaone_write_ld_gz failed")
 
 
+def write_eigvecs2(path: str | os.PathLike, beagle: str | os.PathLike, U2: np.ndarray):
+    """The reference's .eigvecs2 (GRM eigenvectors, one row per BEAGLE sample)."""
+    U2 = np.ascontiguousarray(U2, dtype=np.float64)
+    n = U2.shape[0]
+    if U2.shape != (n, n):
+        raise ValueError("U2 must be square")
+    rc = lib().pcaone_write_eigvecs2(str(path).encode(), str(beagle).encode(), n, _p(U2, C.c_double))
+    if rc != 0:
+        raise GpuError(rc, "pcaone_write_eigvecs2 failed")
+
+
 def fancy_plan(bands: int, pi: int, bandsize: int):
     """Host schedule of one winSVD computeGandH call -> (acc, update, zero, reset, new_bandsize)."""
     bs = C.c_uint64(bandsize)
@@ -399,6 +412,14 @@ class Gpu:
                                                 _p(diffs, C.c_double), _p(Dc, C.c_double) if dc else None,
                                                 _p(Cm, C.c_double) if grm else None))
         return U, S, V, it.value, diffs[:it.value], Dc, Cm
+
+    def grm_eig(self):
+        """Eigenpairs of the GRM kept by the last pcangsd(grm=True): (|lambda|
+        descending, U2 (N, N) with row i = sample i), as JacobiSVD(C) orders them."""
+        ev = np.empty(self.N)
+        U2 = np.empty((self.N, self.N))
+        self._check(self._L.pcaone_gpu_grm_eig(self.h, _p(ev, C.c_double), _p(U2, C.c_double)))
+        return ev, U2
 
     def ld_usv(self, U: np.ndarray):
         """--USV on the loaded PLINK set: the LD matrix becomes (I - U U^T) G,

@@ -58,7 +58,7 @@ def build(verbose: bool = True) -> Path:
     lib = OUT / LIBNAME
     tmp = OUT / (LIBNAME + ".tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
-           "-lpthread", "-lz"]
+           "-lpthread", "-lz", "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")

@@ -2,10 +2,15 @@
 // handle and returns a negative status (the reference's cao.error throws
 // std::runtime_error, src/Logger.hpp:96-105; the C++ wrapper on the other side
 // of this ABI turns the status back into an exception).
+#include <zlib.h>
+
+#include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/pcaone_gpu.h"
 #include "engine.hpp"
@@ -274,6 +279,79 @@ int pcaone_write_eigs(const char* prefix, uint32_t nsamples, uint32_t nsnps, int
     rows(outv, V, nsnps);
   }
   return PCAONE_OK;
+}
+
+int pcaone_gpu_grm_eig(pcaone_gpu* h, double* evals, double* U2) {
+  return guarded(h, [&](pcaone::Engine& e) { e.grm_eig(evals, U2); });
+}
+
+extern "C++" {
+namespace {
+// parse_beagle_samples (src/Utils.cpp:407-424): the header line's tokens (split
+// at runs of tab, space, newline); after the first, every third one is a sample
+std::vector<std::string> beagle_samples(const char* path) {
+  gzFile fp = gzopen(path, "r");
+  if (!fp) throw pcaone::Error(PCAONE_EINVAL, std::string("can not open ") + path);
+  std::string line;
+  char buf[1 << 16];
+  while (gzgets(fp, buf, sizeof(buf))) {
+    line += buf;
+    if (!line.empty() && line.back() == '\n') break;
+  }
+  gzclose(fp);
+  std::vector<std::string> tok;
+  size_t i = 0;
+  while (i < line.size()) {
+    while (i < line.size() && (line[i] == '\t' || line[i] == ' ' || line[i] == '\n')) ++i;
+    size_t j = i;
+    while (j < line.size() && !(line[j] == '\t' || line[j] == ' ' || line[j] == '\n')) ++j;
+    if (j > i) tok.emplace_back(line, i, j - i);
+    i = j;
+  }
+  if (tok.empty() || tok.size() % 3)
+    throw pcaone::Error(PCAONE_EINVAL, "Number of columns should be a multiple of 3.");
+  std::vector<std::string> out;
+  for (size_t c = 3; c < tok.size(); c += 3) out.push_back(tok[c]);
+  return out;
+}
+}  // namespace
+}  // extern "C++"
+
+int pcaone_write_eigvecs2(const char* fileout, const char* filebeagle, uint32_t n, const double* U2) {
+  try {
+    if (!fileout || !filebeagle || !U2 || n == 0) return PCAONE_EINVAL;
+    const std::vector<std::string> names = beagle_samples(filebeagle);
+    if (names.size() != n) return PCAONE_EINVAL;
+    std::ofstream out(fileout);
+    if (!out) return PCAONE_EINVAL;
+    std::string line = "#FID\tIID";
+    for (uint32_t j = 0; j < n; ++j) line += "\tPC" + std::to_string(j + 1);
+    out << line << '\n';
+    // Eigen's default row print: "%g"-style 6 significant digits (the stream's
+    // precision), every value right-aligned to the row's widest, " " between
+    std::vector<std::string> v(n);
+    char num[64];
+    for (uint32_t i = 0; i < n; ++i) {
+      size_t w = 0;
+      for (uint32_t j = 0; j < n; ++j) {
+        const int len = std::snprintf(num, sizeof(num), "%.6g", U2[(uint64_t)i * n + j]);
+        v[j].assign(num, (size_t)len);
+        w = std::max(w, v[j].size());
+      }
+      line = names[i] + "\t" + names[i] + "\t";
+      for (uint32_t j = 0; j < n; ++j) {
+        if (j) line += ' ';
+        line.append(w - v[j].size(), ' ');
+        line += v[j];
+      }
+      out << line << '\n';
+    }
+    return out ? PCAONE_OK : PCAONE_EINVAL;
+  } catch (const pcaone::Error& e) {
+    return e.code < 0 ? e.code : PCAONE_EINVAL;
+  } catch (...) {
+    return PCAONE_EINVAL;
+  }
 }
 
 int pcaone_gpu_set_profiling(pcaone_gpu* h, int on) {

@@ -83,6 +83,7 @@ Engine::Engine(int device) : device_(device) {
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  destroy_blas(blas_);
   timer.resolve();
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -123,7 +124,7 @@ void Engine::load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps)
   }
   have_bed_ = true;
   src_ = SRC_BED;
-  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = have_ldg_ = false;
+  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = have_ldg_ = have_grm_ = false;
 }
 
 // Replaces FileBin::read_all (src/FileBinary.cpp:21-30): nfeatures rows of
@@ -149,7 +150,7 @@ void Engine::load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfea
   counts_orig_.clear();
   have_bed_ = true;
   src_ = SRC_DENSE;
-  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = have_ldg_ = false;
+  gl_mode_ = have_glF_ = gl_bed_ = have_resid_ = have_ldg_ = have_grm_ = false;
 }
 
 void Engine::feature_means(double* mu) const {

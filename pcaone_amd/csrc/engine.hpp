@@ -78,6 +78,8 @@ struct SegBufs {
 uint64_t window_offsets(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m, std::vector<int64_t>& off,
                         std::vector<int32_t>& win_of_row);
 // band tiles (row block of tm SNPs, column block of tn SNPs) of the windows
+// rocBLAS handle teardown of grm_eig (grm_eig.cpp); null is a no-op
+void destroy_blas(void* handle);
 // the .ld.gz text of the window pairs (ld_write.cpp)
 void write_ld_gz(const char* fileout, const char* filebim, const int32_t* ws, const int32_t* we, uint32_t nwin,
                  const double* r2, uint64_t npairs, int level, int nthreads);
@@ -105,6 +107,9 @@ class Engine {
   void gl_em_maf(int maxiter, double tol, double* F, int* iters);
   void pcangsd_update(const double* U, const double* S, const double* V, int k, int standardize, float* E,
                       double* Dc);
+  // eigenpairs of the kept GRM as JacobiSVD(C) gives them (src/Halko.cpp:326-330):
+  // |lambda| descending, U2 row-major N x N (row i = sample i)
+  void grm_eig(double* evals, double* U2);
   void pcangsd(int maxp, double tol, int maxiter, double tolem, double* U, double* S, double* V, int* iters,
                double* diffs, double* Dc, double* C);
   // LD r^2 over windows on the loaded dense matrix, or on the --USV-adjusted genotypes
@@ -210,6 +215,10 @@ class Engine {
   // terms, built on the first call after a load; tuned tile shape per
   // (window-size class)
   bool have_ldg_ = false;
+  // the GRM of the last pcangsd call that returned it (N x N, kept for grm_eig)
+  DevBuf<double> grm_;
+  bool have_grm_ = false;
+  void* blas_ = nullptr;  // rocBLAS handle of the GRM eigensolve, made on first use
   uint32_t ldg_row_bytes_ = 0;
   DevBuf<uint8_t> ldg_bed_;
   DevBuf<double> ldg_F_, ldg_mun_, ldg_ss_;

@@ -40,7 +40,7 @@ void Engine::load_gl(const float* gl, uint32_t nsamples, uint32_t nsites, int pe
   have_bed_ = true;
   src_ = SRC_DENSE;
   gl_mode_ = true;
-  have_glF_ = gl_bed_ = have_resid_ = have_ldg_ = false;
+  have_glF_ = gl_bed_ = have_resid_ = have_ldg_ = have_grm_ = false;
 }
 
 // --pcangsd on in-core PLINK input (src/FilePlink.cpp:68, 89-114;
@@ -64,7 +64,7 @@ void Engine::gl_from_bed() {
   counts_orig_.clear();
   src_ = SRC_DENSE;
   gl_mode_ = have_glF_ = gl_bed_ = true;
-  have_resid_ = have_ldg_ = false;
+  have_resid_ = have_ldg_ = have_grm_ = false;
 }
 
 // emMAF_with_GL (src/Utils.cpp:501-531) from F = 0.25 (src/FileBeagle.cpp:127):
@@ -207,12 +207,11 @@ void Engine::pcangsd(int maxp, double tol, int maxiter, double tolem, double* U,
     PCAONE_HIP(hipMemcpyAsync(Sdev_.p, last_S_.data(), (size_t)k * 8, hipMemcpyHostToDevice, stream_));
     gl_fill_E(Ucur_.p, Sdev_.p, V_.p, k, true, gl_Dc_.p);
     if (Dc) PCAONE_HIP(hipMemcpyAsync(Dc, gl_Dc_.p, (size_t)N_ * 8, hipMemcpyDeviceToHost, stream_));
-    if (C) {
-      DevBuf<double> dc;
-      dc.reserve((uint64_t)N_ * N_);
-      timed("grm", [&] { launch_grm(dns_.p, kg_, N_, M_, gl_Dc_.p, dc.p, stream_); });
-      PCAONE_HIP(hipMemcpyAsync(C, dc.p, (size_t)N_ * N_ * 8, hipMemcpyDeviceToHost, stream_));
-      sync();  // dc is freed on return
+    if (C) {  // kept on the device for grm_eig
+      grm_.reserve((uint64_t)N_ * N_);
+      timed("grm", [&] { launch_grm(dns_.p, kg_, N_, M_, gl_Dc_.p, grm_.p, stream_); });
+      PCAONE_HIP(hipMemcpyAsync(C, grm_.p, (size_t)N_ * N_ * 8, hipMemcpyDeviceToHost, stream_));
+      have_grm_ = true;
     }
     sync();
   }

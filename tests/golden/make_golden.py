@@ -246,6 +246,9 @@ def main():
                     continue
                 res = run_case(tmp, name + ".beagle.gz", args, case, flag="-G")
                 res["cov"] = np.loadtxt(tmp / (case + ".cov"))
+                # the GRM's JacobiSVD vectors, and the BEAGLE file their sample names come from
+                res["text_eigvecs2"] = (tmp / (case + ".eigvecs2")).read_text()
+                res["beagle_gz"] = np.frombuffer((tmp / (name + ".beagle.gz")).read_bytes(), dtype=np.uint8)
                 np.savez_compressed(OUT / f"{case}.npz", gl=gl, nsamples=n, nsnps=m, args=args, **res)
                 print(case, res["eigvals"])
         for name, (seed, n, per_chr, ld_bp) in LD.items():

@@ -94,6 +94,48 @@ def test_pcangsd_matches_reference_cli(gpu, arm):
 
 
 @pytest.mark.parametrize("arm", list(ARMS))
+def test_grm_eigvecs2_matches_reference_cli(gpu, arm, tmp_path):
+    """The GRM eigendecomposition on the device (pcaone_gpu_grm_eig) and the
+    .eigvecs2 writer against the reference CLI's file: an orthonormal
+    eigenbasis of the device's own C to FP64 rounding, |lambda| descending and
+    equal to numpy's eigenvalues of that C, the leading well-separated vectors
+    equal to the reference's up to sign, and the written file's header and
+    sample names equal to the reference's."""
+    z = load_golden(GOLDEN / "gl_pcangsd_svd1.npz")
+    d = parse_args(z["args"])
+    gpu.load_gl(z["gl"])
+    gpu.gl_em_maf(100, 1e-6)
+    gpu.rsvd_init(d["k"], d["os"], variant=d["svd"], precision=ARMS[arm], seed=d["seed"], rand=d["rand"])
+    _, _, _, _, _, _, Cm = gpu.pcangsd(d["p"], d["tol"], 100, 1e-5)
+    ev, U2 = gpu.grm_eig()
+    n = Cm.shape[0]
+    assert np.abs(U2.T @ U2 - np.eye(n)).max() < 1e-12
+    lam = np.linalg.eigvalsh(Cm)
+    want = np.sort(np.abs(lam))[::-1]
+    assert np.abs(ev - want).max() < 1e-12 * want[0]
+    R = Cm @ U2
+    sgn = np.sign(np.einsum("ij,ij->j", U2, R))
+    assert np.abs(R - U2 * (sgn * ev)).max() < 1e-12 * want[0]
+    text = str(z["text_eigvecs2"])
+    gold = np.array([[float(x) for x in ln.split("\t")[2].split()] for ln in text.splitlines()[1:]])
+    for j in range(5):
+        s = np.sign(U2[:, j] @ gold[:, j])
+        assert np.abs(U2[:, j] - s * gold[:, j]).max() < 1e-4
+    (tmp_path / "g.beagle.gz").write_bytes(z["beagle_gz"].tobytes())
+    pa.write_eigvecs2(tmp_path / "g.eigvecs2", tmp_path / "g.beagle.gz", U2)
+    got = (tmp_path / "g.eigvecs2").read_text().splitlines()
+    ref = text.splitlines()
+    assert got[0] == ref[0] and len(got) == len(ref)
+    assert [ln.split("\t")[:2] for ln in got[1:]] == [ln.split("\t")[:2] for ln in ref[1:]]
+
+
+def test_grm_eig_needs_a_grm(gpu):
+    gpu.load_gl(_gl(3, 20, 100))
+    with pytest.raises(pa.GpuError):
+        gpu.grm_eig()
+
+
+@pytest.mark.parametrize("arm", list(ARMS))
 def test_pcangsd_plink_matches_reference_cli(gpu, arm):
     """--pcangsd on in-core PLINK input: hard-call likelihoods, genotype-count
     F, centred-genotype initial E, then the EM (no GRM step)."""
