@@ -165,6 +165,40 @@ int pcaone_gpu_pcangsd_update(pcaone_gpu* h, const double* U, const double* S, c
 int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
                        double* V, int* iters, double* diffs, double* Dc, double* C);
 
+/* ---- LD inputs and consumers ------------------------------------------- */
+/* Data::write_residuals (src/Data.cpp:196-245) for the loaded PLINK genotypes
+ * (the --ld route of a PLINK PCA, which runs unstandardised): the centred
+ * genotypes BED2GENO - F (missing 0), minus U S V^T when ld_stats == 0 (U
+ * nsamples x k, V nsnps x k column-major, S k; 1 <= k <= 64; ignored for
+ * ld_stats == 1), each SNP re-centred by its mean over samples, written as a
+ * uint32 nsnps, a uint32 nsamples and then one float32 row of nsamples per SNP
+ * in the .bed's SNP order. */
+int pcaone_gpu_write_residuals(pcaone_gpu* h, const char* fileout, const double* U, const double* S,
+                               const double* V, uint32_t k, int ld_stats);
+/* Data::save_snps_in_bim (src/Data.cpp:96-135), host only: each line of
+ * filebim, a tab and F[j] (6 significant digits); the .bim must have nsnps
+ * lines. */
+int pcaone_write_mbim(const char* fileout, const char* filebim, const double* F, uint32_t nsnps);
+/* ld_prune_big (src/LD.cpp:194-242) on the handle's LD matrix (a dense
+ * .residuals matrix, or the loaded PLINK genotypes): windows of ld_bp from
+ * filebim's positions (as pcaone_ld_windows), r^2 of their pairs on the
+ * device, then the greedy pass: a window's lead, if still kept, drops each
+ * kept partner with r^2 > r2_tol (the partner, or when filebim has a 7th
+ * column of allele frequencies, the one of lower MAF).  Writes
+ * prefix.ld.prune.in and prefix.ld.prune.out (the kept / dropped .bim lines). */
+int pcaone_gpu_ld_prune(pcaone_gpu* h, const char* filebim, int ld_bp, double r2_tol, const char* prefix);
+/* ld_clump_single_pheno with valid_assoc_file, map_index_snps and
+ * get_target_snp_idx (src/LD.cpp:244-370, 83-122) on the handle's LD matrix:
+ * fileassoc is tab-separated with a header holding the chromosome, position
+ * and p-value columns (colnames "CHR,BP,P" when NULL or empty); index SNPs
+ * (p <= clump_p1, in p order) absorb the SNPs with p <= clump_p2 within
+ * clump_bp whose r^2 >= clump_r2.  Writes fileout: the header + "\tSP2", then
+ * per index SNP its line, a tab and the clumped positions by p (or NONE).
+ * Equal p-values are taken by position (the reference's order for ties
+ * follows its hash map). */
+int pcaone_gpu_ld_clump(pcaone_gpu* h, const char* fileassoc, const char* colnames, const char* filebim, int clump_bp,
+                        double clump_r2, double clump_p1, double clump_p2, const char* fileout);
+
 /* The GRM's eigenvectors for .eigvecs2: the PCAngsd branch of the EM driver
  * decomposes C with Eigen::JacobiSVD and writes its left singular vectors
  * (src/Halko.cpp:317-331; src/Arnoldi.cpp:94-103).  C is symmetric, so these

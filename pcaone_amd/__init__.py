@@ -66,6 +66,11 @@ SIGNATURES = {
                                                 C.POINTER(C.c_uint64), _I32, C.c_int]),
     "pcaone_write_eigs": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, _D, _D, _D, C.c_int]),
     "pcaone_gpu_grm_eig": (C.c_int, [C.c_void_p, _D, _D]),
+    "pcaone_gpu_write_residuals": (C.c_int, [C.c_void_p, C.c_char_p, _D, _D, _D, C.c_uint32, C.c_int]),
+    "pcaone_write_mbim": (C.c_int, [C.c_char_p, C.c_char_p, _D, C.c_uint32]),
+    "pcaone_gpu_ld_prune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_char_p]),
+    "pcaone_gpu_ld_clump": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_double,
+                                      C.c_double, C.c_double, C.c_char_p]),
     "pcaone_write_eigvecs2": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, _D]),
     "pcaone_write_ld_gz": (C.c_int, [C.c_char_p, C.c_char_p, _I32, _I32, C.c_uint32, _D, C.c_uint64, C.c_int,
                                      C.c_int]),
@@ -283,6 +288,14 @@ def write_ld_gz(path: str | os.PathLike, bim: str | os.PathLike, ws: np.ndarray,
         raise GpuError(rc, "pcaone_write_ld_gz failed")
 
 
+def write_mbim(path: str | os.PathLike, bim: str | os.PathLike, F: np.ndarray):
+    """The reference's .mbim: every .bim line with its allele frequency appended."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    rc = lib().pcaone_write_mbim(str(path).encode(), str(bim).encode(), _p(F, C.c_double), len(F))
+    if rc != 0:
+        raise GpuError(rc, "pcaone_write_mbim failed")
+
+
 def write_eigvecs2(path: str | os.PathLike, beagle: str | os.PathLike, U2: np.ndarray):
     """The reference's .eigvecs2 (GRM eigenvectors, one row per BEAGLE sample)."""
     U2 = np.ascontiguousarray(U2, dtype=np.float64)
@@ -412,6 +425,32 @@ class Gpu:
                                                 _p(diffs, C.c_double), _p(Dc, C.c_double) if dc else None,
                                                 _p(Cm, C.c_double) if grm else None))
         return U, S, V, it.value, diffs[:it.value], Dc, Cm
+
+    def write_residuals(self, path, U=None, S=None, V=None, ld_stats: int = 0):
+        """The reference's .residuals of the loaded PLINK genotypes (ld_stats 0:
+        minus U S V^T of a PCA on them; 1: the centred genotypes)."""
+        if ld_stats == 0:
+            U = np.asfortranarray(U, dtype=np.float64)
+            V = np.asfortranarray(V, dtype=np.float64)
+            S = np.ascontiguousarray(S, dtype=np.float64)
+            k = U.shape[1]
+        else:
+            k = 0
+        self._check(self._L.pcaone_gpu_write_residuals(self.h, str(path).encode(), _p(U, C.c_double) if k else None,
+                                                       _p(S, C.c_double) if k else None,
+                                                       _p(V, C.c_double) if k else None, k, int(ld_stats)))
+
+    def ld_prune(self, bim, ld_bp: int, r2_tol: float, prefix):
+        """--ld-r2 pruning on the handle's LD matrix: prefix.ld.prune.in / .out."""
+        self._check(self._L.pcaone_gpu_ld_prune(self.h, str(bim).encode(), int(ld_bp), float(r2_tol),
+                                                 str(prefix).encode()))
+
+    def ld_clump(self, assoc, bim, out, clump_bp: int = 250000, clump_r2: float = 0.5, clump_p1: float = 1e-4,
+                 clump_p2: float = 0.01, colnames: str | None = None):
+        """--clump on the handle's LD matrix: the .clump text to `out`."""
+        self._check(self._L.pcaone_gpu_ld_clump(self.h, str(assoc).encode(), (colnames or "").encode(),
+                                                 str(bim).encode(), int(clump_bp), float(clump_r2),
+                                                 float(clump_p1), float(clump_p2), str(out).encode()))
 
     def grm_eig(self):
         """Eigenpairs of the GRM kept by the last pcangsd(grm=True): (|lambda|

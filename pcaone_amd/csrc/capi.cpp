@@ -281,6 +281,33 @@ int pcaone_write_eigs(const char* prefix, uint32_t nsamples, uint32_t nsnps, int
   return PCAONE_OK;
 }
 
+int pcaone_gpu_write_residuals(pcaone_gpu* h, const char* fileout, const double* U, const double* S,
+                               const double* V, uint32_t k, int ld_stats) {
+  return guarded(h, [&](pcaone::Engine& e) { e.write_residuals(fileout, U, S, V, k, ld_stats); });
+}
+
+int pcaone_write_mbim(const char* fileout, const char* filebim, const double* F, uint32_t nsnps) {
+  try {
+    pcaone::write_mbim(fileout, filebim, F, nsnps);
+    return PCAONE_OK;
+  } catch (const pcaone::Error& e) {
+    return e.code < 0 ? e.code : PCAONE_EINVAL;
+  } catch (...) {
+    return PCAONE_EINVAL;
+  }
+}
+
+int pcaone_gpu_ld_prune(pcaone_gpu* h, const char* filebim, int ld_bp, double r2_tol, const char* prefix) {
+  return guarded(h, [&](pcaone::Engine& e) { e.ld_prune(filebim, ld_bp, r2_tol, prefix); });
+}
+
+int pcaone_gpu_ld_clump(pcaone_gpu* h, const char* fileassoc, const char* colnames, const char* filebim, int clump_bp,
+                        double clump_r2, double clump_p1, double clump_p2, const char* fileout) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    e.ld_clump(fileassoc, colnames, filebim, clump_bp, clump_r2, clump_p1, clump_p2, fileout);
+  });
+}
+
 int pcaone_gpu_grm_eig(pcaone_gpu* h, double* evals, double* U2) {
   return guarded(h, [&](pcaone::Engine& e) { e.grm_eig(evals, U2); });
 }

@@ -78,6 +78,8 @@ struct SegBufs {
 uint64_t window_offsets(const int32_t* ws, const int32_t* we, uint32_t nwin, uint32_t m, std::vector<int64_t>& off,
                         std::vector<int32_t>& win_of_row);
 // band tiles (row block of tm SNPs, column block of tn SNPs) of the windows
+// .mbim: the .bim lines with each SNP's allele frequency appended (ld_tools.hip)
+void write_mbim(const char* fileout, const char* filebim, const double* F, uint32_t nsnps);
 // rocBLAS handle teardown of grm_eig (grm_eig.cpp); null is a no-op
 void destroy_blas(void* handle);
 // the .ld.gz text of the window pairs (ld_write.cpp)
@@ -107,6 +109,14 @@ class Engine {
   void gl_em_maf(int maxiter, double tol, double* F, int* iters);
   void pcangsd_update(const double* U, const double* S, const double* V, int k, int standardize, float* E,
                       double* Dc);
+  // .residuals of the loaded PLINK genotypes (ld_stats 0: minus U S V^T)
+  void write_residuals(const char* fileout, const double* U, const double* S, const double* V, uint32_t k,
+                       int ld_stats);
+  // r^2 of window pairs on the LD matrix the handle holds (dense or PLINK)
+  std::vector<double> ld_pairs(const std::vector<int32_t>& ws, const std::vector<int32_t>& we);
+  void ld_prune(const char* filebim, int ld_bp, double r2_tol, const char* prefix);
+  void ld_clump(const char* fileassoc, const char* colnames, const char* filebim, int clump_bp, double clump_r2,
+                double clump_p1, double clump_p2, const char* fileout);
   // eigenpairs of the kept GRM as JacobiSVD(C) gives them (src/Halko.cpp:326-330):
   // |lambda| descending, U2 row-major N x N (row i = sample i)
   void grm_eig(double* evals, double* U2);

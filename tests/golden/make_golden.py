@@ -196,6 +196,70 @@ def run_ld_stats1(tmp: Path, name: str, seed, n, per_chr, ld_bp):
                 snp_b=ib, r2=r2, ldgz=ldgz, mbim=mbim)
 
 
+LD_TOOLS = {"ld_tools_bp3000": (71, 157, (230, 170), 3000)}
+
+
+def run_ld_tools(tmp: Path, name: str, seed, n, per_chr, ld_bp):
+    """The LD consumers and producers on one PLINK set (3 % missing, one
+    monomorphic SNP, 1..400 bp gaps, two chromosomes):
+    - --ld (ld-stats 0) after a PCA: .residuals, .mbim and the U, S, V it used;
+    - --ld-r2 pruning on those residuals with the .mbim (MAF choice), with a
+      plain .bim (random choice), and on the .bed with --USV of that PCA;
+    - --clump of an assoc-like file (CHR, SNP, BP, P; distinct p-values, a few
+      SNPs left out) on the residuals."""
+    m = int(sum(per_chr))
+    pay = pa.sim_bed(seed, n, m, 3, 0.1, 0.05, 0.5, 0.03).reshape(m, -1).copy()
+    rng = np.random.default_rng(seed)
+    # LD: most SNPs copy their left neighbour's codes for ~85 % of the samples
+    nb = pay.shape[1]
+    codes = np.stack([(pay >> (2 * b)) & 3 for b in range(4)], axis=2).reshape(m, 4 * nb)
+    for i in range(1, m):
+        if rng.random() < 0.7:
+            same = rng.random(4 * nb) < 0.85
+            codes[i, same] = codes[i - 1, same]
+    codes = codes.reshape(m, nb, 4)
+    pay = np.zeros((m, nb), dtype=np.uint8)
+    for b in range(4):
+        pay |= (codes[:, :, b].astype(np.uint8) << (2 * b))
+    pay[:, -1] &= np.uint8((1 << (2 * (n - 4 * (nb - 1)))) - 1) if n % 4 else np.uint8(0xFF)
+    pay[:, -1] |= np.uint8(0x55 & ~((1 << (2 * (n - 4 * (nb - 1)))) - 1) & 0xFF) if n % 4 else np.uint8(0)
+    pay[17, :] = 0xFF  # every sample hom A2
+    pay = pay.reshape(-1)
+    chrom = np.repeat(np.arange(1, len(per_chr) + 1), per_chr)
+    pos = np.concatenate([np.cumsum(rng.integers(1, 400, c)) for c in per_chr]).astype(np.int64)
+    pa.write_plink(tmp / name, pay, n, m, chrom, pos)
+
+    def run(*args):
+        subprocess.run([str(REF), *args, "-n", "1", "-v", "0"], check=True, cwd=tmp, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+
+    run("-b", name, "--svd", "1", "-k", "3", "-p", "4", "--tol-rsvd", "0", "--ld", "-V", "-o", name + "_r0")
+    raw = np.fromfile(tmp / (name + "_r0.residuals"), dtype=np.uint32, count=2)
+    resid0 = np.fromfile(tmp / (name + "_r0.residuals"), dtype=np.float32, offset=8).reshape(int(raw[0]), int(raw[1]))
+    res = dict(payload=pay, nsamples=n, nsnps=m, pos=pos, chrom=chrom, ld_bp=ld_bp, resid0=resid0,
+               U=np.loadtxt(tmp / (name + "_r0.eigvecs"), ndmin=2),
+               S=np.loadtxt(tmp / (name + "_r0.sigvals"), ndmin=1, skiprows=1),
+               V=np.loadtxt(tmp / (name + "_r0.loadings"), ndmin=2),
+               bim=(tmp / (name + ".bim")).read_text(), mbim=(tmp / (name + "_r0.mbim")).read_text())
+    r0 = name + "_r0.residuals"
+    run("-B", r0, "--match-bim", name + "_r0.mbim", "--ld-r2", "0.2", "--ld-bp", str(ld_bp), "-o", name + "_pm")
+    run("-B", r0, "--match-bim", name + ".bim", "--ld-r2", "0.2", "--ld-bp", str(ld_bp), "-o", name + "_pb")
+    run("-b", name, "--USV", name + "_r0", "--ld-r2", "0.2", "--ld-bp", str(ld_bp), "-o", name + "_pp")
+    for tag in ("pm", "pb", "pp"):
+        for kind in ("in", "out"):
+            res[f"prune_{tag}_{kind}"] = (tmp / f"{name}_{tag}.ld.prune.{kind}").read_text()
+    keep = rng.random(m) > 0.1
+    pv = rng.permutation(m) + 1.0
+    pv = pv / (m + 1.0) * 10 ** rng.uniform(-6, 0, m)  # distinct p-values over 6 decades
+    lines = ["CHR\tSNP\tBP\tP"] + [f"{chrom[i]}\trs{i}\t{pos[i]}\t{pv[i]:.6g}" for i in range(m) if keep[i]]
+    (tmp / "assoc.txt").write_text("\n".join(lines) + "\n")
+    run("-B", r0, "--match-bim", name + "_r0.mbim", "--clump", "assoc.txt", "--clump-p1", "0.001", "--clump-p2",
+        "0.05", "--clump-r2", "0.1", "--clump-bp", "2000", "-o", name + "_cl")
+    res["assoc"] = (tmp / "assoc.txt").read_text()
+    res["clump"] = (tmp / (name + "_cl.p0.clump")).read_text()
+    return res
+
+
 def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
     cmd = [str(REF), flag, prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v", "0"]
     subprocess.run(cmd, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
@@ -265,6 +329,12 @@ def main():
             res = run_ld_usv(tmp, name, seed, n, per_chr, ld_bp, k)
             np.savez_compressed(OUT / f"{name}.npz", **res)
             print(name, len(res["r2"]), "pairs")
+        for name, (seed, n, per_chr, ld_bp) in LD_TOOLS.items():
+            if not want(name):
+                continue
+            res = run_ld_tools(tmp, name, seed, n, per_chr, ld_bp)
+            np.savez_compressed(OUT / f"{name}.npz", **res)
+            print(name, res["clump"].count("\n") - 1, "clumps,", res["prune_pm_out"].count("\n"), "pruned")
         for name, (seed, n, per_chr, ld_bp) in LD_STATS1.items():
             if not want(name):
                 continue

@@ -213,3 +213,94 @@ def test_ld_geno_errors(gpu):
         gpu.ld_r2_geno(np.array([3, 2]), np.array([2, 2]))  # leads must increase
     with pytest.raises(pa.GpuError):
         gpu.ld_r2_geno(np.array([0]), np.array([3]), config=99)  # no such tile shape
+
+
+# ---- producers and consumers around the LD path (ld_tools_bp3000) ----------
+def _ld_tools():
+    return load_golden(GOLDEN / "ld_tools_bp3000.npz")
+
+
+def _read_residuals(path):
+    raw = np.fromfile(path, dtype=np.uint32, count=2)
+    return raw, np.fromfile(path, dtype=np.float32, offset=8).reshape(int(raw[0]), int(raw[1]))
+
+
+def test_write_residuals_matches_reference_cli(gpu, tmp_path):
+    """--ld after a PLINK PCA (ld-stats 0: G - U S V^T, re-centred) with the
+    U, S, V the reference printed (6 significant digits), and ld-stats 1 (the
+    centred genotypes) against the reference's file of the ld_stats1 fixture."""
+    z = _ld_tools()
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    gpu.load_bed(z["payload"], n, m)
+    gpu.write_residuals(tmp_path / "r0.residuals", z["U"], z["S"], z["V"], ld_stats=0)
+    hdr, R = _read_residuals(tmp_path / "r0.residuals")
+    assert list(hdr) == [m, n] and R.shape == z["resid0"].shape
+    assert np.abs(R - z["resid0"]).max() < 2e-5  # the printed U, S, V carry 6 digits
+    s = load_golden(GOLDEN / "ld_stats1_bp3000.npz")
+    n1, m1 = int(s["nsamples"]), int(s["nsnps"])
+    gpu.load_bed(s["payload"], n1, m1)
+    gpu.write_residuals(tmp_path / "r1.residuals", ld_stats=1)
+    hdr1, R1 = _read_residuals(tmp_path / "r1.residuals")
+    assert list(hdr1) == [m1, n1]
+    assert np.abs(R1 - s["resid"]).max() <= 2.0 ** -23  # float32 rounding of the same FP64 values
+
+
+def test_write_mbim_matches_reference_cli(gpu, tmp_path):
+    z = _ld_tools()
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    gpu.load_bed(z["payload"], n, m)
+    F, _ = gpu.allele_freq()
+    (tmp_path / "x.bim").write_text(str(z["bim"]))
+    pa.write_mbim(tmp_path / "x.mbim", tmp_path / "x.bim", F)
+    assert (tmp_path / "x.mbim").read_text() == str(z["mbim"])
+
+
+@pytest.mark.parametrize("tag", ["pm", "pb", "pp"])
+def test_ld_prune_matches_reference_cli(gpu, tmp_path, tag):
+    """--ld-r2 0.2 on the reference's ld-stats 0 residuals with its .mbim (the
+    lower-MAF SNP of a pair goes), with the plain .bim (the partner goes), and
+    on the .bed with --USV of the same PCA (its .mbim): .ld.prune.in / .out equal."""
+    z = _ld_tools()
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    (tmp_path / "x.bim").write_text(str(z["bim"]))
+    (tmp_path / "x.mbim").write_text(str(z["mbim"]))
+    if tag == "pp":  # --USV prefix: the SNP table is prefix.mbim (src/Cmd.cpp:162)
+        gpu.load_bed(z["payload"], n, m)
+        gpu.ld_usv(O.read_usv(z["U"]))
+        bim = tmp_path / "x.mbim"
+    else:
+        gpu.load_dense(z["resid0"])
+        bim = tmp_path / ("x.mbim" if tag == "pm" else "x.bim")
+    gpu.ld_prune(bim, int(z["ld_bp"]), 0.2, tmp_path / "out")
+    assert (tmp_path / "out.ld.prune.in").read_text() == str(z[f"prune_{tag}_in"])
+    assert (tmp_path / "out.ld.prune.out").read_text() == str(z[f"prune_{tag}_out"])
+
+
+def test_ld_clump_matches_reference_cli(gpu, tmp_path):
+    """--clump (p1 1e-3, p2 0.05, r2 0.1, 2 kb) of an assoc-like file with
+    distinct p-values and some SNPs absent, on the ld-stats 0 residuals."""
+    z = _ld_tools()
+    (tmp_path / "x.mbim").write_text(str(z["mbim"]))
+    (tmp_path / "assoc.txt").write_text(str(z["assoc"]))
+    gpu.load_dense(z["resid0"])
+    gpu.ld_clump(tmp_path / "assoc.txt", tmp_path / "x.mbim", tmp_path / "x.clump", clump_bp=2000, clump_r2=0.1,
+                 clump_p1=0.001, clump_p2=0.05)
+    assert (tmp_path / "x.clump").read_text() == str(z["clump"])
+
+
+def test_ld_tools_errors(gpu, tmp_path):
+    z = _ld_tools()
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    (tmp_path / "short.bim").write_text("1\trs0\t0\t10\tA\tG\n")
+    gpu.load_dense(z["resid0"])
+    with pytest.raises(pa.GpuError):  # .bim length differs from the LD matrix
+        gpu.ld_prune(tmp_path / "short.bim", 3000, 0.2, tmp_path / "o")
+    (tmp_path / "a.txt").write_text("CHR\tBP\tPVAL\n1\t10\t0.1\n")
+    (tmp_path / "x.mbim").write_text(str(z["mbim"]))
+    with pytest.raises(pa.GpuError):  # no P column
+        gpu.ld_clump(tmp_path / "a.txt", tmp_path / "x.mbim", tmp_path / "o.clump")
+    with pytest.raises(pa.GpuError):  # residuals need PLINK genotypes
+        gpu.write_residuals(tmp_path / "r.residuals", ld_stats=1)
+    gpu.load_bed(z["payload"], n, m)
+    with pytest.raises(pa.GpuError):  # ld-stats 0 needs U, S, V
+        gpu._check(gpu._L.pcaone_gpu_write_residuals(gpu.h, str(tmp_path / "r").encode(), None, None, None, 0, 0))
