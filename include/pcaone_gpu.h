@@ -165,6 +165,24 @@ int pcaone_gpu_pcangsd_update(pcaone_gpu* h, const double* U, const double* S, c
 int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
                        double* V, int* iters, double* diffs, double* Dc, double* C);
 
+/* FileCsv::read_all (src/FileCsv.cpp:11-62, 82-142): a CSV of counts, one
+ * feature per line and one sample per comma-separated field, zstd-compressed
+ * (the reference's input) or plain text, parsed on nthreads host threads.
+ * --scale on the device: 2 log10(count * median(libsize) / libsize + 1),
+ * 3 log1p(count / libsize * scale_factor), 4 count / libsize * scale_factor,
+ * 1 the counts; scales 1-4 then standardise each feature over samples (N - 1,
+ * sd > 1e-10); 0 leaves the counts as they are, uncentred, as the reference's
+ * reader does.  libsize: per sample, the sum of the counts read as integers
+ * (std::stoi).  The matrix then serves as for pcaone_gpu_load_dense_f32;
+ * *nsamples and *nfeatures (optional) receive its size. */
+int pcaone_gpu_load_csv(pcaone_gpu* h, const char* path, int scale, double scale_factor, int nthreads,
+                        uint32_t* nsamples, uint32_t* nfeatures);
+/* The reader of pcaone_gpu_load_csv alone, host only: the counts as read
+ * (feature-major float32, capacity in floats; values NULL: sizes only) and
+ * each sample's integer library size (libsize, nsamples, optional). */
+int pcaone_read_csv(const char* path, int nthreads, float* values, uint64_t capacity, uint32_t* nsamples,
+                    uint64_t* nfeatures, int64_t* libsize);
+
 /* ---- LD inputs and consumers ------------------------------------------- */
 /* Data::write_residuals (src/Data.cpp:196-245) for the loaded PLINK genotypes
  * (the --ld route of a PLINK PCA, which runs unstandardised): the centred

@@ -109,6 +109,38 @@ def dense_centered(featmajor: np.ndarray) -> np.ndarray:
     return X
 
 
+def csv_matrix(counts: np.ndarray, scale: int, scale_factor: float = 1.0) -> np.ndarray:
+    """FileCsv::read_all with parse_csvzstd (src/FileCsv.cpp:11-62, 82-142) on a
+    counts matrix (features x samples, the CSV's lines x fields): library size
+    libsize_j = sum over features of the integer counts, median the integer
+    midpoint of the sorted sizes (get_median on ints, src/Utils.hpp:60-75);
+    scale 2 log10(c * median / libsize + 1), 3 log1p(c / libsize * factor),
+    4 c / libsize * factor, 0/1 c; scale > 0 then standardises each feature
+    (standardize, src/Utils.cpp:34-43: minus its mean, over its norm / sqrt(N-1)
+    when that exceeds 1e-10).  Returns X (N x M), samples by features.
+    (For scales 3 and 4 the reference divides by a library size it never
+    fills; this restatement uses the filled one, as the library does.)"""
+    c = np.asarray(counts, dtype=np.float64)
+    n = c.shape[1]
+    lib = np.asarray(counts, dtype=np.int64).sum(axis=0)
+    v = np.sort(lib)
+    med = float((v[n // 2 - 1] + v[n // 2]) // 2 if n % 2 == 0 else v[n // 2])
+    if scale == 2:
+        X = np.log10(c * med / lib + 1.0)
+    elif scale == 3:
+        X = np.log1p(c / lib * scale_factor)
+    elif scale == 4:
+        X = c / lib * scale_factor
+    else:
+        X = c.copy()
+    if scale > 0:
+        X = X - X.mean(axis=1, keepdims=True)
+        sd = np.sqrt((X * X).sum(axis=1)) / np.sqrt(n - 1.0)
+        ok = sd > 1e-10
+        X[ok] /= sd[ok, None]
+    return X.T.copy()
+
+
 def standardize(X: np.ndarray, F: np.ndarray, ploidy: int = 2) -> np.ndarray:
     """Divide column i by sqrt(ploidy*F_i*(1-F_i)) when above VAR_TOL
     (src/Data.cpp:290-299).  Evaluation order (ploidy*F)*(1-F) as in C++."""

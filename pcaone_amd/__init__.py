@@ -66,6 +66,10 @@ SIGNATURES = {
                                                 C.POINTER(C.c_uint64), _I32, C.c_int]),
     "pcaone_write_eigs": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, _D, _D, _D, C.c_int]),
     "pcaone_gpu_grm_eig": (C.c_int, [C.c_void_p, _D, _D]),
+    "pcaone_gpu_load_csv": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_int,
+                                      C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "pcaone_read_csv": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]),
     "pcaone_gpu_write_residuals": (C.c_int, [C.c_void_p, C.c_char_p, _D, _D, _D, C.c_uint32, C.c_int]),
     "pcaone_write_mbim": (C.c_int, [C.c_char_p, C.c_char_p, _D, C.c_uint32]),
     "pcaone_gpu_ld_prune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_char_p]),
@@ -288,6 +292,21 @@ def write_ld_gz(path: str | os.PathLike, bim: str | os.PathLike, ws: np.ndarray,
         raise GpuError(rc, "pcaone_write_ld_gz failed")
 
 
+def read_csv(path: str | os.PathLike, nthreads: int = 4):
+    """The library's CSV reader alone: (counts (features, samples) float32, libsize int64)."""
+    n, m = C.c_uint32(0), C.c_uint64(0)
+    rc = lib().pcaone_read_csv(str(path).encode(), nthreads, None, 0, C.byref(n), C.byref(m), None)
+    if rc != 0:
+        raise GpuError(rc, "pcaone_read_csv failed")
+    X = np.empty((m.value, n.value), dtype=np.float32)
+    ls = np.empty(n.value, dtype=np.int64)
+    rc = lib().pcaone_read_csv(str(path).encode(), nthreads, _p(X, C.c_float), X.size, C.byref(n), C.byref(m),
+                               _p(ls, C.c_int64))
+    if rc != 0:
+        raise GpuError(rc, "pcaone_read_csv failed")
+    return X, ls
+
+
 def write_mbim(path: str | os.PathLike, bim: str | os.PathLike, F: np.ndarray):
     """The reference's .mbim: every .bim line with its allele frequency appended."""
     F = np.ascontiguousarray(F, dtype=np.float64)
@@ -425,6 +444,13 @@ class Gpu:
                                                 _p(diffs, C.c_double), _p(Dc, C.c_double) if dc else None,
                                                 _p(Cm, C.c_double) if grm else None))
         return U, S, V, it.value, diffs[:it.value], Dc, Cm
+
+    def load_csv(self, path, scale: int = 2, scale_factor: float = 1.0, nthreads: int = 16):
+        """A (zstd-compressed) CSV of counts, one feature per line, --scale on the device."""
+        n, m = C.c_uint32(0), C.c_uint32(0)
+        self._check(self._L.pcaone_gpu_load_csv(self.h, str(path).encode(), int(scale), float(scale_factor),
+                                                 int(nthreads), C.byref(n), C.byref(m)))
+        self.N, self.M = n.value, m.value
 
     def write_residuals(self, path, U=None, S=None, V=None, ld_stats: int = 0):
         """The reference's .residuals of the loaded PLINK genotypes (ld_stats 0:

@@ -281,6 +281,26 @@ int pcaone_write_eigs(const char* prefix, uint32_t nsamples, uint32_t nsnps, int
   return PCAONE_OK;
 }
 
+int pcaone_gpu_load_csv(pcaone_gpu* h, const char* path, int scale, double scale_factor, int nthreads,
+                        uint32_t* nsamples, uint32_t* nfeatures) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    e.load_csv(path, scale, scale_factor, nthreads);
+    e.loaded_dims(nsamples, nfeatures);
+  });
+}
+
+int pcaone_read_csv(const char* path, int nthreads, float* values, uint64_t capacity, uint32_t* nsamples,
+                    uint64_t* nfeatures, int64_t* libsize) {
+  try {
+    pcaone::read_csv_host(path, nthreads, values, capacity, nsamples, nfeatures, libsize);
+    return PCAONE_OK;
+  } catch (const pcaone::Error& e) {
+    return e.code < 0 ? e.code : PCAONE_EINVAL;
+  } catch (...) {
+    return PCAONE_EINVAL;
+  }
+}
+
 int pcaone_gpu_write_residuals(pcaone_gpu* h, const char* fileout, const double* U, const double* S,
                                const double* V, uint32_t k, int ld_stats) {
   return guarded(h, [&](pcaone::Engine& e) { e.write_residuals(fileout, U, S, V, k, ld_stats); });

@@ -138,11 +138,19 @@ void Engine::load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfea
   M_ = nfeatures;
   rawf_.reserve((uint64_t)N_ * M_);
   PCAONE_HIP(hipMemcpyAsync(rawf_.p, featmajor, (uint64_t)N_ * M_ * 4, hipMemcpyHostToDevice, stream_));
-  DevBuf<double> mu;
-  mu.reserve(M_);
-  timed("dense_mean", [&] { launch_dense_mean(rawf_.p, N_, M_, mu.p, stream_); });
+  load_dense_finish(true);
+}
+
+// The features in rawf_ become the operator's matrix: with `centre`, each
+// feature's FP64 mean is what the epilogues subtract; without, nothing is.
+void Engine::load_dense_finish(bool centre) {
   mu_orig_.assign(M_, 0.0);
-  PCAONE_HIP(hipMemcpyAsync(mu_orig_.data(), mu.p, (size_t)M_ * 8, hipMemcpyDeviceToHost, stream_));
+  if (centre) {
+    DevBuf<double> mu;
+    mu.reserve(M_);
+    timed("dense_mean", [&] { launch_dense_mean(rawf_.p, N_, M_, mu.p, stream_); });
+    PCAONE_HIP(hipMemcpyAsync(mu_orig_.data(), mu.p, (size_t)M_ * 8, hipMemcpyDeviceToHost, stream_));
+  }
   sync();
   // the epilogues subtract 2 F_i colsum, so F carries mu / 2
   F_orig_.resize(M_);

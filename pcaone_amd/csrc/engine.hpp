@@ -80,6 +80,9 @@ uint64_t window_offsets(const int32_t* ws, const int32_t* we, uint32_t nwin, uin
 // band tiles (row block of tm SNPs, column block of tn SNPs) of the windows
 // .mbim: the .bim lines with each SNP's allele frequency appended (ld_tools.hip)
 void write_mbim(const char* fileout, const char* filebim, const double* F, uint32_t nsnps);
+// the CSV reader alone (csv.hip): counts feature-major, library sizes per sample
+void read_csv_host(const char* path, int nthreads, float* values, uint64_t capacity, uint32_t* nsamples,
+                   uint64_t* nfeatures, int64_t* libsize);
 // rocBLAS handle teardown of grm_eig (grm_eig.cpp); null is a no-op
 void destroy_blas(void* handle);
 // the .ld.gz text of the window pairs (ld_write.cpp)
@@ -102,6 +105,8 @@ class Engine {
 
   void load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps);
   void load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfeatures);
+  // FileCsv::read_all: a (zstd-compressed) CSV of counts, --scale on the device
+  void load_csv(const char* path, int scale, double scale_factor, int nthreads);
   // genotype likelihoods (PCAngsd): the E matrix is built in HBM from them
   void load_gl(const float* gl, uint32_t nsamples, uint32_t nsites, int per_site);
   // PCAngsd on PLINK input: the loaded genotypes become hard-call likelihoods
@@ -137,6 +142,11 @@ class Engine {
                               int32_t* sums, int config);
   void allele_freq(double* F, int64_t* counts) const;
   void feature_means(double* mu) const;
+  void load_dense_finish(bool centre);
+  void loaded_dims(uint32_t* nsamples, uint32_t* nfeatures) const {
+    if (nsamples) *nsamples = N_;
+    if (nfeatures) *nfeatures = M_;
+  }
 
   void rsvd_init(const RsvdParams& p, const double* omega, const int32_t* perm);
   // computeGandH; `check` reads the CholeskyQR breakdown flag back when an

@@ -44,6 +44,11 @@ def oracle_run(z: dict):
         P0, P1 = gl_planes(z["gl"])
         orc, *_ = O.pcangsd_halko(P0, P1, d["k"], d["os"], d["p"], d["tol"], seed=d["seed"], rand=d["rand"])
         return orc, None, d
+    if "counts" in z:  # CSV of counts: the reader's --scale transform, no further centring
+        X = O.csv_matrix(z["counts"], d["scale"])
+        orc = O.RsvdOracle(X, d["k"], d["os"], fancy=d["svd"] == 2, bands=d["w"], seed=d["seed"], rand=d["rand"],
+                           perm=not d["S"])
+        return orc, orc.compute_usv(d["p"], d["tol"]), d
     if "dense" in z:
         X = O.dense_centered(z["dense"])
         if d["svd"] == 0:  # IRAM: the converged top-k singular triplets
@@ -92,3 +97,23 @@ def geno_sums_fast(codes, ws, we):
         k = slice(i + 1, i + e)
         out.append(np.stack([y[k] @ y[i], w[k] @ y[i], y[k] @ w[i], w[k] @ w[i]], axis=1))
     return np.concatenate(out).astype(np.int64) if out else np.zeros((0, 4), dtype=np.int64)
+
+
+def csv_bytes(counts: np.ndarray, zstd: bool = True) -> bytes:
+    """The counts as the reference's CSV input: one feature per line, the
+    samples comma-separated, zstd-compressed (the system's libzstd) or plain."""
+    text = "".join(",".join(str(int(v)) for v in row) + "\n" for row in np.asarray(counts)).encode()
+    if not zstd:
+        return text
+    import ctypes
+
+    z = ctypes.CDLL("libzstd.so.1")
+    z.ZSTD_compressBound.restype = ctypes.c_size_t
+    z.ZSTD_compress.restype = ctypes.c_size_t
+    z.ZSTD_isError.restype = ctypes.c_uint
+    cap = z.ZSTD_compressBound(ctypes.c_size_t(len(text)))
+    dst = ctypes.create_string_buffer(cap)
+    n = z.ZSTD_compress(dst, ctypes.c_size_t(cap), text, ctypes.c_size_t(len(text)), 3)
+    assert not z.ZSTD_isError(ctypes.c_size_t(n))
+    return dst.raw[:n]
+

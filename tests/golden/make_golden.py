@@ -196,6 +196,42 @@ def run_ld_stats1(tmp: Path, name: str, seed, n, per_chr, ld_bp):
                 snp_b=ib, r2=r2, ldgz=ldgz, mbim=mbim)
 
 
+# CSV of counts (--csv, zstd-compressed): name -> (seed, ncells, ngenes)
+CSV = {"c1": (41, 260, 380)}
+CSV_CASES = {
+    "csv_svd1_scale2": ("c1", "--svd 1 -k 4 --oversamples 6 -p 12 --tol-rsvd 0 --scale 2"),
+    "csv_svd1_scale1": ("c1", "--svd 1 -k 4 --oversamples 6 -p 12 --tol-rsvd 0 --scale 1"),
+}
+
+
+def zstd_compress(data: bytes) -> bytes:
+    """One zstd frame (the system's libzstd, level 3)."""
+    import ctypes
+
+    z = ctypes.CDLL("libzstd.so.1")
+    z.ZSTD_compressBound.restype = ctypes.c_size_t
+    z.ZSTD_compress.restype = ctypes.c_size_t
+    z.ZSTD_isError.restype = ctypes.c_uint
+    cap = z.ZSTD_compressBound(ctypes.c_size_t(len(data)))
+    dst = ctypes.create_string_buffer(cap)
+    n = z.ZSTD_compress(dst, ctypes.c_size_t(cap), data, ctypes.c_size_t(len(data)), 3)
+    assert not z.ZSTD_isError(ctypes.c_size_t(n))
+    return dst.raw[:n]
+
+
+def csv_counts(seed, ncells, ngenes):
+    """Integer counts, genes x cells: negative binomial (gamma-Poisson) around
+    per-gene means, with 3 cell groups shifting a third of the genes."""
+    rng = np.random.default_rng(seed)
+    mu = np.exp(rng.normal(0.0, 1.2, ngenes)) * 0.8
+    grp = rng.integers(0, 3, ncells)
+    shift = np.ones((ngenes, 3))
+    shift[rng.random(ngenes) < 0.33, :] = np.exp(rng.normal(0, 0.8, (1, 3)))
+    lib = np.exp(rng.normal(0.0, 0.4, ncells))
+    lam = mu[:, None] * shift[:, grp] * lib[None, :]
+    return rng.poisson(rng.gamma(2.0, lam / 2.0)).astype(np.int64)
+
+
 LD_TOOLS = {"ld_tools_bp3000": (71, 157, (230, 170), 3000)}
 
 
@@ -329,6 +365,17 @@ def main():
             res = run_ld_usv(tmp, name, seed, n, per_chr, ld_bp, k)
             np.savez_compressed(OUT / f"{name}.npz", **res)
             print(name, len(res["r2"]), "pairs")
+        for name, (seed, ncells, ngenes) in CSV.items():
+            counts = csv_counts(seed, ncells, ngenes)
+            text = "".join(",".join(str(int(v)) for v in row) + "\n" for row in counts)
+            (tmp / (name + ".csv.zst")).write_bytes(zstd_compress(text.encode()))
+            for case, (ds, args) in CSV_CASES.items():
+                if ds != name or not want(case):
+                    continue
+                res = run_case(tmp, name + ".csv.zst", args, case, flag="--csv")
+                np.savez_compressed(OUT / f"{case}.npz", counts=counts, nsamples=ncells, nsnps=ngenes, args=args,
+                                    **res)
+                print(case, res["eigvals"])
         for name, (seed, n, per_chr, ld_bp) in LD_TOOLS.items():
             if not want(name):
                 continue

@@ -402,3 +402,33 @@ def test_ld_geno_sums_brute_force():
         ref = O.ld_r2(O.ld_stats1_matrix(codes), ws, we)
         ok = ~np.isnan(ref)
         assert np.array_equal(np.isnan(r2), np.isnan(ref)) and np.abs(r2[ok] - ref[ok]).max() < 1e-12
+
+
+def test_csv_planted_mistakes():
+    """The CSV reader's transform: a float median, a missing median factor or
+    no standardisation each move the reference's printed eigenvalues (the
+    base of the log does not: standardisation removes it)."""
+    z = load_golden(ROOT / "tests" / "golden" / "csv_svd1_scale2.npz")
+    d = parse_args(z["args"])
+    c = z["counts"].astype(np.float64)
+    n = c.shape[1]
+    lib = c.sum(axis=0)
+
+    def run(X):
+        orc = O.RsvdOracle(X, d["k"], d["os"], fancy=False, seed=d["seed"])
+        orc.compute_usv(d["p"], d["tol"])
+        return orc.eigvals()
+
+    def std(X):
+        X = X - X.mean(axis=1, keepdims=True)
+        return (X / (np.sqrt((X * X).sum(axis=1)) / np.sqrt(n - 1.0))[:, None]).T.copy()
+
+    assert eig_relerr(run(O.csv_matrix(z["counts"], 2)), z["eigvals"]) < 5e-6
+    med_f = float(np.median(lib))  # the float midpoint instead of get_median's integer one
+    assert eig_relerr(run(std(np.log10(c * med_f / lib + 1))), z["eigvals"]) > 1e-5
+    med = float(np.sort(lib.astype(np.int64))[n // 2 - 1] + np.sort(lib.astype(np.int64))[n // 2]) // 2
+    assert eig_relerr(run(std(np.log(c * med / lib + 1))), z["eigvals"]) < 5e-6
+    assert eig_relerr(run(std(np.log10(c / lib + 1))), z["eigvals"]) > 1e-3
+    X = np.log10(c * med / lib + 1)
+    assert eig_relerr(run((X - X.mean(axis=1, keepdims=True)).T.copy()), z["eigvals"]) > 1e-3
+
