@@ -183,6 +183,23 @@ int pcaone_gpu_load_csv(pcaone_gpu* h, const char* path, int scale, double scale
 int pcaone_read_csv(const char* path, int nthreads, float* values, uint64_t capacity, uint32_t* nsamples,
                     uint64_t* nfeatures, int64_t* libsize);
 
+/* ---- inbreeding / HWE (--inbreed 1 with --USV) ---------------------------- */
+/* run_inbreeding_em (src/Inbreeding.cpp:16-190) on the loaded PLINK hard
+ * calls: Pi = clamp((U S V^T + 2F) / 2, 1e-4, 1 - 1e-4) (src/FileUSV.cpp:11-22)
+ * from a PCA's U (nsamples x k), S (k), V (nsnps x k, column-major; 1 <= k <=
+ * 64) and the .mbim frequencies F (nsnps); the per-site inbreeding
+ * coefficients by the SQUAREM-accelerated EM (maxiter, tolem as --maxiter,
+ * --tol-em), then the per-site LRT and its chi-square(1) p-value.  Outputs
+ * (nsnps each, optional): coef, lrt, hwe_p; *iters the EM iterations run. */
+int pcaone_gpu_inbreeding(pcaone_gpu* h, const double* U, const double* S, const double* V, uint32_t k,
+                          const double* F, int maxiter, double tolem, double* coef, double* lrt, double* hwe_p,
+                          int* iters);
+/* write_hwe_per_site (src/Inbreeding.cpp:129-144), host only: "#ID\tHWE_P\tLRT\t
+ * Inbreeding_coefficient", then per .bim line its second field and the three
+ * values (6 significant digits). */
+int pcaone_write_hwe(const char* fileout, const char* filebim, const double* hwe_p, const double* lrt,
+                     const double* coef, uint32_t nsnps);
+
 /* ---- LD inputs and consumers ------------------------------------------- */
 /* Data::write_residuals (src/Data.cpp:196-245) for the loaded PLINK genotypes
  * (the --ld route of a PLINK PCA, which runs unstandardised): the centred

@@ -763,3 +763,69 @@ def principal_angles_max(A: np.ndarray, B: np.ndarray) -> float:
     qb, _ = np.linalg.qr(B)
     s = np.linalg.svd(qa.T @ qb, compute_uv=False)
     return float(np.arccos(np.clip(s.min(), -1.0, 1.0)))
+
+
+def inbreeding_em(codes: np.ndarray, U, S, V, F, maxiter: int = 100, tolem: float = 1e-5):
+    """run_inbreeding_em with type 1 (src/Inbreeding.cpp:16-190) on hard calls
+    (codes M x N as decode_codes returns them, 0..3 as .bed codes): Pi = clamp((U S V^T + 2F) / 2, 1e-4,
+    1 - 1e-4) (src/FileUSV.cpp:11-22); per site the EM update of the
+    inbreeding coefficient from observed / expected heterozygosity, with the
+    SQUAREM step (alpha clamped to [1, 256], the coefficients to [-1, 1], the
+    stabilising pass, RMSE < tolem) and AreClose(sv2, 0); then the LRT
+    2 (log L_alt - log L_null) and its chi-square(1) p-value (kf_gammaq(1/2,
+    x/2) = erfc(sqrt(x / 2)); NaN -> 1).  Returns (coef, lrt, p, iterations)."""
+    from math import erfc, isnan, sqrt
+
+    PI = np.clip((U @ np.diag(S) @ V.T + 2.0 * F[None, :]) * 0.5, 1e-4, 1.0 - 1e-4)
+    c = np.asarray(codes).T  # samples x sites, as Pi
+
+    def priors(fc, lrt):
+        fadj = (1.0 - PI) * PI * fc[None, :]
+        p0 = np.maximum(1e-4, (1.0 - PI) ** 2 + fadj)
+        if lrt:
+            p1 = np.maximum(1e-4, 2.0 * PI * (1.0 - PI) - 2.0 * fadj)
+        else:
+            p1 = np.maximum(1e-4, 2.0 * PI * (1.0 - PI) * (1.0 - fc[None, :]))
+        p2 = np.maximum(1e-4, PI * PI + fadj)
+        ps = 1.0 / (p0 + p1 + p2)
+        return p0 * ps, p1 * ps, p2 * ps
+
+    def coef_pass(fc):
+        p0, p1, p2 = priors(fc, False)
+        q0 = np.where(c == 3, p0, np.where(c == 1, 0.333333 * p0, 0.0))
+        q1 = np.where(c == 2, p1, np.where(c == 1, 0.333333 * p1, 0.0))
+        q2 = np.where(c == 0, p2, np.where(c == 1, 0.333333 * p2, 0.0))
+        obs = (q1 / (q0 + q1 + q2)).sum(axis=0)
+        exp = (2.0 * PI * (1.0 - PI)).sum(axis=0)
+        f = np.clip(1.0 - obs / exp, -1.0, 1.0)
+        return f, f - fc
+
+    def close0(a):
+        return a == 0.0
+
+    Fc = np.zeros(V.shape[0])
+    it = 0
+    for it in range(maxiter):
+        F0 = Fc.copy()
+        Fc, D1 = coef_pass(Fc)
+        sr2 = float((D1 * D1).sum())
+        Fc, D2 = coef_pass(Fc)
+        sv2 = float(((D2 - D1) ** 2).sum())
+        if close0(sv2):
+            break
+        alpha = min(max(1.0, sqrt(sr2 / sv2)), 256.0)
+        Fc = np.clip(F0 + 2 * alpha * D1 + alpha * alpha * (D2 - D1), -1.0, 1.0)
+        Fc, _ = coef_pass(Fc)
+        if sqrt(float(((F0 - Fc) ** 2).sum()) / len(Fc)) < tolem:
+            break
+    p0, p1, p2 = priors(Fc, True)
+    la = np.where(c == 3, np.log(p0), np.where(c == 2, np.log(p1), np.where(c == 0, np.log(p2),
+                                                                                    np.log(0.333333 * (p0 + p1 + p2)))))
+    q = 1.0 - PI
+    ln = np.where(c == 3, np.log(q * q), np.where(c == 2, np.log(2.0 * PI * q), np.where(
+        c == 0, np.log(PI * PI), np.log(0.333333 * (q * q + 2.0 * PI * q + PI * PI)))))
+    T = 2.0 * (la.sum(axis=0) - ln.sum(axis=0))
+    p = np.array([1.0 if (t != t or t < 0) else erfc(sqrt(t / 2.0)) for t in T])
+    p = np.array([1.0 if isnan(x) else x for x in p])
+    return Fc, T, p, it + 1
+

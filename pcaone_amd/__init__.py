@@ -70,6 +70,9 @@ SIGNATURES = {
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "pcaone_read_csv": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32),
                                   C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]),
+    "pcaone_gpu_inbreeding": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_uint32, _D, C.c_int, C.c_double, _D, _D, _D,
+                                        C.POINTER(C.c_int)]),
+    "pcaone_write_hwe": (C.c_int, [C.c_char_p, C.c_char_p, _D, _D, _D, C.c_uint32]),
     "pcaone_gpu_write_residuals": (C.c_int, [C.c_void_p, C.c_char_p, _D, _D, _D, C.c_uint32, C.c_int]),
     "pcaone_write_mbim": (C.c_int, [C.c_char_p, C.c_char_p, _D, C.c_uint32]),
     "pcaone_gpu_ld_prune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_char_p]),
@@ -307,6 +310,17 @@ def read_csv(path: str | os.PathLike, nthreads: int = 4):
     return X, ls
 
 
+def write_hwe(path: str | os.PathLike, bim: str | os.PathLike, hwe_p, lrt, coef):
+    """The reference's .hwe table (ID, HWE_P, LRT, Inbreeding_coefficient)."""
+    h = np.ascontiguousarray(hwe_p, dtype=np.float64)
+    t = np.ascontiguousarray(lrt, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    rc = lib().pcaone_write_hwe(str(path).encode(), str(bim).encode(), _p(h, C.c_double), _p(t, C.c_double),
+                                _p(c, C.c_double), len(c))
+    if rc != 0:
+        raise GpuError(rc, "pcaone_write_hwe failed")
+
+
 def write_mbim(path: str | os.PathLike, bim: str | os.PathLike, F: np.ndarray):
     """The reference's .mbim: every .bim line with its allele frequency appended."""
     F = np.ascontiguousarray(F, dtype=np.float64)
@@ -451,6 +465,21 @@ class Gpu:
         self._check(self._L.pcaone_gpu_load_csv(self.h, str(path).encode(), int(scale), float(scale_factor),
                                                  int(nthreads), C.byref(n), C.byref(m)))
         self.N, self.M = n.value, m.value
+
+    def inbreeding(self, U, S, V, F, maxiter: int = 100, tolem: float = 1e-5):
+        """--inbreed 1 on the loaded PLINK hard calls: (coef, lrt, hwe_p, iterations)."""
+        U = np.asfortranarray(U, dtype=np.float64)
+        V = np.asfortranarray(V, dtype=np.float64)
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        m = V.shape[0]
+        coef, lrt, hwe = np.empty(m), np.empty(m), np.empty(m)
+        it = C.c_int(0)
+        self._check(self._L.pcaone_gpu_inbreeding(self.h, _p(U, C.c_double), _p(S, C.c_double), _p(V, C.c_double),
+                                                   U.shape[1], _p(F, C.c_double), int(maxiter), float(tolem),
+                                                   _p(coef, C.c_double), _p(lrt, C.c_double), _p(hwe, C.c_double),
+                                                   C.byref(it)))
+        return coef, lrt, hwe, it.value
 
     def write_residuals(self, path, U=None, S=None, V=None, ld_stats: int = 0):
         """The reference's .residuals of the loaded PLINK genotypes (ld_stats 0:

@@ -301,6 +301,27 @@ int pcaone_read_csv(const char* path, int nthreads, float* values, uint64_t capa
   }
 }
 
+int pcaone_gpu_inbreeding(pcaone_gpu* h, const double* U, const double* S, const double* V, uint32_t k,
+                          const double* F, int maxiter, double tolem, double* coef, double* lrt, double* hwe_p,
+                          int* iters) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    const int it = e.inbreeding(U, S, V, k, F, maxiter, tolem, coef, lrt, hwe_p);
+    if (iters) *iters = it;
+  });
+}
+
+int pcaone_write_hwe(const char* fileout, const char* filebim, const double* hwe_p, const double* lrt,
+                     const double* coef, uint32_t nsnps) {
+  try {
+    pcaone::write_hwe(fileout, filebim, hwe_p, lrt, coef, nsnps);
+    return PCAONE_OK;
+  } catch (const pcaone::Error& e) {
+    return e.code < 0 ? e.code : PCAONE_EINVAL;
+  } catch (...) {
+    return PCAONE_EINVAL;
+  }
+}
+
 int pcaone_gpu_write_residuals(pcaone_gpu* h, const char* fileout, const double* U, const double* S,
                                const double* V, uint32_t k, int ld_stats) {
   return guarded(h, [&](pcaone::Engine& e) { e.write_residuals(fileout, U, S, V, k, ld_stats); });

@@ -83,6 +83,9 @@ void write_mbim(const char* fileout, const char* filebim, const double* F, uint3
 // the CSV reader alone (csv.hip): counts feature-major, library sizes per sample
 void read_csv_host(const char* path, int nthreads, float* values, uint64_t capacity, uint32_t* nsamples,
                    uint64_t* nfeatures, int64_t* libsize);
+// the .hwe table (inbreed.hip)
+void write_hwe(const char* fileout, const char* filebim, const double* hwe_p, const double* lrt, const double* coef,
+               uint32_t nsnps);
 // rocBLAS handle teardown of grm_eig (grm_eig.cpp); null is a no-op
 void destroy_blas(void* handle);
 // the .ld.gz text of the window pairs (ld_write.cpp)
@@ -122,6 +125,9 @@ class Engine {
   void ld_prune(const char* filebim, int ld_bp, double r2_tol, const char* prefix);
   void ld_clump(const char* fileassoc, const char* colnames, const char* filebim, int clump_bp, double clump_r2,
                 double clump_p1, double clump_p2, const char* fileout);
+  // per-site inbreeding coefficients and HWE LRT on the loaded PLINK hard calls
+  int inbreeding(const double* U, const double* S, const double* V, uint32_t k, const double* F, int maxiter,
+                 double tolem, double* coef, double* lrt, double* hwe_p);
   // eigenpairs of the kept GRM as JacobiSVD(C) gives them (src/Halko.cpp:326-330):
   // |lambda| descending, U2 row-major N x N (row i = sample i)
   void grm_eig(double* evals, double* U2);

@@ -233,6 +233,23 @@ def csv_counts(seed, ncells, ngenes):
 
 
 LD_TOOLS = {"ld_tools_bp3000": (71, 157, (230, 170), 3000)}
+# --inbreed 1 with --USV of a PCA on the same PLINK set: name -> dataset
+INBREED = {"inbreed_d1": "d1"}
+
+
+def run_inbreed(tmp: Path, name: str, ds: str):
+    """A PCA with -V (.eigvecs, .sigvals, .loadings, .mbim), then --USV of it
+    with --inbreed 1 (.hwe)."""
+    def run(*args):
+        subprocess.run([str(REF), *args, "-n", "1", "-v", "0"], check=True, cwd=tmp, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+
+    run("-b", ds, "--svd", "1", "-k", "3", "-p", "5", "--tol-rsvd", "0", "-V", "-o", name + "_pca")
+    run("-b", ds, "--USV", name + "_pca", "--inbreed", "1", "-o", name)
+    return dict(U=np.loadtxt(tmp / (name + "_pca.eigvecs"), ndmin=2),
+                S=np.loadtxt(tmp / (name + "_pca.sigvals"), ndmin=1, skiprows=1),
+                V=np.loadtxt(tmp / (name + "_pca.loadings"), ndmin=2),
+                mbim=(tmp / (name + "_pca.mbim")).read_text(), hwe=(tmp / (name + ".hwe")).read_text())
 
 
 def run_ld_tools(tmp: Path, name: str, seed, n, per_chr, ld_bp):
@@ -365,6 +382,13 @@ def main():
             res = run_ld_usv(tmp, name, seed, n, per_chr, ld_bp, k)
             np.savez_compressed(OUT / f"{name}.npz", **res)
             print(name, len(res["r2"]), "pairs")
+        for name, ds in INBREED.items():
+            if not want(name):
+                continue
+            pay, n, m = payloads[ds]
+            res = run_inbreed(tmp, name, ds)
+            np.savez_compressed(OUT / f"{name}.npz", payload=pay, nsamples=n, nsnps=m, **res)
+            print(name, res["hwe"].count("\n") - 1, "sites")
         for name, (seed, ncells, ngenes) in CSV.items():
             counts = csv_counts(seed, ncells, ngenes)
             text = "".join(",".join(str(int(v)) for v in row) + "\n" for row in counts)
