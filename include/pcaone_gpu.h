@@ -183,6 +183,21 @@ int pcaone_gpu_load_csv(pcaone_gpu* h, const char* path, int scale, double scale
 int pcaone_read_csv(const char* path, int nthreads, float* values, uint64_t capacity, uint32_t* nsamples,
                     uint64_t* nfeatures, int64_t* libsize);
 
+/* ---- projection (--project 1|2 with --USV) -------------------------------- */
+/* run_projection (src/Projection.cpp:147-208) for the loaded PLINK samples:
+ * target .bim lines matched to the reference .mbim (ID, then chromosome and
+ * position; alleles equal, or swapped with F = 1 - freq and the loading row
+ * negated; src/FilePlink.hpp:31-93), g = BED2GENO - F (missing 0), divided by
+ * sqrt(2F(1-F)) when standardize and that exceeds 1e-9.  method 1: U = G V S^-1;
+ * method 2: per sample least squares on its observed SNPs, V S as the design
+ * (normal equations, Cholesky).  S (k), Vref (ref_nsnps x k column-major, the
+ * .loadings), 1 <= k <= 32.  U (nsamples x k column-major); W (optional,
+ * noverlap x k column-major): the matched loadings the reference writes
+ * (V S^-1 or V S); *noverlap the matched SNP count (W needs at most nsnps x k). */
+int pcaone_gpu_project(pcaone_gpu* h, const char* filebim, const char* filembim, const double* S,
+                       const double* Vref, uint32_t k, uint32_t ref_nsnps, int method, int standardize, double* U,
+                       double* W, uint32_t* noverlap);
+
 /* ---- inbreeding / HWE (--inbreed 1 with --USV) ---------------------------- */
 /* run_inbreeding_em (src/Inbreeding.cpp:16-190) on the loaded PLINK hard
  * calls: Pi = clamp((U S V^T + 2F) / 2, 1e-4, 1 - 1e-4) (src/FileUSV.cpp:11-22)

@@ -829,3 +829,54 @@ def inbreeding_em(codes: np.ndarray, U, S, V, F, maxiter: int = 100, tolem: floa
     p = np.array([1.0 if isnan(x) else x for x in p])
     return Fc, T, p, it + 1
 
+
+def project_plink(codes: np.ndarray, bim_lines, mbim_lines, S, V, method: int, standardize: bool = True):
+    """run_projection (src/Projection.cpp:147-208) on PLINK targets with the
+    matching of FileBed's constructor (src/FilePlink.hpp:31-93): target .bim
+    (split at tabs and spaces) against the .mbim (7 tab fields) by ID, then
+    chromosome and position; alleles equal (F = freq) or swapped (F = 1 - freq,
+    the loading row negated).  g = BED2GENO - F (missing 0, masked), divided
+    by sqrt(2F(1-F)) when standardising and above 1e-9.  Method 1: U = G V S^-1;
+    method 2: per sample the least-squares fit of V S on its observed SNPs
+    (numpy lstsq: the reference uses ColPivHouseholderQR).  codes: M x N as
+    decode_codes returns them.  Returns (U, W) with W the written loadings."""
+    ref = {}
+    for idx, line in enumerate(mbim_lines):
+        t = line.split("\t")
+        ref.setdefault(t[1], (idx, t[0], t[3], t[4], t[5], float(t[6])))
+    rows, Fs, Ws = [], [], []
+    S = np.asarray(S, dtype=np.float64)
+    for j, line in enumerate(bim_lines):
+        t = [x for x in __import__("re").split(r"[\t ]", line)]
+        r = ref.get(t[1])
+        if r is None or t[0] != r[1] or t[3] != r[2]:
+            continue
+        if t[4] == r[3] and t[5] == r[4]:
+            f, sg = r[5], 1.0
+        elif t[4] == r[4] and t[5] == r[3]:
+            f, sg = 1.0 - r[5], -1.0
+        else:
+            continue
+        rows.append(j)
+        Fs.append(f)
+        v = np.asarray(V[r[0]], dtype=np.float64) * sg
+        Ws.append(v / S if method == 1 else v * S)
+    c = np.asarray(codes)[rows]                     # matched SNPs x samples
+    F = np.array(Fs)
+    W = np.array(Ws)
+    val = np.select([c == 0, c == 2, c == 3], [1.0, 0.5, 0.0], 0.0) - F[:, None]
+    miss = c == 1
+    val[miss] = 0.0
+    if standardize:
+        sd = np.sqrt(2.0 * F * (1.0 - F))
+        ok = sd > 1e-9
+        val[ok] /= sd[ok, None]
+    n = c.shape[1]
+    if method == 1:
+        return val.T @ W, W
+    U = np.empty((n, W.shape[1]))
+    for i in range(n):
+        o = ~miss[:, i]
+        U[i] = np.linalg.lstsq(W[o], val[o, i], rcond=None)[0]
+    return U, W
+

@@ -70,6 +70,8 @@ SIGNATURES = {
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "pcaone_read_csv": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32),
                                   C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]),
+    "pcaone_gpu_project": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, _D, _D, C.c_uint32, C.c_uint32, C.c_int,
+                                     C.c_int, _D, _D, C.POINTER(C.c_uint32)]),
     "pcaone_gpu_inbreeding": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_uint32, _D, C.c_int, C.c_double, _D, _D, _D,
                                         C.POINTER(C.c_int)]),
     "pcaone_write_hwe": (C.c_int, [C.c_char_p, C.c_char_p, _D, _D, _D, C.c_uint32]),
@@ -465,6 +467,22 @@ class Gpu:
         self._check(self._L.pcaone_gpu_load_csv(self.h, str(path).encode(), int(scale), float(scale_factor),
                                                  int(nthreads), C.byref(n), C.byref(m)))
         self.N, self.M = n.value, m.value
+
+    def project(self, bim, mbim, S, Vref, method: int = 2, standardize: bool = True):
+        """--project on the loaded PLINK samples: (U (N, k), W (noverlap, k) the written loadings)."""
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        Vref = np.asfortranarray(Vref, dtype=np.float64)
+        k = len(S)
+        U = np.empty((self.N, k), order="F")
+        W = np.empty((self.M, k), order="F")
+        nm = C.c_uint32(0)
+        self._check(self._L.pcaone_gpu_project(self.h, str(bim).encode(), str(mbim).encode(), _p(S, C.c_double),
+                                                _p(Vref, C.c_double), k, Vref.shape[0], int(method),
+                                                int(standardize), _p(U, C.c_double), _p(W, C.c_double),
+                                                C.byref(nm)))
+        n = nm.value
+        Wm = np.asfortranarray(W.reshape(-1, order="F")[: n * k].reshape((n, k), order="F"))
+        return U, Wm
 
     def inbreeding(self, U, S, V, F, maxiter: int = 100, tolem: float = 1e-5):
         """--inbreed 1 on the loaded PLINK hard calls: (coef, lrt, hwe_p, iterations)."""

@@ -301,6 +301,15 @@ int pcaone_read_csv(const char* path, int nthreads, float* values, uint64_t capa
   }
 }
 
+int pcaone_gpu_project(pcaone_gpu* h, const char* filebim, const char* filembim, const double* S,
+                       const double* Vref, uint32_t k, uint32_t ref_nsnps, int method, int standardize, double* U,
+                       double* W, uint32_t* noverlap) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    const uint32_t nm = e.project(filebim, filembim, S, Vref, k, ref_nsnps, method, standardize, U, W);
+    if (noverlap) *noverlap = nm;
+  });
+}
+
 int pcaone_gpu_inbreeding(pcaone_gpu* h, const double* U, const double* S, const double* V, uint32_t k,
                           const double* F, int maxiter, double tolem, double* coef, double* lrt, double* hwe_p,
                           int* iters) {

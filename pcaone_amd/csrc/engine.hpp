@@ -125,6 +125,9 @@ class Engine {
   void ld_prune(const char* filebim, int ld_bp, double r2_tol, const char* prefix);
   void ld_clump(const char* fileassoc, const char* colnames, const char* filebim, int clump_bp, double clump_r2,
                 double clump_p1, double clump_p2, const char* fileout);
+  // projection of the loaded PLINK samples onto a reference PCA; returns the matched SNP count
+  uint32_t project(const char* filebim, const char* filembim, const double* S, const double* Vref, uint32_t k,
+                   uint32_t ref_nsnps, int method, int standardize, double* U, double* Wout);
   // per-site inbreeding coefficients and HWE LRT on the loaded PLINK hard calls
   int inbreeding(const double* U, const double* S, const double* V, uint32_t k, const double* F, int maxiter,
                  double tolem, double* coef, double* lrt, double* hwe_p);

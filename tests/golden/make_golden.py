@@ -233,6 +233,70 @@ def csv_counts(seed, ncells, ngenes):
 
 
 LD_TOOLS = {"ld_tools_bp3000": (71, 157, (230, 170), 3000)}
+# --project 1/2 with --USV of a reference PCA: name -> (seed, nref, ntarget, nsnps)
+PROJECT = {"project_p12": (61, 220, 90, 1500)}
+
+
+def _pack_codes(codes: np.ndarray) -> np.ndarray:
+    """codes (M x N, .bed 2-bit codes) -> .bed payload rows (missing padding)."""
+    m, n = codes.shape
+    nb = (n + 3) // 4
+    c = np.full((m, nb * 4), 1, dtype=np.uint8)
+    c[:, :n] = codes
+    c = c.reshape(m, nb, 4)
+    return (c[:, :, 0] | (c[:, :, 1] << 2) | (c[:, :, 2] << 4) | (c[:, :, 3] << 6)).astype(np.uint8).reshape(-1)
+
+
+def run_project(tmp: Path, name: str, seed, nref, ntgt, m):
+    """One simulation split into a reference panel and target samples. The
+    target .bim leaves out some SNPs, swaps the alleles of others (their codes
+    flipped 00 <-> 11 to match) and moves a few to another position; the
+    reference PCA is written with -V, then --USV of it projects the target with
+    --project 1 and --project 2."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pcaone_oracle as O
+
+    n = nref + ntgt
+    pay = pa.sim_bed(seed, n, m, 3, 0.1, 0.05, 0.5, 0.02)
+    codes = O.decode_codes(pay, n, m)
+    rng = np.random.default_rng(seed)
+    ref_codes, tgt_codes = codes[:, :nref], codes[:, nref:]
+    pos = (np.arange(m) + 1) * 100
+    pa.write_plink(tmp / (name + "_ref"), _pack_codes(ref_codes), nref, m, None, pos)
+    keep = rng.random(m) > 0.05
+    swap = (rng.random(m) < 0.04) & keep
+    moved = (rng.random(m) < 0.01) & keep & ~swap
+    t = tgt_codes[keep].copy()
+    sw = swap[keep]
+    t[sw] = np.where(t[sw] == 0, 3, np.where(t[sw] == 3, 0, t[sw]))
+    idx = np.flatnonzero(keep)
+    with open(tmp / (name + "_tgt.bim"), "w") as f:
+        for i in idx:
+            a1, a2 = ("G", "A") if swap[i] else ("A", "G")
+            f.write(f"1\trs{i}\t0\t{pos[i] + (7 if moved[i] else 0)}\t{a1}\t{a2}\n")
+    with open(tmp / (name + "_tgt.bed"), "wb") as f:
+        f.write(bytes([0x6C, 0x1B, 0x01]))
+        f.write(_pack_codes(t).tobytes())
+    with open(tmp / (name + "_tgt.fam"), "w") as f:
+        for j in range(ntgt):
+            f.write(f"t{j} t{j} 0 0 0 -9\n")
+
+    def run(*args):
+        subprocess.run([str(REF), *args, "-n", "1", "-v", "0"], check=True, cwd=tmp, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+
+    run("-b", name + "_ref", "--svd", "1", "-k", "3", "-p", "6", "--tol-rsvd", "0", "-V", "-o", name + "_pca")
+    res = dict(tgt_payload=_pack_codes(t), ntarget=ntgt, ntarget_snps=int(keep.sum()),
+               tgt_bim=(tmp / (name + "_tgt.bim")).read_text(), mbim=(tmp / (name + "_pca.mbim")).read_text(),
+               S=np.loadtxt(tmp / (name + "_pca.sigvals"), ndmin=1, skiprows=1),
+               V=np.loadtxt(tmp / (name + "_pca.loadings"), ndmin=2))
+    for meth in (1, 2):
+        run("-b", name + "_tgt", "--USV", name + "_pca", "--project", str(meth), "-o", f"{name}_m{meth}")
+        res[f"U_m{meth}"] = np.loadtxt(tmp / f"{name}_m{meth}.eigvecs", ndmin=2)
+        res[f"text_eigvals_m{meth}"] = (tmp / f"{name}_m{meth}.eigvals").read_text()
+    return res
+
+
 # --inbreed 1 with --USV of a PCA on the same PLINK set: name -> dataset
 INBREED = {"inbreed_d1": "d1"}
 
@@ -382,6 +446,12 @@ def main():
             res = run_ld_usv(tmp, name, seed, n, per_chr, ld_bp, k)
             np.savez_compressed(OUT / f"{name}.npz", **res)
             print(name, len(res["r2"]), "pairs")
+        for name, (seed, nref, ntgt, m) in PROJECT.items():
+            if not want(name):
+                continue
+            res = run_project(tmp, name, seed, nref, ntgt, m)
+            np.savez_compressed(OUT / f"{name}.npz", **res)
+            print(name, res["ntarget_snps"], "target SNPs", res["U_m2"][:2])
         for name, ds in INBREED.items():
             if not want(name):
                 continue
