@@ -87,6 +87,10 @@ typedef struct {
   int perm;        /* winSVD column shuffle, Param::perm (src/Cmd.cpp:216)         */
   int standardize; /* divide by sqrt(ploidy F (1-F)) (src/Data.cpp:290-299)        */
   int ploidy;      /* 2, or 1 with --haploid                                       */
+  uint32_t blocksize; /* out of core (-m): window length of pcaone_ooc_plan's block
+                         plan (src/Data.cpp:27-71); 0: the in-core windows      */
+  int band_factor;    /* out of core winSVD: Data::bandFactor; the schedule counts
+                         blocks from it (src/Halko.cpp:227-269); 0 or 1 in core */
 } pcaone_rsvd_params;
 
 /* Builds the HBM layout for the operator (SNP shuffle, SNP-major and windowed
@@ -352,11 +356,27 @@ int pcaone_gpu_iram(pcaone_gpu* h, int k, int ncv, int maxit, double tol, double
                     int* nconv, int* nops);
 
 /* Host-only view of the winSVD window schedule of one computeGandH call
- * (src/Halko.cpp:159-223): `bandsize` is the operator state carried between
- * calls (reset at pi == 0).  For each window b: acc[b] in {1, 2} (H1 or H2),
- * update[b] (Omega refreshed after b), zero[b] (accumulator cleared by that
- * refresh, 0 if none); *reset_both = both accumulators cleared first. */
-int pcaone_fancy_plan(int bands, int pi, uint64_t* bandsize, int* acc, int* update, int* zero, int* reset_both);
+ * (in core src/Halko.cpp:159-223; out of core src/Halko.cpp:227-269, with
+ * nblocks blocks and the band factor; nblocks = 0: in core, bands windows):
+ * `bandsize` is the operator state carried between calls (reset at pi == 0).
+ * For each window b: acc[b] in {1, 2} (H1 or H2), update[b] (Omega refreshed
+ * after b), zero[b] (accumulator cleared by that refresh, 0 if none);
+ * *reset_both = both accumulators cleared first. */
+int pcaone_fancy_plan(int bands, uint32_t nblocks, int band_factor, int pi, uint64_t* bandsize, int* acc,
+                      int* update, int* zero, int* reset_both);
+
+/* ---- out of core (-m) ---------------------------------------------------- */
+/* Data::prepare's block plan (src/Data.cpp:27-71) for -m memory_gb: svd 0 IRAM,
+ * 1 Halko, 2 winSVD (then bands = -w).  Returns the block length, the number of
+ * blocks and the winSVD band factor; a plan of one block is the reference's
+ * error (PCAONE_EINVAL).  Host only. */
+int pcaone_ooc_plan(uint32_t nsamples, uint32_t nsnps, int svd, int k, int oversamples, int bands, double memory_gb,
+                    uint32_t* blocksize, uint32_t* nblocks, uint32_t* band_factor);
+/* The SNP order permute_plink gives the out-of-core winSVD
+ * (src/FilePlink.cpp:298-405, --buffer buffer_gb, nbands = -w): perm[ib] = the
+ * input SNP at position ib, the `perm` to pass pcaone_gpu_rsvd_init in place of
+ * the in-core shuffle.  Host only; no file is written. */
+int pcaone_plink_perm(uint32_t nsamples, uint32_t nsnps, uint32_t buffer_gb, uint32_t nbands, int32_t* perm);
 
 /* The .ld.gz file of --print-r2: ld_r2_big / ld_r2_small (src/LD.cpp:388-442)
  * write a header line "CHR_A\tBP_A\tSNP_A\tCHR_B\tBP_B\tSNP_B\tR2", then one line

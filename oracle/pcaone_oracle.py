@@ -305,28 +305,108 @@ def fancy_windows(nsnps: int, bands: int):
     return out
 
 
-class FancyState:
-    """The bandsize / H1 / H2 state that FancyRsvdOpData keeps between calls."""
+def ooc_plan(nsamples: int, nsnps: int, k: int, os: int, svd: int, bands: int, memory: float):
+    """Data::prepare's out-of-core block plan (src/Data.cpp:27-71) for -m
+    ``memory`` GB: returns (blocksize, nblocks, band_factor).  svd: 0 IRAM,
+    1 Halko, 2 winSVD.  The products 3*n*l, 2*m*l and 5*m are 32-bit unsigned
+    as in the reference; the one-block case is the reference's error."""
+    u32 = lambda x: int(x) & 0xFFFFFFFF
+    GB = 134217728  # doubles per GB
+    if svd == 0:
+        blocksize = u32(math.ceil(memory * GB / nsamples))
+    else:
+        l = u32(k + os)
+        a, b, c = u32(3 * nsamples * l), u32(2 * nsnps * l), u32(5 * nsnps)
+        m = float(u32(u32(a + b) + c)) / GB
+        if memory > 1.1 * m:
+            m = 0.0
+        blocksize = u32(math.ceil(((m + memory) * GB - a - b - c) / nsamples))
+    nblocks = u32(math.ceil(nsnps / blocksize))
+    if nblocks == 1:
+        raise ValueError("only one block exists. please remove -m option")
+    factor = 1
+    if svd == 2:
+        if nblocks < bands:
+            blocksize = u32(math.ceil(nsnps / bands))
+        else:
+            factor = u32(math.ceil(nblocks / bands))
+            blocksize = u32(math.ceil(nsnps / (bands * factor)))
+        nblocks = u32(math.ceil(nsnps / blocksize))
+    return blocksize, nblocks, factor
 
-    def __init__(self, bands: int):
+
+def plink_perm_indices(nsamples: int, nsnps: int, gb: int, nbands: int) -> np.ndarray:
+    """The SNP order permute_plink writes (src/FilePlink.cpp:298-405): perm[ib] =
+    ia, SNP ia of the input at position ib of the permuted file.  Buffers of
+    twoGB_snps SNPs (a multiple of nbands) are dealt round-robin into the bands,
+    band b starting at bandidx[b]; the last buffer may be short."""
+    bps = (nsamples + 3) >> 2
+    two = int(math.floor(1073741824.0 * gb / bps))
+    if two > nsnps:
+        two = nsnps
+    bufsize = two // nbands
+    two = bufsize * nbands
+    nblocks = (nsnps + two - 1) // two
+    modr2 = nsnps % two
+    modr = nsnps % nbands
+    bandsize = (nsnps + nbands - 1) // nbands
+    if modr == 0:
+        bandidx = [i * bandsize for i in range(nbands)]
+    else:
+        bandidx = [i * bandsize if i < modr else modr * bandsize + (bandsize - 1) * (i - modr) for i in range(nbands)]
+    bufidx = bufsize
+    perm = np.full(nsnps, -1, dtype=np.int64)
+    for i in range(nblocks):
+        if i == nblocks - 1 and modr2 != 0:
+            two2 = nsnps - (nblocks - 1) * two
+            bufsize = (two2 + nbands - 1) // nbands
+            modr2 = two2 % nbands
+        for b in range(nbands):
+            j = 0
+            for j in range(bufsize - 1):
+                perm[i * bufidx + bandidx[b] + j] = i * two + j * nbands + b
+            j = bufsize - 1 if bufsize >= 1 else 0
+            if i != nblocks - 1 or (i == nblocks - 1 and b < modr2) or modr2 == 0:
+                perm[i * bufidx + bandidx[b] + j] = i * two + j * nbands + b
+    if (perm < 0).any() or len(np.unique(perm)) != nsnps:
+        raise ValueError("permute_plink's interleave is not a permutation for these sizes")
+    return perm
+
+
+def ooc_windows(nsnps: int, blocksize: int):
+    """Block bounds [start, stop) of the out-of-core plan (src/Data.cpp:66-70)."""
+    nb = int(math.ceil(nsnps / blocksize))
+    return [(b * blocksize, min((b + 1) * blocksize, nsnps)) for b in range(nb)]
+
+
+class FancyState:
+    """The bandsize / H1 / H2 state that FancyRsvdOpData keeps between calls.
+    In core nblocks = bands and band_factor = 1; out of core (-m) the bands
+    are cut into nblocks blocks and the schedule counts blocks, starting at
+    band_factor (src/Halko.cpp:227-269)."""
+
+    def __init__(self, bands: int, nblocks: int | None = None, band_factor: int = 1):
         self.bands = bands
+        self.nblocks = bands if nblocks is None else nblocks
+        self.factor = band_factor
         self.bandsize = 1
 
 
 def fancy_epoch_plan(state: FancyState, pi: int):
     """Advance ``state`` by one computeGandH call and return its plan:
-    (reset_both, [(b, acc, update, zero), ...]) following src/Halko.cpp:159-223."""
-    bands = state.bands
+    (reset_both, [(b, acc, update, zero), ...]) following src/Halko.cpp:159-223
+    (in core) and src/Halko.cpp:227-269 (out of core)."""
+    bands, nblocks, factor = state.bands, state.nblocks, state.factor
     reset = 2 ** pi >= bands
     if pi == 0:
-        state.bandsize = 1
-    state.bandsize = min(state.bandsize * 2, bands)
+        state.bandsize = factor
+    state.bandsize = min(state.bandsize * 2, nblocks)
     bandsize = state.bandsize
     steps = []
     i = 1
-    for b in range(bands):
+    for b in range(nblocks):
         acc = 1 if i <= bandsize // 2 else 2
-        adjacent = pi > 0 and (b + 1) == 2 ** (pi - 1) and 2 ** pi < bands
+        adjacent = pi > 0 and (b + 1) == 2 ** (pi - 1) * factor and 2 ** pi < bands
         update, zero = False, 0
         if not ((b + 1) < bandsize and not adjacent):
             if (i == bandsize) or (i == bandsize // 2) or adjacent:
@@ -355,6 +435,11 @@ class RsvdOracle:
     perm: bool = True             # winSVD shuffles SNP columns at pi == 0
     use_mev: bool = True
     omega_override: np.ndarray | None = None
+    # out of core (-m): block length and band factor of ooc_plan, and the
+    # permutation of permute_plink in place of the in-core shuffle
+    blocksize: int = 0
+    band_factor: int = 1
+    perm_override: np.ndarray | None = None
     # state
     omg: np.ndarray = field(init=False, default=None)
     omg2: np.ndarray = field(init=False, default=None)
@@ -366,7 +451,11 @@ class RsvdOracle:
         self.l = self.k + self.os
         self.H1 = np.zeros((self.N, self.l))
         self.H2 = np.zeros((self.N, self.l))
-        self.state = FancyState(self.bands)
+        if self.blocksize:
+            self.windows = ooc_windows(self.M, self.blocksize)
+        else:
+            self.windows = fancy_windows(self.M, self.bands)
+        self.state = FancyState(self.bands, len(self.windows), self.band_factor)
         self.init_omg()
 
     def init_omg(self):
@@ -396,11 +485,12 @@ class RsvdOracle:
             self.H1[:] = 0.0
             self.H2[:] = 0.0
         if pi == 0 and self.perm and self.permutation is None:
-            self.permutation = shuffle_indices(self.M)
+            self.permutation = (np.asarray(self.perm_override) if self.perm_override is not None
+                                else shuffle_indices(self.M))
             self.X = np.ascontiguousarray(self.X[:, self.permutation])
         G = np.zeros((self.M, self.l))
         H = getattr(self, "H", np.zeros((self.N, self.l)))
-        for (b, (s, e)), (_, acc, update, zero) in zip(enumerate(fancy_windows(self.M, self.bands)), steps):
+        for (b, (s, e)), (_, acc, update, zero) in zip(enumerate(self.windows), steps):
             Xb = self.X[:, s:e]
             G[s:e] = Xb.T @ self.omg
             if acc == 1:

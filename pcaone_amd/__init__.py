@@ -34,7 +34,8 @@ class GpuError(RuntimeError):
 
 class RsvdParams(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("variant", "k", "oversamples", "bands", "precision", "seed", "rand",
-                                       "perm", "standardize", "ploidy")]
+                                       "perm", "standardize", "ploidy")] + [("blocksize", C.c_uint32),
+                                                                          ("band_factor", C.c_int)]
 
 
 _D = C.POINTER(C.c_double)
@@ -100,7 +101,10 @@ SIGNATURES = {
     "pcaone_gpu_gram_init": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "pcaone_gpu_gram_matvec": (C.c_int, [C.c_void_p, C.c_int, _D, _D]),
     "pcaone_gpu_iram": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, _D, _D, _D, _INT, _INT]),
-    "pcaone_fancy_plan": (C.c_int, [C.c_int, C.c_int, _U64, _INT, _INT, _INT, _INT]),
+    "pcaone_fancy_plan": (C.c_int, [C.c_int, C.c_uint32, C.c_int, C.c_int, _U64, _INT, _INT, _INT, _INT]),
+    "pcaone_ooc_plan": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "pcaone_plink_perm": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _I32]),
     "pcaone_gpu_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "pcaone_gpu_reset_stats": (C.c_int, [C.c_void_p]),
     "pcaone_gpu_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_size_t, _I64, _D]),
@@ -342,18 +346,39 @@ def write_eigvecs2(path: str | os.PathLike, beagle: str | os.PathLike, U2: np.nd
         raise GpuError(rc, "pcaone_write_eigvecs2 failed")
 
 
-def fancy_plan(bands: int, pi: int, bandsize: int):
-    """Host schedule of one winSVD computeGandH call -> (acc, update, zero, reset, new_bandsize)."""
+def fancy_plan(bands: int, pi: int, bandsize: int, nblocks: int = 0, band_factor: int = 1):
+    """Host schedule of one winSVD computeGandH call -> (acc, update, zero, reset, new_bandsize).
+    nblocks > 0: the out-of-core schedule over nblocks blocks with the band factor."""
     bs = C.c_uint64(bandsize)
-    acc = np.zeros(bands, dtype=np.intc)
-    upd = np.zeros(bands, dtype=np.intc)
-    zer = np.zeros(bands, dtype=np.intc)
+    nw = nblocks if nblocks else bands
+    acc = np.zeros(nw, dtype=np.intc)
+    upd = np.zeros(nw, dtype=np.intc)
+    zer = np.zeros(nw, dtype=np.intc)
     rst = C.c_int(0)
-    rc = lib().pcaone_fancy_plan(bands, pi, C.byref(bs), _p(acc, C.c_int), _p(upd, C.c_int), _p(zer, C.c_int),
-                                 C.byref(rst))
+    rc = lib().pcaone_fancy_plan(bands, nblocks, band_factor, pi, C.byref(bs), _p(acc, C.c_int), _p(upd, C.c_int),
+                                 _p(zer, C.c_int), C.byref(rst))
     if rc != 0:
         raise GpuError(rc, "pcaone_fancy_plan failed")
     return acc, upd.astype(bool), zer, bool(rst.value), bs.value
+
+
+def ooc_plan(nsamples: int, nsnps: int, svd: int, k: int, oversamples: int, bands: int, memory_gb: float):
+    """The -m block plan (src/Data.cpp:27-71) -> (blocksize, nblocks, band_factor)."""
+    bs, nb, f = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+    rc = lib().pcaone_ooc_plan(nsamples, nsnps, svd, k, oversamples, bands, memory_gb, C.byref(bs), C.byref(nb),
+                               C.byref(f))
+    if rc != 0:
+        raise GpuError(rc, "pcaone_ooc_plan failed (one block: remove -m)")
+    return bs.value, nb.value, f.value
+
+
+def plink_perm(nsamples: int, nsnps: int, buffer_gb: int, nbands: int) -> np.ndarray:
+    """permute_plink's SNP order (src/FilePlink.cpp:298-405): perm[ib] = input SNP at ib."""
+    out = np.empty(nsnps, dtype=np.int32)
+    rc = lib().pcaone_plink_perm(nsamples, nsnps, buffer_gb, nbands, _p(out, C.c_int32))
+    if rc != 0:
+        raise GpuError(rc, "pcaone_plink_perm failed")
+    return out
 
 
 class Gpu:
@@ -599,8 +624,9 @@ class Gpu:
     def rsvd_init(self, k: int, oversamples: int, variant: int = RSVD_FANCY, bands: int = 64,
                   precision: int = PREC_SPLIT_BF16, seed: int = 101, rand: int = 1, perm: bool = True,
                   standardize: bool = True, ploidy: int = 2, omega: np.ndarray | None = None,
-                  permutation: np.ndarray | None = None):
-        p = RsvdParams(variant, k, oversamples, bands, precision, seed, rand, int(perm), int(standardize), ploidy)
+                  permutation: np.ndarray | None = None, blocksize: int = 0, band_factor: int = 1):
+        p = RsvdParams(variant, k, oversamples, bands, precision, seed, rand, int(perm), int(standardize), ploidy,
+                       blocksize, band_factor)
         om = None if omega is None else np.asfortranarray(omega, dtype=np.float64)
         pm = None if permutation is None else np.ascontiguousarray(permutation, dtype=np.int32)
         self._check(self._L.pcaone_gpu_rsvd_init(self.h, C.byref(p), _p(om, C.c_double), _p(pm, C.c_int32)))

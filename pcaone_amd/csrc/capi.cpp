@@ -123,6 +123,8 @@ int pcaone_gpu_rsvd_init(pcaone_gpu* h, const pcaone_rsvd_params* p, const doubl
     q.perm = p->perm;
     q.standardize = p->standardize;
     q.ploidy = p->ploidy;
+    q.blocksize = p->blocksize;
+    q.band_factor = p->band_factor > 0 ? p->band_factor : 1;
     e.rsvd_init(q, omega, perm);
   });
 }
@@ -245,16 +247,43 @@ int pcaone_gpu_iram(pcaone_gpu* h, int k, int ncv, int maxit, double tol, double
   return guarded(h, [&](pcaone::Engine& e) { e.iram(k, ncv, maxit, tol, U, S, V, nconv, nops); });
 }
 
-int pcaone_fancy_plan(int bands, int pi, uint64_t* bandsize, int* acc, int* update, int* zero, int* reset_both) {
-  if (bands < 4 || (bands & (bands - 1)) || pi < 0 || !bandsize || !acc || !update || !zero) return PCAONE_EINVAL;
-  const pcaone::FancyPlan pl = pcaone::fancy_plan(bands, pi, *bandsize);
-  for (int b = 0; b < bands; ++b) {
+int pcaone_fancy_plan(int bands, uint32_t nblocks, int band_factor, int pi, uint64_t* bandsize, int* acc,
+                      int* update, int* zero, int* reset_both) {
+  if (bands < 4 || bands % 2 || pi < 0 || !bandsize || !acc || !update || !zero) return PCAONE_EINVAL;
+  if (nblocks > 0 && (nblocks < 2 || band_factor < 1)) return PCAONE_EINVAL;
+  const pcaone::FancyPlan pl = pcaone::fancy_plan(bands, pi, *bandsize, nblocks, nblocks ? (uint32_t)band_factor : 1);
+  for (size_t b = 0; b < pl.steps.size(); ++b) {
     acc[b] = pl.steps[b].acc;
     update[b] = pl.steps[b].update ? 1 : 0;
     zero[b] = pl.steps[b].zero;
   }
   if (reset_both) *reset_both = pl.reset_both ? 1 : 0;
   return PCAONE_OK;
+}
+
+int pcaone_ooc_plan(uint32_t nsamples, uint32_t nsnps, int svd, int k, int oversamples, int bands, double memory_gb,
+                    uint32_t* blocksize, uint32_t* nblocks, uint32_t* band_factor) {
+  if (!blocksize || !nblocks || !band_factor) return PCAONE_EINVAL;
+  try {
+    pcaone::ooc_plan(nsamples, nsnps, svd, k, oversamples, bands, memory_gb, *blocksize, *nblocks, *band_factor);
+    return PCAONE_OK;
+  } catch (const pcaone::Error& e) {
+    return e.code < 0 ? e.code : PCAONE_EINVAL;
+  } catch (...) {
+    return PCAONE_EINVAL;
+  }
+}
+
+int pcaone_plink_perm(uint32_t nsamples, uint32_t nsnps, uint32_t buffer_gb, uint32_t nbands, int32_t* perm) {
+  if (!perm) return PCAONE_EINVAL;
+  try {
+    pcaone::plink_perm(nsamples, nsnps, buffer_gb, nbands, perm);
+    return PCAONE_OK;
+  } catch (const pcaone::Error& e) {
+    return e.code < 0 ? e.code : PCAONE_EINVAL;
+  } catch (...) {
+    return PCAONE_EINVAL;
+  }
 }
 
 int pcaone_write_eigs(const char* prefix, uint32_t nsamples, uint32_t nsnps, int k, const double* S,

@@ -40,17 +40,23 @@ KernelTimer::~KernelTimer() {
 }
 
 // ---------------------------------------------------------------------------
-// winSVD window schedule, src/Halko.cpp:159-223
+// winSVD window schedule, src/Halko.cpp:159-223 (in core: bands windows);
+// out of core (src/Halko.cpp:227-269) the bands are cut into nblocks blocks
+// and the schedule counts blocks, starting at the band factor
 // ---------------------------------------------------------------------------
-FancyPlan fancy_plan(int bands, int pi, uint64_t& bandsize) {
+FancyPlan fancy_plan(int bands, int pi, uint64_t& bandsize, uint32_t nblocks, uint32_t factor) {
   FancyPlan pl;
+  if (nblocks == 0) {
+    nblocks = (uint32_t)bands;
+    factor = 1;
+  }
   pl.reset_both = std::ldexp(1.0, pi) >= (double)bands;
-  if (pi == 0) bandsize = 1;
-  bandsize = std::min<uint64_t>(bandsize * 2, (uint64_t)bands);
+  if (pi == 0) bandsize = factor;
+  bandsize = std::min<uint64_t>(bandsize * 2, (uint64_t)nblocks);
   uint64_t i = 1;
-  for (int b = 0; b < bands; ++b, ++i) {
+  for (uint32_t b = 0; b < nblocks; ++b, ++i) {
     WindowStep st{(i <= bandsize / 2) ? 1 : 2, false, 0};
-    const bool adjacent = pi > 0 && (double)(b + 1) == std::ldexp(1.0, pi - 1) && std::ldexp(1.0, pi) < bands;
+    const bool adjacent = pi > 0 && (double)(b + 1) == std::ldexp(1.0, pi - 1) * factor && std::ldexp(1.0, pi) < bands;
     if (!((uint64_t)(b + 1) < bandsize && !adjacent)) {
       if (i == bandsize || i == bandsize / 2 || adjacent) {
         st.update = true;
@@ -251,15 +257,26 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   // plain RSVD never reads -w (src/Halko.cpp:97-151) and cuts the SNPs into
   // its own windows, only as a tiling of the H contraction: 64, fewer when
   // there are fewer SNPs.
-  if (p.variant == 1) {
-    p_.bands = 64;
-    while (p_.bands > 4 && (uint64_t)p_.bands > M_) p_.bands /= 2;
+  // An out-of-core plan (-m, src/Data.cpp:27-71) gives the windows itself:
+  // blocks of p.blocksize SNPs, and for the winSVD a schedule that counts
+  // blocks from the band factor.
+  if (p.blocksize > 0) {
+    PCAONE_REQUIRE(p.band_factor >= 1, -1, "rsvd_init: band_factor must be >= 1");
+    bs_ = p.blocksize;
+    nwin_ = (uint32_t)ceil_div(M_, bs_);
+    PCAONE_REQUIRE(nwin_ >= 2, -1, "only one block exists. please remove -m option");
+  } else {
+    if (p.variant == 1) {
+      p_.bands = 64;
+      while (p_.bands > 4 && (uint64_t)p_.bands > M_) p_.bands /= 2;
+    }
+    p_.band_factor = 1;
+    bs_ = (uint32_t)ceil_div(M_, (uint64_t)p_.bands);
+    nwin_ = (uint32_t)ceil_div(M_, bs_);
+    if (p.variant == 2)
+      PCAONE_REQUIRE(nwin_ == (uint32_t)p.bands, -1,
+                     "winSVD needs every window non-empty: nsnps too small for the number of windows");
   }
-  bs_ = (uint32_t)ceil_div(M_, (uint64_t)p_.bands);
-  nwin_ = (uint32_t)ceil_div(M_, bs_);
-  if (p.variant == 2)
-    PCAONE_REQUIRE(nwin_ == (uint32_t)p.bands, -1,
-                   "winSVD needs every window non-empty: nsnps too small for the number of windows");
   PCAONE_REQUIRE(N_ >= 2 * l_, -1, "the GPU operator needs nsamples >= 2 (k + oversamples)");
   // permutation (src/RSVD.hpp:63-79): only the winSVD path shuffles
   perm_.resize(M_);
@@ -1080,7 +1097,7 @@ void Engine::rsvd_gandh(int pi, double* Gout, double* Hout, bool check) {
   } else {
     // FancyRsvdOpData in-core, src/Halko.cpp:153-226
     if (pi == 0) init_omega();
-    const FancyPlan pl = fancy_plan(p_.bands, pi, bandsize_);
+    const FancyPlan pl = fancy_plan(p_.bands, pi, bandsize_, nwin_, (uint32_t)p_.band_factor);
     if (pl.reset_both) {
       timed("fill", [&] { launch_fill_f64(H1_.p, (uint64_t)N_ * l_, 0.0, stream_); });
       timed("fill", [&] { launch_fill_f64(H2_.p, (uint64_t)N_ * l_, 0.0, stream_); });

@@ -41,12 +41,23 @@ struct FancyPlan {
   bool reset_both;
   std::vector<WindowStep> steps;
 };
-FancyPlan fancy_plan(int bands, int pi, uint64_t& bandsize);
+// nblocks = 0: the in-core plan (bands windows, factor 1)
+FancyPlan fancy_plan(int bands, int pi, uint64_t& bandsize, uint32_t nblocks = 0, uint32_t factor = 1);
 
 struct RsvdParams {
   int variant = 2, k = 10, os = 10, bands = 64, precision = 2, seed = 101, rand = 1, perm = 1, standardize = 1,
       ploidy = 2;
+  // out-of-core (-m) block plan: windows of `blocksize` SNPs and the winSVD band
+  // factor (0: the in-core windows)
+  uint32_t blocksize = 0;
+  int band_factor = 1;
 };
+
+// -m block plan (src/Data.cpp:27-71) and permute_plink's SNP order
+// (src/FilePlink.cpp:298-405), host only
+void ooc_plan(uint32_t nsamples, uint32_t nsnps, int svd, int k, int os, int bands, double memory, uint32_t& bs,
+              uint32_t& nblocks, uint32_t& factor);
+void plink_perm(uint32_t nsamples, uint32_t nsnps, uint32_t gb, uint32_t nbands, int32_t* perm);
 
 // Buffers and operand format of one set of contractions (the RSVD operator's
 // l columns, or the single column of the Gram mat-vec).

@@ -66,8 +66,14 @@ def oracle_run(z: dict):
         orc, _, _ = O.emu_halko(codes, d["k"], d["os"], d["p"], d["tol"], seed=d["seed"], rand=d["rand"])
         return orc, None, d
     X = oracle_matrix(z["payload"], n, m, bool(d["scale"]))
+    ooc = {}
+    if d["m"] > 0 and d["svd"] == 2:  # out-of-core winSVD: the -m block plan and permute_plink's order
+        bs, _, factor = O.ooc_plan(n, m, d["k"], d["os"], 2, d["w"], d["m"])
+        ooc = {"blocksize": bs, "band_factor": factor}
+        if not d["S"]:
+            ooc["perm_override"] = O.plink_perm_indices(n, m, 2, d["w"])
     orc = O.RsvdOracle(X, d["k"], d["os"], fancy=d["svd"] == 2, bands=d["w"], seed=d["seed"], rand=d["rand"],
-                       perm=not d["S"])
+                       perm=not d["S"], **ooc)
     ep = orc.compute_usv(d["p"], d["tol"])
     return orc, ep, d
 

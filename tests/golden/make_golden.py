@@ -48,6 +48,11 @@ CASES = {
     # EMU through the out-of-core reader (-m): the consistent EM path of the
     # reference (the in-core one indexes a permutation that --svd 1 never sets)
     "emu_svd1": ("d3", "--svd 1 -m 0.0005 --emu -k 3 --oversamples 7 -p 5 --tol-rsvd 0"),
+    # winSVD out of core (-m GB): the block plan of src/Data.cpp:27-71 (blocksize,
+    # nblocks, band factor 5 / 11 / 1) and permute_plink's SNP order
+    "svd2_ooc_w8_m5": ("d1", "--svd 2 -k 5 --oversamples 5 -w 8 -m 0.0005"),
+    "svd2_ooc_w4_d2": ("d2", "--svd 2 -k 3 --oversamples 7 -w 4 -p 2 --tol-rsvd 0 --seed 3 -m 0.0004"),
+    "svd2_ooc_w8_f1": ("d1", "--svd 2 -k 5 --oversamples 5 -w 8 -m 0.001"),
     # PCAngsd on in-core PLINK hard calls (--svd 1: no shuffle, consistent indices)
     "plink_pcangsd_svd1": ("d3", "--svd 1 --pcangsd -k 3 --oversamples 7 -p 5 --tol-rsvd 0"),
 }
@@ -378,7 +383,9 @@ def run_ld_tools(tmp: Path, name: str, seed, n, per_chr, ld_bp):
 
 
 def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
-    cmd = [str(REF), flag, prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v", "0"]
+    ooc_perm = "-m" in args.split() and "--svd 2" in args and "-S" not in args.split()
+    # -v 3 keeps the permuted copy the out-of-core winSVD writes (src/Main.cpp:151-162)
+    cmd = [str(REF), flag, prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v", "3" if ooc_perm else "0"]
     subprocess.run(cmd, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     res = {}
     res["eigvals"] = np.loadtxt(tmp / (out + ".eigvals"), ndmin=1)
@@ -389,6 +396,17 @@ def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
     res["loadings"] = np.loadtxt(tmp / (out + ".loadings"), ndmin=2)
     for ext in ("sigvals", "eigvals", "eigvecs", "loadings"):
         res["text_" + ext] = np.array((tmp / (out + "." + ext)).read_text())
+    if "-m" in args.split() and "--svd 2" in args:
+        # the block plan the reference logs and the SNP order of its permuted copy
+        log = (tmp / (out + ".log")).read_text()
+        plan = re.findall(r"after adjustment by PCAone: -w =\s*(\d+), blocksize =\s*(\d+), nblocks =\s*(\d+), "
+                          r"factor =\s*(\d+)", log)
+        res["ooc_plan"] = np.array([int(x) for x in plan[-1][1:]])
+    if ooc_perm:
+        ids = [ln.split()[1] for ln in (tmp / (prefix + ".bim")).read_text().splitlines()]
+        pos = {v: i for i, v in enumerate(ids)}
+        res["ooc_perm"] = np.array([pos[ln.split()[1]] for ln in
+                                    (tmp / (out + ".perm.bim")).read_text().splitlines()], dtype=np.int64)
     if "--pcangsd" in args.split():
         # the EM trajectory the reference logs (6 decimals), src/Halko.cpp:309-310
         log = (tmp / (out + ".log")).read_text()

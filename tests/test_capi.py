@@ -85,13 +85,46 @@ def test_host_window_schedule_matches_oracle(bands):
         assert bs == st.bandsize
 
 
+@pytest.mark.parametrize("bands,factor", [(8, 5), (4, 11), (8, 1), (16, 3)])
+def test_host_block_schedule_matches_oracle(bands, factor):
+    nblocks = bands * factor - (1 if factor > 2 else 0)  # ceil rounding can leave fewer blocks
+    st = O.FancyState(bands, nblocks, factor)
+    bs = 1
+    for pi in range(int(np.log2(bands)) + 4):
+        reset, steps = O.fancy_epoch_plan(st, pi)
+        acc, upd, zer, rst, bs = pa.fancy_plan(bands, pi, bs, nblocks, factor)
+        assert rst == reset
+        assert [list(acc), list(upd), list(zer)] == [[s[i] for s in steps] for i in (1, 2, 3)]
+        assert bs == st.bandsize
+
+
 def test_host_window_schedule_rejects_bad_bands():
     bs = C.c_uint64(1)
-    a = np.zeros(8, dtype=np.intc)
+    a = np.zeros(64, dtype=np.intc)
     P = C.POINTER(C.c_int)
-    for bands in (2, 6, 0):
-        assert pa.lib().pcaone_fancy_plan(bands, 0, C.byref(bs), a.ctypes.data_as(P), a.ctypes.data_as(P),
+    for bands, nb, f in ((2, 0, 1), (5, 0, 1), (0, 0, 1), (8, 1, 1), (8, 40, 0)):
+        assert pa.lib().pcaone_fancy_plan(bands, nb, f, 0, C.byref(bs), a.ctypes.data_as(P), a.ctypes.data_as(P),
                                           a.ctypes.data_as(P), None) == -1
+
+
+def test_ooc_plan_and_plink_perm_match_oracle():
+    from conftest import GOLDEN, load_golden
+    from _cases import parse_args
+    for name in ("svd2_ooc_w8_m5", "svd2_ooc_w4_d2", "svd2_ooc_w8_f1"):
+        z = load_golden(GOLDEN / f"{name}.npz")
+        d = parse_args(z["args"])
+        n, m = int(z["nsamples"]), int(z["nsnps"])
+        assert list(pa.ooc_plan(n, m, 2, d["k"], d["os"], d["w"], d["m"])) == [int(x) for x in z["ooc_plan"]]
+        assert np.array_equal(pa.plink_perm(n, m, 2, d["w"]), z["ooc_perm"])
+    for args in [(1000, 10 ** 6, 0, 10, 10, 64, 1.0), (1000, 100000, 1, 10, 10, 64, 0.5), (5000, 300000, 2, 20, 20,
+                                                                                         64, 0.7)]:
+        assert pa.ooc_plan(*args) == O.ooc_plan(args[0], args[1], args[3], args[4], args[2], args[5], args[6])
+    for m, nb in [(1000, 8), (1601, 4), (999, 8), (64, 4), (123457, 64)]:
+        assert np.array_equal(pa.plink_perm(100, m, 2, nb), O.plink_perm_indices(100, m, 2, nb))
+    # a buffer of one GB over 2 MB rows: several buffers and a short last one
+    assert np.array_equal(pa.plink_perm(8 * 10 ** 6, 1100, 1, 4), O.plink_perm_indices(8 * 10 ** 6, 1100, 1, 4))
+    with pytest.raises(pa.GpuError):  # a single block
+        pa.ooc_plan(200, 2000, 1, 5, 5, 8, 1.0)
 
 
 def test_ld_windows_match_oracle():

@@ -96,8 +96,16 @@ def test_usv_matches_reference_cli(gpu, case, arm, tmp_path):
         gpu.load_dense(z["dense"])
     else:
         gpu.load_bed(z["payload"], n, m)
+    ooc = {}
+    if d["m"] > 0 and d["svd"] == 2:  # out-of-core winSVD: the -m block plan and permute_plink's order
+        bs, _, factor = pa.ooc_plan(n, m, 2, d["k"], d["os"], d["w"], d["m"])
+        ooc = {"blocksize": bs, "band_factor": factor}
+        if not d["S"]:
+            ooc["permutation"] = pa.plink_perm(n, m, 2, d["w"])
     gpu.rsvd_init(d["k"], d["os"], variant=d["svd"], bands=d["w"], precision=ARMS[arm], seed=d["seed"],
-                  rand=d["rand"], perm=not d["S"], standardize=bool(d["scale"]))
+                  rand=d["rand"], perm=not d["S"], standardize=bool(d["scale"]), **ooc)
+    if ooc:
+        assert np.array_equal(gpu.perm(), ooc.get("permutation", np.arange(m)))
     if d["emu"]:
         U, S, V, it, diffs = gpu.em(d["p"], d["tol"], 100, 1e-5)
         orc, _, _ = oracle_run(z)

@@ -432,3 +432,97 @@ def test_csv_planted_mistakes():
     X = np.log10(c * med / lib + 1)
     assert eig_relerr(run((X - X.mean(axis=1, keepdims=True)).T.copy()), z["eigvals"]) > 1e-3
 
+
+
+# --- out of core (-m): block plan, permute_plink order, block schedule -------
+
+OOC = ["svd2_ooc_w8_m5", "svd2_ooc_w4_d2", "svd2_ooc_w8_f1"]
+
+
+@pytest.mark.parametrize("name", OOC)
+def test_ooc_plan_and_perm_match_reference(name):
+    z = load_golden(ROOT / "tests" / "golden" / f"{name}.npz")
+    d = parse_args(z["args"])
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    bs, nb, factor = O.ooc_plan(n, m, d["k"], d["os"], 2, d["w"], d["m"])
+    assert [bs, nb, factor] == [int(x) for x in z["ooc_plan"]]  # the reference's logged plan
+    assert np.array_equal(O.plink_perm_indices(n, m, 2, d["w"]), z["ooc_perm"])  # its .perm.bim order
+
+
+def test_ooc_plan_closed_forms():
+    # IRAM: memory * 2^27 doubles per sample column, nothing else
+    assert O.ooc_plan(1000, 10 ** 6, 10, 10, 0, 64, 1.0) == (134218, 8, 1)
+    # Halko with room to spare: blocksize = (memory * 2^27 - 3nl - 2ml - 5m) / n
+    n, m, l = 1000, 100000, 20
+    bs, nb, f = O.ooc_plan(n, m, 10, 10, 1, 64, 0.5)
+    assert bs == int(np.ceil((0.5 * 134217728 - 3 * n * l - 2 * m * l - 5 * m) / n)) and nb == -(-m // bs) and f == 1
+    # winSVD: fewer blocks than bands -> one block per band; else bands * factor blocks
+    assert O.ooc_plan(200, 2000, 5, 5, 2, 8, 0.001) == (250, 8, 1)
+    bs, nb, f = O.ooc_plan(200, 2000, 5, 5, 2, 8, 0.0005)
+    assert f == 5 and bs == -(-2000 // 40) and nb == 40
+    with pytest.raises(ValueError):  # a single block is the reference's error
+        O.ooc_plan(200, 2000, 5, 5, 1, 8, 1.0)
+
+
+@pytest.mark.parametrize("m,nb", [(2000, 8), (1601, 4), (999, 8), (64, 4)])
+def test_plink_perm_closed_forms(m, nb):
+    p = O.plink_perm_indices(100, m, 2, nb)
+    assert np.array_equal(np.sort(p), np.arange(m))
+    # band b takes SNPs b, b + nb, b + 2 nb, ... in order from its first position
+    bandsize = -(-m // nb)
+    modr = m % nb
+    starts = [b * bandsize if (modr == 0 or b < modr) else modr * bandsize + (bandsize - 1) * (b - modr)
+              for b in range(nb)] + [m]
+    for b in range(nb):
+        seg = p[starts[b]:starts[b + 1]]
+        assert np.array_equal(seg, b + nb * np.arange(len(seg)))
+
+
+def test_ooc_schedule_closed_forms():
+    # factor F: the first epoch refreshes after every F-th... band of 2F blocks;
+    # the in-core plan is F = 1 with nblocks = bands
+    for bands, F in [(8, 5), (4, 11), (8, 1)]:
+        nblocks = bands * F
+        st = O.FancyState(bands, nblocks, F)
+        reset, steps = O.fancy_epoch_plan(st, 0)
+        upd = [b for b, _, u, _ in steps if u]
+        assert st.bandsize == 2 * F and not reset
+        assert upd == [b for b in range(nblocks) if (b + 1) % F == 0 and b + 1 >= 2 * F] or F == 1
+        lb = int(np.log2(bands))
+        for pi in range(1, lb + 2):
+            reset, steps = O.fancy_epoch_plan(st, pi)
+            upd = [b for b, _, u, _ in steps if u]
+            if pi >= lb:
+                assert upd == [nblocks - 1] and reset
+            else:
+                assert upd[0] == 2 ** (pi - 1) * F - 1  # the "adjacent" refresh, counted in blocks
+    st_in, st_f1 = O.FancyState(8), O.FancyState(8, 8, 1)
+    for pi in range(5):
+        assert O.fancy_epoch_plan(st_in, pi) == O.fancy_epoch_plan(st_f1, pi)
+
+
+@pytest.mark.parametrize("name", ["svd2_ooc_w8_m5", "svd2_ooc_w4_d2"])
+def test_ooc_planted_mistakes(name):
+    # w8_m5 runs to convergence (the order of the SNPs still shows), w4_d2 stops
+    # after 2 epochs (the block schedule shows too)
+    z = load_golden(ROOT / "tests" / "golden" / f"{name}.npz")
+    d = parse_args(z["args"])
+    n, m = int(z["nsamples"]), int(z["nsnps"])
+    X = oracle_matrix(z["payload"], n, m)
+    bs, _, f = O.ooc_plan(n, m, d["k"], d["os"], 2, d["w"], d["m"])
+    perm = O.plink_perm_indices(n, m, 2, d["w"])
+
+    def run(**kw):
+        orc = O.RsvdOracle(X, d["k"], d["os"], fancy=True, bands=d["w"], seed=d["seed"], **kw)
+        orc.compute_usv(d["p"], d["tol"])
+        return orc
+
+    good = run(blocksize=bs, band_factor=f, perm_override=perm)
+    assert eig_relerr(good.eigvals(), z["eigvals"]) < 5e-6
+    # the in-core shuffle in place of permute_plink's interleave
+    assert eig_relerr(run(blocksize=bs, band_factor=f).eigvals(), z["eigvals"]) > 1e-5
+    if name == "svd2_ooc_w4_d2":
+        # the in-core windows (bands, factor 1)
+        assert eig_relerr(run(perm_override=perm).eigvals(), z["eigvals"]) > 1e-5
+        # the blocks without the band factor in the schedule
+        assert eig_relerr(run(blocksize=bs, band_factor=1, perm_override=perm).eigvals(), z["eigvals"]) > 1e-5
