@@ -62,6 +62,18 @@ int pcaone_gpu_host_free(void* p);
  * (nsamples+3)>>2 bytes.  The bytes stay packed in HBM; allele frequencies
  * are computed on the device from exact code counts. */
 int pcaone_gpu_load_bed(pcaone_gpu* h, const uint8_t* payload, uint32_t nsamples, uint32_t nsnps);
+/* A PLINK .bed file (magic 6c 1b 01, then nsnps rows of ceil(nsamples / 4)
+ * bytes): FileBed's reader (src/FilePlink.cpp:18-131).  out_of_core = 0 reads
+ * it into HBM as pcaone_gpu_load_bed does.  out_of_core = 1 is -m
+ * (Param::out_of_core, src/Cmd.cpp:192; read_block_initial,
+ * src/FilePlink.cpp:133-224): the file stays mapped in host memory and every
+ * pass of the operators streams it window by window (rows gathered into pinned
+ * memory, copied to a two-slot device ring, laid out there), so a .bed larger
+ * than HBM runs.  F comes from one streamed pass.  Paths that need the whole
+ * matrix in HBM at once (LD, PCAngsd on PLINK, inbreeding, residuals,
+ * projection) refuse a streamed handle with PCAONE_ESTATE. */
+int pcaone_gpu_load_bed_file(pcaone_gpu* h, const char* filebed, uint32_t nsamples, uint32_t nsnps,
+                             int out_of_core);
 /* Replaces FileBin::read_all (src/FileBinary.cpp:21-30): `featmajor` holds
  * nfeatures rows of nsamples float32 (the .bin / .residuals body after the
  * u32 M, u32 N header, src/FileBinary.hpp:15-16).  The values stay FP32 in

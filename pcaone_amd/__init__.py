@@ -54,6 +54,7 @@ SIGNATURES = {
     "pcaone_gpu_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "pcaone_gpu_host_free": (C.c_int, [C.c_void_p]),
     "pcaone_gpu_load_bed": (C.c_int, [C.c_void_p, _U8, C.c_uint32, C.c_uint32]),
+    "pcaone_gpu_load_bed_file": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_int]),
     "pcaone_gpu_allele_freq": (C.c_int, [C.c_void_p, _D, _I64]),
     "pcaone_gpu_load_dense_f32": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint32, C.c_uint32]),
     "pcaone_gpu_feature_means": (C.c_int, [C.c_void_p, _D]),
@@ -422,6 +423,12 @@ class Gpu:
         if payload.size != nsnps * ((nsamples + 3) >> 2):
             raise ValueError("payload size does not match nsnps * ceil(nsamples/4)")
         self._check(self._L.pcaone_gpu_load_bed(self.h, _p(payload, C.c_uint8), nsamples, nsnps))
+        self.N, self.M = nsamples, nsnps
+
+    def load_bed_file(self, filebed, nsamples: int, nsnps: int, out_of_core: bool = False):
+        """A .bed file: into HBM, or (out_of_core, -m) mapped and streamed through a device ring."""
+        self._check(self._L.pcaone_gpu_load_bed_file(self.h, str(filebed).encode(), nsamples, nsnps,
+                                                     int(out_of_core)))
         self.N, self.M = nsamples, nsnps
 
     def load_dense(self, featmajor: np.ndarray):

@@ -109,6 +109,11 @@ int pcaone_gpu_allele_freq(pcaone_gpu* h, double* F, int64_t* counts) {
   return guarded(h, [&](pcaone::Engine& e) { e.allele_freq(F, counts); });
 }
 
+int pcaone_gpu_load_bed_file(pcaone_gpu* h, const char* filebed, uint32_t nsamples, uint32_t nsnps,
+                             int out_of_core) {
+  return guarded(h, [&](pcaone::Engine& e) { e.load_bed_file(filebed, nsamples, nsnps, out_of_core != 0); });
+}
+
 int pcaone_gpu_rsvd_init(pcaone_gpu* h, const pcaone_rsvd_params* p, const double* omega, const int32_t* perm) {
   return guarded(h, [&](pcaone::Engine& e) {
     PCAONE_REQUIRE(p != nullptr, PCAONE_EINVAL, "rsvd_init: null params");

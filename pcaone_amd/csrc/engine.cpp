@@ -89,6 +89,11 @@ Engine::Engine(int device) : device_(device) {
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  try {
+    end_stream();
+  } catch (...) {
+  }
+  if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
   destroy_blas(blas_);
   timer.resolve();
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -103,6 +108,7 @@ void Engine::load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps)
   PCAONE_REQUIRE(payload && nsamples > 0 && nsnps > 0, -1, "load_bed: empty input");
   PCAONE_HIP(hipSetDevice(device_));
   have_op_ = false;
+  end_stream();
   N_ = nsamples;
   M_ = nsnps;
   const uint64_t nb = (N_ + 3) >> 2;
@@ -119,6 +125,11 @@ void Engine::load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps)
   std::vector<int32_t> h((size_t)M_ * 4);
   PCAONE_HIP(hipMemcpyAsync(h.data(), cnt.p, h.size() * 4, hipMemcpyDeviceToHost, stream_));
   sync();
+  set_code_counts(h);
+}
+
+// F from the per-SNP code counts (00, 01, 10, 11), src/FilePlink.cpp:36-62
+void Engine::set_code_counts(const std::vector<int32_t>& h) {
   F_orig_.assign(M_, 0.0);
   counts_orig_.assign((size_t)M_ * 4, 0);
   for (uint32_t i = 0; i < M_; ++i) {
@@ -150,6 +161,7 @@ void Engine::load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfea
 // The features in rawf_ become the operator's matrix: with `centre`, each
 // feature's FP64 mean is what the epilogues subtract; without, nothing is.
 void Engine::load_dense_finish(bool centre) {
+  end_stream();
   mu_orig_.assign(M_, 0.0);
   if (centre) {
     DevBuf<double> mu;
@@ -437,6 +449,21 @@ void Engine::build_layout() {
   kw_ = (uint64_t)win_bytes_ * 4;
   kg_ = npad_;
   miss_rows_ = npad_;
+  if (streamed_) {
+    // the windows are laid out in the ring at each use; F, d, the permutation
+    // (V's rows come back through it) and the missing lists (one pass over the
+    // rows) stay resident
+    d_perm_.reserve(M_);
+    PCAONE_HIP(hipMemcpyAsync(d_perm_.p, perm_.data(), (size_t)M_ * 4, hipMemcpyHostToDevice, stream_));
+    F_lay_.resize(M_);
+    for (uint32_t r = 0; r < M_; ++r) F_lay_[r] = F_orig_[perm_[r]];
+    F_.reserve(M_);
+    PCAONE_HIP(hipMemcpyAsync(F_.p, F_lay_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
+    upload_scale();
+    stream_setup();
+    stream_missing_lists();
+    return;
+  }
   d_perm_.reserve(M_);
   PCAONE_HIP(hipMemcpyAsync(d_perm_.p, perm_.data(), (size_t)M_ * 4, hipMemcpyHostToDevice, stream_));
   bed_snp_.reserve((uint64_t)row_bytes_ * M_);
@@ -515,7 +542,8 @@ void Engine::partial_rows(int prec, uint64_t& gmax, uint64_t& hmax) const {
   const uint32_t rpb = rows_per_block_(prec);
   const uint32_t kcg = (uint32_t)(kg_ / (dense ? 64 : 256)), kch = (uint32_t)(kw_ / (dense ? 64 : 256));
   gmax = hmax = 0;
-  for (uint32_t nw = 1; nw <= nwin_; ++nw) {
+  // streamed: every launch covers one window
+  for (uint32_t nw = 1; nw <= (streamed_ ? 1u : nwin_); ++nw) {
     const uint64_t rows = std::min<uint64_t>((uint64_t)nw * bs_, M_);
     uint32_t cps;
     const uint32_t sg = dense ? split_for(ceil_div(rows, rpb), kcg, cps) : packed_split(prec, rows, 1, kcg, cps);
@@ -669,8 +697,7 @@ void Engine::gram_matvec(int side, const double* x, double* y) {
     const std::vector<int> route(nwin_, 1);
     set_omega_vec(nullptr, x);
     timed("fill", [&] { launch_fill_f64(mv_H_.p, N_, 0.0, stream_); });
-    seg_G(0, nwin_, mv_);
-    seg_H(0, nwin_, route, mv_);
+    run_segment(0, nwin_, route, mv_);
     PCAONE_HIP(hipMemcpyAsync(y, mv_H_.p, (size_t)N_ * 8, hipMemcpyDeviceToHost, stream_));
     sync();
   } else {
@@ -857,6 +884,19 @@ SegBufs Engine::op_bufs() {
 // G rows of windows [w0, w1): G = X_seg^T Omega, its epilogue (centring,
 // missing-entry corrections, H operand) and the per-window sums S''.
 void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
+  if (!streamed_) {
+    seg_G_on(w0, w1, b, src_ == SRC_BED ? bed_snp_.p + (uint64_t)w0 * bs_ * row_bytes_ : nullptr);
+    return;
+  }
+  for (uint32_t w = w0; w < w1; ++w) {
+    const int s = stream_acquire(w);
+    seg_G_on(w, w + 1, b, slots_[s].snp.p);
+    stream_release(s);
+    stream_prefetch((w + 1) % nwin_);
+  }
+}
+
+void Engine::seg_G_on(uint32_t w0, uint32_t w1, const SegBufs& b, const uint8_t* A) {
   const int prec = b.prec;
   const bool dense = src_ == SRC_DENSE;
   const uint32_t r0 = w0 * bs_, r1 = std::min<uint32_t>(w1 * bs_, M_);
@@ -883,7 +923,7 @@ void Engine::seg_G(uint32_t w0, uint32_t w1, const SegBufs& b) {
   }
   if (!dense) {
     PackedGemm g{};
-    g.A = bed_snp_.p + (uint64_t)r0 * row_bytes_;
+    g.A = A;
     g.lda = row_bytes_;
     g.nrows = rows;
     g.kchunks = row_bytes_ / 64;
@@ -967,6 +1007,20 @@ FinalizeG Engine::finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) cons
 
 // H contributions of windows [w0, w1) (X_w G_w), routed per window to Ha / Hb.
 void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b) {
+  if (!streamed_) {
+    seg_H_on(w0, w1, route, b, src_ == SRC_BED ? bed_smp_.p + (uint64_t)w0 * npad_ * win_bytes_ : nullptr);
+    return;
+  }
+  for (uint32_t w = w0; w < w1; ++w) {
+    const int s = stream_acquire(w);
+    seg_H_on(w, w + 1, route, b, slots_[s].smp.p);
+    stream_release(s);
+    stream_prefetch((w + 1) % nwin_);
+  }
+}
+
+void Engine::seg_H_on(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b,
+                      const uint8_t* A) {
   const int prec = b.prec;
   const bool dense = src_ == SRC_DENSE;
   const uint32_t rpb = rows_per_block_(prec);
@@ -989,7 +1043,7 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
   }
   if (!dense) {
     PackedGemm h{};
-    h.A = bed_smp_.p + (uint64_t)w0 * npad_ * win_bytes_;
+    h.A = A;
     h.lda = win_bytes_;
     h.a_zstride = (uint64_t)npad_ * win_bytes_;
     h.nrows = N_;
@@ -1065,9 +1119,25 @@ void Engine::seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, cons
 }
 
 void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route) {
-  const SegBufs b = op_bufs();
-  seg_G(w0, w1, b);
-  seg_H(w0, w1, route, b);
+  run_segment(w0, w1, route, op_bufs());
+}
+
+// G rows and H contributions of windows [w0, w1).  Streamed, each window
+// travels once: its G rows, then its H contribution, then the next window
+// (the reference's out-of-core loop, src/Halko.cpp:138-149, 232-268).
+void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b) {
+  if (!streamed_) {
+    seg_G(w0, w1, b);
+    seg_H(w0, w1, route, b);
+    return;
+  }
+  for (uint32_t w = w0; w < w1; ++w) {
+    const int s = stream_acquire(w);
+    seg_G_on(w, w + 1, b, slots_[s].snp.p);
+    seg_H_on(w, w + 1, route, b, slots_[s].smp.p);
+    stream_release(s);
+    stream_prefetch((w + 1) % nwin_);
+  }
 }
 
 void Engine::rsvd_gandh(int pi, double* Gout, double* Hout, bool check) {

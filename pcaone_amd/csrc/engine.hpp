@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <utility>
@@ -118,6 +119,10 @@ class Engine {
   ~Engine();
 
   void load_bed(const uint8_t* payload, uint32_t nsamples, uint32_t nsnps);
+  // a .bed file: read into HBM, or (out_of_core) mapped and streamed window by
+  // window through a two-slot device ring at every use
+  void load_bed_file(const char* filebed, uint32_t nsamples, uint32_t nsnps, bool out_of_core);
+  bool streamed() const { return streamed_; }
   void load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfeatures);
   // FileCsv::read_all: a (zstd-compressed) CSV of counts, --scale on the device
   void load_csv(const char* path, int scale, double scale_factor, int nthreads);
@@ -203,6 +208,7 @@ class Engine {
   void omega_refresh(bool from_two, double* zero_acc);
   void omega_refresh_wide(bool from_two, double* zero_acc);  // l > 64, and every l in robust mode
   void run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route);
+  void run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b);
   void usv_step();
   void usv_step_wide();  // l > 64
   void usv_step_robust();  // robust mode: G orthogonalised through its Gram eigenpairs
@@ -218,8 +224,12 @@ class Engine {
   SegBufs op_bufs();
   void refresh_omega32(const SegBufs& b);
   FinalizeG finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) const;
+  // windows [w0, w1): resident, or one window at a time through the ring
   void seg_G(uint32_t w0, uint32_t w1, const SegBufs& b);
   void seg_H(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b);
+  // A: SNP-major rows of window w0 on (seg_G); sample-major block of window w0 on (seg_H)
+  void seg_G_on(uint32_t w0, uint32_t w1, const SegBufs& b, const uint8_t* A);
+  void seg_H_on(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b, const uint8_t* A);
   void partial_rows(int prec, uint64_t& gmax, uint64_t& hmax) const;
   uint32_t packed_split(int prec, uint64_t rows, uint32_t nz, uint32_t kchunks, uint32_t& cps) const;
   void set_omega_vec(const double* dev_v, const double* host_v);
@@ -230,6 +240,41 @@ class Engine {
 
   int device_;
   hipStream_t stream_ = nullptr;  // every launch, in order, at the default priority
+
+  // ---- out of core: the payload in host memory, windows through a ring ---
+  struct StreamSlot {
+    DevBuf<uint8_t> raw, snp, smp;  // window rows as read, SNP-major rows, sample-major block
+    uint8_t* stage = nullptr;       // pinned host staging of the rows
+    size_t stage_bytes = 0;
+    hipEvent_t copied = nullptr;    // the staging's copy to raw is done (copy stream)
+    hipEvent_t used = nullptr;      // the last work on this slot is done (stream_)
+    bool copy_rec = false, used_rec = false, laid = false;
+    int64_t win = -1;
+  };
+  bool streamed_ = false;
+  const uint8_t* host_bed_ = nullptr;  // M rows of ceil(N/4) bytes
+  void* map_base_ = nullptr;
+  size_t map_len_ = 0;
+  StreamSlot slots_[2];
+  int last_slot_ = -1;
+  hipStream_t copy_stream_ = nullptr;  // host-to-device copies of the ring (default priority)
+  void end_stream();                   // back to resident data: unmap, free the ring
+  void set_code_counts(const std::vector<int32_t>& h);  // F and counts from (00, 01, 10, 11) per SNP
+  void stream_counts();                // code counts and F by one pass over the mapped rows
+  void stream_setup();                 // ring buffers for the current windows
+  void stream_missing_lists();
+  void stream_fill(int s, uint32_t w);
+  int stream_acquire(uint32_t w);      // slot holding window w, laid out, ordered on stream_
+  void stream_release(int s);
+  void stream_prefetch(uint32_t w);
+  void gather_rows(uint8_t* dst, uint32_t r0, uint32_t nr, bool permuted) const;
+  uint32_t win_rows(uint32_t w) const { return std::min<uint32_t>(bs_, M_ - w * bs_); }
+  // the payload in HBM, for the paths that need all of it at once
+  const uint8_t* resident_raw(const char* what) const {
+    PCAONE_REQUIRE(!streamed_, -3,
+                   std::string(what) + ": needs the genotypes resident in HBM (load without out_of_core)");
+    return raw_.p;
+  }
 
   // ---- data --------------------------------------------------------------
   enum Source { SRC_NONE, SRC_BED, SRC_DENSE };

@@ -142,7 +142,7 @@ int Engine::inbreeding(const double* U, const double* S, const double* V, uint32
   PCAONE_HIP(hipMemcpyAsync(dSV.p, sv.data(), sv.size() * 8, hipMemcpyHostToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(dF.p, F, (size_t)M * 8, hipMemcpyHostToDevice, stream_));
   InbPass g{};
-  g.bed = raw_.p;
+  g.bed = resident_raw("inbreeding");
   g.row_bytes = (N_ + 3) >> 2;
   g.nsamples = N_;
   g.nsnps = M;

@@ -41,7 +41,8 @@ struct PackedRows {
   uint32_t valid;
   uint32_t valid_last;
   uint32_t ngroups;
-  uint32_t base_step;  // index_base(r) = group * base_step
+  uint32_t base_step;  // index_base(r) = base0 + group * base_step
+  uint32_t base0;      // 0 unless the rows are one window cut out of many
 };
 void launch_count_missing(const PackedRows& pr, int32_t* counts, hipStream_t s);
 void launch_fill_missing(const PackedRows& pr, const int64_t* ptr, int32_t* idx, hipStream_t s);

@@ -150,7 +150,7 @@ __device__ inline bool pr_row(const PackedRows& pr, uint64_t r, uint32_t& nvalid
   const uint32_t g = (uint32_t)(r / pr.rows_per_group), jr = (uint32_t)(r % pr.rows_per_group);
   if (jr >= pr.live_rows) return false;
   nvalid = (g + 1 == pr.ngroups) ? pr.valid_last : pr.valid;
-  base = g * pr.base_step;
+  base = pr.base0 + g * pr.base_step;
   return true;
 }
 

@@ -57,7 +57,7 @@ void Engine::ld_usv(const double* U, uint32_t k) {
   resid_.reserve((uint64_t)M_ * N_);
   PCAONE_HIP(hipMemcpyAsync(du.p, ur.data(), ur.size() * 8, hipMemcpyHostToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(df.p, F_orig_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));
-  timed("ld_usv", [&] { launch_usv_residual(raw_.p, N_, M_, df.p, du.p, k, dw.p, resid_.p, stream_); });
+  timed("ld_usv", [&] { launch_usv_residual(resident_raw("ld_usv"), N_, M_, df.p, du.p, k, dw.p, resid_.p, stream_); });
   sync();  // the staging buffers are freed on return
   have_resid_ = true;
 }
@@ -198,7 +198,7 @@ uint64_t Engine::ld_r2_geno(const int32_t* ws, const int32_t* we, uint32_t nwin,
     ldg_row_bytes_ = (uint32_t)round_up((N_ + 3) >> 2, 64);
     ldg_bed_.reserve((uint64_t)ldg_row_bytes_ * M_);
     timed("ld_geno_layout",
-          [&] { launch_bed_to_snpmajor(raw_.p, N_, M_, nullptr, ldg_bed_.p, ldg_row_bytes_, stream_); });
+          [&] { launch_bed_to_snpmajor(resident_raw("ld_r2_geno"), N_, M_, nullptr, ldg_bed_.p, ldg_row_bytes_, stream_); });
     std::vector<double> mun(M_), ss(M_);
     for (uint32_t i = 0; i < M_; ++i) {
       const int64_t n00 = counts_orig_[4 * (size_t)i], n10 = counts_orig_[4 * (size_t)i + 2],

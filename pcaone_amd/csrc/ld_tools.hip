@@ -206,7 +206,7 @@ void Engine::write_residuals(const char* fileout, const double* U, const double*
     for (uint32_t r0 = 0; r0 < M_; r0 += chunk) {
       const uint32_t nr = std::min(chunk, M_ - r0);
       timed("residuals", [&] {
-        hipLaunchKernelGGL(k_residual_rows, dim3((uint32_t)ceil_div(nr, kResSnps)), dim3(256), 0, stream_, raw_.p,
+        hipLaunchKernelGGL(k_residual_rows, dim3((uint32_t)ceil_div(nr, kResSnps)), dim3(256), 0, stream_, resident_raw("write_residuals"),
                            nb, N_, r0, nr, dF.p, dU.p, dSV.p, k, dout.p);
       });
       PCAONE_HIP(hipMemcpyAsync(host.data(), dout.p, (size_t)nr * N_ * 4, hipMemcpyDeviceToHost, stream_));

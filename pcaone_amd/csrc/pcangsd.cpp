@@ -30,6 +30,7 @@ void Engine::load_gl(const float* gl, uint32_t nsamples, uint32_t nsites, int pe
     }
     src = two.data();
   }
+  end_stream();
   glP_.reserve((uint64_t)N_ * M_ * 2);
   PCAONE_HIP(hipMemcpyAsync(glP_.p, src, (uint64_t)N_ * M_ * 2 * sizeof(float), hipMemcpyHostToDevice, stream_));
   sync();  // `two` goes out of scope
@@ -53,7 +54,7 @@ void Engine::gl_from_bed() {
   PCAONE_HIP(hipSetDevice(device_));
   have_op_ = have_mv_ = false;
   glP_.reserve((uint64_t)N_ * M_ * 2);
-  timed("gl_from_bed", [&] { launch_bed_to_gl(raw_.p, N_, M_, glP_.p, stream_); });
+  timed("gl_from_bed", [&] { launch_bed_to_gl(resident_raw("gl_from_bed"), N_, M_, glP_.p, stream_); });
   glF_.reserve(M_);
   glF_host_ = F_orig_;
   PCAONE_HIP(hipMemcpyAsync(glF_.p, glF_host_.data(), (size_t)M_ * 8, hipMemcpyHostToDevice, stream_));

@@ -240,7 +240,7 @@ uint32_t Engine::project(const char* filebim, const char* filembim, const double
   PCAONE_HIP(hipMemcpyAsync(dW.p, W.data(), W.size() * 8, hipMemcpyHostToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(dG.p, gram.data(), gram.size() * 8, hipMemcpyHostToDevice, stream_));
   ProjArgs a{};
-  a.bed = raw_.p;
+  a.bed = resident_raw("project");
   a.row_bytes = (N_ + 3) >> 2;
   a.nsamples = N_;
   a.nmatch = nm;
