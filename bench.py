@@ -19,6 +19,8 @@ reported under "configs" (each with its own metric, roofline and CPU baseline):
   C4  dense scRNA-like 200,000 cells x 20,000 genes FP32: RSVD k = 50 (l = 100)
       and IRAM top-50 (ncv = 101)
   C3  PCAngsd EM on genotype likelihoods, 5,000 x 200,000 sites, k = 4
+  BGEN  the BGEN reader (8-bit zlib probabilities, 10,000 x 50,000) into the
+      dense path
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -584,6 +586,75 @@ def c3_cpu_baseline(pa, gl, cores):
             "cpu": cpu_facts()}
 
 
+# ---------------------------------------------------------------------------
+# BGEN input: the reader feeding the dense path
+# ---------------------------------------------------------------------------
+def run_bgen(g, pa, args):
+    """pcaone_gpu_load_bgen on a BGEN v1.2 file (layout 2, 8-bit, zlib) in host
+    memory: container walk, decompression on the host's 16 worker threads,
+    the chunk copied to HBM and decoded there (dosage, minor allele, F,
+    centring, standardisation, FP32 matrix)."""
+    n, m = (10000, 50000) if not args.small else (2000, 5000)
+    with tempfile.TemporaryDirectory() as td:
+        path = Path(td) / "x.bgen"
+        t0 = time.perf_counter()
+        pa.sim_bgen(path, 2031, n, m, 5, 0.05, miss=0.01, layout=2, compression=1, bits=8, nthreads=CPU_THREADS)
+        fsize = path.stat().st_size
+        log(f"BGEN: wrote {m} variants x {n} samples ({fsize / 1e6:.0f} MB) in {time.perf_counter() - t0:.1f} s")
+        g.load_bgen(path, nthreads=CPU_THREADS)  # warm-up (page cache, pinned staging)
+        walls, kms = [], []
+        g.profiling(True)
+        for _ in range(max(1, args.steps)):
+            g.reset_stats()
+            t0 = time.perf_counter()
+            g.load_bgen(path, nthreads=CPU_THREADS)
+            g.synchronize()
+            walls.append(time.perf_counter() - t0)
+            kms.append(g.kernel_stats()["bgen_dosage"])
+        g.profiling(False)
+        cpu = None if args.no_cpu else bgen_cpu_baseline(pa, n, min(2000, m), CPU_THREADS)
+    wall = float(np.mean(walls))
+    launches = kms[0][0]
+    kms_ms = float(np.mean([k[1] for k in kms]))
+    dbytes = m * (2.0 * n + n + 10)  # decompressed genotype blocks: header, ploidy bytes, two bytes per sample
+    # the decode kernel's algorithmic bytes: the decompressed block in, the FP32 row out
+    kbytes = m * (3.0 * n + 10 + 4.0 * n)
+    achieved = kbytes / (kms_ms * 1e-3) / 1e9
+    return {"metric": "BGEN read s (FileBgen::read_all: decompress on host threads, decode on the device)",
+            "value": round(wall, 4), "unit": "s", "higher_is_better": False, "dtype": "u8 -> f32",
+            "config": {"workload": f"BGEN v1.2 layout 2, 8-bit probabilities, zlib, {n} samples x {m} variants, 5 "
+                                   "Balding-Nichols pops Fst 0.05, 1% missing; minor-allele dosages, centred and "
+                                   "standardised FP32 matrix in HBM", "file_MB": round(fsize / 1e6, 1)},
+            "variants_per_s": round(m / wall, 1), "decompressed_GBps": round(dbytes / wall / 1e9, 3),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "kernel": "k_bgen_dosage (one workgroup per variant)", "launches": launches,
+                         "kernel_ms_per_read": round(kms_ms, 3),
+                         "note": "the read is bound by host decompression; the device decode is a small share"},
+            "cpu_baseline": cpu}
+
+
+def bgen_cpu_baseline(pa, n, msample, cores):
+    """The reference CLI's read_all of the first `msample` variants of the same
+    simulation (the simulator seeds each variant on its own), from its logged
+    reading time."""
+    with tempfile.TemporaryDirectory() as td:
+        pa.sim_bgen(Path(td) / "s.bgen", 2031, n, msample, 5, 0.05, miss=0.01, layout=2, compression=1, bits=8,
+                    nthreads=cores)
+        log(f"BGEN: reference CLI on {msample} variants x {n} samples")
+        r = run_ref(["-g", "s.bgen", "--svd", "1", "-k", "2", "-p", "1", "--tol-rsvd", "0", "-n", str(cores), "-o",
+                     "out", "-v", "1"], td)
+    if r is None:
+        return None
+    wall, _, text = r
+    rd = re.search(r"total elapsed reading time:\s*([0-9.eE+-]+)", text)
+    read_s = float(rd.group(1)) if rd else wall
+    return {"value": round(msample / read_s, 1), "unit": "variants/s", "cores": cores, "kind": "reference",
+            "sample": f"reference CLI (-n {cores}) -g on the first {msample} variants x {n} samples of the same "
+                      f"simulation: logged reading time {read_s:.3f} s",
+            "cpu": cpu_facts()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -594,7 +665,7 @@ def main():
     ap.add_argument("--cpu-snps", type=int, default=40000)
     ap.add_argument("--arms", default="bf16x2,fp32,fp64",
                     help="C2 arms to run (profiling runs use bf16x2 alone); the JSON line needs all three")
-    ap.add_argument("--configs", default="c2,c5,c4,c3", help="workloads to run; the JSON line needs c2")
+    ap.add_argument("--configs", default="c2,c5,c4,c3,bgen", help="workloads to run; the JSON line needs c2")
     ap.add_argument("--no-iram", action="store_true", help="C4 without the IRAM runs (short profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -611,14 +682,15 @@ def main():
     todo = args.configs.split(",")
     head = run_c2(g, pa, args) if "c2" in todo else None
     configs = {}
-    for name, fn in (("c5", run_c5), ("c4", run_c4), ("c3", run_c3)):
+    for name, fn in (("c5", run_c5), ("c4", run_c4), ("c3", run_c3), ("bgen", run_bgen)):
         if name in todo:
             configs[name.upper()] = fn(g, pa, args)
             log(f"{name.upper()}: " + json.dumps({k: v for k, v in configs[name.upper()].items()
                                                   if k in ("value", "unit", "roofline")}))
     g.close()
     if head is None:
-        log("partial run (--configs / --arms): no JSON line")
+        log("partial run (--configs / --arms): no JSON line; the configs:")
+        log(json.dumps(configs))
         return
     head["configs"] = configs
     print(json.dumps(head), flush=True)

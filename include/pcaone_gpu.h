@@ -193,6 +193,38 @@ int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double 
  * *nsamples and *nfeatures (optional) receive its size. */
 int pcaone_gpu_load_csv(pcaone_gpu* h, const char* path, int scale, double scale_factor, int nthreads,
                         uint32_t* nsamples, uint32_t* nfeatures);
+/* FileBgen (src/FileBgen.hpp:13-46, src/FileBgen.cpp:11-107) with the
+ * reference's vendored BGEN reader (external/bgen): a BGEN v1.1 / v1.2 file,
+ * layout 1 or 2, uncompressed, zlib or zstd (libzstd.so.1 opened at run
+ * time).  The host walks the variant blocks and decompresses them on
+ * nthreads threads; the device decodes the probabilities.
+ * mode 0 (FileBgen::read_all without --pcangsd): per variant the minor
+ * allele's dosage (Genotypes::get_allele_dosage with use_minor: the 8-bit
+ * byte-pair and any-depth paths, the sampled minor-allele test), F = mean of
+ * dose / 2 over the non-missing samples, the variant kept when F > maf (maf
+ * above 0.5 read as 1 - maf, src/Cmd.cpp:196-199), and the feature
+ * dose / 2 - F (missing 0), divided by sqrt(ploidy F (1 - F)) when
+ * standardize and that exceeds 1e-9 (Data::standardize_E, which the
+ * reference's operator applies to BGEN input, src/Halko.cpp:285-289,
+ * src/Arnoldi.cpp:60-62).  The matrix then serves as for
+ * pcaone_gpu_load_dense_f32, without further centring.  F and minor
+ * (optional, nvariants entries each, file order): every variant's estimate
+ * and minor allele index (0: the first allele).
+ * mode 1 (--pcangsd, src/FileBgen.cpp:72-106): the first two genotype
+ * probabilities of every sample (Genotypes::probabilities; missing samples
+ * NaN) become the likelihoods of pcaone_gpu_load_gl (diploid unphased data
+ * only; maf must be 0).  nsamples, nsnps (optional): the loaded size. */
+int pcaone_gpu_load_bgen(pcaone_gpu* h, const char* path, int mode, double maf, int standardize, int ploidy,
+                         int nthreads, uint32_t* nsamples, uint32_t* nsnps, double* F, uint8_t* minor);
+/* Rows [r0, r0 + nrows) of the dense matrix in HBM (FP32, nsamples per row):
+ * what FileBin, FileCsv or FileBgen's read_all leave in data->G, column by
+ * column (src/Data.hpp:49).  No counterpart entry point in the reference,
+ * whose matrix is a host array; for callers that write it out (.residuals)
+ * or check it. */
+int pcaone_gpu_dense_rows(pcaone_gpu* h, uint32_t r0, uint32_t nrows, float* out);
+/* Host only: the BGEN header (CppBgenReader's Header, external/bgen/header.cpp)
+ * and a walk over every variant block. */
+int pcaone_bgen_info(const char* path, uint32_t* nsamples, uint32_t* nvariants, int* layout, int* compression);
 /* The reader of pcaone_gpu_load_csv alone, host only: the counts as read
  * (feature-major float32, capacity in floats; values NULL: sizes only) and
  * each sample's integer library size (libsize, nsamples, optional). */
@@ -437,6 +469,20 @@ int pcaone_sim_bed(uint64_t seed, uint32_t nsamples, uint32_t nsnps, uint32_t np
  * is normalised to sum 1 (depth 0 gives 1/3 each). */
 int pcaone_sim_gl(uint64_t seed, uint32_t nsamples, uint32_t nsites, uint32_t npops, double fst, double pmin,
                   double pmax, double depth, double err, float* out, int nthreads);
+/* A BGEN file (host only) of Balding-Nichols genotypes as in pcaone_sim_bed,
+ * written as genotype probabilities: each call certain with probability
+ * 0.6, otherwise spread (the true genotype keeps a weight ~ U(0.5, 1), the
+ * rest split at random), quantised to `bits` per probability so the stored
+ * values sum to 2^bits - 1 (layout 1: three 16-bit values scaled by 32768).
+ * Variant j's first allele is the modelled one for even j and the other for
+ * odd j.  layout 1 or 2; compression 0 none, 1 zlib, 2 zstd; bits 1-32
+ * (layout 2); missing samples (probability miss_rate) are flagged in the
+ * ploidy byte with zero probabilities (layout 1: all zero); pmin_declared 1
+ * declares a ploidy range 1-2 (every sample still diploid); sample_ids adds
+ * the sample identifier block. */
+int pcaone_sim_bgen(const char* path, uint64_t seed, uint32_t nsamples, uint32_t nsnps, uint32_t npops, double fst,
+                    double pmin, double pmax, double miss_rate, int layout, int compression, int bits,
+                    int pmin_declared, int sample_ids, int nthreads);
 /* scRNA-like dense matrix (host only): ngenes rows of ncells float32 in
  * `out`.  Gene means base * LogNormal(0, sdlog), cell size factors
  * LogNormal(0, 0.3), counts ~ NegBinomial(mean, dispersion phi) drawn as

@@ -970,3 +970,220 @@ def project_plink(codes: np.ndarray, bim_lines, mbim_lines, S, V, method: int, s
         U[i] = np.linalg.lstsq(W[o], val[o, i], rcond=None)[0]
     return U, W
 
+
+
+# ----------------------------------------------------------------------------
+# BGEN input (FileBgen::read_all, src/FileBgen.cpp:11-107) through the
+# reference's vendored bgen library: the container (external/bgen/header.cpp,
+# variant.cpp), the genotype blocks (genotypes.cpp) and minor_certain
+# (utils.cpp:46-51)
+# ----------------------------------------------------------------------------
+def bgen_lut8() -> np.ndarray:
+    """genotypes.cpp's 511-entry table: i / 255 printed to 7 decimals, read as
+    float (the literal of the table)."""
+    return np.array([np.float32(float(f"{i / 255.0:.7f}")) for i in range(511)], dtype=np.float32)
+
+
+def _zstd_decompress(data: bytes, size: int) -> bytes:
+    import ctypes
+
+    z = ctypes.CDLL("libzstd.so.1")
+    z.ZSTD_decompress.restype = ctypes.c_size_t
+    z.ZSTD_isError.restype = ctypes.c_uint
+    dst = ctypes.create_string_buffer(max(size, 1))
+    n = z.ZSTD_decompress(dst, ctypes.c_size_t(size), data, ctypes.c_size_t(len(data)))
+    if z.ZSTD_isError(ctypes.c_size_t(n)) or n != size:
+        raise ValueError("zstd decompression gave data of wrong length")
+    return dst.raw[:size]
+
+
+def bgen_blocks(path):
+    """Header.cpp / Variant.cpp: (nsamples, layout, [(nalleles, decompressed
+    genotype block)] in file order); the first variant sits at offset + 4."""
+    import struct
+    import zlib
+
+    b = open(path, "rb").read()
+    first, hlen, nvar, nsmp = struct.unpack_from("<IIII", b, 0)
+    flags = struct.unpack_from("<I", b, max(20, hlen))[0]
+    comp, layout = flags & 3, (flags >> 2) & 15
+    off = first + 4
+    out = []
+    for _ in range(nvar):
+        p = off
+        if layout == 1:
+            assert struct.unpack_from("<I", b, p)[0] == nsmp
+            p += 4
+        for _ in range(3):  # variant id, rsid, chromosome
+            p += 2 + struct.unpack_from("<H", b, p)[0]
+        p += 4
+        K = 2
+        if layout == 2:
+            K = struct.unpack_from("<H", b, p)[0]
+            p += 2
+        for _ in range(K):
+            p += 4 + struct.unpack_from("<I", b, p)[0]
+        if layout == 1 and comp == 0:
+            length = 6 * nsmp
+        else:
+            length = struct.unpack_from("<I", b, p)[0]
+            p += 4
+        blk = b[p:p + length]
+        off = p + length
+        if comp != 0 and layout == 2:
+            dlen = struct.unpack_from("<I", blk, 0)[0]
+            blk = blk[4:]
+        else:
+            dlen = 6 * nsmp if layout == 1 else len(blk)
+        if comp == 1:
+            blk = zlib.decompress(blk)
+        elif comp == 2:
+            blk = _zstd_decompress(blk, dlen)
+        assert len(blk) == dlen
+        out.append((K, blk))
+    return nsmp, layout, out
+
+
+def _bits(q: np.ndarray, start_bits: np.ndarray, B: int) -> np.ndarray:
+    """the B-bit fields at the given bit offsets of the little-endian stream q"""
+    qq = np.concatenate([q, np.zeros(8, dtype=np.uint8)]).astype(np.uint64)
+    byte = (start_bits >> 3).astype(np.int64)
+    w = np.zeros(len(start_bits), dtype=np.uint64)
+    for i in range(5):
+        w |= qq[byte + i] << np.uint64(8 * i)
+    return ((w >> (start_bits & 7).astype(np.uint64)) & np.uint64((1 << B) - 1)).astype(np.uint32)
+
+
+def _bgen_header2(blk: bytes, N: int):
+    q = np.frombuffer(blk, dtype=np.uint8)
+    pmin, pmax = int(q[6]), int(q[7])
+    ploidy = q[8:8 + N]
+    phased, B = int(q[8 + N]), int(q[9 + N])
+    return q, pmin, pmax, ploidy, phased, B, q[10 + N:]
+
+
+def bgen_ref_dosage(blk: bytes, layout: int, N: int):
+    """Genotypes::get_allele_dosage's first step: the first allele's dosage
+    (float32) of every sample and the missing mask.  ref_dosage_fast
+    (constant diploid unphased 8-bit) takes its AVX2 loop, (2 hom + het) x
+    (1/255f), for samples below 16 ceil((N - 32) / 16) and the lut8 table
+    for the rest; ref_dosage_slow reads two B-bit values per sample,
+    ((hom ploidy) + het (ploidy / 2)) x 1/(2^B - 1) in float (layout 1: three
+    16-bit values, x 1/32768, missing when all three are 0)."""
+    if layout == 1:
+        v = np.frombuffer(blk, dtype="<u2").reshape(N, 3).astype(np.uint32)
+        miss = (v[:, 0] == 0) & (v[:, 1] == 0) & (v[:, 2] == 0)
+        d = (v[:, 0] * np.uint32(2) + v[:, 1]).astype(np.float32) * np.float32(1.0 / 32768)
+        return d, miss
+    q, pmin, pmax, ploidy, phased, B, pr = _bgen_header2(blk, N)
+    miss = (ploidy & 0x80) != 0
+    const = pmin == pmax
+    if const and not phased and pmax == 2 and B == 8:
+        i = 2 * pr[0:2 * N:2].astype(np.uint32) + pr[1:2 * N:2].astype(np.uint32)
+        vec_end = 16 * (-(-(N - 32) // 16)) if N > 32 else 0
+        d = bgen_lut8()[i]
+        d[:vec_end] = i[:vec_end].astype(np.float32) * np.float32(np.float32(1.0) / np.float32(255.0))
+        return d, miss
+    cp = np.full(N, pmax, dtype=np.uint32) if const else (ploidy & 63).astype(np.uint32)
+    n = np.arange(N, dtype=np.uint64)
+    hom = _bits(pr, np.uint64(2 * B) * n, B)
+    het = _bits(pr, np.uint64(2 * B) * n + np.uint64(B), B)
+    maxval = np.uint32(int(2.0 ** B - 1.0))
+    factor = np.float32(np.float32(1.0) / np.float32(maxval))
+    with np.errstate(over="ignore"):
+        d = (hom * cp + het * (cp // np.uint32(2))).astype(np.float32) * factor
+    return d, miss
+
+
+def bgen_minor_index(dose: np.ndarray) -> int:
+    """Genotypes::find_minor_allele: residue classes n = idx2 (mod inc),
+    inc = max(N / 100, 1), summed in order into a running double total;
+    freq = total / (100 (idx2 + 1) 2) after each class (the denominator
+    counts 100 samples per class whatever N is, so freq exceeds 1 for
+    N < 200 at high dosage, where the interval is NaN and the test passes),
+    stop at the first class where minor_certain(freq, 100 (idx2 + 1), 5)
+    holds; minor index 0 when freq <= 0.5."""
+    N = len(dose)
+    inc = max(N // 100, 1)
+    total, freq = 0.0, 0.0
+    for idx2 in range(inc):
+        for n in range(idx2, N, inc):
+            total += float(dose[n])
+        freq = total / (100 * (idx2 + 1) * 2)
+        nchk = 100 * (idx2 + 1)
+        v = (freq * (1 - freq)) / nchk
+        delta = 5.0 * math.sqrt(v) if v >= 0 else math.nan  # std::sqrt of a negative: NaN
+        if not ((freq - delta < 0.5) and (freq + delta > 0.5)):
+            break
+    return 0 if freq <= 0.5 else 1
+
+
+def bgen_read(path, maf: float = 0.0, standardize: bool = True, ploidy: int = 2):
+    """FileBgen::read_all without --pcangsd: per variant the minor allele's
+    dosage (swap_allele_dosage: 2 - d in float; missing NaN), F = sum of d / 2
+    over the non-missing samples / their count, kept when F > maf (maf above
+    0.5 read as 1 - maf); X (N x kept) = d / 2 - F, missing 0, then
+    Data::standardize_E when standardize.  Returns (X, F of every variant,
+    minor index of every variant, kept indices).  The reference files the
+    centred column of variant j under j instead of its kept index k
+    (src/FileBgen.cpp:51); this restatement files it under k."""
+    if maf > 0.5:
+        maf = 1.0 - maf
+    N, layout, blocks = bgen_blocks(path)
+    cols, Fs, minors, keep = [], [], [], []
+    for j, (_, blk) in enumerate(blocks):
+        d, miss = bgen_ref_dosage(blk, layout, N)
+        mi = bgen_minor_index(d)
+        if mi != 0:
+            d = np.float32(2.0) - d
+        gs = float(np.sum(d[~miss].astype(np.float64) / 2.0))
+        gc = int(np.count_nonzero(~miss))
+        af = 0.0 if gc == 0 else gs / gc
+        Fs.append(af)
+        minors.append(mi)
+        if af > maf:
+            x = d.astype(np.float64) / 2.0 - af
+            x[miss] = 0.0
+            cols.append(x)
+            keep.append(j)
+    F = np.array(Fs)
+    X = np.array(cols).T.copy()
+    if standardize:
+        X = standardize_cols(X, F[keep], ploidy)
+    return X, F, np.array(minors, dtype=np.uint8), np.array(keep, dtype=np.int64)
+
+
+def standardize_cols(X: np.ndarray, F: np.ndarray, ploidy: int = 2) -> np.ndarray:
+    """Data::standardize_E on X (N x M) with F (M)."""
+    return standardize(X, F, ploidy)
+
+
+def bgen_probs(path) -> np.ndarray:
+    """FileBgen::read_all with --pcangsd: P (M, N, 2) float32, the first two
+    genotype probabilities of every sample (Genotypes::probabilities: lut8 of
+    the two bytes for constant diploid unphased 8-bit data, else B-bit values x
+    (float)(1 / ((float)2^B - 1)); layout 1: 16-bit values x 1/32768 with
+    missing when all three are 0); missing samples NaN."""
+    N, layout, blocks = bgen_blocks(path)
+    P = np.empty((len(blocks), N, 2), dtype=np.float32)
+    lut = bgen_lut8()
+    for j, (_, blk) in enumerate(blocks):
+        if layout == 1:
+            v = np.frombuffer(blk, dtype="<u2").reshape(N, 3)
+            p = v.astype(np.float32) * np.float32(1.0 / 32768)
+            miss = (v == 0).all(axis=1)
+            P[j] = p[:, :2]
+        else:
+            q, pmin, pmax, ploidy, phased, B, pr = _bgen_header2(blk, N)
+            assert pmin == pmax == 2 and not phased
+            miss = (ploidy & 0x80) != 0
+            if B == 8:
+                P[j, :, 0] = lut[pr[0:2 * N:2]]
+                P[j, :, 1] = lut[pr[1:2 * N:2]]
+            else:
+                fac = np.float32(1.0 / float(np.float32(2.0 ** B) - np.float32(1)))
+                n = np.arange(N, dtype=np.uint64)
+                P[j, :, 0] = _bits(pr, np.uint64(2 * B) * n, B).astype(np.float32) * fac
+                P[j, :, 1] = _bits(pr, np.uint64(2 * B) * n + np.uint64(B), B).astype(np.float32) * fac
+        P[j, miss] = np.nan
+    return P

@@ -70,6 +70,14 @@ SIGNATURES = {
     "pcaone_gpu_grm_eig": (C.c_int, [C.c_void_p, _D, _D]),
     "pcaone_gpu_load_csv": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_int,
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "pcaone_gpu_load_bgen": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _D, _U8]),
+    "pcaone_gpu_dense_rows": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
+    "pcaone_bgen_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int)]),
+    "pcaone_sim_bgen": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_int]),
     "pcaone_read_csv": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_float), C.c_uint64, C.POINTER(C.c_uint32),
                                   C.POINTER(C.c_uint64), C.POINTER(C.c_int64)]),
     "pcaone_gpu_project": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, _D, _D, C.c_uint32, C.c_uint32, C.c_int,
@@ -240,6 +248,25 @@ def sim_gl(seed: int, nsamples: int, nsites: int, npops: int = 4, fst: float = 0
     if rc != 0:
         raise GpuError(rc, "pcaone_sim_gl failed")
     return out
+
+
+def sim_bgen(path: str | os.PathLike, seed: int, nsamples: int, nsnps: int, npops: int = 3, fst: float = 0.1,
+             pmin: float = 0.05, pmax: float = 0.5, miss: float = 0.0, layout: int = 2, compression: int = 1,
+             bits: int = 8, pmin_declared: int = 2, sample_ids: bool = False, nthreads: int = 0):
+    """A BGEN file of Balding-Nichols genotype probabilities (host only)."""
+    rc = lib().pcaone_sim_bgen(str(path).encode(), seed, nsamples, nsnps, npops, fst, pmin, pmax, miss, layout,
+                               compression, bits, 1 if pmin_declared == 1 else 0, 1 if sample_ids else 0, nthreads)
+    if rc != 0:
+        raise GpuError(rc, "pcaone_sim_bgen failed")
+
+
+def bgen_info(path: str | os.PathLike) -> dict:
+    """Header of a BGEN file and a walk over its variant blocks (host only)."""
+    n, v, lay, comp = C.c_uint32(0), C.c_uint32(0), C.c_int(0), C.c_int(0)
+    rc = lib().pcaone_bgen_info(str(path).encode(), C.byref(n), C.byref(v), C.byref(lay), C.byref(comp))
+    if rc != 0:
+        raise GpuError(rc, f"pcaone_bgen_info failed on {path}")
+    return {"nsamples": n.value, "nvariants": v.value, "layout": lay.value, "compression": comp.value}
 
 
 def write_beagle(path: str | os.PathLike, gl: np.ndarray):
@@ -500,6 +527,21 @@ class Gpu:
                                                  int(nthreads), C.byref(n), C.byref(m)))
         self.N, self.M = n.value, m.value
 
+    def load_bgen(self, path, pcangsd: bool = False, maf: float = 0.0, standardize: bool = True, ploidy: int = 2,
+                  nthreads: int = 16):
+        """A BGEN file: minor-allele dosages into the dense path (returns F and
+        the minor allele index of every variant), or with pcangsd the genotype
+        probabilities as likelihoods (returns None)."""
+        nv = bgen_info(path)["nvariants"]
+        F = np.empty(nv) if not pcangsd else None
+        minor = np.empty(nv, dtype=np.uint8) if not pcangsd else None
+        n, m = C.c_uint32(0), C.c_uint32(0)
+        self._check(self._L.pcaone_gpu_load_bgen(self.h, str(path).encode(), 1 if pcangsd else 0, float(maf),
+                                                  1 if standardize else 0, int(ploidy), int(nthreads), C.byref(n),
+                                                  C.byref(m), _p(F, C.c_double), _p(minor, C.c_uint8)))
+        self.N, self.M = n.value, m.value
+        return None if pcangsd else (F, minor)
+
     def project(self, bim, mbim, S, Vref, method: int = 2, standardize: bool = True):
         """--project on the loaded PLINK samples: (U (N, k), W (noverlap, k) the written loadings)."""
         S = np.ascontiguousarray(S, dtype=np.float64)
@@ -615,6 +657,13 @@ class Gpu:
                                                            _p(out, C.c_double), n.value, C.byref(n),
                                                            _p(sm, C.c_int32) if sums else None, config))
         return out, sm
+
+    def dense_rows(self, r0: int = 0, nrows: int | None = None) -> np.ndarray:
+        """Rows of the loaded dense matrix as held in HBM, (nrows, N) float32."""
+        nrows = self.M - r0 if nrows is None else nrows
+        out = np.empty((nrows, self.N), dtype=np.float32)
+        self._check(self._L.pcaone_gpu_dense_rows(self.h, r0, nrows, _p(out, C.c_float)))
+        return out
 
     def feature_means(self):
         mu = np.empty(self.M)

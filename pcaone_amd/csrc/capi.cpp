@@ -323,6 +323,34 @@ int pcaone_gpu_load_csv(pcaone_gpu* h, const char* path, int scale, double scale
   });
 }
 
+int pcaone_gpu_load_bgen(pcaone_gpu* h, const char* path, int mode, double maf, int standardize, int ploidy,
+                         int nthreads, uint32_t* nsamples, uint32_t* nsnps, double* F, uint8_t* minor) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    e.load_bgen(path, mode, maf, standardize, ploidy, nthreads, F, minor);
+    e.loaded_dims(nsamples, nsnps);
+  });
+}
+
+int pcaone_gpu_dense_rows(pcaone_gpu* h, uint32_t r0, uint32_t nrows, float* out) {
+  return guarded(h, [&](pcaone::Engine& e) { e.dense_rows(r0, nrows, out); });
+}
+
+int pcaone_bgen_info(const char* path, uint32_t* nsamples, uint32_t* nvariants, int* layout, int* compression) {
+  try {
+    pcaone::BgenIndex ix;
+    pcaone::bgen_index(path, ix);
+    if (nsamples) *nsamples = ix.nsamples;
+    if (nvariants) *nvariants = ix.nvariants;
+    if (layout) *layout = ix.layout;
+    if (compression) *compression = ix.compression;
+    return PCAONE_OK;
+  } catch (const pcaone::Error& e) {
+    return e.code < 0 ? e.code : PCAONE_EINVAL;
+  } catch (...) {
+    return PCAONE_EINVAL;
+  }
+}
+
 int pcaone_read_csv(const char* path, int nthreads, float* values, uint64_t capacity, uint32_t* nsamples,
                     uint64_t* nfeatures, int64_t* libsize) {
   try {

@@ -12,8 +12,6 @@
 //   scale 4:         count / libsize * scale_factor
 //   scale 0, 1:      count (1: standardised, 0: as is)
 // zstd is the system's libzstd.so.1, opened at run time.
-#include <dlfcn.h>
-
 #include <algorithm>
 #include <cerrno>
 #include <cmath>
@@ -26,42 +24,10 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "zstd_dl.hpp"
 
 namespace pcaone {
 namespace {
-
-// the zstd streaming API (zstd.h, stable since v1.3)
-struct ZInBuf {
-  const void* src;
-  size_t size;
-  size_t pos;
-};
-struct ZOutBuf {
-  void* dst;
-  size_t size;
-  size_t pos;
-};
-struct Zstd {
-  void* so = nullptr;
-  void* (*create)() = nullptr;
-  size_t (*release)(void*) = nullptr;
-  size_t (*stream)(void*, ZOutBuf*, ZInBuf*) = nullptr;
-  unsigned (*is_error)(size_t) = nullptr;
-  const char* (*error_name)(size_t) = nullptr;
-  Zstd() {
-    so = dlopen("libzstd.so.1", RTLD_NOW | RTLD_LOCAL);
-    PCAONE_REQUIRE(so != nullptr, -1, "load_csv: the file is zstd-compressed and libzstd.so.1 is not available");
-    create = reinterpret_cast<void* (*)()>(dlsym(so, "ZSTD_createDCtx"));
-    release = reinterpret_cast<size_t (*)(void*)>(dlsym(so, "ZSTD_freeDCtx"));
-    stream = reinterpret_cast<size_t (*)(void*, ZOutBuf*, ZInBuf*)>(dlsym(so, "ZSTD_decompressStream"));
-    is_error = reinterpret_cast<unsigned (*)(size_t)>(dlsym(so, "ZSTD_isError"));
-    error_name = reinterpret_cast<const char* (*)(size_t)>(dlsym(so, "ZSTD_getErrorName"));
-    PCAONE_REQUIRE(create && release && stream && is_error && error_name, -1, "load_csv: libzstd.so.1 lacks the API");
-  }
-  ~Zstd() {
-    if (so) dlclose(so);
-  }
-};
 
 // std::stoi's reading of a field: leading blanks, sign, digits (none: an error)
 bool field_int(const char* b, const char* e, int64_t& v) {
@@ -180,7 +146,7 @@ void read_csv(const char* path, int nthreads, CsvParser& out) {
       pending.erase(0, used);
     };
     if (zst) {
-      Zstd z;
+      Zstd z("load_csv");
       void* dctx = z.create();
       PCAONE_REQUIRE(dctx != nullptr, -1, "load_csv: ZSTD_createDCtx failed");
       std::vector<char> ob(1 << 22);

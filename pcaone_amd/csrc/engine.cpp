@@ -184,6 +184,15 @@ void Engine::feature_means(double* mu) const {
   std::memcpy(mu, mu_orig_.data(), mu_orig_.size() * sizeof(double));
 }
 
+void Engine::dense_rows(uint32_t r0, uint32_t nr, float* out) {
+  PCAONE_REQUIRE(have_bed_ && src_ == SRC_DENSE && !gl_mode_, -3, "dense_rows: no dense matrix loaded");
+  PCAONE_REQUIRE(out && (uint64_t)r0 + nr <= M_, -1, "dense_rows: rows out of range");
+  PCAONE_HIP(hipSetDevice(device_));
+  PCAONE_HIP(hipMemcpyAsync(out, rawf_.p + (uint64_t)r0 * N_, (size_t)nr * N_ * sizeof(float),
+                            hipMemcpyDeviceToHost, stream_));
+  sync();
+}
+
 void Engine::allele_freq(double* F, int64_t* counts) const {
   PCAONE_REQUIRE(have_bed_ && src_ == SRC_BED, -3, "allele_freq: no genotypes loaded");
   if (F) std::memcpy(F, F_orig_.data(), F_orig_.size() * sizeof(double));

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -109,6 +110,24 @@ std::vector<int2> band_tiles(const int32_t* ws, const int32_t* we, uint32_t nwin
 constexpr uint64_t kLdScratchBytes = 32ull << 20;
 void* ld_scratch(int device);
 
+// A BGEN file mapped read-only with its variant blocks located (bgen.hip):
+// per variant the genotype block's data offset, stored and decompressed
+// lengths and allele count
+struct BgenIndex {
+  void* map = nullptr;
+  size_t len = 0;
+  uint32_t nsamples = 0, nvariants = 0;
+  int layout = 0, compression = 0;
+  std::vector<uint64_t> data;
+  std::vector<uint32_t> clen, dlen;
+  std::vector<uint16_t> nalleles;
+  BgenIndex() = default;
+  BgenIndex(const BgenIndex&) = delete;
+  BgenIndex& operator=(const BgenIndex&) = delete;
+  ~BgenIndex();
+};
+void bgen_index(const char* path, BgenIndex& ix);
+
 // divide_pos_by_window (src/LD.cpp:125-139); returns the number of windows
 uint32_t ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end, uint32_t nchr, int32_t ld_bp,
                     int32_t* ws, int32_t* we);
@@ -126,6 +145,11 @@ class Engine {
   void load_dense(const float* featmajor, uint32_t nsamples, uint32_t nfeatures);
   // FileCsv::read_all: a (zstd-compressed) CSV of counts, --scale on the device
   void load_csv(const char* path, int scale, double scale_factor, int nthreads);
+  // FileBgen: mode 0 minor-allele dosages (MAF filter, centring, optional
+  // standardisation) into the dense path; mode 1 the genotype probabilities as
+  // PCAngsd likelihoods.  F and minor: every variant of the file (optional)
+  void load_bgen(const char* path, int mode, double maf, int standardize, int ploidy, int nthreads, double* F,
+                 uint8_t* minor);
   // genotype likelihoods (PCAngsd): the E matrix is built in HBM from them
   void load_gl(const float* gl, uint32_t nsamples, uint32_t nsites, int per_site);
   // PCAngsd on PLINK input: the loaded genotypes become hard-call likelihoods
@@ -167,6 +191,8 @@ class Engine {
                               int32_t* sums, int config);
   void allele_freq(double* F, int64_t* counts) const;
   void feature_means(double* mu) const;
+  // rows [r0, r0 + nr) of the loaded dense matrix as held in HBM (FP32)
+  void dense_rows(uint32_t r0, uint32_t nr, float* out);
   void load_dense_finish(bool centre);
   void loaded_dims(uint32_t* nsamples, uint32_t* nfeatures) const {
     if (nsamples) *nsamples = N_;

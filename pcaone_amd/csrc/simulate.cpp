@@ -8,10 +8,15 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <zlib.h>
+
+#include <memory>
+#include <string>
 #include <thread>
 #include <vector>
 
 #include "../../include/pcaone_gpu.h"
+#include "zstd_dl.hpp"
 
 namespace {
 
@@ -275,4 +280,237 @@ extern "C" int pcaone_sim_bed(uint64_t seed, uint32_t nsamples, uint32_t nsnps, 
   for (int f : fail)
     if (f) return -3;
   return 0;
+}
+
+// ---- BGEN writer ------------------------------------------------------------
+// Genotype probabilities of Balding-Nichols genotypes in a BGEN v1.2 file
+// (layout 2) or a v1.1 one (layout 1).  The container: u32 offset of the first
+// variant (counted from byte 4), the header block (its length, variant and
+// sample counts, "bgen", flags = compression | layout << 2 | ids << 31), the
+// optional sample identifier block, then per variant its identifiers, position,
+// alleles and genotype block.
+
+namespace {
+
+void put16(std::vector<uint8_t>& o, uint32_t v) {
+  o.push_back((uint8_t)v);
+  o.push_back((uint8_t)(v >> 8));
+}
+void put32(std::vector<uint8_t>& o, uint32_t v) {
+  for (int i = 0; i < 4; ++i) o.push_back((uint8_t)(v >> (8 * i)));
+}
+void put_str16(std::vector<uint8_t>& o, const std::string& s) {
+  put16(o, (uint32_t)s.size());
+  o.insert(o.end(), s.begin(), s.end());
+}
+
+// appends bits [0, nbits) of v to a little-endian bit stream
+struct BitWriter {
+  std::vector<uint8_t>& o;
+  uint64_t acc = 0;
+  int have = 0;
+  explicit BitWriter(std::vector<uint8_t>& out) : o(out) {}
+  void put(uint64_t v, int nbits) {
+    acc |= v << have;
+    have += nbits;
+    while (have >= 8) {
+      o.push_back((uint8_t)acc);
+      acc >>= 8;
+      have -= 8;
+    }
+  }
+  void flush() {
+    if (have > 0) o.push_back((uint8_t)acc);
+    acc = 0;
+    have = 0;
+  }
+};
+
+// probabilities q (summing to 1) quantised to integers summing to scale: floors,
+// then the remainder to the largest fractional parts
+void quantise(const double* q, int n, uint64_t scale, uint64_t* v) {
+  double frac[3];
+  uint64_t sum = 0;
+  for (int i = 0; i < n; ++i) {
+    const double x = q[i] * (double)scale;
+    v[i] = (uint64_t)std::floor(x);
+    frac[i] = x - (double)v[i];
+    sum += v[i];
+  }
+  while (sum < scale) {
+    int best = 0;
+    for (int i = 1; i < n; ++i)
+      if (frac[i] > frac[best]) best = i;
+    ++v[best];
+    frac[best] = -1.0;
+    ++sum;
+  }
+}
+
+}  // namespace
+
+extern "C" int pcaone_sim_bgen(const char* path, uint64_t seed, uint32_t nsamples, uint32_t nsnps, uint32_t npops,
+                               double fst, double pmin, double pmax, double miss_rate, int layout, int compression,
+                               int bits, int pmin_declared, int sample_ids, int nthreads) {
+  if (!path || nsamples == 0 || nsnps == 0 || npops == 0 || npops > nsamples) return -1;
+  if (!(fst >= 0 && fst < 1) || !(pmin > 0 && pmax < 1 && pmin <= pmax) || !(miss_rate >= 0 && miss_rate < 1))
+    return -1;
+  if ((layout != 1 && layout != 2) || compression < 0 || compression > 2 || (layout == 1 && compression == 2))
+    return -1;
+  if (layout == 2 && (bits < 1 || bits > 32)) return -1;
+  std::unique_ptr<pcaone::Zstd> z;
+  try {
+    if (compression == 2) z.reset(new pcaone::Zstd("sim_bgen"));
+  } catch (...) {
+    return -1;
+  }
+  unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  unsigned nt = nthreads > 0 ? (unsigned)nthreads : std::min(hw, 16u);
+  nt = std::max(1u, std::min<unsigned>(nt, nsnps));
+  std::vector<std::vector<uint8_t>> part(nt);
+  std::vector<int> fail(nt, 0);
+  const uint64_t scale = layout == 2 ? ((1ull << bits) - 1) : 32768;
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; ++t) {
+    pool.emplace_back([&, t]() {
+      const uint64_t lo = (uint64_t)nsnps * t / nt, hi = (uint64_t)nsnps * (t + 1) / nt;
+      std::vector<double> pk(npops);
+      std::vector<uint8_t> raw, comp;
+      std::vector<double> q(3ull * nsamples);
+      std::vector<uint8_t> miss(nsamples);
+      auto& o = part[t];
+      for (uint64_t s = lo; s < hi && !fail[t]; ++s) {
+        bool ok = false;
+        for (uint32_t a = 0; a < 1000 && !ok; ++a) {  // redraw monomorphic variants
+          Xoshiro256 rng(snp_seed(seed ^ 0xB6E0A1F4C3D2E5F7ull, s, a));
+          const double p = pmin + (pmax - pmin) * rng.uniform();
+          for (uint32_t k = 0; k < npops; ++k) {
+            pk[k] = p;
+            if (fst > 0) pk[k] = rng.beta(p * (1.0 - fst) / fst, (1.0 - p) * (1.0 - fst) / fst);
+          }
+          uint64_t gsum = 0, nobs = 0;
+          for (uint32_t j = 0; j < nsamples; ++j) {
+            const double f = pk[(uint64_t)j * npops / nsamples];
+            const int g = (rng.uniform() < f) + (rng.uniform() < f);  // copies of the modelled allele
+            double* qq = &q[3ull * j];
+            qq[0] = qq[1] = qq[2] = 0.0;
+            if (rng.uniform() < 0.6) {
+              qq[g] = 1.0;
+            } else {
+              const double w = 0.5 + 0.5 * rng.uniform(), r = rng.uniform();
+              qq[g] = w;
+              qq[(g + 1) % 3] = (1.0 - w) * r;
+              qq[(g + 2) % 3] = (1.0 - w) * (1.0 - r);
+            }
+            miss[j] = rng.uniform() < miss_rate;
+            if (!miss[j]) {
+              gsum += g;
+              ++nobs;
+            }
+          }
+          ok = nobs > 0 && gsum > 0 && gsum < 2 * nobs;
+        }
+        if (!ok) {
+          fail[t] = 1;
+          break;
+        }
+        // genotype order (AA, AB, BB) of the first allele A: the modelled one
+        // for even variants, the other for odd ones
+        const bool first_modelled = (s % 2) == 0;
+        raw.clear();
+        if (layout == 2) {
+          put32(raw, nsamples);
+          put16(raw, 2);
+          raw.push_back((uint8_t)(pmin_declared ? 1 : 2));
+          raw.push_back(2);
+          for (uint32_t j = 0; j < nsamples; ++j) raw.push_back((uint8_t)(miss[j] ? 0x82 : 0x02));
+          raw.push_back(0);  // unphased
+          raw.push_back((uint8_t)bits);
+          BitWriter bw(raw);
+          for (uint32_t j = 0; j < nsamples; ++j) {
+            const double* qq = &q[3ull * j];
+            const double g3[3] = {first_modelled ? qq[2] : qq[0], qq[1], first_modelled ? qq[0] : qq[2]};
+            uint64_t v[3] = {0, 0, 0};
+            if (!miss[j]) quantise(g3, 3, scale, v);
+            bw.put(v[0], bits);
+            bw.put(v[1], bits);
+          }
+          bw.flush();
+        } else {
+          for (uint32_t j = 0; j < nsamples; ++j) {
+            const double* qq = &q[3ull * j];
+            const double g3[3] = {first_modelled ? qq[2] : qq[0], qq[1], first_modelled ? qq[0] : qq[2]};
+            for (int i = 0; i < 3; ++i) put16(raw, miss[j] ? 0u : (uint32_t)std::lround(g3[i] * 32768.0));
+          }
+        }
+        // variant identifying data
+        if (layout == 1) put32(o, nsamples);
+        put_str16(o, "snp" + std::to_string(s + 1));
+        put_str16(o, "rs" + std::to_string(s + 1));
+        put_str16(o, "1");
+        put32(o, (uint32_t)(1000 * (s + 1)));
+        if (layout == 2) put16(o, 2);
+        for (const char* al : {"A", "G"}) {
+          put32(o, 1);
+          o.push_back((uint8_t)al[0]);
+        }
+        // genotype block
+        if (compression == 0) {
+          if (layout == 2) put32(o, (uint32_t)raw.size());
+          o.insert(o.end(), raw.begin(), raw.end());
+          continue;
+        }
+        size_t clen;
+        if (compression == 1) {
+          uLongf cap = compressBound(raw.size());
+          comp.resize(cap);
+          if (compress2(comp.data(), &cap, raw.data(), raw.size(), 1) != Z_OK) {
+            fail[t] = 1;
+            break;
+          }
+          clen = cap;
+        } else {
+          comp.resize(z->bound(raw.size()));
+          clen = z->compress(comp.data(), comp.size(), raw.data(), raw.size(), 1);
+          if (z->is_error(clen)) {
+            fail[t] = 1;
+            break;
+          }
+        }
+        if (layout == 2) {
+          put32(o, (uint32_t)(clen + 4));
+          put32(o, (uint32_t)raw.size());
+        } else {
+          put32(o, (uint32_t)clen);
+        }
+        o.insert(o.end(), comp.begin(), comp.begin() + clen);
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (int f : fail)
+    if (f) return -3;
+  std::vector<uint8_t> head;
+  std::vector<uint8_t> ids;
+  if (sample_ids) {
+    std::vector<uint8_t> body;
+    for (uint32_t j = 0; j < nsamples; ++j) put_str16(body, "ind" + std::to_string(j + 1));
+    put32(ids, (uint32_t)(8 + body.size()));
+    put32(ids, nsamples);
+    ids.insert(ids.end(), body.begin(), body.end());
+  }
+  const uint32_t hlen = 20;
+  put32(head, hlen + (uint32_t)ids.size());
+  put32(head, hlen);
+  put32(head, nsnps);
+  put32(head, nsamples);
+  for (char c : std::string("bgen")) head.push_back((uint8_t)c);
+  put32(head, (uint32_t)compression | ((uint32_t)layout << 2) | (sample_ids ? 0x80000000u : 0u));
+  head.insert(head.end(), ids.begin(), ids.end());
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) return -1;
+  bool good = std::fwrite(head.data(), 1, head.size(), fp) == head.size();
+  for (auto& pv : part) good = good && std::fwrite(pv.data(), 1, pv.size(), fp) == pv.size();
+  good = (std::fclose(fp) == 0) && good;
+  return good ? 0 : -1;
 }

@@ -13,14 +13,14 @@ def parse_args(args) -> dict:
     (src/Cmd.hpp:28-50) and oversamples = max(os, k) (src/Cmd.cpp:190)."""
     a = shlex.split(str(args))
     d = {"svd": 2, "k": 10, "os": 10, "p": 20, "tol": 1e-4, "seed": 101, "rand": 1, "w": 64, "S": False,
-         "scale": 1, "m": 0.0, "emu": False, "pcangsd": False}
+         "scale": 1, "m": 0.0, "emu": False, "pcangsd": False, "ld": False, "ld_stats": 0}
     i = 0
     flags = {"--svd": ("svd", int), "-k": ("k", int), "--oversamples": ("os", int), "-p": ("p", int),
              "--tol-rsvd": ("tol", float), "--seed": ("seed", int), "--rand": ("rand", int), "-w": ("w", int),
-             "--scale": ("scale", int), "-m": ("m", float)}
+             "--scale": ("scale", int), "-m": ("m", float), "--ld-stats": ("ld_stats", int)}
     while i < len(a):
-        if a[i] in ("-S", "--emu", "--pcangsd"):
-            d[{"-S": "S", "--emu": "emu", "--pcangsd": "pcangsd"}[a[i]]] = True
+        if a[i] in ("-S", "--emu", "--pcangsd", "--ld"):
+            d[{"-S": "S", "--emu": "emu", "--pcangsd": "pcangsd", "--ld": "ld"}[a[i]]] = True
             i += 1
         else:
             key, typ = flags[a[i]]
@@ -37,6 +37,23 @@ def oracle_matrix(payload, n, m, standardize=True):
     return O.standardize(X, F) if standardize else X
 
 
+def bgen_matrix(z: dict, d: dict):
+    """FileBgen::read_all's matrix of a BGEN fixture; standardised unless
+    --scale 0 or --ld (which runs the operator unstandardised,
+    src/Halko.cpp:285-289)."""
+    import os
+    import tempfile
+
+    fd, path = tempfile.mkstemp(suffix=".bgen")
+    try:
+        os.write(fd, np.asarray(z["bgen"], dtype=np.uint8).tobytes())
+        os.close(fd)
+        X, _, _, _ = O.bgen_read(path, standardize=bool(d["scale"]) and not d["ld"])
+    finally:
+        os.unlink(path)
+    return X
+
+
 def oracle_run(z: dict):
     d = parse_args(z["args"])
     n, m = int(z["nsamples"]), int(z["nsnps"])
@@ -49,8 +66,8 @@ def oracle_run(z: dict):
         orc = O.RsvdOracle(X, d["k"], d["os"], fancy=d["svd"] == 2, bands=d["w"], seed=d["seed"], rand=d["rand"],
                            perm=not d["S"])
         return orc, orc.compute_usv(d["p"], d["tol"]), d
-    if "dense" in z:
-        X = O.dense_centered(z["dense"])
+    if "dense" in z or "bgen" in z:
+        X = bgen_matrix(z, d) if "bgen" in z else O.dense_centered(z["dense"])
         if d["svd"] == 0:  # IRAM: the converged top-k singular triplets
             u, s, vt = np.linalg.svd(X, full_matrices=False)
             return O.ExactSvd(u[:, :d["k"]], s[:d["k"]], vt[:d["k"]].T, m), None, d

@@ -237,6 +237,28 @@ def csv_counts(seed, ncells, ngenes):
     return rng.poisson(rng.gamma(2.0, lam / 2.0)).astype(np.int64)
 
 
+# BGEN inputs (pcaone_sim_bgen): name -> (seed, N, M, npops, fst, miss, layout, compression, bits,
+# declared minimum ploidy, sample identifier block)
+BGEN = {
+    "bg8": (81, 230, 1000, 3, 0.1, 0.02, 2, 1, 8, 2, True),      # zlib, the 8-bit byte-pair path
+    "bg16z": (82, 150, 600, 3, 0.1, 0.02, 2, 2, 16, 1, False),   # zstd, 16 bits, ploidy range 1-2
+    "bg5": (83, 205, 500, 4, 0.08, 0.0, 2, 0, 5, 2, False),      # uncompressed, 5 bits
+    "bg1": (84, 180, 500, 3, 0.1, 0.02, 1, 1, 16, 2, False),     # layout 1 (v1.1), zlib
+}
+# No --pcangsd case: the reference's BGEN reader writes the probabilities into a
+# P it never sizes (src/FileBgen.cpp:72-80) and the CLI ends with SIGSEGV.
+BGEN_CASES = {
+    "bgen_svd1_8": ("bg8", "--svd 1 -k 4 --oversamples 6 -p 5 --tol-rsvd 0"),
+    "bgen_ld1_8": ("bg8", "--svd 1 -k 3 --oversamples 7 -p 1 --tol-rsvd 0 --ld --ld-stats 1"),
+    "bgen_svd1_16z": ("bg16z", "--svd 1 -k 3 --oversamples 7 -p 6 --tol-rsvd 0"),
+    "bgen_ld1_16z": ("bg16z", "--svd 1 -k 3 --oversamples 7 -p 1 --tol-rsvd 0 --ld --ld-stats 1"),
+    "bgen_svd2_5": ("bg5", "--svd 2 -k 3 --oversamples 7 -w 8 -S"),
+    "bgen_ld1_5": ("bg5", "--svd 1 -k 3 --oversamples 7 -p 1 --tol-rsvd 0 --ld --ld-stats 1"),
+    "bgen_svd0_l1": ("bg1", "--svd 0 -k 4"),
+    "bgen_ld1_l1": ("bg1", "--svd 1 -k 3 --oversamples 7 -p 1 --tol-rsvd 0 --ld --ld-stats 1"),
+    "bgen_nostd_8": ("bg8", "--svd 1 -k 3 --oversamples 7 -p 4 --tol-rsvd 0 --scale 0"),
+}
+
 LD_TOOLS = {"ld_tools_bp3000": (71, 157, (230, 170), 3000)}
 # --project 1/2 with --USV of a reference PCA: name -> (seed, nref, ntarget, nsnps)
 PROJECT = {"project_p12": (61, 220, 90, 1500)}
@@ -500,6 +522,20 @@ def main():
             res = run_ld_stats1(tmp, name, seed, n, per_chr, ld_bp)
             np.savez_compressed(OUT / f"{name}.npz", **res)
             print(name, len(res["r2"]), "pairs", int(np.isnan(res["r2"]).sum()), "NaN")
+        for name, (seed, n, m, npops, fst, miss, lay, comp, bits, pmin, ids) in BGEN.items():
+            path = tmp / (name + ".bgen")
+            pa.sim_bgen(path, seed, n, m, npops, fst, miss=miss, layout=lay, compression=comp, bits=bits,
+                        pmin_declared=pmin, sample_ids=ids)
+            raw = np.frombuffer(path.read_bytes(), dtype=np.uint8)
+            for case, (ds, args) in BGEN_CASES.items():
+                if ds != name or not want(case):
+                    continue
+                res = run_case(tmp, name + ".bgen", args, case, flag="-g")
+                if "--ld" in args.split():
+                    # the centred (unstandardised) matrix the reference writes as float32
+                    res["residuals"] = np.fromfile(tmp / (case + ".residuals"), dtype=np.uint8)
+                np.savez_compressed(OUT / f"{case}.npz", bgen=raw, nsamples=n, nsnps=m, args=args, **res)
+                print(case, res["eigvals"])
         for name, (seed, n, m, base) in DENSE.items():
             D = pa.sim_counts(seed, n, m, base=base)
             pa.write_bin(tmp / (name + ".bin"), D)

@@ -94,6 +94,11 @@ def test_usv_matches_reference_cli(gpu, case, arm, tmp_path):
         assert (gpu.N, gpu.M) == (n, m)
     elif "dense" in z:
         gpu.load_dense(z["dense"])
+    elif "bgen" in z:  # the reference's -g input through the library's BGEN reader
+        path = tmp_path / "x.bgen"
+        path.write_bytes(np.asarray(z["bgen"], dtype=np.uint8).tobytes())
+        gpu.load_bgen(path, standardize=bool(d["scale"]) and not d["ld"])
+        assert (gpu.N, gpu.M) == (n, m)
     else:
         gpu.load_bed(z["payload"], n, m)
     ooc = {}
@@ -127,15 +132,17 @@ def test_usv_matches_reference_cli(gpu, case, arm, tmp_path):
     # and with the oracle on the same inputs, epoch for epoch
     orc, epo, _ = oracle_run(z)
     if arm == "fp64":
-        # a CSV's transformed values are held in FP32 on the device (as the
-        # reference's own .bin conversion of a CSV holds them); the oracle's in FP64
-        tol = 2e-7 if csv else EIG_TOL[arm]
+        # a CSV's transformed values and BGEN's centred dosages are held in FP32
+        # on the device (as the reference's own .bin conversion of a CSV holds
+        # them); the oracle's in FP64
+        held32 = csv or "bgen" in z
+        tol = 2e-7 if held32 else EIG_TOL[arm]
         assert ep == epo
         assert eig_relerr(ev, orc.eigvals()) < tol
-        assert O.principal_angles_max(U, orc.U) < (1e-5 if csv else 1e-6)
+        assert O.principal_angles_max(U, orc.U) < (1e-5 if held32 else 1e-6)
         # 1 - mev per epoch (convergence decisions follow it)
         n_d = min(len(orc.diffs), d["p"])
-        assert np.allclose(diffs[1:1 + n_d], orc.diffs[:n_d], rtol=1e-5 if csv else 1e-8, atol=1e-12)
+        assert np.allclose(diffs[1:1 + n_d], orc.diffs[:n_d], rtol=1e-5 if held32 else 1e-8, atol=1e-12)
 
 
 @pytest.mark.parametrize("arm", list(ARMS))
