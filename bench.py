@@ -557,6 +557,28 @@ def run_c3(g, pa, args):
                            "traffic": None if args.small else TRAFFIC_C3,
                            "kernel": "k_gl_E (E update: two FP32 likelihoods in, FP32 E out)",
                            "launches": em[0], "avg_launch_us": round(em[1] * 1e3 / em[0], 1)}
+    # --project of the same likelihoods onto the FP64 arm's PCA (the last arm
+    # run: its V; every site matched, F the emMAF frequencies): the EM of
+    # run_projection_beagle_gl
+    g.profiling(True)
+    g.reset_stats()
+    t0 = time.perf_counter()
+    Up, _, pit, _ = g.project_gl_matched(np.arange(m), np.zeros(m, np.uint8), F, S64, V, maxiter=100, tolem=1e-5)
+    tp = time.perf_counter() - t0
+    pst = g.kernel_stats()
+    g.profiling(False)
+    es = pst["projgl_estep"]
+    pbytes = es[0] * 8.0 * n * m  # two FP32 likelihoods per sample and site per E step
+    out["projection"] = {
+        "what": "--project of the same likelihoods onto this PCA (run_projection_beagle_gl: device E step, "
+                "per-sample least squares, host sign alignment)", "value_s": round(tp, 4), "iterations": pit,
+        # the projection's least squares and PCAngsd's standardised E are different estimators
+        "angle_top3_vs_pcangsd_U_rad": principal_angle(Up[:, :3], U64[:, :3]),
+        "roofline": {"bound": "hbm", "achieved": round(pbytes / (es[1] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(pbytes / (es[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "kernel": "k_projgl_estep", "launches": es[0], "avg_launch_us": round(es[1] * 1e3 / es[0], 1)},
+        "kernel_ms": {k: round(v[1], 2) for k, v in sorted(pst.items())}}
+    log(f"C3 projection: {tp:.3f} s, {pit} iterations")
     if not args.no_cpu:
         out["cpu_baseline"] = c3_cpu_baseline(pa, gl, CPU_THREADS)
     return out

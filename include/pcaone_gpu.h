@@ -245,6 +245,32 @@ int pcaone_read_csv(const char* path, int nthreads, float* values, uint64_t capa
 int pcaone_gpu_project(pcaone_gpu* h, const char* filebim, const char* filembim, const double* S,
                        const double* Vref, uint32_t k, uint32_t ref_nsnps, int method, int standardize, double* U,
                        double* W, uint32_t* noverlap);
+/* run_projection_beagle_gl (src/Projection.cpp:32-137) on the likelihoods of
+ * pcaone_gpu_load_gl (the BEAGLE body of `filebeagle`), with FileBeagle's
+ * matching of its sites against the reference .mbim (src/FileBeagle.cpp:
+ * 12-118: by ID, else chromosome_position; alleles 0-3 read as A, C, G, T;
+ * equal alleles F = the .mbim frequency, swapped F = 1 - it and the loading
+ * row negated).  S (k) and Vref (ref_nsnps x k, column-major) are the
+ * reference's .sigvals and .loadings.  E step on the device (posterior
+ * genotype under pt = F, then pt = (U_i S) L_j (times sqrt(2F(1-F)) when
+ * standardize) + F, clamped to [1e-4, 1 - 1e-4]; standardised when
+ * standardize), least squares of every sample on V S over its observed sites
+ * (solve_projection_scores, normal equations by Cholesky), columns aligned
+ * in sign with the previous scores, until |U - Uprev| / |Uprev| < tolem (at
+ * most maxiter).  U: nsamples x k column-major; VS (optional, nsites x k
+ * capacity): the matched rows of V S column-major, the loadings the
+ * reference writes; noverlap: matched sites; iters, diffs (maxiter entries):
+ * the iterations and their relative changes. */
+int pcaone_gpu_project_gl(pcaone_gpu* h, const char* filebeagle, const char* filembim, const double* S,
+                          const double* Vref, uint32_t k, uint32_t ref_nsnps, int standardize, int maxiter,
+                          double tolem, double* U, double* VS, uint32_t* noverlap, int* iters, double* diffs);
+/* The same EM with the matching given, as FileBeagle::read_all leaves it in
+ * Data (project_ref_indices, project_flip, F; src/Data.hpp:56-60): per loaded
+ * site ref_idx (-1 unmatched), flip (0/1) and F. */
+int pcaone_gpu_project_gl_matched(pcaone_gpu* h, const int32_t* ref_idx, const uint8_t* flip, const double* F,
+                                  const double* S, const double* Vref, uint32_t k, uint32_t ref_nsnps,
+                                  int standardize, int maxiter, double tolem, double* U, double* VS,
+                                  uint32_t* noverlap, int* iters, double* diffs);
 
 /* ---- inbreeding / HWE (--inbreed 1 with --USV) ---------------------------- */
 /* run_inbreeding_em (src/Inbreeding.cpp:16-190) on the loaded PLINK hard

@@ -1187,3 +1187,116 @@ def bgen_probs(path) -> np.ndarray:
                 P[j, :, 1] = _bits(pr, np.uint64(2 * B) * n + np.uint64(B), B).astype(np.float32) * fac
         P[j, miss] = np.nan
     return P
+
+
+# ----------------------------------------------------------------------------
+# Projection of BEAGLE genotype likelihoods (run_projection_beagle_gl,
+# src/Projection.cpp:32-137, with FileBeagle::read_all's matching,
+# src/FileBeagle.cpp:12-118, and solve_projection_scores, :11-30)
+# ----------------------------------------------------------------------------
+_BEAGLE_ALLELE = {"0": "A", "1": "C", "2": "G", "3": "T"}
+
+
+def beagle_mbim_match(sites, mbim_lines):
+    """sites: (marker, allele1, allele2) per BEAGLE line, alleles 0-3 read as
+    A, C, G, T.  A site finds its .mbim line by ID (column 2), else by
+    chromosome_position (columns 1 and 4), the first line of a key winning;
+    alleles equal give F = the .mbim frequency, swapped give F = 1 - it and a
+    flip; anything else is unmatched.  Returns (ref_idx, -1 unmatched; flip;
+    F) per site."""
+    by_id, by_pos = {}, {}
+    for idx, line in enumerate(mbim_lines):
+        t = line.split("\t")
+        assert len(t) == 7
+        site = (idx, t[4], t[5], float(t[6]))
+        by_id.setdefault(t[1], site)
+        by_pos.setdefault(t[0] + "_" + t[3], site)
+    ref_idx = np.full(len(sites), -1, dtype=np.int64)
+    flip = np.zeros(len(sites), dtype=bool)
+    F = np.zeros(len(sites))
+    for j, (marker, a1, a2) in enumerate(sites):
+        a1, a2 = _BEAGLE_ALLELE.get(a1, a1), _BEAGLE_ALLELE.get(a2, a2)
+        r = by_id.get(marker)
+        if r is None:
+            r = by_pos.get(marker)
+        if r is None:
+            continue
+        if a1 == r[1] and a2 == r[2]:
+            ref_idx[j], F[j] = r[0], r[3]
+        elif a1 == r[2] and a2 == r[1]:
+            ref_idx[j], F[j], flip[j] = r[0], 1.0 - r[3], True
+    return ref_idx, flip, F
+
+
+def project_beagle_gl(P0, P1, ref_idx, flip, F, S, V, standardize: bool = True, maxiter: int = 100,
+                      tolem: float = 1e-5):
+    """P0, P1 (N x M): the first two likelihoods of every site; ref_idx, flip,
+    F from beagle_mbim_match; S (k), V (reference SNPs x k).  On the matched
+    sites L = +-V rows, VS = L S.  E step: pt = F (initial) or
+    clamp((U_i S) L_j (times sqrt(2F(1-F)) when standardising) + F), both
+    clamped to [1e-4, 1 - 1e-4]; posterior p11 = gl11 pt^2, p12 = gl12 2pt(1-pt),
+    p22 = (1 - gl11 - gl12)(1-pt)^2; a non-finite or non-positive sum marks
+    the entry missing (0); else g = (2 p11 + p12) / (2 sum) - F, divided by
+    sqrt(2F(1-F)) when standardising and that exceeds 1e-9 (in the iterations
+    such sites become 0, in the initial step they stay).  Each sample's scores
+    are the least-squares fit of g on VS over its observed sites; after each
+    iteration a column whose dot product with the previous is negative is
+    negated, and the loop stops when |U - Uprev|_F / |Uprev|_F < tolem.
+    Returns (U, VS, iterations, diffs)."""
+    m_ = ref_idx >= 0
+    L = np.asarray(V, dtype=np.float64)[ref_idx[m_]] * np.where(flip[m_], -1.0, 1.0)[:, None]
+    S = np.asarray(S, dtype=np.float64)
+    VS = L * S[None, :]
+    Fm = np.asarray(F, dtype=np.float64)[m_]
+    g11 = np.asarray(P0, dtype=np.float64)[:, m_]
+    g12 = np.asarray(P1, dtype=np.float64)[:, m_]
+    g22 = 1.0 - g11 - g12
+    norm = np.sqrt(2.0 * Fm * (1.0 - Fm))
+    n, k = g11.shape[0], S.size
+
+    def estep(U, init):
+        if init:
+            pt = np.broadcast_to(np.minimum(np.maximum(Fm, 1e-4), 1.0 - 1e-4), g11.shape)
+        else:
+            z = (U * S[None, :]) @ L.T
+            c = z * norm[None, :] if standardize else z
+            pt = np.minimum(np.maximum(c + Fm[None, :], 1e-4), 1.0 - 1e-4)
+        with np.errstate(invalid="ignore"):
+            p11 = g11 * pt * pt
+            p12 = g12 * 2.0 * pt * (1.0 - pt)
+            p22 = g22 * (1.0 - pt) * (1.0 - pt)
+            ps = p11 + p12 + p22
+            miss = ~np.isfinite(ps) | (ps <= 0.0)
+            G = (2.0 * p11 + p12) / (2.0 * ps) - Fm[None, :]
+        G[miss] = 0.0
+        if standardize:
+            ok = norm > VAR_TOL
+            G[:, ok] /= norm[None, ok]
+            if not init:
+                G[:, ~ok] = 0.0
+            G[miss] = 0.0
+        return G, miss
+
+    def solve(G, miss):
+        U = np.empty((n, k))
+        for i in range(n):
+            o = ~miss[i]
+            U[i] = np.linalg.lstsq(VS[o], G[i, o], rcond=None)[0]
+        return U
+
+    G, miss = estep(None, True)
+    U = solve(G, miss)
+    diffs, it = [], 0
+    for it in range(1, maxiter + 1):
+        Up = U.copy()
+        G, miss = estep(U, False)
+        U = solve(G, miss)
+        neg = np.sum(U * Up, axis=0) < 0.0
+        U[:, neg] *= -1.0
+        den = np.linalg.norm(Up)
+        if den < 1e-12:
+            den = 1.0
+        diffs.append(float(np.linalg.norm(U - Up) / den))
+        if diffs[-1] < tolem:
+            break
+    return U, VS, it, diffs

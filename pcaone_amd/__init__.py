@@ -72,6 +72,12 @@ SIGNATURES = {
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "pcaone_gpu_load_bgen": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _D, _U8]),
+    "pcaone_gpu_project_gl": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, _D, _D, C.c_uint32, C.c_uint32,
+                                        C.c_int, C.c_int, C.c_double, _D, _D, C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_int), _D]),
+    "pcaone_gpu_project_gl_matched": (C.c_int, [C.c_void_p, _I32, _U8, _D, _D, _D, C.c_uint32, C.c_uint32,
+                                                C.c_int, C.c_int, C.c_double, _D, _D, C.POINTER(C.c_uint32),
+                                                C.POINTER(C.c_int), _D]),
     "pcaone_gpu_dense_rows": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     "pcaone_bgen_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int)]),
@@ -557,6 +563,45 @@ class Gpu:
         n = nm.value
         Wm = np.asfortranarray(W.reshape(-1, order="F")[: n * k].reshape((n, k), order="F"))
         return U, Wm
+
+    def project_gl(self, beagle, mbim, S, Vref, standardize: bool = True, maxiter: int = 100,
+                   tolem: float = 1e-5):
+        """--project on the loaded BEAGLE likelihoods: (U (N, k), VS (noverlap, k),
+        iterations, diffs)."""
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        Vref = np.asfortranarray(Vref, dtype=np.float64)
+        k = S.size
+        U = np.empty((self.N, k), order="F")
+        VS = np.empty(self.M * k)
+        diffs = np.empty(max(maxiter, 1))
+        nm, it = C.c_uint32(0), C.c_int(0)
+        self._check(self._L.pcaone_gpu_project_gl(self.h, str(beagle).encode(), str(mbim).encode(), _p(S, C.c_double),
+                                                   _p(Vref, C.c_double), k, Vref.shape[0], int(standardize),
+                                                   int(maxiter), float(tolem), _p(U, C.c_double),
+                                                   _p(VS, C.c_double), C.byref(nm), C.byref(it),
+                                                   _p(diffs, C.c_double)))
+        W = VS[:nm.value * k].reshape(k, nm.value).T.copy()
+        return U, W, it.value, diffs[:it.value]
+
+    def project_gl_matched(self, ref_idx, flip, F, S, Vref, standardize: bool = True, maxiter: int = 100,
+                           tolem: float = 1e-5):
+        """The likelihood projection with the site matching given (per loaded site)."""
+        ref_idx = np.ascontiguousarray(ref_idx, dtype=np.int32)
+        flip = np.ascontiguousarray(flip, dtype=np.uint8)
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        Vref = np.asfortranarray(Vref, dtype=np.float64)
+        k = S.size
+        U = np.empty((self.N, k), order="F")
+        VS = np.empty(self.M * k)
+        diffs = np.empty(max(maxiter, 1))
+        nm, it = C.c_uint32(0), C.c_int(0)
+        self._check(self._L.pcaone_gpu_project_gl_matched(
+            self.h, _p(ref_idx, C.c_int32), _p(flip, C.c_uint8), _p(F, C.c_double), _p(S, C.c_double),
+            _p(Vref, C.c_double), k, Vref.shape[0], int(standardize), int(maxiter), float(tolem), _p(U, C.c_double),
+            _p(VS, C.c_double), C.byref(nm), C.byref(it), _p(diffs, C.c_double)))
+        W = VS[:nm.value * k].reshape(k, nm.value).T.copy()
+        return U, W, it.value, diffs[:it.value]
 
     def inbreeding(self, U, S, V, F, maxiter: int = 100, tolem: float = 1e-5):
         """--inbreed 1 on the loaded PLINK hard calls: (coef, lrt, hwe_p, iterations)."""

@@ -331,6 +331,27 @@ int pcaone_gpu_load_bgen(pcaone_gpu* h, const char* path, int mode, double maf, 
   });
 }
 
+int pcaone_gpu_project_gl(pcaone_gpu* h, const char* filebeagle, const char* filembim, const double* S,
+                          const double* Vref, uint32_t k, uint32_t ref_nsnps, int standardize, int maxiter,
+                          double tolem, double* U, double* VS, uint32_t* noverlap, int* iters, double* diffs) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    const uint32_t nm =
+        e.project_gl(filebeagle, filembim, S, Vref, k, ref_nsnps, standardize, maxiter, tolem, U, VS, iters, diffs);
+    if (noverlap) *noverlap = nm;
+  });
+}
+
+int pcaone_gpu_project_gl_matched(pcaone_gpu* h, const int32_t* ref_idx, const uint8_t* flip, const double* F,
+                                  const double* S, const double* Vref, uint32_t k, uint32_t ref_nsnps,
+                                  int standardize, int maxiter, double tolem, double* U, double* VS,
+                                  uint32_t* noverlap, int* iters, double* diffs) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    const uint32_t nm = e.project_gl_matched(ref_idx, flip, F, S, Vref, k, ref_nsnps, standardize, maxiter, tolem, U,
+                                             VS, iters, diffs);
+    if (noverlap) *noverlap = nm;
+  });
+}
+
 int pcaone_gpu_dense_rows(pcaone_gpu* h, uint32_t r0, uint32_t nrows, float* out) {
   return guarded(h, [&](pcaone::Engine& e) { e.dense_rows(r0, nrows, out); });
 }

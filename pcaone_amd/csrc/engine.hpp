@@ -168,6 +168,14 @@ class Engine {
   // projection of the loaded PLINK samples onto a reference PCA; returns the matched SNP count
   uint32_t project(const char* filebim, const char* filembim, const double* S, const double* Vref, uint32_t k,
                    uint32_t ref_nsnps, int method, int standardize, double* U, double* Wout);
+  // projection of the loaded BEAGLE likelihoods onto a reference PCA (the EM of
+  // run_projection_beagle_gl); returns the matched site count
+  uint32_t project_gl(const char* filebeagle, const char* filembim, const double* S, const double* Vref, uint32_t k,
+                      uint32_t ref_nsnps, int standardize, int maxiter, double tolem, double* U, double* VSout,
+                      int* iters, double* diffs);
+  uint32_t project_gl_matched(const int32_t* ref_idx, const uint8_t* flip, const double* F, const double* S,
+                              const double* Vref, uint32_t k, uint32_t ref_nsnps, int standardize, int maxiter,
+                              double tolem, double* U, double* VSout, int* iters, double* diffs);
   // per-site inbreeding coefficients and HWE LRT on the loaded PLINK hard calls
   int inbreeding(const double* U, const double* S, const double* V, uint32_t k, const double* F, int maxiter,
                  double tolem, double* coef, double* lrt, double* hwe_p);

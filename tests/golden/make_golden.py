@@ -13,6 +13,7 @@ OpenMP scheduling.
 """
 from __future__ import annotations
 
+import gzip
 import re
 import subprocess
 import sys
@@ -324,6 +325,84 @@ def run_project(tmp: Path, name: str, seed, nref, ntgt, m):
     return res
 
 
+# BEAGLE likelihood projection onto a PLINK reference PCA: name -> (seed, nref, ntarget, nsnps)
+PROJECT_GL = {"project_gl": (63, 220, 90, 1200)}
+
+
+def run_project_gl(tmp: Path, name: str, seed, nref, ntgt, m):
+    """A reference PCA (-V) on a PLINK panel, then the other samples of the same
+    simulation as BEAGLE likelihoods (read depth ~ Poisson(4), 1% error) on
+    most of its SNPs: markers by ID or as chromosome_position, alleles as
+    letters or as 0-3 codes, some swapped (alleles and triples reversed), a
+    few likelihood triples NaN (missing), some sites unmatched (an unknown
+    marker, a third allele); then -G --USV --project 2, which
+    runs the likelihood EM (run_projection_beagle_gl)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pcaone_oracle as O
+
+    n = nref + ntgt
+    pay = pa.sim_bed(seed, n, m, 3, 0.1, 0.05, 0.5, 0.02)
+    codes = O.decode_codes(pay, n, m)
+    rng = np.random.default_rng(seed)
+    pos = (np.arange(m) + 1) * 100
+    pa.write_plink(tmp / (name + "_ref"), _pack_codes(codes[:, :nref]), nref, m, None, pos)
+    keep = rng.random(m) > 0.05
+    swap = (rng.random(m) < 0.1) & keep
+    by_pos = rng.random(m) < 0.2
+    numeric = rng.random(m) < 0.5
+    ncopy = np.select([codes == 0, codes == 2, codes == 3], [2, 1, 0], -1)  # copies of A (allele 1), -1 missing
+    rows, sites = [], []
+    for i in np.flatnonzero(keep):
+        gl = np.empty((ntgt, 3))
+        for j in range(ntgt):
+            g = ncopy[i, nref + j]
+            d = rng.poisson(4.0) if g >= 0 else 0
+            lik = np.ones(3)
+            for _ in range(d):
+                b = rng.random() < g / 2.0  # the read carries A
+                if rng.random() < 0.01:
+                    b = not b
+                for h, q in enumerate((1.0, 0.5, 0.0)):  # genotypes AA, AG, GG: fraction of A
+                    lik[h] *= q * 0.99 + (1 - q) * 0.01 if b else (1 - q) * 0.99 + q * 0.01
+            gl[j] = lik / lik.sum()
+        a1, a2 = ("2", "0") if numeric[i] else ("G", "A")
+        if not swap[i]:
+            a1, a2 = a2, a1
+        else:
+            gl = gl[:, ::-1]
+        gl = gl.astype(np.float32)
+        gl[rng.random(ntgt) < 0.003] = np.nan
+        marker = f"1_{pos[i]}" if by_pos[i] else f"rs{i}"
+        if rng.random() < 0.02:  # unknown marker: unmatched
+            marker = f"x{i}"
+        if rng.random() < 0.03:  # another allele: unmatched
+            a2 = "1" if numeric[i] else "C"
+        rows.append(gl)
+        sites.append((marker, a1, a2))
+    glm = np.stack(rows)  # sites x samples x 3
+    with gzip.open(tmp / (name + "_tgt.beagle.gz"), "wt") as f:
+        f.write("marker\tallele1\tallele2\t" + "\t".join(f"t{j}\tt{j}\tt{j}" for j in range(ntgt)) + "\n")
+        for (mk, a1, a2), gl in zip(sites, glm):
+            f.write(f"{mk}\t{a1}\t{a2}\t" + "\t".join(repr(float(v)) for v in gl.reshape(-1)) + "\n")
+
+    def run(*args):
+        subprocess.run([str(REF), *args, "-n", "1", "-v", "0"], check=True, cwd=tmp, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+
+    run("-b", name + "_ref", "--svd", "1", "-k", "3", "-p", "6", "--tol-rsvd", "0", "-V", "-o", name + "_pca")
+    run("-G", name + "_tgt.beagle.gz", "--USV", name + "_pca", "--project", "2", "-o", name + "_out")
+    log = (tmp / (name + "_out.log")).read_text()
+    return dict(gl=glm, sites=np.array(["\t".join(s) for s in sites]),
+                beagle_gz=np.frombuffer((tmp / (name + "_tgt.beagle.gz")).read_bytes(), dtype=np.uint8),
+                mbim=(tmp / (name + "_pca.mbim")).read_text(),
+                S=np.loadtxt(tmp / (name + "_pca.sigvals"), ndmin=1, skiprows=1),
+                V=np.loadtxt(tmp / (name + "_pca.loadings"), ndmin=2),
+                U=np.loadtxt(tmp / (name + "_out.eigvecs"), ndmin=2),
+                text_eigvals=(tmp / (name + "_out.eigvals")).read_text(),
+                diffs=np.array([float(x) for x in re.findall(r"GL projection iter =\s*\d+, diff =\s*([0-9.eE+-]+)",
+                                                             log)]))
+
+
 # --inbreed 1 with --USV of a PCA on the same PLINK set: name -> dataset
 INBREED = {"inbreed_d1": "d1"}
 
@@ -492,6 +571,12 @@ def main():
             res = run_project(tmp, name, seed, nref, ntgt, m)
             np.savez_compressed(OUT / f"{name}.npz", **res)
             print(name, res["ntarget_snps"], "target SNPs", res["U_m2"][:2])
+        for name, (seed, nref, ntgt, m) in PROJECT_GL.items():
+            if not want(name):
+                continue
+            res = run_project_gl(tmp, name, seed, nref, ntgt, m)
+            np.savez_compressed(OUT / f"{name}.npz", **res)
+            print(name, len(res["sites"]), "target sites,", len(res["diffs"]), "EM iterations", res["U"][:1])
         for name, ds in INBREED.items():
             if not want(name):
                 continue
