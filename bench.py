@@ -249,9 +249,26 @@ def run_c2(g, pa, args):
                     "algorithmic_flops_per_launch": flops / cnt,
                     "kernel_ms_per_solve": {k: round(v[1], 3) for k, v in sorted(stats.items())},
                     "kernel_launches_per_solve": {k: v[0] for k, v in sorted(stats.items())}}
+        if "fp32" in want:
+            # the FP32 arm's contraction (k_packed_gemm_valu<float>: VALU FMA) against the FP32 vector peak
+            g.profiling(True)
+            g.reset_stats()
+            _, _, _, _, _, ep32 = c2_solve(g, pa, wl, pa.PREC_FP32)
+            st32 = g.kernel_stats()
+            g.profiling(False)
+            ms32 = st32["gemm_G"][1] + st32["gemm_H"][1]
+            a32 = ep32 * 4.0 * m * n * l / (ms32 * 1e-3) / 1e12
+            roofline["fp32_arm"] = {"kernel": "k_packed_gemm_valu<float> (gemm_G + gemm_H)", "bound": "valu",
+                                    "achieved": round(a32, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                    "frac": round(a32 / FP32_PEAK_TFLOPS, 4), "contraction_ms_per_solve": round(ms32, 3),
+                                    "launches": st32["gemm_G"][0] + st32["gemm_H"][0],
+                                    "note": "peak: MI355X's FP32 figure; without packed FP32 (left off in this "
+                                            "build) plain FMAs issue at half of it"}
         if want != {"bf16x2", "fp32", "fp64"}:
             log(f"C2 timing solve: contraction {cnt} launches, avg {roofline['avg_launch_us']} us, "
                 f"{roofline['achieved']} TFLOP/s")
+            log("C2 kernel ms per solve: " + json.dumps({k: [v[0], round(v[1], 3)] for k, v in
+                                                         sorted(stats.items(), key=lambda x: -x[1][1])}))
     if want != {"bf16x2", "fp32", "fp64"}:
         return None
     ref = arms["fp64"]

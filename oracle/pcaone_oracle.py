@@ -848,9 +848,11 @@ def ld_geno_r2(codes: np.ndarray, ws, we, sums: np.ndarray | None = None) -> np.
 
 
 def principal_angles_max(A: np.ndarray, B: np.ndarray) -> float:
-    """Largest principal angle (radians) between span(A) and span(B)."""
-    qa, _ = np.linalg.qr(A)
-    qb, _ = np.linalg.qr(B)
+    """Largest principal angle (radians) between span(A) and span(B), in FP64
+    whatever the inputs' type (an FP32 QR leaves cos within 1e-7 of 1, where
+    arccos resolves nothing below 4e-4)."""
+    qa, _ = np.linalg.qr(np.asarray(A, dtype=np.float64))
+    qb, _ = np.linalg.qr(np.asarray(B, dtype=np.float64))
     s = np.linalg.svd(qa.T @ qb, compute_uv=False)
     return float(np.arccos(np.clip(s.min(), -1.0, 1.0)))
 

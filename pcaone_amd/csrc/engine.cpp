@@ -928,7 +928,7 @@ void Engine::seg_G_on(uint32_t w0, uint32_t w1, const SegBufs& b, const uint8_t*
     // In line before the contraction: run beside it on a second stream, the
     // gather's random table reads evict the contraction's operand tiles from
     // L2 and cost it more time than the gather takes alone.
-    timed("miss_gather", [&] { launch_csr_gather(cg, stream_); });
+    timed("miss_gather_G", [&] { launch_csr_gather(cg, stream_); });
   }
   if (!dense) {
     PackedGemm g{};
@@ -1048,7 +1048,7 @@ void Engine::seg_H_on(uint32_t w0, uint32_t w1, const std::vector<int>& route, c
     cg.per = N_;
     cg.ngroups = nz;
     cg.out = b.corrH;
-    timed("miss_gather", [&] { launch_csr_gather(cg, stream_); });
+    timed("miss_gather_H", [&] { launch_csr_gather(cg, stream_); });
   }
   if (!dense) {
     PackedGemm h{};

@@ -1142,11 +1142,13 @@ void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin,
 // store per lane), which is the type the epilogues' partials have.  Measured on the 500,000-row missing lists of the
 // 10,000-sample workload: the FP32 sums (half the shuffles, fewer registers,
 // more waves resident), two entries per round and the grid stride take a
-// launch from 387 to 257 us.
+// launch from 387 to 257 us; four entries per round in flight take a
+// 25 M-entry list from 296 to 273 us (500,000 rows) and 324 to 292 us
+// (640,000 window rows).
 template <class T, int V>
 __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
   typedef T vec __attribute__((ext_vector_type(V)));
-  constexpr int D = 2;  // entries per group per round
+  constexpr int D = 4;  // entries per group per round
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int lpg = (int)((g.ncols + V - 1) / V);
   const int ng = 64 / lpg;

@@ -36,6 +36,16 @@ DATASETS = {
     "d3": (13, 200, 2000, 3, 0.1, 0.05),    # 5 % missing for EMU
 }
 
+# larger sets regenerated from the simulator's seed (not stored): name -> (seed, N, M, npops, fst, miss)
+BIG = {
+    "c1": (2027, 2000, 20000, 3, 0.1, 0.01),   # the CPU-reference-parity workload
+    "w64": (2028, 1000, 24000, 5, 0.05, 0.005),  # the full 64-window winSVD schedule
+}
+BIG_CASES = {
+    "c1_svd1_p7": ("c1", "--svd 1 -k 10 --oversamples 10 -p 7 --tol-rsvd 0 --seed 101"),
+    "w64_svd2": ("w64", "--svd 2 -k 10 --oversamples 10 -w 64"),
+}
+
 CASES = {
     # name: (dataset, CLI args)
     "svd1_p7": ("d1", "--svd 1 -k 5 --oversamples 5 -p 7 --tol-rsvd 0 --seed 101"),
@@ -607,6 +617,21 @@ def main():
             res = run_ld_stats1(tmp, name, seed, n, per_chr, ld_bp)
             np.savez_compressed(OUT / f"{name}.npz", **res)
             print(name, len(res["r2"]), "pairs", int(np.isnan(res["r2"]).sum()), "NaN")
+        for name, (seed, n, m, npops, fst, miss) in BIG.items():
+            pay = pa.sim_bed(seed, n, m, npops, fst, 0.05, 0.5, miss)
+            pa.write_plink(tmp / name, pay, n, m)
+            for case, (ds, args) in BIG_CASES.items():
+                if ds != name or not want(case):
+                    continue
+                res = run_case(tmp, name, args, case)
+                res.pop("text_loadings")  # the loadings are kept as float32 values
+                res["loadings"] = res["loadings"].astype(np.float32)
+                import hashlib
+                np.savez_compressed(OUT / f"{case}.npz", sim=np.array([seed, n, m, npops]),
+                                    sim_f=np.array([fst, 0.05, 0.5, miss]),
+                                    payload_sha256=hashlib.sha256(pay.tobytes()).hexdigest(), nsamples=n, nsnps=m,
+                                    args=args, **res)
+                print(case, res["eigvals"])
         for name, (seed, n, m, npops, fst, miss, lay, comp, bits, pmin, ids) in BGEN.items():
             path = tmp / (name + ".bgen")
             pa.sim_bgen(path, seed, n, m, npops, fst, miss=miss, layout=lay, compression=comp, bits=bits,

@@ -12,7 +12,7 @@ import pytest
 
 import pcaone_amd as pa
 import pcaone_oracle as O
-from _cases import csv_bytes, eig_relerr, oracle_matrix, oracle_run, parse_args
+from _cases import payload_of, csv_bytes, eig_relerr, oracle_matrix, oracle_run, parse_args
 from conftest import GOLDEN, golden_cases, load_golden
 
 pytestmark = pytest.mark.gpu
@@ -100,7 +100,7 @@ def test_usv_matches_reference_cli(gpu, case, arm, tmp_path):
         gpu.load_bgen(path, standardize=bool(d["scale"]) and not d["ld"])
         assert (gpu.N, gpu.M) == (n, m)
     else:
-        gpu.load_bed(z["payload"], n, m)
+        gpu.load_bed(payload_of(z), n, m)
     ooc = {}
     if d["m"] > 0 and d["svd"] == 2:  # out-of-core winSVD: the -m block plan and permute_plink's order
         bs, _, factor = pa.ooc_plan(n, m, 2, d["k"], d["os"], d["w"], d["m"])
