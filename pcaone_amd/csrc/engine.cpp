@@ -82,6 +82,9 @@ Engine::Engine(int device) : device_(device) {
   PCAONE_REQUIRE(device >= 0 && device < n, -1, "no such HIP device");
   PCAONE_HIP(hipSetDevice(device));
   PCAONE_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  PCAONE_HIP(hipStreamCreateWithFlags(&ustream_, hipStreamNonBlocking));
+  PCAONE_HIP(hipEventCreateWithFlags(&ev_epoch_, hipEventDisableTiming));
+  PCAONE_HIP(hipHostMalloc(&hpipe_, 64, 0));
   PCAONE_HIP(hipDeviceGetAttribute(&ncu_, hipDeviceAttributeMultiprocessorCount, device));
   (void)ld_scratch(device);  // the allocation-free entry point's scratch address, once per device
 }
@@ -93,9 +96,13 @@ Engine::~Engine() {
     end_stream();
   } catch (...) {
   }
+  if (ustream_) (void)hipStreamSynchronize(ustream_);
   if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
   destroy_blas(blas_);
   timer.resolve();
+  if (ev_epoch_) (void)hipEventDestroy(ev_epoch_);
+  if (hpipe_) (void)hipHostFree(hpipe_);
+  if (ustream_) (void)hipStreamDestroy(ustream_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -1238,31 +1245,37 @@ void Engine::usv_step() {
     usv_step_wide();
     return;
   }
+  usv_step_on(stream_, G_.p, H_.p, part_.p, R1i_.p, R2i_.p, status_.p);
+  Gfin_ = G_.p;
+}
+
+void Engine::usv_step_on(hipStream_t s, const double* G, const double* H, double* part, double* R1i, double* R2i,
+                         int* status) {
   const uint32_t l = l_;
   UsvPass u{};
-  u.G = G_.p;
+  u.G = G;
   u.m = M_;
-  u.H = H_.p;
+  u.H = H;
   u.n = N_;
   u.l = l;
-  u.part = part_.p;
+  u.part = part;
   u.gram = gred_.p;
   u.bbt = bbt_.p;
-  u.R1i = R1i_.p;
-  u.R2i = R2i_.p;
+  u.R1i = R1i;
+  u.R2i = R2i;
   u.Rti = Rti_.p;
   u.Bt = Bt_.p;
-  u.status = status_.p;
-  timed("usv_gram", [&] { launch_usv_qr(u, 0, stream_); });
-  timed("usv_q1", [&] { launch_usv_qr(u, 1, stream_); });
-  timed("usv_bt", [&] { launch_usv_qr(u, 2, stream_); });
+  u.status = status;
+  timed_on(s, "usv_gram", [&] { launch_usv_qr(u, 0, s); });
+  timed_on(s, "usv_q1", [&] { launch_usv_qr(u, 1, s); });
+  timed_on(s, "usv_bt", [&] { launch_usv_qr(u, 2, s); });
   // later epochs start from the previous epoch's eigenvectors
-  timed("usv_eig", [&] {
-    launch_sym_eig(bbt_.p, 1, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, have_W_ ? W_.p : nullptr, stream_);
+  timed_on(s, "usv_eig", [&] {
+    launch_sym_eig(bbt_.p, 1, l, evals_.p, W_.p, sinv_.p, (uint32_t)p_.k, have_W_ ? W_.p : nullptr, s);
   });
   have_W_ = true;
-  timed("usv_rows", [&] {
-    launch_rows_times_small(Bt_.p, N_, l, W_.p, l, (uint32_t)p_.k, sinv_.p, Ucur_.p, stream_);
+  timed_on(s, "usv_rows", [&] {
+    launch_rows_times_small(Bt_.p, N_, l, W_.p, l, (uint32_t)p_.k, sinv_.p, Ucur_.p, s);
   });
 }
 
@@ -1270,6 +1283,7 @@ void Engine::usv_step() {
 // B^T = H Rt^-1, the Gram B B^T and its eigensolve, Ucur = B^T U_B S^-1.
 void Engine::usv_step_wide() {
   const uint32_t l = l_;
+  Gfin_ = G_.p;
   timed("usv_gram", [&] { launch_wide_gram(G_.p, nullptr, nullptr, M_, l, part_.p, gred_.p, stream_); });
   timed("usv_chol", [&] { launch_wide_chol(gred_.p, l, R1i_.p, status_.p, stream_); });
   WideApply a1{G_.p, M_, l, R1i_.p, wq_.p, nullptr, nullptr};
@@ -1294,6 +1308,109 @@ void Engine::usv_step_wide() {
 // The epochs of computeUSV (src/Halko.cpp:51-93); `done` is false when the
 // loop ran out of epochs (the last Ucur then sits in Upre).
 int Engine::usv_epochs(int maxp, double tol, double* diffs, bool& done) {
+  if (robust_ || wide_) return usv_epochs_serial(maxp, tol, diffs, done);
+  // Pipelined: computeUSV of epoch pi (its QR, eigensolve and mev) runs on
+  // ustream_ from a copy of H and the G buffer just written, while epoch
+  // pi + 1's computeGandH, which needs only Omega and the accumulators, runs on
+  // stream_.  That next epoch is speculative: when epoch pi ends the solve, the
+  // accumulators (which carry over between solves) and the window schedule
+  // state are put back as they were, so every result equals the serial order's.
+  const uint32_t k = (uint32_t)p_.k;
+  const size_t hb = (size_t)N_ * l_ * 8;
+  Gu_.reserve((uint64_t)M_ * l_);
+  Hu_.reserve((uint64_t)N_ * l_);
+  sH1_.reserve((uint64_t)N_ * l_);
+  sH2_.reserve((uint64_t)N_ * l_);
+  upart_.reserve(part_.n);
+  uR1i_.reserve((uint64_t)l_ * l_);
+  uR2i_.reserve((uint64_t)l_ * l_);
+  ustat_.reserve(4);
+  PCAONE_HIP(hipMemsetAsync(ustat_.p, 0, 4 * sizeof(int), stream_));
+  int p = maxp, ep = 0;
+  have_W_ = false;
+  if (diffs)
+    for (int i = 0; i <= maxp; ++i) diffs[i] = 0.0;
+  done = false;
+  const bool winsvd = p_.variant == 2;
+  const int pwin = (int)std::log2((double)p_.bands);
+  double* hm = static_cast<double*>(hpipe_);
+  int* hst = reinterpret_cast<int*>(hm + 1);
+  rsvd_gandh(0, nullptr, nullptr, false);
+  ep = 1;
+  for (int pi = 0; pi <= p; ++pi) {
+    // epoch pi's G and H for its computeUSV
+    std::swap(G_, Gu_);
+    PCAONE_HIP(hipMemcpyAsync(Hu_.p, H_.p, hb, hipMemcpyDeviceToDevice, stream_));
+    PCAONE_HIP(hipMemcpyAsync(ustat_.p, status_.p, 2 * sizeof(int), hipMemcpyDeviceToDevice, stream_));
+    PCAONE_HIP(hipEventRecord(ev_epoch_, stream_));
+    PCAONE_HIP(hipStreamWaitEvent(ustream_, ev_epoch_, 0));
+    usv_step_on(ustream_, Gu_.p, Hu_.p, upart_.p, uR1i_.p, uR2i_.p, ustat_.p + 2);
+    if (pi > 0) {
+      const int nbN = gram_blocks(N_);
+      timed_on(ustream_, "mev", [&] { launch_xty_partial(Ucur_.p, Upre_.p, N_, k, upart_.p, nbN, ustream_); });
+      timed_on(ustream_, "mev", [&] { launch_mev_finish(upart_.p, nbN, k, scal_.p, ustream_); });
+      PCAONE_HIP(hipMemcpyAsync(hm, scal_.p, 8, hipMemcpyDeviceToHost, ustream_));
+    }
+    PCAONE_HIP(hipMemcpyAsync(hst, ustat_.p, 4 * sizeof(int), hipMemcpyDeviceToHost, ustream_));
+    // the next epoch, if the schedule may still run one
+    const int pcap = (winsvd && pi > 0) ? std::max(p, pwin) : p;
+    const bool spec = pi + 1 <= pcap;
+    const uint64_t bandsize_before = bandsize_;
+    if (spec) {
+      PCAONE_HIP(hipMemcpyAsync(sH1_.p, H1_.p, hb, hipMemcpyDeviceToDevice, stream_));
+      PCAONE_HIP(hipMemcpyAsync(sH2_.p, H2_.p, hb, hipMemcpyDeviceToDevice, stream_));
+      rsvd_gandh(pi + 1, nullptr, nullptr, false);
+    }
+    PCAONE_HIP(hipStreamSynchronize(ustream_));
+    auto undo_spec = [&]() {
+      if (!spec) return;
+      PCAONE_HIP(hipMemcpyAsync(H1_.p, sH1_.p, hb, hipMemcpyDeviceToDevice, stream_));
+      PCAONE_HIP(hipMemcpyAsync(H2_.p, sH2_.p, hb, hipMemcpyDeviceToDevice, stream_));
+      PCAONE_HIP(hipMemsetAsync(status_.p, 0, 2 * sizeof(int), stream_));
+      bandsize_ = bandsize_before;
+      sync();
+    };
+    if (hst[0] || hst[1] || hst[2] || hst[3]) {
+      const int piv = hst[0] ? hst[0] : hst[1] ? hst[1] : hst[2] ? hst[2] : hst[3];
+      undo_spec();
+      PCAONE_HIP(hipMemsetAsync(status_.p, 0, 2 * sizeof(int), stream_));
+      PCAONE_HIP(hipMemsetAsync(ustat_.p, 0, 4 * sizeof(int), stream_));
+      sync();
+      Gfin_ = Gu_.p;
+      throw Error(-4, "CholeskyQR breakdown (pivot " + std::to_string(piv) +
+                          "): the sketch is numerically rank deficient");
+    }
+    bool stop = false;
+    if (pi > 0) {
+      const double diff = 1.0 - hm[0];
+      if (diffs && pi <= maxp) diffs[pi] = diff;
+      if (diff < tol || pi == p) {
+        if (winsvd && std::ldexp(1.0, pi) < (double)p_.bands) {
+          p = pwin;  // winSVD runs log2(bands)+1 epochs at least
+        } else {
+          done = true;
+          stop = true;
+        }
+      } else {
+        std::swap(Ucur_.p, Upre_.p);
+      }
+    } else {
+      std::swap(Ucur_.p, Upre_.p);
+    }
+    if (!stop && pi + 1 > p) stop = true;  // out of epochs
+    if (stop) {
+      undo_spec();
+      break;
+    }
+    ++ep;  // epoch pi + 1, running
+  }
+  // G_ holds the last computeUSV step's G again (any speculative work is done)
+  if (Gu_.p != G_.p) std::swap(G_, Gu_);
+  Gfin_ = G_.p;
+  return ep;
+}
+
+int Engine::usv_epochs_serial(int maxp, double tol, double* diffs, bool& done) {
   const uint32_t k = (uint32_t)p_.k;
   int p = maxp, ep = 0;
   have_W_ = false;
@@ -1340,6 +1457,7 @@ int Engine::usv_epochs(int maxp, double tol, double* diffs, bool& done) {
 // eigenvectors, Ucur = B^T U_B S^-1, and at the end V = G (T U_B).
 void Engine::usv_step_robust() {
   const uint32_t l = l_;
+  Gfin_ = G_.p;
   if (part_.n < wide_part_doubles()) part_.reserve(wide_part_doubles());
   if (!eigtmp_.p) eigtmp_.reserve((uint64_t)l * l);
   Wg_.reserve((uint64_t)l * l);
@@ -1399,7 +1517,7 @@ void Engine::rsvd_usv(int maxp, double tol, double* U, double* S, double* V, int
   if (!done) std::swap(Ucur_.p, Upre_.p);  // the last Ucur sits in Upre after the swap
   // V = Q U_B[:, :k] = G (R^-1 U_B[:, :k]) (rows in laid-out order), S = sqrt(eig(B B^T))[:k]
   timed("usv_rows", [&] { launch_rows_times_small(Rti_.p, l_, l_, W_.p, l_, k, nullptr, Rt_.p, stream_); });
-  timed("usv_rows", [&] { launch_rows_times_small(G_.p, M_, l_, Rt_.p, k, k, nullptr, V_.p, stream_); });
+  timed("usv_rows", [&] { launch_rows_times_small(Gfin_, M_, l_, Rt_.p, k, k, nullptr, V_.p, stream_); });
   std::vector<double> ev(l_);
   PCAONE_HIP(hipMemcpyAsync(ev.data(), evals_.p, l_ * 8, hipMemcpyDeviceToHost, stream_));
   sync();

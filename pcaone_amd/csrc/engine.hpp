@@ -244,6 +244,11 @@ class Engine {
   void run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route);
   void run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b);
   void usv_step();
+  // usv_step's normal path on stream s with the given G, H and scratch
+  void usv_step_on(hipStream_t s, const double* G, const double* H, double* part, double* R1i, double* R2i,
+                   int* status);
+  // computeUSV's epochs one at a time (robust and l > 64 solves)
+  int usv_epochs_serial(int maxp, double tol, double* diffs, bool& done);
   void usv_step_wide();  // l > 64
   void usv_step_robust();  // robust mode: G orthogonalised through its Gram eigenpairs
   // computeUSV's epochs; returns the number of computeGandH calls
@@ -274,6 +279,16 @@ class Engine {
 
   int device_;
   hipStream_t stream_ = nullptr;  // every launch, in order, at the default priority
+  // computeUSV's linear algebra of epoch pi runs here, beside epoch pi + 1's
+  // computeGandH on stream_ (default priority as well)
+  hipStream_t ustream_ = nullptr;
+  hipEvent_t ev_epoch_ = nullptr;   // stream_: epoch pi's G and H are ready
+  DevBuf<double> Gu_, Hu_;          // epoch pi's G (swapped with G_) and H (a copy)
+  DevBuf<double> sH1_, sH2_;        // the accumulators before a speculative epoch
+  DevBuf<double> upart_, uR1i_, uR2i_;
+  DevBuf<int> ustat_;               // [0..1] the refreshes' flags of epoch pi, [2..3] its usv's
+  void* hpipe_ = nullptr;           // page-locked: 1 - mev and the four flags
+  const double* Gfin_ = nullptr;    // the G of the last computeUSV step
 
   // ---- out of core: the payload in host memory, windows through a ring ---
   struct StreamSlot {
