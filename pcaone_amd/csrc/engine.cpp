@@ -1144,6 +1144,14 @@ void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route
 void Engine::run_segment(uint32_t w0, uint32_t w1, const std::vector<int>& route, const SegBufs& b) {
   if (!streamed_) {
     seg_G(w0, w1, b);
+    if (spec_gate_) {
+      const std::function<bool()> gate = std::move(spec_gate_);
+      spec_gate_ = nullptr;
+      if (!gate()) {
+        spec_abort_ = true;
+        return;
+      }
+    }
     seg_H(w0, w1, route, b);
     return;
   }
@@ -1180,6 +1188,7 @@ void Engine::rsvd_gandh(int pi, double* Gout, double* Hout, bool check) {
     }
     timed("fill", [&] { launch_fill_f64(H_.p, (uint64_t)N_ * l_, 0.0, stream_); });
     run_segment(0, nwin_, std::vector<int>(nwin_, 1));
+    if (spec_abort_) return;
   } else {
     // FancyRsvdOpData in-core, src/Halko.cpp:153-226
     if (pi == 0) init_omega();
@@ -1195,6 +1204,7 @@ void Engine::rsvd_gandh(int pi, double* Gout, double* Hout, bool check) {
       const WindowStep& st = pl.steps[b];
       if (!st.update && b + 1 < nwin_) continue;
       run_segment(start, b + 1, route);
+      if (spec_abort_) return;
       start = b + 1;
       if (st.update) {
         omega_refresh(true, st.zero == 1 ? H1_.p : H2_.p);
@@ -1356,12 +1366,55 @@ int Engine::usv_epochs(int maxp, double tol, double* diffs, bool& done) {
     const int pcap = (winsvd && pi > 0) ? std::max(p, pwin) : p;
     const bool spec = pi + 1 <= pcap;
     const uint64_t bandsize_before = bandsize_;
+    bool decided = false, stop = false, broke = false;
+    // epoch pi's verdict from its computeUSV (waits for it): true to go on
+    auto decide = [&]() -> bool {
+      if (decided) return !stop;
+      decided = true;
+      PCAONE_HIP(hipStreamSynchronize(ustream_));
+      if (hst[0] || hst[1] || hst[2] || hst[3]) {
+        broke = stop = true;
+        return false;
+      }
+      if (pi > 0) {
+        const double diff = 1.0 - hm[0];
+        if (diffs && pi <= maxp) diffs[pi] = diff;
+        if (diff < tol || pi == p) {
+          if (winsvd && std::ldexp(1.0, pi) < (double)p_.bands) {
+            p = pwin;  // winSVD runs log2(bands)+1 epochs at least
+          } else {
+            done = true;
+            stop = true;
+          }
+        } else {
+          std::swap(Ucur_.p, Upre_.p);
+        }
+      } else {
+        std::swap(Ucur_.p, Upre_.p);
+      }
+      if (!stop && pi + 1 > p) stop = true;  // out of epochs
+      return !stop;
+    };
     if (spec) {
       PCAONE_HIP(hipMemcpyAsync(sH1_.p, H1_.p, hb, hipMemcpyDeviceToDevice, stream_));
       PCAONE_HIP(hipMemcpyAsync(sH2_.p, H2_.p, hb, hipMemcpyDeviceToDevice, stream_));
-      rsvd_gandh(pi + 1, nullptr, nullptr, false);
+      // where epoch pi may end the solve, the next epoch stops after its first
+      // G product unless epoch pi's verdict is to go on (the winSVD's first
+      // log2(bands) epochs always go on: no wait there)
+      const bool may_stop = pi > 0 && !(winsvd && std::ldexp(1.0, pi) < (double)p_.bands && pwin > pi);
+      spec_abort_ = false;
+      if (may_stop) spec_gate_ = decide;
+      try {
+        rsvd_gandh(pi + 1, nullptr, nullptr, false);
+      } catch (...) {
+        spec_gate_ = nullptr;
+        spec_abort_ = false;
+        throw;
+      }
+      spec_gate_ = nullptr;
+      spec_abort_ = false;
     }
-    PCAONE_HIP(hipStreamSynchronize(ustream_));
+    decide();
     auto undo_spec = [&]() {
       if (!spec) return;
       PCAONE_HIP(hipMemcpyAsync(H1_.p, sH1_.p, hb, hipMemcpyDeviceToDevice, stream_));
@@ -1370,34 +1423,15 @@ int Engine::usv_epochs(int maxp, double tol, double* diffs, bool& done) {
       bandsize_ = bandsize_before;
       sync();
     };
-    if (hst[0] || hst[1] || hst[2] || hst[3]) {
+    if (broke) {
       const int piv = hst[0] ? hst[0] : hst[1] ? hst[1] : hst[2] ? hst[2] : hst[3];
       undo_spec();
       PCAONE_HIP(hipMemsetAsync(status_.p, 0, 2 * sizeof(int), stream_));
       PCAONE_HIP(hipMemsetAsync(ustat_.p, 0, 4 * sizeof(int), stream_));
       sync();
-      Gfin_ = Gu_.p;
       throw Error(-4, "CholeskyQR breakdown (pivot " + std::to_string(piv) +
                           "): the sketch is numerically rank deficient");
     }
-    bool stop = false;
-    if (pi > 0) {
-      const double diff = 1.0 - hm[0];
-      if (diffs && pi <= maxp) diffs[pi] = diff;
-      if (diff < tol || pi == p) {
-        if (winsvd && std::ldexp(1.0, pi) < (double)p_.bands) {
-          p = pwin;  // winSVD runs log2(bands)+1 epochs at least
-        } else {
-          done = true;
-          stop = true;
-        }
-      } else {
-        std::swap(Ucur_.p, Upre_.p);
-      }
-    } else {
-      std::swap(Ucur_.p, Upre_.p);
-    }
-    if (!stop && pi + 1 > p) stop = true;  // out of epochs
     if (stop) {
       undo_spec();
       break;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -289,6 +290,11 @@ class Engine {
   DevBuf<int> ustat_;               // [0..1] the refreshes' flags of epoch pi, [2..3] its usv's
   void* hpipe_ = nullptr;           // page-locked: 1 - mev and the four flags
   const double* Gfin_ = nullptr;    // the G of the last computeUSV step
+  // A speculative epoch asks spec_gate_ (once, after its first segment's G
+  // product) whether to go on; a false answer sets spec_abort_ and the rest of
+  // the epoch is never enqueued.
+  std::function<bool()> spec_gate_;
+  bool spec_abort_ = false;
 
   // ---- out of core: the payload in host memory, windows through a ring ---
   struct StreamSlot {
