@@ -396,8 +396,9 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   snapH1_.reserve((uint64_t)N_ * l_);
   snapH2_.reserve((uint64_t)N_ * l_);
   bbt_.reserve((uint64_t)l_ * l_);
-  for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &Rf_, &W_}) b->reserve((uint64_t)l_ * l_);
+  for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &W_}) b->reserve((uint64_t)l_ * l_);
   sign_.reserve(l_);
+  fsign_.reserve(l_);
   evals_.reserve(l_);
   sinv_.reserve(l_);
   Bt_.reserve((uint64_t)N_ * l_);
@@ -761,6 +762,7 @@ void Engine::init_omega() {
   const size_t nb = (size_t)N_ * l_ * 8;
   PCAONE_HIP(hipMemcpyAsync(omega_.p, omega0_.p, nb, hipMemcpyDeviceToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(colsum_.p, colsum0_.p, (size_t)l_ * 8, hipMemcpyDeviceToDevice, stream_));
+  launch_fill_f64(fsign_.p, l_, 1.0, stream_);  // the drawn Omega carries its signs
   timed("omega_operand",
         [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, lo_off_, kg_, kperm_(p_.precision),
                                    stream_); });
@@ -770,8 +772,9 @@ void Engine::init_omega() {
 // Omega <- Householder Q of H (H = H1 + H2 for winSVD), then flipOmg
 // (src/Halko.cpp:118-121, 213-216): CholeskyQR2 in FP64 with the Householder
 // R-diagonal signs reproduced, so Q equals Eigen's householderQ() columns,
-// fused into four passes that also write the contraction operand and the
-// FP32 gather table.
+// fused into three passes, the last of which also writes the contraction
+// operand and the FP32 gather table.  Omega is stored unsigned; the column
+// signs (fsign_) are applied where G is finalised.
 void Engine::omega_refresh(bool from_two, double* zero_acc) {
   if (wide_ || robust_) {
     omega_refresh_wide(from_two, zero_acc);
@@ -790,10 +793,9 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
   q.R1i = R1i_.p;
   q.R2i = R2i_.p;
   q.sign = sign_.p;
-  q.Rf = Rf_.p;
+  q.fsign = fsign_.p;
   q.colsum = colsum_.p;
   q.omega = omega_.p;
-  q.omg2 = omega_.p;  // the previous Omega for flipOmg is the one being replaced
   q.zero_acc = zero_acc;
   q.omega32 = (b.corrG && b.g2_f32) ? b.omega32 : nullptr;
   q.bg = bg_.p;
@@ -804,7 +806,6 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
   timed("qr_gram_h", [&] { launch_qr_pass(q, 0, stream_); });
   timed("qr_gram_q", [&] { launch_qr_pass(q, 1, stream_); });
   timed("qr_flip", [&] { launch_qr_pass(q, 2, stream_); });
-  timed("qr_write", [&] { launch_qr_pass(q, 3, stream_); });
 }
 
 // The same refresh for l > 64 in general form: Gram of H (= H1 + H2, written),
@@ -853,6 +854,7 @@ void Engine::omega_refresh_wide(bool from_two, double* zero_acc) {
   w.kpad = kg_;
   w.kperm = kperm_(p_.precision);
   timed("qr_write", [&] { launch_wide_write(w, stream_); });
+  launch_fill_f64(fsign_.p, l_, 1.0, stream_);  // written signed
 }
 
 // the FP32 table the missing-entry gathers of the G contraction read
@@ -878,6 +880,7 @@ SegBufs Engine::op_bufs() {
   b.hf = hf_;
   b.omega = omega_.p;
   b.colsum = colsum_.p;
+  b.fsign = fsign_.p;
   b.G = G_.p;
   b.g2 = g2_.p;
   b.g2_f32 = p_.precision != PREC_FP64;
@@ -997,6 +1000,7 @@ FinalizeG Engine::finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) cons
   fg.F = F_.p;
   fg.d = d_.p;
   fg.colsum = b.colsum;
+  fg.fsign = b.fsign;
   fg.miss_ptr = miss_snp_ptr_.p;
   fg.miss_idx = miss_snp_idx_.p;
   fg.omega = b.omega;

@@ -77,6 +77,7 @@ struct SegBufs {
   const double* omega = nullptr;
   float* omega32 = nullptr;  // FP32 copy of Omega: the missing-entry gather table of the non-FP64 arms
   const double* colsum = nullptr;
+  const double* fsign = nullptr;  // Omega's column signs (null: all +1)
   double *G = nullptr, *S2 = nullptr, *s2p = nullptr;
   bool affine = false;  // the packed split-BF16 contraction runs on 2 - 2 y
   void* g2 = nullptr;        // 2 F d G, float when g2_f32
@@ -408,7 +409,8 @@ class Engine {
   std::vector<double> last_S_;
   DevBuf<double> miss_val_, Sdev_, Vpre_;
   // linear algebra scratch
-  DevBuf<double> part_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, Rf_, sign_;
+  DevBuf<double> part_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, sign_;
+  DevBuf<double> fsign_;           // Omega's column signs (omega_ and colsum_ are unsigned)
   DevBuf<double> gred_, bbt_;  // computeUSV: reduced Gram between passes, B B^T
   DevBuf<double> qred_;        // Omega update: reduced partials between passes
   DevBuf<double> Bt_, evals_, W_, sinv_, Ucur_, Upre_, V_, scal_;

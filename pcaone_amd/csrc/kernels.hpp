@@ -93,6 +93,8 @@ struct FinalizeG {
   const double* F;      // [M] allele frequency, laid-out order
   const double* d;      // [M] 1/sd or 1
   const double* colsum; // [l]  sum_j Omega[j][c]
+  const double* fsign;  // [l] sign of Omega's column c (omega, colsum and corr hold the unsigned
+                        // columns; G takes the sign), or null (all +1)
   const int64_t* miss_ptr;  // [M+1]  (EMU values only)
   const int32_t* miss_idx;  // sample indices
   const double* omega;  // [N][l]
@@ -302,27 +304,27 @@ void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t s);
 // ---- FP64 tall-skinny linear algebra --------------------------------------
 // workgroups of the partial-product passes over nrows rows (the mev products)
 int gram_blocks(uint64_t nrows);
-// ---- the winSVD Omega update in four passes (kernels_qr.hip) ----------------
+// ---- the winSVD Omega update in three passes (kernels_qr.hip) ---------------
 // Omega <- Householder-signed CholeskyQR2 Q of H (H = H1 + H2 when H2 is
-// given), flipped against the previous Omega (omg2), written to every consumer.
+// given), flipped against the previous Omega, written unsigned to every
+// consumer with the column signs in fsign.
 struct QrPass {
   const double* H1;  // pass 0 input (the only input when H2 is null)
   const double* H2;  // optional second accumulator: H1 + H2 is written to H
   double* H;         // H (passes 1..3 read it; must equal H1 when H2 is null)
   uint64_t n;        // rows (samples), n >= 2 l
   uint32_t l;        // columns, 1..64
-  double* part;      // qr_part_doubles(l) scratch, kept between the four passes
+  double* part;      // qr_part_doubles(l) scratch, kept between the three passes
   double* gram;      // usv_gram_doubles(l): the reduced partials handed to the next pass
   int nparts;        // blocks of the reduction passes (set by launch_qr_pass)
   int* status;       // [2]: first failing pivot of the two factorisations (sticky, 0 = none)
-  double* R1i;       // l x l outputs: R1^-1, R2^-1, Householder signs, R2^-1 diag(sign flip)
+  double* R1i;       // l x l outputs: R1^-1, R2^-1
   double* R2i;
-  double* sign;
-  double* Rf;
-  double* colsum;    // l: column sums of the new Omega
-  double* omega;     // n x l FP64
-  double* omg2;      // n x l: previous Omega in (new Omega out unless it is `omega` itself:
-                     // the flip pass reads it before the write pass overwrites omega)
+  double* sign;      // l: Householder signs of the new Omega's columns
+  double* fsign;     // l: column signs of the previous Omega in, of the new one out
+                     // (Householder sign times flipOmg's flip)
+  double* colsum;    // l: column sums of the new (unsigned) Omega
+  double* omega;     // n x l FP64, unsigned: the previous Omega in, the new one out
   double* zero_acc;  // optional n x l accumulator to clear
   float* omega32;    // optional n x l FP32 copy
   void* bg;          // contraction operand [bcols][kpad] of bprec
@@ -332,7 +334,8 @@ struct QrPass {
   int kperm;         // operand positions in kperm16 order
 };
 uint64_t qr_part_doubles(uint32_t l);
-// pass 0: H and R1^-1; 1: R2^-1 and signs; 2: flip and colsum; 3: the writes.
+// pass 0: H and the Gram of H; 1: R1^-1 and the Gram of Q1; 2: R2^-1, the
+// Householder signs, the unsigned Omega into every consumer, colsum and fsign.
 void launch_qr_pass(const QrPass& q, int pass, hipStream_t s);
 // computeUSV's linear algebra without materialising Q: CholeskyQR2 factors of
 // G (M x l) and B^T = H (R2 R1)^-1 with B B^T for the eigensolve.
@@ -396,7 +399,7 @@ void launch_wide_chol(const double* gram, uint32_t l, double* rinv, int* status,
 // the Householder signs of the top l x l block of Q1 R2^-1
 void launch_wide_r2(const double* gram, uint32_t l, const double* r1i, double* r2i, double* rti, const double* q1,
                     double* sign, int* status, hipStream_t s);
-// Omega = omu diag(flip * sign) written to every consumer (as k_qr_write)
+// Omega = omu diag(flip * sign) written signed to every consumer (fsign stays +1)
 struct WideWrite {
   const double* omu;   // n x l unsigned Omega (Q1 R2^-1)
   uint64_t n;

@@ -784,7 +784,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
   // clamped into range, and its value is dropped afterwards when out of range:
   // the loads then issue together instead of one branch at a time.
   uint32_t qe[kFinEl], ce[kFinEl], off[kFinEl];
-  double raw[kFinEl], cs[kFinEl], Fi[kFinEl], di[kFinEl];
+  double raw[kFinEl], cs[kFinEl], Fi[kFinEl], di[kFinEl], sg[kFinEl];
   P corr[kFinEl];  // widened where used
   const uint32_t rlast = f.r1 - 1;
 #pragma unroll
@@ -797,6 +797,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
     off[u] = (r - f.r0) * (uint32_t)f.ldp + ce[u];
     raw[u] = 0.0;
     cs[u] = colsum[ce[u]];
+    sg[u] = f.fsign ? f.fsign[ce[u]] : 1.0;
     Fi[u] = F[r];
     di[u] = d[r];
   }
@@ -835,7 +836,8 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_G(FinalizeG f) {
       double unused = 0.0;
       miss_sums(f.miss_ptr, f.miss_idx, r, f.omega, l, c, f.mval, unused, corr_e);
     }
-    const double Gv = di[u] * (f.hf * (rv - 2.0 * Fi[u] * (cs[u] - (double)corr[u])) + corr_e);
+    // every term is linear in Omega's column c: its sign applies to the whole
+    const double Gv = sg[u] * (di[u] * (f.hf * (rv - 2.0 * Fi[u] * (cs[u] - (double)corr[u])) + corr_e));
     f.G[(uint64_t)r * l + c] = Gv;
     const double gp = di[u] * Gv;
     const double g2 = 2.0 * Fi[u] * gp;

@@ -1,4 +1,4 @@
-// The winSVD Omega update in four launches: Omega <- Householder Q of H
+// The winSVD Omega update in three launches: Omega <- Householder Q of H
 // (H = H1 + H2), then flipOmg (src/Halko.cpp:213-216, src/RSVD.hpp:81-90 on
 // Eigen's HouseholderQR).  The Q is CholeskyQR2's, with the Householder
 // R-diagonal signs reproduced:
@@ -7,21 +7,22 @@
 //   k_qr_gram_q  prologue: A1 = R1^T R1, R1^-1;  pass: Q1 = H R1^-1 on the fly,
 //                block partials of A2 = Q1^T Q1
 //   k_qr_flip    prologue: A2 = R2^T R2, R2^-1, the Householder signs of the top
-//                l x l block of Q1 R2^-1;  pass: Omega_u = Q1 R2^-1 on the fly,
-//                block partials of flipOmg's column statistics against the
-//                previous Omega
-//   k_qr_write   prologue: the flips, Rf = R2^-1 diag(sign flip);  pass:
-//                Omega = Q1 Rf into every consumer: the FP64 matrix, the copy
-//                kept for the next flip, the contraction operand (split BF16 /
-//                FP32 / FP64, K-permuted for the packed-genotype kernels), the
-//                FP32 table of the missing-entry gathers; the emptied H
-//                accumulator is zeroed.
+//                l x l block of Q1 R2^-1;  pass: Omega_u = Q1 R2^-1 on the fly
+//                into every consumer (the FP64 matrix, the contraction operand
+//                in split BF16 / FP32 / FP64, K-permuted for the packed-genotype
+//                kernels, the FP32 table of the missing-entry gathers; the
+//                emptied H accumulator is zeroed) and block partials of
+//                flipOmg's column statistics against the previous Omega
+//   k_qr_signs   one block: the statistics summed, each column's sign settled
 //
-// The tall N x l data is read once per pass.  A small launch after each of
-// the first three passes sums its block partials in a fixed order
-// (deterministic); each prologue reads that sum and runs the l x l step
-// redundantly in every block (block 0 stores the results), so no block ever
-// waits for another inside a launch.
+// Omega is kept unsigned: a column's sign (Householder sign times flipOmg's
+// flip) lives in `fsign`, and G = X Omega takes it in its epilogue, where every
+// term (the product, the column sums, the missing-entry sums) is linear in the
+// column.  The tall N x l data is read once per pass.  A small launch after
+// each pass sums its block partials in a fixed order (deterministic); each
+// prologue reads that sum and runs the l x l step redundantly in every block
+// (block 0 stores the results), so no block ever waits for another inside a
+// launch.
 #include <algorithm>
 
 #include "common.hpp"
@@ -638,6 +639,144 @@ __device__ void householder_signs_block(double* B, uint32_t l, double* sign) {
   __syncthreads();
 }
 
+// The unsigned Omega of one tile (tile_omega layout) into every consumer,
+// through the wave's 16 x (W + 1) LDS rows `stg`: the tile's rows are
+// contiguous in the row-major copies, and the split operand takes four
+// positions per 8-byte store (rows past n are zero in the tile, as the
+// operand's padding is).  Rolled loops keep the write phase's registers few.
+// The same writes element by element from the registers (l > 48, where the
+// staging rows do not fit the LDS beside the prologue's matrices).
+template <int NB>
+__device__ inline void write_omega_tile_direct(const QrPass& q, uint64_t tile, const f64x4 (&om)[NB]) {
+  const int lane = threadIdx.x & 63, kr = lane >> 4, ci = lane & 15;
+  const uint64_t row = tile * 16 + ci;
+  if (row >= q.n) return;
+  const uint32_t l = q.l;
+  const uint64_t kpos = q.kperm ? kperm16(row) : row;
+#pragma unroll
+  for (int c2 = 0; c2 < NB; ++c2)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t c = c2 * 16 + kr + 4 * v;
+      if (c >= l) continue;
+      const double x = om[c2][v];
+      const uint64_t gi = row * l + c;
+      q.omega[gi] = x;
+      if (q.zero_acc) q.zero_acc[gi] = 0.0;
+      if (q.omega32) q.omega32[gi] = (float)x;
+      if (q.bprec == PREC_SPLIT_BF16) {
+        const __bf16 hi = (__bf16)(float)x;
+        reinterpret_cast<__bf16*>(q.bg)[(uint64_t)c * q.kpad + kpos] = hi;
+        reinterpret_cast<__bf16*>(q.bg)[(uint64_t)(q.lo_off + c) * q.kpad + kpos] =
+            (__bf16)(float)(x - (double)(float)hi);
+      } else if (q.bprec == PREC_FP32) {
+        reinterpret_cast<float*>(q.bg)[(uint64_t)c * q.kpad + kpos] = (float)x;
+      } else {
+        reinterpret_cast<double*>(q.bg)[(uint64_t)c * q.kpad + kpos] = x;
+      }
+    }
+}
+
+template <int NB>
+__device__ inline void write_omega_tile(const QrPass& q, uint64_t tile, const f64x4 (&om)[NB], double* stg) {
+  constexpr int LD = NB * 16 + 1;
+  const int lane = threadIdx.x & 63, kr = lane >> 4, ci = lane & 15;
+  const uint32_t l = q.l;
+#pragma unroll
+  for (int c2 = 0; c2 < NB; ++c2)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) stg[ci * LD + c2 * 16 + kr + 4 * v] = om[c2][v];
+  __builtin_amdgcn_wave_barrier();
+  const uint64_t r0 = tile * 16;
+  const uint32_t nr = (uint32_t)std::min<uint64_t>(16, q.n - r0);
+  const uint32_t ne = nr * l;
+  for (uint32_t e = lane; e < ne; e += 64) {
+    const uint32_t r = e / l, c = e - r * l;
+    const double x = stg[r * LD + c];
+    const uint64_t gi = r0 * l + e;
+    q.omega[gi] = x;
+    if (q.zero_acc) q.zero_acc[gi] = 0.0;
+    if (q.omega32) q.omega32[gi] = (float)x;
+  }
+  if (q.bprec == PREC_SPLIT_BF16) {
+    __bf16* bg = reinterpret_cast<__bf16*>(q.bg);
+    for (uint32_t e = lane; e < l * 4; e += 64) {
+      const uint32_t c = e >> 2, g = e & 3;
+      __bf16 hi[4], lo[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {  // position 4 g + j holds row kperm16(4 g + j) = 4 j + g of the tile
+        const double x = stg[(q.kperm ? 4 * j + g : 4 * g + j) * LD + c];
+        hi[j] = (__bf16)(float)x;
+        lo[j] = (__bf16)(float)(x - (double)(float)hi[j]);
+      }
+      uint2 h2, l2;
+      __builtin_memcpy(&h2, hi, 8);
+      __builtin_memcpy(&l2, lo, 8);
+      *reinterpret_cast<uint2*>(bg + (uint64_t)c * q.kpad + r0 + 4 * g) = h2;
+      *reinterpret_cast<uint2*>(bg + (uint64_t)(q.lo_off + c) * q.kpad + r0 + 4 * g) = l2;
+    }
+  } else {
+    for (uint32_t e = lane; e < ne; e += 64) {
+      const uint32_t r = e / l, c = e - r * l;
+      const double x = stg[r * LD + c];
+      const uint64_t kpos = q.kperm ? kperm16(r0 + r) : r0 + r;
+      if (q.bprec == PREC_FP32)
+        reinterpret_cast<float*>(q.bg)[(uint64_t)c * q.kpad + kpos] = (float)x;
+      else
+        reinterpret_cast<double*>(q.bg)[(uint64_t)c * q.kpad + kpos] = x;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// flipOmg's decision from the blocks' column statistics: wave w sums partials
+// w, w + 8, ... (four loads in flight per lane), the waves' sums add in wave
+// order; then each column's sign is the flip on the signed Omega (the
+// Householder sign of a column swaps its two distances) times that sign, and
+// colsum is the unsigned column sum.  red: kWaves * STAT doubles of LDS.
+template <int NB>
+__device__ void settle_signs(const QrPass& q, const double* stats, uint32_t nparts, double* red) {
+  using S = Shape<NB>;
+  constexpr int W = S::W, J = (S::STAT + 63) / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t t = threadIdx.x;
+  double acc[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) acc[j] = 0.0;
+  for (uint32_t b0 = wv; b0 < nparts; b0 += 4 * kWaves) {
+    double v[4][J];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const uint32_t b = b0 + u * kWaves, e = lane + 64 * j;
+        v[u][j] = (b < nparts && e < (uint32_t)S::STAT) ? stats[(uint64_t)b * S::STAT + e] : 0.0;
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < J; ++j) acc[j] += v[u][j];
+  }
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+    if (lane + 64 * j < S::STAT) red[wv * S::STAT + lane + 64 * j] = acc[j];
+  __syncthreads();
+  if (t < q.l) {
+    double tot[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int e = k * W + t;
+      tot[k] = ((red[e] + red[S::STAT + e]) + (red[2 * S::STAT + e] + red[3 * S::STAT + e])) +
+               ((red[4 * S::STAT + e] + red[5 * S::STAT + e]) + (red[6 * S::STAT + e] + red[7 * S::STAT + e]));
+    }
+    const double hs = q.sign[t];
+    const double a = (hs > 0.0) ? tot[0] : tot[1];
+    const double b = (hs > 0.0) ? tot[1] : tot[0];
+    q.fsign[t] = ((a > 2.0 * b) ? -1.0 : 1.0) * hs;
+    q.colsum[t] = tot[2];
+  }
+}
+
 template <int NB>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_qr_flip(QrPass q) {
   using S = Shape<NB>;
@@ -648,11 +787,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
   // the top block of Omega for the one-wave sign LU and its row / column exchange
   __shared__ double btw[wave_v<NB>() > 0 ? W * W : 1];
   __shared__ double lus[wave_v<NB>() > 0 ? 2 * W : 1];
+  constexpr bool kStage = NB <= 3;  // LDS room for the staged writes
+  __shared__ double stg[kStage ? kWaves * 16 * (W + 1) : 1];  // each wave's tile on its way out
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int kr = lane >> 4, ci = lane & 15;
   const uint32_t l = q.l, t = threadIdx.x, nt = blockDim.x;
   constexpr int V = wave_v<NB>();
   const bool wave_lu = l <= 8 * V;
+  // the previous Omega's column signs (read here, before the last block
+  // replaces them; the prologue's barriers order them before their use)
+  __shared__ double fps[W];
+  for (uint32_t c = t; c < (uint32_t)W; c += nt) fps[c] = c < l ? q.fsign[c] : 1.0;
   // the wave's first tile of H is in flight during the prologue (for l <= 48;
   // above, the registers go to the prologue)
   constexpr bool kPre = NB <= 3;
@@ -695,10 +840,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
     for (int v = 0; v < 4; ++v) dm[c2][v] = dp[c2][v] = cs[c2][v] = 0.0;
   for (uint64_t tile = t0; tile < t1; ++tile) {
     if (!kPre || tile != t0) load_tile<NB>(q.H, q.n, l, tile, x);
-    if (kPre) load_prev<NB>(q.omg2, q.n, l, tile, o);  // waited for after the products
+    // the previous Omega's rows of this tile are read before the tile's new
+    // rows replace them (same wave, same addresses)
+    if (kPre) load_prev<NB>(q.omega, q.n, l, tile, o);  // waited for after the products
     f64x4 om[NB];
     tile_omega<NB>(x, l, r1, r2, om);
-    if (!kPre) load_prev<NB>(q.omg2, q.n, l, tile, o);
+    if (!kPre) load_prev<NB>(q.omega, q.n, l, tile, o);
     const uint64_t row = tile * 16 + ci;
 #pragma unroll
     for (int c2 = 0; c2 < NB; ++c2)
@@ -706,12 +853,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
       for (int v = 0; v < 4; ++v) {
         const uint32_t c = c2 * 16 + kr + 4 * v;
         if (row < q.n && c < l) {
-          const double y = om[c2][v], p = o[c2][v];
+          const double y = om[c2][v], p = fps[c] * o[c2][v];
           dm[c2][v] += fabs(p - y);
           dp[c2][v] += fabs(p + y);
           cs[c2][v] += y;
         }
       }
+    if constexpr (kStage)
+      write_omega_tile<NB>(q, tile, om, stg + wv * 16 * (W + 1));
+    else
+      write_omega_tile_direct<NB>(q, tile, om);
   }
   // rows are spread over lane % 16: fold them, then the waves
 #pragma unroll
@@ -736,76 +887,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
         red[wv * S::STAT + 2 * W + c] = cs[c2][v];
       }
   __syncthreads();
-  double* dst = q.part + (uint64_t)2 * kMaxBlocks * S::GRAM + (uint64_t)blockIdx.x * S::STAT;
+  double* const stats = q.part + (uint64_t)2 * kMaxBlocks * S::GRAM;
   for (int e = t; e < S::STAT; e += nt) {
     double s = 0.0;
     for (int w = 0; w < kWaves; ++w) s += red[w * S::STAT + e];
-    dst[e] = s;
+    stats[(uint64_t)blockIdx.x * S::STAT + e] = s;
   }
 }
 
+// flipOmg's decision in one block once every block's statistics are in.
 template <int NB>
-__global__ __launch_bounds__(256) void k_qr_write(QrPass q) {
+__global__ __launch_bounds__(kThreads) void k_qr_signs(QrPass q) {
   using S = Shape<NB>;
-  constexpr int W = S::W;
-  __shared__ double r1[W * W], rf[W * W];
-  __shared__ double fs[W];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int kr = lane >> 4, ci = lane & 15;
-  const uint32_t l = q.l, t = threadIdx.x;
-  // the wave's tile of H is in flight during the prologue
-  const uint64_t tiles = ceil_div(q.n, 16);
-  uint64_t tile = (uint64_t)blockIdx.x * 4 + wv;
-  double x[NB * 4];
-  if (tile < tiles) load_tile<NB>(q.H, q.n, l, tile, x);
-  stage_padded(q.R1i, l, W, r1);
-  stage_padded(q.R2i, l, W, rf);
-  // flipOmg on the signed Omega (the sign of a column swaps its two
-  // distances): the statistics partials summed in block order
-  __shared__ double stat[S::STAT];
-  if (t < S::STAT) stat[t] = q.gram[t];
-  __syncthreads();
-  if (t < l) {
-    const double s = q.sign[t];
-    const double a = (s > 0.0) ? stat[t] : stat[W + t];
-    const double b = (s > 0.0) ? stat[W + t] : stat[t];
-    const double f = (a > 2.0 * b) ? -1.0 : 1.0;
-    fs[t] = f * s;
-    if (blockIdx.x == 0) q.colsum[t] = f * s * stat[2 * W + t];
-  }
-  __syncthreads();
-  for (int e = t; e < W * W; e += blockDim.x) rf[e] *= (e % W < (int)l) ? fs[e % W] : 0.0;
-  __syncthreads();
-  for (; tile < tiles; tile += (uint64_t)gridDim.x * 4) {
-    f64x4 om[NB];
-    tile_omega<NB>(x, l, r1, rf, om);
-    if (tile + (uint64_t)gridDim.x * 4 < tiles) load_tile<NB>(q.H, q.n, l, tile + (uint64_t)gridDim.x * 4, x);
-    const uint64_t row = tile * 16 + ci;
-    const uint64_t kpos = q.kperm ? kperm16(row) : row;
-#pragma unroll
-    for (int c2 = 0; c2 < NB; ++c2)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const uint32_t c = c2 * 16 + kr + 4 * v;
-        if (c >= l || row >= q.n) continue;
-        const double x = om[c2][v];
-        const uint64_t gi = row * l + c;
-        q.omega[gi] = x;
-        if (q.omg2 != q.omega) q.omg2[gi] = x;
-        if (q.zero_acc) q.zero_acc[gi] = 0.0;
-        if (q.omega32) q.omega32[gi] = (float)x;
-        if (q.bprec == PREC_SPLIT_BF16) {
-          const __bf16 hi = (__bf16)(float)x;
-          reinterpret_cast<__bf16*>(q.bg)[(uint64_t)c * q.kpad + kpos] = hi;
-          reinterpret_cast<__bf16*>(q.bg)[(uint64_t)(q.lo_off + c) * q.kpad + kpos] =
-              (__bf16)(float)(x - (double)(float)hi);
-        } else if (q.bprec == PREC_FP32) {
-          reinterpret_cast<float*>(q.bg)[(uint64_t)c * q.kpad + kpos] = (float)x;
-        } else {
-          reinterpret_cast<double*>(q.bg)[(uint64_t)c * q.kpad + kpos] = x;
-        }
-      }
-  }
+  __shared__ double red[kWaves * S::STAT];
+  settle_signs<NB>(q, q.part + (uint64_t)2 * kMaxBlocks * S::GRAM, q.nparts, red);
 }
 
 // out = the sum of nparts partials of count doubles (stride apart), in block
@@ -861,7 +956,6 @@ void launch_nb(QrPass q, int pass, hipStream_t s) {
   const uint64_t tiles = ceil_div(q.n, 16);
   // reduction passes: about one tile per wave, at most kMaxBlocks partials
   q.nparts = (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(tiles, kWaves), 1), kMaxBlocks);
-  const uint32_t nwrite = (uint32_t)std::max<uint64_t>(ceil_div(tiles, 4), 1);
   using S = Shape<NB>;
   switch (pass) {
     case 0:
@@ -872,12 +966,10 @@ void launch_nb(QrPass q, int pass, hipStream_t s) {
       hipLaunchKernelGGL(k_qr_gram_q<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q);
       launch_parts_sum<NB>(q.part + (uint64_t)kMaxBlocks * S::GRAM, q.nparts, S::GRAM, S::GRAM, q.l, 0, q.gram, s);
       break;
-    case 2:
+    default:
       hipLaunchKernelGGL(k_qr_flip<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q);
-      launch_parts_sum<NB>(q.part + (uint64_t)2 * kMaxBlocks * S::GRAM, q.nparts, S::STAT, S::STAT, q.l, 0, q.gram,
-                           s);
+      hipLaunchKernelGGL(k_qr_signs<NB>, dim3(1), dim3(kThreads), 0, s, q);
       break;
-    default: hipLaunchKernelGGL(k_qr_write<NB>, dim3(nwrite), dim3(256), 0, s, q); break;
   }
 }
 
@@ -1014,6 +1106,8 @@ void launch_qr_pass(const QrPass& q, int pass, hipStream_t s) {
   PCAONE_REQUIRE(q.l >= 1 && q.l <= 64, -1, "l must be in [1, 64]");
   PCAONE_REQUIRE(q.n >= 2ull * q.l, -1, "the Omega update needs at least 2 l rows");
   PCAONE_REQUIRE(!q.kperm || q.kpad % 16 == 0, -1, "omega operand: permuted K needs 16-code groups");
+  PCAONE_REQUIRE(q.bprec != PREC_SPLIT_BF16 || (q.kpad % 16 == 0 && q.kpad >= ceil_div(q.n, 16) * 16), -1,
+                 "omega operand: the split operand is written in whole 16-position groups");
   switch ((q.l + 15) / 16) {
     case 1: launch_nb<1>(q, pass, s); break;
     case 2: launch_nb<2>(q, pass, s); break;
