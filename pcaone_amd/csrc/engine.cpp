@@ -267,6 +267,7 @@ uint32_t Engine::packed_split(int prec, uint64_t rows, uint32_t nz, uint32_t kch
 void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* perm) {
   PCAONE_REQUIRE(have_bed_, -3, "rsvd_init: no data loaded");
   PCAONE_HIP(hipSetDevice(device_));
+  side_ = SideJobs{};  // pending jobs point into buffers this may reallocate
   have_mv_ = false;
   PCAONE_REQUIRE(p.variant == 1 || p.variant == 2, -1, "rsvd_init: variant must be 1 (normal) or 2 (fancy)");
   PCAONE_REQUIRE(p.precision >= 0 && p.precision <= 2, -1, "rsvd_init: bad precision");
@@ -399,6 +400,7 @@ void Engine::rsvd_init(const RsvdParams& p, const double* omega, const int32_t* 
   for (auto* b : {&R1_, &R1i_, &R2_, &R2i_, &Rt_, &Rti_, &W_}) b->reserve((uint64_t)l_ * l_);
   sign_.reserve(l_);
   fsign_.reserve(l_);
+  qstat_.reserve(qr_stat_doubles());
   evals_.reserve(l_);
   sinv_.reserve(l_);
   Bt_.reserve((uint64_t)N_ * l_);
@@ -578,6 +580,7 @@ void Engine::partial_rows(int prec, uint64_t& gmax, uint64_t& hmax) const {
 // ---------------------------------------------------------------------------
 void Engine::gram_init(int precision, bool standardize, int ploidy) {
   PCAONE_REQUIRE(have_bed_, -3, "gram_init: no data loaded");
+  side_ = SideJobs{};  // pending jobs point into buffers this may reallocate
   PCAONE_REQUIRE(precision >= 0 && precision <= 2, -1, "gram_init: bad precision");
   RsvdParams p;
   p.variant = 1;  // identity SNP order, windows only tile the H contraction
@@ -762,6 +765,7 @@ void Engine::init_omega() {
   const size_t nb = (size_t)N_ * l_ * 8;
   PCAONE_HIP(hipMemcpyAsync(omega_.p, omega0_.p, nb, hipMemcpyDeviceToDevice, stream_));
   PCAONE_HIP(hipMemcpyAsync(colsum_.p, colsum0_.p, (size_t)l_ * 8, hipMemcpyDeviceToDevice, stream_));
+  side_.stats = nullptr;  // a pending sign job of an earlier refresh is stale
   launch_fill_f64(fsign_.p, l_, 1.0, stream_);  // the drawn Omega carries its signs
   timed("omega_operand",
         [&] { launch_omega_operand(omega_.p, N_, l_, p_.precision, bg_.p, bcols_, lo_off_, kg_, kperm_(p_.precision),
@@ -780,6 +784,7 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
     omega_refresh_wide(from_two, zero_acc);
     return;
   }
+  flush_side(true, true);  // the previous refresh's signs before this one's flip reads them
   const SegBufs b = op_bufs();
   QrPass q{};
   q.H1 = from_two ? H1_.p : H_.p;
@@ -794,6 +799,7 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
   q.R2i = R2i_.p;
   q.sign = sign_.p;
   q.fsign = fsign_.p;
+  q.stats = qstat_.p;
   q.colsum = colsum_.p;
   q.omega = omega_.p;
   q.zero_acc = zero_acc;
@@ -806,6 +812,15 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
   timed("qr_gram_h", [&] { launch_qr_pass(q, 0, stream_); });
   timed("qr_gram_q", [&] { launch_qr_pass(q, 1, stream_); });
   timed("qr_flip", [&] { launch_qr_pass(q, 2, stream_); });
+  // the column signs settle in the next gather's first block (or on their own)
+  const SideJobs sj = qr_sign_job(q);
+  side_.stats = sj.stats;
+  side_.nparts = sj.nparts;
+  side_.W = sj.W;
+  side_.l = sj.l;
+  side_.sign = sj.sign;
+  side_.fsign = sj.fsign;
+  side_.colsum = sj.colsum;
 }
 
 // The same refresh for l > 64 in general form: Gram of H (= H1 + H2, written),
@@ -814,6 +829,7 @@ void Engine::omega_refresh(bool from_two, double* zero_acc) {
 // against the previous Omega, then the signed, flipped Omega into every
 // consumer.
 void Engine::omega_refresh_wide(bool from_two, double* zero_acc) {
+  flush_side(true, true);
   const SegBufs b = op_bufs();
   const double* H1 = from_two ? H1_.p : H_.p;
   const double* H2 = from_two ? H2_.p : nullptr;
@@ -855,6 +871,42 @@ void Engine::omega_refresh_wide(bool from_two, double* zero_acc) {
   w.kperm = kperm_(p_.precision);
   timed("qr_write", [&] { launch_wide_write(w, stream_); });
   launch_fill_f64(fsign_.p, l_, 1.0, stream_);  // written signed
+}
+
+// The pending side jobs selected, taken out of side_.
+SideJobs Engine::take_side(bool signs, bool colsum) {
+  SideJobs j;
+  if (signs && side_.stats) {
+    j.stats = side_.stats;
+    j.nparts = side_.nparts;
+    j.W = side_.W;
+    j.l = side_.l;
+    j.sign = side_.sign;
+    j.fsign = side_.fsign;
+    j.colsum = side_.colsum;
+    side_.stats = nullptr;
+  }
+  if (colsum && side_.s2p) {
+    j.s2p = side_.s2p;
+    j.nchunk = side_.nchunk;
+    j.w0 = side_.w0;
+    j.w1 = side_.w1;
+    j.l2 = side_.l2;
+    j.S2 = side_.S2;
+    side_.s2p = nullptr;
+  }
+  return j;
+}
+
+void Engine::flush_side(bool signs, bool colsum) {
+  if (signs && side_.stats) {
+    const SideJobs j = take_side(true, false);
+    timed("qr_signs", [&] { launch_side_jobs(j, stream_); });
+  }
+  if (colsum && side_.s2p) {
+    const SideJobs j = take_side(false, true);
+    timed("window_colsum", [&] { launch_side_jobs(j, stream_); });
+  }
 }
 
 // the FP32 table the missing-entry gathers of the G contraction read
@@ -935,10 +987,13 @@ void Engine::seg_G_on(uint32_t w0, uint32_t w1, const SegBufs& b, const uint8_t*
     cg.per = rows;
     cg.ngroups = 1;
     cg.out = b.corrG;
+    cg.side = take_side(true, false);  // the last refresh's column signs
     // In line before the contraction: run beside it on a second stream, the
     // gather's random table reads evict the contraction's operand tiles from
     // L2 and cost it more time than the gather takes alone.
     timed("miss_gather_G", [&] { launch_csr_gather(cg, stream_); });
+  } else {
+    flush_side(true, false);
   }
   if (!dense) {
     PackedGemm g{};
@@ -986,7 +1041,19 @@ void Engine::seg_G_on(uint32_t w0, uint32_t w1, const SegBufs& b, const uint8_t*
     fg.split_cols = 0;
   }
   timed("finalize_G", [&] { launch_finalize_G(fg, stream_); });
-  timed("window_colsum", [&] { launch_window_colsum(b.s2p, b.l, bs_, w0, w1, b.S2, stream_); });
+  // the window sums of g'' are first read by finalize_H: they ride on the H
+  // product's gather when there is one (resident windows)
+  flush_side(false, true);
+  const SideJobs cj = window_colsum_job(b.s2p, b.l, bs_, w0, w1, b.S2);
+  side_.s2p = cj.s2p;
+  side_.nchunk = cj.nchunk;
+  side_.w0 = cj.w0;
+  side_.w1 = cj.w1;
+  side_.l2 = cj.l2;
+  side_.S2 = cj.S2;
+  // the operator's own windows in HBM only (the mat-vec's buffers and the
+  // streamed windows sum at once)
+  if (streamed_ || b.S2 != S2_.p) flush_side(false, true);
 }
 
 FinalizeG Engine::finalize_desc(uint32_t w0, uint32_t w1, const SegBufs& b) const {
@@ -1059,6 +1126,7 @@ void Engine::seg_H_on(uint32_t w0, uint32_t w1, const std::vector<int>& route, c
     cg.per = N_;
     cg.ngroups = nz;
     cg.out = b.corrH;
+    cg.side = take_side(false, true);  // finalize_G's window sums
     timed("miss_gather_H", [&] { launch_csr_gather(cg, stream_); });
   }
   if (!dense) {
@@ -1105,6 +1173,7 @@ void Engine::seg_H_on(uint32_t w0, uint32_t w1, const std::vector<int>& route, c
     h_nsplit = h.nsplit;
     timed("gemm_H", [&] { launch_dense_gemm(h, prec, (int)b.l, b.ntiles, stream_); });
   }
+  flush_side(false, true);  // S2 when no gather carried it
   FinalizeH fh{};
   fh.part = b.hpart;
   fh.part_is_f64 = b.part_f64;

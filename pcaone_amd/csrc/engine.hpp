@@ -411,6 +411,13 @@ class Engine {
   // linear algebra scratch
   DevBuf<double> part_, R1_, R1i_, R2_, R2i_, Rt_, Rti_, sign_;
   DevBuf<double> fsign_;           // Omega's column signs (omega_ and colsum_ are unsigned)
+  DevBuf<double> qstat_;           // the flip pass's statistics, kept until the signs settle
+  // Jobs deferred to the next missing-entry gather (which does not need them):
+  // the sign settle after an Omega refresh (stats != null) and the window sums
+  // after finalize_G (s2p != null).  flush_side launches them on their own.
+  SideJobs side_;
+  SideJobs take_side(bool signs, bool colsum);
+  void flush_side(bool signs, bool colsum);
   DevBuf<double> gred_, bbt_;  // computeUSV: reduced Gram between passes, B B^T
   DevBuf<double> qred_;        // Omega update: reduced partials between passes
   DevBuf<double> Bt_, evals_, W_, sinv_, Ucur_, Upre_, V_, scal_;

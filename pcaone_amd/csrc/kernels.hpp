@@ -172,6 +172,25 @@ void launch_finalize_H(const FinalizeH& f, hipStream_t s);
 // the table's type, FP32 or FP64).  List rows come in ngroups groups of `per` rows: row o reads
 // the list at in_base + (o / per) * stride_in + o % per and writes out row
 // out_base + o.
+// Jobs for extra blocks of a launch that does not need their results (see
+// side_jobs.hpp); a null stats / s2p pointer leaves that job out.
+struct SideJobs {
+  // flipOmg's column signs: the flip pass's statistics [nparts][3 W]
+  const double* stats = nullptr;
+  uint32_t nparts = 0, W = 0, l = 0;
+  const double* sign = nullptr;  // [l] Householder signs
+  double* fsign = nullptr;       // [l] out: column signs
+  double* colsum = nullptr;      // [l] out: unsigned column sums
+  // window sums S2[w][c] = sum over chunks of s2p[w][chunk][c], windows [w0, w1)
+  const double* s2p = nullptr;
+  uint32_t nchunk = 0, w0 = 0, w1 = 0, l2 = 0;
+  double* S2 = nullptr;
+};
+// the jobs in a launch of their own
+void launch_side_jobs(const SideJobs& j, hipStream_t s);
+// the window sums of finalize_G's chunk partials (k_finalize_G's s2p) as a job
+SideJobs window_colsum_job(const double* s2p, uint32_t l, uint32_t blocksize, uint32_t w0, uint32_t w1, double* S2);
+
 struct CsrGather {
   const int64_t* ptr;
   const int32_t* idx;
@@ -182,6 +201,8 @@ struct CsrGather {
   uint64_t in_base, stride_in, out_base;
   uint32_t per, ngroups;
   void* out;  // [.][l] of the table's type
+  SideJobs side;     // run by the first blocks of the launch
+  uint32_t gblocks;  // set by the launcher: the blocks that gather
 };
 void launch_csr_gather(const CsrGather& g, hipStream_t s);
 // FP64 -> FP32 copy
@@ -324,6 +345,7 @@ struct QrPass {
   double* fsign;     // l: column signs of the previous Omega in, of the new one out
                      // (Householder sign times flipOmg's flip)
   double* colsum;    // l: column sums of the new (unsigned) Omega
+  double* stats;     // qr_stat_doubles(): the flip pass's block statistics (read by qr_sign_job's job)
   double* omega;     // n x l FP64, unsigned: the previous Omega in, the new one out
   double* zero_acc;  // optional n x l accumulator to clear
   float* omega32;    // optional n x l FP32 copy
@@ -334,9 +356,13 @@ struct QrPass {
   int kperm;         // operand positions in kperm16 order
 };
 uint64_t qr_part_doubles(uint32_t l);
+uint64_t qr_stat_doubles();
 // pass 0: H and the Gram of H; 1: R1^-1 and the Gram of Q1; 2: R2^-1, the
-// Householder signs, the unsigned Omega into every consumer, colsum and fsign.
+// Householder signs, the unsigned Omega into every consumer and the flip
+// statistics.  qr_sign_job then settles fsign and colsum (on its own or as the
+// side job of the next gather).
 void launch_qr_pass(const QrPass& q, int pass, hipStream_t s);
+SideJobs qr_sign_job(const QrPass& q);
 // computeUSV's linear algebra without materialising Q: CholeskyQR2 factors of
 // G (M x l) and B^T = H (R2 R1)^-1 with B B^T for the eigensolve.
 struct UsvPass {

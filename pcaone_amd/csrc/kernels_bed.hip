@@ -15,6 +15,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "side_jobs.hpp"
 
 namespace pcaone {
 
@@ -945,39 +946,22 @@ void launch_operand_from_G(const FinalizeG& f, hipStream_t s) {
 
 // S2[w][c] = sum of the chunk partials of window w: 16 slices of chunks (all
 // of a slice's loads in flight), combined in a fixed order
-__global__ __launch_bounds__(1024) void k_window_colsum(const double* __restrict__ s2p, uint32_t l, uint32_t nchunk,
-                                                        uint32_t w0, double* __restrict__ S2) {
-  constexpr int kSlices = 16, kPer = 8;
-  __shared__ double sh[kSlices][64];
-  const uint32_t w = w0 + blockIdx.x, cl = threadIdx.x & 63, c = blockIdx.y * 64 + cl, sl = threadIdx.x >> 6;
-  double tot = 0.0;
-  if (c < l)
-    for (uint32_t q0 = sl; q0 < nchunk; q0 += kSlices * kPer) {
-      double v[kPer];
-#pragma unroll
-      for (int i = 0; i < kPer; ++i) {
-        const uint32_t q = q0 + kSlices * i;
-        v[i] = q < nchunk ? s2p[((uint64_t)w * nchunk + q) * l + c] : 0.0;
-      }
-#pragma unroll
-      for (int i = 0; i < kPer; ++i) tot += v[i];
-    }
-  sh[sl][cl] = tot;
-  __syncthreads();
-  if (threadIdx.x < 64 && c < l) {
-    double v[kSlices / 2];
-#pragma unroll
-    for (int i = 0; i < kSlices / 2; ++i) v[i] = sh[2 * i][cl] + sh[2 * i + 1][cl];
-    S2[(uint64_t)w * l + c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-  }
-}
-
 void launch_window_colsum(const double* s2p, uint32_t l, uint32_t blocksize, uint32_t w0, uint32_t w1, double* S2,
                           hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= 128, -1, "l out of range");
   if (w1 <= w0) return;
-  hipLaunchKernelGGL(k_window_colsum, dim3(w1 - w0, (uint32_t)ceil_div(l, 64)), dim3(1024), 0, s, s2p, l,
-                     finalize_chunks(blocksize), w0, S2);
+  launch_side_jobs(window_colsum_job(s2p, l, blocksize, w0, w1, S2), s);
+}
+
+SideJobs window_colsum_job(const double* s2p, uint32_t l, uint32_t blocksize, uint32_t w0, uint32_t w1, double* S2) {
+  SideJobs j;
+  j.s2p = s2p;
+  j.nchunk = finalize_chunks(blocksize);
+  j.w0 = w0;
+  j.w1 = w1;
+  j.l2 = l;
+  j.S2 = S2;
+  return j;
 }
 
 // thread (sample j, column c); the windows' contributions are summed in
@@ -1147,10 +1131,18 @@ void launch_map_missing(const int64_t* wptr, const int32_t* widx, uint32_t nwin,
 // launch from 387 to 257 us; four entries per round in flight take a
 // 25 M-entry list from 296 to 273 us (500,000 rows) and 324 to 292 us
 // (640,000 window rows).
+// The first side_blocks(g.side) blocks run the side jobs instead.
 template <class T, int V>
 __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
   typedef T vec __attribute__((ext_vector_type(V)));
   constexpr int D = 4;  // entries per group per round
+  const uint32_t nside = side_blocks(g.side);
+  if (blockIdx.x < nside) {
+    __shared__ double red[kSideLds];
+    side_block(g.side, blockIdx.x, red);
+    return;
+  }
+  const uint32_t bid = blockIdx.x - nside;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int lpg = (int)((g.ncols + V - 1) / V);
   const int ng = 64 / lpg;
@@ -1159,7 +1151,7 @@ __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
   const uint32_t coff = act ? g.col0 + (uint32_t)(li * V) : g.col0;
   const T* __restrict__ tab = reinterpret_cast<const T*>(g.table);
   const uint64_t total = (uint64_t)g.per * g.ngroups;
-  for (uint64_t o = (uint64_t)blockIdx.x * 4 + wv; o < total; o += (uint64_t)gridDim.x * 4) {
+  for (uint64_t o = (uint64_t)bid * 4 + wv; o < total; o += (uint64_t)g.gblocks * 4) {
     const uint64_t q = o / g.per;
     const uint64_t row = g.in_base + q * g.stride_in + (o - q * g.per);
     const int64_t p0 = g.ptr[row], p1 = g.ptr[row + 1];
@@ -1207,8 +1199,11 @@ __global__ __launch_bounds__(256) void k_csr_gather(CsrGather g) {
 void launch_csr_gather(const CsrGather& g0, hipStream_t s) {
   PCAONE_REQUIRE(g0.l >= 1 && g0.l <= 128, -1, "gather: l must be in [1, 128]");
   const uint64_t total = (uint64_t)g0.per * g0.ngroups;
-  if (!total) return;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>(ceil_div(total, 4), 32768);
+  if (!total) {
+    launch_side_jobs(g0.side, s);
+    return;
+  }
+  const uint32_t gblocks = (uint32_t)std::min<uint64_t>(ceil_div(total, 4), 32768);
   // 16-byte vector loads need rows that are whole vectors (tables are hipMalloc'd);
   // a lane group spans at most the 64 lanes of a wave, so rows of single
   // columns wider than 64 run as two column windows
@@ -1218,6 +1213,9 @@ void launch_csr_gather(const CsrGather& g0, hipStream_t s) {
     CsrGather g = g0;
     g.col0 = c0;
     g.ncols = std::min(wmax, g0.l - c0);
+    g.gblocks = gblocks;
+    if (c0 > 0) g.side = SideJobs{};  // the jobs ride on the first launch only
+    const uint32_t blocks = gblocks + side_blocks(g.side);
     if (g.table_f32) {
       if (V == 4)
         hipLaunchKernelGGL((k_csr_gather<float, 4>), dim3(blocks), dim3(256), 0, s, g);
@@ -1232,6 +1230,18 @@ void launch_csr_gather(const CsrGather& g0, hipStream_t s) {
         hipLaunchKernelGGL((k_csr_gather<double, 1>), dim3(blocks), dim3(256), 0, s, g);
     }
   }
+}
+
+__global__ __launch_bounds__(kSideThreads) void k_side_jobs(SideJobs j) {
+  __shared__ double red[kSideLds];
+  side_block(j, blockIdx.x, red);
+}
+
+void launch_side_jobs(const SideJobs& j, hipStream_t s) {
+  PCAONE_REQUIRE(!j.stats || (j.W >= 1 && j.W <= 64 && j.l <= j.W), -1, "side jobs: bad sign job");
+  PCAONE_REQUIRE(!j.s2p || j.l2 >= 1, -1, "side jobs: bad column-sum job");
+  const uint32_t nb = side_blocks(j);
+  if (nb) hipLaunchKernelGGL(k_side_jobs, dim3(nb), dim3(kSideThreads), 0, s, j);
 }
 
 __global__ void k_cast_f32(const double* __restrict__ src, float* __restrict__ dst, uint64_t n) {

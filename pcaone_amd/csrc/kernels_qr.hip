@@ -13,13 +13,16 @@
 //                kernels, the FP32 table of the missing-entry gathers; the
 //                emptied H accumulator is zeroed) and block partials of
 //                flipOmg's column statistics against the previous Omega
-//   k_qr_signs   one block: the statistics summed, each column's sign settled
+//   (signs)      one block sums the statistics and settles each column's sign:
+//                a side job (side_jobs.hpp) of the next missing-entry gather,
+//                or a launch of its own
 //
 // Omega is kept unsigned: a column's sign (Householder sign times flipOmg's
 // flip) lives in `fsign`, and G = X Omega takes it in its epilogue, where every
 // term (the product, the column sums, the missing-entry sums) is linear in the
-// column.  The tall N x l data is read once per pass.  A small launch after
-// each pass sums its block partials in a fixed order (deterministic); each
+// column.  The tall N x l data is read once per pass.  A small launch (or
+// side job) after each pass sums its block partials in a fixed order
+// (deterministic); each
 // prologue reads that sum and runs the l x l step redundantly in every block
 // (block 0 stores the results), so no block ever waits for another inside a
 // launch.
@@ -642,8 +645,8 @@ __device__ void householder_signs_block(double* B, uint32_t l, double* sign) {
 // The unsigned Omega of one tile (tile_omega layout) into every consumer,
 // through the wave's 16 x (W + 1) LDS rows `stg`: the tile's rows are
 // contiguous in the row-major copies, and the split operand takes four
-// positions per 8-byte store (rows past n are zero in the tile, as the
-// operand's padding is).  Rolled loops keep the write phase's registers few.
+// positions per 8-byte store (zeros for rows past n: the operand's padding).
+// Rolled loops keep the write phase's registers few.
 // The same writes element by element from the registers (l > 48, where the
 // staging rows do not fit the LDS beside the prologue's matrices).
 template <int NB>
@@ -705,7 +708,10 @@ __device__ inline void write_omega_tile(const QrPass& q, uint64_t tile, const f6
       __bf16 hi[4], lo[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {  // position 4 g + j holds row kperm16(4 g + j) = 4 j + g of the tile
-        const double x = stg[(q.kperm ? 4 * j + g : 4 * g + j) * LD + c];
+        const uint32_t r = q.kperm ? 4 * j + g : 4 * g + j;
+        // rows past n are the operand's zero padding, whatever the tile holds
+        // there (a broken-down factor leaves 0 * inf = NaN)
+        const double x = r < nr ? stg[r * LD + c] : 0.0;
         hi[j] = (__bf16)(float)x;
         lo[j] = (__bf16)(float)(x - (double)(float)hi[j]);
       }
@@ -727,54 +733,6 @@ __device__ inline void write_omega_tile(const QrPass& q, uint64_t tile, const f6
     }
   }
   __builtin_amdgcn_wave_barrier();
-}
-
-// flipOmg's decision from the blocks' column statistics: wave w sums partials
-// w, w + 8, ... (four loads in flight per lane), the waves' sums add in wave
-// order; then each column's sign is the flip on the signed Omega (the
-// Householder sign of a column swaps its two distances) times that sign, and
-// colsum is the unsigned column sum.  red: kWaves * STAT doubles of LDS.
-template <int NB>
-__device__ void settle_signs(const QrPass& q, const double* stats, uint32_t nparts, double* red) {
-  using S = Shape<NB>;
-  constexpr int W = S::W, J = (S::STAT + 63) / 64;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const uint32_t t = threadIdx.x;
-  double acc[J];
-#pragma unroll
-  for (int j = 0; j < J; ++j) acc[j] = 0.0;
-  for (uint32_t b0 = wv; b0 < nparts; b0 += 4 * kWaves) {
-    double v[4][J];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int j = 0; j < J; ++j) {
-        const uint32_t b = b0 + u * kWaves, e = lane + 64 * j;
-        v[u][j] = (b < nparts && e < (uint32_t)S::STAT) ? stats[(uint64_t)b * S::STAT + e] : 0.0;
-      }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int j = 0; j < J; ++j) acc[j] += v[u][j];
-  }
-#pragma unroll
-  for (int j = 0; j < J; ++j)
-    if (lane + 64 * j < S::STAT) red[wv * S::STAT + lane + 64 * j] = acc[j];
-  __syncthreads();
-  if (t < q.l) {
-    double tot[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int e = k * W + t;
-      tot[k] = ((red[e] + red[S::STAT + e]) + (red[2 * S::STAT + e] + red[3 * S::STAT + e])) +
-               ((red[4 * S::STAT + e] + red[5 * S::STAT + e]) + (red[6 * S::STAT + e] + red[7 * S::STAT + e]));
-    }
-    const double hs = q.sign[t];
-    const double a = (hs > 0.0) ? tot[0] : tot[1];
-    const double b = (hs > 0.0) ? tot[1] : tot[0];
-    q.fsign[t] = ((a > 2.0 * b) ? -1.0 : 1.0) * hs;
-    q.colsum[t] = tot[2];
-  }
 }
 
 template <int NB>
@@ -887,20 +845,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 2))
         red[wv * S::STAT + 2 * W + c] = cs[c2][v];
       }
   __syncthreads();
-  double* const stats = q.part + (uint64_t)2 * kMaxBlocks * S::GRAM;
+  double* const stats = q.stats;
   for (int e = t; e < S::STAT; e += nt) {
     double s = 0.0;
     for (int w = 0; w < kWaves; ++w) s += red[w * S::STAT + e];
     stats[(uint64_t)blockIdx.x * S::STAT + e] = s;
   }
-}
-
-// flipOmg's decision in one block once every block's statistics are in.
-template <int NB>
-__global__ __launch_bounds__(kThreads) void k_qr_signs(QrPass q) {
-  using S = Shape<NB>;
-  __shared__ double red[kWaves * S::STAT];
-  settle_signs<NB>(q, q.part + (uint64_t)2 * kMaxBlocks * S::GRAM, q.nparts, red);
 }
 
 // out = the sum of nparts partials of count doubles (stride apart), in block
@@ -951,11 +901,15 @@ void launch_parts_sum(const double* part, int nparts, uint64_t stride, int count
                      count, l, square, out);
 }
 
+// blocks of the reduction passes over n rows: about one 16-row tile per wave,
+// at most kMaxBlocks partials
+int qr_nparts(uint64_t n) {
+  return (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(ceil_div(n, 16), kWaves), 1), kMaxBlocks);
+}
+
 template <int NB>
 void launch_nb(QrPass q, int pass, hipStream_t s) {
-  const uint64_t tiles = ceil_div(q.n, 16);
-  // reduction passes: about one tile per wave, at most kMaxBlocks partials
-  q.nparts = (int)std::min<uint64_t>(std::max<uint64_t>(ceil_div(tiles, kWaves), 1), kMaxBlocks);
+  q.nparts = qr_nparts(q.n);
   using S = Shape<NB>;
   switch (pass) {
     case 0:
@@ -966,10 +920,7 @@ void launch_nb(QrPass q, int pass, hipStream_t s) {
       hipLaunchKernelGGL(k_qr_gram_q<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q);
       launch_parts_sum<NB>(q.part + (uint64_t)kMaxBlocks * S::GRAM, q.nparts, S::GRAM, S::GRAM, q.l, 0, q.gram, s);
       break;
-    default:
-      hipLaunchKernelGGL(k_qr_flip<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q);
-      hipLaunchKernelGGL(k_qr_signs<NB>, dim3(1), dim3(kThreads), 0, s, q);
-      break;
+    default: hipLaunchKernelGGL(k_qr_flip<NB>, dim3(q.nparts), dim3(kThreads), 0, s, q); break;
   }
 }
 
@@ -1076,6 +1027,8 @@ void launch_usv_nb(UsvPass u, hipStream_t s, int step) {
 
 }  // namespace
 
+uint64_t qr_stat_doubles() { return (uint64_t)kMaxBlocks * 3 * 64; }
+
 uint64_t qr_part_doubles(uint32_t l) {
   const uint64_t nb = ceil_div(l, 16), np = nb * (nb + 1) / 2;
   return (uint64_t)kMaxBlocks * (2 * np * 256 + 3 * nb * 16);
@@ -1100,6 +1053,20 @@ void launch_usv_qr(const UsvPass& u, int step, hipStream_t s) {
     case 3: launch_usv_nb<3>(u, s, step); break;
     default: launch_usv_nb<4>(u, s, step); break;
   }
+}
+
+SideJobs qr_sign_job(const QrPass& q) {
+  const uint32_t nb = (q.l + 15) / 16, np = nb * (nb + 1) / 2;
+  (void)np;
+  SideJobs j;
+  j.stats = q.stats;
+  j.nparts = (uint32_t)qr_nparts(q.n);
+  j.W = nb * 16;
+  j.l = q.l;
+  j.sign = q.sign;
+  j.fsign = q.fsign;
+  j.colsum = q.colsum;
+  return j;
 }
 
 void launch_qr_pass(const QrPass& q, int pass, hipStream_t s) {

@@ -887,7 +887,11 @@ void launch_wide_repair(const double* A, const double* gram, uint64_t n, uint32_
                         double* out, hipStream_t s) {
   PCAONE_REQUIRE(l >= 1 && l <= kWideMaxL, -1, "rank repair: l must be in [1, 128]");
   big_lds(k_wide_deficient, kSmallLds);
-  hipLaunchKernelGGL(k_wide_deficient, dim3(1), dim3(kSmallThreads), kSmallLds, s, gram, l, n, 1e-12, mask, scale);
+  // a column whose part outside the earlier ones is below 1e-5 of its norm
+  // (pivot below 1e-10 of its Gram diagonal) is replaced: at 1e-12 a column of
+  // split-BF16 rounding noise (about 1e-6 relative) could be kept and the
+  // Cholesky of the repaired matrix, eliminating in another order, still fail
+  hipLaunchKernelGGL(k_wide_deficient, dim3(1), dim3(kSmallThreads), kSmallLds, s, gram, l, n, 1e-10, mask, scale);
   hipLaunchKernelGGL(k_fill_deficient, dim3((uint32_t)ceil_div(n * l, 256)), dim3(256), 0, s, A, n, l, mask, scale,
                      out);
 }
