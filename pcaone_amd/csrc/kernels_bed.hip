@@ -678,9 +678,9 @@ void launch_packed_gemm_f64(const PackedGemm& g, int ncols, hipStream_t s) { lau
 // ---------------------------------------------------------------------------
 
 // Grid (row chunk of 32, window).  Besides G, g'' and the H operand the
-// epilogue writes the chunk's partial column sums of g'' (s2p), which
-// k_window_colsum reduces into S''.  The operand (column-major per window) is
-// staged through LDS so its stores run along the window's SNPs.
+// epilogue writes the chunk's partial column sums of g'' (s2p), which the
+// window-sum job (side_jobs.hpp) reduces into S''.  The operand (column-major
+// per window) is staged through LDS so its stores run along the window's SNPs.
 constexpr uint32_t kFinRows = 32;  // two 512-thread finalize blocks fit one CU
 
 // bh[w][c][k0 + p] for the operand positions p < 64 of a chunk, from
