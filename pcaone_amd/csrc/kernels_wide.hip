@@ -235,6 +235,10 @@ __global__ __launch_bounds__(512) void k_wide_apply(WideApply w) {
 constexpr int kSm = WW + 1;
 constexpr int kSmallThreads = 1024;
 constexpr size_t kSmallLds = (size_t)WW * kSm * sizeof(double) + WW * sizeof(double);
+// The elimination steps map threads as rows x kStepCols columns (l <= WW), so
+// no step divides a thread index by a run-time width.
+constexpr uint32_t kStepCols = WW;
+static_assert(kSmallThreads % kStepCols == 0, "whole rows of step threads");
 
 // Upper Cholesky A = R^T R and R^-1 by elimination in one W x W array: row
 // operations reduce the upper triangle to U = D L^T while the strictly lower
@@ -249,6 +253,7 @@ __device__ int chol_inverse_lds(double* a, uint32_t l, double* isd) {
     if (j < i) a[i * kSm + j] = 0.0;
   }
   int bad = 0;
+  const uint32_t tj = t % kStepCols, ti = t / kStepCols, nr = nt / kStepCols;
   for (uint32_t k = 0; k < l; ++k) {
     __syncthreads();
     double d = a[k * kSm + k];
@@ -256,15 +261,15 @@ __device__ int chol_inverse_lds(double* a, uint32_t l, double* isd) {
     if (!ok && bad == 0) bad = (int)k + 1;
     d = ok ? d : 1e-300;
     const double inv = 1.0 / d;
-    const uint32_t items = (l - k - 1) * l;
-    for (uint32_t e = t; e < items; e += nt) {
-      const uint32_t i = k + 1 + e / l, j = e % l;
+    for (uint32_t i = k + 1 + ti; i < l; i += nr) {
       const double m = a[k * kSm + i] * inv;
-      if (j >= i) {
-        a[i * kSm + j] = fma(-m, a[k * kSm + j], a[i * kSm + j]);
-      } else if (j <= k) {
-        const double ek = (j == k) ? 1.0 : a[k * kSm + j];
-        a[i * kSm + j] = fma(-m, ek, a[i * kSm + j]);
+      for (uint32_t j = tj; j < l; j += kStepCols) {
+        if (j >= i) {
+          a[i * kSm + j] = fma(-m, a[k * kSm + j], a[i * kSm + j]);
+        } else if (j <= k) {
+          const double ek = (j == k) ? 1.0 : a[k * kSm + j];
+          a[i * kSm + j] = fma(-m, ek, a[i * kSm + j]);
+        }
       }
     }
   }
@@ -340,17 +345,16 @@ __global__ __launch_bounds__(kSmallThreads) void k_wide_r2(const double* __restr
     const uint32_t e = t + nt * u;
     if (e < l * l) a[(e / l) * kSm + e % l] = b[u];
   }
+  const uint32_t tj = t % kStepCols, ti = t / kStepCols, nr = nt / kStepCols;
   for (uint32_t k = 0; k < l; ++k) {
     __syncthreads();
     const double bkk = a[k * kSm + k];
     const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
     const double ip = 1.0 / (bkk - sk);
     if (t == 0) sign[k] = sk;
-    const uint32_t w = l - k - 1;
-    for (uint32_t e = t; e < w * w; e += nt) {
-      const uint32_t i = k + 1 + e / w, j = k + 1 + e % w;
-      a[i * kSm + j] = fma(-a[i * kSm + k] * ip, a[k * kSm + j], a[i * kSm + j]);
-    }
+    for (uint32_t i = k + 1 + ti; i < l; i += nr)
+      for (uint32_t j = k + 1 + tj; j < l; j += kStepCols)
+        a[i * kSm + j] = fma(-a[i * kSm + k] * ip, a[k * kSm + j], a[i * kSm + j]);
   }
 }
 
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_wide_deficient(const double* 
                                                                  double* __restrict__ scale) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   double* a = reinterpret_cast<double*>(smem_raw);
-  const uint32_t t = threadIdx.x, nt = blockDim.x;
+  const uint32_t t = threadIdx.x, tj = t % kStepCols, ti = t / kStepCols, nr = blockDim.x / kStepCols;
   load_gram_lds(gram, l, a);
   for (uint32_t k = 0; k < l; ++k) {
     __syncthreads();
@@ -375,11 +379,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_wide_deficient(const double* 
     if (t == 0) mask[k] = keep ? 0 : 1;
     if (!keep) continue;  // uniform: every thread read the same d
     const double inv = 1.0 / d;
-    const uint32_t items = (l - k - 1) * l;
-    for (uint32_t e = t; e < items; e += nt) {
-      const uint32_t i = k + 1 + e / l, j = e % l;
-      if (j >= i) a[i * kSm + j] = fma(-a[k * kSm + i] * inv, a[k * kSm + j], a[i * kSm + j]);
-    }
+    for (uint32_t i = k + 1 + ti; i < l; i += nr)
+      for (uint32_t j = tj; j < l; j += kStepCols)
+        if (j >= i) a[i * kSm + j] = fma(-a[k * kSm + i] * inv, a[k * kSm + j], a[i * kSm + j]);
   }
   if (t == 0) {
     double tr = 0.0;
