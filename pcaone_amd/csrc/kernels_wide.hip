@@ -234,7 +234,9 @@ __global__ __launch_bounds__(512) void k_wide_apply(WideApply w) {
 // one-block l x l steps; the matrix lives in LDS with row stride kSm
 constexpr int kSm = WW + 1;
 constexpr int kSmallThreads = 1024;
-constexpr size_t kSmallLds = (size_t)WW * kSm * sizeof(double) + WW * sizeof(double);
+// the W x W array, isd (W), and two row and two column buffers of the
+// register-resident eliminations (4 W)
+constexpr size_t kSmallLds = (size_t)WW * kSm * sizeof(double) + 5 * WW * sizeof(double);
 // The elimination steps map threads as rows x kStepCols columns (l <= WW), so
 // no step divides a thread index by a run-time width.
 constexpr uint32_t kStepCols = WW;
@@ -246,32 +248,49 @@ static_assert(kSmallThreads % kStepCols == 0, "whole rows of step threads");
 // L^-1[j][i] / sqrt(d_j).  On entry the upper triangle of a holds the Gram;
 // isd (LDS, l) receives 1 / sqrt(d_j).  Returns the first failing pivot
 // (1-based), 0 when none.
-__device__ int chol_inverse_lds(double* a, uint32_t l, double* isd) {
-  const uint32_t t = threadIdx.x, nt = blockDim.x;
-  for (uint32_t e = t; e < l * l; e += nt) {
-    const uint32_t i = e / l, j = e % l;
-    if (j < i) a[i * kSm + j] = 0.0;
+//
+// The array lives in registers during the elimination: thread (ti, tj) =
+// (t / kStepCols, t % kStepCols) holds entries (ti + 8 r, tj), r < kRegRows.
+// Step k publishes row k through one of two LDS row buffers (rows, 2 WW
+// doubles), so a step costs one barrier, a few LDS reads and at most kRegRows
+// FMAs per thread; the operations on each entry are those of the plain
+// elimination.  The result goes back to a.
+constexpr int kRowGroups = kSmallThreads / kStepCols;  // 8
+constexpr int kRegRows = WW / kRowGroups;               // 16
+__device__ int chol_inverse_lds(double* a, uint32_t l, double* isd, double* rows) {
+  const uint32_t t = threadIdx.x, nt = blockDim.x, tj = t % kStepCols, ti = t / kStepCols;
+  double v[kRegRows];
+#pragma unroll
+  for (int r = 0; r < kRegRows; ++r) {
+    const uint32_t i = ti + kRowGroups * r;
+    v[r] = (i < l && tj < l && tj >= i) ? a[i * kSm + tj] : 0.0;
   }
   int bad = 0;
-  const uint32_t tj = t % kStepCols, ti = t / kStepCols, nr = nt / kStepCols;
-  for (uint32_t k = 0; k < l; ++k) {
-    __syncthreads();
-    double d = a[k * kSm + k];
-    const bool ok = d > 0.0 && isfinite(d);
-    if (!ok && bad == 0) bad = (int)k + 1;
-    d = ok ? d : 1e-300;
-    const double inv = 1.0 / d;
-    for (uint32_t i = k + 1 + ti; i < l; i += nr) {
-      const double m = a[k * kSm + i] * inv;
-      for (uint32_t j = tj; j < l; j += kStepCols) {
-        if (j >= i) {
-          a[i * kSm + j] = fma(-m, a[k * kSm + j], a[i * kSm + j]);
-        } else if (j <= k) {
-          const double ek = (j == k) ? 1.0 : a[k * kSm + j];
-          a[i * kSm + j] = fma(-m, ek, a[i * kSm + j]);
-        }
+#pragma unroll
+  for (int r = 0; r < kRegRows; ++r) {
+    for (uint32_t p = 0; p < (uint32_t)kRowGroups; ++p) {
+      const uint32_t k = kRowGroups * r + p;
+      if (k >= l) break;
+      double* rowk = rows + (k & 1) * WW;
+      if (ti == p) rowk[tj] = v[r];  // row k (its L^-1 part left of the diagonal)
+      __syncthreads();
+      double d = rowk[k];
+      const bool ok = d > 0.0 && isfinite(d);
+      if (!ok && bad == 0) bad = (int)k + 1;
+      d = ok ? d : 1e-300;
+      const double inv = 1.0 / d;
+      const double akj = (tj == k) ? 1.0 : rowk[tj];
+#pragma unroll
+      for (int rr = 0; rr < kRegRows; ++rr) {
+        const uint32_t i = ti + kRowGroups * rr;
+        if (i > k && i < l && tj < l && (tj >= i || tj <= k)) v[rr] = fma(-(rowk[i] * inv), akj, v[rr]);
       }
     }
+  }
+#pragma unroll
+  for (int r = 0; r < kRegRows; ++r) {
+    const uint32_t i = ti + kRowGroups * r;
+    if (i < l && tj < l) a[i * kSm + tj] = v[r];
   }
   __syncthreads();
   for (uint32_t j = t; j < l; j += nt) {
@@ -298,7 +317,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_wide_chol(const double* __res
   double* a = reinterpret_cast<double*>(smem_raw);
   double* isd = a + WW * kSm;
   load_gram_lds(gram, l, a);
-  const int bad = chol_inverse_lds(a, l, isd);
+  const int bad = chol_inverse_lds(a, l, isd, isd + WW);
   for (uint32_t e = threadIdx.x; e < l * l; e += blockDim.x) rinv[e] = rinv_at(a, isd, e / l, e % l);
   if (threadIdx.x == 0 && bad && *status == 0) *status = bad;
 }
@@ -314,8 +333,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_wide_r2(const double* __restr
   double* a = reinterpret_cast<double*>(smem_raw);
   double* isd = a + WW * kSm;
   const uint32_t t = threadIdx.x, nt = blockDim.x;
+  double* rows = isd + WW;  // 2 W row buffers, then 2 W column buffers
   load_gram_lds(gram, l, a);
-  const int bad = chol_inverse_lds(a, l, isd);
+  const int bad = chol_inverse_lds(a, l, isd, rows);
   if (t == 0 && bad && *status == 0) *status = bad;
   for (uint32_t e = t; e < l * l; e += nt) {
     const uint32_t i = e / l, j = e % l;
@@ -345,16 +365,40 @@ __global__ __launch_bounds__(kSmallThreads) void k_wide_r2(const double* __restr
     const uint32_t e = t + nt * u;
     if (e < l * l) a[(e / l) * kSm + e % l] = b[u];
   }
-  const uint32_t tj = t % kStepCols, ti = t / kStepCols, nr = nt / kStepCols;
-  for (uint32_t k = 0; k < l; ++k) {
-    __syncthreads();
-    const double bkk = a[k * kSm + k];
-    const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
-    const double ip = 1.0 / (bkk - sk);
-    if (t == 0) sign[k] = sk;
-    for (uint32_t i = k + 1 + ti; i < l; i += nr)
-      for (uint32_t j = k + 1 + tj; j < l; j += kStepCols)
-        a[i * kSm + j] = fma(-a[i * kSm + k] * ip, a[k * kSm + j], a[i * kSm + j]);
+  // the LU of B - S in registers, as the Cholesky: step k publishes row k and
+  // column k
+  __syncthreads();
+  const uint32_t tj = t % kStepCols, ti = t / kStepCols;
+  double v[kRegRows];
+#pragma unroll
+  for (int r = 0; r < kRegRows; ++r) {
+    const uint32_t i = ti + kRowGroups * r;
+    v[r] = (i < l && tj < l) ? a[i * kSm + tj] : 0.0;
+  }
+  double* cols = rows + 2 * WW;
+#pragma unroll
+  for (int r = 0; r < kRegRows; ++r) {
+    for (uint32_t p = 0; p < (uint32_t)kRowGroups; ++p) {
+      const uint32_t k = kRowGroups * r + p;
+      if (k >= l) break;
+      double* rowk = rows + (k & 1) * WW;
+      double* colk = cols + (k & 1) * WW;
+      if (ti == p) rowk[tj] = v[r];
+      if (tj == k)
+#pragma unroll
+        for (int rr = 0; rr < kRegRows; ++rr) colk[ti + kRowGroups * rr] = v[rr];
+      __syncthreads();
+      const double bkk = rowk[k];
+      const double sk = (bkk >= 0.0) ? -1.0 : 1.0;
+      const double ip = 1.0 / (bkk - sk);
+      if (t == 0) sign[k] = sk;
+      const double akj = rowk[tj];
+#pragma unroll
+      for (int rr = 0; rr < kRegRows; ++rr) {
+        const uint32_t i = ti + kRowGroups * rr;
+        if (i > k && i < l && tj > k && tj < l) v[rr] = fma(-colk[i] * ip, akj, v[rr]);
+      }
+    }
   }
 }
 
