@@ -47,7 +47,7 @@ constexpr int kGramLd = WW + 1;
 constexpr int kGramWaves = 8;
 constexpr int kPairsPerWave = (WNP + kGramWaves - 1) / kGramWaves;
 constexpr int kGramPart = WNP * 256;  // doubles per Gram partial (tile layout)
-constexpr int kMaxGramBlocks = 128;
+constexpr int kMaxGramBlocks = 512;  // two 66 KB-LDS blocks per CU
 
 // pair p (row by row over ca <= cb) -> tile indices
 __device__ inline void pair_of(int p, int& ca, int& cb) {
