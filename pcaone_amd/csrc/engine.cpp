@@ -221,12 +221,14 @@ void Engine::rsvd_perm(int32_t* perm) const {
 // ---------------------------------------------------------------------------
 // operator set-up
 // ---------------------------------------------------------------------------
-// split-K factor: enough workgroups to cover the 256 CUs twice, at least
-// four K chunks per slice.  A window segment of a few hundred rows against a
-// long K (200,000 samples) needs a deep split: capped at 8, the dense G
-// product of one 313-row window ran 16 blocks and read 0.8 TB/s.
+// split-K factor: about 4,096 workgroups (16 per CU, so each CU holds several
+// blocks whose K-chunk loads overlap), at least four K chunks per slice, at most
+// 128 slices.  A window segment of a few hundred rows against a long K (200,000
+// samples) needs a deep split; the all-window dense G product (79 row blocks)
+// moved its 16 GB at 1.9 TB/s on 553 blocks (a target of 512) and takes 18%
+// less time at 4,096.
 static uint32_t split_for(uint64_t blocks, uint32_t kchunks, uint32_t& cps) {
-  uint64_t want = ceil_div(512, std::max<uint64_t>(blocks, 1));
+  uint64_t want = ceil_div(4096, std::max<uint64_t>(blocks, 1));
   want = std::min<uint64_t>(std::max<uint64_t>(want, 1), std::min<uint32_t>(std::max<uint32_t>(kchunks / 4, 1), 128));
   cps = (uint32_t)ceil_div(kchunks, want);
   return (uint32_t)ceil_div(kchunks, cps);
