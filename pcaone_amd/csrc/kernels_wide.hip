@@ -149,9 +149,9 @@ __global__ __launch_bounds__(256) void k_wide_sum(const double* __restrict__ par
 // ---------------------------------------------------------------------------
 constexpr int kApRows = 128;  // eight waves x one 16-row tile
 constexpr int kApK = 16;
-constexpr int kApBlocks = 256;
+constexpr int kApBlocks = 512;  // two blocks per CU (58 KB of LDS, 128 VGPRs)
 
-__global__ __launch_bounds__(512) void k_wide_apply(WideApply w) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_wide_apply(WideApply w) {
   __shared__ double sa[kApRows * (kApK + 1)];
   __shared__ double ss[kApK * (WW + 1)];
   __shared__ double red[8][3 * WW];
