@@ -101,21 +101,39 @@ __device__ inline void block_tiles_out(const f64x4 (&acc)[NP], bool upper, uint3
 // ---------------------------------------------------------------------------
 // small dense helpers
 // ---------------------------------------------------------------------------
+// C (n x kc, row-major) = A (n x l, row-major) W[:, :kc] (W row stride ldw),
+// times scale[j] when given.  A thread takes four consecutive outputs of one
+// row: the row of A is read once for the four, and each output sums its l
+// terms in order.
 __global__ void k_rows_times_small(const double* __restrict__ A, uint64_t n, uint32_t l,
                                    const double* __restrict__ W, uint32_t ldw, uint32_t kc,
                                    const double* __restrict__ scale, double* __restrict__ C) {
+  const uint32_t q = (kc + 3) / 4;  // column quads per row
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= n * kc) return;
-  const uint64_t r = gid / kc;
-  const uint32_t j = (uint32_t)(gid % kc);
-  double v = 0.0;
-  for (uint32_t a = 0; a < l; ++a) v = fma(A[r * l + a], W[(uint64_t)a * ldw + j], v);
-  C[gid] = scale ? v * scale[j] : v;
+  if (gid >= n * q) return;
+  const uint64_t r = gid / q;
+  const uint32_t j0 = (uint32_t)(gid % q) * 4;
+  const uint32_t nj = min(4u, kc - j0);
+  const double* __restrict__ ar = A + r * l;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+  for (uint32_t a = 0; a < l; ++a) {
+    const double x = ar[a];
+    const double* __restrict__ wr = W + (uint64_t)a * ldw + j0;
+    v0 = fma(x, wr[0], v0);
+    if (nj > 1) v1 = fma(x, wr[1], v1);
+    if (nj > 2) v2 = fma(x, wr[2], v2);
+    if (nj > 3) v3 = fma(x, wr[3], v3);
+  }
+  double* cr = C + r * kc + j0;
+  cr[0] = scale ? v0 * scale[j0] : v0;
+  if (nj > 1) cr[1] = scale ? v1 * scale[j0 + 1] : v1;
+  if (nj > 2) cr[2] = scale ? v2 * scale[j0 + 2] : v2;
+  if (nj > 3) cr[3] = scale ? v3 * scale[j0 + 3] : v3;
 }
 
 void launch_rows_times_small(const double* A, uint64_t n, uint32_t l, const double* W, uint32_t ldw,
                              uint32_t kc, const double* scale, double* C, hipStream_t s) {
-  const uint64_t total = n * kc;
+  const uint64_t total = n * ((kc + 3) / 4);
   if (!total) return;
   hipLaunchKernelGGL(k_rows_times_small, dim3((uint32_t)ceil_div(total, 256)), dim3(256), 0, s, A, n, l, W, ldw,
                      kc, scale, C);
