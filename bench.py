@@ -57,7 +57,7 @@ TRAFFIC = 291.3e6
 TRAFFIC_C5 = 26.03e9
 # k_dense_gemm_bf16x3, averaged over the 280 launches of a solve
 # (profiles/k_dense_gemm_bf16x3_c4_dense_rsvd.md)
-TRAFFIC_C4 = 3.089e9
+TRAFFIC_C4 = 3.130e9
 # k_gl_E, one launch per EM iteration (profiles/k_gl_E_c3_pcangsd.md)
 TRAFFIC_C3 = 12.49e9
 REF = ROOT / "oracle" / "_ref" / "PCAone"
