@@ -181,6 +181,21 @@ int pcaone_gpu_pcangsd_update(pcaone_gpu* h, const double* U, const double* S, c
 int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double tolem, double* U, double* S,
                        double* V, int* iters, double* diffs, double* Dc, double* C);
 
+/* run_pca_with_arnoldi's in-core EM branch for PCAngsd (src/Arnoldi.cpp:57-104,
+ * --svd 0 with -G): IRAM (k, ncv, maxit restarts, Ritz tolerance tol, as
+ * pcaone_gpu_iram) on the initial E, flip_UV, then E <- fit_with_pi(U, S, V)
+ * and a new IRAM until rmse(V_new, V) < tolem (src/Utils.cpp:116-167), at
+ * most maxiter times; diffs[i - 1] is iteration i's RMSE.  As the reference's
+ * solver object does (it caches the eigenvectors it first returns,
+ * external/Spectra/contrib/PartialSVDSolver.h:177-206), the singular vectors of
+ * the smaller side stay those of the first IRAM; later IRAMs give S, and the
+ * other side is the current E times those vectors over S.  Then
+ * pcangsd_standardize_E: Dc (nsamples) and, when C is given, the GRM
+ * (nsamples x nsamples, diagonal Dc / M) as pcaone_gpu_pcangsd.  Needs
+ * pcaone_gpu_load_gl, pcaone_gpu_gl_em_maf and pcaone_gpu_gram_init. */
+int pcaone_gpu_pcangsd_iram(pcaone_gpu* h, int k, int ncv, int maxit, double tol, int maxiter, double tolem,
+                            double* U, double* S, double* V, int* iters, double* diffs, double* Dc, double* C);
+
 /* FileCsv::read_all (src/FileCsv.cpp:11-62, 82-142): a CSV of counts, one
  * feature per line and one sample per comma-separated field, zstd-compressed
  * (the reference's input) or plain text, parsed on nthreads host threads.

@@ -689,6 +689,73 @@ def pcangsd_halko(P0, P1, k: int, os: int, maxp: int, tol: float, maxiter: int =
     return op, F, it, diffs, Dc, C
 
 
+def flip_uv(U: np.ndarray, V: np.ndarray):
+    """flip_UV with ubase (src/Utils.cpp:116-131): a column pair changes sign when
+    U's entry of largest magnitude (the first on ties) is negative."""
+    U, V = U.copy(), V.copy()
+    for i in range(U.shape[1]):
+        x = int(np.argmax(np.abs(U[:, i])))
+        if U[x, i] < 0:
+            U[:, i] *= -1.0
+            V[:, i] *= -1.0
+    return U, V
+
+
+def rmse_flip(X: np.ndarray, Y: np.ndarray) -> float:
+    """rmse (src/Utils.cpp:154-167): Y's columns take the sign nearer X (flip_Y),
+    then the root mean square of X - Y."""
+    Z = Y.copy()
+    for i in range(X.shape[1]):
+        if np.abs(X[:, i] - Z[:, i]).sum() > 2.0 * np.abs(X[:, i] + Z[:, i]).sum():
+            Z[:, i] *= -1.0
+    return float(np.sqrt(((X - Z) ** 2).mean()))
+
+
+def pcangsd_iram(P0, P1, k: int, maxiter: int = 100, tolem: float = 1e-5, tolmaf: float = 1e-6,
+                 refresh_vectors: bool = False):
+    """run_pca_with_arnoldi's in-core EM branch for PCAngsd (src/Arnoldi.cpp:57-104)
+    with an exact eigendecomposition of the smaller Gram matrix in place of IRAM.
+    The reference's PartialSVDSolver hands out the eigenvectors of its first
+    solve on every later matrix_U / matrix_V call (it caches m_evecs,
+    external/Spectra/contrib/PartialSVDSolver.h:177-206), so after the first
+    solve only the singular values change: the smaller side's vectors stay, the
+    other side is E times them over S.  refresh_vectors=True takes every solve's
+    own vectors instead (a planted mistake for the tests).  Returns (U, S, V, F,
+    iterations, RMSEs, Dc, GRM)."""
+    n, m = P0.shape
+    F, _ = em_maf_with_gl(P0, P1, 100, tolmaf)
+    E = gl_initial_E(P0, P1, F)
+    tall = n > m
+
+    def eig_top(E):
+        G = E.T @ E if tall else E @ E.T
+        w, Q = np.linalg.eigh(G)
+        o = np.argsort(w)[::-1][:k]
+        return w[o], Q[:, o]
+
+    def usv(E, lam, ev):
+        s = np.sqrt(lam)
+        return (E @ ev / s, s, ev.copy()) if tall else (ev.copy(), s, E.T @ ev / s)
+
+    lam, ev0 = eig_top(E)
+    U, S, V = usv(E, lam, ev0)
+    U, V = flip_uv(U, V)
+    diffs, it = [], 0
+    for it in range(1, maxiter + 1):
+        E = pcangsd_E(P0, P1, F, U, S, V)
+        lam, evn = eig_top(E)
+        U, S, V2 = usv(E, lam, evn if refresh_vectors else ev0)
+        U, V2 = flip_uv(U, V2)
+        diffs.append(rmse_flip(V2, V))
+        V = V2
+        if diffs[-1] < tolem:
+            break
+    Es, Dc = pcangsd_standardize(P0, P1, F, U, S, V)
+    C = Es @ Es.T / m
+    C[np.diag_indices(n)] = Dc / m
+    return U, S, V, F, it, diffs, Dc, C
+
+
 def plink_hard_call_gl(codes: np.ndarray):
     """The two GLs FileBed::read_all keeps per sample for --pcangsd on PLINK
     input (src/FilePlink.cpp:89-104): code 3 (RR) (1, 0), code 0 (AA) (0, 0),

@@ -105,6 +105,8 @@ SIGNATURES = {
     "pcaone_gpu_pcangsd_update": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_int, C.c_int, C.POINTER(C.c_float), _D]),
     "pcaone_gpu_pcangsd": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, _D, _D, _D,
                                      C.POINTER(C.c_int), _D, _D, _D]),
+    "pcaone_gpu_pcangsd_iram": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                          _D, _D, _D, C.POINTER(C.c_int), _D, _D, _D]),
     "pcaone_gpu_rsvd_init": (C.c_int, [C.c_void_p, C.POINTER(RsvdParams), _D, _I32]),
     "pcaone_gpu_rsvd_dims": (C.c_int, [C.c_void_p, _I64, _I64, _I64]),
     "pcaone_gpu_rsvd_perm": (C.c_int, [C.c_void_p, _I32]),
@@ -524,6 +526,24 @@ class Gpu:
                                                 _p(S, C.c_double), _p(V, C.c_double), C.byref(it),
                                                 _p(diffs, C.c_double), _p(Dc, C.c_double) if dc else None,
                                                 _p(Cm, C.c_double) if grm else None))
+        return U, S, V, it.value, diffs[:it.value], Dc, Cm
+
+    def pcangsd_iram(self, k: int, ncv: int | None = None, maxit: int = 1000, tol: float = 1e-6,
+                     maxiter: int = 100, tolem: float = 1e-5, grm: bool = True, dc: bool = True):
+        """PCAngsd EM with IRAM (--svd 0 on likelihoods) after gram_init; returns
+        U, S, V, iterations, RMSE per iteration, Dc (or None), C (or None)."""
+        ncv = ncv if ncv is not None else max(20, 2 * k + 1)  # src/Cmd.cpp:189
+        U = np.empty((self.N, k), order="F")
+        S = np.empty(k)
+        V = np.empty((self.M, k), order="F")
+        it = C.c_int(0)
+        diffs = np.zeros(max(maxiter, 1))
+        Dc = np.empty(self.N) if dc else None
+        Cm = np.empty((self.N, self.N)) if grm else None
+        self._check(self._L.pcaone_gpu_pcangsd_iram(self.h, k, ncv, maxit, tol, maxiter, tolem, _p(U, C.c_double),
+                                                     _p(S, C.c_double), _p(V, C.c_double), C.byref(it),
+                                                     _p(diffs, C.c_double), _p(Dc, C.c_double) if dc else None,
+                                                     _p(Cm, C.c_double) if grm else None))
         return U, S, V, it.value, diffs[:it.value], Dc, Cm
 
     def load_csv(self, path, scale: int = 2, scale_factor: float = 1.0, nthreads: int = 16):

@@ -185,6 +185,13 @@ int pcaone_gpu_pcangsd(pcaone_gpu* h, int maxp, double tol, int maxiter, double 
   return guarded(h, [&](pcaone::Engine& e) { e.pcangsd(maxp, tol, maxiter, tolem, U, S, V, iters, diffs, Dc, C); });
 }
 
+int pcaone_gpu_pcangsd_iram(pcaone_gpu* h, int k, int ncv, int maxit, double tol, int maxiter, double tolem,
+                            double* U, double* S, double* V, int* iters, double* diffs, double* Dc, double* C) {
+  return guarded(h, [&](pcaone::Engine& e) {
+    e.pcangsd_iram(k, ncv, maxit, tol, maxiter, tolem, U, S, V, iters, diffs, Dc, C);
+  });
+}
+
 int pcaone_ld_windows(const int32_t* pos, uint32_t nsnps, const int32_t* chr_end, uint32_t nchr, int32_t ld_bp,
                       int32_t* ws, int32_t* we, uint32_t* nwin) {
   try {

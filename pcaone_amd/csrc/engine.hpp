@@ -184,6 +184,9 @@ class Engine {
   // eigenpairs of the kept GRM as JacobiSVD(C) gives them (src/Halko.cpp:326-330):
   // |lambda| descending, U2 row-major N x N (row i = sample i)
   void grm_eig(double* evals, double* U2);
+  // run_pca_with_arnoldi's in-core PCAngsd EM (IRAM on E each iteration)
+  void pcangsd_iram(int k, int ncv, int maxit, double tol, int maxiter, double tolem, double* U, double* S,
+                    double* V, int* iters, double* diffs, double* Dc, double* C);
   void pcangsd(int maxp, double tol, int maxiter, double tolem, double* U, double* S, double* V, int* iters,
                double* diffs, double* Dc, double* C);
   // LD r^2 over windows on the loaded dense matrix, or on the --USV-adjusted genotypes
@@ -276,6 +279,8 @@ class Engine {
   void set_omega_vec(const double* dev_v, const double* host_v);
   void apply_X(const double* x, double* v);
   void apply_Xt(const double* v, double* y);
+  // U, S, V (column-major) from the top-k Gram eigenpairs of the smaller side
+  void usv_from_eigvecs(int k, const double* lam, const double* vecs, double* U, double* S, double* V);
   void copy_usv(double* U, double* S, double* V);
   void gl_fill_E(const double* U, const double* S, const double* V, uint32_t k, bool standardize, double* Dc);
 

@@ -176,6 +176,15 @@ void Engine::iram(int k, int ncv, int maxit, double tol, double* U, double* S, d
   if (nconv) *nconv = nc;
   // run_pca_with_arnoldi stops here (src/Arnoldi.cpp:64)
   PCAONE_REQUIRE(nc == k, -4, "the nconv is not equal to k.");
+  usv_from_eigvecs(k, lam.data(), vecs.data(), U, S, V);
+}
+
+// PartialSVDSolver's matrix_U / matrix_V (external/Spectra/contrib/
+// PartialSVDSolver.h:177-206): S = sqrt(lam); on the smaller Gram side the
+// eigenvectors are the singular vectors, the other side is X V / s or X^T U / s
+void Engine::usv_from_eigvecs(int k, const double* lam, const double* vecs, double* U, double* S, double* V) {
+  const int side = (N_ > M_) ? 1 : 0;
+  const uint64_t n = side ? M_ : N_;
   std::vector<double> s(k);
   for (int i = 0; i < k; ++i) {
     s[i] = std::sqrt(std::max(lam[i], 0.0));
@@ -184,7 +193,7 @@ void Engine::iram(int k, int ncv, int maxit, double tol, double* U, double* S, d
   if (S) std::memcpy(S, s.data(), k * sizeof(double));
   std::vector<double> tmp(side ? N_ : M_);
   for (int i = 0; i < k; ++i) {
-    const double* e = vecs.data() + (size_t)i * n;
+    const double* e = vecs + (size_t)i * n;
     if (side) {  // V = eigvecs, U = X V / s
       if (V) std::memcpy(V + (size_t)i * M_, e, M_ * sizeof(double));
       apply_X(e, tmp.data());

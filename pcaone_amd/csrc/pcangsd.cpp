@@ -173,6 +173,122 @@ void Engine::pcangsd_update(const double* U, const double* S, const double* V, i
       std::memcpy(E + (size_t)perm_[r] * N_, rows.data() + (size_t)r * kg_, (size_t)N_ * 4);
 }
 
+namespace {
+
+// flip_UV with ubase (src/Utils.cpp:116-130): column i of U (n x k, column-major)
+// and of V (m x k) change sign when U's entry of largest magnitude (the first
+// such on ties) is negative.
+void flip_uv(std::vector<double>& U, std::vector<double>& V, size_t n, size_t m, int k) {
+  for (int i = 0; i < k; ++i) {
+    double* u = U.data() + (size_t)i * n;
+    size_t x = 0;
+    for (size_t r = 1; r < n; ++r)
+      if (std::fabs(u[r]) > std::fabs(u[x])) x = r;
+    if (u[x] < 0) {
+      for (size_t r = 0; r < n; ++r) u[r] = -u[r];
+      double* v = V.data() + (size_t)i * m;
+      for (size_t r = 0; r < m; ++r) v[r] = -v[r];
+    }
+  }
+}
+
+// rmse(X, Y) (src/Utils.cpp:153-167): Y's columns take the sign nearer X
+// (flip_Y), then the root mean square of X - Y.
+double rmse_flip(const std::vector<double>& X, const std::vector<double>& Y, size_t m, int k) {
+  double sq = 0.0;
+  for (int i = 0; i < k; ++i) {
+    const double* x = X.data() + (size_t)i * m;
+    const double* y = Y.data() + (size_t)i * m;
+    double dm = 0.0, dp = 0.0;
+    for (size_t r = 0; r < m; ++r) {
+      dm += std::fabs(x[r] - y[r]);
+      dp += std::fabs(x[r] + y[r]);
+    }
+    const double s = dm > 2.0 * dp ? -1.0 : 1.0;
+    for (size_t r = 0; r < m; ++r) {
+      const double d = x[r] - s * y[r];
+      sq += d * d;
+    }
+  }
+  return std::sqrt(sq / ((double)m * (double)k));
+}
+
+}  // namespace
+
+// run_pca_with_arnoldi's EM branch in core for PCAngsd (src/Arnoldi.cpp:57-104):
+// IRAM on the initial E, flip_UV, then E <- fit_with_pi(U, S, V) and a new IRAM
+// until rmse(V_new, V) < tolem (at most maxiter times); then
+// pcangsd_standardize_E gives Dc and, when C is given, the GRM C = E E^T / M with
+// diagonal Dc / M.  The operator is the Gram mat-vec of gram_init on the
+// likelihoods' E (identity site order).
+//
+// The solver object there keeps the eigenvectors it handed out first
+// (PartialSVDSolver caches m_evecs on the first matrix_U / matrix_V call and
+// never drops it, external/Spectra/contrib/PartialSVDSolver.h:177-206), so
+// every later solve contributes only its singular values: on the smaller Gram
+// side the singular vectors stay those of the first solve, and the other side is
+// the current E times them over the new singular values.  The same is done here.
+void Engine::pcangsd_iram(int k, int ncv, int maxit, double tol, int maxiter, double tolem, double* U, double* S,
+                          double* V, int* iters, double* diffs, double* Dc, double* C) {
+  PCAONE_REQUIRE(gl_mode_ && have_mv_, -3, "pcangsd_iram: load likelihoods, run gl_em_maf and gram_init first");
+  PCAONE_REQUIRE(k >= 1 && maxiter >= 0, -1, "pcangsd_iram: k must be >= 1 and maxiter >= 0");
+  PCAONE_HIP(hipSetDevice(device_));
+  const size_t n = N_, m = M_;
+  std::vector<double> u(n * k), s(k), v(m * k), v2(m * k), rowu(n * k), rowv(m * k);
+  // fit_with_pi's operands on the device: U (n x k) and V (m x k) row-major
+  Ucur_.reserve(n * k);
+  V_.reserve(m * k);
+  Sdev_.reserve(k);
+  auto upload = [&]() {
+    for (size_t r = 0; r < n; ++r)
+      for (int c = 0; c < k; ++c) rowu[r * k + c] = u[(size_t)c * n + r];
+    for (size_t r = 0; r < m; ++r)
+      for (int c = 0; c < k; ++c) rowv[r * k + c] = v[(size_t)c * m + r];
+    PCAONE_HIP(hipMemcpyAsync(Ucur_.p, rowu.data(), n * k * 8, hipMemcpyHostToDevice, stream_));
+    PCAONE_HIP(hipMemcpyAsync(V_.p, rowv.data(), m * k * 8, hipMemcpyHostToDevice, stream_));
+    PCAONE_HIP(hipMemcpyAsync(Sdev_.p, s.data(), (size_t)k * 8, hipMemcpyHostToDevice, stream_));
+  };
+  const int side = (N_ > M_) ? 1 : 0;
+  const size_t ne = side ? m : n;
+  std::vector<double> lam(k), ev0(ne * k), evn(ne * k);
+  int nc = 0;
+  eigs(side, k, ncv, maxit, tol, lam.data(), ev0.data(), &nc, nullptr);
+  PCAONE_REQUIRE(nc == k, -4, "the nconv is not equal to k.");
+  usv_from_eigvecs(k, lam.data(), ev0.data(), u.data(), s.data(), v.data());
+  flip_uv(u, v, n, m, k);
+  int it = 0;
+  for (int i = 1; i <= maxiter; ++i) {
+    upload();
+    gl_fill_E(Ucur_.p, Sdev_.p, V_.p, (uint32_t)k, false, nullptr);
+    eigs(side, k, ncv, maxit, tol, lam.data(), evn.data(), &nc, nullptr);
+    PCAONE_REQUIRE(nc == k, -4, "pcangsd_iram: fewer than k eigenpairs converged");
+    usv_from_eigvecs(k, lam.data(), ev0.data(), u.data(), s.data(), v2.data());
+    flip_uv(u, v2, n, m, k);
+    const double diff = rmse_flip(v2, v, m, k);
+    v.swap(v2);
+    it = i;
+    if (diffs) diffs[i - 1] = diff;
+    if (diff < tolem) break;
+  }
+  if (Dc || C) {
+    upload();
+    gl_Dc_.reserve(N_);
+    gl_fill_E(Ucur_.p, Sdev_.p, V_.p, (uint32_t)k, true, gl_Dc_.p);
+    if (Dc) PCAONE_HIP(hipMemcpyAsync(Dc, gl_Dc_.p, n * 8, hipMemcpyDeviceToHost, stream_));
+    if (C) {  // kept on the device for grm_eig
+      grm_.reserve((uint64_t)N_ * N_);
+      timed("grm", [&] { launch_grm(dns_.p, kg_, N_, M_, gl_Dc_.p, grm_.p, stream_); });
+      PCAONE_HIP(hipMemcpyAsync(C, grm_.p, n * n * 8, hipMemcpyDeviceToHost, stream_));
+      have_grm_ = true;
+    }
+  }
+  sync();  // the staging vectors go out of scope
+  if (U) std::memcpy(U, u.data(), n * k * 8);
+  if (S) std::memcpy(S, s.data(), (size_t)k * 8);
+  if (V) std::memcpy(V, v.data(), m * k * 8);
+  if (iters) *iters = it;
+}
+
 // run_pca_with_halko's EM branch for PCAngsd (src/Halko.cpp:294-331): one
 // solve on the initial E, then E <- fit_with_pi(U, S, V) and a new solve until
 // 1 - mev(V, Vpre) < tolem; then pcangsd_standardize_E gives Dc and, when C is

@@ -75,6 +75,9 @@ def oracle_run(z: dict):
     n, m = int(z["nsamples"]), int(z["nsnps"])
     if "gl" in z:  # PCAngsd on BEAGLE likelihoods: the op of the last EM solve
         P0, P1 = gl_planes(z["gl"])
+        if d["svd"] == 0:  # the IRAM EM: its final U, S, V
+            U, S, V, *_ = O.pcangsd_iram(P0, P1, d["k"])
+            return O.ExactSvd(U, S, V, m), None, d
         orc, *_ = O.pcangsd_halko(P0, P1, d["k"], d["os"], d["p"], d["tol"], seed=d["seed"], rand=d["rand"])
         return orc, None, d
     if "counts" in z:  # CSV of counts: the reader's --scale transform, no further centring

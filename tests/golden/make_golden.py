@@ -75,6 +75,8 @@ GL = {"g1": (31, 120, 700, 4, 0.1, 2.0, 0.01)}
 GL_CASES = {
     # PCAngsd EM; --svd 1 keeps the reference's in-core indices consistent
     "gl_pcangsd_svd1": ("g1", "--svd 1 -k 3 --oversamples 7 -p 5 --tol-rsvd 0"),
+    # the same EM with IRAM (run_pca_with_arnoldi's in-core EM branch)
+    "gl_pcangsd_svd0": ("g1", "--svd 0 -k 3"),
 }
 DENSE_CASES = {
     "bin_svd1_p5": ("b1", "--svd 1 -k 5 --oversamples 5 -p 5 --tol-rsvd 0"),
@@ -496,7 +498,10 @@ def run_ld_tools(tmp: Path, name: str, seed, n, per_chr, ld_bp):
 def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
     ooc_perm = "-m" in args.split() and "--svd 2" in args and "-S" not in args.split()
     # -v 3 keeps the permuted copy the out-of-core winSVD writes (src/Main.cpp:151-162)
-    cmd = [str(REF), flag, prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v", "3" if ooc_perm else "0"]
+    iram_em = flag == "-G" and "--svd 0" in args
+    # -v 1 logs the IRAM EM's RMSE per iteration (src/Arnoldi.cpp:77-78)
+    cmd = [str(REF), flag, prefix, *args.split(), "-o", out, "-V", "-n", "1", "-v",
+           "3" if ooc_perm else "1" if iram_em else "0"]
     subprocess.run(cmd, check=True, cwd=tmp, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     res = {}
     res["eigvals"] = np.loadtxt(tmp / (out + ".eigvals"), ndmin=1)
@@ -523,6 +528,10 @@ def run_case(tmp: Path, prefix: str, args: str, out: str, flag: str = "-b"):
         log = (tmp / (out + ".log")).read_text()
         res["em_diffs"] = np.array([float(x) for x in re.findall(
             r"individual allele frequencies estimated iter =\s*\d+, diff =\s*([0-9.eE+-]+)", log)])
+    if iram_em:
+        log = (tmp / (out + ".log")).read_text()
+        res["em_rmse"] = np.array([float(x) for x in re.findall(r"RMSE =\s*([0-9.eE+-]+)", log)])
+
     return res
 
 

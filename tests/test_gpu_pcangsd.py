@@ -1,8 +1,9 @@
 """GPU parity of the PCAngsd path on genotype likelihoods (tests marked `gpu`):
 emMAF_with_GL, the initial E, the fit_with_pi PCAngsd update and
 pcangsd_standardize_E against the oracle on the same inputs, and the whole EM
-(plus the GRM written to .cov) against the reference CLI's BEAGLE run
-(tests/golden/gl_pcangsd_svd1.npz); --pcangsd on PLINK hard calls against the
+(plus the GRM written to .cov) against the reference CLI's BEAGLE runs with
+RSVD (tests/golden/gl_pcangsd_svd1.npz) and with IRAM
+(tests/golden/gl_pcangsd_svd0.npz); --pcangsd on PLINK hard calls against the
 reference CLI's in-core PLINK run (tests/golden/plink_pcangsd_svd1.npz).
 
 Tolerances: F and Dc within FP64 rounding (sums in a different order); E is
@@ -200,3 +201,30 @@ def test_pcangsd_errors(gpu):
         gpu.rsvd_init(2, 4, variant=pa.RSVD_NORMAL)  # allele frequencies not estimated yet
     with pytest.raises(pa.GpuError):
         gpu.load_gl(gl[:, :, :1])  # one likelihood per sample
+
+
+@pytest.mark.parametrize("arm", ["fp64", "bf16x2"])
+def test_pcangsd_iram_matches_reference(gpu, arm):
+    """--svd 0 on likelihoods: IRAM in every EM iteration, flip_UV and the RMSE
+    stop (src/Arnoldi.cpp:57-104), then pcangsd_standardize_E and the GRM,
+    against the reference CLI's run with its defaults (ncv 20, itol 1e-6,
+    maxiter 100, tol-em 1e-5): the RMSE it logs per iteration (6 significant
+    digits), its eigenvalues, vectors and GRM."""
+    z = load_golden(GOLDEN / "gl_pcangsd_svd0.npz")
+    d = parse_args(z["args"])
+    m = int(z["nsnps"])
+    gpu.load_gl(z["gl"])
+    gpu.gl_em_maf(100, 1e-6)
+    gpu.gram_init(ARMS[arm], standardize=False)
+    U, S, V, it, diffs, Dc, Cm = gpu.pcangsd_iram(d["k"])
+    assert it == len(z["em_rmse"]) and diffs[-1] < 1e-5 and np.all(diffs[:-1] >= 1e-5)
+    # the split-BF16 Gram products move the RMSEs by up to ~3e-7; the EM with
+    # each solve's own vectors starts 4e-4 away (1.698e-3)
+    assert np.allclose(diffs, z["em_rmse"], rtol=1e-4, atol=1e-9 if arm == "fp64" else 1e-6)
+    assert eig_relerr(S ** 2 / m * 2, z["eigvals"]) < EIG_TOL[arm]
+    assert O.principal_angles_max(U, z["eigvecs"]) < 1e-4
+    assert O.principal_angles_max(V, z["loadings"]) < 1e-4
+    # flip_UV: each column's entry of largest magnitude in U is positive
+    assert np.all(U[np.abs(U).argmax(axis=0), np.arange(U.shape[1])] > 0)
+    assert np.max(np.abs(Cm - z["cov"])) < 2e-5 * np.max(np.abs(z["cov"]))
+    assert np.all(np.isfinite(Dc)) and np.allclose(np.diag(Cm), Dc / m)

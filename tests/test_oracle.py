@@ -245,6 +245,32 @@ def test_pcangsd_grm_matches_reference():
     assert np.max(np.abs(np.diag(E @ E.T / P0.shape[1]) - np.diag(z["cov"]))) > 1e-3
 
 
+def test_pcangsd_iram_matches_reference():
+    """The IRAM EM (--svd 0 on likelihoods) against the reference CLI's run: the
+    RMSE it logs per iteration (6 significant digits), its eigenvalues, vectors
+    and GRM.  Planted mistake: the EM with every solve's own singular vectors
+    (the textbook update, not what the reference's solver object hands out)
+    follows another trajectory and stops elsewhere."""
+    z = load_golden(ROOT / "tests" / "golden" / "gl_pcangsd_svd0.npz")
+    from _cases import gl_planes
+    P0, P1 = gl_planes(z["gl"])
+    m = P0.shape[1]
+    k = parse_args(z["args"])["k"]
+    U, S, V, F, it, diffs, Dc, C = O.pcangsd_iram(P0, P1, k)
+    ref = z["em_rmse"]
+    assert it == len(ref) == 17
+    assert np.allclose(diffs, ref, rtol=1e-5, atol=0)
+    assert eig_relerr(S ** 2 / m * 2, z["eigvals"]) < 5e-6
+    assert O.principal_angles_max(U, z["eigvecs"]) < 1e-5
+    assert O.principal_angles_max(V, z["loadings"]) < 1e-5
+    assert np.all(U[np.abs(U).argmax(axis=0), np.arange(k)] > 0)
+    assert np.max(np.abs(C - z["cov"])) < 1e-5 * np.max(np.abs(z["cov"]))
+    *_, it2, diffs2, _, _ = O.pcangsd_iram(P0, P1, k, refresh_vectors=True)
+    assert abs(diffs2[0] - ref[0]) > 0.1 * ref[0]
+    U2, S2, *_ = O.pcangsd_iram(P0, P1, k, refresh_vectors=True)
+    assert eig_relerr(S2 ** 2 / m * 2, z["eigvals"]) > 1e-3
+
+
 def test_pcangsd_plink_planted_mistakes(monkeypatch):
     """The PLINK PCAngsd fixture pins the oracle: the EM trajectory the
     reference logs (6 significant digits) and its outputs.  An initial E in
