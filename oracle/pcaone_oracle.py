@@ -28,6 +28,9 @@ the reference's behaviour, of:
   src/FileBeagle.cpp:121-143, src/Data.cpp:249-343, src/Halko.cpp:317-331)
 * PCAngsd on PLINK hard calls                  (src/FilePlink.cpp:89-104,
   src/Halko.cpp:294-316)
+* the IRAM EM for PCAngsd, flip_UV, rmse       (src/Arnoldi.cpp:57-104,
+  src/Utils.cpp:116-167; pinned by the reference CLI's --svd 0 run: the
+  RMSE it logs at each iteration, its eigenvalues, vectors and GRM)
 * LD windows and r^2, the --USV adjustment    (src/LD.cpp:20-23, 125-139,
   419-442, 460-466)
 * the --ld-stats 1 residual matrix of PLINK input and its r^2 from integer
